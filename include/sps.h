@@ -1,0 +1,184 @@
+/*
+ * sps.h - C-ABI of the Stress-Particle SPH hot path on one MI355X (gfx950).
+ *
+ * The reference has no operator/FFI interface: its hot path is the
+ * argument-less `call time_integration` (code/1_SPH_2018.f90:82, public at
+ * code/2_SPH_main_2018.f90:22) working on module globals
+ * (code/5_SPH_global_vars_2018.f90:27-182).  This header is the seam that
+ * replaces it: the Fortran host marshals the same globals through
+ * iso_c_binding (sps_binding.f90) and calls sps_step once per time step.
+ *
+ * Conventions
+ *  - Plain C types only.  real(DP) <-> double, default real <-> float,
+ *    integer <-> int32_t (code/7_variable_types.f90:7).
+ *  - Arrays use the reference's Fortran column-major layout: x(2,ntotal2) is
+ *    [x1,y1,x2,y2,...], stress(4,ntotal2) is [sxx,syy,sxy,szz] per particle,
+ *    grad_u(2,2,ntotal) is [g11,g21,g12,g22] per particle with
+ *    g_ab = d v_a / d x_b, internal_vars(nint_vars,ntotal).
+ *  - Particle numbering follows the reference (code/3_SPH_material_2018.f90
+ *    :942-1007): velocity nodes 1..nnode, stress points nnode+1..ntotal,
+ *    wall dummies ntotal+1..ntotal2, the first ndummy2 dummies being the
+ *    innermost wall layer (code/3_SPH_material_2018.f90:793-812).  Pair
+ *    indices returned by sps_get_pairs are 1-based like Fortran.
+ *  - Every call is synchronous on return.  One host thread per handle.
+ *  - Return value: SPS_OK (0) or a negative SPS_E* code; the message is
+ *    available from sps_last_error() (per thread).
+ *
+ * Two shared libraries implement this header: the product
+ * (libsps_hip.so, hand-written HIP kernels) and the CPU oracle used only as
+ * the checker in tests (oracle/libsps_oracle.so).
+ */
+#ifndef SPS_H
+#define SPS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPS_ABI_VERSION 1
+
+enum {
+    SPS_OK = 0,
+    SPS_EINVAL = -1,      /* bad argument or unsupported option combination */
+    SPS_ENOMEM = -2,      /* host or device allocation failed */
+    SPS_EDEVICE = -3,     /* HIP runtime error */
+    SPS_ECAPACITY = -4,   /* a grid or pair capacity was exceeded */
+    SPS_ESTATE = -5,      /* call out of order (e.g. step before set_state) */
+    SPS_ENUMERIC = -6     /* non-finite value detected in the state */
+};
+
+/* summation/ordering mode of the pair loops */
+enum {
+    SPS_MODE_ORDERED = 0, /* reference list order, bitwise parity target */
+    SPS_MODE_FAST = 1     /* cell order, reassociated sums (tolerance-checked) */
+};
+
+#define SPS_MAX_TCURVE 64
+
+/*
+ * Problem constants.  Field names follow the reference globals; each comment
+ * names where the reference reads or derives the value.
+ */
+typedef struct sps_params {
+    int32_t abi_version;     /* must be SPS_ABI_VERSION */
+    int32_t nnode;           /* nnode   (3_...:421) */
+    int32_t nstress;         /* nstress (3_...:424-440) */
+    int32_t ndummy;          /* ndummy  (3_...:785), 0 without dummy walls */
+    int32_t ndummy2;         /* innermost wall layer (3_...:776-782) */
+    int32_t nint_vars;       /* leading dim of internal_vars (3_...:275), >= 1 */
+    int32_t ndimn;           /* must be 2 */
+    int32_t nstre;           /* must be 4 (plane strain) */
+    int32_t skf;             /* kernel: must be 1, cubic spline (2_...:1468) */
+    /* control logicals (0/1), read at 3_...:286-299, 364-375 */
+    int32_t sp_sph;          /* must be 1 */
+    int32_t inside_approach; /* must be 1 */
+    int32_t cspm;            /* CSPM gradient correction (2_...:539) */
+    int32_t update_x;        /* position update + Jaumann spin (2_...:140, 751) */
+    int32_t xsph;            /* must be 0 */
+    int32_t cont_density;    /* must be 0 */
+    int32_t art_stress;      /* must be 0 */
+    int32_t dummy_nodes;     /* wall dummies present (.pts) */
+    int32_t sph_shift;       /* must be 0 (outside approach only) */
+    int32_t no_bcs;          /* must be 0 */
+    int32_t bforce_gate;     /* 0: wall force when inside_approach .and.
+                                dummy_nodes (code/2_...:742); 1: when
+                                inside_approach (Bui fork 2_...:742) */
+    int32_t mode;            /* SPS_MODE_* */
+    int32_t ic_grav;         /* 0/1 (3_...:302-311) */
+    int32_t tcurve_npts;     /* points in the gravity time curve, 2..64 */
+    double props[20];        /* props(1,1:20) (3_...:116-139) */
+    double pi;               /* 4*atan(1.0) in single precision (3_...:144) */
+    double dx;               /* lattice spacing (.pts, 3_...:393) */
+    double alpha, beta;      /* artificial viscosity (3_...:296) */
+    double damping;          /* DampingTG (3_...:293) */
+    double cspm_det_tol;     /* 1e-03 (code/2_...:599) or 1e-07 (Bui fork),
+                                as the single-precision literal widened */
+    double D11, D22, D12, D33, D41, D42;   /* elastic matrix (3_...:719-730) */
+    double xmin_domain[2], xmax_domain[2]; /* domain box (3_...:270) */
+    double cgrav[2];         /* gravity vector (3_...:308) */
+    double ft_grav;          /* gravity factor (3_...:308) */
+    double tcurve_t[SPS_MAX_TCURVE];  /* ttcurves(tcurve_grav,:), real(DP) */
+    float  tcurve_f[SPS_MAX_TCURVE];  /* ftcurves(tcurve_grav,:), default real */
+} sps_params;
+
+/*
+ * Pointers to host arrays (Fortran layout).  For sps_set_state the pointers
+ * marked "in" are required; for sps_get_state any NULL pointer is skipped.
+ */
+typedef struct sps_state {
+    double  *x;                 /* (2,ntotal2)  in/out */
+    double  *vel;               /* (2,ntotal2)  in/out */
+    double  *stress;            /* (4,ntotal2)  in/out */
+    double  *mass;              /* (ntotal2)    in/out */
+    double  *rho;               /* (ntotal2)    in/out */
+    double  *hsml;              /* (ntotal2)    in/out */
+    int32_t *itype;             /* (ntotal2)    in/out: 2 node, 1 SP, 25 dummy */
+    int32_t *if_out_domain;     /* (ntotal2)    in/out, sticky 0/1 */
+    double  *internal_vars;     /* (nint_vars,ntotal) in/out; only row 1 is
+                                   touched by the hot path */
+    double  *f_drucker;         /* (ntotal)     in/out, DP yield-function memory */
+    double  *x00;               /* (2,ntotal2)  in: initial positions (displ) */
+    double  *displ;             /* (2,nnode)    in/out */
+    float   *wall_position;     /* (ntotal2)    in, dummies only (may be NULL
+                                   without dummies) */
+    float   *horizontal_or_not; /* (ntotal2)    in, dummies only */
+    double  *grad_u;            /* (2,2,ntotal) out: last stage velocity gradient */
+    double  *ddev_strn;         /* (ntotal)     out: Ddev_strn of the last step */
+    double  *f_bound;           /* (2,nnode)    out: last stage wall force */
+    double  *art_visc;          /* (2,nnode)    out: last stage art. viscosity */
+} sps_state;
+
+typedef struct sps_stats {
+    int64_t n_pairs;        /* pairs found by the last search (2_...:1354) */
+    int64_t m_pairs;        /* list high-water mark after it (2_...:1369) */
+    int64_t n_by_type[10];  /* pint_type histogram 0..9 (3_...:1598-1608) */
+    int32_t ndivx, ndivy;   /* cell grid of the last search (2_...:1250) */
+    int64_t steps;          /* steps taken by this handle */
+} sps_stats;
+
+typedef struct sps_handle sps_handle;
+
+/* Create a handle for one problem.  Replaces the allocation part of
+ * Setup_Global_Arrays (3_...:883-1026) for the hot-path state.  The product
+ * library selects the current HIP device. */
+int sps_create(const sps_params *params, sps_handle **out);
+
+/* Copy the state in (host -> library).  Replaces the implicit sharing of the
+ * module globals listed in sps_state.  The pair-list high-water mark is not
+ * touched: like the reference (2_...:1290-1294) it restarts at zero on the
+ * step whose itimestep_sph is 1. */
+int sps_set_state(sps_handle *h, const sps_state *st);
+
+/* Advance nsteps steps of dt.  One step is one `call time_integration`
+ * (2_...:78-184) followed by the driver's time update
+ * (1_SPH_2018.f90:80-82): itimestep_sph += 1 before, time_sph += dt after. */
+int sps_step(sps_handle *h, int32_t nsteps, double dt,
+             double *time_sph, int32_t *itimestep_sph);
+
+/* Copy the state out (library -> host). */
+int sps_get_state(sps_handle *h, sps_state *st);
+
+/* Pairs of the last search after Pint_Update (3_...:1555-1615), in the order
+ * a traversal of the reference's linked list from `last` visits them.
+ * Indices are 1-based.  *n receives the pair count; at most cap entries are
+ * written (SPS_ECAPACITY if cap < *n).  Any output pointer may be NULL. */
+int sps_get_pairs(sps_handle *h, int64_t cap, int32_t *pair_i, int32_t *pair_j,
+                  int32_t *pint_type, float *w, float *dwdx, float *dwdy,
+                  int64_t *n);
+
+int sps_get_stats(sps_handle *h, sps_stats *out);
+
+/* Implementation name, e.g. "hip-gfx950" or "oracle-cpu". */
+const char *sps_backend(void);
+
+const char *sps_last_error(void);
+
+void sps_destroy(sps_handle *h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPS_H */

@@ -1,0 +1,131 @@
+!-------------------------------------------------------------------------------
+! Fixture-capture driver for the reference Fortran (test infrastructure only).
+!
+! This program is linked against the reference's own modules, compiled from the
+! untouched sources under /root/reference by oracle/Makefile.  It replaces only
+! the reference driver program (code/1_SPH_2018.f90): it reads the same
+! input.txt / <name>.dat / <name>.pts decks through the reference's Init_sph,
+! advances the reference's time_integration step by step exactly as the
+! reference driver does (code/1_SPH_2018.f90:72-102: itimestep_sph += 1,
+! time_integration, time_sph += dt_sph), and writes raw binary dumps of the
+! hot-path state and of the pair list in linked-list traversal order.
+! No output files of the reference (CSV / GiD) are written.
+!
+! usage:  sps_ref_dump OUTDIR NSTEPS [STEP ...]
+!   dumps state_000000.bin before the first step, and state_NNNNNN.bin plus
+!   pairs_NNNNNN.bin after every listed step.
+!-------------------------------------------------------------------------------
+program sps_ref_dump
+  use variable_types
+  use SPH_time_vars_2018
+  use SPH_global_vars_2018
+  use SPH_material_vars_2018, only: cgrav, ft_grav, dampingTG
+  use SPH_main_2018, only: Init_SPH, time_integration
+  implicit none
+
+  character(len=512) :: outdir, arg
+  integer :: nsteps, nargs, ia, istep, ndump
+  integer, allocatable :: dump_steps(:)
+
+  nargs = command_argument_count()
+  if (nargs < 2) then
+     write(*,*) 'usage: sps_ref_dump OUTDIR NSTEPS [STEP ...]'
+     stop 2
+  end if
+  call get_command_argument(1, outdir)
+  call get_command_argument(2, arg)
+  read(arg, *) nsteps
+  ndump = nargs - 2
+  allocate(dump_steps(max(ndump, 1)))
+  dump_steps = -1
+  do ia = 1, ndump
+     call get_command_argument(ia + 2, arg)
+     read(arg, *) dump_steps(ia)
+  end do
+
+  input_file = 997
+  chk_file = 998
+  output_file = 996
+  open(input_file, file='input.txt')
+  open(output_file, file='output.txt')
+
+  call Init_sph
+
+  time = 0.0; time_sph = 0.0; itimestep_sph = 0; current_ts = 0
+  read(input_file, *) text
+  read(input_file, *) dt, time_end, maxtimestep
+  read(input_file, *) text
+  read(input_file, *) print_step, save_step, plot_step
+  dt_sph = dt
+
+  call dump_state(0)
+
+  do istep = 1, nsteps
+     current_ts = current_ts + 1
+     itimestep_sph = itimestep_sph + 1
+     call time_integration
+     time_sph = time_sph + dt_sph
+     time = time + dt
+     if (any(dump_steps(1:max(ndump,1)) == istep)) then
+        call dump_state(istep)
+        call dump_pairs(istep)
+     end if
+  end do
+
+contains
+
+  subroutine dump_state(step)
+    integer, intent(in) :: step
+    character(len=640) :: fname
+    integer :: u, hdr(8), i
+    real(kind=DP) :: scal(16)
+
+    write(fname, '(A,A,I6.6,A)') trim(outdir), '/state_', step, '.bin'
+    open(newunit=u, file=trim(fname), access='stream', form='unformatted', status='replace')
+    hdr = (/ int(z'53505331'), nnode, nstress, ndummy, ntotal, ntotal2, itimestep_sph, ndummy2 /)
+    write(u) hdr
+    scal = 0.0_DP
+    scal(1) = time_sph; scal(2) = dt_sph; scal(3) = pi
+    scal(4) = D11; scal(5) = D22; scal(6) = D12; scal(7) = D33; scal(8) = D41; scal(9) = D42
+    scal(10) = alpha; scal(11) = beta; scal(12) = sml; scal(13) = dx; scal(14) = dy
+    scal(15) = ft_grav; scal(16) = dampingTG
+    write(u) scal
+    write(u) Xmin_Domain(1:2), Xmax_Domain(1:2), cgrav(1:2), props(1,1:20)
+    write(u) x(1:2, 1:ntotal2), vel(1:2, 1:ntotal2), stress(1:4, 1:ntotal2)
+    write(u) mass(1:ntotal2), rho(1:ntotal2), hsml(1:ntotal2)
+    write(u) itype(1:ntotal2), If_Out_Domain(1:ntotal2)
+    write(u) (internal_vars(1, i), i = 1, ntotal)
+    write(u) f_drucker(1:ntotal), Ddev_strn(1:ntotal)
+    write(u) grad_u(1:2, 1:2, 1:ntotal)
+    write(u) displ(1:2, 1:nnode), f_bound(1:2, 1:nnode), art_visc(1:2, 1:nnode)
+    write(u) x00(1:2, 1:ntotal2)
+    if (allocated(wall_position)) then
+       write(u) wall_position(1:ntotal2), horizontal_or_not(1:ntotal2)
+    end if
+    close(u)
+  end subroutine dump_state
+
+  subroutine dump_pairs(step)
+    integer, intent(in) :: step
+    character(len=640) :: fname
+    integer :: u, n
+    type(pairs), pointer :: p
+
+    n = 0
+    p => last
+    do while (associated(p))
+       n = n + 1
+       p => p%next
+    end do
+    write(fname, '(A,A,I6.6,A)') trim(outdir), '/pairs_', step, '.bin'
+    open(newunit=u, file=trim(fname), access='stream', form='unformatted', status='replace')
+    write(u) int(z'53505032'), n
+    p => last
+    do while (associated(p))
+       write(u) p%pair_i, p%pair_j, p%pint_type, p%w, p%dwdx, p%dwdy
+       p => p%next
+    end do
+    close(u)
+  end subroutine dump_pairs
+
+end program sps_ref_dump

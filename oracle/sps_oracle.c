@@ -1,0 +1,1002 @@
+/*
+ * sps_oracle.c - CPU oracle of the Stress-Particle SPH hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP product and the timed CPU
+ * baseline.  It restates code/2_SPH_main_2018.f90 and the hot-path routines of
+ * code/3_SPH_material_2018.f90 in plain C, one function per reference routine,
+ * with the reference's own algorithm: the pair list is built in discovery
+ * order, traversed in the order the reference's linked list has, and every
+ * pair loop scatters into both endpoints.  Mixed single/double precision
+ * follows Fortran's rules (a default-real operand met by a real(DP) one is
+ * widened, the operation is done in DP, the result is rounded when it is
+ * assigned to a default-real variable).  Build with -ffp-contract=off.
+ *
+ * Pinning (see tests/test_oracle_golden.py): every routine is checked
+ * bit-for-bit against dumps of the reference Fortran compiled from source
+ * (oracle/_ref), and against closed forms on small inputs.
+ *
+ * Two stated departures, neither observable:
+ *  - the dead store `adapt_stress = 0` (code/3_...:1961) is not executed;
+ *  - boundary_forces (code/2_...:1117-1161) visits, for each node, only the
+ *    inner-layer dummies whose double-precision offset can give a nonzero
+ *    term, in ascending dummy index; every skipped term is an exact +-0.
+ */
+#include "sps_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdarg.h>
+
+static _Thread_local char g_err[512];
+
+static int fail(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+const char *sps_last_error(void) { return g_err; }
+const char *sps_backend(void) { return "oracle-cpu"; }
+
+struct sps_handle {
+    sps_params p;
+    int nn, ns, nd, nd2, nt, nt2, niv;
+    int have_state;
+    int64_t steps;
+    /* state (Fortran layouts, 0-based) */
+    double *x, *vel, *stress, *mass, *rho, *hsml, *ivars, *fdr, *x00, *displ;
+    double *grad_u, *ddev, *f_bound, *art_visc;
+    int32_t *itype, *ood;
+    float *wallpos, *horiz;
+    /* start-of-step copies (2_...:114) */
+    double *x0, *vx0;
+    /* pair list in discovery order + traversal order */
+    int64_t pcap, n_pairs, m_pairs;
+    int32_t *pi, *pj, *pt;
+    float *pw, *pgx, *pgy;
+    int64_t *order;
+    int ndivx, ndivy;
+    /* grid scratch */
+    int *which_cell, *cell_list;
+    int *cell_info; /* 3 per cell: count, start, fill */
+    size_t cell_cap;
+    /* near-wall dummy grid for boundary_forces */
+    double bw_cell, bw_x0, bw_y0;
+    int bw_nx, bw_ny;
+    int *bw_start, *bw_list;
+    /* RK4 scratch */
+    double *vel0, *stress0, *rk_vel, *rk_stress, *rk_dev, *rhs1, *rhs2;
+    double *divf1, *divf2, *source_stress, *g_grav, *spin;
+    double *grad1, *grad2, *ae, *dummy_vel, *cspm_norm, *vel_tmp, *stress_tmp, *ivar_tmp;
+    float *av_tmp;
+};
+
+/* ------------------------------------------------------------------ utils */
+
+static void *xcalloc(size_t n, size_t sz)
+{
+    if (n == 0) n = 1;
+    return calloc(n, sz);
+}
+
+static int alloc_all(struct sps_handle *h)
+{
+    int nt2 = h->nt2, nt = h->nt, nn = h->nn;
+#define A(ptr, n, T) do { h->ptr = (T *)xcalloc((size_t)(n), sizeof(T)); if (!h->ptr) return -1; } while (0)
+    A(x, 2 * nt2, double); A(vel, 2 * nt2, double); A(stress, 4 * nt2, double);
+    A(mass, nt2, double); A(rho, nt2, double); A(hsml, nt2, double);
+    A(ivars, (size_t)h->niv * nt, double); A(fdr, nt, double); A(x00, 2 * nt2, double);
+    A(displ, 2 * nn, double); A(grad_u, 4 * nt, double); A(ddev, nt, double);
+    A(f_bound, 2 * nn, double); A(art_visc, 2 * nn, double);
+    A(itype, nt2, int32_t); A(ood, nt2, int32_t);
+    A(wallpos, nt2, float); A(horiz, nt2, float);
+    A(x0, 2 * nt, double); A(vx0, 2 * nt, double);
+    A(which_cell, nt2, int); A(cell_list, nt2, int);
+    A(vel0, 2 * nt, double); A(stress0, 4 * nt, double);
+    A(rk_vel, 2 * nt, double); A(rk_stress, 4 * nt, double); A(rk_dev, nt, double);
+    A(rhs1, 4 * nt, double); A(rhs2, 2 * nt, double);
+    A(divf1, 4 * nt, double); A(divf2, 2 * nt, double);
+    A(source_stress, 4 * nt, double); A(g_grav, 2 * nn, double); A(spin, 4 * nt, double);
+    A(grad1, 4 * nt, double); A(grad2, 6 * nt, double); A(ae, 5 * nt, double);
+    A(dummy_vel, 2 * nt2, double); A(cspm_norm, nt, double);
+    A(vel_tmp, 2 * nt, double); A(stress_tmp, 4 * nt, double); A(ivar_tmp, nt, double);
+    A(av_tmp, 2 * nn, float);
+#undef A
+    return 0;
+}
+
+static void free_all(struct sps_handle *h)
+{
+    void *ptrs[] = {h->x, h->vel, h->stress, h->mass, h->rho, h->hsml, h->ivars, h->fdr,
+                    h->x00, h->displ, h->grad_u, h->ddev, h->f_bound, h->art_visc,
+                    h->itype, h->ood, h->wallpos, h->horiz, h->x0, h->vx0,
+                    h->pi, h->pj, h->pt, h->pw, h->pgx, h->pgy, h->order,
+                    h->which_cell, h->cell_list, h->cell_info, h->bw_start, h->bw_list,
+                    h->vel0, h->stress0, h->rk_vel, h->rk_stress, h->rk_dev, h->rhs1,
+                    h->rhs2, h->divf1, h->divf2, h->source_stress, h->g_grav, h->spin,
+                    h->grad1, h->grad2, h->ae, h->dummy_vel, h->cspm_norm, h->vel_tmp,
+                    h->stress_tmp, h->ivar_tmp, h->av_tmp};
+    for (size_t k = 0; k < sizeof ptrs / sizeof ptrs[0]; ++k) free(ptrs[k]);
+}
+
+/* ------------------------------------------------------- Check_Out_Domain */
+/* code/2_SPH_main_2018.f90:1170-1194 (sticky flag) */
+static void check_out_domain(struct sps_handle *h)
+{
+    for (int i = 0; i < h->nt2; ++i)
+        for (int d = 0; d < 2; ++d) {
+            double dxx = (h->x[2 * i + d] - h->p.xmin_domain[d]) *
+                         (h->x[2 * i + d] - h->p.xmax_domain[d]);
+            if (dxx > 0.0) h->ood[i] = 1;
+        }
+}
+
+/* ----------------------------------------------------------------- kernel */
+/* code/2_SPH_main_2018.f90:1440-1492, skf = 1, ndimn = 2.  Computed in DP
+ * with the single-precision literals 2./3. and 3./2. of the source. */
+static void kernel_cubic(double r, const double dxv[2], double hs, double pi,
+                         double *w, double dw[2])
+{
+    double q = r / hs;
+    double factor = 15.0 / (7.0 * pi * hs * hs);
+    if (q >= 0.0 && q <= 1.0) {
+        *w = factor * ((double)(2.0f / 3.0f) - q * q + (q * (q * q)) / 2.0);
+        for (int k = 0; k < 2; ++k)
+            dw[k] = factor * (-2.0 + (double)(3.0f / 2.0f) * q) / (hs * hs) * dxv[k];
+    } else if (q > 1.0 && q <= 2.0) {
+        double t = 2.0 - q;
+        *w = factor * 1.0 / 6.0 * (t * (t * t));
+        for (int k = 0; k < 2; ++k)
+            dw[k] = -(factor * 1.0 / 6.0 * 3.0 * (t * t) / hs * (dxv[k] / r));
+    } else {
+        *w = 0.0;
+        dw[0] = dw[1] = 0.0;
+    }
+}
+
+/* ---------------------------------------------------------- grid_find_NEW */
+/* code/2_SPH_main_2018.f90:1199-1435 */
+static int push_pair(struct sps_handle *h, int it, int jt, double w, const double dw[2])
+{
+    if (h->n_pairs == h->pcap) {
+        int64_t nc = h->pcap ? h->pcap * 2 : 4096;
+        int32_t *a = realloc(h->pi, nc * sizeof *a), *b, *c;
+        if (!a) return -1; h->pi = a;
+        b = realloc(h->pj, nc * sizeof *b); if (!b) return -1; h->pj = b;
+        c = realloc(h->pt, nc * sizeof *c); if (!c) return -1; h->pt = c;
+        float *f1 = realloc(h->pw, nc * sizeof *f1); if (!f1) return -1; h->pw = f1;
+        float *f2 = realloc(h->pgx, nc * sizeof *f2); if (!f2) return -1; h->pgx = f2;
+        float *f3 = realloc(h->pgy, nc * sizeof *f3); if (!f3) return -1; h->pgy = f3;
+        int64_t *o = realloc(h->order, nc * sizeof *o); if (!o) return -1; h->order = o;
+        h->pcap = nc;
+    }
+    int64_t k = h->n_pairs++;
+    h->pi[k] = it;
+    h->pj[k] = jt;
+    h->pt[k] = 0;               /* 2_...:1374 */
+    h->pw[k] = (float)w;        /* FP32 pair fields, 5_...:19-21 */
+    h->pgx[k] = (float)dw[0];
+    h->pgy[k] = (float)dw[1];
+    return 0;
+}
+
+static int grid_find(struct sps_handle *h, int itimestep)
+{
+    const int nt2 = h->nt2;
+    double xmin[2], xmax[2], deltx[2];
+    int ndiv[2];
+    /* Task 1 (2_...:1245-1258) */
+    for (int d = 0; d < 2; ++d) {
+        xmin[d] = (double)1.e+10f;
+        xmax[d] = -xmin[d];
+        deltx[d] = 0.0;
+    }
+    for (int d = 0; d < 2; ++d) {
+        for (int i = 0; i < nt2; ++i) {
+            if (h->ood[i] == 1) continue;
+            double v = h->x[2 * i + d];
+            if (v < xmin[d]) xmin[d] = v;
+            if (v > xmax[d]) xmax[d] = v;
+            if (deltx[d] < h->hsml[i]) deltx[d] = h->hsml[i];
+        }
+        deltx[d] = deltx[d] * 2;
+        double length = xmax[d] - xmin[d];
+        ndiv[d] = (int)(length / deltx[d] + 1);
+        double length_new = ndiv[d] * deltx[d];
+        xmin[d] = xmin[d] - (length_new - length) / 2 - (double)0.001f * length;
+        xmax[d] = xmax[d] + (length_new - length) / 2 + (double)0.001f * length;
+    }
+    size_t ndivt = (size_t)ndiv[0] * (size_t)ndiv[1];
+    if (ndivt > ((size_t)1 << 31)) return fail(SPS_ECAPACITY, "cell grid too large (%zu cells)", ndivt);
+    if (ndivt > h->cell_cap) {
+        free(h->cell_info);
+        h->cell_info = malloc(3 * ndivt * sizeof(int));
+        if (!h->cell_info) return fail(SPS_ENOMEM, "cell grid allocation");
+        h->cell_cap = ndivt;
+    }
+    h->ndivx = ndiv[0];
+    h->ndivy = ndiv[1];
+    int *cnt = h->cell_info, *start = h->cell_info + ndivt, *fill = h->cell_info + 2 * ndivt;
+    memset(cnt, 0, ndivt * sizeof(int));
+    /* Task 2 (2_...:1277-1305): counting sort, ascending index inside a cell */
+    for (int i = 0; i < nt2; ++i) {
+        h->which_cell[i] = 0;
+        if (h->ood[i] == 1) continue;
+        int idiv[2];
+        for (int d = 0; d < 2; ++d) {
+            idiv[d] = (int)((h->x[2 * i + d] - xmin[d]) / deltx[d] + 1);
+            if (idiv[d] > ndiv[d]) idiv[d] = ndiv[d];
+        }
+        int c = ndiv[0] * (idiv[1] - 1) + idiv[0]; /* 1-based */
+        h->which_cell[i] = c;
+        cnt[c - 1] += 1;
+    }
+    int ipost = 0;
+    for (size_t c = 0; c < ndivt; ++c) {
+        start[c] = ipost;
+        ipost += cnt[c];
+        fill[c] = 0;
+    }
+    for (int i = 0; i < nt2; ++i) {
+        if (h->ood[i] == 1) continue;
+        int c = h->which_cell[i] - 1;
+        h->cell_list[start[c] + fill[c]++] = i;
+    }
+    /* Task 3 (2_...:1309-1401): half-stencil enumeration */
+    if (itimestep == 1) h->m_pairs = 0;   /* fresh list, 2_...:1290-1294 */
+    h->n_pairs = 0;
+    for (int idy = 1; idy <= ndiv[1]; ++idy)
+        for (int idx = 1; idx <= ndiv[0]; ++idx) {
+            int idivt = ndiv[0] * (idy - 1) + idx;
+            int npicsi = cnt[idivt - 1];
+            for (int ii = 1; ii <= npicsi; ++ii) {
+                int it = h->cell_list[start[idivt - 1] + ii - 1];
+                int idx0 = idx - 1 > 1 ? idx - 1 : 1;
+                int idx1 = idx + 1 < ndiv[0] ? idx + 1 : ndiv[0];
+                int idy1 = idy + 1 < ndiv[1] ? idy + 1 : ndiv[1];
+                for (int jdy = idy; jdy <= idy1; ++jdy)
+                    for (int jdx = idx0; jdx <= idx1; ++jdx) {
+                        int jdivt = ndiv[0] * (jdy - 1) + jdx;
+                        if (jdivt < idivt) continue;
+                        int jj = (idivt == jdivt) ? ii + 1 : 1;
+                        int npicsj = cnt[jdivt - 1];
+                        for (int jc = jj; jc <= npicsj; ++jc) {
+                            int jt = h->cell_list[start[jdivt - 1] + jc - 1];
+                            double dxv[2];
+                            dxv[0] = h->x[2 * it] - h->x[2 * jt];
+                            double driac = dxv[0] * dxv[0];
+                            dxv[1] = h->x[2 * it + 1] - h->x[2 * jt + 1];
+                            driac = driac + dxv[1] * dxv[1];
+                            double mhsml = (h->hsml[it] + h->hsml[jt]) / 2.0;
+                            double r = sqrt(driac);
+                            if (r < 2 * mhsml) {
+                                double w, dw[2];
+                                kernel_cubic(r, dxv, mhsml, h->p.pi, &w, dw);
+                                if (push_pair(h, it, jt, w, dw))
+                                    return fail(SPS_ENOMEM, "pair list allocation");
+                            }
+                        }
+                    }
+            }
+        }
+    /* traversal order of the reused linked list (2_...:1309-1319, 1361-1369,
+     * 1380-1383, 1398-1401): with M the high-water mark before this search,
+     * the list from `last` is [n, n-1, ..., M+1, 1, 2, ..., min(n, M)]. */
+    int64_t n = h->n_pairs, M = h->m_pairs, k = 0;
+    if (n > M) {
+        for (int64_t d = n - 1; d >= M; --d) h->order[k++] = d;
+        for (int64_t d = 0; d < M; ++d) h->order[k++] = d;
+        h->m_pairs = n;
+    } else {
+        for (int64_t d = 0; d < n; ++d) h->order[k++] = d;
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------ Pint_Update */
+/* code/3_SPH_material_2018.f90:1555-1615 */
+static void pint_update(struct sps_handle *h)
+{
+    for (int64_t k = 0; k < h->n_pairs; ++k) {
+        int i = h->pi[k], j = h->pj[k];
+        int ii = abs(h->itype[i]), jj = abs(h->itype[j]);
+        int isumm = h->itype[i] + h->itype[j];
+        if ((ii == 2 && jj == 1) || (ii == 2 && jj == 25) || (ii == 1 && jj == 25)) {
+            h->pi[k] = j;
+            h->pj[k] = i;
+            h->pgx[k] = -h->pgx[k];
+            h->pgy[k] = -h->pgy[k];
+        }
+        if (isumm == 3) h->pt[k] = 1;
+        else if (isumm == 2) h->pt[k] = 2;
+        else if (isumm == 4) h->pt[k] = 3;
+        else if (isumm == 27) h->pt[k] = 6;
+        else if (isumm == 26) h->pt[k] = 9;
+    }
+}
+
+/* ---------------------------------------------------------- adapt_stress2 */
+/* code/3_SPH_material_2018.f90:2066-2140, all of 1..ntotal, all DP */
+static void adapt_stress2(struct sps_handle *h)
+{
+    double tanfi = h->p.props[12], coh = h->p.props[13];
+    double alpha2 = tanfi / (sqrt(9 + 12 * (tanfi * tanfi)));
+    double kc = (3 * coh) / (sqrt(9 + 12 * (tanfi * tanfi)));
+    for (int i = 0; i < h->nt; ++i) {
+        double *s = h->stress + 4 * i;
+        double smean = (s[0] + s[1] + s[3]) / 3.0;
+        double dv[4] = {s[0] - smean, s[1] - smean, s[2], s[3] - smean};
+        double varj2 = dv[2] * dv[2] + 0.5 * (dv[0] * dv[0] + dv[1] * dv[1] + dv[3] * dv[3]);
+        double yield = -(alpha2 * 3 * smean) + kc;
+        if (yield < 0) {
+            s[0] = s[0] - smean + kc / (3 * alpha2);
+            s[1] = s[1] - smean + kc / (3 * alpha2);
+            s[3] = s[3] - smean + kc / (3 * alpha2);
+            smean = (s[0] + s[1] + s[3]) / 3.0;
+            dv[0] = s[0] - smean;
+            dv[1] = s[1] - smean;
+            dv[2] = s[2];
+            dv[3] = s[3] - smean;
+            varj2 = dv[2] * dv[2] + 0.5 * (dv[0] * dv[0] + dv[1] * dv[1] + dv[3] * dv[3]);
+            yield = -(alpha2 * 3 * smean) + kc;
+        }
+        if (yield < sqrt(varj2)) {
+            double rn = (-(3 * alpha2 * smean) + kc) / (sqrt(varj2));
+            if (sqrt(varj2) <= (double)10e-06f) rn = 0;
+            s[0] = rn * dv[0] + smean;
+            s[1] = rn * dv[1] + smean;
+            s[3] = rn * dv[3] + smean;
+            s[2] = rn * dv[2];
+        }
+    }
+}
+
+/* ---------------------------------------------------- stress_point_update */
+/* code/2_SPH_main_2018.f90:403-482, SP_SPH branch */
+static void stress_point_update(struct sps_handle *h)
+{
+    const int nn = h->nn, nt = h->nt;
+    memset(h->stress_tmp, 0, 4 * (size_t)nt * sizeof(double));
+    memset(h->vel_tmp, 0, 2 * (size_t)nt * sizeof(double));
+    memset(h->ivar_tmp, 0, (size_t)nt * sizeof(double));
+    memset(h->cspm_norm, 0, (size_t)nt * sizeof(double));
+    for (int64_t k = 0; k < h->n_pairs; ++k) {
+        int64_t d = h->order[k];
+        if (h->pt[d] != 1) continue;
+        int j = h->pi[d]; /* stress point */
+        int i = h->pj[d]; /* node */
+        double w = (double)h->pw[d];
+        double h1 = (h->mass[j] / h->rho[j]) * w;
+        double h2 = (h->mass[i] / h->rho[i]) * w;
+        for (int a = 0; a < 2; ++a) h->vel_tmp[2 * j + a] = h->vel_tmp[2 * j + a] + h->vel[2 * i + a] * h2;
+        for (int s = 0; s < 4; ++s)
+            h->stress_tmp[4 * i + s] = h->stress_tmp[4 * i + s] + h->stress[4 * j + s] * h1;
+        h->ivar_tmp[i] = h->ivar_tmp[i] + h->ivars[(size_t)h->niv * j] * h1;
+        h->cspm_norm[j] = h->cspm_norm[j] + (w * h->mass[i]) / h->rho[i];
+        h->cspm_norm[i] = h->cspm_norm[i] + (w * h->mass[j]) / h->rho[j];
+    }
+    for (int i = nn; i < nt; ++i) {
+        if (h->cspm_norm[i] == 0) continue;
+        h->vel[2 * i] = h->vel_tmp[2 * i] / h->cspm_norm[i];
+        h->vel[2 * i + 1] = h->vel_tmp[2 * i + 1] / h->cspm_norm[i];
+    }
+    for (int i = 0; i < nn; ++i) {
+        if (h->cspm_norm[i] != 0) {
+            for (int s = 0; s < 4; ++s) h->stress[4 * i + s] = h->stress_tmp[4 * i + s] / h->cspm_norm[i];
+            h->ivars[(size_t)h->niv * i] = h->ivar_tmp[i] / h->cspm_norm[i];
+        } else {
+            h->vel[2 * i] = 0;
+            h->vel[2 * i + 1] = 0;
+        }
+    }
+}
+
+/* -------------------------------------------------------- get_derivatives */
+/* code/2_SPH_main_2018.f90:487-648.  fi1 is vel, fi2 is stress (the actual
+ * arguments at 2_...:725), so the type-6 copy into stress(:,dummy) is seen
+ * through fi2 as in the reference. */
+#define G1(a, b, p) h->grad1[4 * (p) + 2 * (b) + (a)] /* grad1_tmp(a,b,p) */
+#define G2(s, b, p) h->grad2[6 * (p) + 3 * (b) + (s)] /* grad2_tmp(s,b,p) */
+#define AE(c, p) h->ae[5 * (p) + (c)]
+static void get_derivatives(struct sps_handle *h)
+{
+    const int nn = h->nn, nt = h->nt;
+    const double *x = h->x, *m = h->mass, *rho = h->rho;
+    double *vel = h->vel, *st = h->stress;
+    memset(h->grad1, 0, 4 * (size_t)nt * sizeof(double));
+    memset(h->grad2, 0, 6 * (size_t)nt * sizeof(double));
+    memset(h->divf1, 0, 4 * (size_t)nt * sizeof(double));
+    memset(h->divf2, 0, 2 * (size_t)nt * sizeof(double));
+    memset(h->ae, 0, 5 * (size_t)nt * sizeof(double));
+    memset(h->dummy_vel, 0, 2 * (size_t)h->nt2 * sizeof(double));
+    for (int64_t k = 0; k < h->n_pairs; ++k) {
+        int64_t d = h->order[k];
+        int j = h->pi[d], i = h->pj[d];
+        double gx = (double)h->pgx[d], gy = (double)h->pgy[d];
+        double h1 = gx * m[i] / rho[i], h2 = 0.0;
+        int t = h->pt[d];
+        if (t == 1) {
+            for (int a = 0; a < 2; ++a) {
+                G1(a, 0, j) = G1(a, 0, j) + (vel[2 * i + a] - vel[2 * j + a]) * h1;
+                h2 = gy * m[i] / rho[i];
+                G1(a, 1, j) = G1(a, 1, j) + (vel[2 * i + a] - vel[2 * j + a]) * h2;
+            }
+            for (int s = 0; s < 3; ++s) {
+                double g1 = gx * (st[4 * i + s] / (rho[i] * rho[i]) + st[4 * j + s] / (rho[j] * rho[j]));
+                G2(s, 0, i) = G2(s, 0, i) - m[j] * g1;
+                double g2 = gy * (st[4 * i + s] / (rho[i] * rho[i]) + st[4 * j + s] / (rho[j] * rho[j]));
+                G2(s, 1, i) = G2(s, 1, i) - m[j] * g2;
+            }
+            if (h->p.cspm) {
+                double h1b = -(gx * m[j] / rho[j]);
+                double h2b = -(gy * m[j] / rho[j]);
+                AE(0, j) = AE(0, j) + (x[2 * i] - x[2 * j]) * h1;
+                AE(1, j) = AE(1, j) + (x[2 * i + 1] - x[2 * j + 1]) * h1;
+                AE(2, j) = AE(2, j) + (x[2 * i] - x[2 * j]) * h2;
+                AE(3, j) = AE(3, j) + (x[2 * i + 1] - x[2 * j + 1]) * h2;
+                AE(0, i) = AE(0, i) + (x[2 * j] - x[2 * i]) * h1b;
+                AE(1, i) = AE(1, i) + (x[2 * j + 1] - x[2 * i + 1]) * h1b;
+                AE(2, i) = AE(2, i) + (x[2 * j] - x[2 * i]) * h2b;
+                AE(3, i) = AE(3, i) + (x[2 * j + 1] - x[2 * i + 1]) * h2b;
+            }
+        } else if (t == 9) { /* i: stress point, j: dummy */
+            double beta_max = 1.5, vel_wall = 0.0, da, db;
+            double wall = (double)h->wallpos[j];
+            if (h->horiz[j] == 1.0f) {
+                da = fabs(x[2 * i + 1] - wall);
+                db = fabs(x[2 * j + 1] - wall);
+            } else {
+                da = fabs(x[2 * i] - wall);
+                db = fabs(x[2 * j] - wall);
+            }
+            double bt = 1 + (db / da);
+            double beta = beta_max < bt ? beta_max : bt; /* min(beta_max, 1+db/da) */
+            for (int a = 0; a < 2; ++a)
+                h->dummy_vel[2 * j + a] = vel[2 * i + a] * (1 - beta) + beta * vel_wall;
+            h1 = gx * m[j] / rho[j];
+            for (int a = 0; a < 2; ++a) {
+                G1(a, 0, i) = G1(a, 0, i) + (vel[2 * i + a] - h->dummy_vel[2 * j + a]) * h1;
+                h2 = gy * m[j] / rho[j];
+                G1(a, 1, i) = G1(a, 1, i) + (vel[2 * i + a] - h->dummy_vel[2 * j + a]) * h2;
+            }
+        } else if (t == 6) { /* i: node, j: dummy */
+            for (int s = 0; s < 4; ++s) st[4 * j + s] = st[4 * i + s];
+            for (int s = 0; s < 3; ++s) {
+                double a1 = gx * (st[4 * i + s] / (rho[i] * rho[i]) + st[4 * j + s] / (rho[j] * rho[j]));
+                G2(s, 0, i) = G2(s, 0, i) - m[j] * a1;
+                double a2 = gy * (st[4 * i + s] / (rho[i] * rho[i]) + st[4 * j + s] / (rho[j] * rho[j]));
+                G2(s, 1, i) = G2(s, 1, i) - m[j] * a2;
+            }
+        }
+    }
+    if (h->p.cspm) { /* 2_...:594-604 */
+        double tol = h->p.cspm_det_tol;
+        for (int i = 0; i < nt; ++i) {
+            AE(4, i) = AE(0, i) * AE(3, i) - AE(1, i) * AE(2, i);
+            if (fabs(AE(4, i)) < tol) {
+                AE(4, i) = 1; AE(0, i) = 1; AE(1, i) = 0; AE(2, i) = 0; AE(3, i) = 1;
+            } else {
+                AE(4, i) = 1 / AE(4, i);
+            }
+        }
+        /* 2_...:616-627: the second statement of each pair reads the already
+         * corrected first column, as the array assignments in sequence do */
+        for (int a = 0; a < 2; ++a) {
+            for (int p = nn; p < nt; ++p) G1(a, 0, p) = AE(4, p) * (AE(0, p) * G1(a, 0, p) + AE(1, p) * G1(a, 1, p));
+            for (int p = nn; p < nt; ++p) G1(a, 1, p) = AE(4, p) * (AE(2, p) * G1(a, 0, p) + AE(3, p) * G1(a, 1, p));
+            for (int p = 0; p < nn; ++p) G2(a, 0, p) = AE(4, p) * (AE(0, p) * G2(a, 0, p) + AE(1, p) * G2(a, 1, p));
+            for (int p = 0; p < nn; ++p) G2(a, 1, p) = AE(4, p) * (AE(2, p) * G2(a, 0, p) + AE(3, p) * G2(a, 1, p));
+        }
+    }
+    const sps_params *P = &h->p;
+    for (int p = nn; p < nt; ++p) { /* 2_...:633-636 */
+        h->divf1[4 * p + 0] = -(P->D11 * G1(0, 0, p) + P->D12 * G1(1, 1, p));
+        h->divf1[4 * p + 1] = -(P->D12 * G1(0, 0, p) + P->D22 * G1(1, 1, p));
+        h->divf1[4 * p + 2] = -(P->D33 * G1(1, 0, p) + P->D33 * G1(0, 1, p));
+        h->divf1[4 * p + 3] = -(P->D41 * G1(0, 0, p) + P->D42 * G1(1, 1, p));
+    }
+    for (int p = 0; p < nn; ++p) { /* 2_...:641-642 */
+        h->divf2[2 * p + 0] = -(G2(0, 0, p) + G2(2, 1, p));
+        h->divf2[2 * p + 1] = -(G2(2, 0, p) + G2(1, 1, p));
+    }
+    memcpy(h->grad_u, h->grad1, 4 * (size_t)nt * sizeof(double)); /* 2_...:644 */
+}
+#undef G1
+#undef G2
+#undef AE
+
+/* --------------------------------------------- plastic_terms / drucker_prager */
+/* code/3_SPH_material_2018.f90:1867-1933 -> 1937-2062 -> 2557-2635.
+ * Returns the source Gs(4) = -G and der_intvars(1). */
+static void plastic_terms(struct sps_handle *h, int ie, double time_sph, double Gs[4], double *der1)
+{
+    const double *props = h->p.props;
+    int ntype_eco = (int)props[0];
+    int ncrit = (int)props[1];
+    double G[4] = {0, 0, 0, 0}, vivel[4] = {0, 0, 0, 0};
+    Gs[0] = Gs[1] = Gs[2] = Gs[3] = 0.0;
+    if (ntype_eco > 1) {
+        if (ncrit == 12) {
+            const double *st = h->stress + 4 * ie;
+            double stress2[4] = {st[0], st[1], st[2], st[3]};
+            double f0 = h->fdr[ie];
+            float tanfi = (float)props[12], coh = (float)props[13];
+            float young = (float)props[2], poiss = (float)props[3];
+            float smean = (float)((stress2[0] + stress2[1] + stress2[3]) / 3.0);
+            float dv[4];
+            dv[0] = (float)(stress2[0] - (double)smean);
+            dv[1] = (float)(stress2[1] - (double)smean);
+            dv[2] = (float)stress2[2];
+            dv[3] = (float)(stress2[3] - (double)smean);
+            float varj2 = dv[2] * dv[2] + 0.5f * (dv[0] * dv[0] + dv[1] * dv[1] + dv[3] * dv[3]);
+            float vari1 = 3 * smean;
+            const double *gu = h->grad_u + 4 * ie; /* [g11, g21, g12, g22] */
+            float eps11 = (float)gu[0];
+            float eps12 = (float)(0.5 * (gu[2] + gu[1]));
+            float eps22 = (float)gu[3];
+            float emean = eps11 + eps22;
+            float alpha2 = tanfi / (sqrtf(9 + 12 * (tanfi * tanfi)));
+            float kc = (3 * coh) / (sqrtf(9 + 12 * (tanfi * tanfi)));
+            float yield = -(alpha2 * vari1) + kc;
+            h->fdr[ie] = (double)(sqrtf(varj2) - yield);
+            float f1 = (float)h->fdr[ie];
+            double df = (double)f1 - f0;
+            float G_mod = young / (2.0f * (1.0f + poiss));
+            float K_mod = young / (3.0f * (1.0f - 2.0f * poiss));
+            float s_eps = dv[0] * eps11 + 2 * dv[2] * eps12 + dv[1] * eps22;
+            if (time_sph > 0 && f1 >= 0 && df >= 0 && sqrtf(varj2) >= 10e-06f) {
+                float G1v[4];
+                for (int i = 0; i < 4; ++i) G1v[i] = (G_mod / sqrtf(varj2)) * dv[i];
+                float lambda_1 = 3 * alpha2 * K_mod * emean;
+                float lambda_2 = (G_mod / sqrtf(varj2)) * s_eps;
+                float lambda_3 = G_mod;
+                float G2 = (lambda_1 + lambda_2) / lambda_3;
+                for (int i = 0; i < 4; ++i) G[i] = (double)(G1v[i] * G2);
+                vivel[0] = (double)((1.0f / 6.0f) * (1.0f / sqrtf(varj2)) * (2 * dv[0] - dv[1] - dv[3]));
+                vivel[1] = (double)((1.0f / 6.0f) * (1.0f / sqrtf(varj2)) * (2 * dv[1] - dv[0] - dv[3]));
+                vivel[2] = (double)((1.0f / sqrtf(varj2)) * dv[2]);
+                vivel[3] = (double)((1.0f / 6.0f) * (1 / sqrtf(varj2)) * (2 * dv[3] - dv[0] - dv[1]));
+                for (int i = 0; i < 4; ++i) vivel[i] = vivel[i] * (double)G2;
+            }
+            for (int i = 0; i < 4; ++i) Gs[i] = -G[i];
+        }
+        /* Get_derivative_intvars (3_...:2614-2617) */
+        *der1 = sqrt((2.0 * (vivel[0] * vivel[0] + vivel[1] * vivel[1] + vivel[3] * vivel[3]) +
+                      vivel[2] * vivel[2]) / 3.0);
+    } else {
+        *der1 = 0.0;
+    }
+}
+
+/* ---------------------------------------------------------- gravity_force */
+/* code/3_SPH_material_2018.f90:2640-2702: the time factor at step time */
+static double gravity_factor(const sps_params *P, double time_sph)
+{
+    double factg;
+    if (P->ic_grav == 0) return 0.0;
+    double tt0 = -1000.0, tt1 = 0.0;
+    int ip, found = 0;
+    for (ip = 0; ip < P->tcurve_npts - 1; ++ip) {
+        tt0 = P->tcurve_t[ip];
+        tt1 = P->tcurve_t[ip + 1];
+        if (time_sph >= tt0 && time_sph <= tt1) { found = 1; break; }
+        tt0 = -1000.0;
+    }
+    if (tt0 >= 0.0 && found) {
+        double xi = (time_sph - tt0) / (tt1 - tt0);
+        factg = (1.0 - xi) * (double)P->tcurve_f[ip] + xi * (double)P->tcurve_f[ip + 1];
+    } else {
+        factg = 0.0;
+    }
+    return factg * P->ft_grav;
+}
+
+static void gravity_force(struct sps_handle *h, double factg)
+{
+    for (int i = 0; i < h->nn; ++i)
+        for (int a = 0; a < 2; ++a) {
+            double g = h->p.ic_grav ? factg * h->p.cgrav[a] : 0.0;
+            h->g_grav[2 * i + a] = g - h->p.damping * h->vel[2 * i + a];
+        }
+}
+
+/* -------------------------------------------------------- boundary_forces */
+/* code/2_SPH_main_2018.f90:1117-1161 (test = 2): every node against every
+ * inner-layer dummy, FP32 terms summed into DP.  A term is nonzero only when
+ * 0 < r2 < 1.5*d0 in single precision; dummies are binned on a grid whose
+ * cell exceeds that reach, so each node gathers its candidates from 3x3
+ * cells and adds them in ascending dummy index like the reference j loop. */
+static float bf_d0(const struct sps_handle *h) { return (float)(h->p.dx / 2.); }
+
+static int bw_setup(struct sps_handle *h)
+{
+    free(h->bw_start); free(h->bw_list);
+    h->bw_start = NULL; h->bw_list = NULL;
+    int nd2 = h->nd2;
+    if (nd2 == 0) return 0;
+    double reach = 2.0 * (double)(1.5f * bf_d0(h)) + 1e-12;
+    double x0 = 1e300, y0 = 1e300, x1 = -1e300, y1 = -1e300;
+    for (int k = 0; k < nd2; ++k) {
+        int j = h->nt + k;
+        double a = h->x[2 * j], b = h->x[2 * j + 1];
+        if (a < x0) x0 = a; if (a > x1) x1 = a;
+        if (b < y0) y0 = b; if (b > y1) y1 = b;
+    }
+    h->bw_cell = reach;
+    h->bw_x0 = x0 - reach;
+    h->bw_y0 = y0 - reach;
+    h->bw_nx = (int)((x1 - h->bw_x0) / reach) + 3;
+    h->bw_ny = (int)((y1 - h->bw_y0) / reach) + 3;
+    if ((double)h->bw_nx * h->bw_ny > 5e7) return -1;
+    size_t nc = (size_t)h->bw_nx * h->bw_ny;
+    h->bw_start = calloc(nc + 1, sizeof(int));
+    h->bw_list = calloc((size_t)nd2, sizeof(int));
+    if (!h->bw_start || !h->bw_list) return -1;
+    for (int k = 0; k < nd2; ++k) {
+        int j = h->nt + k;
+        int cx = (int)((h->x[2 * j] - h->bw_x0) / reach), cy = (int)((h->x[2 * j + 1] - h->bw_y0) / reach);
+        h->bw_start[(size_t)cy * h->bw_nx + cx + 1]++;
+    }
+    for (size_t c = 0; c < nc; ++c) h->bw_start[c + 1] += h->bw_start[c];
+    int *fill = calloc(nc, sizeof(int));
+    if (!fill) return -1;
+    for (int k = 0; k < nd2; ++k) { /* ascending index inside a cell */
+        int j = h->nt + k;
+        int cx = (int)((h->x[2 * j] - h->bw_x0) / reach), cy = (int)((h->x[2 * j + 1] - h->bw_y0) / reach);
+        size_t c = (size_t)cy * h->bw_nx + cx;
+        h->bw_list[h->bw_start[c] + fill[c]++] = j;
+    }
+    free(fill);
+    return 0;
+}
+
+static int cmp_int(const void *a, const void *b)
+{
+    int x = *(const int *)a, y = *(const int *)b;
+    return (x > y) - (x < y);
+}
+
+static float bf_term_scale(void)
+{
+    float c = 20;
+    return 0.01f * (c * c);
+}
+
+static void boundary_forces(struct sps_handle *h)
+{
+    memset(h->f_bound, 0, 2 * (size_t)h->nn * sizeof(double));
+    if (h->nd2 == 0) return;
+    const float d0 = bf_d0(h);
+    const float scale = bf_term_scale();
+    const double reach = h->bw_cell;
+    int cand[4096];
+    for (int i = 0; i < h->nn; ++i) {
+        double xi = h->x[2 * i], yi = h->x[2 * i + 1];
+        int cx = (int)floor((xi - h->bw_x0) / reach), cy = (int)floor((yi - h->bw_y0) / reach);
+        int nc = 0;
+        for (int by = cy - 1; by <= cy + 1; ++by) {
+            if (by < 0 || by >= h->bw_ny) continue;
+            for (int bx = cx - 1; bx <= cx + 1; ++bx) {
+                if (bx < 0 || bx >= h->bw_nx) continue;
+                size_t c = (size_t)by * h->bw_nx + bx;
+                for (int q = h->bw_start[c]; q < h->bw_start[c + 1] && nc < 4096; ++q) cand[nc++] = h->bw_list[q];
+            }
+        }
+        qsort(cand, (size_t)nc, sizeof(int), cmp_int);
+        for (int q = 0; q < nc; ++q) {
+            int j = cand[q];
+            float r[2];
+            r[0] = (float)(h->x[2 * i] - h->x[2 * j]);
+            r[1] = (float)(h->x[2 * i + 1] - h->x[2 * j + 1]);
+            float r2 = sqrtf(r[0] * r[0] + r[1] * r[1]);
+            float f2;
+            float hh = (float)(0.5 * (h->hsml[i] + h->hsml[j]));
+            float r_bound = r2 / (0.75f * hh);
+            if (r2 > 0 && r2 < 1.5f * d0) f2 = 1 - (r2 / (1.5f * d0));
+            else f2 = 0.0f;
+            float f;
+            if (0 < r_bound && r_bound <= 2.0f / 3.0f) f = 2.0f / 3.0f;
+            else if (2.0f / 3.0f < r_bound && r_bound <= 1) f = 2 * r_bound - 1.5f * (r_bound * r_bound);
+            else if (1 < r_bound && r_bound < 2) f = 0.5f * ((2 - r_bound) * (2 - r_bound));
+            else f = 0.0f;
+            for (int a = 0; a < 2; ++a)
+                h->f_bound[2 * i + a] = h->f_bound[2 * i + a] + (double)(scale * f2 * f * (r[a] / (r2 * r2)));
+        }
+    }
+}
+
+/* --------------------------------------------------- artificial_viscosity */
+/* code/2_SPH_main_2018.f90:826-904, FP32 locals and accumulators */
+static void artificial_viscosity(struct sps_handle *h)
+{
+    const int nn = h->nn;
+    const double *x = h->x, *v = h->vel, *m = h->mass;
+    memset(h->av_tmp, 0, 2 * (size_t)nn * sizeof(float));
+    float visc = 0.0f;
+    const float c = 600;
+    for (int64_t k = 0; k < h->n_pairs; ++k) {
+        int64_t d = h->order[k];
+        if (h->pt[d] != 3) continue;
+        int i = h->pi[d], j = h->pj[d];
+        float xij = (float)(x[2 * i] - x[2 * j]);
+        float yij = (float)(x[2 * i + 1] - x[2 * j + 1]);
+        float hh = (float)(0.5 * (h->hsml[i] + h->hsml[j]));
+        float rho2 = (float)(0.5 * (h->rho[i] + h->rho[j]));
+        float div_u = (float)((double)xij * (v[2 * i] - v[2 * j]));
+        div_u = (float)((double)div_u + (double)yij * (v[2 * i + 1] - v[2 * j + 1]));
+        float sr = sqrtf(xij * xij + yij * yij);
+        float theta = (hh * div_u) / (sr * sr + 0.01f * (hh * hh));
+        if (div_u < 0) {
+            visc = (float)((-(h->p.alpha * (double)c * (double)theta) + h->p.beta * (double)(theta * theta)) /
+                           (double)rho2);
+        } else if (div_u >= 0) {
+            visc = 0;
+        }
+        float gx = h->pgx[d], gy = h->pgy[d];
+        h->av_tmp[2 * i] = (float)((double)h->av_tmp[2 * i] + (double)(visc * gx) * m[j]);
+        h->av_tmp[2 * j] = (float)((double)h->av_tmp[2 * j] - (double)(visc * gx) * m[i]);
+        h->av_tmp[2 * i + 1] = (float)((double)h->av_tmp[2 * i + 1] + (double)(visc * gy) * m[j]);
+        h->av_tmp[2 * j + 1] = (float)((double)h->av_tmp[2 * j + 1] - (double)(visc * gy) * m[i]);
+    }
+    for (int i = 0; i < 2 * nn; ++i) h->art_visc[i] = -(double)h->av_tmp[i];
+}
+
+/* -------------------------------------------------------------------- RK4 */
+/* code/2_SPH_main_2018.f90:653-802 */
+static void rk4(struct sps_handle *h, double dt_sph, double time_sph)
+{
+    const int nn = h->nn, nt = h->nt;
+    const double f1rk[4] = {0., 0.5, 0.5, 1.0}, f2rk[4] = {1., 2., 2., 1.0};
+    const int ncrit = (int)h->p.props[1];
+    memcpy(h->vel0, h->vel, 2 * (size_t)nt * sizeof(double));
+    memcpy(h->stress0, h->stress, 4 * (size_t)nt * sizeof(double));
+    memset(h->rk_stress, 0, 4 * (size_t)nt * sizeof(double));
+    memset(h->rk_vel, 0, 2 * (size_t)nt * sizeof(double));
+    memset(h->rk_dev, 0, (size_t)nt * sizeof(double));
+    memset(h->rhs1, 0, 4 * (size_t)nt * sizeof(double));
+    memset(h->rhs2, 0, 2 * (size_t)nt * sizeof(double));
+    memset(h->source_stress, 0, 4 * (size_t)nt * sizeof(double));
+    memset(h->spin, 0, 4 * (size_t)nt * sizeof(double));
+    memset(h->art_visc, 0, 2 * (size_t)nn * sizeof(double));
+    memset(h->vel, 0, 2 * (size_t)h->nt2 * sizeof(double));
+    memset(h->stress, 0, 4 * (size_t)h->nt2 * sizeof(double));
+    const double factg = gravity_factor(&h->p, time_sph);
+    const int do_bforce = h->p.inside_approach && (h->p.bforce_gate ? 1 : h->p.dummy_nodes);
+    for (int s = 0; s < 4; ++s) {
+        const double c1 = f1rk[s] * (dt_sph);
+        for (int i = 0; i < nn; ++i)
+            for (int a = 0; a < 2; ++a)
+                h->vel[2 * i + a] = h->vel0[2 * i + a] + c1 * h->rhs2[2 * i + a];
+        for (int i = nn; i < nt; ++i)
+            for (int q = 0; q < 4; ++q)
+                h->stress[4 * i + q] = h->stress0[4 * i + q] + c1 * h->rhs1[4 * i + q];
+        if (ncrit == 12) adapt_stress2(h);
+        stress_point_update(h);
+        if (ncrit == 12) adapt_stress2(h);
+        get_derivatives(h);
+        for (int j = nn; j < nt; ++j) {
+            double Gs[4], der1;
+            plastic_terms(h, j, time_sph, Gs, &der1);
+            for (int q = 0; q < 4; ++q) h->source_stress[4 * j + q] = Gs[q];
+            h->rk_dev[j] = h->rk_dev[j] + der1 * f2rk[s];
+        }
+        gravity_force(h, factg);
+        if (do_bforce) boundary_forces(h);
+        if (h->p.alpha > 0 || h->p.beta > 0) artificial_viscosity(h);
+        if (h->p.update_x) { /* 2_...:751-757 with get_spin_rate_tensor 2_...:1021-1034 */
+            for (int i = nn; i < nt; ++i) {
+                const double *gu = h->grad_u + 4 * i;
+                double om1 = 0.5 * (gu[2] - gu[1]);
+                double om2 = -(0.5 * (gu[2] - gu[1]));
+                const double *st = h->stress + 4 * i;
+                h->spin[4 * i + 0] = 2 * om1 * st[2];
+                h->spin[4 * i + 1] = 2 * om2 * st[2];
+                h->spin[4 * i + 2] = om2 * st[0] + om1 * st[1];
+            }
+        }
+        for (int i = nn; i < nt; ++i)
+            for (int q = 0; q < 4; ++q)
+                h->rhs1[4 * i + q] = -h->divf1[4 * i + q] + h->spin[4 * i + q] + h->source_stress[4 * i + q];
+        for (int i = 0; i < nn; ++i)
+            for (int a = 0; a < 2; ++a)
+                h->rhs2[2 * i + a] = -h->divf2[2 * i + a] + h->g_grav[2 * i + a] + h->art_visc[2 * i + a] +
+                                     h->f_bound[2 * i + a] + 0.0;
+        for (int i = 0; i < nn; ++i)
+            for (int a = 0; a < 2; ++a)
+                h->rk_vel[2 * i + a] = h->rk_vel[2 * i + a] + f2rk[s] * h->rhs2[2 * i + a];
+        for (int i = nn; i < nt; ++i)
+            for (int q = 0; q < 4; ++q)
+                h->rk_stress[4 * i + q] = h->rk_stress[4 * i + q] + f2rk[s] * h->rhs1[4 * i + q];
+    }
+    for (int i = 0; i < nn; ++i)
+        for (int a = 0; a < 2; ++a)
+            h->vel[2 * i + a] = h->vel0[2 * i + a] + (dt_sph / 6) * h->rk_vel[2 * i + a];
+    for (int i = nn; i < nt; ++i)
+        for (int q = 0; q < 4; ++q)
+            h->stress[4 * i + q] = h->stress0[4 * i + q] + (dt_sph / 6) * h->rk_stress[4 * i + q];
+    if (ncrit == 12) adapt_stress2(h);
+    for (int i = 0; i < nt; ++i) h->ddev[i] = h->rk_dev[i] / 6;
+}
+
+/* ------------------------------------------------------- time_integration */
+/* code/2_SPH_main_2018.f90:78-184 for the supported flag set */
+static int time_integration(struct sps_handle *h, int itimestep, double time_sph, double dt)
+{
+    const int nn = h->nn, nt = h->nt;
+    const int ncrit = (int)h->p.props[1];
+    check_out_domain(h);
+    int rc = grid_find(h, itimestep);
+    if (rc) return rc;
+    pint_update(h);
+    memcpy(h->vx0, h->vel, 2 * (size_t)nt * sizeof(double));
+    memcpy(h->x0, h->x, 2 * (size_t)nt * sizeof(double));
+    rk4(h, dt, time_sph);
+    for (int i = nn; i < nt; ++i) /* update_strain, 3_...:1847-1863 */
+        h->ivars[(size_t)h->niv * i] = h->ivars[(size_t)h->niv * i] + dt * h->ddev[i];
+    stress_point_update(h);
+    if (ncrit == 12) adapt_stress2(h);
+    if (h->p.update_x) {
+        for (int i = 0; i < nt; ++i)
+            for (int a = 0; a < 2; ++a) {
+                float vel_half = (float)(0.5 * (h->vx0[2 * i + a] + h->vel[2 * i + a]));
+                h->x[2 * i + a] = h->x0[2 * i + a] + (double)vel_half * dt;
+            }
+        for (int i = 0; i < nn; ++i)
+            for (int a = 0; a < 2; ++a) h->displ[2 * i + a] = h->x[2 * i + a] - h->x00[2 * i + a];
+    } else {
+        for (int i = 0; i < nn; ++i)
+            for (int a = 0; a < 2; ++a)
+                h->displ[2 * i + a] = h->displ[2 * i + a] + 0.5 * (h->vx0[2 * i + a] + h->vel[2 * i + a]) * dt;
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ C-ABI */
+
+static int check_params(const sps_params *p)
+{
+    if (p->abi_version != SPS_ABI_VERSION) return fail(SPS_EINVAL, "abi_version %d != %d", p->abi_version, SPS_ABI_VERSION);
+    if (p->nnode <= 0 || p->nstress <= 0 || p->ndummy < 0 || p->ndummy2 < 0 || p->ndummy2 > p->ndummy)
+        return fail(SPS_EINVAL, "bad particle counts");
+    if (p->nint_vars < 1) return fail(SPS_EINVAL, "nint_vars must be >= 1");
+    if (p->ndimn != 2 || p->nstre != 4) return fail(SPS_EINVAL, "only ndimn=2, nstre=4 (plane strain) are supported");
+    if (p->skf != 1) return fail(SPS_EINVAL, "only the cubic spline kernel (skf=1) is supported");
+    if (!p->sp_sph || !p->inside_approach) return fail(SPS_EINVAL, "only stress-particle SPH with the inside approach is supported");
+    if (p->xsph || p->cont_density || p->art_stress || p->sph_shift || p->no_bcs)
+        return fail(SPS_EINVAL, "XSPH, continuity density, artificial stress, stress-point shifting and prescribed BCs are not supported");
+    int ntype_eco = (int)p->props[0], ncrit = (int)p->props[1];
+    if (ntype_eco > 1 && ncrit <= 5) return fail(SPS_EINVAL, "Perzyna models (ncrit<=5) are not supported");
+    if (p->ic_grav && (p->tcurve_npts < 2 || p->tcurve_npts > SPS_MAX_TCURVE))
+        return fail(SPS_EINVAL, "gravity time curve needs 2..%d points", SPS_MAX_TCURVE);
+    if (p->mode != SPS_MODE_ORDERED && p->mode != SPS_MODE_FAST) return fail(SPS_EINVAL, "unknown mode");
+    return 0;
+}
+
+int sps_create(const sps_params *params, sps_handle **out)
+{
+    if (!params || !out) return fail(SPS_EINVAL, "null argument");
+    int rc = check_params(params);
+    if (rc) return rc;
+    struct sps_handle *h = calloc(1, sizeof *h);
+    if (!h) return fail(SPS_ENOMEM, "handle allocation");
+    h->p = *params;
+    h->nn = params->nnode;
+    h->ns = params->nstress;
+    h->nd = params->ndummy;
+    h->nd2 = params->ndummy2;
+    h->nt = h->nn + h->ns;
+    h->nt2 = h->nt + h->nd;
+    h->niv = params->nint_vars;
+    if (alloc_all(h)) {
+        free_all(h);
+        free(h);
+        return fail(SPS_ENOMEM, "state allocation");
+    }
+    *out = h;
+    return 0;
+}
+
+int sps_set_state(sps_handle *h, const sps_state *st)
+{
+    if (!h || !st) return fail(SPS_EINVAL, "null argument");
+    if (!st->x || !st->vel || !st->stress || !st->mass || !st->rho || !st->hsml || !st->itype ||
+        !st->if_out_domain || !st->internal_vars || !st->f_drucker || !st->x00 || !st->displ)
+        return fail(SPS_EINVAL, "sps_set_state: a required array is NULL");
+    if (h->nd > 0 && (!st->wall_position || !st->horizontal_or_not))
+        return fail(SPS_EINVAL, "sps_set_state: wall_position/horizontal_or_not required with dummies");
+    size_t n2 = (size_t)h->nt2, n = (size_t)h->nt;
+    memcpy(h->x, st->x, 2 * n2 * sizeof(double));
+    memcpy(h->vel, st->vel, 2 * n2 * sizeof(double));
+    memcpy(h->stress, st->stress, 4 * n2 * sizeof(double));
+    memcpy(h->mass, st->mass, n2 * sizeof(double));
+    memcpy(h->rho, st->rho, n2 * sizeof(double));
+    memcpy(h->hsml, st->hsml, n2 * sizeof(double));
+    memcpy(h->itype, st->itype, n2 * sizeof(int32_t));
+    memcpy(h->ood, st->if_out_domain, n2 * sizeof(int32_t));
+    memcpy(h->ivars, st->internal_vars, (size_t)h->niv * n * sizeof(double));
+    memcpy(h->fdr, st->f_drucker, n * sizeof(double));
+    memcpy(h->x00, st->x00, 2 * n2 * sizeof(double));
+    memcpy(h->displ, st->displ, 2 * (size_t)h->nn * sizeof(double));
+    if (h->nd > 0) {
+        memcpy(h->wallpos, st->wall_position, n2 * sizeof(float));
+        memcpy(h->horiz, st->horizontal_or_not, n2 * sizeof(float));
+    }
+    if (bw_setup(h)) return fail(SPS_ENOMEM, "wall-dummy grid");
+    h->have_state = 1;
+    return 0;
+}
+
+int sps_step(sps_handle *h, int32_t nsteps, double dt, double *time_sph, int32_t *itimestep_sph)
+{
+    if (!h || !time_sph || !itimestep_sph) return fail(SPS_EINVAL, "null argument");
+    if (!h->have_state) return fail(SPS_ESTATE, "sps_step before sps_set_state");
+    for (int32_t s = 0; s < nsteps; ++s) {
+        *itimestep_sph += 1;
+        int rc = time_integration(h, *itimestep_sph, *time_sph, dt);
+        if (rc) return rc;
+        *time_sph = *time_sph + dt;
+        h->steps++;
+    }
+    return 0;
+}
+
+int sps_get_state(sps_handle *h, sps_state *st)
+{
+    if (!h || !st) return fail(SPS_EINVAL, "null argument");
+    size_t n2 = (size_t)h->nt2, n = (size_t)h->nt, nn = (size_t)h->nn;
+#define OUT(f, src, cnt) do { if (st->f) memcpy(st->f, h->src, (cnt) * sizeof(*st->f)); } while (0)
+    OUT(x, x, 2 * n2); OUT(vel, vel, 2 * n2); OUT(stress, stress, 4 * n2);
+    OUT(mass, mass, n2); OUT(rho, rho, n2); OUT(hsml, hsml, n2);
+    OUT(itype, itype, n2); OUT(if_out_domain, ood, n2);
+    OUT(internal_vars, ivars, (size_t)h->niv * n); OUT(f_drucker, fdr, n);
+    OUT(x00, x00, 2 * n2); OUT(displ, displ, 2 * nn);
+    OUT(wall_position, wallpos, n2); OUT(horizontal_or_not, horiz, n2);
+    OUT(grad_u, grad_u, 4 * n); OUT(ddev_strn, ddev, n);
+    OUT(f_bound, f_bound, 2 * nn); OUT(art_visc, art_visc, 2 * nn);
+#undef OUT
+    return 0;
+}
+
+int sps_get_pairs(sps_handle *h, int64_t cap, int32_t *pair_i, int32_t *pair_j, int32_t *pint_type,
+                  float *w, float *dwdx, float *dwdy, int64_t *n)
+{
+    if (!h || !n) return fail(SPS_EINVAL, "null argument");
+    *n = h->n_pairs;
+    if (cap < h->n_pairs) return fail(SPS_ECAPACITY, "pair buffer holds %lld < %lld", (long long)cap, (long long)h->n_pairs);
+    for (int64_t k = 0; k < h->n_pairs; ++k) {
+        int64_t d = h->order[k];
+        if (pair_i) pair_i[k] = h->pi[d] + 1;
+        if (pair_j) pair_j[k] = h->pj[d] + 1;
+        if (pint_type) pint_type[k] = h->pt[d];
+        if (w) w[k] = h->pw[d];
+        if (dwdx) dwdx[k] = h->pgx[d];
+        if (dwdy) dwdy[k] = h->pgy[d];
+    }
+    return 0;
+}
+
+int sps_get_stats(sps_handle *h, sps_stats *out)
+{
+    if (!h || !out) return fail(SPS_EINVAL, "null argument");
+    memset(out, 0, sizeof *out);
+    out->n_pairs = h->n_pairs;
+    out->m_pairs = h->m_pairs;
+    for (int64_t k = 0; k < h->n_pairs; ++k)
+        if (h->pt[k] >= 0 && h->pt[k] < 10) out->n_by_type[h->pt[k]]++;
+    out->ndivx = h->ndivx;
+    out->ndivy = h->ndivy;
+    out->steps = h->steps;
+    return 0;
+}
+
+void sps_destroy(sps_handle *h)
+{
+    if (!h) return;
+    free_all(h);
+    free(h);
+}

@@ -1,0 +1,159 @@
+"""Initial particle state from a deck, as the reference builds it.
+
+Mirrors Read_2DMesh (code/3_SPH_material_2018.f90:346-732),
+set_up_dummy_nodes (:736-878) and Setup_Global_Arrays (:883-1026) for the
+inside approach with one stress point per lattice cell, including their
+single-precision arithmetic (the wall geometry is default real) and the
+lattice-count truncations.  Vectorised with numpy; every expression keeps the
+reference's operation order so the result is bit-identical (checked against
+dumps of the reference in tests/test_setup_golden.py).
+"""
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+
+from .deck import Deck, DeckError
+
+F32 = np.float32
+
+
+@dataclasses.dataclass
+class Particles:
+    nnode: int
+    nstress: int
+    ndummy: int
+    ndummy2: int
+    ndivx: int
+    ndivy: int
+    x: np.ndarray            # (ntotal2, 2) float64
+    mass: np.ndarray
+    rho: np.ndarray
+    hsml: np.ndarray
+    itype: np.ndarray        # int32
+    wall_position: np.ndarray  # (ntotal2,) float32
+    horizontal_or_not: np.ndarray
+    D: dict                  # D11..D42
+    pi: float
+
+    @property
+    def ntotal(self):
+        return self.nnode + self.nstress
+
+    @property
+    def ntotal2(self):
+        return self.ntotal + self.ndummy
+
+
+def reference_pi():
+    """pi = 4*atan(1.0) evaluated in default real (3_...:144): atan(1.0) is
+    the binary32 value nearest pi/4, and scaling by 4 is exact, so this is
+    the binary32 value nearest pi, widened."""
+    return float(F32(4) * F32(np.arctan(F32(1))))
+
+
+def elastic_matrix(young, poiss):
+    """3_...:719-730 (DP with the single-precision literals 4./3., 2./3.)."""
+    K_mod = young / (3 * (1 - 2 * poiss))
+    G_mod = young / (2 * (1 + poiss))
+    D11 = float(F32(4) / F32(3)) * G_mod + K_mod
+    D12 = -(float(F32(2) / F32(3)) * G_mod) + K_mod
+    return dict(D11=D11, D22=D11, D12=D12, D33=G_mod, D41=D12, D42=D12)
+
+
+def build_particles(deck: Deck) -> Particles:
+    if deck.geom_type != 1:
+        raise DeckError("only geom_type 1 (rectangle) is implemented, as in the reference (3_...:388-390)")
+    if not (deck.sp_sph and deck.inside_approach and deck.npoints == 1):
+        raise DeckError("only the inside approach with one stress point per cell is supported")
+    x1, _x2, _x3, x4 = deck.xs
+    y1, _y2, _y3, y4 = deck.ys
+    dx, dy = deck.dx, deck.dy
+    lx = x4 - x1
+    ly = y4 - y1
+    ndivx = int(lx / dx + 1)          # integer truncation, 3_...:415-416
+    ndivy = int(ly / dy + 1)
+    nnode = ndivx * ndivy
+    nstress = (ndivx - 1) * (ndivy - 1)
+
+    # nodes: i (x) outer, j (y) inner, 3_...:498-514
+    ii = np.repeat(np.arange(ndivx, dtype=np.float64), ndivy)
+    jj = np.tile(np.arange(ndivy, dtype=np.float64), ndivx)
+    xn = x1 + ii * dx
+    yn = y1 + jj * dy
+    no_int = np.where((xn == x1) | (xn == x4) | (yn == y4), 4, 8)
+    corner = ((xn == x1) & ((yn == y1) | (yn == y4))) | ((xn == x4) & ((yn == y1) | (yn == y4)))
+    no_int = np.where(corner, 2, no_int)
+
+    # stress points, inside approach npoints = 1: 3_...:541-549
+    si = np.repeat(np.arange(1, ndivx, dtype=np.float64), ndivy - 1)
+    sj = np.tile(np.arange(1, ndivy, dtype=np.float64), ndivx - 1)
+    xs = x1 - dx / 2. + si * dx
+    ys = y1 - dx / 2. + sj * dy
+
+    rho0 = deck.props[5]
+    hs = deck.sml * dx
+    area = (rho0 * dx * dy) / 8
+    mass_n = no_int * area
+    mass_s = np.full(nstress, (8 * area) / deck.npoints)
+
+    # dummy walls, default-real arithmetic (3_...:736-878)
+    walls = deck.walls if deck.dummy_nodes else []
+    nrow = 3
+    dx2, dy2 = F32(dx), F32(dy)
+    ndiv_wall = [int(F32((w.x2 - w.x1) / dx2 + F32(1))) for w in walls]
+    ndummy2 = sum(ndiv_wall)
+    ndummy = nrow * ndummy2
+    dxs, dys, hor, wpos = [], [], [], []
+
+    def layer(w, nd, m):
+        j = np.arange(nd, dtype=np.float32)
+        if w.wall_id in (1, 3):
+            xs_ = np.full(nd, F32(w.position - F32(m - 1) * dx2), dtype=np.float32) if m > 1 else \
+                np.full(nd, w.position, dtype=np.float32)
+            ys_ = (w.x1 + j * dy2).astype(np.float32)
+            h = 2.0 if w.wall_id == 1 else 22.0
+        elif w.wall_id == 2:
+            xs_ = (w.x1 + j * dx2).astype(np.float32)
+            ys_ = np.full(nd, F32(w.position - F32(m - 1) * dy2), dtype=np.float32) if m > 1 else \
+                np.full(nd, w.position, dtype=np.float32)
+            h = 1.0
+        else:
+            raise DeckError(f"unknown wall id {w.wall_id}")
+        dxs.append(xs_); dys.append(ys_)
+        hor.append(np.full(nd, h, dtype=np.float32))
+        wpos.append(np.full(nd, w.position, dtype=np.float32))
+
+    for w, nd in zip(walls, ndiv_wall):
+        layer(w, nd, 1)
+    for w, nd in zip(walls, ndiv_wall):
+        for m in range(2, nrow + 1):
+            layer(w, nd, m)
+
+    ntotal = nnode + nstress
+    ntotal2 = ntotal + ndummy
+    x = np.empty((ntotal2, 2), dtype=np.float64)
+    x[:nnode, 0], x[:nnode, 1] = xn, yn
+    x[nnode:ntotal, 0], x[nnode:ntotal, 1] = xs, ys
+    if ndummy:
+        x[ntotal:, 0] = np.concatenate(dxs).astype(np.float64)
+        x[ntotal:, 1] = np.concatenate(dys).astype(np.float64)
+    mass = np.empty(ntotal2)
+    mass[:nnode] = mass_n
+    mass[nnode:ntotal] = mass_s
+    mass[ntotal:] = float(dx2 * dx2) * rho0
+    rho = np.full(ntotal2, rho0)
+    hsml = np.full(ntotal2, hs)
+    itype = np.empty(ntotal2, dtype=np.int32)
+    itype[:nnode], itype[nnode:ntotal], itype[ntotal:] = 2, 1, 25
+    wall_position = np.zeros(ntotal2, dtype=np.float32)
+    horizontal_or_not = np.zeros(ntotal2, dtype=np.float32)
+    if ndummy:
+        wall_position[ntotal:] = np.concatenate(wpos)
+        horizontal_or_not[ntotal:] = np.concatenate(hor)
+    return Particles(nnode=nnode, nstress=nstress, ndummy=ndummy, ndummy2=ndummy2,
+                     ndivx=ndivx, ndivy=ndivy, x=x, mass=mass, rho=rho, hsml=hsml,
+                     itype=itype, wall_position=wall_position,
+                     horizontal_or_not=horizontal_or_not,
+                     D=elastic_matrix(deck.props[2], deck.props[3]), pi=reference_pi())
