@@ -1,0 +1,646 @@
+// sps_api.hip - C-ABI of the Stress-Particle SPH hot path (include/sps.h) on
+// one gfx950 GPU: handle, device buffers and the launch sequence of a step.
+//
+// One step is the reference's time_integration (code/2_SPH_main_2018.f90
+// :78-184) for the supported flag set:
+//   Check_Out_Domain, grid_find_NEW + Pint_Update  -> k_check_ood .. k_build
+//   RK4 (2_...:653-802)                            -> k_rk_init, 4 x stage, k_rk_final
+//   update_strain, stress_point_update, adapt      -> k_rk_final, k_spu_*
+//   position update and displacement               -> k_position
+// Every kernel is a gather over the ordered neighbour lists; see
+// sps_kernels.hip for the ordering argument.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+
+#include "../../include/sps.h"
+#include "sps_kernels.hip"
+
+using namespace sps;
+
+static thread_local char g_err[512];
+
+static int fail(int code, const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(SPS_EDEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                                \
+    } while (0)
+
+static inline unsigned nblk(long long n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
+
+template <typename T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+};
+
+template <typename T>
+static int dalloc(DBuf<T> &b, size_t n) {
+    if (b.n >= n && b.p) return 0;
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.n = 0;
+    size_t cnt = n ? n : 1;
+    hipError_t e = hipMalloc((void **)&b.p, cnt * sizeof(T));
+    if (e != hipSuccess) return fail(SPS_ENOMEM, "hipMalloc of %zu bytes: %s", cnt * sizeof(T), hipGetErrorString(e));
+    b.n = cnt;
+    return 0;
+}
+
+template <typename T>
+static void dfree(DBuf<T> &b) {
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.n = 0;
+}
+
+struct StateSet {
+    DBuf<double2> x, vel, x00, displ;
+    DBuf<double4> stress;
+    DBuf<double> mass, rho, hsml, ivar1, fdr;
+    DBuf<int> orig, ood;
+    DBuf<float> wallpos, horiz;
+
+    StateBufs view() {
+        StateBufs s;
+        s.x = x.p; s.vel = vel.p; s.x00 = x00.p; s.displ = displ.p; s.stress = stress.p;
+        s.mass = mass.p; s.rho = rho.p; s.hsml = hsml.p; s.ivar1 = ivar1.p; s.fdr = fdr.p;
+        s.orig = orig.p; s.ood = ood.p; s.wallpos = wallpos.p; s.horiz = horiz.p;
+        return s;
+    }
+    int alloc(int nn, int nt, int nt2) {
+        int rc = 0;
+        rc |= dalloc(x, nt2); rc |= dalloc(vel, nt2); rc |= dalloc(x00, nt2); rc |= dalloc(displ, nn);
+        rc |= dalloc(stress, nt2); rc |= dalloc(mass, nt2); rc |= dalloc(rho, nt2);
+        rc |= dalloc(hsml, nt2); rc |= dalloc(ivar1, nt); rc |= dalloc(fdr, nt);
+        rc |= dalloc(orig, nt2); rc |= dalloc(ood, nt2); rc |= dalloc(wallpos, nt2);
+        rc |= dalloc(horiz, nt2);
+        return rc;
+    }
+    void release() {
+        dfree(x); dfree(vel); dfree(x00); dfree(displ); dfree(stress); dfree(mass); dfree(rho);
+        dfree(hsml); dfree(ivar1); dfree(fdr); dfree(orig); dfree(ood); dfree(wallpos); dfree(horiz);
+    }
+};
+
+struct sps_handle {
+    sps_params p;
+    DevParams dp;
+    int nn, ns, nd, nd2, nt, nt2, niv;
+    int device;
+    hipStream_t stream = nullptr;
+    bool have_state = false;
+    bool poisoned = false;
+    long long steps = 0;
+    // host copy of internal_vars rows the hot path never touches
+    std::vector<double> ivars_host;
+    // state: current order and a scratch set for the permutation
+    StateSet S[2];
+    int cur = 0;
+    // step work arrays
+    DBuf<double2> vx0, x0, vel0, rk_vel, rhs2, f_bound, art_visc;
+    DBuf<double4> stress0, rk_stress, rhs1, grad_u;
+    DBuf<double> rk_dev, ddev;
+    // search
+    int cell_cap = 0;
+    DBuf<Grid> grid;
+    DBuf<Split> split;
+    DBuf<double> bparts;
+    int nbparts = 0;
+    DBuf<int> cellkey, new_cell, cell, kcount, kstart, kfill, tmp_orig, tmp_src, proc, proc_to_slot;
+    DBuf<unsigned> cnt, off, scan_tmp, widthA, widthB, offA, offB;
+    DBuf<int> degA, degB;
+    DBuf<unsigned long long> totals;   // [0] list A size, [1] list B size
+    DBuf<int> err;
+    // lists
+    DBuf<int> a_s, b_s;
+    DBuf<float> a_w, a_gx, a_gy, b_gx, b_gy;
+    // host-visible summary of the last search
+    unsigned long long n_pairs = 0, m_pairs = 0, m_prev = 0;
+    unsigned long long n_by_type[10] = {0};
+    int ndivx = 0, ndivy = 0;
+    unsigned long long *pinned = nullptr;   // readback staging
+};
+
+extern "C" const char *sps_last_error(void) { return g_err; }
+extern "C" const char *sps_backend(void) { return "hip-gfx950"; }
+
+// ------------------------------------------------------------------ scans
+// Exclusive scan of n unsigned values; optional device total.
+static int scan_u32(sps_handle *h, const unsigned *in, unsigned *out, long long n,
+                    unsigned long long *dtotal) {
+    long long nb = (n + BLOCK * 4 - 1) / (BLOCK * 4);
+    if (nb == 0) nb = 1;
+    int rc = dalloc(h->scan_tmp, (size_t)nb);
+    if (rc) return rc;
+    k_scan_local<<<(unsigned)nb, BLOCK, 0, h->stream>>>(in, out, h->scan_tmp.p, n);
+    k_scan_sums<<<1, BLOCK, 0, h->stream>>>(h->scan_tmp.p, (int)nb, dtotal);
+    k_scan_add<<<(unsigned)nb, BLOCK, 0, h->stream>>>(out, h->scan_tmp.p, n);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// --------------------------------------------------------------- params
+static int check_params(const sps_params *p) {
+    if (p->abi_version != SPS_ABI_VERSION) return fail(SPS_EINVAL, "abi_version %d != %d", p->abi_version, SPS_ABI_VERSION);
+    if (p->nnode <= 0 || p->nstress <= 0 || p->ndummy < 0 || p->ndummy2 < 0 || p->ndummy2 > p->ndummy)
+        return fail(SPS_EINVAL, "bad particle counts");
+    if ((long long)p->nnode + p->nstress + p->ndummy >= (1LL << 30)) return fail(SPS_EINVAL, "too many particles");
+    if (p->nint_vars < 1) return fail(SPS_EINVAL, "nint_vars must be >= 1");
+    if (p->ndimn != 2 || p->nstre != 4) return fail(SPS_EINVAL, "only ndimn=2, nstre=4 (plane strain) are supported");
+    if (p->skf != 1) return fail(SPS_EINVAL, "only the cubic spline kernel (skf=1) is supported");
+    if (!p->sp_sph || !p->inside_approach) return fail(SPS_EINVAL, "only stress-particle SPH with the inside approach is supported");
+    if (p->xsph || p->cont_density || p->art_stress || p->sph_shift || p->no_bcs)
+        return fail(SPS_EINVAL, "XSPH, continuity density, artificial stress, stress-point shifting and prescribed BCs are not supported");
+    int ntype_eco = (int)p->props[0], ncrit = (int)p->props[1];
+    if (ntype_eco > 1 && ncrit <= 5) return fail(SPS_EINVAL, "Perzyna models (ncrit<=5) are not supported");
+    if (p->ic_grav && (p->tcurve_npts < 2 || p->tcurve_npts > SPS_MAX_TCURVE))
+        return fail(SPS_EINVAL, "gravity time curve needs 2..%d points", SPS_MAX_TCURVE);
+    if (p->mode != SPS_MODE_ORDERED) return fail(SPS_EINVAL, "only SPS_MODE_ORDERED is available in this library");
+    return 0;
+}
+
+// gravity_force time factor at step time (3_...:2655-2680)
+static double gravity_factor(const sps_params *P, double time_sph) {
+    if (P->ic_grav == 0) return 0.0;
+    double tt0 = -1000.0, tt1 = 0.0;
+    int ip, found = 0;
+    for (ip = 0; ip < P->tcurve_npts - 1; ++ip) {
+        tt0 = P->tcurve_t[ip];
+        tt1 = P->tcurve_t[ip + 1];
+        if (time_sph >= tt0 && time_sph <= tt1) {
+            found = 1;
+            break;
+        }
+        tt0 = -1000.0;
+    }
+    double factg = 0.0;
+    if (tt0 >= 0.0 && found) {
+        double xi = (time_sph - tt0) / (tt1 - tt0);
+        factg = (1.0 - xi) * (double)P->tcurve_f[ip] + xi * (double)P->tcurve_f[ip + 1];
+    }
+    return factg * P->ft_grav;
+}
+
+extern "C" int sps_create(const sps_params *params, sps_handle **out) {
+    if (!params || !out) return fail(SPS_EINVAL, "null argument");
+    int rc = check_params(params);
+    if (rc) return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(SPS_EDEVICE, "no HIP device visible: the product library needs an MI355X (gfx950)");
+    sps_handle *h = new (std::nothrow) sps_handle();
+    if (!h) return fail(SPS_ENOMEM, "handle allocation");
+    h->p = *params;
+    h->nn = params->nnode;
+    h->ns = params->nstress;
+    h->nd = params->ndummy;
+    h->nd2 = params->ndummy2;
+    h->nt = h->nn + h->ns;
+    h->nt2 = h->nt + h->nd;
+    h->niv = params->nint_vars;
+    (void)hipGetDevice(&h->device);
+    DevParams &d = h->dp;
+    d.nn = h->nn; d.ns = h->ns; d.nd = h->nd; d.nd2 = h->nd2; d.nt = h->nt; d.nt2 = h->nt2;
+    d.cspm = params->cspm; d.update_x = params->update_x;
+    d.ntype_eco = (int)params->props[0]; d.ncrit = (int)params->props[1];
+    d.do_bforce = params->inside_approach && (params->bforce_gate ? 1 : params->dummy_nodes);
+    d.do_av = (params->alpha > 0 || params->beta > 0);
+    d.ic_grav = params->ic_grav;
+    d.pi = params->pi; d.dx = params->dx; d.alpha = params->alpha; d.beta = params->beta;
+    d.damping = params->damping; d.cspm_tol = params->cspm_det_tol;
+    d.D11 = params->D11; d.D22 = params->D22; d.D12 = params->D12; d.D33 = params->D33;
+    d.D41 = params->D41; d.D42 = params->D42;
+    d.tanfi = params->props[12]; d.coh = params->props[13];
+    d.young = params->props[2]; d.poiss = params->props[3];
+    for (int a = 0; a < 2; ++a) {
+        d.xmin_dom[a] = params->xmin_domain[a];
+        d.xmax_dom[a] = params->xmax_domain[a];
+        d.cgrav[a] = params->cgrav[a];
+    }
+    auto bail = [&](int code) {
+        sps_destroy(h);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(SPS_EDEVICE, "hipStreamCreate failed"));
+    int nt2 = h->nt2, nt = h->nt, nn = h->nn;
+    rc = 0;
+    rc |= h->S[0].alloc(nn, nt, nt2);
+    rc |= h->S[1].alloc(nn, nt, nt2);
+    rc |= dalloc(h->vx0, nt); rc |= dalloc(h->x0, nt); rc |= dalloc(h->vel0, nt);
+    rc |= dalloc(h->rk_vel, nn); rc |= dalloc(h->rhs2, nn); rc |= dalloc(h->f_bound, nn);
+    rc |= dalloc(h->art_visc, nn); rc |= dalloc(h->stress0, nt); rc |= dalloc(h->rk_stress, nt);
+    rc |= dalloc(h->rhs1, nt); rc |= dalloc(h->grad_u, nt); rc |= dalloc(h->rk_dev, nt);
+    rc |= dalloc(h->ddev, nt);
+    rc |= dalloc(h->grid, 1); rc |= dalloc(h->split, 1); rc |= dalloc(h->err, 1);
+    rc |= dalloc(h->totals, 2);
+    h->nbparts = std::min(1024, (int)nblk(nt2));
+    rc |= dalloc(h->bparts, 5 * (size_t)h->nbparts);
+    rc |= dalloc(h->cellkey, nt2); rc |= dalloc(h->new_cell, nt2); rc |= dalloc(h->cell, nt2);
+    rc |= dalloc(h->tmp_orig, nt2); rc |= dalloc(h->tmp_src, nt2); rc |= dalloc(h->proc, nt2);
+    rc |= dalloc(h->proc_to_slot, nt2); rc |= dalloc(h->cnt, nt2); rc |= dalloc(h->off, nt2);
+    rc |= dalloc(h->degA, nt2); rc |= dalloc(h->degB, nn);
+    rc |= dalloc(h->widthA, (nt2 + 63) / 64 + 1); rc |= dalloc(h->widthB, (nn + 63) / 64 + 1);
+    rc |= dalloc(h->offA, (nt2 + 63) / 64 + 1); rc |= dalloc(h->offB, (nn + 63) / 64 + 1);
+    if (rc) return bail(rc);
+    if (hipHostMalloc((void **)&h->pinned, 64 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
+        return bail(fail(SPS_ENOMEM, "pinned staging allocation"));
+    if (hipMemsetAsync(h->f_bound.p, 0, nn * sizeof(double2), h->stream) != hipSuccess ||
+        hipMemsetAsync(h->art_visc.p, 0, nn * sizeof(double2), h->stream) != hipSuccess ||
+        hipMemsetAsync(h->grad_u.p, 0, nt * sizeof(double4), h->stream) != hipSuccess ||
+        hipMemsetAsync(h->ddev.p, 0, nt * sizeof(double), h->stream) != hipSuccess ||
+        hipMemsetAsync(h->err.p, 0, sizeof(int), h->stream) != hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess)
+        return bail(fail(SPS_EDEVICE, "initial clears failed"));
+    *out = h;
+    return 0;
+}
+
+// Cell capacity: the grid spans the in-domain particles, which lie inside
+// the domain box, with cells of twice the largest h.
+static int size_cells(sps_handle *h, const double *hsml) {
+    double hmin = 1e300;
+    for (int i = 0; i < h->nt2; ++i) hmin = std::min(hmin, hsml[i]);
+    if (!(hmin > 0)) return fail(SPS_EINVAL, "smoothing lengths must be positive");
+    double lx = h->p.xmax_domain[0] - h->p.xmin_domain[0], ly = h->p.xmax_domain[1] - h->p.xmin_domain[1];
+    double cx = lx / (2 * hmin) + 3, cy = ly / (2 * hmin) + 3;
+    double cells = cx * cy;
+    if (!(cells > 0) || cells > (double)(1 << 28))
+        return fail(SPS_EINVAL, "domain box too large for the cell grid (%.3g cells)", cells);
+    int cap = (int)cells + 16;
+    int rc = 0;
+    size_t L = 3 * ((size_t)cap + 1) + 1;
+    rc |= dalloc(h->kcount, L); rc |= dalloc(h->kstart, L); rc |= dalloc(h->kfill, L);
+    if (rc) return rc;
+    h->cell_cap = cap;
+    return 0;
+}
+
+extern "C" int sps_set_state(sps_handle *h, const sps_state *st) {
+    if (!h || !st) return fail(SPS_EINVAL, "null argument");
+    if (!st->x || !st->vel || !st->stress || !st->mass || !st->rho || !st->hsml || !st->itype ||
+        !st->if_out_domain || !st->internal_vars || !st->f_drucker || !st->x00 || !st->displ)
+        return fail(SPS_EINVAL, "sps_set_state: a required array is NULL");
+    if (h->nd > 0 && (!st->wall_position || !st->horizontal_or_not))
+        return fail(SPS_EINVAL, "sps_set_state: wall_position/horizontal_or_not required with dummies");
+    for (int i = 0; i < h->nt2; ++i) {
+        int want = i < h->nn ? 2 : (i < h->nt ? 1 : 25);
+        if (st->itype[i] != want)
+            return fail(SPS_EINVAL, "itype(%d) = %d; the hot path needs 2 for nodes, 1 for stress points, 25 for dummies",
+                        i + 1, st->itype[i]);
+    }
+    int rc = size_cells(h, st->hsml);
+    if (rc) return rc;
+    if (h->dp.do_bforce) {
+        double hmin = 1e300;
+        for (int i = 0; i < h->nt2; ++i) hmin = std::min(hmin, st->hsml[i]);
+        if (!(0.75 * h->p.dx < 2 * hmin))
+            return fail(SPS_EINVAL, "wall force reach 0.75*dx must lie inside the kernel support");
+    }
+    const int nt2 = h->nt2, nt = h->nt, nn = h->nn;
+    StateSet &S = h->S[h->cur];
+    std::vector<double> tmp((size_t)nt2 * 4);
+    hipStream_t s = h->stream;
+    HIP_TRY(hipMemcpyAsync(S.x.p, st->x, nt2 * sizeof(double2), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(S.vel.p, st->vel, nt2 * sizeof(double2), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(S.stress.p, st->stress, nt2 * sizeof(double4), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(S.mass.p, st->mass, nt2 * sizeof(double), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(S.rho.p, st->rho, nt2 * sizeof(double), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(S.hsml.p, st->hsml, nt2 * sizeof(double), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(S.ood.p, st->if_out_domain, nt2 * sizeof(int), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(S.fdr.p, st->f_drucker, nt * sizeof(double), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(S.x00.p, st->x00, nt2 * sizeof(double2), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(S.displ.p, st->displ, nn * sizeof(double2), hipMemcpyHostToDevice, s));
+    h->ivars_host.assign(st->internal_vars, st->internal_vars + (size_t)h->niv * nt);
+    for (int i = 0; i < nt; ++i) tmp[i] = st->internal_vars[(size_t)h->niv * i];
+    HIP_TRY(hipMemcpyAsync(S.ivar1.p, tmp.data(), nt * sizeof(double), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));   // tmp is reused below
+    std::vector<float> wp(nt2, 0.0f), hz(nt2, 0.0f);
+    if (h->nd > 0) {
+        std::memcpy(wp.data(), st->wall_position, nt2 * sizeof(float));
+        std::memcpy(hz.data(), st->horizontal_or_not, nt2 * sizeof(float));
+    }
+    HIP_TRY(hipMemcpyAsync(S.wallpos.p, wp.data(), nt2 * sizeof(float), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(S.horiz.p, hz.data(), nt2 * sizeof(float), hipMemcpyHostToDevice, s));
+    k_scatter_in_identity<<<nblk(nt2), BLOCK, 0, s>>>(S.orig.p, nt2);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s));
+    h->have_state = true;
+    h->poisoned = false;
+    return 0;
+}
+
+// Neighbour search of one step: binning, processing order, pair counts, the
+// list split and the ordered lists.
+static int search(sps_handle *h, bool fresh_list) {
+    hipStream_t s = h->stream;
+    const int nn = h->nn, nt = h->nt, nt2 = h->nt2;
+    StateSet &A = h->S[h->cur];
+    StateSet &B = h->S[1 - h->cur];
+    // domain check and grid
+    k_check_ood<<<nblk(nt2), BLOCK, 0, s>>>(A.x.p, A.ood.p, nt2, h->p.xmin_domain[0], h->p.xmin_domain[1],
+                                           h->p.xmax_domain[0], h->p.xmax_domain[1]);
+    k_bounds_partial<<<h->nbparts, BLOCK, 0, s>>>(A.x.p, A.hsml.p, A.ood.p, nt2, h->bparts.p);
+    k_bounds_final<<<1, 64, 0, s>>>(h->bparts.p, h->nbparts, h->grid.p, h->cell_cap);
+    // the cell count is needed on the host for the bucket arrays' extent
+    HIP_TRY(hipMemcpyAsync(h->pinned, h->grid.p, sizeof(Grid), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    Grid gh;
+    std::memcpy(&gh, h->pinned, sizeof(Grid));
+    if (gh.overflow) {
+        h->poisoned = true;
+        return fail(SPS_ECAPACITY, "cell grid %d x %d exceeds the capacity %d sized from the domain box",
+                    gh.ndivx, gh.ndivy, h->cell_cap);
+    }
+    h->ndivx = gh.ndivx;
+    h->ndivy = gh.ndivy;
+    const int ndivt = gh.ndivt;
+    const long long L3 = 3LL * (ndivt + 1);
+    HIP_TRY(hipMemsetAsync(h->kcount.p, 0, (L3 + 1) * sizeof(int), s));
+    HIP_TRY(hipMemsetAsync(h->kfill.p, 0, (L3 + 1) * sizeof(int), s));
+    k_cell_count<<<nblk(nt2), BLOCK, 0, s>>>(A.x.p, A.ood.p, h->grid.p, h->cellkey.p, h->kcount.p, nn, nt, nt2);
+    int rc = scan_u32(h, (const unsigned *)h->kcount.p, (unsigned *)h->kstart.p, L3 + 1, nullptr);
+    if (rc) return rc;
+    k_cell_scatter<<<nblk(nt2), BLOCK, 0, s>>>(h->cellkey.p, A.orig.p, h->kstart.p, h->kfill.p, h->grid.p,
+                                              h->tmp_orig.p, h->tmp_src.p, h->cell.p, nn, nt, nt2);
+    k_cell_sort<<<nblk(L3), BLOCK, 0, s>>>(h->kstart.p, h->tmp_orig.p, h->tmp_src.p, h->grid.p, h->err.p);
+    // permute the state into (kind, cell, original index) order
+    k_permute<<<nblk(nt2), BLOCK, 0, s>>>(B.view(), A.view(), h->tmp_src.p, nn, nt, nt2);
+    HIP_TRY(hipGetLastError());
+    h->cur = 1 - h->cur;
+    StateSet &C = h->S[h->cur];
+    k_proc_slot<<<nblk(nt2), BLOCK, 0, s>>>(h->cell.p, h->kstart.p, h->grid.p, h->proc.p, nn, nt, nt2);
+    k_proc_to_slot<<<nblk(nt2), BLOCK, 0, s>>>(h->proc.p, h->proc_to_slot.p, nt2);
+    // counts, offsets, split
+    HIP_TRY(hipMemsetAsync(h->cnt.p, 0, nt2 * sizeof(unsigned), s));
+    const unsigned long long M = fresh_list ? 0ull : h->m_pairs;
+    k_split_reset<<<1, 64, 0, s>>>(h->split.p, M);
+    k_count<<<nblk(nt2), BLOCK, 0, s>>>(C.x.p, C.hsml.p, h->cell.p, h->kstart.p, h->proc.p, h->grid.p,
+                                       h->cnt.p, h->degA.p, h->degB.p, h->split.p, nn, nt, nt2);
+    HIP_TRY(hipGetLastError());
+    // number of in-domain particles = end of the last in-domain bucket
+    int nproc_h = 0;
+    {
+        int ks[4];
+        HIP_TRY(hipMemcpyAsync(ks, h->kstart.p + (size_t)ndivt, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(ks + 1, h->kstart.p + (size_t)(ndivt + 1) + ndivt, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(ks + 2, h->kstart.p + (size_t)2 * (ndivt + 1) + ndivt, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        nproc_h = (ks[0] - 0) + (ks[1] - nn) + (ks[2] - nt);
+    }
+    rc = scan_u32(h, h->cnt.p, h->off.p, nproc_h, nullptr);
+    if (rc) return rc;
+    if (nproc_h > 0) k_split<<<nblk(nproc_h), BLOCK, 0, s>>>(h->cnt.p, h->off.p, nproc_h, h->split.p);
+    k_split_partner<<<1, 64, 0, s>>>(C.x.p, C.hsml.p, h->cell.p, h->kstart.p, h->proc.p, h->off.p,
+                                     h->proc_to_slot.p, h->grid.p, h->split.p, nn, nt);
+    // list shapes
+    k_slice_width<<<nblk(((nt2 + 63) / 64) * 64), BLOCK, 0, s>>>(h->degA.p, nt2, h->widthA.p);
+    k_slice_width<<<nblk(((nn + 63) / 64) * 64), BLOCK, 0, s>>>(h->degB.p, nn, h->widthB.p);
+    rc = scan_u32(h, h->widthA.p, h->offA.p, (nt2 + 63) / 64, h->totals.p);
+    if (rc) return rc;
+    rc = scan_u32(h, h->widthB.p, h->offB.p, (nn + 63) / 64, h->totals.p + 1);
+    if (rc) return rc;
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(h->pinned, h->totals.p, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(h->pinned + 2, h->split.p, sizeof(Split), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    unsigned long long sizeA = h->pinned[0], sizeB = h->pinned[1];
+    Split sh;
+    std::memcpy(&sh, h->pinned + 2, sizeof(Split));
+    if (sizeA >= (1ull << 32) || sizeB >= (1ull << 32))
+        return fail(SPS_ECAPACITY, "neighbour lists exceed 2^32 entries");
+    rc = 0;
+    if (h->a_s.n < sizeA) {
+        size_t want = (size_t)(sizeA * 1.25) + 4096;
+        rc |= dalloc(h->a_s, want); rc |= dalloc(h->a_w, want); rc |= dalloc(h->a_gx, want); rc |= dalloc(h->a_gy, want);
+    }
+    if (h->b_s.n < sizeB) {
+        size_t want = (size_t)(sizeB * 1.25) + 4096;
+        rc |= dalloc(h->b_s, want); rc |= dalloc(h->b_gx, want); rc |= dalloc(h->b_gy, want);
+    }
+    if (rc) return rc;
+    Lists Lz{h->a_s.p, h->a_w.p, h->a_gx.p, h->a_gy.p, h->offA.p, h->b_s.p, h->b_gx.p, h->b_gy.p, h->offB.p};
+    k_build<<<nblk(nt2), BLOCK, 0, s>>>(C.x.p, C.hsml.p, h->cell.p, h->kstart.p, h->proc.p, h->cnt.p, h->off.p,
+                                       h->grid.p, h->split.p, Lz, h->p.pi, nn, nt, nt2);
+    HIP_TRY(hipGetLastError());
+    h->n_pairs = sh.n;
+    h->m_prev = sh.M;
+    h->m_pairs = std::max(sh.M, sh.n);
+    for (int k = 0; k < 10; ++k) h->n_by_type[k] = sh.n_by_type[k];
+    return 0;
+}
+
+static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
+    int rc = search(h, itimestep == 1);
+    if (rc) return rc;
+    hipStream_t s = h->stream;
+    const int nn = h->nn, nt = h->nt, nt2 = h->nt2;
+    StateBufs SB = h->S[h->cur].view();
+    Work W{h->vx0.p, h->x0.p, h->vel0.p, h->rk_vel.p, h->rhs2.p, h->f_bound.p, h->art_visc.p,
+           h->stress0.p, h->rk_stress.p, h->rhs1.p, h->grad_u.p, h->rk_dev.p, h->ddev.p};
+    Lists Lz{h->a_s.p, h->a_w.p, h->a_gx.p, h->a_gy.p, h->offA.p, h->b_s.p, h->b_gx.p, h->b_gy.p, h->offB.p};
+    const int adapt = h->dp.ncrit == 12;
+    const double tanfi = h->dp.tanfi, coh = h->dp.coh;
+    const double f1rk[4] = {0., 0.5, 0.5, 1.0}, f2rk[4] = {1., 2., 2., 1.0};
+    const double factg = gravity_factor(&h->p, time_sph);
+    const double gxg = h->p.ic_grav ? factg * h->p.cgrav[0] : 0.0;
+    const double gyg = h->p.ic_grav ? factg * h->p.cgrav[1] : 0.0;
+    k_rk_init<<<nblk(nt2), BLOCK, 0, s>>>(SB, W, nn, nt, nt2);
+    for (int st = 0; st < 4; ++st) {
+        const double c1 = f1rk[st] * (dt);
+        k_stage_begin<<<nblk(nt), BLOCK, 0, s>>>(SB, W, c1, adapt, tanfi, coh, nn, nt);
+        k_spu_nodes<<<nblk(nn), BLOCK, 0, s>>>(SB, h->degA.p, Lz, adapt, tanfi, coh, nn, nt);
+        k_spu_sps<<<nblk(nt - nn), BLOCK, 0, s>>>(SB, h->degA.p, Lz, adapt, tanfi, coh, nn, nt);
+        k_deriv_sps<<<nblk(nt - nn), BLOCK, 0, s>>>(SB, W, h->degA.p, Lz, h->dp, time_sph, f2rk[st], nn, nt);
+        k_deriv_nodes<<<nblk(nn), BLOCK, 0, s>>>(SB, W, h->degA.p, h->degB.p, Lz, h->dp, gxg, gyg, f2rk[st],
+                                                 h->err.p, nn, nt);
+        if (st == 3 && nt2 > nt)
+            k_dummy_stress<<<nblk(nt2 - nt), BLOCK, 0, s>>>(SB, h->degA.p, Lz, nt, nt2);
+    }
+    k_rk_final<<<nblk(nt), BLOCK, 0, s>>>(SB, W, dt, adapt, tanfi, coh, nn, nt);
+    k_spu_nodes<<<nblk(nn), BLOCK, 0, s>>>(SB, h->degA.p, Lz, adapt, tanfi, coh, nn, nt);
+    k_spu_sps<<<nblk(nt - nn), BLOCK, 0, s>>>(SB, h->degA.p, Lz, adapt, tanfi, coh, nn, nt);
+    k_position<<<nblk(nt), BLOCK, 0, s>>>(SB, W, dt, h->p.update_x, nn, nt);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+extern "C" int sps_step(sps_handle *h, int32_t nsteps, double dt, double *time_sph, int32_t *itimestep_sph) {
+    if (!h || !time_sph || !itimestep_sph) return fail(SPS_EINVAL, "null argument");
+    if (!h->have_state) return fail(SPS_ESTATE, "sps_step before sps_set_state");
+    if (h->poisoned) return fail(SPS_ESTATE, "a previous step failed; set the state again");
+    HIP_TRY(hipSetDevice(h->device));
+    for (int32_t k = 0; k < nsteps; ++k) {
+        *itimestep_sph += 1;
+        int rc = one_step(h, *itimestep_sph, *time_sph, dt);
+        if (rc) {
+            h->poisoned = true;
+            return rc;
+        }
+        *time_sph = *time_sph + dt;
+        h->steps++;
+    }
+    HIP_TRY(hipMemcpyAsync(h->pinned, h->err.p, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (*(int *)h->pinned) {
+        h->poisoned = true;
+        return fail(SPS_ECAPACITY, "device capacity flag %d: more than %d wall-force terms on one node (1) "
+                    "or more than %d particles of one kind in one cell (2)", *(int *)h->pinned, MAX_WALL_TERMS,
+                    MAX_BUCKET_SORT);
+    }
+    return 0;
+}
+
+template <typename T, typename K>
+static int fetch(sps_handle *h, const T *dsrc, const int *orig, int n, T *dst, K kern) {
+    DBuf<T> tmp;
+    int rc = dalloc(tmp, (size_t)n);
+    if (rc) return rc;
+    kern<<<nblk(n), BLOCK, 0, h->stream>>>(dsrc, orig, tmp.p, n);
+    hipError_t e = hipMemcpyAsync(dst, tmp.p, (size_t)n * sizeof(T), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    dfree(tmp);
+    if (e != hipSuccess) return fail(SPS_EDEVICE, "state download: %s", hipGetErrorString(e));
+    return 0;
+}
+
+extern "C" int sps_get_state(sps_handle *h, sps_state *st) {
+    if (!h || !st) return fail(SPS_EINVAL, "null argument");
+    if (!h->have_state) return fail(SPS_ESTATE, "sps_get_state before sps_set_state");
+    HIP_TRY(hipSetDevice(h->device));
+    StateSet &S = h->S[h->cur];
+    const int *orig = S.orig.p;
+    const int nn = h->nn, nt = h->nt, nt2 = h->nt2;
+    int rc = 0;
+    if (st->x) rc = rc ? rc : fetch(h, S.x.p, orig, nt2, (double2 *)st->x, k_gather_out2);
+    if (st->vel) rc = rc ? rc : fetch(h, S.vel.p, orig, nt2, (double2 *)st->vel, k_gather_out2);
+    if (st->stress) rc = rc ? rc : fetch(h, S.stress.p, orig, nt2, (double4 *)st->stress, k_gather_out4);
+    if (st->mass) rc = rc ? rc : fetch(h, S.mass.p, orig, nt2, st->mass, k_gather_out1);
+    if (st->rho) rc = rc ? rc : fetch(h, S.rho.p, orig, nt2, st->rho, k_gather_out1);
+    if (st->hsml) rc = rc ? rc : fetch(h, S.hsml.p, orig, nt2, st->hsml, k_gather_out1);
+    if (st->if_out_domain) rc = rc ? rc : fetch(h, S.ood.p, orig, nt2, st->if_out_domain, k_gather_outi);
+    if (st->f_drucker) rc = rc ? rc : fetch(h, S.fdr.p, orig, nt, st->f_drucker, k_gather_out1);
+    if (st->x00) rc = rc ? rc : fetch(h, S.x00.p, orig, nt2, (double2 *)st->x00, k_gather_out2);
+    if (st->displ) rc = rc ? rc : fetch(h, S.displ.p, orig, nn, (double2 *)st->displ, k_gather_out2);
+    if (st->grad_u) rc = rc ? rc : fetch(h, h->grad_u.p, orig, nt, (double4 *)st->grad_u, k_gather_out4);
+    if (st->ddev_strn) rc = rc ? rc : fetch(h, h->ddev.p, orig, nt, st->ddev_strn, k_gather_out1);
+    if (st->f_bound) rc = rc ? rc : fetch(h, h->f_bound.p, orig, nn, (double2 *)st->f_bound, k_gather_out2);
+    if (st->art_visc) rc = rc ? rc : fetch(h, h->art_visc.p, orig, nn, (double2 *)st->art_visc, k_gather_out2);
+    if (rc) return rc;
+    if (st->internal_vars) {
+        std::vector<double> iv(nt);
+        rc = fetch(h, S.ivar1.p, orig, nt, iv.data(), k_gather_out1);
+        if (rc) return rc;
+        std::memcpy(st->internal_vars, h->ivars_host.data(), h->ivars_host.size() * sizeof(double));
+        for (int i = 0; i < nt; ++i) st->internal_vars[(size_t)h->niv * i] = iv[i];
+    }
+    if (st->itype)
+        for (int i = 0; i < nt2; ++i) st->itype[i] = i < nn ? 2 : (i < nt ? 1 : 25);
+    if (st->wall_position) rc = fetch(h, S.wallpos.p, orig, nt2, st->wall_position, k_gather_outf);
+    if (!rc && st->horizontal_or_not) rc = fetch(h, S.horiz.p, orig, nt2, st->horizontal_or_not, k_gather_outf);
+    return rc;
+}
+
+extern "C" int sps_get_pairs(sps_handle *h, int64_t cap, int32_t *pair_i, int32_t *pair_j, int32_t *pint_type,
+                             float *w, float *dwdx, float *dwdy, int64_t *n) {
+    if (!h || !n) return fail(SPS_EINVAL, "null argument");
+    *n = (int64_t)h->n_pairs;
+    if (h->steps == 0) {
+        *n = 0;
+        return 0;
+    }
+    if (cap < (int64_t)h->n_pairs)
+        return fail(SPS_ECAPACITY, "pair buffer holds %lld < %llu", (long long)cap, h->n_pairs);
+    HIP_TRY(hipSetDevice(h->device));
+    size_t np = h->n_pairs;
+    DBuf<int> di, dj, dt;
+    DBuf<float> dw, dgx, dgy;
+    int rc = 0;
+    rc |= dalloc(di, np); rc |= dalloc(dj, np); rc |= dalloc(dt, np);
+    rc |= dalloc(dw, np); rc |= dalloc(dgx, np); rc |= dalloc(dgy, np);
+    if (rc) return rc;
+    StateSet &S = h->S[h->cur];
+    k_dump_pairs<<<nblk(h->nt2), BLOCK, 0, h->stream>>>(S.x.p, h->x0.p, S.hsml.p, h->cell.p, h->kstart.p, S.orig.p,
+                                                     h->off.p, h->proc.p, h->grid.p, h->p.pi, di.p, dj.p, dt.p,
+                                                     dw.p, dgx.p, dgy.p, h->nn, h->nt, h->nt2);
+    std::vector<int> hi(np), hj(np), ht(np);
+    std::vector<float> hw(np), hgx(np), hgy(np);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(hi.data(), di.p, np * sizeof(int), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(hj.data(), dj.p, np * sizeof(int), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(ht.data(), dt.p, np * sizeof(int), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(hw.data(), dw.p, np * sizeof(float), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(hgx.data(), dgx.p, np * sizeof(float), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(hgy.data(), dgy.p, np * sizeof(float), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    dfree(di); dfree(dj); dfree(dt); dfree(dw); dfree(dgx); dfree(dgy);
+    if (e != hipSuccess) return fail(SPS_EDEVICE, "pair download: %s", hipGetErrorString(e));
+    // list order: [n-1 .. M_prev] then [0 .. M_prev-1] (discovery indices)
+    const unsigned long long nn_ = h->n_pairs, mp = h->m_prev;
+    size_t k = 0;
+    auto put = [&](size_t d) {
+        if (pair_i) pair_i[k] = hi[d];
+        if (pair_j) pair_j[k] = hj[d];
+        if (pint_type) pint_type[k] = ht[d];
+        if (w) w[k] = hw[d];
+        if (dwdx) dwdx[k] = hgx[d];
+        if (dwdy) dwdy[k] = hgy[d];
+        k++;
+    };
+    if (nn_ > mp) {
+        for (unsigned long long d = nn_; d-- > mp;) put((size_t)d);
+        for (unsigned long long d = 0; d < mp; ++d) put((size_t)d);
+    } else {
+        for (unsigned long long d = 0; d < nn_; ++d) put((size_t)d);
+    }
+    return 0;
+}
+
+extern "C" int sps_get_stats(sps_handle *h, sps_stats *out) {
+    if (!h || !out) return fail(SPS_EINVAL, "null argument");
+    std::memset(out, 0, sizeof *out);
+    out->n_pairs = (int64_t)h->n_pairs;
+    out->m_pairs = (int64_t)h->m_pairs;
+    for (int k = 0; k < 10; ++k) out->n_by_type[k] = (int64_t)h->n_by_type[k];
+    out->ndivx = h->ndivx;
+    out->ndivy = h->ndivy;
+    out->steps = h->steps;
+    return 0;
+}
+
+extern "C" void sps_destroy(sps_handle *h) {
+    if (!h) return;
+    (void)hipSetDevice(h->device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    h->S[0].release();
+    h->S[1].release();
+    dfree(h->vx0); dfree(h->x0); dfree(h->vel0); dfree(h->rk_vel); dfree(h->rhs2); dfree(h->f_bound);
+    dfree(h->art_visc); dfree(h->stress0); dfree(h->rk_stress); dfree(h->rhs1); dfree(h->grad_u);
+    dfree(h->rk_dev); dfree(h->ddev); dfree(h->grid); dfree(h->split); dfree(h->bparts);
+    dfree(h->cellkey); dfree(h->new_cell); dfree(h->cell); dfree(h->kcount); dfree(h->kstart);
+    dfree(h->kfill); dfree(h->tmp_orig); dfree(h->tmp_src); dfree(h->proc); dfree(h->proc_to_slot);
+    dfree(h->cnt); dfree(h->off); dfree(h->scan_tmp); dfree(h->widthA); dfree(h->widthB);
+    dfree(h->offA); dfree(h->offB); dfree(h->degA); dfree(h->degB); dfree(h->totals); dfree(h->err);
+    dfree(h->a_s); dfree(h->a_w); dfree(h->a_gx); dfree(h->a_gy); dfree(h->b_s); dfree(h->b_gx);
+    dfree(h->b_gy);
+    if (h->pinned) (void)hipHostFree(h->pinned);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}

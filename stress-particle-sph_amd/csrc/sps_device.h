@@ -1,0 +1,193 @@
+// sps_device.h - per-particle and per-pair arithmetic of the reference hot
+// path, as device functions for gfx950.
+//
+// Every function restates one reference routine with its operation order and
+// its mixed precision: a single-precision operand met by a double one is
+// widened and the operation done in double; a result assigned to a
+// single-precision variable is rounded then.  The translation unit is built
+// with -ffp-contract=off, so no product is fused into a sum, and with
+// correctly rounded single-precision division and square root.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sps {
+
+// Constants of one problem, read by every kernel.
+struct DevParams {
+    int nn, ns, nd, nd2, nt, nt2;      // counts (reference ranges)
+    int cspm, update_x, ncrit, ntype_eco, do_bforce, do_av, ic_grav;
+    double pi, dx, alpha, beta, damping, cspm_tol;
+    double D11, D22, D12, D33, D41, D42;
+    double tanfi, coh;                 // props(1,13), props(1,14)
+    double young, poiss;               // props(1,3), props(1,4)
+    double xmin_dom[2], xmax_dom[2];
+    double cgrav[2];
+};
+
+// Kind of a storage slot: the storage keeps the reference's index ranges
+// (nodes, then stress points, then dummies), only reordered inside each.
+enum { KIND_NODE = 0, KIND_SP = 1, KIND_DUMMY = 2 };
+
+__device__ __forceinline__ int kind_of(int s, int nn, int nt) {
+    return s < nn ? KIND_NODE : (s < nt ? KIND_SP : KIND_DUMMY);
+}
+
+// kernel(): cubic spline, 2-D (code/2_SPH_main_2018.f90:1440-1492).
+// dxv = x_receiver - x_partner; the gradient is d W / d x_receiver.  The
+// double results are rounded to single precision as the pair list stores
+// them (code/5_SPH_global_vars_2018.f90:19-21, 2_...:1376-1378).
+__device__ __forceinline__ void cubic_kernel(double r, double dx0, double dx1, double hs,
+                                             double pi, float &w, float &gx, float &gy) {
+    double q = r / hs;
+    double factor = 15.0 / (7.0 * pi * hs * hs);
+    double wd, g0, g1;
+    if (q >= 0.0 && q <= 1.0) {
+        wd = factor * ((double)(2.0f / 3.0f) - q * q + (q * (q * q)) / 2.0);
+        double c = factor * (-2.0 + (double)(3.0f / 2.0f) * q) / (hs * hs);
+        g0 = c * dx0;
+        g1 = c * dx1;
+    } else if (q > 1.0 && q <= 2.0) {
+        double t = 2.0 - q;
+        wd = factor * 1.0 / 6.0 * (t * (t * t));
+        double c = factor * 1.0 / 6.0 * 3.0 * (t * t) / hs;
+        g0 = -(c * (dx0 / r));
+        g1 = -(c * (dx1 / r));
+    } else {
+        wd = 0.0;
+        g0 = 0.0;
+        g1 = 0.0;
+    }
+    w = (float)wd;
+    gx = (float)g0;
+    gy = (float)g1;
+}
+
+// Pair acceptance of grid_find_NEW (2_...:1345-1353): r < 2 * (h_i + h_j)/2.
+// Returns r (>= 0) when accepted, -1 otherwise.  Symmetric in i, j.
+__device__ __forceinline__ double pair_distance(double dx0, double dx1, double hi, double hj,
+                                                double &mhsml) {
+    double driac = dx0 * dx0;
+    driac = driac + dx1 * dx1;
+    mhsml = (hi + hj) / 2.0;
+    double r = sqrt(driac);
+    return (r < 2 * mhsml) ? r : -1.0;
+}
+
+// adapt_stress2 for one particle (code/3_SPH_material_2018.f90:2066-2140).
+__device__ __forceinline__ void adapt_stress(double4 &s, double tanfi, double coh) {
+    double alpha2 = tanfi / (sqrt(9 + 12 * (tanfi * tanfi)));
+    double kc = (3 * coh) / (sqrt(9 + 12 * (tanfi * tanfi)));
+    double smean = (s.x + s.y + s.w) / 3.0;
+    double d0 = s.x - smean, d1 = s.y - smean, d2 = s.z, d3 = s.w - smean;
+    double varj2 = d2 * d2 + 0.5 * (d0 * d0 + d1 * d1 + d3 * d3);
+    double yield = -(alpha2 * 3 * smean) + kc;
+    if (yield < 0) {
+        s.x = s.x - smean + kc / (3 * alpha2);
+        s.y = s.y - smean + kc / (3 * alpha2);
+        s.w = s.w - smean + kc / (3 * alpha2);
+        smean = (s.x + s.y + s.w) / 3.0;
+        d0 = s.x - smean;
+        d1 = s.y - smean;
+        d2 = s.z;
+        d3 = s.w - smean;
+        varj2 = d2 * d2 + 0.5 * (d0 * d0 + d1 * d1 + d3 * d3);
+        yield = -(alpha2 * 3 * smean) + kc;
+    }
+    if (yield < sqrt(varj2)) {
+        double rn = (-(3 * alpha2 * smean) + kc) / (sqrt(varj2));
+        if (sqrt(varj2) <= (double)10e-06f) rn = 0;
+        s.x = rn * d0 + smean;
+        s.y = rn * d1 + smean;
+        s.w = rn * d3 + smean;
+        s.z = rn * d2;
+    }
+}
+
+// plastic_terms -> drucker_prager -> Get_derivative_intvars for one stress
+// point (code/3_SPH_material_2018.f90:1867-1933, 1937-2062, 2557-2635).
+// st: stress, gu: grad_u as [g11, g21, g12, g22]; fdr: f_drucker in/out.
+// Gs receives the stress source (-G), der1 the deviatoric plastic strain rate.
+__device__ __forceinline__ void plastic_terms(const DevParams &P, const double4 &st,
+                                              const double4 &gu, double &fdr, double time_sph,
+                                              double4 &Gs, double &der1) {
+    double G0 = 0.0, G1 = 0.0, G2a = 0.0, G3 = 0.0;
+    double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+    Gs = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (P.ntype_eco > 1) {
+        if (P.ncrit == 12) {
+            double f0 = fdr;
+            float tanfi = (float)P.tanfi, coh = (float)P.coh;
+            float young = (float)P.young, poiss = (float)P.poiss;
+            float smean = (float)((st.x + st.y + st.w) / 3.0);
+            float dv0 = (float)(st.x - (double)smean);
+            float dv1 = (float)(st.y - (double)smean);
+            float dv2 = (float)st.z;
+            float dv3 = (float)(st.w - (double)smean);
+            float varj2 = dv2 * dv2 + 0.5f * (dv0 * dv0 + dv1 * dv1 + dv3 * dv3);
+            float vari1 = 3 * smean;
+            float eps11 = (float)gu.x;
+            float eps12 = (float)(0.5 * (gu.z + gu.y));
+            float eps22 = (float)gu.w;
+            float emean = eps11 + eps22;
+            float alpha2 = tanfi / (sqrtf(9 + 12 * (tanfi * tanfi)));
+            float kc = (3 * coh) / (sqrtf(9 + 12 * (tanfi * tanfi)));
+            float yield = -(alpha2 * vari1) + kc;
+            fdr = (double)(sqrtf(varj2) - yield);
+            float f1 = (float)fdr;
+            double df = (double)f1 - f0;
+            float G_mod = young / (2.0f * (1.0f + poiss));
+            float K_mod = young / (3.0f * (1.0f - 2.0f * poiss));
+            float s_eps = dv0 * eps11 + 2 * dv2 * eps12 + dv1 * eps22;
+            if (time_sph > 0 && f1 >= 0 && df >= 0 && sqrtf(varj2) >= 10e-06f) {
+                float sj = sqrtf(varj2);
+                float gq = G_mod / sj;
+                float lambda_1 = 3 * alpha2 * K_mod * emean;
+                float lambda_2 = gq * s_eps;
+                float lambda_3 = G_mod;
+                float G2 = (lambda_1 + lambda_2) / lambda_3;
+                G0 = (double)((gq * dv0) * G2);
+                G1 = (double)((gq * dv1) * G2);
+                G2a = (double)((gq * dv2) * G2);
+                G3 = (double)((gq * dv3) * G2);
+                float inv = 1.0f / sj;
+                v0 = (double)((1.0f / 6.0f) * inv * (2 * dv0 - dv1 - dv3));
+                v1 = (double)((1.0f / 6.0f) * inv * (2 * dv1 - dv0 - dv3));
+                v2 = (double)(inv * dv2);
+                v3 = (double)((1.0f / 6.0f) * inv * (2 * dv3 - dv0 - dv1));
+                v0 = v0 * (double)G2;
+                v1 = v1 * (double)G2;
+                v2 = v2 * (double)G2;
+                v3 = v3 * (double)G2;
+            }
+            Gs = make_double4(-G0, -G1, -G2a, -G3);
+        }
+        der1 = sqrt((2.0 * (v0 * v0 + v1 * v1 + v3 * v3) + v2 * v2) / 3.0);
+    } else {
+        der1 = 0.0;
+    }
+}
+
+// One term of boundary_forces test = 2 (code/2_SPH_main_2018.f90:1117-1161),
+// node i against inner-layer dummy j, single precision.  ri = x_i - x_j.
+__device__ __forceinline__ void wall_force_term(double ri0, double ri1, double hi, double hj,
+                                                double dx, float &t0, float &t1) {
+    float r0 = (float)ri0, r1 = (float)ri1;
+    float r2 = sqrtf(r0 * r0 + r1 * r1);
+    float d0 = (float)(dx / 2.);
+    float hh = (float)(0.5 * (hi + hj));
+    float r_bound = r2 / (0.75f * hh);
+    float f2 = (r2 > 0 && r2 < 1.5f * d0) ? 1 - (r2 / (1.5f * d0)) : 0.0f;
+    float f;
+    if (0 < r_bound && r_bound <= 2.0f / 3.0f) f = 2.0f / 3.0f;
+    else if (2.0f / 3.0f < r_bound && r_bound <= 1) f = 2 * r_bound - 1.5f * (r_bound * r_bound);
+    else if (1 < r_bound && r_bound < 2) f = 0.5f * ((2 - r_bound) * (2 - r_bound));
+    else f = 0.0f;
+    float c = 20;
+    float scale = 0.01f * (c * c);
+    t0 = scale * f2 * f * (r0 / (r2 * r2));
+    t1 = scale * f2 * f * (r1 / (r2 * r2));
+}
+
+}  // namespace sps

@@ -1,0 +1,98 @@
+"""Write the golden fixtures of the hot path from the compiled reference.
+
+Runs oracle/_ref/code/sps_ref_dump (the reference Fortran built from its own
+sources by `make -C oracle ref`) on the decks under
+stress-particle-sph_amd/decks and reduces its dumps to small .npz fixtures:
+
+  <deck>.npz  per dumped step s:
+      s<step>_<field>       full arrays (small decks, or the last step of c1)
+      s<step>_sha_<field>   SHA-256 of the raw little-endian bytes of a field
+      s<step>_pairs_*       pair list in linked-list order (small decks)
+      s<step>_sha_pairs     SHA-256 of the pair list records
+
+Only this container can run it (the GPU box has no /root/reference).
+Usage: python tests/golden/make_golden.py [deck ...]
+"""
+import hashlib
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, HERE)
+from refdump import read_pairs, read_state  # noqa: E402
+
+REF_BIN = os.path.join(ROOT, "oracle", "_ref", "code", "sps_ref_dump")
+DECKS = os.path.join(ROOT, "stress-particle-sph_amd", "decks")
+
+# deck: (steps to dump, steps stored in full, store full pair lists)
+PLAN = {
+    "t1_tiny": ([1, 2, 3, 10, 50, 200], "all", True),
+    "t2_cspm": ([1, 2, 50, 200], "all", True),
+    "t3_elastic": ([1, 50, 200], "all", True),
+    "c1_collapse": ([1, 2, 10, 100, 1000], [1000], False),
+}
+
+STATE_FIELDS = ("x", "vel", "stress", "ivar1", "f_drucker", "ddev_strn", "grad_u", "displ",
+                "f_bound", "art_visc")
+FULL_FIELDS_BIG = ("x", "vel", "stress", "ivar1", "f_drucker", "displ")
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def pair_records(p):
+    return np.ascontiguousarray(p).tobytes()
+
+
+def run(deck):
+    steps, full, pairs_full = PLAN[deck]
+    work = tempfile.mkdtemp(prefix=f"golden_{deck}_")
+    try:
+        for f in os.listdir(os.path.join(DECKS, deck)):
+            shutil.copy(os.path.join(DECKS, deck, f), work)
+        out = os.path.join(work, "out")
+        os.makedirs(out)
+        cmd = f"ulimit -s unlimited && {REF_BIN} {out} {max(steps)} " + " ".join(map(str, steps))
+        subprocess.run(["bash", "-c", cmd], cwd=work, check=True, stdout=subprocess.DEVNULL)
+        fx = {}
+        s0 = read_state(os.path.join(out, "state_000000.bin"))
+        for k in ("x", "mass", "rho", "hsml", "itype", "wall_position", "horizontal_or_not"):
+            if k in s0:
+                if deck == "c1_collapse":
+                    fx[f"s0_sha_{k}"] = np.array(sha(s0[k]))
+                else:
+                    fx[f"s0_{k}"] = s0[k]
+        for k in ("pi", "D11", "D22", "D12", "D33", "D41", "D42"):
+            fx[f"s0_{k}"] = np.array(s0[k])
+        fx["counts"] = np.array([s0["nnode"], s0["nstress"], s0["ndummy"], s0["ndummy2"]])
+        for st in steps:
+            s = read_state(os.path.join(out, f"state_{st:06d}.bin"))
+            keep_full = full == "all" or st in full
+            for k in STATE_FIELDS:
+                fx[f"s{st}_sha_{k}"] = np.array(sha(s[k]))
+                if keep_full and (full == "all" or k in FULL_FIELDS_BIG):
+                    fx[f"s{st}_{k}"] = s[k]
+            p = read_pairs(os.path.join(out, f"pairs_{st:06d}.bin"))
+            fx[f"s{st}_npairs"] = np.array(len(p))
+            fx[f"s{st}_sha_pairs"] = np.array(hashlib.sha256(pair_records(p)).hexdigest())
+            if pairs_full:
+                for k in ("i", "j", "type", "w", "dwdx", "dwdy"):
+                    fx[f"s{st}_pairs_{k}"] = p[k]
+        np.savez_compressed(os.path.join(HERE, f"{deck}.npz"), **fx)
+        print(deck, "->", os.path.getsize(os.path.join(HERE, f"{deck}.npz")), "bytes")
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
+if __name__ == "__main__":
+    if not os.path.exists(REF_BIN):
+        sys.exit("build the reference first: make -C oracle ref")
+    for d in (sys.argv[1:] or list(PLAN)):
+        run(d)
