@@ -1,0 +1,181 @@
+#!/usr/bin/env python3
+"""Throughput of the Stress-Particle SPH hot path on one MI355X.
+
+Workload: the cohesive vertical slope at full size -- 10 m x 10 m block on a
+0.01 m lattice (1,002,001 velocity particles + 1,000,000 stress particles)
+with a left wall and a 30 m bottom wall (12,177 dummies), Drucker-Prager
+c = 10 kPa, phi = 20 deg, E = 10 MPa, nu = 0.3, rho = 1850, dt = 1e-5 s, from
+rest under gravity (deck stress-particle-sph_amd/decks/c2_slope_dx01).  The
+form measured is the plain FP64 one whose results are bitwise equal to the
+reference (ordered summation).
+
+One step = one `time_integration` of the reference: neighbour search and the
+four RK4 stages with all their pair loops, plastic update, return mapping and
+position update.  The metric is pair interactions/s = (pairs found by each
+step's search, summed over the timed steps) / wall time.  Inputs are resident
+on the GPU before the timed region; nothing is copied to the host inside it.
+
+The CPU baseline is the oracle (a C restatement of the reference, 1 thread)
+on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "pair interactions/s and steps/s at 1M particles; surrogate+FP32 vs plain FP64"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def load_pkg():
+    import importlib
+    return importlib.import_module("stress-particle-sph_amd")
+
+
+def algorithmic_bytes(kernel, st, counts, params):
+    """Minimal single-touch bytes of one launch of `kernel`.
+
+    Neighbour-list entries are read once each (partner slot 4 B, w 4 B,
+    gradient 2 x 4 B, as far as the kernel uses them), every particle's state
+    that the launch reads or writes is counted once, however many receivers
+    gather it.  P_t = pairs of type t of the last search."""
+    nn, ns, nd = counts
+    P = st["n_by_type"]
+    P1, P3, P6, P9 = P[1], P[3], P[6], P[9]
+    cspm = params.cspm
+    if kernel == "k_deriv_nodes":
+        lists = 12 * (P1 + P6) + (12 * 2 * P3 if (params.alpha > 0 or params.beta > 0) else 0)
+        node = 16 + 16 + 32 + 8 + 8 + 8 + 16 + 16 + 16 + 16 + 16   # x v sigma m rho h rk_vel(rw) rhs2 av fb
+        sp = 32 + 8 + 8 + (16 if cspm else 0)                        # sigma m rho (x)
+        dummy = 16 + 8 + 8 + 8 + 4                                   # x m rho h orig
+        return lists + nn * node + ns * sp + nd * dummy
+    if kernel == "k_deriv_sps":
+        lists = 12 * (P1 + P9)
+        sp = 16 + 16 + 32 + 8 + 32 + 32 + 32 + 8 + 8 + 8            # x v sigma f_dp(rw) grad_u rhs1 rk_sigma(rw) rk_dev(rw)
+        node = 16 + 8 + 8 + (16 if cspm else 0)                      # v m rho (x)
+        dummy = 16 + 8 + 8 + 4 + 4                                   # x m rho wall horiz
+        return lists + ns * sp + nn * node + nd * dummy
+    if kernel == "k_spu_nodes":
+        return 8 * (P1 + P6) + nn * (32 + 32 + 8 + 16) + ns * (32 + 8 + 8 + 8)
+    if kernel == "k_spu_sps":
+        return 8 * (P1 + P9) + ns * (16 + 32 + 32) + nn * (16 + 8 + 8)
+    if kernel == "k_build":
+        entries = (P1 + P6) + (P1 + P9) + P6 + 2 * P3
+        return 16 * entries + (nn + ns + nd) * (16 + 8 + 4 + 4)
+    if kernel == "k_count":
+        return (nn + ns + nd) * (16 + 8 + 4 + 4 + 4 + 4)
+    return None
+
+
+def run_gpu(args, sps, deck):
+    import torch
+    sim = sps.Simulation.from_deck(deck)
+    counts = (sim.parts.nnode, sim.parts.nstress, sim.parts.ndummy)
+    print(f"[bench] deck {args.deck}: {counts[0]} nodes, {counts[1]} stress points, {counts[2]} dummies",
+          file=sys.stderr, flush=True)
+    for k in range(args.warmup):
+        sim.step(1)
+    print(f"[bench] warmup {args.warmup} steps done", file=sys.stderr, flush=True)
+    sim.set_profiling(True)
+    sim.profile(reset=True)
+    torch.cuda.synchronize()
+    pairs = 0
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        sim.step(1)
+        pairs += sim.stats()["n_pairs"]
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    prof = sim.profile(reset=True)
+    st = sim.stats()
+    elapsed = t1 - t0
+    return sim, counts, pairs, elapsed, prof, st
+
+
+def run_cpu(args, sps, deck, oracle_path):
+    sim = sps.Simulation.from_deck(deck, lib_path=oracle_path)
+    pairs = 0
+    t0 = time.perf_counter()
+    for k in range(args.cpu_steps):
+        sim.step(1)
+        pairs += sim.stats()["n_pairs"]
+    t1 = time.perf_counter()
+    sim.close()
+    return pairs, t1 - t0
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--deck", default="c2_slope_dx01")
+    ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps timed for the CPU baseline")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported by this build "
+              "(the multi-GPU sharding is described, not built)", file=sys.stderr)
+        sys.exit(2)
+    sps = load_pkg()
+    deck = sps.read_deck(os.path.join(ROOT, "stress-particle-sph_amd", "decks", args.deck))
+    sim, counts, pairs, elapsed, prof, st = run_gpu(args, sps, deck)
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = pairs / elapsed
+    # dominant kernel and its roofline
+    dom = max(prof.items(), key=lambda kv: kv[1][1])
+    dname, (dlaunch, dms) = dom
+    avg_s = dms / dlaunch / 1e3
+    ab = algorithmic_bytes(dname, st, counts, sim.params)
+    roofline = None
+    if ab is not None:
+        achieved = ab / avg_s / 1e9
+        roofline = {"kernel": dname, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                    "bytes_per_launch": int(ab), "avg_launch_ms": round(dms / dlaunch, 5),
+                    "launches": dlaunch}
+    breakdown = {k: {"launches": n, "ms_per_step": round(ms / args.steps, 4)} for k, (n, ms) in
+                 sorted(prof.items(), key=lambda kv: -kv[1][1])}
+    print("[bench] kernel breakdown (ms per step): " +
+          ", ".join(f"{k} {v['ms_per_step']}" for k, v in breakdown.items()), file=sys.stderr, flush=True)
+    sim.close()
+    cpu = None
+    if not args.no_cpu:
+        oracle = os.path.join(ROOT, "oracle", "libsps_oracle.so")
+        cpairs, ct = run_cpu(args, sps, deck, oracle)
+        cpu = {"value": round(cpairs / ct, 1), "unit": "pair interactions/s", "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_steps} steps of {args.deck} from rest ({sum(counts)} particles), "
+                         f"C oracle on 1 thread, {ct:.1f} s",
+               "steps_per_s": round(args.cpu_steps / ct, 4)}
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "pair interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{args.deck}: cohesive slope 10 m x 10 m on a 0.01 m lattice, "
+                               f"{counts[0]} velocity + {counts[1]} stress particles + {counts[2]} wall "
+                               "dummies, plain FP64 form bitwise equal to the reference",
+                   "deck": args.deck, "particles": sum(counts), "pairs_per_step": st["n_pairs"],
+                   "steps_per_s": round(args.steps / elapsed, 3), "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels": breakdown,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

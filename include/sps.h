@@ -170,6 +170,21 @@ int sps_get_pairs(sps_handle *h, int64_t cap, int32_t *pair_i, int32_t *pair_j,
 
 int sps_get_stats(sps_handle *h, sps_stats *out);
 
+/* Kernel timing.  No counterpart in the reference (which has no timers);
+ * for performance reports.  While enabled, every kernel launch of sps_step is
+ * bracketed by HIP events on the handle's stream and the elapsed times are
+ * accumulated per kernel; sps_get_profile returns them (and clears them when
+ * reset != 0).  The CPU oracle reports no kernels. */
+typedef struct sps_kernel_time {
+    char name[32];
+    int64_t launches;
+    double total_ms;
+} sps_kernel_time;
+
+int sps_set_profiling(sps_handle *h, int32_t enable);
+int sps_get_profile(sps_handle *h, int32_t cap, sps_kernel_time *out, int32_t *n,
+                    int32_t reset);
+
 /* Implementation name, e.g. "hip-gfx950" or "oracle-cpu". */
 const char *sps_backend(void);
 

@@ -166,7 +166,8 @@ static int push_pair(struct sps_handle *h, int it, int jt, double w, const doubl
     if (h->n_pairs == h->pcap) {
         int64_t nc = h->pcap ? h->pcap * 2 : 4096;
         int32_t *a = realloc(h->pi, nc * sizeof *a), *b, *c;
-        if (!a) return -1; h->pi = a;
+        if (!a) return -1;
+        h->pi = a;
         b = realloc(h->pj, nc * sizeof *b); if (!b) return -1; h->pj = b;
         c = realloc(h->pt, nc * sizeof *c); if (!c) return -1; h->pt = c;
         float *f1 = realloc(h->pw, nc * sizeof *f1); if (!f1) return -1; h->pw = f1;
@@ -624,8 +625,10 @@ static int bw_setup(struct sps_handle *h)
     for (int k = 0; k < nd2; ++k) {
         int j = h->nt + k;
         double a = h->x[2 * j], b = h->x[2 * j + 1];
-        if (a < x0) x0 = a; if (a > x1) x1 = a;
-        if (b < y0) y0 = b; if (b > y1) y1 = b;
+        if (a < x0) x0 = a;
+        if (a > x1) x1 = a;
+        if (b < y0) y0 = b;
+        if (b > y1) y1 = b;
     }
     h->bw_cell = reach;
     h->bw_x0 = x0 - reach;
@@ -991,6 +994,19 @@ int sps_get_stats(sps_handle *h, sps_stats *out)
     out->ndivx = h->ndivx;
     out->ndivy = h->ndivy;
     out->steps = h->steps;
+    return 0;
+}
+
+int sps_set_profiling(sps_handle *h, int32_t enable)
+{
+    if (!h) return fail(SPS_EINVAL, "null argument");
+    return 0;
+}
+
+int sps_get_profile(sps_handle *h, int32_t cap, sps_kernel_time *out, int32_t *n, int32_t reset)
+{
+    if (!h || !n) return fail(SPS_EINVAL, "null argument");
+    *n = 0;
     return 0;
 }
 

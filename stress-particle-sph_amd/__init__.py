@@ -80,6 +80,10 @@ class Stats(ctypes.Structure):
                 ("ndivy", ctypes.c_int32), ("steps", ctypes.c_int64)]
 
 
+class KernelTime(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 32), ("launches", ctypes.c_int64), ("total_ms", ctypes.c_double)]
+
+
 def load_library(path):
     """Load a library implementing include/sps.h and declare its signatures."""
     if not os.path.exists(path):
@@ -93,12 +97,15 @@ def load_library(path):
     lib.sps_get_state.argtypes = [h, ctypes.POINTER(State)]
     lib.sps_get_pairs.argtypes = [h, ctypes.c_int64] + [_P] * 6 + [ctypes.POINTER(ctypes.c_int64)]
     lib.sps_get_stats.argtypes = [h, ctypes.POINTER(Stats)]
+    lib.sps_set_profiling.argtypes = [h, ctypes.c_int32]
+    lib.sps_get_profile.argtypes = [h, ctypes.c_int32, ctypes.POINTER(KernelTime),
+                                    ctypes.POINTER(ctypes.c_int32), ctypes.c_int32]
     lib.sps_destroy.argtypes = [h]
     lib.sps_destroy.restype = None
     lib.sps_last_error.restype = ctypes.c_char_p
     lib.sps_backend.restype = ctypes.c_char_p
     for f in ("sps_create", "sps_set_state", "sps_step", "sps_get_state", "sps_get_pairs",
-              "sps_get_stats"):
+              "sps_get_stats", "sps_set_profiling", "sps_get_profile"):
         getattr(lib, f).restype = ctypes.c_int
     return lib
 
@@ -253,6 +260,17 @@ class Simulation:
         self._check(self.lib.sps_get_stats(self.handle, ctypes.byref(s)))
         return dict(n_pairs=s.n_pairs, m_pairs=s.m_pairs, n_by_type=list(s.n_by_type),
                     ndivx=s.ndivx, ndivy=s.ndivy, steps=s.steps)
+
+    def set_profiling(self, on=True):
+        self._check(self.lib.sps_set_profiling(self.handle, int(bool(on))))
+
+    def profile(self, reset=True):
+        """Per-kernel launch counts and total milliseconds (HIP events on the
+        library's stream) since the last reset."""
+        buf = (KernelTime * 64)()
+        n = ctypes.c_int32(0)
+        self._check(self.lib.sps_get_profile(self.handle, 64, buf, ctypes.byref(n), int(bool(reset))))
+        return {buf[k].name.decode(): (int(buf[k].launches), float(buf[k].total_ms)) for k in range(n.value)}
 
     def close(self):
         if self.handle:

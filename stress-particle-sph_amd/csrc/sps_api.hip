@@ -136,7 +136,77 @@ struct sps_handle {
     unsigned long long n_by_type[10] = {0};
     int ndivx = 0, ndivy = 0;
     unsigned long long *pinned = nullptr;   // readback staging
+    // kernel timing
+    bool prof = false;
+    struct PendingEv {
+        int kid;
+        hipEvent_t a, b;
+    };
+    std::vector<PendingEv> pending;
+    std::vector<hipEvent_t> ev_pool;
+    double prof_ms[32] = {0};
+    long long prof_n[32] = {0};
 };
+
+// ------------------------------------------------------------ profiling
+enum KernelId {
+    K_CHECK_OOD, K_BOUNDS, K_CELL_COUNT, K_CELL_SCATTER, K_CELL_SORT, K_PERMUTE, K_PROC,
+    K_COUNT_PAIRS, K_SCAN, K_SPLIT, K_SLICE, K_BUILD, K_RK_INIT, K_STAGE_BEGIN, K_SPU_NODES,
+    K_SPU_SPS, K_DERIV_SPS, K_DERIV_NODES, K_DUMMY_STRESS, K_RK_FINAL, K_POSITION, K_NKERNELS
+};
+static const char *const kKernelNames[K_NKERNELS] = {
+    "k_check_ood", "k_bounds", "k_cell_count", "k_cell_scatter", "k_cell_sort", "k_permute",
+    "k_proc_slot", "k_count", "k_scan", "k_split", "k_slice_width", "k_build", "k_rk_init",
+    "k_stage_begin", "k_spu_nodes", "k_spu_sps", "k_deriv_sps", "k_deriv_nodes",
+    "k_dummy_stress", "k_rk_final", "k_position"};
+
+static hipEvent_t prof_begin(sps_handle *h) {
+    if (!h->prof) return nullptr;
+    hipEvent_t e;
+    if (!h->ev_pool.empty()) {
+        e = h->ev_pool.back();
+        h->ev_pool.pop_back();
+    } else if (hipEventCreate(&e) != hipSuccess) {
+        return nullptr;
+    }
+    (void)hipEventRecord(e, h->stream);
+    return e;
+}
+
+static void prof_end(sps_handle *h, int kid, hipEvent_t e0) {
+    if (!h->prof || !e0) return;
+    hipEvent_t e1;
+    if (!h->ev_pool.empty()) {
+        e1 = h->ev_pool.back();
+        h->ev_pool.pop_back();
+    } else if (hipEventCreate(&e1) != hipSuccess) {
+        h->ev_pool.push_back(e0);
+        return;
+    }
+    (void)hipEventRecord(e1, h->stream);
+    h->pending.push_back({kid, e0, e1});
+}
+
+// after a stream synchronisation: fold the recorded intervals into totals
+static void prof_collect(sps_handle *h) {
+    for (auto &pe : h->pending) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, pe.a, pe.b) == hipSuccess) {
+            h->prof_ms[pe.kid] += ms;
+            h->prof_n[pe.kid] += 1;
+        }
+        h->ev_pool.push_back(pe.a);
+        h->ev_pool.push_back(pe.b);
+    }
+    h->pending.clear();
+}
+
+#define LAUNCH(KID, GRID, BLK, KERN, ...)                    \
+    do {                                                     \
+        hipEvent_t e0_ = prof_begin(h);                      \
+        KERN<<<(GRID), (BLK), 0, h->stream>>>(__VA_ARGS__);  \
+        prof_end(h, (KID), e0_);                             \
+    } while (0)
 
 extern "C" const char *sps_last_error(void) { return g_err; }
 extern "C" const char *sps_backend(void) { return "hip-gfx950"; }
@@ -149,9 +219,9 @@ static int scan_u32(sps_handle *h, const unsigned *in, unsigned *out, long long 
     if (nb == 0) nb = 1;
     int rc = dalloc(h->scan_tmp, (size_t)nb);
     if (rc) return rc;
-    k_scan_local<<<(unsigned)nb, BLOCK, 0, h->stream>>>(in, out, h->scan_tmp.p, n);
-    k_scan_sums<<<1, BLOCK, 0, h->stream>>>(h->scan_tmp.p, (int)nb, dtotal);
-    k_scan_add<<<(unsigned)nb, BLOCK, 0, h->stream>>>(out, h->scan_tmp.p, n);
+    LAUNCH(K_SCAN, (unsigned)nb, BLOCK, k_scan_local, in, out, h->scan_tmp.p, n);
+    LAUNCH(K_SCAN, 1, BLOCK, k_scan_sums, h->scan_tmp.p, (int)nb, dtotal);
+    LAUNCH(K_SCAN, (unsigned)nb, BLOCK, k_scan_add, out, h->scan_tmp.p, n);
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -355,10 +425,10 @@ static int search(sps_handle *h, bool fresh_list) {
     StateSet &A = h->S[h->cur];
     StateSet &B = h->S[1 - h->cur];
     // domain check and grid
-    k_check_ood<<<nblk(nt2), BLOCK, 0, s>>>(A.x.p, A.ood.p, nt2, h->p.xmin_domain[0], h->p.xmin_domain[1],
+    LAUNCH(K_CHECK_OOD, nblk(nt2), BLOCK, k_check_ood, A.x.p, A.ood.p, nt2, h->p.xmin_domain[0], h->p.xmin_domain[1],
                                            h->p.xmax_domain[0], h->p.xmax_domain[1]);
-    k_bounds_partial<<<h->nbparts, BLOCK, 0, s>>>(A.x.p, A.hsml.p, A.ood.p, nt2, h->bparts.p);
-    k_bounds_final<<<1, 64, 0, s>>>(h->bparts.p, h->nbparts, h->grid.p, h->cell_cap);
+    LAUNCH(K_BOUNDS, h->nbparts, BLOCK, k_bounds_partial, A.x.p, A.hsml.p, A.ood.p, nt2, h->bparts.p);
+    LAUNCH(K_BOUNDS, 1, 64, k_bounds_final, h->bparts.p, h->nbparts, h->grid.p, h->cell_cap);
     // the cell count is needed on the host for the bucket arrays' extent
     HIP_TRY(hipMemcpyAsync(h->pinned, h->grid.p, sizeof(Grid), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
@@ -375,44 +445,35 @@ static int search(sps_handle *h, bool fresh_list) {
     const long long L3 = 3LL * (ndivt + 1);
     HIP_TRY(hipMemsetAsync(h->kcount.p, 0, (L3 + 1) * sizeof(int), s));
     HIP_TRY(hipMemsetAsync(h->kfill.p, 0, (L3 + 1) * sizeof(int), s));
-    k_cell_count<<<nblk(nt2), BLOCK, 0, s>>>(A.x.p, A.ood.p, h->grid.p, h->cellkey.p, h->kcount.p, nn, nt, nt2);
+    LAUNCH(K_CELL_COUNT, nblk(nt2), BLOCK, k_cell_count, A.x.p, A.ood.p, h->grid.p, h->cellkey.p, h->kcount.p, nn, nt, nt2);
     int rc = scan_u32(h, (const unsigned *)h->kcount.p, (unsigned *)h->kstart.p, L3 + 1, nullptr);
     if (rc) return rc;
-    k_cell_scatter<<<nblk(nt2), BLOCK, 0, s>>>(h->cellkey.p, A.orig.p, h->kstart.p, h->kfill.p, h->grid.p,
+    LAUNCH(K_CELL_SCATTER, nblk(nt2), BLOCK, k_cell_scatter, h->cellkey.p, A.orig.p, h->kstart.p, h->kfill.p, h->grid.p,
                                               h->tmp_orig.p, h->tmp_src.p, h->cell.p, nn, nt, nt2);
-    k_cell_sort<<<nblk(L3), BLOCK, 0, s>>>(h->kstart.p, h->tmp_orig.p, h->tmp_src.p, h->grid.p, h->err.p);
+    LAUNCH(K_CELL_SORT, nblk(L3), BLOCK, k_cell_sort, h->kstart.p, h->tmp_orig.p, h->tmp_src.p, h->grid.p, h->err.p);
     // permute the state into (kind, cell, original index) order
-    k_permute<<<nblk(nt2), BLOCK, 0, s>>>(B.view(), A.view(), h->tmp_src.p, nn, nt, nt2);
+    LAUNCH(K_PERMUTE, nblk(nt2), BLOCK, k_permute, B.view(), A.view(), h->tmp_src.p, nn, nt, nt2);
     HIP_TRY(hipGetLastError());
     h->cur = 1 - h->cur;
     StateSet &C = h->S[h->cur];
-    k_proc_slot<<<nblk(nt2), BLOCK, 0, s>>>(h->cell.p, h->kstart.p, h->grid.p, h->proc.p, nn, nt, nt2);
-    k_proc_to_slot<<<nblk(nt2), BLOCK, 0, s>>>(h->proc.p, h->proc_to_slot.p, nt2);
+    LAUNCH(K_PROC, nblk(nt2), BLOCK, k_proc_slot, h->cell.p, h->kstart.p, h->grid.p, h->proc.p, nn, nt, nt2);
+    LAUNCH(K_PROC, nblk(nt2), BLOCK, k_proc_to_slot, h->proc.p, h->proc_to_slot.p, nt2);
     // counts, offsets, split
     HIP_TRY(hipMemsetAsync(h->cnt.p, 0, nt2 * sizeof(unsigned), s));
     const unsigned long long M = fresh_list ? 0ull : h->m_pairs;
-    k_split_reset<<<1, 64, 0, s>>>(h->split.p, M);
-    k_count<<<nblk(nt2), BLOCK, 0, s>>>(C.x.p, C.hsml.p, h->cell.p, h->kstart.p, h->proc.p, h->grid.p,
+    LAUNCH(K_SPLIT, 1, 64, k_split_reset, h->split.p, M);
+    LAUNCH(K_COUNT_PAIRS, nblk(nt2), BLOCK, k_count, C.x.p, C.hsml.p, h->cell.p, h->kstart.p, h->proc.p, h->grid.p,
                                        h->cnt.p, h->degA.p, h->degB.p, h->split.p, nn, nt, nt2);
     HIP_TRY(hipGetLastError());
-    // number of in-domain particles = end of the last in-domain bucket
-    int nproc_h = 0;
-    {
-        int ks[4];
-        HIP_TRY(hipMemcpyAsync(ks, h->kstart.p + (size_t)ndivt, sizeof(int), hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipMemcpyAsync(ks + 1, h->kstart.p + (size_t)(ndivt + 1) + ndivt, sizeof(int), hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipMemcpyAsync(ks + 2, h->kstart.p + (size_t)2 * (ndivt + 1) + ndivt, sizeof(int), hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipStreamSynchronize(s));
-        nproc_h = (ks[0] - 0) + (ks[1] - nn) + (ks[2] - nt);
-    }
-    rc = scan_u32(h, h->cnt.p, h->off.p, nproc_h, nullptr);
+    // discovery offsets over all slots (slots past the in-domain count hold 0)
+    rc = scan_u32(h, h->cnt.p, h->off.p, nt2, &h->split.p->n);
     if (rc) return rc;
-    if (nproc_h > 0) k_split<<<nblk(nproc_h), BLOCK, 0, s>>>(h->cnt.p, h->off.p, nproc_h, h->split.p);
-    k_split_partner<<<1, 64, 0, s>>>(C.x.p, C.hsml.p, h->cell.p, h->kstart.p, h->proc.p, h->off.p,
+    LAUNCH(K_SPLIT, nblk(nt2), BLOCK, k_split, h->cnt.p, h->off.p, nt2, h->split.p);
+    LAUNCH(K_SPLIT, 1, 64, k_split_partner, C.x.p, C.hsml.p, h->cell.p, h->kstart.p, h->proc.p, h->off.p,
                                      h->proc_to_slot.p, h->grid.p, h->split.p, nn, nt);
     // list shapes
-    k_slice_width<<<nblk(((nt2 + 63) / 64) * 64), BLOCK, 0, s>>>(h->degA.p, nt2, h->widthA.p);
-    k_slice_width<<<nblk(((nn + 63) / 64) * 64), BLOCK, 0, s>>>(h->degB.p, nn, h->widthB.p);
+    LAUNCH(K_SLICE, nblk(((nt2 + 63) / 64) * 64), BLOCK, k_slice_width, h->degA.p, nt2, h->widthA.p);
+    LAUNCH(K_SLICE, nblk(((nn + 63) / 64) * 64), BLOCK, k_slice_width, h->degB.p, nn, h->widthB.p);
     rc = scan_u32(h, h->widthA.p, h->offA.p, (nt2 + 63) / 64, h->totals.p);
     if (rc) return rc;
     rc = scan_u32(h, h->widthB.p, h->offB.p, (nn + 63) / 64, h->totals.p + 1);
@@ -437,7 +498,7 @@ static int search(sps_handle *h, bool fresh_list) {
     }
     if (rc) return rc;
     Lists Lz{h->a_s.p, h->a_w.p, h->a_gx.p, h->a_gy.p, h->offA.p, h->b_s.p, h->b_gx.p, h->b_gy.p, h->offB.p};
-    k_build<<<nblk(nt2), BLOCK, 0, s>>>(C.x.p, C.hsml.p, h->cell.p, h->kstart.p, h->proc.p, h->cnt.p, h->off.p,
+    LAUNCH(K_BUILD, nblk(nt2), BLOCK, k_build, C.x.p, C.hsml.p, h->cell.p, h->kstart.p, h->proc.p, h->cnt.p, h->off.p,
                                        h->grid.p, h->split.p, Lz, h->p.pi, nn, nt, nt2);
     HIP_TRY(hipGetLastError());
     h->n_pairs = sh.n;
@@ -462,22 +523,22 @@ static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
     const double factg = gravity_factor(&h->p, time_sph);
     const double gxg = h->p.ic_grav ? factg * h->p.cgrav[0] : 0.0;
     const double gyg = h->p.ic_grav ? factg * h->p.cgrav[1] : 0.0;
-    k_rk_init<<<nblk(nt2), BLOCK, 0, s>>>(SB, W, nn, nt, nt2);
+    LAUNCH(K_RK_INIT, nblk(nt2), BLOCK, k_rk_init, SB, W, nn, nt, nt2);
     for (int st = 0; st < 4; ++st) {
         const double c1 = f1rk[st] * (dt);
-        k_stage_begin<<<nblk(nt), BLOCK, 0, s>>>(SB, W, c1, adapt, tanfi, coh, nn, nt);
-        k_spu_nodes<<<nblk(nn), BLOCK, 0, s>>>(SB, h->degA.p, Lz, adapt, tanfi, coh, nn, nt);
-        k_spu_sps<<<nblk(nt - nn), BLOCK, 0, s>>>(SB, h->degA.p, Lz, adapt, tanfi, coh, nn, nt);
-        k_deriv_sps<<<nblk(nt - nn), BLOCK, 0, s>>>(SB, W, h->degA.p, Lz, h->dp, time_sph, f2rk[st], nn, nt);
-        k_deriv_nodes<<<nblk(nn), BLOCK, 0, s>>>(SB, W, h->degA.p, h->degB.p, Lz, h->dp, gxg, gyg, f2rk[st],
+        LAUNCH(K_STAGE_BEGIN, nblk(nt), BLOCK, k_stage_begin, SB, W, c1, adapt, tanfi, coh, nn, nt);
+        LAUNCH(K_SPU_NODES, nblk(nn), BLOCK, k_spu_nodes, SB, h->degA.p, Lz, adapt, tanfi, coh, nn, nt);
+        LAUNCH(K_SPU_SPS, nblk(nt - nn), BLOCK, k_spu_sps, SB, h->degA.p, Lz, adapt, tanfi, coh, nn, nt);
+        LAUNCH(K_DERIV_SPS, nblk(nt - nn), BLOCK, k_deriv_sps, SB, W, h->degA.p, Lz, h->dp, time_sph, f2rk[st], nn, nt);
+        LAUNCH(K_DERIV_NODES, nblk(nn), BLOCK, k_deriv_nodes, SB, W, h->degA.p, h->degB.p, Lz, h->dp, gxg, gyg, f2rk[st],
                                                  h->err.p, nn, nt);
         if (st == 3 && nt2 > nt)
-            k_dummy_stress<<<nblk(nt2 - nt), BLOCK, 0, s>>>(SB, h->degA.p, Lz, nt, nt2);
+            LAUNCH(K_DUMMY_STRESS, nblk(nt2 - nt), BLOCK, k_dummy_stress, SB, h->degA.p, Lz, nt, nt2);
     }
-    k_rk_final<<<nblk(nt), BLOCK, 0, s>>>(SB, W, dt, adapt, tanfi, coh, nn, nt);
-    k_spu_nodes<<<nblk(nn), BLOCK, 0, s>>>(SB, h->degA.p, Lz, adapt, tanfi, coh, nn, nt);
-    k_spu_sps<<<nblk(nt - nn), BLOCK, 0, s>>>(SB, h->degA.p, Lz, adapt, tanfi, coh, nn, nt);
-    k_position<<<nblk(nt), BLOCK, 0, s>>>(SB, W, dt, h->p.update_x, nn, nt);
+    LAUNCH(K_RK_FINAL, nblk(nt), BLOCK, k_rk_final, SB, W, dt, adapt, tanfi, coh, nn, nt);
+    LAUNCH(K_SPU_NODES, nblk(nn), BLOCK, k_spu_nodes, SB, h->degA.p, Lz, adapt, tanfi, coh, nn, nt);
+    LAUNCH(K_SPU_SPS, nblk(nt - nn), BLOCK, k_spu_sps, SB, h->degA.p, Lz, adapt, tanfi, coh, nn, nt);
+    LAUNCH(K_POSITION, nblk(nt), BLOCK, k_position, SB, W, dt, h->p.update_x, nn, nt);
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -499,6 +560,7 @@ extern "C" int sps_step(sps_handle *h, int32_t nsteps, double dt, double *time_s
     }
     HIP_TRY(hipMemcpyAsync(h->pinned, h->err.p, sizeof(int), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
+    prof_collect(h);
     if (*(int *)h->pinned) {
         h->poisoned = true;
         return fail(SPS_ECAPACITY, "device capacity flag %d: more than %d wall-force terms on one node (1) "
@@ -625,10 +687,42 @@ extern "C" int sps_get_stats(sps_handle *h, sps_stats *out) {
     return 0;
 }
 
+extern "C" int sps_set_profiling(sps_handle *h, int32_t enable) {
+    if (!h) return fail(SPS_EINVAL, "null argument");
+    h->prof = enable != 0;
+    return 0;
+}
+
+extern "C" int sps_get_profile(sps_handle *h, int32_t cap, sps_kernel_time *out, int32_t *n, int32_t reset) {
+    if (!h || !n) return fail(SPS_EINVAL, "null argument");
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    prof_collect(h);
+    int k = 0;
+    for (int id = 0; id < K_NKERNELS; ++id) {
+        if (h->prof_n[id] == 0) continue;
+        if (out && k < cap) {
+            std::memset(out[k].name, 0, sizeof out[k].name);
+            std::strncpy(out[k].name, kKernelNames[id], sizeof out[k].name - 1);
+            out[k].launches = h->prof_n[id];
+            out[k].total_ms = h->prof_ms[id];
+        }
+        k++;
+    }
+    *n = k;
+    if (reset)
+        for (int id = 0; id < K_NKERNELS; ++id) {
+            h->prof_ms[id] = 0.0;
+            h->prof_n[id] = 0;
+        }
+    return 0;
+}
+
 extern "C" void sps_destroy(sps_handle *h) {
     if (!h) return;
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    prof_collect(h);
+    for (auto e : h->ev_pool) (void)hipEventDestroy(e);
     h->S[0].release();
     h->S[1].release();
     dfree(h->vx0); dfree(h->x0); dfree(h->vel0); dfree(h->rk_vel); dfree(h->rhs2); dfree(h->f_bound);

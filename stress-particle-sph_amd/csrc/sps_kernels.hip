@@ -369,14 +369,14 @@ __global__ void k_split_reset(Split *__restrict__ split, unsigned long long M) {
     for (int k = 0; k < 10; ++k) split->n_by_type[k] = 0;
 }
 
-// n = total pairs; locate the processing slot that holds discovery index M.
+// Locate the processing slot whose discoveries straddle index M (n is the
+// scan total of the per-slot counts, written by the scan).
 __global__ void k_split(const unsigned *__restrict__ cnt, const unsigned *__restrict__ off,
-                        int nproc, Split *__restrict__ split) {
+                        int nslots, Split *__restrict__ split) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= nproc) return;
+    if (p >= nslots) return;
     unsigned long long M = split->M;
     unsigned long long o = off[p], c = cnt[p];
-    if (p == nproc - 1) split->n = o + c;
     if (o < M && M < o + c) split->sstar = p;
 }
 
