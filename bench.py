@@ -117,7 +117,20 @@ def main():
     ap.add_argument("--deck", default="c2_slope_dx01")
     ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps timed for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (5 times) and exit; its "
+                         "FETCH_SIZE / WRITE_SIZE counters calibrate the HBM traffic of the kernels")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_each_way": n * 8, "launches": 5}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported by this build "
               "(the multi-GPU sharding is described, not built)", file=sys.stderr)

@@ -1,0 +1,98 @@
+"""The C-ABI libraries load and export every entry point include/sps.h
+declares; the ctypes mirrors match the C struct layouts.  No compute call is
+made on the product library here (this suite runs without a GPU)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+from conftest import ORACLE_LIB, ROOT, load_pkg
+
+HEADER = os.path.join(ROOT, "include", "sps.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(sps_[a-z_]+)\s*\(", text)))
+
+
+def test_header_declares_the_boundary():
+    names = declared_functions()
+    for required in ("sps_create", "sps_set_state", "sps_step", "sps_get_state", "sps_get_pairs",
+                     "sps_get_stats", "sps_last_error", "sps_destroy"):
+        assert required in names
+
+
+@pytest.mark.parametrize("which", ["product", "oracle"])
+def test_library_exports_every_declared_symbol(which):
+    sps = load_pkg()
+    path = sps.PRODUCT_LIB if which == "product" else ORACLE_LIB
+    assert os.path.exists(path), f"{path} not built"
+    lib = ctypes.CDLL(path)
+    for name in declared_functions():
+        assert hasattr(lib, name), f"{os.path.basename(path)} lacks {name}"
+
+
+def test_product_library_is_gfx950_code_object():
+    """The library built here carries a gfx950 device code object."""
+    sps = load_pkg()
+    blob = open(sps.PRODUCT_LIB, "rb").read()
+    assert b"amdgcn-amd-amdhsa--gfx950" in blob
+
+
+def test_struct_layouts_match_c():
+    """Compile a probe against include/sps.h and compare sizeof/offsetof with
+    the ctypes mirrors used by the Python host."""
+    sps = load_pkg()
+    src = r'''
+#include <stddef.h>
+#include <stdio.h>
+#include "sps.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu %zu %zu\n", sizeof(sps_params), offsetof(sps_params, props),
+         offsetof(sps_params, tcurve_f), sizeof(sps_state), sizeof(sps_stats),
+         offsetof(sps_stats, ndivx), sizeof(sps_kernel_time));
+  return 0;
+}
+'''
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "probe.c")
+        open(c, "w").write(src)
+        exe = os.path.join(d, "probe")
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", exe, c], check=True)
+        got = [int(v) for v in subprocess.run([exe], capture_output=True, text=True).stdout.split()]
+    want = [ctypes.sizeof(sps.Params), sps.Params.props.offset, sps.Params.tcurve_f.offset,
+            ctypes.sizeof(sps.State), ctypes.sizeof(sps.Stats), sps.Stats.ndivx.offset,
+            ctypes.sizeof(sps.KernelTime)]
+    assert got == want
+
+
+def test_product_refuses_without_gpu_or_runs_with_one():
+    """On a machine without a GPU the product must fail loudly (no CPU
+    fallback); with a GPU it creates a handle."""
+    sps = load_pkg()
+    deck = sps.read_deck(os.path.join(ROOT, "stress-particle-sph_amd", "decks", "t1_tiny"))
+    parts = sps.build_particles(deck)
+    prm = sps.make_params(deck, parts)
+    try:
+        sim = sps.Simulation(prm)
+    except sps.SpsError as e:
+        assert e.code == -3 and "no HIP device" in str(e)
+        return
+    assert sim.backend == "hip-gfx950"
+    sim.close()
+
+
+def test_product_rejects_unsupported_options():
+    sps = load_pkg()
+    deck = sps.read_deck(os.path.join(ROOT, "stress-particle-sph_amd", "decks", "t1_tiny"))
+    parts = sps.build_particles(deck)
+    prm = sps.make_params(deck, parts)
+    prm.cont_density = 1
+    with pytest.raises(sps.SpsError) as e:
+        sps.Simulation(prm)
+    assert e.value.code == -1
