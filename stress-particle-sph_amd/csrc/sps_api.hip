@@ -428,7 +428,7 @@ static int search(sps_handle *h, bool fresh_list) {
     LAUNCH(K_CHECK_OOD, nblk(nt2), BLOCK, k_check_ood, A.x.p, A.ood.p, nt2, h->p.xmin_domain[0], h->p.xmin_domain[1],
                                            h->p.xmax_domain[0], h->p.xmax_domain[1]);
     LAUNCH(K_BOUNDS, h->nbparts, BLOCK, k_bounds_partial, A.x.p, A.hsml.p, A.ood.p, nt2, h->bparts.p);
-    LAUNCH(K_BOUNDS, 1, 64, k_bounds_final, h->bparts.p, h->nbparts, h->grid.p, h->cell_cap);
+    LAUNCH(K_BOUNDS, 1, BLOCK, k_bounds_final, h->bparts.p, h->nbparts, h->grid.p, h->cell_cap);
     // the cell count is needed on the host for the bucket arrays' extent
     HIP_TRY(hipMemcpyAsync(h->pinned, h->grid.p, sizeof(Grid), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));

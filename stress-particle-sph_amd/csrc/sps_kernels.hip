@@ -92,20 +92,37 @@ __global__ void k_bounds_partial(const double2 *__restrict__ x, const double *__
         for (int k = 0; k < 5; ++k) part[5 * blockIdx.x + k] = s[k][0];
 }
 
-// Grid parameters with the reference's double-precision expressions.
-__global__ void k_bounds_final(const double *__restrict__ part, int nparts, Grid *__restrict__ g,
-                               int cell_cap) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    double mn[2] = {(double)1.e+10f, (double)1.e+10f}, mx[2] = {-(double)1.e+10f, -(double)1.e+10f};
-    double hm = 0.0;
-    for (int b = 0; b < nparts; ++b) {
-        mn[0] = fmin(mn[0], part[5 * b + 0]);
-        mn[1] = fmin(mn[1], part[5 * b + 1]);
-        mx[0] = fmax(mx[0], part[5 * b + 2]);
-        mx[1] = fmax(mx[1], part[5 * b + 3]);
+// Grid parameters with the reference's double-precision expressions
+// (one block: reduce the partials, then thread 0 derives the grid).
+// Launch with exactly BLOCK threads.
+__global__ void __launch_bounds__(BLOCK) k_bounds_final(const double *__restrict__ part, int nparts,
+                                                        Grid *__restrict__ g, int cell_cap) {
+    __shared__ double s[5][BLOCK];
+    int t = threadIdx.x;
+    double mn0 = (double)1.e+10f, mn1 = (double)1.e+10f, mx0 = -(double)1.e+10f,
+           mx1 = -(double)1.e+10f, hm = 0.0;
+    for (int b = t; b < nparts; b += blockDim.x) {
+        mn0 = fmin(mn0, part[5 * b + 0]);
+        mn1 = fmin(mn1, part[5 * b + 1]);
+        mx0 = fmax(mx0, part[5 * b + 2]);
+        mx1 = fmax(mx1, part[5 * b + 3]);
         hm = fmax(hm, part[5 * b + 4]);
     }
-    double deltx = hm * 2;
+    s[0][t] = mn0; s[1][t] = mn1; s[2][t] = mx0; s[3][t] = mx1; s[4][t] = hm;
+    __syncthreads();
+    for (int w = BLOCK / 2; w > 0; w >>= 1) {
+        if (t < w) {
+            s[0][t] = fmin(s[0][t], s[0][t + w]);
+            s[1][t] = fmin(s[1][t], s[1][t + w]);
+            s[2][t] = fmax(s[2][t], s[2][t + w]);
+            s[3][t] = fmax(s[3][t], s[3][t + w]);
+            s[4][t] = fmax(s[4][t], s[4][t + w]);
+        }
+        __syncthreads();
+    }
+    if (t != 0) return;
+    double mn[2] = {s[0][0], s[1][0]}, mx[2] = {s[2][0], s[3][0]};
+    double deltx = s[4][0] * 2;
     int nd[2];
     for (int d = 0; d < 2; ++d) {
         double length = mx[d] - mn[d];
@@ -248,24 +265,23 @@ __global__ void k_permute(StateBufs dst, StateBufs src, const int *__restrict__ 
 }
 
 // --------------------------------------------------------- stencil helpers
-// The 3x3 stencil of a cell, as (first cell, last cell) of each row.
+// The 3x3 stencil of a cell: rows y0..y1, columns x0..x1 (clamped at the
+// grid edges like 2_...:1326-1327).  Kept in scalars: an indexed array here
+// would be placed in scratch or LDS by the compiler.
 struct Stencil {
-    int lo[3], hi[3];
-    int nrows;
+    int x0, x1, y0, y1, ndivx;
+    __device__ __forceinline__ int lo(int y) const { return y * ndivx + x0; }
+    __device__ __forceinline__ int hi(int y) const { return y * ndivx + x1; }
 };
 
 __device__ __forceinline__ Stencil make_stencil(int c, int ndivx, int ndivy) {
     Stencil S;
     int cx = c % ndivx, cy = c / ndivx;
-    int x0 = cx > 0 ? cx - 1 : 0, x1 = cx + 1 < ndivx ? cx + 1 : ndivx - 1;
-    S.nrows = 0;
-    for (int dy = -1; dy <= 1; ++dy) {
-        int y = cy + dy;
-        if (y < 0 || y >= ndivy) continue;
-        S.lo[S.nrows] = y * ndivx + x0;
-        S.hi[S.nrows] = y * ndivx + x1;
-        S.nrows++;
-    }
+    S.x0 = cx > 0 ? cx - 1 : 0;
+    S.x1 = cx + 1 < ndivx ? cx + 1 : ndivx - 1;
+    S.y0 = cy > 0 ? cy - 1 : 0;
+    S.y1 = cy + 1 < ndivy ? cy + 1 : ndivy - 1;
+    S.ndivx = ndivx;
     return S;
 }
 
@@ -302,8 +318,7 @@ __global__ void k_count(const double2 *__restrict__ x, const double *__restrict_
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     unsigned fwd = 0;
     int da = 0, db = 0;
-    unsigned tc[10];
-    for (int k = 0; k < 10; ++k) tc[k] = 0;
+    unsigned t0 = 0, t1 = 0, t2 = 0, t3 = 0, t6 = 0, t9 = 0;   // pairs by pint_type
     int ndivt = g->ndivt;
     if (r < nt2 && cell[r] < ndivt) {
         const int L = ndivt + 1;
@@ -312,9 +327,10 @@ __global__ void k_count(const double2 *__restrict__ x, const double *__restrict_
         double2 xr = x[r];
         double hr = hsml[r];
         Stencil S = make_stencil(c, g->ndivx, g->ndivy);
-        for (int row = 0; row < S.nrows; ++row) {
+        for (int y = S.y0; y <= S.y1; ++y) {
+            const int rlo = S.lo(y), rhi = S.hi(y);
             for (int ks = 0; ks < 3; ++ks) {
-                int lo = kstart[ks * L + S.lo[row]], hi = kstart[ks * L + S.hi[row] + 1];
+                int lo = kstart[ks * L + rlo], hi = kstart[ks * L + rhi + 1];
                 int c_beg = kstart[ks * L + c], c_end = kstart[ks * L + c + 1];
                 for (int s = lo; s < hi; ++s) {
                     if (s == r) continue;
@@ -323,14 +339,21 @@ __global__ void k_count(const double2 *__restrict__ x, const double *__restrict_
                     double rr = pair_distance(xr.x - xs.x, xr.y - xs.y, hr, hsml[s], mh);
                     if (rr < 0.0) continue;
                     bool after;
-                    if (S.lo[row] > c) after = true;                 // row above
-                    else if (S.hi[row] < c) after = false;           // row below
+                    if (rlo > c) after = true;                       // row above
+                    else if (rhi < c) after = false;                 // row below
                     else if (s >= c_end) after = true;               // right cell
                     else if (s < c_beg) after = false;               // left cell
                     else after = ks > kr || (ks == kr && s > r);     // same cell
                     if (after) {
                         fwd++;
-                        tc[pair_type(kr, ks)]++;
+                        switch (pair_type(kr, ks)) {
+                            case 1: t1++; break;
+                            case 2: t2++; break;
+                            case 3: t3++; break;
+                            case 6: t6++; break;
+                            case 9: t9++; break;
+                            default: t0++; break;
+                        }
                     }
                     if (in_list_a(kr, ks)) da++;
                     if (kr == KIND_NODE && ks == KIND_NODE) db++;
@@ -344,10 +367,13 @@ __global__ void k_count(const double2 *__restrict__ x, const double *__restrict_
         if (r < nn) degB[r] = db;
     }
     // per-type histogram: wave reduction then one atomic per wave
-    for (int k = 0; k < 10; ++k) {
-        unsigned v = tc[k];
+    unsigned tv[6] = {t0, t1, t2, t3, t6, t9};
+    const int tid[6] = {0, 1, 2, 3, 6, 9};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        unsigned v = tv[k];
         for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-        if ((threadIdx.x & 63) == 0 && v) atomicAdd(&split->n_by_type[k], (unsigned long long)v);
+        if ((threadIdx.x & 63) == 0 && v) atomicAdd(&split->n_by_type[tid[k]], (unsigned long long)v);
     }
 }
 
@@ -400,8 +426,8 @@ __global__ void k_split_partner(const double2 *__restrict__ x, const double *__r
     double2 xr = x[r];
     Stencil S = make_stencil(c, g->ndivx, g->ndivy);
     unsigned long long local = 0;
-    for (int row = 0; row < S.nrows; ++row)
-        for (int cc = S.lo[row]; cc <= S.hi[row]; ++cc) {
+    for (int y = S.y0; y <= S.y1; ++y)
+        for (int cc = S.lo(y); cc <= S.hi(y); ++cc) {
             if (cc < c) continue;
             for (int ks = 0; ks < 3; ++ks)
                 for (int s = kstart[ks * L + cc]; s < kstart[ks * L + cc + 1]; ++s) {
@@ -474,11 +500,13 @@ __global__ void k_build(const double2 *__restrict__ x, const double *__restrict_
         const bool rev = pass == 0;
         if (rev && !any_rev) continue;
         if (!rev && !any_fwd) continue;
-        for (int ri = 0; ri < S.nrows; ++ri) {
-            int row = rev ? S.nrows - 1 - ri : ri;
-            int ncell = S.hi[row] - S.lo[row] + 1;
+        const int nrow = S.y1 - S.y0 + 1;
+        for (int ri = 0; ri < nrow; ++ri) {
+            const int y = rev ? S.y1 - ri : S.y0 + ri;
+            const int rlo = S.lo(y), rhi = S.hi(y);
+            const int ncell = rhi - rlo + 1;
             for (int ci = 0; ci < ncell; ++ci) {
-                int cc = rev ? S.hi[row] - ci : S.lo[row] + ci;
+                int cc = rev ? rhi - ci : rlo + ci;
                 for (int ki = 0; ki < 3; ++ki) {
                     int ks = rev ? 2 - ki : ki;
                     bool la = in_list_a(kr, ks), lb = kr == KIND_NODE && ks == KIND_NODE;
@@ -734,6 +762,44 @@ __global__ void k_deriv_sps(StateBufs S, Work W, const int *__restrict__ degA, L
 // (2_...:763-764, 769).
 constexpr int MAX_WALL_TERMS = 16;
 
+// boundary_forces for one node: the nonzero terms of its inner-layer dummies
+// in ascending dummy index (the reference's j loop), found by repeated
+// minimum selection over the node's dummy partners (a handful near walls).
+__device__ __forceinline__ double2 wall_force(const StateBufs &S, const Lists &Lz, unsigned e0, int deg,
+                                              double2 xr, double hr, const DevParams &P,
+                                              int *__restrict__ err) {
+    double2 fb = make_double2(0.0, 0.0);
+    int last = -1;
+    for (int it = 0; it <= MAX_WALL_TERMS; ++it) {
+        int best = 0x7fffffff;
+        float b0 = 0.0f, b1 = 0.0f;
+        unsigned e = e0;
+        for (int k = 0; k < deg; ++k, e += SLICE) {
+            int s = Lz.a_s[e];
+            if (s < P.nt) continue;
+            int o = S.orig[s];
+            if (o >= P.nt + P.nd2 || o <= last || o >= best) continue;
+            double2 xs = S.x[s];
+            float t0, t1;
+            wall_force_term(xr.x - xs.x, xr.y - xs.y, hr, S.hsml[s], P.dx, t0, t1);
+            if (t0 != 0.0f || t1 != 0.0f || t0 != t0 || t1 != t1) {
+                best = o;
+                b0 = t0;
+                b1 = t1;
+            }
+        }
+        if (best == 0x7fffffff) return fb;
+        if (it == MAX_WALL_TERMS) {
+            atomicOr(err, 1);
+            return fb;
+        }
+        fb.x = fb.x + (double)b0;
+        fb.y = fb.y + (double)b1;
+        last = best;
+    }
+    return fb;
+}
+
 __global__ void k_deriv_nodes(StateBufs S, Work W, const int *__restrict__ degA,
                               const int *__restrict__ degB, Lists Lz, DevParams P, double gx_grav,
                               double gy_grav, double f2, int *__restrict__ err, int nn, int nt) {
@@ -741,13 +807,11 @@ __global__ void k_deriv_nodes(StateBufs S, Work W, const int *__restrict__ degA,
     if (r >= nn) return;
     double2 xr = S.x[r], vr = S.vel[r];
     double4 sr = S.stress[r];
-    double mr = S.mass[r], rhor = S.rho[r], hr = S.hsml[r];
+    double rhor = S.rho[r], hr = S.hsml[r];
     double rr2 = rhor * rhor;
     double q00 = 0, q01 = 0, q10 = 0, q11 = 0, q20 = 0, q21 = 0;  // grad2_tmp(s,b)
     double A0 = 0, A1 = 0, A2 = 0, A3 = 0;
-    int nwall = 0;
-    int w_orig[MAX_WALL_TERMS];
-    float w_t0[MAX_WALL_TERMS], w_t1[MAX_WALL_TERMS];
+    int ndum = 0;
     unsigned e = Lz.a_off[r >> 6] + (r & 63);
     int deg = degA[r];
     for (int k = 0; k < deg; ++k, e += SLICE) {
@@ -781,24 +845,7 @@ __global__ void k_deriv_nodes(StateBufs S, Work W, const int *__restrict__ degA,
             a = gy * (sr.y / rr2 + sr.y / rs2); q11 = q11 - ms * a;
             a = gx * (sr.z / rr2 + sr.z / rs2); q20 = q20 - ms * a;
             a = gy * (sr.z / rr2 + sr.z / rs2); q21 = q21 - ms * a;
-            if (P.do_bforce) {
-                int o = S.orig[s];
-                if (o < P.nt + P.nd2) {               // inner wall layer
-                    double2 xs = S.x[s];
-                    float t0, t1;
-                    wall_force_term(xr.x - xs.x, xr.y - xs.y, hr, S.hsml[s], P.dx, t0, t1);
-                    if (t0 != 0.0f || t1 != 0.0f || t0 != t0 || t1 != t1) {
-                        if (nwall < MAX_WALL_TERMS) {
-                            w_orig[nwall] = o;
-                            w_t0[nwall] = t0;
-                            w_t1[nwall] = t1;
-                            nwall++;
-                        } else {
-                            atomicOr(err, 1);
-                        }
-                    }
-                }
-            }
+            ndum++;
         }
     }
     if (P.cspm) {
@@ -821,25 +868,8 @@ __global__ void k_deriv_nodes(StateBufs S, Work W, const int *__restrict__ degA,
     // boundary_forces: nonzero terms in ascending dummy index
     double2 fb = W.f_bound[r];
     if (P.do_bforce) {
-        for (int a = 1; a < nwall; ++a) {
-            int ko = w_orig[a];
-            float k0 = w_t0[a], k1 = w_t1[a];
-            int b = a - 1;
-            while (b >= 0 && w_orig[b] > ko) {
-                w_orig[b + 1] = w_orig[b];
-                w_t0[b + 1] = w_t0[b];
-                w_t1[b + 1] = w_t1[b];
-                --b;
-            }
-            w_orig[b + 1] = ko;
-            w_t0[b + 1] = k0;
-            w_t1[b + 1] = k1;
-        }
-        fb = make_double2(0.0, 0.0);
-        for (int a = 0; a < nwall; ++a) {
-            fb.x = fb.x + (double)w_t0[a];
-            fb.y = fb.y + (double)w_t1[a];
-        }
+        fb = ndum ? wall_force(S, Lz, Lz.a_off[r >> 6] + (r & 63), degA[r], xr, hr, P, err)
+                  : make_double2(0.0, 0.0);
         W.f_bound[r] = fb;
     }
     // artificial_viscosity over node-node pairs, single-precision sums
@@ -880,7 +910,6 @@ __global__ void k_deriv_nodes(StateBufs S, Work W, const int *__restrict__ degA,
     W.rhs2[r] = rh;
     double2 rk = W.rk_vel[r];
     W.rk_vel[r] = make_double2(rk.x + f2 * rh.x, rk.y + f2 * rh.y);
-    (void)mr;
 }
 
 // The type-6 loop of get_derivatives overwrites each dummy's stress with
@@ -965,8 +994,8 @@ __global__ void k_dump_pairs(const double2 *__restrict__ x, const double2 *__res
     double2 xr = r < nt ? x0[r] : x[r];
     Stencil S = make_stencil(c, g->ndivx, g->ndivy);
     unsigned d = off[proc[r]];
-    for (int row = 0; row < S.nrows; ++row)
-        for (int cc = S.lo[row]; cc <= S.hi[row]; ++cc) {
+    for (int y = S.y0; y <= S.y1; ++y)
+        for (int cc = S.lo(y); cc <= S.hi(y); ++cc) {
             if (cc < c) continue;
             for (int ks = 0; ks < 3; ++ks)
                 for (int s = kstart[ks * L + cc]; s < kstart[ks * L + cc + 1]; ++s) {
