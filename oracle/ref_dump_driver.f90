@@ -14,6 +14,12 @@
 ! usage:  sps_ref_dump OUTDIR NSTEPS [STEP ...]
 !   dumps state_000000.bin before the first step, and state_NNNNNN.bin plus
 !   pairs_NNNNNN.bin after every listed step.
+!
+! Built with -DSPS_DROPIN the same driver calls the drop-in glue
+! (stress-particle-sph_amd/fortran/sps_glue.f90) instead of the reference's
+! time_integration: the reference's readers set the problem up, the library
+! behind include/sps.h advances it, and the state dumps can be compared with
+! the reference's own (pair lists are not dumped in that variant).
 !-------------------------------------------------------------------------------
 program sps_ref_dump
   use variable_types
@@ -21,6 +27,9 @@ program sps_ref_dump
   use SPH_global_vars_2018
   use SPH_material_vars_2018, only: cgrav, ft_grav, dampingTG
   use SPH_main_2018, only: Init_SPH, time_integration
+#ifdef SPS_DROPIN
+  use sps_glue, only: sps_time_integration, sps_glue_finish
+#endif
   implicit none
 
   character(len=512) :: outdir, arg
@@ -63,14 +72,23 @@ program sps_ref_dump
   do istep = 1, nsteps
      current_ts = current_ts + 1
      itimestep_sph = itimestep_sph + 1
+#ifdef SPS_DROPIN
+     call sps_time_integration
+#else
      call time_integration
+#endif
      time_sph = time_sph + dt_sph
      time = time + dt
      if (any(dump_steps(1:max(ndump,1)) == istep)) then
         call dump_state(istep)
+#ifndef SPS_DROPIN
         call dump_pairs(istep)
+#endif
      end if
   end do
+#ifdef SPS_DROPIN
+  call sps_glue_finish
+#endif
 
 contains
 

@@ -44,6 +44,7 @@ static int fail(int code, const char *fmt, ...) {
 
 static inline unsigned nblk(long long n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
 
+
 template <typename T>
 struct DBuf {
     T *p = nullptr;
@@ -116,7 +117,8 @@ struct sps_handle {
     // step work arrays
     DBuf<double2> vx0, x0, vel0, rk_vel, rhs2, f_bound, art_visc;
     DBuf<double4> stress0, rk_stress, rhs1, grad_u;
-    DBuf<double> rk_dev, ddev;
+    DBuf<double> rk_dev, ddev, mrho;
+    DBuf<double4> sr2;
     // search
     int cell_cap = 0;
     DBuf<Grid> grid;
@@ -318,7 +320,7 @@ extern "C" int sps_create(const sps_params *params, sps_handle **out) {
     rc |= dalloc(h->rk_vel, nn); rc |= dalloc(h->rhs2, nn); rc |= dalloc(h->f_bound, nn);
     rc |= dalloc(h->art_visc, nn); rc |= dalloc(h->stress0, nt); rc |= dalloc(h->rk_stress, nt);
     rc |= dalloc(h->rhs1, nt); rc |= dalloc(h->grad_u, nt); rc |= dalloc(h->rk_dev, nt);
-    rc |= dalloc(h->ddev, nt);
+    rc |= dalloc(h->ddev, nt); rc |= dalloc(h->mrho, nt2); rc |= dalloc(h->sr2, nt);
     rc |= dalloc(h->grid, 1); rc |= dalloc(h->split, 1); rc |= dalloc(h->err, 1);
     rc |= dalloc(h->totals, 2);
     h->nbparts = std::min(1024, (int)nblk(nt2));
@@ -515,7 +517,8 @@ static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
     const int nn = h->nn, nt = h->nt, nt2 = h->nt2;
     StateBufs SB = h->S[h->cur].view();
     Work W{h->vx0.p, h->x0.p, h->vel0.p, h->rk_vel.p, h->rhs2.p, h->f_bound.p, h->art_visc.p,
-           h->stress0.p, h->rk_stress.p, h->rhs1.p, h->grad_u.p, h->rk_dev.p, h->ddev.p};
+           h->stress0.p, h->rk_stress.p, h->rhs1.p, h->grad_u.p, h->rk_dev.p, h->ddev.p,
+           h->mrho.p, h->sr2.p};
     Lists Lz{h->a_s.p, h->a_w.p, h->a_gx.p, h->a_gy.p, h->offA.p, h->b_s.p, h->b_gx.p, h->b_gy.p, h->offB.p};
     const int adapt = h->dp.ncrit == 12;
     const double tanfi = h->dp.tanfi, coh = h->dp.coh;
@@ -527,8 +530,8 @@ static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
     for (int st = 0; st < 4; ++st) {
         const double c1 = f1rk[st] * (dt);
         LAUNCH(K_STAGE_BEGIN, nblk(nt), BLOCK, k_stage_begin, SB, W, c1, adapt, tanfi, coh, nn, nt);
-        LAUNCH(K_SPU_NODES, nblk(nn), BLOCK, k_spu_nodes, SB, h->degA.p, Lz, adapt, tanfi, coh, nn, nt);
-        LAUNCH(K_SPU_SPS, nblk(nt - nn), BLOCK, k_spu_sps, SB, h->degA.p, Lz, adapt, tanfi, coh, nn, nt);
+        LAUNCH(K_SPU_NODES, nblk(nn), BLOCK, k_spu_nodes, SB, W, h->degA.p, Lz, adapt, tanfi, coh, nn, nt);
+        LAUNCH(K_SPU_SPS, nblk(nt - nn), BLOCK, k_spu_sps, SB, W, h->degA.p, Lz, adapt, tanfi, coh, nn, nt);
         LAUNCH(K_DERIV_SPS, nblk(nt - nn), BLOCK, k_deriv_sps, SB, W, h->degA.p, Lz, h->dp, time_sph, f2rk[st], nn, nt);
         LAUNCH(K_DERIV_NODES, nblk(nn), BLOCK, k_deriv_nodes, SB, W, h->degA.p, h->degB.p, Lz, h->dp, gxg, gyg, f2rk[st],
                                                  h->err.p, nn, nt);
@@ -536,8 +539,8 @@ static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
             LAUNCH(K_DUMMY_STRESS, nblk(nt2 - nt), BLOCK, k_dummy_stress, SB, h->degA.p, Lz, nt, nt2);
     }
     LAUNCH(K_RK_FINAL, nblk(nt), BLOCK, k_rk_final, SB, W, dt, adapt, tanfi, coh, nn, nt);
-    LAUNCH(K_SPU_NODES, nblk(nn), BLOCK, k_spu_nodes, SB, h->degA.p, Lz, adapt, tanfi, coh, nn, nt);
-    LAUNCH(K_SPU_SPS, nblk(nt - nn), BLOCK, k_spu_sps, SB, h->degA.p, Lz, adapt, tanfi, coh, nn, nt);
+    LAUNCH(K_SPU_NODES, nblk(nn), BLOCK, k_spu_nodes, SB, W, h->degA.p, Lz, adapt, tanfi, coh, nn, nt);
+    LAUNCH(K_SPU_SPS, nblk(nt - nn), BLOCK, k_spu_sps, SB, W, h->degA.p, Lz, adapt, tanfi, coh, nn, nt);
     LAUNCH(K_POSITION, nblk(nt), BLOCK, k_position, SB, W, dt, h->p.update_x, nn, nt);
     HIP_TRY(hipGetLastError());
     return 0;
@@ -727,7 +730,7 @@ extern "C" void sps_destroy(sps_handle *h) {
     h->S[1].release();
     dfree(h->vx0); dfree(h->x0); dfree(h->vel0); dfree(h->rk_vel); dfree(h->rhs2); dfree(h->f_bound);
     dfree(h->art_visc); dfree(h->stress0); dfree(h->rk_stress); dfree(h->rhs1); dfree(h->grad_u);
-    dfree(h->rk_dev); dfree(h->ddev); dfree(h->grid); dfree(h->split); dfree(h->bparts);
+    dfree(h->rk_dev); dfree(h->ddev); dfree(h->mrho); dfree(h->sr2); dfree(h->grid); dfree(h->split); dfree(h->bparts);
     dfree(h->cellkey); dfree(h->new_cell); dfree(h->cell); dfree(h->kcount); dfree(h->kstart);
     dfree(h->kfill); dfree(h->tmp_orig); dfree(h->tmp_src); dfree(h->proc); dfree(h->proc_to_slot);
     dfree(h->cnt); dfree(h->off); dfree(h->scan_tmp); dfree(h->widthA); dfree(h->widthB);

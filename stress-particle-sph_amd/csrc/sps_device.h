@@ -66,13 +66,19 @@ __device__ __forceinline__ void cubic_kernel(double r, double dx0, double dx1, d
 
 // Pair acceptance of grid_find_NEW (2_...:1345-1353): r < 2 * (h_i + h_j)/2.
 // Returns r (>= 0) when accepted, -1 otherwise.  Symmetric in i, j.
+// A candidate whose squared distance exceeds (2*mhsml)^2 by a relative
+// 1e-12 is rejected without the square root: then sqrt(driac) > 2*mhsml
+// exactly, and its correctly rounded value cannot fall below the
+// representable 2*mhsml.
 __device__ __forceinline__ double pair_distance(double dx0, double dx1, double hi, double hj,
                                                 double &mhsml) {
     double driac = dx0 * dx0;
     driac = driac + dx1 * dx1;
     mhsml = (hi + hj) / 2.0;
+    double lim = 2 * mhsml;
+    if (driac > (lim * lim) * (1.0 + 1e-12)) return -1.0;
     double r = sqrt(driac);
-    return (r < 2 * mhsml) ? r : -1.0;
+    return (r < lim) ? r : -1.0;
 }
 
 // adapt_stress2 for one particle (code/3_SPH_material_2018.f90:2066-2140).

@@ -550,6 +550,8 @@ struct Work {
     double2 *vx0, *x0, *vel0, *rk_vel, *rhs2, *f_bound, *art_visc;
     double4 *stress0, *rk_stress, *rhs1, *grad_u;
     double *rk_dev, *ddev;
+    double *mrho;      // m / rho per particle (mass and density are fixed on the path)
+    double4 *sr2;      // sigma / rho^2 per stress point, refreshed by k_spu_sps
 };
 
 // Start of a step: vx0, x0 (2_...:114) and the RK4 prologue (2_...:681-690).
@@ -572,6 +574,7 @@ __global__ void k_rk_init(StateBufs S, Work W, int nn, int nt, int nt2) {
     }
     S.vel[i] = make_double2(0, 0);
     S.stress[i] = make_double4(0, 0, 0, 0);
+    W.mrho[i] = S.mass[i] / S.rho[i];
 }
 
 // Stage state (2_...:700-701) and the first adapt_stress2 of the stage.
@@ -597,7 +600,7 @@ __global__ void k_stage_begin(StateBufs S, Work W, double c1, int adapt, double 
 
 // stress_point_update (2_...:403-482) for node receivers: CSPM-normalised
 // stress and plastic strain from the stress points, then adapt_stress2.
-__global__ void k_spu_nodes(StateBufs S, const int *__restrict__ degA, Lists Lz, int adapt,
+__global__ void k_spu_nodes(StateBufs S, Work W, const int *__restrict__ degA, Lists Lz, int adapt,
                             double tanfi, double coh, int nn, int nt) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nn) return;
@@ -609,7 +612,7 @@ __global__ void k_spu_nodes(StateBufs S, const int *__restrict__ degA, Lists Lz,
         if (s >= nt) continue;                      // type 6 entry
         double w = (double)Lz.a_w[e];
         double ms = S.mass[s], rs = S.rho[s];
-        double h1 = (ms / rs) * w;
+        double h1 = W.mrho[s] * w;                  // (m/rho)*w, m/rho hoisted per particle
         double4 st = S.stress[s];
         t0 = t0 + st.x * h1;
         t1 = t1 + st.y * h1;
@@ -631,7 +634,7 @@ __global__ void k_spu_nodes(StateBufs S, const int *__restrict__ degA, Lists Lz,
 
 // stress_point_update for stress-point receivers: CSPM-normalised velocity
 // from the nodes, then adapt_stress2 of the stress point.
-__global__ void k_spu_sps(StateBufs S, const int *__restrict__ degA, Lists Lz, int adapt,
+__global__ void k_spu_sps(StateBufs S, Work W, const int *__restrict__ degA, Lists Lz, int adapt,
                           double tanfi, double coh, int nn, int nt) {
     int r = nn + blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nt) return;
@@ -643,18 +646,23 @@ __global__ void k_spu_sps(StateBufs S, const int *__restrict__ degA, Lists Lz, i
         if (s >= nn) continue;                      // type 9 entry
         double w = (double)Lz.a_w[e];
         double ms = S.mass[s], rs = S.rho[s];
-        double h2 = (ms / rs) * w;
+        double h2 = W.mrho[s] * w;
         double2 vs = S.vel[s];
         v0 = v0 + vs.x * h2;
         v1 = v1 + vs.y * h2;
         norm = norm + (w * ms) / rs;
     }
     if (norm != 0) S.vel[r] = make_double2(v0 / norm, v1 / norm);
+    double4 s = S.stress[r];
     if (adapt) {
-        double4 s = S.stress[r];
         adapt_stress(s, tanfi, coh);
         S.stress[r] = s;
     }
+    // sigma / rho^2 of this stress point, as get_derivatives divides it
+    // (2_...:530) for every node partner
+    double rr = S.rho[r];
+    double r2 = rr * rr;
+    W.sr2[r] = make_double4(s.x / r2, s.y / r2, s.z / r2, 0.0);
 }
 
 // get_derivatives (2_...:487-648) for stress-point receivers, fused with
@@ -809,6 +817,7 @@ __global__ void k_deriv_nodes(StateBufs S, Work W, const int *__restrict__ degA,
     double4 sr = S.stress[r];
     double rhor = S.rho[r], hr = S.hsml[r];
     double rr2 = rhor * rhor;
+    const double or0 = sr.x / rr2, or1 = sr.y / rr2, or2 = sr.z / rr2;   // own sigma / rho^2
     double q00 = 0, q01 = 0, q10 = 0, q11 = 0, q20 = 0, q21 = 0;  // grad2_tmp(s,b)
     double A0 = 0, A1 = 0, A2 = 0, A3 = 0;
     int ndum = 0;
@@ -820,14 +829,14 @@ __global__ void k_deriv_nodes(StateBufs S, Work W, const int *__restrict__ degA,
         double gx = (double)(-Lz.a_gx[e]), gy = (double)(-Lz.a_gy[e]);   // pair_i is the partner
         double rs2 = rs * rs;
         if (s < nt) {                                // type 1
-            double4 ss = S.stress[s];
+            double4 ss = W.sr2[s];                   // sigma_s / rho_s^2
             double a;
-            a = gx * (sr.x / rr2 + ss.x / rs2); q00 = q00 - ms * a;
-            a = gy * (sr.x / rr2 + ss.x / rs2); q01 = q01 - ms * a;
-            a = gx * (sr.y / rr2 + ss.y / rs2); q10 = q10 - ms * a;
-            a = gy * (sr.y / rr2 + ss.y / rs2); q11 = q11 - ms * a;
-            a = gx * (sr.z / rr2 + ss.z / rs2); q20 = q20 - ms * a;
-            a = gy * (sr.z / rr2 + ss.z / rs2); q21 = q21 - ms * a;
+            a = gx * (or0 + ss.x); q00 = q00 - ms * a;
+            a = gy * (or0 + ss.x); q01 = q01 - ms * a;
+            a = gx * (or1 + ss.y); q10 = q10 - ms * a;
+            a = gy * (or1 + ss.y); q11 = q11 - ms * a;
+            a = gx * (or2 + ss.z); q20 = q20 - ms * a;
+            a = gy * (or2 + ss.z); q21 = q21 - ms * a;
             if (P.cspm) {
                 double h1b = -(gx * ms / rs);
                 double h2b = -(gy * ms / rs);
@@ -838,13 +847,14 @@ __global__ void k_deriv_nodes(StateBufs S, Work W, const int *__restrict__ degA,
                 A3 = A3 + (xs.y - xr.y) * h2b;
             }
         } else {                                     // type 6: dummy stress := node stress
+            double d0 = sr.x / rs2, d1 = sr.y / rs2, d2 = sr.z / rs2;
             double a;
-            a = gx * (sr.x / rr2 + sr.x / rs2); q00 = q00 - ms * a;
-            a = gy * (sr.x / rr2 + sr.x / rs2); q01 = q01 - ms * a;
-            a = gx * (sr.y / rr2 + sr.y / rs2); q10 = q10 - ms * a;
-            a = gy * (sr.y / rr2 + sr.y / rs2); q11 = q11 - ms * a;
-            a = gx * (sr.z / rr2 + sr.z / rs2); q20 = q20 - ms * a;
-            a = gy * (sr.z / rr2 + sr.z / rs2); q21 = q21 - ms * a;
+            a = gx * (or0 + d0); q00 = q00 - ms * a;
+            a = gy * (or0 + d0); q01 = q01 - ms * a;
+            a = gx * (or1 + d1); q10 = q10 - ms * a;
+            a = gy * (or1 + d1); q11 = q11 - ms * a;
+            a = gx * (or2 + d2); q20 = q20 - ms * a;
+            a = gy * (or2 + d2); q21 = q21 - ms * a;
             ndum++;
         }
     }

@@ -1,0 +1,143 @@
+!-------------------------------------------------------------------------------
+! sps_glue - drop-in replacement of the reference's time_integration
+!
+! A maintainer adds this file to the reference's build (after its modules)
+! and replaces `call time_integration` in the driver
+! (code/1_SPH_2018.f90:82) by `call sps_time_integration`.  Everything else
+! of the reference stays: its driver loop, its input.txt/.dat/.pts readers
+! (Init_sph, code/2_SPH_main_2018.f90:32-46) and its CSV/GiD writers.
+!
+! On its first call the glue packs the problem constants and the state that
+! the readers left in the reference's module globals
+! (code/5_SPH_global_vars_2018.f90:27-182) into the C-ABI (include/sps.h),
+! creates a handle and uploads the state.  Every call then advances one
+! step on the device and downloads the state back into the same globals, so
+! the writers and the driver see what time_integration would have left.
+! With sps_glue_sync_every_step set to .false. the download is skipped and a
+! driver that needs the state only at output times calls sps_glue_sync.
+!-------------------------------------------------------------------------------
+module sps_glue
+  use, intrinsic :: iso_c_binding
+  use sps_binding
+  use variable_types
+  use SPH_time_vars_2018
+  use SPH_global_vars_2018
+  use SPH_material_vars_2018, only: ic_grav, tcurve_grav, cgrav, ft_grav, dampingTG
+  implicit none
+  private
+  public :: sps_time_integration, sps_glue_sync, sps_glue_finish
+
+  type(c_ptr), save :: handle = c_null_ptr
+  ! fork of the reference the host was built from: code/ (0) or the Bui et
+  ! al. (2008) fork (1) -- wall-force gate and CSPM determinant threshold
+  integer, save, public :: sps_glue_fork = 0
+  logical, save, public :: sps_glue_sync_every_step = .true.
+
+contains
+
+  subroutine sps_time_integration
+    real(c_double) :: t
+    integer(c_int32_t) :: it
+    if (.not. c_associated(handle)) call open_handle()
+    ! the driver has already advanced itimestep_sph (1_SPH_2018.f90:80) and
+    ! advances time_sph after the call (:83): step from the previous counter
+    t = time_sph
+    it = int(itimestep_sph - 1, c_int32_t)
+    call sps_check(sps_step(handle, 1_c_int32_t, real(dt_sph, c_double), t, it), 'sps_step')
+    if (sps_glue_sync_every_step) call sps_glue_sync()
+  end subroutine sps_time_integration
+
+  subroutine sps_glue_sync
+    if (.not. c_associated(handle)) return
+    call get_arrays(x, vel, stress, internal_vars, f_drucker, displ, If_Out_Domain, grad_u, &
+                    Ddev_strn, f_bound, art_visc)
+  end subroutine sps_glue_sync
+
+  subroutine sps_glue_finish
+    if (c_associated(handle)) call sps_destroy(handle)
+    handle = c_null_ptr
+  end subroutine sps_glue_finish
+
+  subroutine open_handle
+    type(sps_params_t) :: p
+    integer :: k, n
+    real, save :: no_walls(1) = 0.0
+    p%abi_version = SPS_ABI_VERSION
+    p%nnode = nnode; p%nstress = nstress
+    p%ndummy = merge(ndummy, 0, dummy_nodes)
+    p%ndummy2 = merge(ndummy2, 0, dummy_nodes)
+    p%nint_vars = nint_Vars
+    p%ndimn = ndimn; p%nstre = nstre; p%skf = skf
+    p%sp_sph = merge(1, 0, SP_SPH); p%inside_approach = merge(1, 0, inside_approach)
+    p%cspm = merge(1, 0, cspm); p%update_x = merge(1, 0, update_x); p%xsph = merge(1, 0, XSPH)
+    p%cont_density = merge(1, 0, cont_density); p%art_stress = merge(1, 0, art_stress)
+    p%dummy_nodes = merge(1, 0, dummy_nodes); p%sph_shift = merge(1, 0, SPH_shift)
+    p%no_bcs = no_bcs
+    p%bforce_gate = sps_glue_fork
+    p%mode = SPS_MODE_ORDERED
+    p%props = 0.0_c_double
+    p%props(1:20) = props(1, 1:20)
+    p%pi = pi; p%dx = dx; p%alpha = alpha; p%beta = beta; p%damping = dampingTG
+    if (sps_glue_fork == 1) then
+       p%cspm_det_tol = real(1e-07, c_double)
+    else
+       p%cspm_det_tol = real(1e-03, c_double)
+    end if
+    p%D11 = D11; p%D22 = D22; p%D12 = D12; p%D33 = D33; p%D41 = D41; p%D42 = D42
+    p%xmin_domain = Xmin_Domain(1:2); p%xmax_domain = Xmax_Domain(1:2)
+    p%ic_grav = ic_grav
+    p%cgrav = 0.0_c_double; p%ft_grav = 0.0_c_double; p%tcurve_npts = 0
+    p%tcurve_t = 0.0_c_double; p%tcurve_f = 0.0_c_float
+    if (ic_grav == 1) then
+       p%cgrav = cgrav(1:2); p%ft_grav = ft_grav
+       n = nptstcurves(tcurve_grav)
+       p%tcurve_npts = n
+       do k = 1, min(n, SPS_MAX_TCURVE)
+          p%tcurve_t(k) = ttcurves(tcurve_grav, k)
+          p%tcurve_f(k) = ftcurves(tcurve_grav, k)
+       end do
+    end if
+    call sps_check(sps_create(p, handle), 'sps_create')
+    if (dummy_nodes) then
+       call set_arrays(x, vel, stress, mass, rho, hsml, itype, If_Out_Domain, internal_vars, &
+                       f_drucker, x00, displ, wall_position, horizontal_or_not, .true.)
+    else
+       call set_arrays(x, vel, stress, mass, rho, hsml, itype, If_Out_Domain, internal_vars, &
+                       f_drucker, x00, displ, no_walls, no_walls, .false.)
+    end if
+  end subroutine open_handle
+
+  ! The module arrays arrive as assumed-size TARGET dummies, so their
+  ! addresses are valid for the duration of the C call made here.
+  subroutine set_arrays(x_, vel_, stress_, mass_, rho_, hsml_, itype_, ood_, ivars_, fdr_, x00_, &
+                        displ_, wall_, horiz_, with_walls)
+    real(DP), target, intent(in) :: x_(*), vel_(*), stress_(*), mass_(*), rho_(*), hsml_(*)
+    real(DP), target, intent(in) :: ivars_(*), fdr_(*), x00_(*), displ_(*)
+    integer, target, intent(in) :: itype_(*), ood_(*)
+    real, target, intent(in) :: wall_(*), horiz_(*)
+    logical, intent(in) :: with_walls
+    type(sps_state_t) :: st
+    st%x = c_loc(x_); st%vel = c_loc(vel_); st%stress = c_loc(stress_)
+    st%mass = c_loc(mass_); st%rho = c_loc(rho_); st%hsml = c_loc(hsml_)
+    st%itype = c_loc(itype_); st%if_out_domain = c_loc(ood_)
+    st%internal_vars = c_loc(ivars_); st%f_drucker = c_loc(fdr_)
+    st%x00 = c_loc(x00_); st%displ = c_loc(displ_)
+    if (with_walls) then
+       st%wall_position = c_loc(wall_); st%horizontal_or_not = c_loc(horiz_)
+    end if
+    call sps_check(sps_set_state(handle, st), 'sps_set_state')
+  end subroutine set_arrays
+
+  subroutine get_arrays(x_, vel_, stress_, ivars_, fdr_, displ_, ood_, gradu_, ddev_, fb_, av_)
+    real(DP), target, intent(inout) :: x_(*), vel_(*), stress_(*), ivars_(*), fdr_(*), displ_(*)
+    real(DP), target, intent(inout) :: gradu_(*), ddev_(*), fb_(*), av_(*)
+    integer, target, intent(inout) :: ood_(*)
+    type(sps_state_t) :: st
+    st%x = c_loc(x_); st%vel = c_loc(vel_); st%stress = c_loc(stress_)
+    st%internal_vars = c_loc(ivars_); st%f_drucker = c_loc(fdr_); st%displ = c_loc(displ_)
+    st%if_out_domain = c_loc(ood_); st%grad_u = c_loc(gradu_); st%ddev_strn = c_loc(ddev_)
+    st%f_bound = c_loc(fb_); st%art_visc = c_loc(av_)
+    call sps_check(sps_get_state(handle, st), 'sps_get_state')
+  end subroutine get_arrays
+
+end module sps_glue
