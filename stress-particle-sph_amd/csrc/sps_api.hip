@@ -43,6 +43,11 @@ static int fail(int code, const char *fmt, ...) {
     } while (0)
 
 static inline unsigned nblk(long long n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
+// grid of the kernels that place their tiles with xcd_block(): a multiple of 8
+static inline unsigned nblk8(long long n) {
+    unsigned b = nblk(n);
+    return (b + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
+}
 
 
 template <typename T>
@@ -117,8 +122,7 @@ struct sps_handle {
     // step work arrays
     DBuf<double2> vx0, x0, vel0, rk_vel, rhs2, f_bound, art_visc;
     DBuf<double4> stress0, rk_stress, rhs1, grad_u;
-    DBuf<double> rk_dev, ddev, mrho;
-    DBuf<double4> sr2;
+    DBuf<double> rk_dev, ddev;
     // search
     int cell_cap = 0;
     DBuf<Grid> grid;
@@ -126,7 +130,7 @@ struct sps_handle {
     DBuf<double> bparts;
     int nbparts = 0;
     DBuf<int> cellkey, new_cell, cell, kcount, kstart, kfill, tmp_orig, tmp_src, proc, proc_to_slot;
-    DBuf<unsigned> cnt, off, scan_tmp, widthA, widthB, offA, offB;
+    DBuf<unsigned> cnt, off, scan_tmp, widthA, widthB, offA, offB, ctot, cstart_rm;
     DBuf<int> degA, degB;
     DBuf<unsigned long long> totals;   // [0] list A size, [1] list B size
     DBuf<int> err;
@@ -320,7 +324,7 @@ extern "C" int sps_create(const sps_params *params, sps_handle **out) {
     rc |= dalloc(h->rk_vel, nn); rc |= dalloc(h->rhs2, nn); rc |= dalloc(h->f_bound, nn);
     rc |= dalloc(h->art_visc, nn); rc |= dalloc(h->stress0, nt); rc |= dalloc(h->rk_stress, nt);
     rc |= dalloc(h->rhs1, nt); rc |= dalloc(h->grad_u, nt); rc |= dalloc(h->rk_dev, nt);
-    rc |= dalloc(h->ddev, nt); rc |= dalloc(h->mrho, nt2); rc |= dalloc(h->sr2, nt);
+    rc |= dalloc(h->ddev, nt);
     rc |= dalloc(h->grid, 1); rc |= dalloc(h->split, 1); rc |= dalloc(h->err, 1);
     rc |= dalloc(h->totals, 2);
     h->nbparts = std::min(1024, (int)nblk(nt2));
@@ -360,6 +364,7 @@ static int size_cells(sps_handle *h, const double *hsml) {
     int rc = 0;
     size_t L = 3 * ((size_t)cap + 1) + 1;
     rc |= dalloc(h->kcount, L); rc |= dalloc(h->kstart, L); rc |= dalloc(h->kfill, L);
+    rc |= dalloc(h->ctot, (size_t)cap + 2); rc |= dalloc(h->cstart_rm, (size_t)cap + 2);
     if (rc) return rc;
     h->cell_cap = cap;
     return 0;
@@ -458,13 +463,17 @@ static int search(sps_handle *h, bool fresh_list) {
     HIP_TRY(hipGetLastError());
     h->cur = 1 - h->cur;
     StateSet &C = h->S[h->cur];
-    LAUNCH(K_PROC, nblk(nt2), BLOCK, k_proc_slot, h->cell.p, h->kstart.p, h->grid.p, h->proc.p, nn, nt, nt2);
+    LAUNCH(K_PROC, nblk(ndivt + 1), BLOCK, k_cell_totals, h->kstart.p, h->grid.p, h->ctot.p);
+    rc = scan_u32(h, h->ctot.p, h->cstart_rm.p, ndivt + 1, nullptr);
+    if (rc) return rc;
+    LAUNCH(K_PROC, nblk(nt2), BLOCK, k_proc_slot, h->cell.p, h->kstart.p, h->cstart_rm.p, h->grid.p, h->proc.p,
+           nn, nt, nt2);
     LAUNCH(K_PROC, nblk(nt2), BLOCK, k_proc_to_slot, h->proc.p, h->proc_to_slot.p, nt2);
     // counts, offsets, split
     HIP_TRY(hipMemsetAsync(h->cnt.p, 0, nt2 * sizeof(unsigned), s));
     const unsigned long long M = fresh_list ? 0ull : h->m_pairs;
     LAUNCH(K_SPLIT, 1, 64, k_split_reset, h->split.p, M);
-    LAUNCH(K_COUNT_PAIRS, nblk(nt2), BLOCK, k_count, C.x.p, C.hsml.p, h->cell.p, h->kstart.p, h->proc.p, h->grid.p,
+    LAUNCH(K_COUNT_PAIRS, nblk8(nt2), BLOCK, k_count, C.x.p, C.hsml.p, h->cell.p, h->kstart.p, h->proc.p, h->grid.p,
                                        h->cnt.p, h->degA.p, h->degB.p, h->split.p, nn, nt, nt2);
     HIP_TRY(hipGetLastError());
     // discovery offsets over all slots (slots past the in-domain count hold 0)
@@ -500,7 +509,7 @@ static int search(sps_handle *h, bool fresh_list) {
     }
     if (rc) return rc;
     Lists Lz{h->a_s.p, h->a_w.p, h->a_gx.p, h->a_gy.p, h->offA.p, h->b_s.p, h->b_gx.p, h->b_gy.p, h->offB.p};
-    LAUNCH(K_BUILD, nblk(nt2), BLOCK, k_build, C.x.p, C.hsml.p, h->cell.p, h->kstart.p, h->proc.p, h->cnt.p, h->off.p,
+    LAUNCH(K_BUILD, nblk8(nt2), BLOCK, k_build, C.x.p, C.hsml.p, h->cell.p, h->kstart.p, h->proc.p, h->cnt.p, h->off.p,
                                        h->grid.p, h->split.p, Lz, h->p.pi, nn, nt, nt2);
     HIP_TRY(hipGetLastError());
     h->n_pairs = sh.n;
@@ -517,8 +526,7 @@ static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
     const int nn = h->nn, nt = h->nt, nt2 = h->nt2;
     StateBufs SB = h->S[h->cur].view();
     Work W{h->vx0.p, h->x0.p, h->vel0.p, h->rk_vel.p, h->rhs2.p, h->f_bound.p, h->art_visc.p,
-           h->stress0.p, h->rk_stress.p, h->rhs1.p, h->grad_u.p, h->rk_dev.p, h->ddev.p,
-           h->mrho.p, h->sr2.p};
+           h->stress0.p, h->rk_stress.p, h->rhs1.p, h->grad_u.p, h->rk_dev.p, h->ddev.p};
     Lists Lz{h->a_s.p, h->a_w.p, h->a_gx.p, h->a_gy.p, h->offA.p, h->b_s.p, h->b_gx.p, h->b_gy.p, h->offB.p};
     const int adapt = h->dp.ncrit == 12;
     const double tanfi = h->dp.tanfi, coh = h->dp.coh;
@@ -530,17 +538,17 @@ static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
     for (int st = 0; st < 4; ++st) {
         const double c1 = f1rk[st] * (dt);
         LAUNCH(K_STAGE_BEGIN, nblk(nt), BLOCK, k_stage_begin, SB, W, c1, adapt, tanfi, coh, nn, nt);
-        LAUNCH(K_SPU_NODES, nblk(nn), BLOCK, k_spu_nodes, SB, W, h->degA.p, Lz, adapt, tanfi, coh, nn, nt);
-        LAUNCH(K_SPU_SPS, nblk(nt - nn), BLOCK, k_spu_sps, SB, W, h->degA.p, Lz, adapt, tanfi, coh, nn, nt);
-        LAUNCH(K_DERIV_SPS, nblk(nt - nn), BLOCK, k_deriv_sps, SB, W, h->degA.p, Lz, h->dp, time_sph, f2rk[st], nn, nt);
-        LAUNCH(K_DERIV_NODES, nblk(nn), BLOCK, k_deriv_nodes, SB, W, h->degA.p, h->degB.p, Lz, h->dp, gxg, gyg, f2rk[st],
+        LAUNCH(K_SPU_NODES, nblk8(nn), BLOCK, k_spu_nodes, SB, W, h->degA.p, Lz, adapt, tanfi, coh, nn, nt);
+        LAUNCH(K_SPU_SPS, nblk8(nt - nn), BLOCK, k_spu_sps, SB, W, h->degA.p, Lz, adapt, tanfi, coh, nn, nt);
+        LAUNCH(K_DERIV_SPS, nblk8(nt - nn), BLOCK, k_deriv_sps, SB, W, h->degA.p, Lz, h->dp, time_sph, f2rk[st], nn, nt);
+        LAUNCH(K_DERIV_NODES, nblk8(nn), BLOCK, k_deriv_nodes, SB, W, h->degA.p, h->degB.p, Lz, h->dp, gxg, gyg, f2rk[st],
                                                  h->err.p, nn, nt);
         if (st == 3 && nt2 > nt)
             LAUNCH(K_DUMMY_STRESS, nblk(nt2 - nt), BLOCK, k_dummy_stress, SB, h->degA.p, Lz, nt, nt2);
     }
     LAUNCH(K_RK_FINAL, nblk(nt), BLOCK, k_rk_final, SB, W, dt, adapt, tanfi, coh, nn, nt);
-    LAUNCH(K_SPU_NODES, nblk(nn), BLOCK, k_spu_nodes, SB, W, h->degA.p, Lz, adapt, tanfi, coh, nn, nt);
-    LAUNCH(K_SPU_SPS, nblk(nt - nn), BLOCK, k_spu_sps, SB, W, h->degA.p, Lz, adapt, tanfi, coh, nn, nt);
+    LAUNCH(K_SPU_NODES, nblk8(nn), BLOCK, k_spu_nodes, SB, W, h->degA.p, Lz, adapt, tanfi, coh, nn, nt);
+    LAUNCH(K_SPU_SPS, nblk8(nt - nn), BLOCK, k_spu_sps, SB, W, h->degA.p, Lz, adapt, tanfi, coh, nn, nt);
     LAUNCH(K_POSITION, nblk(nt), BLOCK, k_position, SB, W, dt, h->p.update_x, nn, nt);
     HIP_TRY(hipGetLastError());
     return 0;
@@ -730,11 +738,11 @@ extern "C" void sps_destroy(sps_handle *h) {
     h->S[1].release();
     dfree(h->vx0); dfree(h->x0); dfree(h->vel0); dfree(h->rk_vel); dfree(h->rhs2); dfree(h->f_bound);
     dfree(h->art_visc); dfree(h->stress0); dfree(h->rk_stress); dfree(h->rhs1); dfree(h->grad_u);
-    dfree(h->rk_dev); dfree(h->ddev); dfree(h->mrho); dfree(h->sr2); dfree(h->grid); dfree(h->split); dfree(h->bparts);
+    dfree(h->rk_dev); dfree(h->ddev); dfree(h->grid); dfree(h->split); dfree(h->bparts);
     dfree(h->cellkey); dfree(h->new_cell); dfree(h->cell); dfree(h->kcount); dfree(h->kstart);
     dfree(h->kfill); dfree(h->tmp_orig); dfree(h->tmp_src); dfree(h->proc); dfree(h->proc_to_slot);
     dfree(h->cnt); dfree(h->off); dfree(h->scan_tmp); dfree(h->widthA); dfree(h->widthB);
-    dfree(h->offA); dfree(h->offB); dfree(h->degA); dfree(h->degB); dfree(h->totals); dfree(h->err);
+    dfree(h->offA); dfree(h->offB); dfree(h->ctot); dfree(h->cstart_rm); dfree(h->degA); dfree(h->degB); dfree(h->totals); dfree(h->err);
     dfree(h->a_s); dfree(h->a_w); dfree(h->a_gx); dfree(h->a_gy); dfree(h->b_s); dfree(h->b_gx);
     dfree(h->b_gy);
     if (h->pinned) (void)hipHostFree(h->pinned);

@@ -81,6 +81,16 @@ __device__ __forceinline__ double pair_distance(double dx0, double dx1, double h
     return (r < lim) ? r : -1.0;
 }
 
+// Workgroups are dealt round-robin to the 8 XCDs.  With a grid that is a
+// multiple of 8 this gives XCD k one contiguous eighth of the tiles, so the
+// receivers it handles walk up the same cell strips and share partner data
+// in that XCD's L2.
+constexpr int NUM_XCD = 8;
+__device__ __forceinline__ int xcd_block() {
+    int per = gridDim.x / NUM_XCD;
+    return (blockIdx.x % NUM_XCD) * per + blockIdx.x / NUM_XCD;
+}
+
 // adapt_stress2 for one particle (code/3_SPH_material_2018.f90:2066-2140).
 __device__ __forceinline__ void adapt_stress(double4 &s, double tanfi, double coh) {
     double alpha2 = tanfi / (sqrt(9 + 12 * (tanfi * tanfi)));

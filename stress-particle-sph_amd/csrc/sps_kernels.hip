@@ -138,6 +138,28 @@ __global__ void __launch_bounds__(BLOCK) k_bounds_final(const double *__restrict
     g->ndivt = g->overflow ? 0 : (int)ndivt;
 }
 
+// ------------------------------------------------------------- cell keys
+// Buckets are kept in strip-major cell order: the grid is cut into vertical
+// strips of STRIP cells, cells run row by row inside a strip.  Walking the
+// storage (and so the receivers) up a strip keeps the partner data of three
+// consecutive rows in one L2.  The reference's own visiting order (row-major
+// cells, 2_...:1322-1324) is kept separately for the list order (k_proc_slot).
+constexpr int STRIP = 64;
+
+__device__ __forceinline__ int cell_key(int cx, int cy, const Grid &g) {
+    int x0 = (cx / STRIP) * STRIP;
+    int w = g.ndivx - x0 < STRIP ? g.ndivx - x0 : STRIP;
+    return x0 * g.ndivy + cy * w + (cx - x0);
+}
+
+__device__ __forceinline__ void cell_xy(int key, const Grid &g, int &cx, int &cy) {
+    int x0 = (key / (STRIP * g.ndivy)) * STRIP;
+    int w = g.ndivx - x0 < STRIP ? g.ndivx - x0 : STRIP;
+    int rem = key - x0 * g.ndivy;
+    cy = rem / w;
+    cx = x0 + rem % w;
+}
+
 // ------------------------------------------------------------- cell binning
 // Task 2 of grid_find_NEW (2_...:1277-1305): cell of every particle and
 // per-(kind, cell) counts.  Out-of-domain particles go to the sentinel cell
@@ -155,7 +177,7 @@ __global__ void k_cell_count(const double2 *__restrict__ x, const int *__restric
         int iy = (int)((p.y - g->xmin[1]) / g->deltx + 1);
         if (ix > g->ndivx) ix = g->ndivx;
         if (iy > g->ndivy) iy = g->ndivy;
-        c = g->ndivx * (iy - 1) + ix - 1;
+        c = cell_key(ix - 1, iy - 1, *g);
     }
     cellkey[i] = c;
     int k = kind_of(i, nn, nt);
@@ -209,12 +231,28 @@ __global__ void k_cell_sort(const int *__restrict__ kstart, int *__restrict__ tm
     }
 }
 
-// Processing slot (the reference's cell-major visiting order) of each
-// in-domain particle: cell start over all kinds plus the rank inside the
-// cell, nodes before stress points before dummies.
+// Particles per cell in the reference's row-major visiting order.
+__global__ void k_cell_totals(const int *__restrict__ kstart, const Grid *__restrict__ g,
+                              unsigned *__restrict__ tot) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    int ndivt = g->ndivt;
+    if (c > ndivt) return;
+    if (c == ndivt) {
+        tot[c] = 0;
+        return;
+    }
+    const int L = ndivt + 1;
+    int key = cell_key(c % g->ndivx, c / g->ndivx, *g);
+    tot[c] = (unsigned)((kstart[key + 1] - kstart[key]) + (kstart[L + key + 1] - kstart[L + key]) +
+                        (kstart[2 * L + key + 1] - kstart[2 * L + key]));
+}
+
+// Processing slot (the reference's visiting order: row-major cells, inside
+// a cell ascending original index, i.e. nodes, stress points, dummies) of
+// each in-domain particle.
 __global__ void k_proc_slot(const int *__restrict__ cell, const int *__restrict__ kstart,
-                            const Grid *__restrict__ g, int *__restrict__ proc, int nn, int nt,
-                            int nt2) {
+                            const unsigned *__restrict__ cstart_rm, const Grid *__restrict__ g,
+                            int *__restrict__ proc, int nn, int nt, int nt2) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nt2) return;
     int ndivt = g->ndivt;
@@ -224,12 +262,13 @@ __global__ void k_proc_slot(const int *__restrict__ cell, const int *__restrict_
         return;
     }
     const int L = ndivt + 1;
+    int cx, cy;
+    cell_xy(c, *g, cx, cy);
     int st0 = kstart[c], st1 = kstart[L + c], st2 = kstart[2 * L + c];
-    int allstart = (st0 - kstart[0]) + (st1 - kstart[L]) + (st2 - kstart[2 * L]);
     int n0 = kstart[c + 1] - st0, n1 = kstart[L + c + 1] - st1;
     int k = kind_of(s, nn, nt);
     int pos = k == KIND_NODE ? s - st0 : (k == KIND_SP ? n0 + s - st1 : n0 + n1 + s - st2);
-    proc[s] = allstart + pos;
+    proc[s] = (int)cstart_rm[cy * g->ndivx + cx] + pos;
 }
 
 // ------------------------------------------------------------- permutation
@@ -266,22 +305,19 @@ __global__ void k_permute(StateBufs dst, StateBufs src, const int *__restrict__ 
 
 // --------------------------------------------------------- stencil helpers
 // The 3x3 stencil of a cell: rows y0..y1, columns x0..x1 (clamped at the
-// grid edges like 2_...:1326-1327).  Kept in scalars: an indexed array here
-// would be placed in scratch or LDS by the compiler.
+// grid edges like 2_...:1326-1327), visited cell by cell.  Kept in scalars:
+// an indexed array here would be placed in scratch or LDS by the compiler.
 struct Stencil {
-    int x0, x1, y0, y1, ndivx;
-    __device__ __forceinline__ int lo(int y) const { return y * ndivx + x0; }
-    __device__ __forceinline__ int hi(int y) const { return y * ndivx + x1; }
+    int cx, cy, x0, x1, y0, y1;
 };
 
-__device__ __forceinline__ Stencil make_stencil(int c, int ndivx, int ndivy) {
+__device__ __forceinline__ Stencil make_stencil(int key, const Grid &g) {
     Stencil S;
-    int cx = c % ndivx, cy = c / ndivx;
-    S.x0 = cx > 0 ? cx - 1 : 0;
-    S.x1 = cx + 1 < ndivx ? cx + 1 : ndivx - 1;
-    S.y0 = cy > 0 ? cy - 1 : 0;
-    S.y1 = cy + 1 < ndivy ? cy + 1 : ndivy - 1;
-    S.ndivx = ndivx;
+    cell_xy(key, g, S.cx, S.cy);
+    S.x0 = S.cx > 0 ? S.cx - 1 : 0;
+    S.x1 = S.cx + 1 < g.ndivx ? S.cx + 1 : g.ndivx - 1;
+    S.y0 = S.cy > 0 ? S.cy - 1 : 0;
+    S.y1 = S.cy + 1 < g.ndivy ? S.cy + 1 : g.ndivy - 1;
     return S;
 }
 
@@ -315,48 +351,45 @@ __global__ void k_count(const double2 *__restrict__ x, const double *__restrict_
                         const int *__restrict__ proc, const Grid *__restrict__ g,
                         unsigned *__restrict__ cnt, int *__restrict__ degA, int *__restrict__ degB,
                         Split *__restrict__ split, int nn, int nt, int nt2) {
-    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    int r = xcd_block() * blockDim.x + threadIdx.x;
     unsigned fwd = 0;
     int da = 0, db = 0;
     unsigned t0 = 0, t1 = 0, t2 = 0, t3 = 0, t6 = 0, t9 = 0;   // pairs by pint_type
     int ndivt = g->ndivt;
     if (r < nt2 && cell[r] < ndivt) {
         const int L = ndivt + 1;
-        int c = cell[r];
+        const Grid gg = *g;
         int kr = kind_of(r, nn, nt);
         double2 xr = x[r];
         double hr = hsml[r];
-        Stencil S = make_stencil(c, g->ndivx, g->ndivy);
+        Stencil S = make_stencil(cell[r], gg);
+        const int crm_r = S.cy * gg.ndivx + S.cx;
         for (int y = S.y0; y <= S.y1; ++y) {
-            const int rlo = S.lo(y), rhi = S.hi(y);
-            for (int ks = 0; ks < 3; ++ks) {
-                int lo = kstart[ks * L + rlo], hi = kstart[ks * L + rhi + 1];
-                int c_beg = kstart[ks * L + c], c_end = kstart[ks * L + c + 1];
-                for (int s = lo; s < hi; ++s) {
-                    if (s == r) continue;
-                    double2 xs = x[s];
-                    double mh;
-                    double rr = pair_distance(xr.x - xs.x, xr.y - xs.y, hr, hsml[s], mh);
-                    if (rr < 0.0) continue;
-                    bool after;
-                    if (rlo > c) after = true;                       // row above
-                    else if (rhi < c) after = false;                 // row below
-                    else if (s >= c_end) after = true;               // right cell
-                    else if (s < c_beg) after = false;               // left cell
-                    else after = ks > kr || (ks == kr && s > r);     // same cell
-                    if (after) {
-                        fwd++;
-                        switch (pair_type(kr, ks)) {
-                            case 1: t1++; break;
-                            case 2: t2++; break;
-                            case 3: t3++; break;
-                            case 6: t6++; break;
-                            case 9: t9++; break;
-                            default: t0++; break;
+            for (int xx = S.x0; xx <= S.x1; ++xx) {
+                const int key = cell_key(xx, y, gg), crm = y * gg.ndivx + xx;
+                for (int ks = 0; ks < 3; ++ks) {
+                    int lo = kstart[ks * L + key], hi = kstart[ks * L + key + 1];
+                    for (int s = lo; s < hi; ++s) {
+                        if (s == r) continue;
+                        double2 xs = x[s];
+                        double mh;
+                        double rr = pair_distance(xr.x - xs.x, xr.y - xs.y, hr, hsml[s], mh);
+                        if (rr < 0.0) continue;
+                        bool after = crm > crm_r || (crm == crm_r && (ks > kr || (ks == kr && s > r)));
+                        if (after) {
+                            fwd++;
+                            switch (pair_type(kr, ks)) {
+                                case 1: t1++; break;
+                                case 2: t2++; break;
+                                case 3: t3++; break;
+                                case 6: t6++; break;
+                                case 9: t9++; break;
+                                default: t0++; break;
+                            }
                         }
+                        if (in_list_a(kr, ks)) da++;
+                        if (kr == KIND_NODE && ks == KIND_NODE) db++;
                     }
-                    if (in_list_a(kr, ks)) da++;
-                    if (kr == KIND_NODE && ks == KIND_NODE) db++;
                 }
             }
         }
@@ -419,19 +452,20 @@ __global__ void k_split_partner(const double2 *__restrict__ x, const double *__r
     if (ps < 0) return;
     unsigned long long kstar = split->M - off[ps];
     int r = proc_to_slot[ps];
-    int ndivt = g->ndivt;
-    const int L = ndivt + 1;
-    int c = cell[r];
+    const Grid gg = *g;
+    const int L = gg.ndivt + 1;
     int kr = kind_of(r, nn, nt);
     double2 xr = x[r];
-    Stencil S = make_stencil(c, g->ndivx, g->ndivy);
+    Stencil S = make_stencil(cell[r], gg);
+    const int crm_r = S.cy * gg.ndivx + S.cx;
     unsigned long long local = 0;
     for (int y = S.y0; y <= S.y1; ++y)
-        for (int cc = S.lo(y); cc <= S.hi(y); ++cc) {
-            if (cc < c) continue;
+        for (int xx = S.x0; xx <= S.x1; ++xx) {
+            const int key = cell_key(xx, y, gg), crm = y * gg.ndivx + xx;
+            if (crm < crm_r) continue;
             for (int ks = 0; ks < 3; ++ks)
-                for (int s = kstart[ks * L + cc]; s < kstart[ks * L + cc + 1]; ++s) {
-                    if (cc == c && !(ks > kr || (ks == kr && s > r))) continue;
+                for (int s = kstart[ks * L + key]; s < kstart[ks * L + key + 1]; ++s) {
+                    if (crm == crm_r && !(ks > kr || (ks == kr && s > r))) continue;
                     double2 xs = x[s];
                     double mh;
                     if (pair_distance(xr.x - xs.x, xr.y - xs.y, hsml[r], hsml[s], mh) < 0.0) continue;
@@ -478,7 +512,7 @@ __global__ void k_build(const double2 *__restrict__ x, const double *__restrict_
                         const unsigned *__restrict__ off, const Grid *__restrict__ g,
                         const Split *__restrict__ split, Lists Lz, double pi, int nn, int nt,
                         int nt2) {
-    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    int r = xcd_block() * blockDim.x + threadIdx.x;
     if (r >= nt2) return;
     int ndivt = g->ndivt;
     int c = cell[r];
@@ -492,52 +526,51 @@ __global__ void k_build(const double2 *__restrict__ x, const double *__restrict_
     int pr = proc[r];
     double2 xr = x[r];
     double hr = hsml[r];
-    Stencil S = make_stencil(c, g->ndivx, g->ndivy);
+    const Grid gg = *g;
+    Stencil S = make_stencil(c, gg);
     unsigned base_a = Lz.a_off[r >> 6] + (r & 63);
     unsigned base_b = r < nn ? Lz.b_off[r >> 6] + (r & 63) : 0;
     int na = 0, nb = 0;
+    const int nx = S.x1 - S.x0 + 1, ny = S.y1 - S.y0 + 1;
     for (int pass = 0; pass < 2; ++pass) {
         const bool rev = pass == 0;
         if (rev && !any_rev) continue;
         if (!rev && !any_fwd) continue;
-        const int nrow = S.y1 - S.y0 + 1;
-        for (int ri = 0; ri < nrow; ++ri) {
-            const int y = rev ? S.y1 - ri : S.y0 + ri;
-            const int rlo = S.lo(y), rhi = S.hi(y);
-            const int ncell = rhi - rlo + 1;
-            for (int ci = 0; ci < ncell; ++ci) {
-                int cc = rev ? rhi - ci : rlo + ci;
-                for (int ki = 0; ki < 3; ++ki) {
-                    int ks = rev ? 2 - ki : ki;
-                    bool la = in_list_a(kr, ks), lb = kr == KIND_NODE && ks == KIND_NODE;
-                    if (!la && !lb) continue;
-                    int lo = kstart[ks * L + cc], hi = kstart[ks * L + cc + 1];
-                    int cntk = hi - lo;
-                    for (int q = 0; q < cntk; ++q) {
-                        int s = rev ? hi - 1 - q : lo + q;
-                        if (s == r) continue;
-                        double2 xs = x[s];
-                        double dx0 = xr.x - xs.x, dx1 = xr.y - xs.y;
-                        double mh;
-                        double rr = pair_distance(dx0, dx1, hr, hsml[s], mh);
-                        if (rr < 0.0) continue;
-                        if (mixed && pair_reversed(pr, proc[s], cnt, off, sp) != rev) continue;
-                        float w, gx, gy;
-                        cubic_kernel(rr, dx0, dx1, mh, pi, w, gx, gy);
-                        if (la) {
-                            unsigned e = base_a + (unsigned)na * SLICE;
-                            Lz.a_s[e] = s;
-                            Lz.a_w[e] = w;
-                            Lz.a_gx[e] = gx;
-                            Lz.a_gy[e] = gy;
-                            na++;
-                        } else {
-                            unsigned e = base_b + (unsigned)nb * SLICE;
-                            Lz.b_s[e] = s;
-                            Lz.b_gx[e] = gx;
-                            Lz.b_gy[e] = gy;
-                            nb++;
-                        }
+        for (int ci = 0; ci < nx * ny; ++ci) {
+            // cells in ascending (forward) or descending (reverse) row-major order
+            const int cj = rev ? nx * ny - 1 - ci : ci;
+            const int y = S.y0 + cj / nx, xx = S.x0 + cj % nx;
+            const int key = cell_key(xx, y, gg);
+            for (int ki = 0; ki < 3; ++ki) {
+                int ks = rev ? 2 - ki : ki;
+                bool la = in_list_a(kr, ks), lb = kr == KIND_NODE && ks == KIND_NODE;
+                if (!la && !lb) continue;
+                int lo = kstart[ks * L + key], hi = kstart[ks * L + key + 1];
+                int cntk = hi - lo;
+                for (int q = 0; q < cntk; ++q) {
+                    int s = rev ? hi - 1 - q : lo + q;
+                    if (s == r) continue;
+                    double2 xs = x[s];
+                    double dx0 = xr.x - xs.x, dx1 = xr.y - xs.y;
+                    double mh;
+                    double rr = pair_distance(dx0, dx1, hr, hsml[s], mh);
+                    if (rr < 0.0) continue;
+                    if (mixed && pair_reversed(pr, proc[s], cnt, off, sp) != rev) continue;
+                    float w, gx, gy;
+                    cubic_kernel(rr, dx0, dx1, mh, pi, w, gx, gy);
+                    if (la) {
+                        unsigned e = base_a + (unsigned)na * SLICE;
+                        Lz.a_s[e] = s;
+                        Lz.a_w[e] = w;
+                        Lz.a_gx[e] = gx;
+                        Lz.a_gy[e] = gy;
+                        na++;
+                    } else {
+                        unsigned e = base_b + (unsigned)nb * SLICE;
+                        Lz.b_s[e] = s;
+                        Lz.b_gx[e] = gx;
+                        Lz.b_gy[e] = gy;
+                        nb++;
                     }
                 }
             }
@@ -550,8 +583,6 @@ struct Work {
     double2 *vx0, *x0, *vel0, *rk_vel, *rhs2, *f_bound, *art_visc;
     double4 *stress0, *rk_stress, *rhs1, *grad_u;
     double *rk_dev, *ddev;
-    double *mrho;      // m / rho per particle (mass and density are fixed on the path)
-    double4 *sr2;      // sigma / rho^2 per stress point, refreshed by k_spu_sps
 };
 
 // Start of a step: vx0, x0 (2_...:114) and the RK4 prologue (2_...:681-690).
@@ -574,7 +605,6 @@ __global__ void k_rk_init(StateBufs S, Work W, int nn, int nt, int nt2) {
     }
     S.vel[i] = make_double2(0, 0);
     S.stress[i] = make_double4(0, 0, 0, 0);
-    W.mrho[i] = S.mass[i] / S.rho[i];
 }
 
 // Stage state (2_...:700-701) and the first adapt_stress2 of the stage.
@@ -602,17 +632,18 @@ __global__ void k_stage_begin(StateBufs S, Work W, double c1, int adapt, double 
 // stress and plastic strain from the stress points, then adapt_stress2.
 __global__ void k_spu_nodes(StateBufs S, Work W, const int *__restrict__ degA, Lists Lz, int adapt,
                             double tanfi, double coh, int nn, int nt) {
-    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    int r = xcd_block() * blockDim.x + threadIdx.x;
     if (r >= nn) return;
     unsigned e = Lz.a_off[r >> 6] + (r & 63);
     int deg = degA[r];
     double t0 = 0, t1 = 0, t2 = 0, t3 = 0, tv = 0, norm = 0;
+#pragma unroll 4
     for (int k = 0; k < deg; ++k, e += SLICE) {
         int s = Lz.a_s[e];
         if (s >= nt) continue;                      // type 6 entry
         double w = (double)Lz.a_w[e];
         double ms = S.mass[s], rs = S.rho[s];
-        double h1 = W.mrho[s] * w;                  // (m/rho)*w, m/rho hoisted per particle
+        double h1 = (ms / rs) * w;
         double4 st = S.stress[s];
         t0 = t0 + st.x * h1;
         t1 = t1 + st.y * h1;
@@ -636,33 +667,29 @@ __global__ void k_spu_nodes(StateBufs S, Work W, const int *__restrict__ degA, L
 // from the nodes, then adapt_stress2 of the stress point.
 __global__ void k_spu_sps(StateBufs S, Work W, const int *__restrict__ degA, Lists Lz, int adapt,
                           double tanfi, double coh, int nn, int nt) {
-    int r = nn + blockIdx.x * blockDim.x + threadIdx.x;
+    int r = nn + xcd_block() * blockDim.x + threadIdx.x;
     if (r >= nt) return;
     unsigned e = Lz.a_off[r >> 6] + (r & 63);
     int deg = degA[r];
     double v0 = 0, v1 = 0, norm = 0;
+#pragma unroll 4
     for (int k = 0; k < deg; ++k, e += SLICE) {
         int s = Lz.a_s[e];
         if (s >= nn) continue;                      // type 9 entry
         double w = (double)Lz.a_w[e];
         double ms = S.mass[s], rs = S.rho[s];
-        double h2 = W.mrho[s] * w;
+        double h2 = (ms / rs) * w;
         double2 vs = S.vel[s];
         v0 = v0 + vs.x * h2;
         v1 = v1 + vs.y * h2;
         norm = norm + (w * ms) / rs;
     }
     if (norm != 0) S.vel[r] = make_double2(v0 / norm, v1 / norm);
-    double4 s = S.stress[r];
     if (adapt) {
+        double4 s = S.stress[r];
         adapt_stress(s, tanfi, coh);
         S.stress[r] = s;
     }
-    // sigma / rho^2 of this stress point, as get_derivatives divides it
-    // (2_...:530) for every node partner
-    double rr = S.rho[r];
-    double r2 = rr * rr;
-    W.sr2[r] = make_double4(s.x / r2, s.y / r2, s.z / r2, 0.0);
 }
 
 // get_derivatives (2_...:487-648) for stress-point receivers, fused with
@@ -670,13 +697,14 @@ __global__ void k_spu_sps(StateBufs S, Work W, const int *__restrict__ degA, Lis
 // stress right-hand side and RK accumulation (2_...:762, 772).
 __global__ void k_deriv_sps(StateBufs S, Work W, const int *__restrict__ degA, Lists Lz,
                             DevParams P, double time_sph, double f2, int nn, int nt) {
-    int r = nn + blockIdx.x * blockDim.x + threadIdx.x;
+    int r = nn + xcd_block() * blockDim.x + threadIdx.x;
     if (r >= nt) return;
     unsigned e = Lz.a_off[r >> 6] + (r & 63);
     int deg = degA[r];
     double g00 = 0, g01 = 0, g10 = 0, g11 = 0;   // grad1_tmp(a,b): d v_a / d x_b
     double A0 = 0, A1 = 0, A2 = 0, A3 = 0;
     double2 xr = S.x[r], vr = S.vel[r];
+#pragma unroll 4
     for (int k = 0; k < deg; ++k, e += SLICE) {
         int s = Lz.a_s[e];
         double ms = S.mass[s], rs = S.rho[s];
@@ -811,7 +839,7 @@ __device__ __forceinline__ double2 wall_force(const StateBufs &S, const Lists &L
 __global__ void k_deriv_nodes(StateBufs S, Work W, const int *__restrict__ degA,
                               const int *__restrict__ degB, Lists Lz, DevParams P, double gx_grav,
                               double gy_grav, double f2, int *__restrict__ err, int nn, int nt) {
-    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    int r = xcd_block() * blockDim.x + threadIdx.x;
     if (r >= nn) return;
     double2 xr = S.x[r], vr = S.vel[r];
     double4 sr = S.stress[r];
@@ -823,20 +851,22 @@ __global__ void k_deriv_nodes(StateBufs S, Work W, const int *__restrict__ degA,
     int ndum = 0;
     unsigned e = Lz.a_off[r >> 6] + (r & 63);
     int deg = degA[r];
+#pragma unroll 4
     for (int k = 0; k < deg; ++k, e += SLICE) {
         int s = Lz.a_s[e];
         double ms = S.mass[s], rs = S.rho[s];
         double gx = (double)(-Lz.a_gx[e]), gy = (double)(-Lz.a_gy[e]);   // pair_i is the partner
         double rs2 = rs * rs;
         if (s < nt) {                                // type 1
-            double4 ss = W.sr2[s];                   // sigma_s / rho_s^2
+            double4 ss = S.stress[s];
+            const double p0 = ss.x / rs2, p1 = ss.y / rs2, p2 = ss.z / rs2;   // sigma_s / rho_s^2
             double a;
-            a = gx * (or0 + ss.x); q00 = q00 - ms * a;
-            a = gy * (or0 + ss.x); q01 = q01 - ms * a;
-            a = gx * (or1 + ss.y); q10 = q10 - ms * a;
-            a = gy * (or1 + ss.y); q11 = q11 - ms * a;
-            a = gx * (or2 + ss.z); q20 = q20 - ms * a;
-            a = gy * (or2 + ss.z); q21 = q21 - ms * a;
+            a = gx * (or0 + p0); q00 = q00 - ms * a;
+            a = gy * (or0 + p0); q01 = q01 - ms * a;
+            a = gx * (or1 + p1); q10 = q10 - ms * a;
+            a = gy * (or1 + p1); q11 = q11 - ms * a;
+            a = gx * (or2 + p2); q20 = q20 - ms * a;
+            a = gy * (or2 + p2); q21 = q21 - ms * a;
             if (P.cspm) {
                 double h1b = -(gx * ms / rs);
                 double h2b = -(gy * ms / rs);
@@ -890,6 +920,7 @@ __global__ void k_deriv_nodes(StateBufs S, Work W, const int *__restrict__ degA,
         const float c = 600;
         unsigned eb = Lz.b_off[r >> 6] + (r & 63);
         int degb = degB[r];
+#pragma unroll 4
         for (int k = 0; k < degb; ++k, eb += SLICE) {
             int s = Lz.b_s[eb];
             double2 xs = S.x[s], vs = S.vel[s];
@@ -1002,14 +1033,17 @@ __global__ void k_dump_pairs(const double2 *__restrict__ x, const double2 *__res
     const int L = ndivt + 1;
     int kr = kind_of(r, nn, nt);
     double2 xr = r < nt ? x0[r] : x[r];
-    Stencil S = make_stencil(c, g->ndivx, g->ndivy);
+    const Grid gg = *g;
+    Stencil S = make_stencil(c, gg);
+    const int crm_r = S.cy * gg.ndivx + S.cx;
     unsigned d = off[proc[r]];
     for (int y = S.y0; y <= S.y1; ++y)
-        for (int cc = S.lo(y); cc <= S.hi(y); ++cc) {
-            if (cc < c) continue;
+        for (int xx = S.x0; xx <= S.x1; ++xx) {
+            const int key = cell_key(xx, y, gg), crm = y * gg.ndivx + xx;
+            if (crm < crm_r) continue;
             for (int ks = 0; ks < 3; ++ks)
-                for (int s = kstart[ks * L + cc]; s < kstart[ks * L + cc + 1]; ++s) {
-                    if (cc == c && !(ks > kr || (ks == kr && s > r))) continue;
+                for (int s = kstart[ks * L + key]; s < kstart[ks * L + key + 1]; ++s) {
+                    if (crm == crm_r && !(ks > kr || (ks == kr && s > r))) continue;
                     double2 xs = s < nt ? x0[s] : x[s];
                     double dx0 = xr.x - xs.x, dx1 = xr.y - xs.y, mh;
                     double rr = pair_distance(dx0, dx1, hsml[r], hsml[s], mh);
