@@ -72,6 +72,16 @@ def algorithmic_bytes(kernel, st, counts, params):
     return None
 
 
+def measured_traffic(deck_name, kernel):
+    """HBM bytes per launch of `kernel` from the calibrated FETCH_SIZE and
+    WRITE_SIZE counter runs summarised in profiles/ (None if not profiled)."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{deck_name}.json")
+    if not os.path.exists(path):
+        return None
+    k = json.load(open(path))["kernels"].get(kernel)
+    return None if k is None else k["bytes"]
+
+
 def run_gpu(args, sps, deck):
     import torch
     sim = sps.Simulation.from_deck(deck)
@@ -149,7 +159,8 @@ def main():
     if ab is not None:
         achieved = ab / avg_s / 1e9
         roofline = {"kernel": dname, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                    "traffic": measured_traffic(args.deck, dname),
                     "bytes_per_launch": int(ab), "avg_launch_ms": round(dms / dlaunch, 5),
                     "launches": dlaunch}
     breakdown = {k: {"launches": n, "ms_per_step": round(ms / args.steps, 4)} for k, (n, ms) in
