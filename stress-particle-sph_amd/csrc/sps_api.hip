@@ -3,12 +3,13 @@
 //
 // One step is the reference's time_integration (code/2_SPH_main_2018.f90
 // :78-184) for the supported flag set:
-//   Check_Out_Domain, grid_find_NEW + Pint_Update  -> k_check_ood .. k_build
+//   Check_Out_Domain, grid_find_NEW + Pint_Update  -> k_check_ood .. k_split_partner
 //   RK4 (2_...:653-802)                            -> k_rk_init, 4 x stage, k_rk_final
 //   update_strain, stress_point_update, adapt      -> k_rk_final, k_spu_*
 //   position update and displacement               -> k_position
-// Every kernel is a gather over the ordered neighbour lists; see
-// sps_kernels.hip for the ordering argument.
+// The search bins the particles and fixes the order of the pair list; every
+// stage kernel then walks each receiver's cell stencil in that order and
+// recomputes w and grad w, so no pair list is stored.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -130,13 +131,14 @@ struct sps_handle {
     DBuf<double> bparts;
     int nbparts = 0;
     DBuf<int> cellkey, new_cell, cell, kcount, kstart, kfill, tmp_orig, tmp_src, proc, proc_to_slot;
-    DBuf<unsigned> cnt, off, scan_tmp, widthA, widthB, offA, offB, ctot, cstart_rm;
-    DBuf<int> degA, degB;
-    DBuf<unsigned long long> totals;   // [0] list A size, [1] list B size
+    DBuf<unsigned> cnt, off, scan_tmp, ctot, cstart_rm;
+    // per-step pair lists of the tiled stage kernels
+    DBuf<unsigned long long> list_ia, list_ib;
+    DBuf<float4> list_wa, list_gxa, list_gya, list_gxb, list_gyb;
+    DBuf<unsigned short> deg_a, deg_b;
+    DBuf<unsigned char> tile_ok;
+    int ntile_n = 0, ntile_s = 0;
     DBuf<int> err;
-    // lists
-    DBuf<int> a_s, b_s;
-    DBuf<float> a_w, a_gx, a_gy, b_gx, b_gy;
     // host-visible summary of the last search
     unsigned long long n_pairs = 0, m_pairs = 0, m_prev = 0;
     unsigned long long n_by_type[10] = {0};
@@ -152,18 +154,23 @@ struct sps_handle {
     std::vector<hipEvent_t> ev_pool;
     double prof_ms[32] = {0};
     long long prof_n[32] = {0};
+
+    PairLists lists() {
+        return PairLists{list_ia.p, list_ib.p, list_wa.p, list_gxa.p, list_gya.p, list_gxb.p, list_gyb.p,
+                         deg_a.p, deg_b.p, tile_ok.p, ntile_n};
+    }
 };
 
 // ------------------------------------------------------------ profiling
 enum KernelId {
     K_CHECK_OOD, K_BOUNDS, K_CELL_COUNT, K_CELL_SCATTER, K_CELL_SORT, K_PERMUTE, K_PROC,
-    K_COUNT_PAIRS, K_SCAN, K_SPLIT, K_SLICE, K_BUILD, K_RK_INIT, K_STAGE_BEGIN, K_SPU_NODES,
-    K_SPU_SPS, K_DERIV_SPS, K_DERIV_NODES, K_DUMMY_STRESS, K_RK_FINAL, K_POSITION, K_NKERNELS
+    K_COUNT_PAIRS, K_SCAN, K_SPLIT, K_RK_INIT, K_STAGE_BEGIN, K_SPU_NODES,
+    K_SP_UPDATE, K_DERIV_NODES, K_DUMMY_STRESS, K_RK_FINAL, K_POSITION, K_NKERNELS
 };
 static const char *const kKernelNames[K_NKERNELS] = {
     "k_check_ood", "k_bounds", "k_cell_count", "k_cell_scatter", "k_cell_sort", "k_permute",
-    "k_proc_slot", "k_count", "k_scan", "k_split", "k_slice_width", "k_build", "k_rk_init",
-    "k_stage_begin", "k_spu_nodes", "k_spu_sps", "k_deriv_sps", "k_deriv_nodes",
+    "k_proc_slot", "k_count", "k_scan", "k_split", "k_rk_init",
+    "k_stage_begin", "k_spu_nodes", "k_sp_update", "k_deriv_nodes",
     "k_dummy_stress", "k_rk_final", "k_position"};
 
 static hipEvent_t prof_begin(sps_handle *h) {
@@ -207,12 +214,13 @@ static void prof_collect(sps_handle *h) {
     h->pending.clear();
 }
 
-#define LAUNCH(KID, GRID, BLK, KERN, ...)                    \
-    do {                                                     \
-        hipEvent_t e0_ = prof_begin(h);                      \
-        KERN<<<(GRID), (BLK), 0, h->stream>>>(__VA_ARGS__);  \
-        prof_end(h, (KID), e0_);                             \
+#define LAUNCH_LDS(KID, GRID, BLK, LDS, KERN, ...)               \
+    do {                                                         \
+        hipEvent_t e0_ = prof_begin(h);                          \
+        KERN<<<(GRID), (BLK), (LDS), h->stream>>>(__VA_ARGS__);  \
+        prof_end(h, (KID), e0_);                                 \
     } while (0)
+#define LAUNCH(KID, GRID, BLK, KERN, ...) LAUNCH_LDS(KID, GRID, BLK, 0, KERN, __VA_ARGS__)
 
 extern "C" const char *sps_last_error(void) { return g_err; }
 extern "C" const char *sps_backend(void) { return "hip-gfx950"; }
@@ -326,15 +334,21 @@ extern "C" int sps_create(const sps_params *params, sps_handle **out) {
     rc |= dalloc(h->rhs1, nt); rc |= dalloc(h->grad_u, nt); rc |= dalloc(h->rk_dev, nt);
     rc |= dalloc(h->ddev, nt);
     rc |= dalloc(h->grid, 1); rc |= dalloc(h->split, 1); rc |= dalloc(h->err, 1);
-    rc |= dalloc(h->totals, 2);
     h->nbparts = std::min(1024, (int)nblk(nt2));
     rc |= dalloc(h->bparts, 5 * (size_t)h->nbparts);
     rc |= dalloc(h->cellkey, nt2); rc |= dalloc(h->new_cell, nt2); rc |= dalloc(h->cell, nt2);
     rc |= dalloc(h->tmp_orig, nt2); rc |= dalloc(h->tmp_src, nt2); rc |= dalloc(h->proc, nt2);
     rc |= dalloc(h->proc_to_slot, nt2); rc |= dalloc(h->cnt, nt2); rc |= dalloc(h->off, nt2);
-    rc |= dalloc(h->degA, nt2); rc |= dalloc(h->degB, nn);
-    rc |= dalloc(h->widthA, (nt2 + 63) / 64 + 1); rc |= dalloc(h->widthB, (nn + 63) / 64 + 1);
-    rc |= dalloc(h->offA, (nt2 + 63) / 64 + 1); rc |= dalloc(h->offB, (nn + 63) / 64 + 1);
+    h->ntile_n = (int)nblk8(nn);
+    h->ntile_s = (int)nblk8(nt - nn);
+    {
+        const size_t wa = (size_t)(h->ntile_n + h->ntile_s) * LGROUPS * TILE, wb = (size_t)h->ntile_n * LGROUPS * TILE;
+        rc |= dalloc(h->list_ia, wa); rc |= dalloc(h->list_wa, wa); rc |= dalloc(h->list_gxa, wa);
+        rc |= dalloc(h->list_gya, wa);
+        rc |= dalloc(h->list_ib, wb); rc |= dalloc(h->list_gxb, wb); rc |= dalloc(h->list_gyb, wb);
+    }
+    rc |= dalloc(h->deg_a, nt); rc |= dalloc(h->deg_b, nn);
+    rc |= dalloc(h->tile_ok, h->ntile_n + h->ntile_s + 1);
     if (rc) return bail(rc);
     if (hipHostMalloc((void **)&h->pinned, 64 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
         return bail(fail(SPS_ENOMEM, "pinned staging allocation"));
@@ -473,8 +487,19 @@ static int search(sps_handle *h, bool fresh_list) {
     HIP_TRY(hipMemsetAsync(h->cnt.p, 0, nt2 * sizeof(unsigned), s));
     const unsigned long long M = fresh_list ? 0ull : h->m_pairs;
     LAUNCH(K_SPLIT, 1, 64, k_split_reset, h->split.p, M);
-    LAUNCH(K_COUNT_PAIRS, nblk8(nt2), BLOCK, k_count, C.x.p, C.hsml.p, h->cell.p, h->kstart.p, h->proc.p, h->grid.p,
-                                       h->cnt.p, h->degA.p, h->degB.p, h->split.p, nn, nt, nt2);
+    {
+        const SearchView V{h->cell.p, h->kstart.p, h->proc.p, h->cnt.p, h->off.p, h->grid.p, h->split.p, h->p.pi};
+        const PairLists PL = h->lists();
+        if (nn > 0)
+            LAUNCH_LDS(K_COUNT_PAIRS, nblk8(nn), TILE, LAY_COUNT.bytes, k_count_lists<KIND_NODE>, V, C.x.p, C.hsml.p,
+                       h->cnt.p, h->split.p, PL, 0, nn);
+        if (nt > nn)
+            LAUNCH_LDS(K_COUNT_PAIRS, nblk8(nt - nn), TILE, LAY_COUNT.bytes, k_count_lists<KIND_SP>, V, C.x.p,
+                       C.hsml.p, h->cnt.p, h->split.p, PL, nn, nt - nn);
+        if (nt2 > nt)
+            LAUNCH(K_COUNT_PAIRS, nblk8(nt2 - nt), BLOCK, k_count, C.x.p, C.hsml.p, h->cell.p, h->kstart.p, h->proc.p,
+                   h->grid.p, h->cnt.p, h->split.p, nn, nt, nt, nt2);
+    }
     HIP_TRY(hipGetLastError());
     // discovery offsets over all slots (slots past the in-domain count hold 0)
     rc = scan_u32(h, h->cnt.p, h->off.p, nt2, &h->split.p->n);
@@ -482,36 +507,11 @@ static int search(sps_handle *h, bool fresh_list) {
     LAUNCH(K_SPLIT, nblk(nt2), BLOCK, k_split, h->cnt.p, h->off.p, nt2, h->split.p);
     LAUNCH(K_SPLIT, 1, 64, k_split_partner, C.x.p, C.hsml.p, h->cell.p, h->kstart.p, h->proc.p, h->off.p,
                                      h->proc_to_slot.p, h->grid.p, h->split.p, nn, nt);
-    // list shapes
-    LAUNCH(K_SLICE, nblk(((nt2 + 63) / 64) * 64), BLOCK, k_slice_width, h->degA.p, nt2, h->widthA.p);
-    LAUNCH(K_SLICE, nblk(((nn + 63) / 64) * 64), BLOCK, k_slice_width, h->degB.p, nn, h->widthB.p);
-    rc = scan_u32(h, h->widthA.p, h->offA.p, (nt2 + 63) / 64, h->totals.p);
-    if (rc) return rc;
-    rc = scan_u32(h, h->widthB.p, h->offB.p, (nn + 63) / 64, h->totals.p + 1);
-    if (rc) return rc;
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(h->pinned, h->totals.p, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipMemcpyAsync(h->pinned + 2, h->split.p, sizeof(Split), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(h->pinned, h->split.p, sizeof(Split), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
-    unsigned long long sizeA = h->pinned[0], sizeB = h->pinned[1];
     Split sh;
-    std::memcpy(&sh, h->pinned + 2, sizeof(Split));
-    if (sizeA >= (1ull << 32) || sizeB >= (1ull << 32))
-        return fail(SPS_ECAPACITY, "neighbour lists exceed 2^32 entries");
-    rc = 0;
-    if (h->a_s.n < sizeA) {
-        size_t want = (size_t)(sizeA * 1.25) + 4096;
-        rc |= dalloc(h->a_s, want); rc |= dalloc(h->a_w, want); rc |= dalloc(h->a_gx, want); rc |= dalloc(h->a_gy, want);
-    }
-    if (h->b_s.n < sizeB) {
-        size_t want = (size_t)(sizeB * 1.25) + 4096;
-        rc |= dalloc(h->b_s, want); rc |= dalloc(h->b_gx, want); rc |= dalloc(h->b_gy, want);
-    }
-    if (rc) return rc;
-    Lists Lz{h->a_s.p, h->a_w.p, h->a_gx.p, h->a_gy.p, h->offA.p, h->b_s.p, h->b_gx.p, h->b_gy.p, h->offB.p};
-    LAUNCH(K_BUILD, nblk8(nt2), BLOCK, k_build, C.x.p, C.hsml.p, h->cell.p, h->kstart.p, h->proc.p, h->cnt.p, h->off.p,
-                                       h->grid.p, h->split.p, Lz, h->p.pi, nn, nt, nt2);
-    HIP_TRY(hipGetLastError());
+    std::memcpy(&sh, h->pinned, sizeof(Split));
     h->n_pairs = sh.n;
     h->m_prev = sh.M;
     h->m_pairs = std::max(sh.M, sh.n);
@@ -527,7 +527,8 @@ static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
     StateBufs SB = h->S[h->cur].view();
     Work W{h->vx0.p, h->x0.p, h->vel0.p, h->rk_vel.p, h->rhs2.p, h->f_bound.p, h->art_visc.p,
            h->stress0.p, h->rk_stress.p, h->rhs1.p, h->grad_u.p, h->rk_dev.p, h->ddev.p};
-    Lists Lz{h->a_s.p, h->a_w.p, h->a_gx.p, h->a_gy.p, h->offA.p, h->b_s.p, h->b_gx.p, h->b_gy.p, h->offB.p};
+    const SearchView V{h->cell.p, h->kstart.p, h->proc.p, h->cnt.p, h->off.p, h->grid.p, h->split.p, h->p.pi};
+    const PairLists PL = h->lists();
     const int adapt = h->dp.ncrit == 12;
     const double tanfi = h->dp.tanfi, coh = h->dp.coh;
     const double f1rk[4] = {0., 0.5, 0.5, 1.0}, f2rk[4] = {1., 2., 2., 1.0};
@@ -538,17 +539,23 @@ static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
     for (int st = 0; st < 4; ++st) {
         const double c1 = f1rk[st] * (dt);
         LAUNCH(K_STAGE_BEGIN, nblk(nt), BLOCK, k_stage_begin, SB, W, c1, adapt, tanfi, coh, nn, nt);
-        LAUNCH(K_SPU_NODES, nblk8(nn), BLOCK, k_spu_nodes, SB, W, h->degA.p, Lz, adapt, tanfi, coh, nn, nt);
-        LAUNCH(K_SPU_SPS, nblk8(nt - nn), BLOCK, k_spu_sps, SB, W, h->degA.p, Lz, adapt, tanfi, coh, nn, nt);
-        LAUNCH(K_DERIV_SPS, nblk8(nt - nn), BLOCK, k_deriv_sps, SB, W, h->degA.p, Lz, h->dp, time_sph, f2rk[st], nn, nt);
-        LAUNCH(K_DERIV_NODES, nblk8(nn), BLOCK, k_deriv_nodes, SB, W, h->degA.p, h->degB.p, Lz, h->dp, gxg, gyg, f2rk[st],
-                                                 h->err.p, nn, nt);
+        if (nn > 0)
+            LAUNCH_LDS(K_SPU_NODES, nblk8(nn), TILE, LAY_SPU_NODES.bytes, k_spu_nodes, SB, V, PL, adapt, tanfi, coh, nn);
+        if (nt > nn)
+            LAUNCH_LDS(K_SP_UPDATE, nblk8(nt - nn), TILE, LAY_SP_UPDATE.bytes, k_sp_update, SB, W, V, PL, h->dp, time_sph,
+                       f2rk[st], 1, adapt, nn, nt);
+        if (nn > 0)
+            LAUNCH_LDS(K_DERIV_NODES, nblk8(nn), TILE, LAY_DERIV_NODES.bytes, k_deriv_nodes, SB, W, V, PL, h->dp, gxg, gyg,
+                       f2rk[st], h->err.p, nn);
         if (st == 3 && nt2 > nt)
-            LAUNCH(K_DUMMY_STRESS, nblk(nt2 - nt), BLOCK, k_dummy_stress, SB, h->degA.p, Lz, nt, nt2);
+            LAUNCH(K_DUMMY_STRESS, nblk(nt2 - nt), BLOCK, k_dummy_stress, SB, V, nt, nt2);
     }
     LAUNCH(K_RK_FINAL, nblk(nt), BLOCK, k_rk_final, SB, W, dt, adapt, tanfi, coh, nn, nt);
-    LAUNCH(K_SPU_NODES, nblk8(nn), BLOCK, k_spu_nodes, SB, W, h->degA.p, Lz, adapt, tanfi, coh, nn, nt);
-    LAUNCH(K_SPU_SPS, nblk8(nt - nn), BLOCK, k_spu_sps, SB, W, h->degA.p, Lz, adapt, tanfi, coh, nn, nt);
+    if (nn > 0)
+        LAUNCH_LDS(K_SPU_NODES, nblk8(nn), TILE, LAY_SPU_NODES.bytes, k_spu_nodes, SB, V, PL, adapt, tanfi, coh, nn);
+    if (nt > nn)
+        LAUNCH_LDS(K_SP_UPDATE, nblk8(nt - nn), TILE, LAY_SP_UPDATE.bytes, k_sp_update, SB, W, V, PL, h->dp, time_sph, 0.0,
+                   0, adapt, nn, nt);
     LAUNCH(K_POSITION, nblk(nt), BLOCK, k_position, SB, W, dt, h->p.update_x, nn, nt);
     HIP_TRY(hipGetLastError());
     return 0;
@@ -741,10 +748,9 @@ extern "C" void sps_destroy(sps_handle *h) {
     dfree(h->rk_dev); dfree(h->ddev); dfree(h->grid); dfree(h->split); dfree(h->bparts);
     dfree(h->cellkey); dfree(h->new_cell); dfree(h->cell); dfree(h->kcount); dfree(h->kstart);
     dfree(h->kfill); dfree(h->tmp_orig); dfree(h->tmp_src); dfree(h->proc); dfree(h->proc_to_slot);
-    dfree(h->cnt); dfree(h->off); dfree(h->scan_tmp); dfree(h->widthA); dfree(h->widthB);
-    dfree(h->offA); dfree(h->offB); dfree(h->ctot); dfree(h->cstart_rm); dfree(h->degA); dfree(h->degB); dfree(h->totals); dfree(h->err);
-    dfree(h->a_s); dfree(h->a_w); dfree(h->a_gx); dfree(h->a_gy); dfree(h->b_s); dfree(h->b_gx);
-    dfree(h->b_gy);
+    dfree(h->cnt); dfree(h->off); dfree(h->scan_tmp); dfree(h->ctot); dfree(h->cstart_rm); dfree(h->err);
+    dfree(h->list_ia); dfree(h->list_wa); dfree(h->list_gxa); dfree(h->list_gya); dfree(h->list_ib);
+    dfree(h->list_gxb); dfree(h->list_gyb); dfree(h->deg_a); dfree(h->deg_b); dfree(h->tile_ok);
     if (h->pinned) (void)hipHostFree(h->pinned);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;

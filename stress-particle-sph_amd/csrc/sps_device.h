@@ -64,6 +64,76 @@ __device__ __forceinline__ void cubic_kernel(double r, double dx0, double dx1, d
     gy = (float)g1;
 }
 
+// The factor 15 / (7 pi h^2) of the kernel for the last smoothing length
+// seen: the same operations on the same h, evaluated once per distinct h.
+struct KernelFactor {
+    double h = -1.0, f = 0.0;
+    __device__ __forceinline__ double get(double hs, double pi) {
+        if (!(hs == h)) {
+            h = hs;
+            f = 15.0 / (7.0 * pi * hs * hs);
+        }
+        return f;
+    }
+};
+
+// cubic_kernel with the factor supplied.
+__device__ __forceinline__ void cubic_kernel_f(double r, double dx0, double dx1, double hs, double factor,
+                                               float &w, float &gx, float &gy) {
+    double q = r / hs;
+    double wd, g0, g1;
+    if (q >= 0.0 && q <= 1.0) {
+        wd = factor * ((double)(2.0f / 3.0f) - q * q + (q * (q * q)) / 2.0);
+        double c = factor * (-2.0 + (double)(3.0f / 2.0f) * q) / (hs * hs);
+        g0 = c * dx0;
+        g1 = c * dx1;
+    } else if (q > 1.0 && q <= 2.0) {
+        double t = 2.0 - q;
+        wd = factor * 1.0 / 6.0 * (t * (t * t));
+        double c = factor * 1.0 / 6.0 * 3.0 * (t * t) / hs;
+        g0 = -(c * (dx0 / r));
+        g1 = -(c * (dx1 / r));
+    } else {
+        wd = 0.0;
+        g0 = 0.0;
+        g1 = 0.0;
+    }
+    w = (float)wd;
+    gx = (float)g0;
+    gy = (float)g1;
+}
+
+// The w of cubic_kernel alone, with the same operations.
+__device__ __forceinline__ float cubic_w_f(double r, double hs, double factor) {
+    double q = r / hs;
+    double wd;
+    if (q >= 0.0 && q <= 1.0) {
+        wd = factor * ((double)(2.0f / 3.0f) - q * q + (q * (q * q)) / 2.0);
+    } else if (q > 1.0 && q <= 2.0) {
+        double t = 2.0 - q;
+        wd = factor * 1.0 / 6.0 * (t * (t * t));
+    } else {
+        wd = 0.0;
+    }
+    return (float)wd;
+}
+
+// The w of cubic_kernel alone, with the same operations.
+__device__ __forceinline__ float cubic_w(double r, double hs, double pi) {
+    double q = r / hs;
+    double factor = 15.0 / (7.0 * pi * hs * hs);
+    double wd;
+    if (q >= 0.0 && q <= 1.0) {
+        wd = factor * ((double)(2.0f / 3.0f) - q * q + (q * (q * q)) / 2.0);
+    } else if (q > 1.0 && q <= 2.0) {
+        double t = 2.0 - q;
+        wd = factor * 1.0 / 6.0 * (t * (t * t));
+    } else {
+        wd = 0.0;
+    }
+    return (float)wd;
+}
+
 // Pair acceptance of grid_find_NEW (2_...:1345-1353): r < 2 * (h_i + h_j)/2.
 // Returns r (>= 0) when accepted, -1 otherwise.  Symmetric in i, j.
 // A candidate whose squared distance exceeds (2*mhsml)^2 by a relative

@@ -1,6 +1,6 @@
 // sps_kernels.hip - hand-written HIP kernels (gfx950) of the Stress-Particle
-// SPH hot path: cell binning, ordered neighbour lists, and the per-stage
-// gather kernels of the RK4 step.
+// SPH hot path: cell binning, the pair-order split of the neighbour search,
+// and the per-stage gather kernels of the RK4 step.
 //
 // Storage order.  Per-particle state lives in one set of structure-of-arrays
 // buffers whose index ranges are the reference's (nodes [0,nn), stress
@@ -17,16 +17,17 @@
 // before the search and n the pairs found, pairs d >= M come first in
 // descending d, then pairs d < M ascending.  For a receiver, ascending d is
 // the order of its partners in (cell, kind, original index) -- the forward
-// stencil walk -- so the kernels emit each receiver's partners with a reverse
-// walk (pairs d >= M) followed by a forward walk (pairs d < M), and the
-// stage kernels add them one by one in that order: one lane per receiver,
-// no atomics, results bitwise equal to the reference.
+// stencil walk -- so each stage kernel visits a receiver's partners with a
+// reverse walk (pairs d >= M) followed by a forward walk (pairs d < M) and
+// adds them one by one in that order, recomputing the single-precision w
+// and grad w of each pair (positions are fixed between the search and the
+// end of the step): one lane per receiver, no atomics, no stored pair list,
+// results bitwise equal to the reference.
 #include "sps_device.h"
 
 namespace sps {
 
 constexpr int BLOCK = 256;
-constexpr int SLICE = 64;   // receivers per sliced-ELL slice (one wave)
 
 // Grid of the current search (device-resident).
 struct Grid {
@@ -42,6 +43,8 @@ struct Split {
     unsigned long long M;       // high-water mark before this search
     int sstar;                  // processing slot holding discovery index M, or -1
     int pstar;                  // processing slot of its first reversed partner
+    int first_rev;              // smallest slot discovering a pair d >= M (INT_MAX: none)
+    int pad_;
     unsigned long long n_by_type[10];
 };
 
@@ -144,7 +147,7 @@ __global__ void __launch_bounds__(BLOCK) k_bounds_final(const double *__restrict
 // storage (and so the receivers) up a strip keeps the partner data of three
 // consecutive rows in one L2.  The reference's own visiting order (row-major
 // cells, 2_...:1322-1324) is kept separately for the list order (k_proc_slot).
-constexpr int STRIP = 64;
+constexpr int STRIP = 8;
 
 __device__ __forceinline__ int cell_key(int cx, int cy, const Grid &g) {
     int x0 = (cx / STRIP) * STRIP;
@@ -321,14 +324,6 @@ __device__ __forceinline__ Stencil make_stencil(int key, const Grid &g) {
     return S;
 }
 
-// Relevance of a partner kind for the receiver's lists:
-//  list A: node <- stress points (type 1) and dummies (type 6);
-//          stress point <- nodes (type 1) and dummies (type 9);
-//          dummy <- nodes (type 6, for the dummy stress copy of 2_...:580).
-//  list B: node <- nodes (type 3, artificial viscosity).
-__device__ __forceinline__ bool in_list_a(int kr, int ks) {
-    return kr == KIND_NODE ? ks != KIND_NODE : (kr == KIND_SP ? ks != KIND_SP : ks == KIND_NODE);
-}
 
 // pint_type of a pair from the kinds of its ends (3_...:1598-1608).
 __device__ __forceinline__ int pair_type(int ka, int kb) {
@@ -345,18 +340,19 @@ __device__ __forceinline__ int pair_type(int ka, int kb) {
 
 // ------------------------------------------------------------------- count
 // For every receiver: pairs it discovers (partners later in processing
-// order; the reference counts n_pairs with these), and its list degrees.
+// order; the reference counts n_pairs with these), by pint_type.  The node
+// and stress-point receivers are counted by the tiled k_count_lists below.
+// (Slots [r0, r1), global storage; the dummy receivers.)
 __global__ void k_count(const double2 *__restrict__ x, const double *__restrict__ hsml,
                         const int *__restrict__ cell, const int *__restrict__ kstart,
                         const int *__restrict__ proc, const Grid *__restrict__ g,
-                        unsigned *__restrict__ cnt, int *__restrict__ degA, int *__restrict__ degB,
-                        Split *__restrict__ split, int nn, int nt, int nt2) {
-    int r = xcd_block() * blockDim.x + threadIdx.x;
+                        unsigned *__restrict__ cnt, Split *__restrict__ split, int nn, int nt,
+                        int r0, int r1) {
+    int r = r0 + xcd_block() * blockDim.x + threadIdx.x;
     unsigned fwd = 0;
-    int da = 0, db = 0;
     unsigned t0 = 0, t1 = 0, t2 = 0, t3 = 0, t6 = 0, t9 = 0;   // pairs by pint_type
     int ndivt = g->ndivt;
-    if (r < nt2 && cell[r] < ndivt) {
+    if (r < r1 && cell[r] < ndivt) {
         const int L = ndivt + 1;
         const Grid gg = *g;
         int kr = kind_of(r, nn, nt);
@@ -387,17 +383,11 @@ __global__ void k_count(const double2 *__restrict__ x, const double *__restrict_
                                 default: t0++; break;
                             }
                         }
-                        if (in_list_a(kr, ks)) da++;
-                        if (kr == KIND_NODE && ks == KIND_NODE) db++;
                     }
                 }
             }
         }
         cnt[proc[r]] = fwd;
-    }
-    if (r < nt2) {
-        degA[r] = da;
-        if (r < nn) degB[r] = db;
     }
     // per-type histogram: wave reduction then one atomic per wave
     unsigned tv[6] = {t0, t1, t2, t3, t6, t9};
@@ -410,14 +400,6 @@ __global__ void k_count(const double2 *__restrict__ x, const double *__restrict_
     }
 }
 
-// Widest row of each 64-receiver slice.
-__global__ void k_slice_width(const int *__restrict__ deg, int n, unsigned *__restrict__ width) {
-    int r = blockIdx.x * blockDim.x + threadIdx.x;
-    int v = r < n ? deg[r] : 0;
-    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_down(v, off, 64));
-    if ((threadIdx.x & 63) == 0 && (r >> 6) < (n + 63) / 64) width[r >> 6] = (unsigned)v * SLICE;
-}
-
 // Fresh split record: M is the list high-water mark before this search.
 __global__ void k_split_reset(Split *__restrict__ split, unsigned long long M) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
@@ -425,6 +407,7 @@ __global__ void k_split_reset(Split *__restrict__ split, unsigned long long M) {
     split->M = M;
     split->sstar = -1;
     split->pstar = -1;
+    split->first_rev = 0x7fffffff;
     for (int k = 0; k < 10; ++k) split->n_by_type[k] = 0;
 }
 
@@ -437,6 +420,11 @@ __global__ void k_split(const unsigned *__restrict__ cnt, const unsigned *__rest
     unsigned long long M = split->M;
     unsigned long long o = off[p], c = cnt[p];
     if (o < M && M < o + c) split->sstar = p;
+    // a slot below first_rev discovers only pairs d < M; the wave's lowest
+    // qualifying slot is its first active lane (slots ascend with the lane)
+    const bool q = o + c > M;
+    const unsigned long long b = __ballot(q);
+    if (b && (int)(threadIdx.x & 63) == __ffsll((long long)b) - 1) atomicMin(&split->first_rev, p);
 }
 
 // First reversed partner of the split slot: its (M - off)-th discovery in
@@ -483,99 +471,436 @@ __global__ void k_proc_to_slot(const int *__restrict__ proc, int *__restrict__ p
     if (s < nt2 && proc[s] >= 0) proc_to_slot[proc[s]] = s;
 }
 
-// ------------------------------------------------------------------- build
-// Emit each receiver's partners in reference list order into the sliced-ELL
-// lists, with w and d W / d x_receiver rounded to single precision.
-struct Lists {
-    int *a_s;
-    float *a_w, *a_gx, *a_gy;
-    const unsigned *a_off;
-    int *b_s;
-    float *b_gx, *b_gy;
-    const unsigned *b_off;
-};
-
+// ---------------------------------------------------------- ordered walk
+// Which side of the high-water mark M a pair's discovery index falls on:
+// the discoverer is the end visited first (smaller processing slot); only
+// the split slot straddles M, and its partners from pstar on are reversed.
 __device__ __forceinline__ bool pair_reversed(int pr, int ps, const unsigned *__restrict__ cnt,
                                               const unsigned *__restrict__ off,
                                               const Split &sp) {
-    // discoverer: the end visited first (smaller processing slot)
     int pd = pr < ps ? pr : ps, po = pr < ps ? ps : pr;
     unsigned long long o = off[pd], c = cnt[pd];
     if (o >= sp.M) return true;
     if (o + c <= sp.M) return false;
-    return po >= sp.pstar;   // pd is the split slot
+    return po >= sp.pstar;
 }
 
-__global__ void k_build(const double2 *__restrict__ x, const double *__restrict__ hsml,
-                        const int *__restrict__ cell, const int *__restrict__ kstart,
-                        const int *__restrict__ proc, const unsigned *__restrict__ cnt,
-                        const unsigned *__restrict__ off, const Grid *__restrict__ g,
-                        const Split *__restrict__ split, Lists Lz, double pi, int nn, int nt,
-                        int nt2) {
-    int r = xcd_block() * blockDim.x + threadIdx.x;
-    if (r >= nt2) return;
-    int ndivt = g->ndivt;
-    int c = cell[r];
-    if (c >= ndivt) return;
-    const int L = ndivt + 1;
-    const Split sp = *split;
-    const bool any_rev = sp.n > sp.M;             // some pairs have d >= M
-    const bool any_fwd = sp.M > 0;                // some pairs have d < M
-    const bool mixed = any_rev && any_fwd;
-    int kr = kind_of(r, nn, nt);
-    int pr = proc[r];
-    double2 xr = x[r];
-    double hr = hsml[r];
-    const Grid gg = *g;
-    Stencil S = make_stencil(c, gg);
-    unsigned base_a = Lz.a_off[r >> 6] + (r & 63);
-    unsigned base_b = r < nn ? Lz.b_off[r >> 6] + (r & 63) : 0;
-    int na = 0, nb = 0;
-    const int nx = S.x1 - S.x0 + 1, ny = S.y1 - S.y0 + 1;
-    for (int pass = 0; pass < 2; ++pass) {
-        const bool rev = pass == 0;
-        if (rev && !any_rev) continue;
-        if (!rev && !any_fwd) continue;
-        for (int ci = 0; ci < nx * ny; ++ci) {
-            // cells in ascending (forward) or descending (reverse) row-major order
-            const int cj = rev ? nx * ny - 1 - ci : ci;
-            const int y = S.y0 + cj / nx, xx = S.x0 + cj % nx;
-            const int key = cell_key(xx, y, gg);
-            for (int ki = 0; ki < 3; ++ki) {
-                int ks = rev ? 2 - ki : ki;
-                bool la = in_list_a(kr, ks), lb = kr == KIND_NODE && ks == KIND_NODE;
-                if (!la && !lb) continue;
-                int lo = kstart[ks * L + key], hi = kstart[ks * L + key + 1];
-                int cntk = hi - lo;
-                for (int q = 0; q < cntk; ++q) {
-                    int s = rev ? hi - 1 - q : lo + q;
-                    if (s == r) continue;
-                    double2 xs = x[s];
-                    double dx0 = xr.x - xs.x, dx1 = xr.y - xs.y;
-                    double mh;
-                    double rr = pair_distance(dx0, dx1, hr, hsml[s], mh);
-                    if (rr < 0.0) continue;
-                    if (mixed && pair_reversed(pr, proc[s], cnt, off, sp) != rev) continue;
-                    float w, gx, gy;
-                    cubic_kernel(rr, dx0, dx1, mh, pi, w, gx, gy);
-                    if (la) {
-                        unsigned e = base_a + (unsigned)na * SLICE;
-                        Lz.a_s[e] = s;
-                        Lz.a_w[e] = w;
-                        Lz.a_gx[e] = gx;
-                        Lz.a_gy[e] = gy;
-                        na++;
-                    } else {
-                        unsigned e = base_b + (unsigned)nb * SLICE;
-                        Lz.b_s[e] = s;
-                        Lz.b_gx[e] = gx;
-                        Lz.b_gy[e] = gy;
-                        nb++;
-                    }
-                }
-            }
+// Everything a stage kernel needs to re-find the pairs of the step's search.
+// A receiver whose processing slot is below first_rev has only pairs d < M
+// (every pair's discoverer is at or below the receiver's slot), so it needs
+// the forward walk alone and no per-pair classification.
+struct SearchView {
+    const int *cell, *kstart, *proc;
+    const unsigned *cnt, *off;
+    const Grid *g;
+    const Split *split;
+    double pi;
+};
+
+struct WalkCtx {
+    Grid g;
+    Split sp;
+    int L;
+    bool any_rev, any_fwd, mixed;
+};
+
+__device__ __forceinline__ WalkCtx walk_ctx(const SearchView &V) {
+    WalkCtx w;
+    w.g = *V.g;
+    w.sp = *V.split;
+    w.L = w.g.ndivt + 1;
+    w.any_rev = w.sp.n > w.sp.M;    // some pairs have d >= M (listed first, descending d)
+    w.any_fwd = w.sp.M > 0;         // some pairs have d < M (listed after, ascending d)
+    w.mixed = w.any_rev && w.any_fwd;
+    return w;
+}
+
+constexpr int MASK_NODE = 1 << KIND_NODE, MASK_SP = 1 << KIND_SP, MASK_DUMMY = 1 << KIND_DUMMY;
+
+template <int K> struct KindTag {
+    static constexpr int value = K;
+};
+
+// Where the partners of kind K in cell (gx, gy) are: global storage slots.
+struct GlobalCells {
+    const int *kstart;
+    Grid g;
+    int L;
+    __device__ __forceinline__ int2 range(int kind, int gx, int gy) const {
+        const int key = cell_key(gx, gy, g);
+        const int lo = kstart[kind * L + key];
+        return make_int2(lo, kstart[kind * L + key + 1] - lo);
+    }
+};
+
+// Partner fields from the global state (j is a storage slot).
+struct GlobalFields {
+    StateBufs S;
+    __device__ __forceinline__ double2 xk(int, int j) const { return S.x[j]; }
+    __device__ __forceinline__ double hk(int, int j) const { return S.hsml[j]; }
+    template <int K> __device__ __forceinline__ double2 x(KindTag<K>, int j) const { return S.x[j]; }
+    template <int K> __device__ __forceinline__ double h(KindTag<K>, int j) const { return S.hsml[j]; }
+    template <int K> __device__ __forceinline__ double m(KindTag<K>, int j) const { return S.mass[j]; }
+    template <int K> __device__ __forceinline__ double rho(KindTag<K>, int j) const { return S.rho[j]; }
+    template <int K> __device__ __forceinline__ double2 vel(KindTag<K>, int j) const { return S.vel[j]; }
+    template <int K> __device__ __forceinline__ double4 st(KindTag<K>, int j) const { return S.stress[j]; }
+    template <int K> __device__ __forceinline__ double iv(KindTag<K>, int j) const { return S.ivar1[j]; }
+    template <int K> __device__ __forceinline__ double mr(KindTag<K>, int j) const { return S.mass[j] / S.rho[j]; }
+    template <int K> __device__ __forceinline__ double3 sr2(KindTag<K>, int j) const {
+        const double rs = S.rho[j], rs2 = rs * rs;
+        const double4 st = S.stress[j];
+        return make_double3(st.x / rs2, st.y / rs2, st.z / rs2);
+    }
+    template <int K> __device__ __forceinline__ float wall(KindTag<K>, int j) const { return S.wallpos[j]; }
+    template <int K> __device__ __forceinline__ float horiz(KindTag<K>, int j) const { return S.horiz[j]; }
+};
+
+// --------------------------------------------------------------- LDS tiles
+// A tiled stage kernel gives each workgroup TILE consecutive receivers of one
+// kind.  In storage they are a run of cells of one strip, rows cy_first ..
+// cy_last; the union of their 3x3 stencils is the window: those rows +-1,
+// the strip's columns and one halo column on each side.  Per partner kind
+// the strip part of the window is ONE contiguous storage run (rows follow
+// each other inside a strip) and each halo cell a short run, so staging is
+// a coalesced copy.  A table gives every window cell its LDS run.  A tile
+// whose receivers span two strips or too many rows, whose window overflows
+// the LDS capacity, or that holds a receiver with pairs on both sides of M
+// walks the global storage instead; both paths visit the same pairs in the
+// same order.
+constexpr int TILE = 256;
+constexpr int TROWS = 24;              // most window rows staged
+constexpr int TCOLS = STRIP + 2;
+constexpr int TCELLS = TROWS * TCOLS;
+constexpr int CAP = 640;               // staged partners of a main kind per window
+constexpr int CAP_D = 192;             // staged dummies per window
+
+// Dynamic LDS of the tiled stage kernels.  A kernel's layout gives the byte
+// offset of every field it stages per partner kind (-1: not staged) and of
+// the per-kind window-cell tables; all offsets are compile-time constants,
+// so every access is a direct LDS access.
+extern __shared__ __align__(16) unsigned char sps_lds[];
+
+template <typename T> __device__ __forceinline__ T *lds_at(int off) {
+    return reinterpret_cast<T *>(sps_lds + off);
+}
+
+struct LdsLayout {
+    int x[3], v[3], h[3], m[3], rho[3], s0[3], s1[3], s2[3], s3[3], iv[3], tab[3];
+    int mr[3], p0[3], p1[3], p2[3];   // m / rho and sigma / rho^2 (xx, yy, xy) of a partner
+    int wall, horiz, bytes;
+};
+
+constexpr LdsLayout layout_none() {
+    LdsLayout L{};
+    for (int k = 0; k < 3; ++k)
+        L.x[k] = L.v[k] = L.h[k] = L.m[k] = L.rho[k] = L.s0[k] = L.s1[k] = L.s2[k] = L.s3[k] = L.iv[k] =
+            L.tab[k] = L.mr[k] = L.p0[k] = L.p1[k] = L.p2[k] = -1;
+    L.wall = L.horiz = -1;
+    L.bytes = 0;
+    return L;
+}
+
+// count / list build: positions and smoothing lengths of all three kinds and
+// the window-cell tables
+constexpr LdsLayout layout_count() {
+    LdsLayout L = layout_none();
+    int o = 0;
+    for (int k = 0; k < 3; ++k) {
+        const int cap = k == KIND_DUMMY ? CAP_D : CAP;
+        L.x[k] = o; o += 16 * cap;
+        L.h[k] = o; o += 8 * cap;
+    }
+    for (int k = 0; k < 3; ++k) {
+        L.tab[k] = o;
+        o += 4 * TCELLS;
+    }
+    L.bytes = o;
+    return L;
+}
+
+// The stage kernels read w and grad w from the pair lists and only the
+// fields they sum from the window.
+// node receivers <- stress points: m, rho, stress (4), plastic strain
+constexpr LdsLayout layout_spu_nodes() {
+    LdsLayout L = layout_none();
+    int o = 0;
+    L.m[KIND_SP] = o; o += 8 * CAP;
+    L.rho[KIND_SP] = o; o += 8 * CAP;
+    L.s0[KIND_SP] = o; o += 8 * CAP;
+    L.s1[KIND_SP] = o; o += 8 * CAP;
+    L.s2[KIND_SP] = o; o += 8 * CAP;
+    L.s3[KIND_SP] = o; o += 8 * CAP;
+    L.iv[KIND_SP] = o; o += 8 * CAP;
+    L.bytes = o;
+    return L;
+}
+
+// stress-point receivers <- nodes (x, v, m, rho, m / rho) and dummies (x, m,
+// rho, wall position and orientation)
+constexpr LdsLayout layout_sp_update() {
+    LdsLayout L = layout_none();
+    int o = 0;
+    L.x[KIND_NODE] = o; o += 16 * CAP;
+    L.v[KIND_NODE] = o; o += 16 * CAP;
+    L.m[KIND_NODE] = o; o += 8 * CAP;
+    L.rho[KIND_NODE] = o; o += 8 * CAP;
+    L.mr[KIND_NODE] = o; o += 8 * CAP;
+    L.x[KIND_DUMMY] = o; o += 16 * CAP_D;
+    L.m[KIND_DUMMY] = o; o += 8 * CAP_D;
+    L.rho[KIND_DUMMY] = o; o += 8 * CAP_D;
+    L.wall = o; o += 4 * CAP_D;
+    L.horiz = o; o += 4 * CAP_D;
+    L.bytes = o;
+    return L;
+}
+
+// node receivers, phase 1 <- stress points (x, m, rho, sigma / rho^2 of xx
+// yy xy) and dummies (m, rho); phase 2 <- nodes (x, v, h, m, rho) over the
+// same bytes
+constexpr LdsLayout layout_deriv_nodes() {
+    LdsLayout L = layout_none();
+    int o = 0;
+    L.x[KIND_SP] = o; o += 16 * CAP;
+    L.m[KIND_SP] = o; o += 8 * CAP;
+    L.rho[KIND_SP] = o; o += 8 * CAP;
+    L.p0[KIND_SP] = o; o += 8 * CAP;
+    L.p1[KIND_SP] = o; o += 8 * CAP;
+    L.p2[KIND_SP] = o; o += 8 * CAP;
+    L.m[KIND_DUMMY] = o; o += 8 * CAP_D;
+    L.rho[KIND_DUMMY] = o; o += 8 * CAP_D;
+    const int phase1 = o;
+    o = 0;
+    L.x[KIND_NODE] = o; o += 16 * CAP;
+    L.v[KIND_NODE] = o; o += 16 * CAP;
+    L.h[KIND_NODE] = o; o += 8 * CAP;
+    L.m[KIND_NODE] = o; o += 8 * CAP;
+    L.rho[KIND_NODE] = o; o += 8 * CAP;
+    L.bytes = o > phase1 ? o : phase1;
+    return L;
+}
+
+constexpr LdsLayout LAY_COUNT = layout_count();
+constexpr LdsLayout LAY_SPU_NODES = layout_spu_nodes();
+constexpr LdsLayout LAY_SP_UPDATE = layout_sp_update();
+constexpr LdsLayout LAY_DERIV_NODES = layout_deriv_nodes();
+
+__device__ __forceinline__ constexpr int sel3(int kind, int a, int b, int c) {
+    return kind == 0 ? a : (kind == 1 ? b : c);
+}
+
+// Window cells of the staged tile: per kind, start | count << 16.
+template <const LdsLayout &LY> struct LdsCells {
+    int gx0, gy0;   // grid cell of window cell (0, 0)
+    __device__ __forceinline__ int2 range(int kind, int gx, int gy) const {
+        const int e = lds_at<int>(sel3(kind, LY.tab[0], LY.tab[1], LY.tab[2]))[(gy - gy0) * TCOLS + (gx - gx0)];
+        return make_int2(e & 0xffff, e >> 16);
+    }
+};
+
+// Partner fields of the staged window (j is an LDS index of kind K).
+template <const LdsLayout &LY> struct LdsFields {
+    __device__ __forceinline__ double2 xk(int kind, int j) const {
+        return lds_at<double2>(sel3(kind, LY.x[0], LY.x[1], LY.x[2]))[j];
+    }
+    __device__ __forceinline__ double hk(int kind, int j) const {
+        return lds_at<double>(sel3(kind, LY.h[0], LY.h[1], LY.h[2]))[j];
+    }
+    template <int K> __device__ __forceinline__ double2 x(KindTag<K>, int j) const {
+        return lds_at<double2>(LY.x[K])[j];
+    }
+    template <int K> __device__ __forceinline__ double h(KindTag<K>, int j) const {
+        return lds_at<double>(LY.h[K])[j];
+    }
+    template <int K> __device__ __forceinline__ double m(KindTag<K>, int j) const {
+        return lds_at<double>(LY.m[K])[j];
+    }
+    template <int K> __device__ __forceinline__ double rho(KindTag<K>, int j) const {
+        return lds_at<double>(LY.rho[K])[j];
+    }
+    template <int K> __device__ __forceinline__ double2 vel(KindTag<K>, int j) const {
+        return lds_at<double2>(LY.v[K])[j];
+    }
+    template <int K> __device__ __forceinline__ double4 st(KindTag<K>, int j) const {
+        const double w = LY.s3[K] >= 0 ? lds_at<double>(LY.s3[K])[j] : 0.0;
+        return make_double4(lds_at<double>(LY.s0[K])[j], lds_at<double>(LY.s1[K])[j],
+                            lds_at<double>(LY.s2[K])[j], w);
+    }
+    template <int K> __device__ __forceinline__ double iv(KindTag<K>, int j) const {
+        return lds_at<double>(LY.iv[K])[j];
+    }
+    template <int K> __device__ __forceinline__ double mr(KindTag<K>, int j) const {
+        return lds_at<double>(LY.mr[K])[j];
+    }
+    template <int K> __device__ __forceinline__ double3 sr2(KindTag<K>, int j) const {
+        return make_double3(lds_at<double>(LY.p0[K])[j], lds_at<double>(LY.p1[K])[j], lds_at<double>(LY.p2[K])[j]);
+    }
+    template <int K> __device__ __forceinline__ float wall(KindTag<K>, int j) const {
+        return lds_at<float>(LY.wall)[j];
+    }
+    template <int K> __device__ __forceinline__ float horiz(KindTag<K>, int j) const {
+        return lds_at<float>(LY.horiz)[j];
+    }
+};
+
+// Pass classification of a lane that may hold pairs on both sides of M.
+struct NoClass {
+    __device__ __forceinline__ bool operator()(int, bool) const { return true; }
+};
+struct SplitClass {
+    bool on;
+    int pr;
+    const int *proc;
+    const unsigned *cnt, *off;
+    Split sp;
+    __device__ __forceinline__ bool operator()(int j, bool rev) const {
+        return !on || pair_reversed(pr, proc[j], cnt, off, sp) == rev;
+    }
+};
+
+// One run of a receiver's walk: the partners of kind K in cell (gx, gy),
+// slots ascending (forward) or descending (reverse).  For each accepted pair
+// -- the acceptance test and dx = x_r - x_s of grid_find_NEW (2_...:1345) --
+// f(kind_tag, fields, j, w, dW/dx_r, dW/dy_r) with kernel()'s values rounded
+// to single precision as the reference's pair list stores them.
+template <int KMASK, int K, bool REV, typename Cells, typename Fields, typename Cls, typename F>
+__device__ __forceinline__ void walk_run(const Cells &Cs, const Fields &A, int gx, int gy, double2 xr,
+                                         double hr, int kr, int self, double pi, KernelFactor &kf,
+                                         const Cls &cls, F &f) {
+    if constexpr (((KMASK >> K) & 1) != 0) {
+        const int2 rg = Cs.range(K, gx, gy);
+        for (int q = 0; q < rg.y; ++q) {
+            const int j = REV ? rg.x + rg.y - 1 - q : rg.x + q;
+            if (K == kr && j == self) continue;
+            const double2 xs = A.xk(K, j);
+            const double dx0 = xr.x - xs.x, dx1 = xr.y - xs.y;
+            double mh;
+            const double rr = pair_distance(dx0, dx1, hr, A.hk(K, j), mh);
+            if (rr < 0.0) continue;
+            if (!cls(j, REV)) continue;
+            float w, gwx, gwy;
+            cubic_kernel_f(rr, dx0, dx1, mh, kf.get(mh, pi), w, gwx, gwy);
+            f(KindTag<K>{}, A, j, w, gwx, gwy);
         }
     }
+}
+
+// Every pair of the receiver in cell (cx, cy) with a partner of a kind in
+// KMASK, in the order the reference's linked list holds them: the reverse
+// stencil walk (cells descending row-major, kinds and slots descending) for
+// pairs with d >= M, then the forward walk (row-major cells, kinds, original
+// index) for pairs with d < M.
+template <int KMASK, typename Cells, typename Fields, typename Cls, typename F>
+__device__ __forceinline__ void walk_stencil(const Cells &Cs, const Fields &A, const Grid &g, int cx, int cy,
+                                             double2 xr, double hr, int kr, int self, bool do_rev,
+                                             bool do_fwd, double pi, const Cls &cls, F &&f) {
+    KernelFactor kf;
+    const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx + 1 < g.ndivx ? cx + 1 : g.ndivx - 1;
+    const int y0 = cy > 0 ? cy - 1 : 0, y1 = cy + 1 < g.ndivy ? cy + 1 : g.ndivy - 1;
+    if (do_rev)
+        for (int y = y1; y >= y0; --y)
+            for (int xx = x1; xx >= x0; --xx) {
+                walk_run<KMASK, 2, true>(Cs, A, xx, y, xr, hr, kr, self, pi, kf, cls, f);
+                walk_run<KMASK, 1, true>(Cs, A, xx, y, xr, hr, kr, self, pi, kf, cls, f);
+                walk_run<KMASK, 0, true>(Cs, A, xx, y, xr, hr, kr, self, pi, kf, cls, f);
+            }
+    if (do_fwd)
+        for (int y = y0; y <= y1; ++y)
+            for (int xx = x0; xx <= x1; ++xx) {
+                walk_run<KMASK, 0, false>(Cs, A, xx, y, xr, hr, kr, self, pi, kf, cls, f);
+                walk_run<KMASK, 1, false>(Cs, A, xx, y, xr, hr, kr, self, pi, kf, cls, f);
+                walk_run<KMASK, 2, false>(Cs, A, xx, y, xr, hr, kr, self, pi, kf, cls, f);
+            }
+}
+
+// One workgroup's receivers and window.
+struct Tile {
+    int first, end, last;   // receivers [first, end); [first, last] are in the domain
+    int x0, w, wy0, nrows;  // window: strip columns [x0, x0 + w) +- 1, rows [wy0, wy0 + nrows)
+    bool fits;              // one strip, at most TROWS rows
+};
+
+__device__ __forceinline__ Tile make_tile(int base, int count, int kr, const SearchView &V, const WalkCtx &C) {
+    Tile T;
+    T.first = base + xcd_block() * TILE;
+    T.end = min(T.first + TILE, base + count);
+    T.last = min(T.end, V.kstart[kr * C.L + C.g.ndivt]) - 1;
+    T.fits = false;
+    T.x0 = T.w = T.wy0 = T.nrows = 0;
+    if (T.last < T.first) return T;
+    int cxf, cyf, cxl, cyl;
+    cell_xy(V.cell[T.first], C.g, cxf, cyf);
+    cell_xy(V.cell[T.last], C.g, cxl, cyl);
+    T.x0 = (cxf / STRIP) * STRIP;
+    T.w = C.g.ndivx - T.x0 < STRIP ? C.g.ndivx - T.x0 : STRIP;
+    T.wy0 = cyf > 0 ? cyf - 1 : 0;
+    const int wy1 = cyl + 1 < C.g.ndivy ? cyl + 1 : C.g.ndivy - 1;
+    T.nrows = wy1 - T.wy0 + 1;
+    T.fits = (cxl / STRIP) * STRIP == T.x0 && T.nrows <= TROWS;
+    return T;
+}
+
+// LDS bookkeeping of the staging (halo cells: left column rows, then right).
+struct StageScratch {
+    int hglo[2 * TROWS], hpos[2 * TROWS], hcnt[2 * TROWS];
+    int total, mainlo, mainn;
+};
+
+// Stage the window's partners of kind K: LDS table of the window cells (when
+// tab is given) and load(slot, lds_index) for every partner.  Returns false on every thread
+// when they exceed cap (nothing usable is staged then).  Ends with a barrier.
+template <int K, typename Load>
+__device__ __forceinline__ bool stage_kind(const Tile &T, const SearchView &V, const WalkCtx &C, int cap,
+                                           int *__restrict__ tab, StageScratch &sc, Load &&load) {
+    const int t = threadIdx.x;
+    const int *ks = V.kstart + K * C.L;
+    const int kbase = T.x0 * C.g.ndivy;
+    if (t < 64) {
+        int n = 0, lo = 0;
+        const int side = t / T.nrows, ly = t - side * T.nrows;
+        if (t < 2 * T.nrows) {
+            const int gx = side ? T.x0 + T.w : T.x0 - 1;
+            if (gx >= 0 && gx < C.g.ndivx) {
+                const int key = cell_key(gx, T.wy0 + ly, C.g);
+                lo = ks[key];
+                n = ks[key + 1] - lo;
+            }
+        }
+        int incl = n;
+        for (int o = 1; o < 64; o <<= 1) {
+            int v = __shfl_up(incl, o, 64);
+            if (t >= o) incl += v;
+        }
+        const int mainlo = ks[kbase + T.wy0 * T.w];
+        const int mainn = ks[kbase + (T.wy0 + T.nrows) * T.w] - mainlo;
+        if (t < 2 * T.nrows) {
+            sc.hglo[t] = lo;
+            sc.hpos[t] = mainn + incl - n;
+            sc.hcnt[t] = n;
+            if (tab) tab[ly * TCOLS + (side ? T.w + 1 : 0)] = (mainn + incl - n) | (n << 16);
+        }
+        if (t == 63) {
+            sc.total = mainn + incl;
+            sc.mainlo = mainlo;
+            sc.mainn = mainn;
+        }
+    }
+    __syncthreads();
+    const int total = sc.total, mainlo = sc.mainlo, mainn = sc.mainn;
+    if (total > cap) {
+        __syncthreads();
+        return false;
+    }
+    for (int c = t; tab && c < T.nrows * T.w; c += blockDim.x) {
+        const int ly = c / T.w, lx = c - ly * T.w;
+        const int key = kbase + (T.wy0 + ly) * T.w + lx;
+        const int lo = ks[key];
+        tab[ly * TCOLS + lx + 1] = (lo - mainlo) | ((ks[key + 1] - lo) << 16);
+    }
+    for (int i = t; i < mainn; i += blockDim.x) load(mainlo + i, i);
+    if (t < 2 * T.nrows)
+        for (int q = 0; q < sc.hcnt[t]; ++q) load(sc.hglo[t] + q, sc.hpos[t] + q);
+    __syncthreads();
+    return true;
 }
 
 // ------------------------------------------------------------ step kernels
@@ -628,29 +953,320 @@ __global__ void k_stage_begin(StateBufs S, Work W, double c1, int adapt, double 
     }
 }
 
+// ------------------------------------------------------------ pair lists
+// The pairs the stage kernels need, found once per step by k_count_lists:
+// for every receiver of a tile whose window fits, its accepted partners in
+// forward walk order as 16-bit entries kind << 14 | window index (the window
+// laid out exactly as the stage kernels stage it), with w and grad w rounded
+// to single precision as the reference's pair list stores them
+// (2_...:1376-1378).  The reverse walk order is the forward list read
+// backwards.  List A: a node's stress-point and dummy partners, a stress
+// point's node and dummy partners; list B: a node's node partners.  Entries
+// come four to a word, words of a tile lane-interleaved, so a wave reads its
+// lanes' next four entries with one coalesced load per array.
+constexpr int LMAX = 48;            // entries per receiver and list; more marks the tile unlisted
+constexpr int LGROUPS = LMAX / 4;
+
+struct PairLists {
+    unsigned long long *ia, *ib;    // [tile][LGROUPS][TILE]: four entries per word
+    float4 *wa, *gxa, *gya;         // list A: w, dW/dx_r, dW/dy_r of those entries
+    float4 *gxb, *gyb;              // list B: dW/dx_r, dW/dy_r
+    unsigned short *deg_a, *deg_b;  // per slot: nodes and stress points / nodes
+    unsigned char *ok;              // per tile (node tiles, then SP tiles): lists usable
+    int ntile_n;                    // node tiles
+};
+
+__device__ __forceinline__ size_t word_at(int tile, int g) {
+    return ((size_t)tile * LGROUPS + g) * TILE + threadIdx.x;
+}
+
+__device__ __forceinline__ void set4(float4 &v, int k, float a) {
+    if (k == 0) v.x = a;
+    else if (k == 1) v.y = a;
+    else if (k == 2) v.z = a;
+    else v.w = a;
+}
+
+__device__ __forceinline__ float get4(const float4 &v, int k) {
+    return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w));
+}
+
+// One lane's list being written: four entries gathered in registers, then
+// stored as whole words.
+struct ListWriter {
+    unsigned long long idx;
+    float4 w, gx, gy;
+    int n;
+    __device__ __forceinline__ ListWriter() : idx(0), n(0) {
+        w = gx = gy = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    __device__ __forceinline__ void flush(int tile, unsigned long long *ip, float4 *wp, float4 *gxp,
+                                          float4 *gyp) {
+        const size_t o = word_at(tile, n >> 2);
+        ip[o] = idx;
+        if (wp) wp[o] = w;
+        gxp[o] = gx;
+        gyp[o] = gy;
+    }
+    __device__ __forceinline__ void put(int tile, unsigned long long *ip, float4 *wp, float4 *gxp, float4 *gyp,
+                                        unsigned e, float wv, float gxv, float gyv) {
+        const int k = n & 3;
+        if (k == 0) idx = 0;
+        idx |= (unsigned long long)e << (16 * k);
+        set4(w, k, wv);
+        set4(gx, k, gxv);
+        set4(gy, k, gyv);
+        if (k == 3 && n < LMAX) flush(tile, ip, wp, gxp, gyp);
+        ++n;
+    }
+    __device__ __forceinline__ void finish(int tile, unsigned long long *ip, float4 *wp, float4 *gxp,
+                                           float4 *gyp) {
+        if ((n & 3) != 0 && n < LMAX) flush(tile, ip, wp, gxp, gyp);
+    }
+};
+
+// Positions and smoothing lengths from the global state (j is a slot).
+struct XHFields {
+    const double2 *x;
+    const double *h;
+    __device__ __forceinline__ double2 xk(int, int j) const { return x[j]; }
+    __device__ __forceinline__ double hk(int, int j) const { return h[j]; }
+};
+
+__device__ __forceinline__ bool in_list_a(int kr, int ks) {
+    return kr == KIND_NODE ? ks != KIND_NODE : ks != KIND_SP;
+}
+
+// Counting of k_count for the node (KR = 0) or stress-point (KR = 1)
+// receivers [base, base + count), tiled as the stage kernels are, and the
+// tile's pair lists.
+template <int KR>
+__global__ __launch_bounds__(TILE) void k_count_lists(SearchView V, const double2 *__restrict__ x,
+                                                      const double *__restrict__ hsml,
+                                                      unsigned *__restrict__ cnt, Split *__restrict__ split,
+                                                      PairLists PL, int base, int count) {
+    __shared__ StageScratch sc;
+    constexpr const LdsLayout &LY = LAY_COUNT;
+    const WalkCtx C = walk_ctx(V);
+    const Tile T = make_tile(base, count, KR, V, C);
+    if (T.first >= T.end) return;
+    const int tile = xcd_block();
+    const int ltile = KR == KIND_NODE ? tile : PL.ntile_n + tile;
+    const int r = T.first + (int)threadIdx.x;
+    int self_base = 0;
+    bool lds = T.fits;
+    auto load = [&](auto kt) {
+        constexpr int K = decltype(kt)::value;
+        return [&](int g, int i) {
+            lds_at<double2>(LAY_COUNT.x[K])[i] = x[g];
+            lds_at<double>(LAY_COUNT.h[K])[i] = hsml[g];
+        };
+    };
+    if (lds) {
+        lds = stage_kind<KIND_NODE>(T, V, C, CAP, lds_at<int>(LY.tab[KIND_NODE]), sc, load(KindTag<KIND_NODE>{}));
+        if (KR == KIND_NODE) self_base = sc.mainlo;
+    }
+    if (lds) {
+        lds = stage_kind<KIND_SP>(T, V, C, CAP, lds_at<int>(LY.tab[KIND_SP]), sc, load(KindTag<KIND_SP>{}));
+        if (KR == KIND_SP) self_base = sc.mainlo;
+    }
+    if (lds)
+        lds = stage_kind<KIND_DUMMY>(T, V, C, CAP_D, lds_at<int>(LY.tab[KIND_DUMMY]), sc,
+                                     load(KindTag<KIND_DUMMY>{}));
+    unsigned fwd = 0;
+    unsigned t0 = 0, t1 = 0, t2 = 0, t3 = 0, t6 = 0, t9 = 0;   // pairs by pint_type
+    ListWriter la, lb;
+    const bool active = r < T.end && V.cell[r] < C.g.ndivt;
+    if (active) {
+        int cx, cy;
+        cell_xy(V.cell[r], C.g, cx, cy);
+        const int crm_r = cy * C.g.ndivx + cx;
+        const double2 xr = x[r];
+        const double hr = hsml[r];
+        const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx + 1 < C.g.ndivx ? cx + 1 : C.g.ndivx - 1;
+        const int y0 = cy > 0 ? cy - 1 : 0, y1 = cy + 1 < C.g.ndivy ? cy + 1 : C.g.ndivy - 1;
+        const int self = lds ? r - self_base : r;
+        KernelFactor kf;
+        auto walk = [&](const auto &Cs, const auto &A, bool lists) __attribute__((always_inline)) {
+            for (int y = y0; y <= y1; ++y)
+                for (int xx = x0; xx <= x1; ++xx) {
+                    const int crm = y * C.g.ndivx + xx;
+#pragma unroll
+                    for (int ks = 0; ks < 3; ++ks) {
+                        const int2 rg = Cs.range(ks, xx, y);
+                        for (int j = rg.x; j < rg.x + rg.y; ++j) {
+                            if (ks == KR && j == self) continue;
+                            const double2 xs = A.xk(ks, j);
+                            const double dx0 = xr.x - xs.x, dx1 = xr.y - xs.y;
+                            double mh;
+                            const double rr = pair_distance(dx0, dx1, hr, A.hk(ks, j), mh);
+                            if (rr < 0.0) continue;
+                            // slots ascend with the window index inside a cell and kind
+                            const bool after =
+                                crm > crm_r || (crm == crm_r && (ks > KR || (ks == KR && j > self)));
+                            if (after) {
+                                fwd++;
+                                switch (pair_type(KR, ks)) {
+                                    case 1: t1++; break;
+                                    case 2: t2++; break;
+                                    case 3: t3++; break;
+                                    case 6: t6++; break;
+                                    case 9: t9++; break;
+                                    default: t0++; break;
+                                }
+                            }
+                            if (!lists) continue;
+                            const bool to_a = in_list_a(KR, ks), to_b = KR == KIND_NODE && ks == KIND_NODE;
+                            if (!to_a && !to_b) continue;
+                            float w, gwx, gwy;
+                            cubic_kernel_f(rr, dx0, dx1, mh, kf.get(mh, V.pi), w, gwx, gwy);
+                            const unsigned e = (unsigned)((ks << 14) | j);
+                            if (to_a)
+                                la.put(ltile, PL.ia, PL.wa, PL.gxa, PL.gya, e, w, gwx, gwy);
+                            else
+                                lb.put(ltile, PL.ib, nullptr, PL.gxb, PL.gyb, e, w, gwx, gwy);
+                        }
+                    }
+                }
+        };
+        if (lds)
+            walk(LdsCells<LY>{T.x0 - 1, T.wy0}, LdsFields<LY>{}, true);
+        else
+            walk(GlobalCells{V.kstart, C.g, C.L}, XHFields{x, hsml}, false);
+        if (lds) {
+            la.finish(ltile, PL.ia, PL.wa, PL.gxa, PL.gya);
+            if (KR == KIND_NODE) lb.finish(ltile, PL.ib, nullptr, PL.gxb, PL.gyb);
+        }
+        cnt[V.proc[r]] = fwd;
+    }
+    unsigned tv[6] = {t0, t1, t2, t3, t6, t9};
+    const int tid[6] = {0, 1, 2, 3, 6, 9};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        unsigned v = tv[k];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+        if ((threadIdx.x & 63) == 0 && v) atomicAdd(&split->n_by_type[tid[k]], (unsigned long long)v);
+    }
+    const bool over = la.n > LMAX || lb.n > LMAX;
+    const bool any_over = lds ? __syncthreads_or(over) : true;
+    if (threadIdx.x == 0) PL.ok[ltile] = !any_over;
+    if (r < T.end) {
+        PL.deg_a[r] = (unsigned short)(la.n < LMAX ? la.n : LMAX);
+        if (KR == KIND_NODE) PL.deg_b[r] = (unsigned short)(lb.n < LMAX ? lb.n : LMAX);
+    }
+}
+
+// Call f(kind_tag, fields, j, w, dW/dx_r, dW/dy_r) over a receiver's list
+// entries of the kinds in KMASK, forwards or backwards.  NW / NG: the
+// kernel reads w / the gradient.  The next words are in flight while the
+// entries of the current ones are summed.
+template <int KMASK, bool NW, bool NG, typename Fields, typename F>
+__device__ __forceinline__ void walk_list(const unsigned long long *__restrict__ ip, const float4 *__restrict__ wp,
+                                          const float4 *__restrict__ gxp, const float4 *__restrict__ gyp,
+                                          int tile, int deg, bool rev, const Fields &A, F &&f) {
+    const int ng = (deg + 3) >> 2;
+    if (ng == 0) return;
+    const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+    size_t o = word_at(tile, rev ? ng - 1 : 0);
+    unsigned long long ni = ip[o];
+    float4 nw = NW ? wp[o] : z, nx = NG ? gxp[o] : z, ny = NG ? gyp[o] : z;
+    for (int gi = 0; gi < ng; ++gi) {
+        const int g = rev ? ng - 1 - gi : gi;
+        const unsigned long long ci = ni;
+        const float4 cw = nw, cx = nx, cy = ny;
+        if (gi + 1 < ng) {
+            o = word_at(tile, rev ? g - 1 : g + 1);
+            ni = ip[o];
+            if (NW) nw = wp[o];
+            if (NG) {
+                nx = gxp[o];
+                ny = gyp[o];
+            }
+        }
+        const int ne = deg - 4 * g < 4 ? deg - 4 * g : 4;
+        for (int k = 0; k < ne; ++k) {
+            const int kk = rev ? ne - 1 - k : k;
+            const int ent = (int)((ci >> (16 * kk)) & 0xffff);
+            const int kind = ent >> 14, j = ent & 0x3fff;
+            if (!((KMASK >> kind) & 1)) continue;
+            const float w = NW ? get4(cw, kk) : 0.0f;
+            const float gx = NG ? get4(cx, kk) : 0.0f, gy = NG ? get4(cy, kk) : 0.0f;
+            if constexpr ((KMASK & MASK_NODE) != 0)
+                if (kind == KIND_NODE) {
+                    f(KindTag<KIND_NODE>{}, A, j, w, gx, gy);
+                    continue;
+                }
+            if constexpr ((KMASK & MASK_SP) != 0)
+                if (kind == KIND_SP) {
+                    f(KindTag<KIND_SP>{}, A, j, w, gx, gy);
+                    continue;
+                }
+            if constexpr ((KMASK & MASK_DUMMY) != 0) f(KindTag<KIND_DUMMY>{}, A, j, w, gx, gy);
+        }
+    }
+}
+
+// Workgroup-wide tiling decision of a tiled stage kernel: the tile fits the
+// window layout and no receiver of it needs per-pair pass classification.
+// check is this lane's need for that classification (global path).
+__device__ __forceinline__ bool tile_uses_lds(const Tile &T, const SearchView &V, const WalkCtx &C, int r,
+                                              bool &check) {
+    check = C.mixed && r <= T.last && V.proc[r] >= C.sp.first_rev;
+    const bool any = C.mixed ? __syncthreads_or(check) : false;
+    return T.fits && !any;
+}
+
 // stress_point_update (2_...:403-482) for node receivers: CSPM-normalised
 // stress and plastic strain from the stress points, then adapt_stress2.
-__global__ void k_spu_nodes(StateBufs S, Work W, const int *__restrict__ degA, Lists Lz, int adapt,
-                            double tanfi, double coh, int nn, int nt) {
-    int r = xcd_block() * blockDim.x + threadIdx.x;
-    if (r >= nn) return;
-    unsigned e = Lz.a_off[r >> 6] + (r & 63);
-    int deg = degA[r];
+__global__ __launch_bounds__(TILE) void k_spu_nodes(StateBufs S, SearchView V, PairLists PL, int adapt,
+                                                    double tanfi, double coh, int nn) {
+    __shared__ StageScratch sc;
+    constexpr const LdsLayout &LY = LAY_SPU_NODES;
+    const WalkCtx C = walk_ctx(V);
+    const Tile T = make_tile(0, nn, KIND_NODE, V, C);
+    if (T.first >= T.end) return;
+    const int r = T.first + (int)threadIdx.x;
+    const int tile = xcd_block();
+    bool check;
+    bool lds = tile_uses_lds(T, V, C, r, check) && PL.ok[tile];
+    if (lds)
+        lds = stage_kind<KIND_SP>(T, V, C, CAP, nullptr, sc, [&](int g, int i) {
+            lds_at<double>(LY.m[KIND_SP])[i] = S.mass[g];
+            lds_at<double>(LY.rho[KIND_SP])[i] = S.rho[g];
+            const double4 st = S.stress[g];
+            lds_at<double>(LY.s0[KIND_SP])[i] = st.x;
+            lds_at<double>(LY.s1[KIND_SP])[i] = st.y;
+            lds_at<double>(LY.s2[KIND_SP])[i] = st.z;
+            lds_at<double>(LY.s3[KIND_SP])[i] = st.w;
+            lds_at<double>(LY.iv[KIND_SP])[i] = S.ivar1[g];
+        });
+    if (r >= T.end) return;
     double t0 = 0, t1 = 0, t2 = 0, t3 = 0, tv = 0, norm = 0;
-#pragma unroll 4
-    for (int k = 0; k < deg; ++k, e += SLICE) {
-        int s = Lz.a_s[e];
-        if (s >= nt) continue;                      // type 6 entry
-        double w = (double)Lz.a_w[e];
-        double ms = S.mass[s], rs = S.rho[s];
+    auto body = [&](auto kt, const auto &A, int j, float wf, float, float) __attribute__((always_inline)) {
+        double w = (double)wf;
+        double ms = A.m(kt, j), rs = A.rho(kt, j);
         double h1 = (ms / rs) * w;
-        double4 st = S.stress[s];
+        double4 st = A.st(kt, j);
         t0 = t0 + st.x * h1;
         t1 = t1 + st.y * h1;
         t2 = t2 + st.z * h1;
         t3 = t3 + st.w * h1;
-        tv = tv + S.ivar1[s] * h1;
+        tv = tv + A.iv(kt, j) * h1;
         norm = norm + (w * ms) / rs;
+    };
+    const int key_r = V.cell[r];
+    if (key_r < C.g.ndivt) {
+        if (lds) {
+            walk_list<MASK_SP, true, false>(PL.ia, PL.wa, PL.gxa, PL.gya, tile, PL.deg_a[r],
+                                            !C.any_fwd && C.any_rev, LdsFields<LY>{}, body);
+        } else {
+            int cx, cy;
+            cell_xy(key_r, C.g, cx, cy);
+            const SplitClass cls{check, check ? V.proc[r] : 0, V.proc, V.cnt, V.off, C.sp};
+            walk_stencil<MASK_SP>(GlobalCells{V.kstart, C.g, C.L}, GlobalFields{S}, C.g, cx, cy, S.x[r],
+                                  S.hsml[r], KIND_NODE, r, C.any_fwd ? check : C.any_rev, C.any_fwd, V.pi, cls,
+                                  body);
+        }
     }
     double4 s = S.stress[r];
     if (norm != 0) {
@@ -663,73 +1279,113 @@ __global__ void k_spu_nodes(StateBufs S, Work W, const int *__restrict__ degA, L
     S.stress[r] = s;
 }
 
-// stress_point_update for stress-point receivers: CSPM-normalised velocity
-// from the nodes, then adapt_stress2 of the stress point.
-__global__ void k_spu_sps(StateBufs S, Work W, const int *__restrict__ degA, Lists Lz, int adapt,
-                          double tanfi, double coh, int nn, int nt) {
-    int r = nn + xcd_block() * blockDim.x + threadIdx.x;
-    if (r >= nt) return;
-    unsigned e = Lz.a_off[r >> 6] + (r & 63);
-    int deg = degA[r];
+// Stress-point receivers: stress_point_update (CSPM-normalised velocity from
+// the nodes, then adapt_stress2) and, when deriv is set, get_derivatives
+// (2_...:487-648) fused with plastic_terms (loop 2_...:730-734), the Jaumann
+// spin (2_...:751-757) and the stress right-hand side and RK accumulation
+// (2_...:762, 772).  The derivative of a stress point reads its own updated
+// velocity and stress and the nodes' (which the update leaves alone), so one
+// pass over the node partners after the update gives what the reference's
+// get_derivatives sees after its stress_point_update.
+__global__ __launch_bounds__(TILE) void k_sp_update(StateBufs S, Work W, SearchView V, PairLists PL,
+                                                    DevParams P, double time_sph, double f2, int deriv,
+                                                    int adapt, int nn, int nt) {
+    __shared__ StageScratch sc;
+    constexpr const LdsLayout &LY = LAY_SP_UPDATE;
+    const WalkCtx C = walk_ctx(V);
+    const Tile T = make_tile(nn, nt - nn, KIND_SP, V, C);
+    if (T.first >= T.end) return;
+    const int r = T.first + (int)threadIdx.x;
+    const int tile = xcd_block();
+    const int ltile = PL.ntile_n + tile;
+    bool check;
+    bool lds = tile_uses_lds(T, V, C, r, check) && PL.ok[ltile];
+    if (lds)
+        lds = stage_kind<KIND_NODE>(T, V, C, CAP, nullptr, sc, [&](int g, int i) {
+            lds_at<double2>(LY.x[KIND_NODE])[i] = S.x[g];
+            lds_at<double2>(LY.v[KIND_NODE])[i] = S.vel[g];
+            lds_at<double>(LY.m[KIND_NODE])[i] = S.mass[g];
+            lds_at<double>(LY.rho[KIND_NODE])[i] = S.rho[g];
+            lds_at<double>(LY.mr[KIND_NODE])[i] = S.mass[g] / S.rho[g];
+        });
+    if (lds && deriv)
+        lds = stage_kind<KIND_DUMMY>(T, V, C, CAP_D, nullptr, sc, [&](int g, int i) {
+            lds_at<double2>(LY.x[KIND_DUMMY])[i] = S.x[g];
+            lds_at<double>(LY.m[KIND_DUMMY])[i] = S.mass[g];
+            lds_at<double>(LY.rho[KIND_DUMMY])[i] = S.rho[g];
+            lds_at<float>(LY.wall)[i] = S.wallpos[g];
+            lds_at<float>(LY.horiz)[i] = S.horiz[g];
+        });
+    if (r >= T.end) return;
+    const int key_r = V.cell[r];
+    const bool inside = key_r < C.g.ndivt;
+    int cx = 0, cy = 0;
+    if (inside && !lds) cell_xy(key_r, C.g, cx, cy);
+    const double2 xr = S.x[r];
+    const double hr = S.hsml[r];
+    const LdsFields<LY> LF{};
+    const int deg = lds && inside ? PL.deg_a[r] : 0;
+    const SplitClass cls{check, check ? V.proc[r] : 0, V.proc, V.cnt, V.off, C.sp};
+    const bool rev_l = !C.any_fwd && C.any_rev, rev_g = C.any_fwd ? check : C.any_rev;
+
+    // stress_point_update
     double v0 = 0, v1 = 0, norm = 0;
-#pragma unroll 4
-    for (int k = 0; k < deg; ++k, e += SLICE) {
-        int s = Lz.a_s[e];
-        if (s >= nn) continue;                      // type 9 entry
-        double w = (double)Lz.a_w[e];
-        double ms = S.mass[s], rs = S.rho[s];
-        double h2 = (ms / rs) * w;
-        double2 vs = S.vel[s];
+    auto spu = [&](auto kt, const auto &A, int j, float wf, float, float) __attribute__((always_inline)) {
+        double w = (double)wf;
+        double ms = A.m(kt, j), rs = A.rho(kt, j);
+        double h2 = A.mr(kt, j) * w;
+        double2 vs = A.vel(kt, j);
         v0 = v0 + vs.x * h2;
         v1 = v1 + vs.y * h2;
         norm = norm + (w * ms) / rs;
+    };
+    if (inside) {
+        if (lds)
+            walk_list<MASK_NODE, true, false>(PL.ia, PL.wa, PL.gxa, PL.gya, ltile, deg, rev_l, LF, spu);
+        else
+            walk_stencil<MASK_NODE>(GlobalCells{V.kstart, C.g, C.L}, GlobalFields{S}, C.g, cx, cy, xr, hr,
+                                    KIND_SP, r, rev_g, C.any_fwd, V.pi, cls, spu);
     }
-    if (norm != 0) S.vel[r] = make_double2(v0 / norm, v1 / norm);
+    double2 vr = S.vel[r];
+    if (norm != 0) {
+        vr = make_double2(v0 / norm, v1 / norm);
+        S.vel[r] = vr;
+    }
+    double4 st = S.stress[r];
     if (adapt) {
-        double4 s = S.stress[r];
-        adapt_stress(s, tanfi, coh);
-        S.stress[r] = s;
+        adapt_stress(st, P.tanfi, P.coh);
+        S.stress[r] = st;
     }
-}
+    if (!deriv) return;
 
-// get_derivatives (2_...:487-648) for stress-point receivers, fused with
-// plastic_terms (loop 2_...:730-734), the Jaumann spin (2_...:751-757) and the
-// stress right-hand side and RK accumulation (2_...:762, 772).
-__global__ void k_deriv_sps(StateBufs S, Work W, const int *__restrict__ degA, Lists Lz,
-                            DevParams P, double time_sph, double f2, int nn, int nt) {
-    int r = nn + xcd_block() * blockDim.x + threadIdx.x;
-    if (r >= nt) return;
-    unsigned e = Lz.a_off[r >> 6] + (r & 63);
-    int deg = degA[r];
+    // get_derivatives
     double g00 = 0, g01 = 0, g10 = 0, g11 = 0;   // grad1_tmp(a,b): d v_a / d x_b
     double A0 = 0, A1 = 0, A2 = 0, A3 = 0;
-    double2 xr = S.x[r], vr = S.vel[r];
-#pragma unroll 4
-    for (int k = 0; k < deg; ++k, e += SLICE) {
-        int s = Lz.a_s[e];
-        double ms = S.mass[s], rs = S.rho[s];
-        if (s < nn) {                               // type 1: pair_i = this SP
-            double gx = (double)Lz.a_gx[e], gy = (double)Lz.a_gy[e];
+    auto grad = [&](auto kt, const auto &A, int j, float, float gxf, float gyf) __attribute__((always_inline)) {
+        constexpr int K = decltype(kt)::value;
+        double ms = A.m(kt, j), rs = A.rho(kt, j);
+        if constexpr (K == KIND_NODE) {             // type 1: pair_i = this SP
+            double gx = (double)gxf, gy = (double)gyf;
             double h1 = gx * ms / rs;
             double h2 = gy * ms / rs;
-            double2 vs = S.vel[s];
+            double2 vs = A.vel(kt, j);
             g00 = g00 + (vs.x - vr.x) * h1;
             g01 = g01 + (vs.x - vr.x) * h2;
             g10 = g10 + (vs.y - vr.y) * h1;
             g11 = g11 + (vs.y - vr.y) * h2;
             if (P.cspm) {
-                double2 xs = S.x[s];
+                double2 xs = A.x(kt, j);
                 A0 = A0 + (xs.x - xr.x) * h1;
                 A1 = A1 + (xs.y - xr.y) * h1;
                 A2 = A2 + (xs.x - xr.x) * h2;
                 A3 = A3 + (xs.y - xr.y) * h2;
             }
         } else {                                    // type 9: pair_i = dummy
-            double gx = (double)(-Lz.a_gx[e]), gy = (double)(-Lz.a_gy[e]);
-            double wall = (double)S.wallpos[s];
-            double2 xs = S.x[s];
+            double gx = (double)(-gxf), gy = (double)(-gyf);
+            double wall = (double)A.wall(kt, j);
+            double2 xs = A.x(kt, j);
             double da, db;
-            if (S.horiz[s] == 1.0f) {
+            if (A.horiz(kt, j) == 1.0f) {
                 da = fabs(xr.y - wall);
                 db = fabs(xs.y - wall);
             } else {
@@ -747,6 +1403,14 @@ __global__ void k_deriv_sps(StateBufs S, Work W, const int *__restrict__ degA, L
             g10 = g10 + (vr.y - dv1) * h1;
             g11 = g11 + (vr.y - dv1) * h2;
         }
+    };
+    if (inside) {
+        if (lds)
+            walk_list<MASK_NODE | MASK_DUMMY, false, true>(PL.ia, PL.wa, PL.gxa, PL.gya, ltile, deg, rev_l, LF,
+                                                           grad);
+        else
+            walk_stencil<MASK_NODE | MASK_DUMMY>(GlobalCells{V.kstart, C.g, C.L}, GlobalFields{S}, C.g, cx, cy,
+                                                 xr, hr, KIND_SP, r, rev_g, C.any_fwd, V.pi, cls, grad);
     }
     if (P.cspm) {
         double A4 = A0 * A3 - A1 * A2;
@@ -767,7 +1431,6 @@ __global__ void k_deriv_sps(StateBufs S, Work W, const int *__restrict__ degA, L
     d1.w = -(P.D41 * g00 + P.D42 * g11);
     double4 gu = make_double4(g00, g10, g01, g11);
     W.grad_u[r] = gu;
-    double4 st = S.stress[r];
     double4 Gs;
     double der1;
     double fdr = S.fdr[r];
@@ -792,38 +1455,40 @@ __global__ void k_deriv_sps(StateBufs S, Work W, const int *__restrict__ degA, L
     W.rk_dev[r] = W.rk_dev[r] + der1 * f2;
 }
 
-// get_derivatives for node receivers, fused with gravity_force
-// (3_...:2640-2702), boundary_forces (2_...:1117-1161), artificial_viscosity
-// (2_...:826-904) and the momentum right-hand side and RK accumulation
-// (2_...:763-764, 769).
+// boundary_forces for one node: the nonzero terms of its inner-layer dummies
+// in ascending dummy index (the reference's j loop over every dummy,
+// 2_...:1117-1161), found by repeated minimum selection over the dummies of
+// its 3x3 cell stencil; cells are 2*max(h) wide, beyond the 0.75*dx reach of
+// a nonzero term.
 constexpr int MAX_WALL_TERMS = 16;
 
-// boundary_forces for one node: the nonzero terms of its inner-layer dummies
-// in ascending dummy index (the reference's j loop), found by repeated
-// minimum selection over the node's dummy partners (a handful near walls).
-__device__ __forceinline__ double2 wall_force(const StateBufs &S, const Lists &Lz, unsigned e0, int deg,
-                                              double2 xr, double hr, const DevParams &P,
+__device__ __forceinline__ double2 wall_force(const StateBufs &S, const SearchView &V, const WalkCtx &C,
+                                              int cx, int cy, double2 xr, double hr, const DevParams &P,
                                               int *__restrict__ err) {
     double2 fb = make_double2(0.0, 0.0);
+    const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx + 1 < C.g.ndivx ? cx + 1 : C.g.ndivx - 1;
+    const int y0 = cy > 0 ? cy - 1 : 0, y1 = cy + 1 < C.g.ndivy ? cy + 1 : C.g.ndivy - 1;
     int last = -1;
     for (int it = 0; it <= MAX_WALL_TERMS; ++it) {
         int best = 0x7fffffff;
         float b0 = 0.0f, b1 = 0.0f;
-        unsigned e = e0;
-        for (int k = 0; k < deg; ++k, e += SLICE) {
-            int s = Lz.a_s[e];
-            if (s < P.nt) continue;
-            int o = S.orig[s];
-            if (o >= P.nt + P.nd2 || o <= last || o >= best) continue;
-            double2 xs = S.x[s];
-            float t0, t1;
-            wall_force_term(xr.x - xs.x, xr.y - xs.y, hr, S.hsml[s], P.dx, t0, t1);
-            if (t0 != 0.0f || t1 != 0.0f || t0 != t0 || t1 != t1) {
-                best = o;
-                b0 = t0;
-                b1 = t1;
+        for (int y = y0; y <= y1; ++y)
+            for (int xx = x0; xx <= x1; ++xx) {
+                const int key = cell_key(xx, y, C.g);
+                const int lo = V.kstart[2 * C.L + key], hi = V.kstart[2 * C.L + key + 1];
+                for (int s = lo; s < hi; ++s) {
+                    int o = S.orig[s];
+                    if (o >= P.nt + P.nd2 || o <= last || o >= best) continue;
+                    double2 xs = S.x[s];
+                    float t0, t1;
+                    wall_force_term(xr.x - xs.x, xr.y - xs.y, hr, S.hsml[s], P.dx, t0, t1);
+                    if (t0 != 0.0f || t1 != 0.0f || t0 != t0 || t1 != t1) {
+                        best = o;
+                        b0 = t0;
+                        b1 = t1;
+                    }
+                }
             }
-        }
         if (best == 0x7fffffff) return fb;
         if (it == MAX_WALL_TERMS) {
             atomicOr(err, 1);
@@ -836,58 +1501,143 @@ __device__ __forceinline__ double2 wall_force(const StateBufs &S, const Lists &L
     return fb;
 }
 
-__global__ void k_deriv_nodes(StateBufs S, Work W, const int *__restrict__ degA,
-                              const int *__restrict__ degB, Lists Lz, DevParams P, double gx_grav,
-                              double gy_grav, double f2, int *__restrict__ err, int nn, int nt) {
-    int r = xcd_block() * blockDim.x + threadIdx.x;
-    if (r >= nn) return;
-    double2 xr = S.x[r], vr = S.vel[r];
-    double4 sr = S.stress[r];
-    double rhor = S.rho[r], hr = S.hsml[r];
-    double rr2 = rhor * rhor;
+// get_derivatives for node receivers, fused with gravity_force
+// (3_...:2640-2702), boundary_forces (2_...:1117-1161), artificial_viscosity
+// (2_...:826-904) and the momentum right-hand side and RK accumulation
+// (2_...:763-764, 769).  The stress gradient sums over stress-point and dummy
+// partners (list A), the viscosity over node partners (list B); each sum
+// keeps list order, so the window is staged in two phases (stress points
+// and dummies, then nodes) through the same LDS.
+__global__ __launch_bounds__(TILE) void k_deriv_nodes(StateBufs S, Work W, SearchView V, PairLists PL,
+                                                      DevParams P, double gx_grav, double gy_grav, double f2,
+                                                      int *__restrict__ err, int nn) {
+    __shared__ StageScratch sc;
+    constexpr const LdsLayout &LY = LAY_DERIV_NODES;
+    const WalkCtx C = walk_ctx(V);
+    const Tile T = make_tile(0, nn, KIND_NODE, V, C);
+    if (T.first >= T.end) return;
+    const int r = T.first + (int)threadIdx.x;
+    const int tile = xcd_block();
+    bool check;
+    const bool tiled = tile_uses_lds(T, V, C, r, check) && PL.ok[tile];
+    const bool active = r < T.end;
+    const int key_r = active ? V.cell[r] : C.g.ndivt;
+    const bool inside = key_r < C.g.ndivt;
+    int cx = 0, cy = 0;
+    if (inside) cell_xy(key_r, C.g, cx, cy);
+    const int ra = active ? r : T.first;   // a valid slot for the idle lanes' loads
+    const double2 xr = S.x[ra], vr = S.vel[ra];
+    const double4 sr = S.stress[ra];
+    const double rhor = S.rho[ra], hr = S.hsml[ra];
+    const double rr2 = rhor * rhor;
     const double or0 = sr.x / rr2, or1 = sr.y / rr2, or2 = sr.z / rr2;   // own sigma / rho^2
-    double q00 = 0, q01 = 0, q10 = 0, q11 = 0, q20 = 0, q21 = 0;  // grad2_tmp(s,b)
+    const SplitClass cls{check, check ? V.proc[r] : 0, V.proc, V.cnt, V.off, C.sp};
+    const bool rev_l = !C.any_fwd && C.any_rev, rev_g = C.any_fwd ? check : C.any_rev;
+    const LdsFields<LY> LF{};
+
+    // phase 1: stress gradient over stress points (type 1) and dummies (type 6)
+    bool lds = tiled;
+    if (lds)
+        lds = stage_kind<KIND_SP>(T, V, C, CAP, nullptr, sc, [&](int g, int i) {
+            lds_at<double2>(LY.x[KIND_SP])[i] = S.x[g];
+            lds_at<double>(LY.m[KIND_SP])[i] = S.mass[g];
+            const double rs = S.rho[g], rs2 = rs * rs;
+            lds_at<double>(LY.rho[KIND_SP])[i] = rs;
+            const double4 st = S.stress[g];
+            lds_at<double>(LY.p0[KIND_SP])[i] = st.x / rs2;
+            lds_at<double>(LY.p1[KIND_SP])[i] = st.y / rs2;
+            lds_at<double>(LY.p2[KIND_SP])[i] = st.z / rs2;
+        });
+    if (lds)
+        lds = stage_kind<KIND_DUMMY>(T, V, C, CAP_D, nullptr, sc, [&](int g, int i) {
+            lds_at<double>(LY.m[KIND_DUMMY])[i] = S.mass[g];
+            lds_at<double>(LY.rho[KIND_DUMMY])[i] = S.rho[g];
+        });
+    double q00 = 0, q01 = 0, q10 = 0, q11 = 0, q20 = 0, q21 = 0;        // grad2_tmp(s,b)
     double A0 = 0, A1 = 0, A2 = 0, A3 = 0;
     int ndum = 0;
-    unsigned e = Lz.a_off[r >> 6] + (r & 63);
-    int deg = degA[r];
-#pragma unroll 4
-    for (int k = 0; k < deg; ++k, e += SLICE) {
-        int s = Lz.a_s[e];
-        double ms = S.mass[s], rs = S.rho[s];
-        double gx = (double)(-Lz.a_gx[e]), gy = (double)(-Lz.a_gy[e]);   // pair_i is the partner
-        double rs2 = rs * rs;
-        if (s < nt) {                                // type 1
-            double4 ss = S.stress[s];
-            const double p0 = ss.x / rs2, p1 = ss.y / rs2, p2 = ss.z / rs2;   // sigma_s / rho_s^2
-            double a;
-            a = gx * (or0 + p0); q00 = q00 - ms * a;
-            a = gy * (or0 + p0); q01 = q01 - ms * a;
-            a = gx * (or1 + p1); q10 = q10 - ms * a;
-            a = gy * (or1 + p1); q11 = q11 - ms * a;
-            a = gx * (or2 + p2); q20 = q20 - ms * a;
-            a = gy * (or2 + p2); q21 = q21 - ms * a;
+    auto grad = [&](auto kt, const auto &A, int j, float, float gxf, float gyf) __attribute__((always_inline)) {
+        constexpr int K = decltype(kt)::value;
+        double ms = A.m(kt, j), rs = A.rho(kt, j);
+        double gx = (double)(-gxf), gy = (double)(-gyf);   // pair_i is the partner
+        double p0, p1, p2;
+        if constexpr (K == KIND_SP) {               // type 1: sigma_s / rho_s^2
+            const double3 ps = A.sr2(kt, j);
+            p0 = ps.x; p1 = ps.y; p2 = ps.z;
             if (P.cspm) {
                 double h1b = -(gx * ms / rs);
                 double h2b = -(gy * ms / rs);
-                double2 xs = S.x[s];
+                double2 xs = A.x(kt, j);
                 A0 = A0 + (xs.x - xr.x) * h1b;
                 A1 = A1 + (xs.y - xr.y) * h1b;
                 A2 = A2 + (xs.x - xr.x) * h2b;
                 A3 = A3 + (xs.y - xr.y) * h2b;
             }
-        } else {                                     // type 6: dummy stress := node stress
-            double d0 = sr.x / rs2, d1 = sr.y / rs2, d2 = sr.z / rs2;
-            double a;
-            a = gx * (or0 + d0); q00 = q00 - ms * a;
-            a = gy * (or0 + d0); q01 = q01 - ms * a;
-            a = gx * (or1 + d1); q10 = q10 - ms * a;
-            a = gy * (or1 + d1); q11 = q11 - ms * a;
-            a = gx * (or2 + d2); q20 = q20 - ms * a;
-            a = gy * (or2 + d2); q21 = q21 - ms * a;
+        } else {                                    // type 6: dummy stress := node stress
+            const double rs2 = rs * rs;
+            p0 = sr.x / rs2; p1 = sr.y / rs2; p2 = sr.z / rs2;
             ndum++;
         }
+        double a;
+        a = gx * (or0 + p0); q00 = q00 - ms * a;
+        a = gy * (or0 + p0); q01 = q01 - ms * a;
+        a = gx * (or1 + p1); q10 = q10 - ms * a;
+        a = gy * (or1 + p1); q11 = q11 - ms * a;
+        a = gx * (or2 + p2); q20 = q20 - ms * a;
+        a = gy * (or2 + p2); q21 = q21 - ms * a;
+    };
+    if (inside) {
+        if (lds)
+            walk_list<MASK_SP | MASK_DUMMY, false, true>(PL.ia, PL.wa, PL.gxa, PL.gya, tile, PL.deg_a[r], rev_l,
+                                                         LF, grad);
+        else
+            walk_stencil<MASK_SP | MASK_DUMMY>(GlobalCells{V.kstart, C.g, C.L}, GlobalFields{S}, C.g, cx, cy,
+                                               xr, hr, KIND_NODE, r, rev_g, C.any_fwd, V.pi, cls, grad);
     }
+
+    // phase 2: artificial viscosity over node partners (type 3), single precision
+    float ax = 0.0f, ay = 0.0f, visc = 0.0f;
+    if (P.do_av) {
+        bool lds2 = tiled;
+        if (lds2) {
+            __syncthreads();   // phase-1 walks are done with the shared bytes
+            lds2 = stage_kind<KIND_NODE>(T, V, C, CAP, nullptr, sc, [&](int g, int i) {
+                lds_at<double2>(LY.x[KIND_NODE])[i] = S.x[g];
+                lds_at<double2>(LY.v[KIND_NODE])[i] = S.vel[g];
+                lds_at<double>(LY.h[KIND_NODE])[i] = S.hsml[g];
+                lds_at<double>(LY.m[KIND_NODE])[i] = S.mass[g];
+                lds_at<double>(LY.rho[KIND_NODE])[i] = S.rho[g];
+            });
+        }
+        auto av = [&](auto kt, const auto &A, int j, float, float gxf, float gyf) __attribute__((always_inline)) {
+            double2 xs = A.x(kt, j), vs = A.vel(kt, j);
+            float xij = (float)(xr.x - xs.x);
+            float yij = (float)(xr.y - xs.y);
+            float hh = (float)(0.5 * (hr + A.h(kt, j)));
+            float rho2 = (float)(0.5 * (rhor + A.rho(kt, j)));
+            float div_u = (float)((double)xij * (vr.x - vs.x));
+            div_u = (float)((double)div_u + (double)yij * (vr.y - vs.y));
+            float sr_ = sqrtf(xij * xij + yij * yij);
+            float theta = (hh * div_u) / (sr_ * sr_ + 0.01f * (hh * hh));
+            if (div_u < 0) {
+                visc = (float)((-(P.alpha * (double)600.0f * (double)theta) + P.beta * (double)(theta * theta)) /
+                               (double)rho2);
+            } else if (div_u >= 0) {
+                visc = 0;
+            }
+            double ms = A.m(kt, j);
+            ax = (float)((double)ax + (double)(visc * gxf) * ms);
+            ay = (float)((double)ay + (double)(visc * gyf) * ms);
+        };
+        if (inside) {
+            if (lds2)
+                walk_list<MASK_NODE, false, true>(PL.ib, nullptr, PL.gxb, PL.gyb, tile, PL.deg_b[r], rev_l, LF, av);
+            else
+                walk_stencil<MASK_NODE>(GlobalCells{V.kstart, C.g, C.L}, GlobalFields{S}, C.g, cx, cy, xr, hr,
+                                        KIND_NODE, r, rev_g, C.any_fwd, V.pi, cls, av);
+        }
+    }
+    if (!active) return;
     if (P.cspm) {
         double A4 = A0 * A3 - A1 * A2;
         if (fabs(A4) < P.cspm_tol) {
@@ -900,54 +1650,25 @@ __global__ void k_deriv_nodes(StateBufs S, Work W, const int *__restrict__ degA,
         q10 = A4 * (A0 * q10 + A1 * q11);
         q11 = A4 * (A2 * q10 + A3 * q11);
     }
-    double d2x = -(q00 + q21);
-    double d2y = -(q20 + q11);
+    const double d2x = -(q00 + q21);
+    const double d2y = -(q20 + q11);
     // gravity_force: G = factg*cgrav - DampingTG*v
-    double Gx = gx_grav - P.damping * vr.x;
-    double Gy = gy_grav - P.damping * vr.y;
-    // boundary_forces: nonzero terms in ascending dummy index
+    const double Gx = gx_grav - P.damping * vr.x;
+    const double Gy = gy_grav - P.damping * vr.y;
+    // boundary_forces
     double2 fb = W.f_bound[r];
     if (P.do_bforce) {
-        fb = ndum ? wall_force(S, Lz, Lz.a_off[r >> 6] + (r & 63), degA[r], xr, hr, P, err)
-                  : make_double2(0.0, 0.0);
+        fb = ndum ? wall_force(S, V, C, cx, cy, xr, hr, P, err) : make_double2(0.0, 0.0);
         W.f_bound[r] = fb;
     }
-    // artificial_viscosity over node-node pairs, single-precision sums
-    double2 av = make_double2(0.0, 0.0);
+    double2 avf = make_double2(0.0, 0.0);
     if (P.do_av) {
-        float ax = 0.0f, ay = 0.0f;
-        float visc = 0.0f;
-        const float c = 600;
-        unsigned eb = Lz.b_off[r >> 6] + (r & 63);
-        int degb = degB[r];
-#pragma unroll 4
-        for (int k = 0; k < degb; ++k, eb += SLICE) {
-            int s = Lz.b_s[eb];
-            double2 xs = S.x[s], vs = S.vel[s];
-            float xij = (float)(xr.x - xs.x);
-            float yij = (float)(xr.y - xs.y);
-            float hh = (float)(0.5 * (hr + S.hsml[s]));
-            float rho2 = (float)(0.5 * (rhor + S.rho[s]));
-            float div_u = (float)((double)xij * (vr.x - vs.x));
-            div_u = (float)((double)div_u + (double)yij * (vr.y - vs.y));
-            float sr_ = sqrtf(xij * xij + yij * yij);
-            float theta = (hh * div_u) / (sr_ * sr_ + 0.01f * (hh * hh));
-            if (div_u < 0) {
-                visc = (float)((-(P.alpha * (double)c * (double)theta) + P.beta * (double)(theta * theta)) /
-                               (double)rho2);
-            } else if (div_u >= 0) {
-                visc = 0;
-            }
-            double ms = S.mass[s];
-            ax = (float)((double)ax + (double)(visc * Lz.b_gx[eb]) * ms);
-            ay = (float)((double)ay + (double)(visc * Lz.b_gy[eb]) * ms);
-        }
-        av = make_double2(-(double)ax, -(double)ay);
-        W.art_visc[r] = av;
+        avf = make_double2(-(double)ax, -(double)ay);
+        W.art_visc[r] = avf;
     }
     double2 rh;
-    rh.x = -d2x + Gx + av.x + fb.x + 0.0;
-    rh.y = -d2y + Gy + av.y + fb.y + 0.0;
+    rh.x = -d2x + Gx + avf.x + fb.x + 0.0;
+    rh.y = -d2y + Gy + avf.y + fb.y + 0.0;
     W.rhs2[r] = rh;
     double2 rk = W.rk_vel[r];
     W.rk_vel[r] = make_double2(rk.x + f2 * rh.x, rk.y + f2 * rh.y);
@@ -955,14 +1676,21 @@ __global__ void k_deriv_nodes(StateBufs S, Work W, const int *__restrict__ degA,
 
 // The type-6 loop of get_derivatives overwrites each dummy's stress with
 // that of its node partners in list order (2_...:580); the last one stays.
-__global__ void k_dummy_stress(StateBufs S, const int *__restrict__ degA, Lists Lz, int nt,
-                               int nt2) {
+__global__ void k_dummy_stress(StateBufs S, SearchView V, int nt, int nt2) {
     int r = nt + blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nt2) return;
-    int deg = degA[r];
-    if (deg == 0) return;
-    unsigned e = Lz.a_off[r >> 6] + (r & 63) + (unsigned)(deg - 1) * SLICE;
-    S.stress[r] = S.stress[Lz.a_s[e]];
+    const WalkCtx C = walk_ctx(V);
+    const int key_r = V.cell[r];
+    if (key_r >= C.g.ndivt) return;
+    int cx, cy;
+    cell_xy(key_r, C.g, cx, cy);
+    const bool check = C.mixed && V.proc[r] >= C.sp.first_rev;
+    const SplitClass cls{check, check ? V.proc[r] : 0, V.proc, V.cnt, V.off, C.sp};
+    int last = -1;
+    walk_stencil<MASK_NODE>(GlobalCells{V.kstart, C.g, C.L}, GlobalFields{S}, C.g, cx, cy, S.x[r], S.hsml[r],
+                            KIND_DUMMY, r, C.any_fwd ? check : C.any_rev, C.any_fwd, V.pi, cls,
+                            [&](auto, const auto &, int j, float, float, float) { last = j; });
+    if (last >= 0) S.stress[r] = S.stress[last];
 }
 
 // End of RK4 (2_...:784-799) and update_strain (3_...:1847-1863).
