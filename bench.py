@@ -38,37 +38,37 @@ def load_pkg():
 
 
 def algorithmic_bytes(kernel, st, counts, params):
-    """Minimal single-touch bytes of one launch of `kernel`.
+    """Minimal single-touch bytes of one launch of `kernel` (the average over
+    its launches in a step where they differ).
 
-    Neighbour-list entries are read once each (partner slot 4 B, w 4 B,
-    gradient 2 x 4 B, as far as the kernel uses them), every particle's state
+    Pair-list entries are read once each (16-bit window index 2 B, w 4 B,
+    gradient 2 x 4 B, as far as the kernel uses them); every particle's state
     that the launch reads or writes is counted once, however many receivers
     gather it.  P_t = pairs of type t of the last search."""
     nn, ns, nd = counts
     P = st["n_by_type"]
     P1, P3, P6, P9 = P[1], P[3], P[6], P[9]
     cspm = params.cspm
+    av = params.alpha > 0 or params.beta > 0
     if kernel == "k_deriv_nodes":
-        lists = 12 * (P1 + P6) + (12 * 2 * P3 if (params.alpha > 0 or params.beta > 0) else 0)
+        lists = 10 * (P1 + P6) + (10 * 2 * P3 if av else 0)
         node = 16 + 16 + 32 + 8 + 8 + 8 + 16 + 16 + 16 + 16 + 16   # x v sigma m rho h rk_vel(rw) rhs2 av fb
-        sp = 32 + 8 + 8 + (16 if cspm else 0)                        # sigma m rho (x)
+        sp = 24 + 8 + 8 + (16 if cspm else 0)                        # sigma(xx,yy,xy) m rho (x)
         dummy = 16 + 8 + 8 + 8 + 4                                   # x m rho h orig
         return lists + nn * node + ns * sp + nd * dummy
-    if kernel == "k_deriv_sps":
-        lists = 12 * (P1 + P9)
-        sp = 16 + 16 + 32 + 8 + 32 + 32 + 32 + 8 + 8 + 8            # x v sigma f_dp(rw) grad_u rhs1 rk_sigma(rw) rk_dev(rw)
-        node = 16 + 8 + 8 + (16 if cspm else 0)                      # v m rho (x)
-        dummy = 16 + 8 + 8 + 4 + 4                                   # x m rho wall horiz
-        return lists + ns * sp + nn * node + nd * dummy
+    if kernel == "k_sp_update":
+        # four launches per step with the derivative, one (after the last stage) without
+        deriv = 14 * (P1 + P9) + ns * (16 + 16 + 32 + 32 + 8 + 32 + 32 + 64 + 16) \
+            + nn * (16 + 16 + 8 + 8) + nd * (16 + 8 + 8 + 4 + 4)
+        update = 6 * P1 + ns * (16 + 64) + nn * (16 + 8 + 8)
+        return (4 * deriv + update) / 5
     if kernel == "k_spu_nodes":
-        return 8 * (P1 + P6) + nn * (32 + 32 + 8 + 16) + ns * (32 + 8 + 8 + 8)
-    if kernel == "k_spu_sps":
-        return 8 * (P1 + P9) + ns * (16 + 32 + 32) + nn * (16 + 8 + 8)
-    if kernel == "k_build":
-        entries = (P1 + P6) + (P1 + P9) + P6 + 2 * P3
-        return 16 * entries + (nn + ns + nd) * (16 + 8 + 4 + 4)
+        return 6 * (P1 + P6) + nn * (32 + 32 + 8 + 16) + ns * (32 + 8 + 8 + 8)
     if kernel == "k_count":
-        return (nn + ns + nd) * (16 + 8 + 4 + 4 + 4 + 4)
+        # three launches per step (node, stress-point and dummy receivers)
+        entries_a = (P1 + P6) + (P1 + P9)
+        lists = 14 * entries_a + 10 * 2 * P3 + 2 * (nn + ns) + 2 * nn
+        return ((nn + ns + nd) * (16 + 8 + 4 + 4 + 4) + lists) / 3
     return None
 
 

@@ -138,6 +138,7 @@ struct sps_handle {
     DBuf<unsigned short> deg_a, deg_b;
     DBuf<unsigned char> tile_ok;
     int ntile_n = 0, ntile_s = 0;
+    double h_uniform = 0.0;   // the smoothing length of every particle, or 0 when they differ
     DBuf<int> err;
     // host-visible summary of the last search
     unsigned long long n_pairs = 0, m_pairs = 0, m_prev = 0;
@@ -398,6 +399,11 @@ extern "C" int sps_set_state(sps_handle *h, const sps_state *st) {
                         i + 1, st->itype[i]);
     }
     int rc = size_cells(h, st->hsml);
+    // smoothing lengths do not change on the path: one value for all lets
+    // the search precompute its acceptance bounds
+    h->h_uniform = st->hsml[0] > 0 ? st->hsml[0] : 0.0;
+    for (int i = 1; i < h->nt2 && h->h_uniform > 0; ++i)
+        if (!(st->hsml[i] == h->h_uniform)) h->h_uniform = 0.0;
     if (rc) return rc;
     if (h->dp.do_bforce) {
         double hmin = 1e300;
@@ -492,10 +498,10 @@ static int search(sps_handle *h, bool fresh_list) {
         const PairLists PL = h->lists();
         if (nn > 0)
             LAUNCH_LDS(K_COUNT_PAIRS, nblk8(nn), TILE, LAY_COUNT.bytes, k_count_lists<KIND_NODE>, V, C.x.p, C.hsml.p,
-                       h->cnt.p, h->split.p, PL, 0, nn);
+                       h->cnt.p, h->split.p, PL, h->h_uniform, 0, nn);
         if (nt > nn)
             LAUNCH_LDS(K_COUNT_PAIRS, nblk8(nt - nn), TILE, LAY_COUNT.bytes, k_count_lists<KIND_SP>, V, C.x.p,
-                       C.hsml.p, h->cnt.p, h->split.p, PL, nn, nt - nn);
+                       C.hsml.p, h->cnt.p, h->split.p, PL, h->h_uniform, nn, nt - nn);
         if (nt2 > nt)
             LAUNCH(K_COUNT_PAIRS, nblk8(nt2 - nt), BLOCK, k_count, C.x.p, C.hsml.p, h->cell.p, h->kstart.p, h->proc.p,
                    h->grid.p, h->cnt.p, h->split.p, nn, nt, nt, nt2);

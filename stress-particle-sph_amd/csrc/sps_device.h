@@ -134,6 +134,39 @@ __device__ __forceinline__ float cubic_w(double r, double hs, double pi) {
     return (float)wd;
 }
 
+// The acceptance decision of pair_distance alone, for a pair whose limit
+// 2*mhsml and squared-distance bounds are precomputed (per pair, or once
+// per receiver when every h is the same: (h + h)/2 is then h exactly).
+// Outside a relative band of 1e-12 around the limit the squared distance
+// decides it: the computed bounds (2*mhsml)^2 * (1 -+ 1e-12) are within 3
+// units in the last place of the exact ones, so driac below the lower one
+// gives sqrt(driac) < 2*mhsml by more than the square root's rounding, and
+// driac above the upper one the converse.  Inside the band the square root
+// is taken as pair_distance takes it.
+struct AcceptBounds {
+    double lim, up, lo;
+};
+
+__device__ __forceinline__ AcceptBounds accept_bounds(double hi, double hj) {
+    AcceptBounds b;
+    b.lim = 2 * ((hi + hj) / 2.0);
+    const double l2 = b.lim * b.lim;
+    b.up = l2 * (1.0 + 1e-12);
+    b.lo = l2 * (1.0 - 1e-12);
+    return b;
+}
+
+__device__ __forceinline__ bool accepted_by(double dx0, double dx1, const AcceptBounds &b) {
+    double driac = dx0 * dx0;
+    driac = driac + dx1 * dx1;
+    if (driac > b.up) return false;
+    if (driac < b.lo) return true;
+    // no instruction: makes the square root depend on this (rare) branch so
+    // that it is not evaluated for every candidate
+    asm volatile("" : "+v"(driac));
+    return sqrt(driac) < b.lim;
+}
+
 // Pair acceptance of grid_find_NEW (2_...:1345-1353): r < 2 * (h_i + h_j)/2.
 // Returns r (>= 0) when accepted, -1 otherwise.  Symmetric in i, j.
 // A candidate whose squared distance exceeds (2*mhsml)^2 by a relative
@@ -159,6 +192,22 @@ constexpr int NUM_XCD = 8;
 __device__ __forceinline__ int xcd_block() {
     int per = gridDim.x / NUM_XCD;
     return (blockIdx.x % NUM_XCD) * per + blockIdx.x / NUM_XCD;
+}
+
+// Tile of this workgroup in the tiled pair kernels.  The hardware hands
+// consecutive workgroups to the XCDs in turn; each XCD gets runs of TILE_RUN
+// consecutive tiles, the runs dealt out round-robin, so that neighbouring
+// tiles (which share window data) mostly share an L2 while every XCD gets
+// tiles from the whole domain (tiles differ in cost by region).  A bijection
+// of [0, gridDim.x) when gridDim.x is a multiple of NUM_XCD: the blocks
+// beyond the last whole run of each XCD take the remaining tiles in order.
+constexpr int TILE_RUN = 16;
+__device__ __forceinline__ int tile_index() {
+    const int xcd = blockIdx.x % NUM_XCD, k = blockIdx.x / NUM_XCD;
+    const int per = gridDim.x / NUM_XCD;
+    const int full = per / TILE_RUN * TILE_RUN;
+    if (k < full) return ((k / TILE_RUN) * NUM_XCD + xcd) * TILE_RUN + (k % TILE_RUN);
+    return full * NUM_XCD + xcd * (per - full) + (k - full);
 }
 
 // adapt_stress2 for one particle (code/3_SPH_material_2018.f90:2066-2140).

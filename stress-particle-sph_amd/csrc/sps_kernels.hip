@@ -23,6 +23,8 @@
 // and grad w of each pair (positions are fixed between the search and the
 // end of the step): one lane per receiver, no atomics, no stored pair list,
 // results bitwise equal to the reference.
+#include <type_traits>
+
 #include "sps_device.h"
 
 namespace sps {
@@ -420,11 +422,9 @@ __global__ void k_split(const unsigned *__restrict__ cnt, const unsigned *__rest
     unsigned long long M = split->M;
     unsigned long long o = off[p], c = cnt[p];
     if (o < M && M < o + c) split->sstar = p;
-    // a slot below first_rev discovers only pairs d < M; the wave's lowest
-    // qualifying slot is its first active lane (slots ascend with the lane)
-    const bool q = o + c > M;
-    const unsigned long long b = __ballot(q);
-    if (b && (int)(threadIdx.x & 63) == __ffsll((long long)b) - 1) atomicMin(&split->first_rev, p);
+    // off + cnt grows with the slot, so the first slot discovering a pair
+    // d >= M is the one slot with off <= M < off + cnt
+    if (o <= M && M < o + c) split->first_rev = p;
 }
 
 // First reversed partner of the split slot: its (M - off)-th discovery in
@@ -821,7 +821,7 @@ struct Tile {
 
 __device__ __forceinline__ Tile make_tile(int base, int count, int kr, const SearchView &V, const WalkCtx &C) {
     Tile T;
-    T.first = base + xcd_block() * TILE;
+    T.first = base + tile_index() * TILE;
     T.end = min(T.first + TILE, base + count);
     T.last = min(T.end, V.kstart[kr * C.L + C.g.ndivt]) - 1;
     T.fits = false;
@@ -839,68 +839,96 @@ __device__ __forceinline__ Tile make_tile(int base, int count, int kr, const Sea
     return T;
 }
 
-// LDS bookkeeping of the staging (halo cells: left column rows, then right).
-struct StageScratch {
-    int hglo[2 * TROWS], hpos[2 * TROWS], hcnt[2 * TROWS];
-    int total, mainlo, mainn;
+// Staging plan of one kind's window, in LDS: the strip's main run and the
+// halo cells (left column rows, then right column rows).
+struct StagePlan {
+    int hglo[2 * TROWS], hpos[2 * TROWS];
+    int nh, total, mainlo, mainn;
 };
 
-// Stage the window's partners of kind K: LDS table of the window cells (when
-// tab is given) and load(slot, lds_index) for every partner.  Returns false on every thread
-// when they exceed cap (nothing usable is staged then).  Ends with a barrier.
-template <int K, typename Load>
-__device__ __forceinline__ bool stage_kind(const Tile &T, const SearchView &V, const WalkCtx &C, int cap,
-                                           int *__restrict__ tab, StageScratch &sc, Load &&load) {
-    const int t = threadIdx.x;
+// Plan the window of kind K on wave `wave` of the block (the other waves
+// return at once): main run, halo cell runs and their LDS positions, and the
+// halo entries of the cell table when tab is given.  The caller puts a
+// barrier between planning and copying.
+template <int K>
+__device__ __forceinline__ void plan_kind(const Tile &T, const SearchView &V, const WalkCtx &C, int wave,
+                                          int *__restrict__ tab, StagePlan &sc) {
+    if ((int)(threadIdx.x >> 6) != wave) return;
+    const int t = threadIdx.x & 63;
     const int *ks = V.kstart + K * C.L;
     const int kbase = T.x0 * C.g.ndivy;
-    if (t < 64) {
-        int n = 0, lo = 0;
-        const int side = t / T.nrows, ly = t - side * T.nrows;
-        if (t < 2 * T.nrows) {
-            const int gx = side ? T.x0 + T.w : T.x0 - 1;
-            if (gx >= 0 && gx < C.g.ndivx) {
-                const int key = cell_key(gx, T.wy0 + ly, C.g);
-                lo = ks[key];
-                n = ks[key + 1] - lo;
-            }
-        }
-        int incl = n;
-        for (int o = 1; o < 64; o <<= 1) {
-            int v = __shfl_up(incl, o, 64);
-            if (t >= o) incl += v;
-        }
-        const int mainlo = ks[kbase + T.wy0 * T.w];
-        const int mainn = ks[kbase + (T.wy0 + T.nrows) * T.w] - mainlo;
-        if (t < 2 * T.nrows) {
-            sc.hglo[t] = lo;
-            sc.hpos[t] = mainn + incl - n;
-            sc.hcnt[t] = n;
-            if (tab) tab[ly * TCOLS + (side ? T.w + 1 : 0)] = (mainn + incl - n) | (n << 16);
-        }
-        if (t == 63) {
-            sc.total = mainn + incl;
-            sc.mainlo = mainlo;
-            sc.mainn = mainn;
+    int n = 0, lo = 0;
+    const int side = t / T.nrows, ly = t - side * T.nrows;
+    if (t < 2 * T.nrows) {
+        const int gx = side ? T.x0 + T.w : T.x0 - 1;
+        if (gx >= 0 && gx < C.g.ndivx) {
+            const int key = cell_key(gx, T.wy0 + ly, C.g);
+            lo = ks[key];
+            n = ks[key + 1] - lo;
         }
     }
-    __syncthreads();
+    const int mainlo = ks[kbase + T.wy0 * T.w];
+    const int mainn = ks[kbase + (T.wy0 + T.nrows) * T.w] - mainlo;
+    int incl = n;
+    for (int o = 1; o < 64; o <<= 1) {
+        int v = __shfl_up(incl, o, 64);
+        if (t >= o) incl += v;
+    }
+    if (t < 2 * T.nrows) {
+        sc.hglo[t] = lo;
+        sc.hpos[t] = mainn + incl - n;
+        if (tab) tab[ly * TCOLS + (side ? T.w + 1 : 0)] = (mainn + incl - n) | (n << 16);
+    }
+    if (t == 63) {
+        sc.nh = 2 * T.nrows;
+        sc.total = mainn + incl;
+        sc.mainlo = mainlo;
+        sc.mainn = mainn;
+    }
+}
+
+constexpr int NPER = (CAP + TILE - 1) / TILE;   // window partners per thread, at most
+
+// Copy the planned window of kind K (sc.total <= NPER * TILE) into LDS:
+// fetch(slot) -> item, put(lds_index, item), and the strip entries of the
+// cell table when tab is given.  Every thread issues all its loads before
+// its first store, so the copy costs one memory latency.  The caller ends
+// the staging with a barrier.
+template <int K, typename Fetch, typename Put>
+__device__ __forceinline__ void copy_kind(const Tile &T, const SearchView &V, const WalkCtx &C,
+                                          int *__restrict__ tab, const StagePlan &sc, Fetch &&fetch, Put &&put) {
+    const int t = threadIdx.x;
+    const int *ks = V.kstart + K * C.L;
     const int total = sc.total, mainlo = sc.mainlo, mainn = sc.mainn;
-    if (total > cap) {
-        __syncthreads();
-        return false;
+    const bool has_tab = tab != nullptr && t < T.nrows * T.w;
+    int tlo = 0, thi = 0, tpos = 0;
+    if (has_tab) {
+        const int ly = t / T.w, lx = t - ly * T.w;
+        const int key = T.x0 * C.g.ndivy + (T.wy0 + ly) * T.w + lx;
+        tlo = ks[key];
+        thi = ks[key + 1];
+        tpos = ly * TCOLS + lx + 1;
     }
-    for (int c = t; tab && c < T.nrows * T.w; c += blockDim.x) {
-        const int ly = c / T.w, lx = c - ly * T.w;
-        const int key = kbase + (T.wy0 + ly) * T.w + lx;
-        const int lo = ks[key];
-        tab[ly * TCOLS + lx + 1] = (lo - mainlo) | ((ks[key + 1] - lo) << 16);
+    decltype(fetch(0)) it[NPER];
+#pragma unroll
+    for (int k = 0; k < NPER; ++k) {
+        const int i = t + k * TILE;
+        if (i < mainn) {
+            it[k] = fetch(mainlo + i);
+        } else if (i < total) {   // halo: the last cell starting at or before i
+            int lo = 0, hi = sc.nh - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (sc.hpos[mid] <= i) lo = mid;
+                else hi = mid - 1;
+            }
+            it[k] = fetch(sc.hglo[lo] + (i - sc.hpos[lo]));
+        }
     }
-    for (int i = t; i < mainn; i += blockDim.x) load(mainlo + i, i);
-    if (t < 2 * T.nrows)
-        for (int q = 0; q < sc.hcnt[t]; ++q) load(sc.hglo[t] + q, sc.hpos[t] + q);
-    __syncthreads();
-    return true;
+    if (has_tab) tab[tpos] = (tlo - mainlo) | ((thi - tlo) << 16);
+#pragma unroll
+    for (int k = 0; k < NPER; ++k)
+        if (t + k * TILE < total) put(t + k * TILE, it[k]);
 }
 
 // ------------------------------------------------------------ step kernels
@@ -991,39 +1019,53 @@ __device__ __forceinline__ float get4(const float4 &v, int k) {
     return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w));
 }
 
-// One lane's list being written: four entries gathered in registers, then
-// stored as whole words.
-struct ListWriter {
-    unsigned long long idx;
-    float4 w, gx, gy;
-    int n;
-    __device__ __forceinline__ ListWriter() : idx(0), n(0) {
-        w = gx = gy = make_float4(0.f, 0.f, 0.f, 0.f);
-    }
-    __device__ __forceinline__ void flush(int tile, unsigned long long *ip, float4 *wp, float4 *gxp,
-                                          float4 *gyp) {
-        const size_t o = word_at(tile, n >> 2);
-        ip[o] = idx;
-        if (wp) wp[o] = w;
-        gxp[o] = gx;
-        gyp[o] = gy;
-    }
-    __device__ __forceinline__ void put(int tile, unsigned long long *ip, float4 *wp, float4 *gxp, float4 *gyp,
-                                        unsigned e, float wv, float gxv, float gyv) {
+// One lane's list being written: entries gathered four to a word in a
+// register, each full word stored.
+struct IdxWriter {
+    unsigned long long idx = 0;
+    int n = 0;
+    __device__ __forceinline__ void put(int tile, unsigned long long *ip, unsigned e) {
         const int k = n & 3;
         if (k == 0) idx = 0;
         idx |= (unsigned long long)e << (16 * k);
-        set4(w, k, wv);
-        set4(gx, k, gxv);
-        set4(gy, k, gyv);
-        if (k == 3 && n < LMAX) flush(tile, ip, wp, gxp, gyp);
+        if (k == 3 && n < LMAX) ip[word_at(tile, n >> 2)] = idx;
         ++n;
     }
-    __device__ __forceinline__ void finish(int tile, unsigned long long *ip, float4 *wp, float4 *gxp,
-                                           float4 *gyp) {
-        if ((n & 3) != 0 && n < LMAX) flush(tile, ip, wp, gxp, gyp);
+    __device__ __forceinline__ void finish(int tile, unsigned long long *ip) {
+        if ((n & 3) != 0 && n < LMAX) ip[word_at(tile, n >> 2)] = idx;
     }
 };
+
+// Kernel values of a lane's n list entries (n <= LMAX), computed a word at a
+// time with the lanes of a wave in step, so that the stores are whole
+// coalesced words: kernel() of the receiver-side dx, as the stage kernels'
+// global walk computes it.
+template <typename Fields>
+__device__ __forceinline__ void fill_kernel_values(const unsigned long long *ip, float4 *wp, float4 *gxp,
+                                                   float4 *gyp, int tile, int n, const Fields &A, double2 xr,
+                                                   double hr, double pi, KernelFactor &kf) {
+    const int ng = (n + 3) >> 2;
+    for (int g = 0; g < ng; ++g) {
+        const size_t o = word_at(tile, g);
+        const unsigned long long word = ip[o];
+        float wv[4] = {0.f, 0.f, 0.f, 0.f}, gxv[4] = {0.f, 0.f, 0.f, 0.f}, gyv[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (4 * g + k < n) {
+                const int ent = (int)((word >> (16 * k)) & 0xffff);
+                const int kind = ent >> 14, j = ent & 0x3fff;
+                const double2 xs = A.xk(kind, j);
+                const double dx0 = xr.x - xs.x, dx1 = xr.y - xs.y;
+                double mh;
+                const double rr = pair_distance(dx0, dx1, hr, A.hk(kind, j), mh);
+                cubic_kernel_f(rr, dx0, dx1, mh, kf.get(mh, pi), wv[k], gxv[k], gyv[k]);
+            }
+        }
+        if (wp) wp[o] = make_float4(wv[0], wv[1], wv[2], wv[3]);
+        gxp[o] = make_float4(gxv[0], gxv[1], gxv[2], gxv[3]);
+        gyp[o] = make_float4(gyv[0], gyv[1], gyv[2], gyv[3]);
+    }
+}
 
 // Positions and smoothing lengths from the global state (j is a slot).
 struct XHFields {
@@ -1044,38 +1086,42 @@ template <int KR>
 __global__ __launch_bounds__(TILE) void k_count_lists(SearchView V, const double2 *__restrict__ x,
                                                       const double *__restrict__ hsml,
                                                       unsigned *__restrict__ cnt, Split *__restrict__ split,
-                                                      PairLists PL, int base, int count) {
-    __shared__ StageScratch sc;
+                                                      PairLists PL, double h_uniform, int base, int count) {
+    __shared__ StagePlan pl[3];
     constexpr const LdsLayout &LY = LAY_COUNT;
     const WalkCtx C = walk_ctx(V);
     const Tile T = make_tile(base, count, KR, V, C);
     if (T.first >= T.end) return;
-    const int tile = xcd_block();
+    const int tile = tile_index();
     const int ltile = KR == KIND_NODE ? tile : PL.ntile_n + tile;
     const int r = T.first + (int)threadIdx.x;
     int self_base = 0;
     bool lds = T.fits;
-    auto load = [&](auto kt) {
-        constexpr int K = decltype(kt)::value;
-        return [&](int g, int i) {
-            lds_at<double2>(LAY_COUNT.x[K])[i] = x[g];
-            lds_at<double>(LAY_COUNT.h[K])[i] = hsml[g];
+    if (lds) {
+        plan_kind<KIND_NODE>(T, V, C, 0, lds_at<int>(LY.tab[KIND_NODE]), pl[KIND_NODE]);
+        plan_kind<KIND_SP>(T, V, C, 1, lds_at<int>(LY.tab[KIND_SP]), pl[KIND_SP]);
+        plan_kind<KIND_DUMMY>(T, V, C, 2, lds_at<int>(LY.tab[KIND_DUMMY]), pl[KIND_DUMMY]);
+        __syncthreads();
+        lds = pl[KIND_NODE].total <= CAP && pl[KIND_SP].total <= CAP && pl[KIND_DUMMY].total <= CAP_D;
+        self_base = pl[KR].mainlo;
+    }
+    if (lds) {
+        auto fetch = [&](int g) { return make_double3(x[g].x, x[g].y, hsml[g]); };
+        auto put = [&](auto kt) {
+            constexpr int K = decltype(kt)::value;
+            return [&](int i, const double3 &v) {
+                lds_at<double2>(LAY_COUNT.x[K])[i] = make_double2(v.x, v.y);
+                lds_at<double>(LAY_COUNT.h[K])[i] = v.z;
+            };
         };
-    };
-    if (lds) {
-        lds = stage_kind<KIND_NODE>(T, V, C, CAP, lds_at<int>(LY.tab[KIND_NODE]), sc, load(KindTag<KIND_NODE>{}));
-        if (KR == KIND_NODE) self_base = sc.mainlo;
+        copy_kind<KIND_NODE>(T, V, C, lds_at<int>(LY.tab[KIND_NODE]), pl[KIND_NODE], fetch, put(KindTag<KIND_NODE>{}));
+        copy_kind<KIND_SP>(T, V, C, lds_at<int>(LY.tab[KIND_SP]), pl[KIND_SP], fetch, put(KindTag<KIND_SP>{}));
+        copy_kind<KIND_DUMMY>(T, V, C, lds_at<int>(LY.tab[KIND_DUMMY]), pl[KIND_DUMMY], fetch,
+                              put(KindTag<KIND_DUMMY>{}));
+        __syncthreads();
     }
-    if (lds) {
-        lds = stage_kind<KIND_SP>(T, V, C, CAP, lds_at<int>(LY.tab[KIND_SP]), sc, load(KindTag<KIND_SP>{}));
-        if (KR == KIND_SP) self_base = sc.mainlo;
-    }
-    if (lds)
-        lds = stage_kind<KIND_DUMMY>(T, V, C, CAP_D, lds_at<int>(LY.tab[KIND_DUMMY]), sc,
-                                     load(KindTag<KIND_DUMMY>{}));
-    unsigned fwd = 0;
-    unsigned t0 = 0, t1 = 0, t2 = 0, t3 = 0, t6 = 0, t9 = 0;   // pairs by pint_type
-    ListWriter la, lb;
+    unsigned fk[3] = {0, 0, 0};   // pairs this receiver discovers, by partner kind
+    IdxWriter la, lb;
     const bool active = r < T.end && V.cell[r] < C.g.ndivt;
     if (active) {
         int cx, cy;
@@ -1086,125 +1132,166 @@ __global__ __launch_bounds__(TILE) void k_count_lists(SearchView V, const double
         const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx + 1 < C.g.ndivx ? cx + 1 : C.g.ndivx - 1;
         const int y0 = cy > 0 ? cy - 1 : 0, y1 = cy + 1 < C.g.ndivy ? cy + 1 : C.g.ndivy - 1;
         const int self = lds ? r - self_base : r;
-        KernelFactor kf;
-        auto walk = [&](const auto &Cs, const auto &A, bool lists) __attribute__((always_inline)) {
+        // h_uniform > 0: every particle has this smoothing length
+        const AcceptBounds ub = accept_bounds(h_uniform, h_uniform);
+        auto walk = [&](const auto &Cs, const auto &A, bool lists, auto uni) __attribute__((always_inline)) {
             for (int y = y0; y <= y1; ++y)
                 for (int xx = x0; xx <= x1; ++xx) {
                     const int crm = y * C.g.ndivx + xx;
+                    const bool cell_after = crm > crm_r, same_cell = crm == crm_r;
 #pragma unroll
                     for (int ks = 0; ks < 3; ++ks) {
                         const int2 rg = Cs.range(ks, xx, y);
+                        // discovered by this receiver: a later cell, or this cell and a later
+                        // kind, or this kind and a later slot (slots ascend with the window
+                        // index inside a cell and kind)
+                        const bool kind_after = cell_after || (same_cell && ks > KR);
+                        const bool slot_after = same_cell && ks == KR;
                         for (int j = rg.x; j < rg.x + rg.y; ++j) {
                             if (ks == KR && j == self) continue;
                             const double2 xs = A.xk(ks, j);
-                            const double dx0 = xr.x - xs.x, dx1 = xr.y - xs.y;
-                            double mh;
-                            const double rr = pair_distance(dx0, dx1, hr, A.hk(ks, j), mh);
-                            if (rr < 0.0) continue;
-                            // slots ascend with the window index inside a cell and kind
-                            const bool after =
-                                crm > crm_r || (crm == crm_r && (ks > KR || (ks == KR && j > self)));
-                            if (after) {
-                                fwd++;
-                                switch (pair_type(KR, ks)) {
-                                    case 1: t1++; break;
-                                    case 2: t2++; break;
-                                    case 3: t3++; break;
-                                    case 6: t6++; break;
-                                    case 9: t9++; break;
-                                    default: t0++; break;
-                                }
+                            if constexpr (decltype(uni)::value) {
+                                if (!accepted_by(xr.x - xs.x, xr.y - xs.y, ub)) continue;
+                            } else {
+                                if (!accepted_by(xr.x - xs.x, xr.y - xs.y, accept_bounds(hr, A.hk(ks, j)))) continue;
                             }
+                            fk[ks] += (kind_after || (slot_after && j > self)) ? 1u : 0u;
                             if (!lists) continue;
-                            const bool to_a = in_list_a(KR, ks), to_b = KR == KIND_NODE && ks == KIND_NODE;
-                            if (!to_a && !to_b) continue;
-                            float w, gwx, gwy;
-                            cubic_kernel_f(rr, dx0, dx1, mh, kf.get(mh, V.pi), w, gwx, gwy);
                             const unsigned e = (unsigned)((ks << 14) | j);
-                            if (to_a)
-                                la.put(ltile, PL.ia, PL.wa, PL.gxa, PL.gya, e, w, gwx, gwy);
-                            else
-                                lb.put(ltile, PL.ib, nullptr, PL.gxb, PL.gyb, e, w, gwx, gwy);
+                            if (in_list_a(KR, ks))
+                                la.put(ltile, PL.ia, e);
+                            else if (KR == KIND_NODE && ks == KIND_NODE)
+                                lb.put(ltile, PL.ib, e);
                         }
                     }
                 }
         };
-        if (lds)
-            walk(LdsCells<LY>{T.x0 - 1, T.wy0}, LdsFields<LY>{}, true);
-        else
-            walk(GlobalCells{V.kstart, C.g, C.L}, XHFields{x, hsml}, false);
+        using Uni = std::integral_constant<bool, true>;
+        using Gen = std::integral_constant<bool, false>;
         if (lds) {
-            la.finish(ltile, PL.ia, PL.wa, PL.gxa, PL.gya);
-            if (KR == KIND_NODE) lb.finish(ltile, PL.ib, nullptr, PL.gxb, PL.gyb);
+            if (h_uniform > 0)
+                walk(LdsCells<LY>{T.x0 - 1, T.wy0}, LdsFields<LY>{}, true, Uni{});
+            else
+                walk(LdsCells<LY>{T.x0 - 1, T.wy0}, LdsFields<LY>{}, true, Gen{});
+        } else {
+            walk(GlobalCells{V.kstart, C.g, C.L}, XHFields{x, hsml}, false, Gen{});
         }
-        cnt[V.proc[r]] = fwd;
+        if (lds) {
+            la.finish(ltile, PL.ia);
+            if (KR == KIND_NODE) lb.finish(ltile, PL.ib);
+        }
+        cnt[V.proc[r]] = fk[0] + fk[1] + fk[2];
     }
-    unsigned tv[6] = {t0, t1, t2, t3, t6, t9};
-    const int tid[6] = {0, 1, 2, 3, 6, 9};
+    // per-type histogram (one pint_type per partner kind of a receiver kind):
+    // wave reduction, then one atomic per wave
 #pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        unsigned v = tv[k];
+    for (int k = 0; k < 3; ++k) {
+        unsigned v = fk[k];
         for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-        if ((threadIdx.x & 63) == 0 && v) atomicAdd(&split->n_by_type[tid[k]], (unsigned long long)v);
+        if ((threadIdx.x & 63) == 0 && v) atomicAdd(&split->n_by_type[pair_type(KR, k)], (unsigned long long)v);
     }
     const bool over = la.n > LMAX || lb.n > LMAX;
     const bool any_over = lds ? __syncthreads_or(over) : true;
     if (threadIdx.x == 0) PL.ok[ltile] = !any_over;
+    if (!any_over && active) {
+        const double2 xr = x[r];
+        const double hr = hsml[r];
+        KernelFactor kf;
+        fill_kernel_values(PL.ia, PL.wa, PL.gxa, PL.gya, ltile, la.n, LdsFields<LY>{}, xr, hr, V.pi, kf);
+        if (KR == KIND_NODE)
+            fill_kernel_values(PL.ib, nullptr, PL.gxb, PL.gyb, ltile, lb.n, LdsFields<LY>{}, xr, hr, V.pi, kf);
+    }
     if (r < T.end) {
         PL.deg_a[r] = (unsigned short)(la.n < LMAX ? la.n : LMAX);
         if (KR == KIND_NODE) PL.deg_b[r] = (unsigned short)(lb.n < LMAX ? lb.n : LMAX);
     }
 }
 
-// Call f(kind_tag, fields, j, w, dW/dx_r, dW/dy_r) over a receiver's list
-// entries of the kinds in KMASK, forwards or backwards.  NW / NG: the
-// kernel reads w / the gradient.  The next words are in flight while the
-// entries of the current ones are summed.
-template <int KMASK, bool NW, bool NG, typename Fields, typename F>
-__device__ __forceinline__ void walk_list(const unsigned long long *__restrict__ ip, const float4 *__restrict__ wp,
-                                          const float4 *__restrict__ gxp, const float4 *__restrict__ gyp,
-                                          int tile, int deg, bool rev, const Fields &A, F &&f) {
-    const int ng = (deg + 3) >> 2;
-    if (ng == 0) return;
-    const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
-    size_t o = word_at(tile, rev ? ng - 1 : 0);
-    unsigned long long ni = ip[o];
-    float4 nw = NW ? wp[o] : z, nx = NG ? gxp[o] : z, ny = NG ? gyp[o] : z;
-    for (int gi = 0; gi < ng; ++gi) {
-        const int g = rev ? ng - 1 - gi : gi;
-        const unsigned long long ci = ni;
-        const float4 cw = nw, cx = nx, cy = ny;
-        if (gi + 1 < ng) {
-            o = word_at(tile, rev ? g - 1 : g + 1);
-            ni = ip[o];
-            if (NW) nw = wp[o];
-            if (NG) {
-                nx = gxp[o];
-                ny = gyp[o];
+// The ne entries of one list word, forwards or backwards; every word index
+// is a compile-time constant.
+template <int KMASK, bool NW, bool NG, bool REV, typename Fields, typename F>
+__device__ __forceinline__ void walk_word(unsigned long long word, const float4 &cw, const float4 &cx,
+                                          const float4 &cy, int ne, const Fields &A, F &f) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int kk = REV ? 3 - q : q;
+        if (kk >= ne) continue;
+        const int ent = (int)((word >> (16 * kk)) & 0xffff);
+        const int kind = ent >> 14, j = ent & 0x3fff;
+        if (!((KMASK >> kind) & 1)) continue;
+        const float w = NW ? get4(cw, kk) : 0.0f;
+        const float gx = NG ? get4(cx, kk) : 0.0f, gy = NG ? get4(cy, kk) : 0.0f;
+        if constexpr ((KMASK & MASK_NODE) != 0)
+            if (kind == KIND_NODE) {
+                f(KindTag<KIND_NODE>{}, A, j, w, gx, gy);
+                continue;
             }
-        }
-        const int ne = deg - 4 * g < 4 ? deg - 4 * g : 4;
-        for (int k = 0; k < ne; ++k) {
-            const int kk = rev ? ne - 1 - k : k;
-            const int ent = (int)((ci >> (16 * kk)) & 0xffff);
-            const int kind = ent >> 14, j = ent & 0x3fff;
-            if (!((KMASK >> kind) & 1)) continue;
-            const float w = NW ? get4(cw, kk) : 0.0f;
-            const float gx = NG ? get4(cx, kk) : 0.0f, gy = NG ? get4(cy, kk) : 0.0f;
-            if constexpr ((KMASK & MASK_NODE) != 0)
-                if (kind == KIND_NODE) {
-                    f(KindTag<KIND_NODE>{}, A, j, w, gx, gy);
-                    continue;
-                }
-            if constexpr ((KMASK & MASK_SP) != 0)
-                if (kind == KIND_SP) {
-                    f(KindTag<KIND_SP>{}, A, j, w, gx, gy);
-                    continue;
-                }
-            if constexpr ((KMASK & MASK_DUMMY) != 0) f(KindTag<KIND_DUMMY>{}, A, j, w, gx, gy);
-        }
+        if constexpr ((KMASK & MASK_SP) != 0)
+            if (kind == KIND_SP) {
+                f(KindTag<KIND_SP>{}, A, j, w, gx, gy);
+                continue;
+            }
+        if constexpr ((KMASK & MASK_DUMMY) != 0) f(KindTag<KIND_DUMMY>{}, A, j, w, gx, gy);
     }
 }
+
+// One list word and its kernel values as loaded.
+struct ListWord {
+    unsigned long long idx;
+    float4 w, gx, gy;
+};
+
+// A receiver's list walk with two words in flight: start() issues the loads
+// of the first two words (callers start before the window staging so that
+// these loads overlap it), run() calls f(kind_tag, fields, j, w, dW/dx_r,
+// dW/dy_r) over the entries of the kinds in KMASK, forwards or backwards
+// (rev is the same for every lane).  NW / NG: the kernel reads w / the
+// gradient.
+template <int KMASK, bool NW, bool NG> struct ListWalk {
+    const unsigned long long *ip;
+    const float4 *wp, *gxp, *gyp;
+    int tile, deg, ng;
+    bool rev;
+    ListWord a, b;
+
+    __device__ __forceinline__ void load(ListWord &d, int g) const {
+        const size_t o = word_at(tile, rev ? ng - 1 - g : g);
+        d.idx = ip[o];
+        if (NW) d.w = wp[o];
+        if (NG) {
+            d.gx = gxp[o];
+            d.gy = gyp[o];
+        }
+    }
+    __device__ __forceinline__ void start(const unsigned long long *ip_, const float4 *wp_, const float4 *gxp_,
+                                          const float4 *gyp_, int tile_, int deg_, bool rev_) {
+        ip = ip_;
+        wp = wp_;
+        gxp = gxp_;
+        gyp = gyp_;
+        tile = tile_;
+        deg = deg_;
+        rev = rev_;
+        ng = (deg + 3) >> 2;
+        const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+        a = b = ListWord{0ull, z, z, z};
+        if (ng > 0) load(a, 0);
+        if (ng > 1) load(b, 1);
+    }
+    template <typename Fields, typename F> __device__ __forceinline__ void run(const Fields &A, F &&f) {
+        for (int gi = 0; gi < ng; ++gi) {
+            const ListWord c = a;
+            a = b;
+            if (gi + 2 < ng) load(b, gi + 2);
+            const int g = rev ? ng - 1 - gi : gi;
+            const int ne = deg - 4 * g < 4 ? deg - 4 * g : 4;
+            if (rev)
+                walk_word<KMASK, NW, NG, true>(c.idx, c.w, c.gx, c.gy, ne, A, f);
+            else
+                walk_word<KMASK, NW, NG, false>(c.idx, c.w, c.gx, c.gy, ne, A, f);
+        }
+    }
+};
 
 // Workgroup-wide tiling decision of a tiled stage kernel: the tile fits the
 // window layout and no receiver of it needs per-pair pass classification.
@@ -1220,26 +1307,40 @@ __device__ __forceinline__ bool tile_uses_lds(const Tile &T, const SearchView &V
 // stress and plastic strain from the stress points, then adapt_stress2.
 __global__ __launch_bounds__(TILE) void k_spu_nodes(StateBufs S, SearchView V, PairLists PL, int adapt,
                                                     double tanfi, double coh, int nn) {
-    __shared__ StageScratch sc;
+    __shared__ StagePlan pl;
     constexpr const LdsLayout &LY = LAY_SPU_NODES;
     const WalkCtx C = walk_ctx(V);
     const Tile T = make_tile(0, nn, KIND_NODE, V, C);
     if (T.first >= T.end) return;
     const int r = T.first + (int)threadIdx.x;
-    const int tile = xcd_block();
+    const int tile = tile_index();
+    const bool inside = r < T.end && V.cell[r] < C.g.ndivt;
+    ListWalk<MASK_SP, true, false> lw;
+    lw.start(PL.ia, PL.wa, PL.gxa, PL.gya, tile, inside ? PL.deg_a[r] : 0, !C.any_fwd && C.any_rev);
     bool check;
     bool lds = tile_uses_lds(T, V, C, r, check) && PL.ok[tile];
-    if (lds)
-        lds = stage_kind<KIND_SP>(T, V, C, CAP, nullptr, sc, [&](int g, int i) {
-            lds_at<double>(LY.m[KIND_SP])[i] = S.mass[g];
-            lds_at<double>(LY.rho[KIND_SP])[i] = S.rho[g];
-            const double4 st = S.stress[g];
-            lds_at<double>(LY.s0[KIND_SP])[i] = st.x;
-            lds_at<double>(LY.s1[KIND_SP])[i] = st.y;
-            lds_at<double>(LY.s2[KIND_SP])[i] = st.z;
-            lds_at<double>(LY.s3[KIND_SP])[i] = st.w;
-            lds_at<double>(LY.iv[KIND_SP])[i] = S.ivar1[g];
-        });
+    if (lds) {
+        plan_kind<KIND_SP>(T, V, C, 0, nullptr, pl);
+        __syncthreads();
+        lds = pl.total <= CAP;
+    }
+    if (lds) {
+        struct Item {
+            double m, rho, iv;
+            double4 st;
+        };
+        copy_kind<KIND_SP>(T, V, C, nullptr, pl, [&](int g) { return Item{S.mass[g], S.rho[g], S.ivar1[g], S.stress[g]}; },
+                           [&](int i, const Item &v) {
+                               lds_at<double>(LY.m[KIND_SP])[i] = v.m;
+                               lds_at<double>(LY.rho[KIND_SP])[i] = v.rho;
+                               lds_at<double>(LY.s0[KIND_SP])[i] = v.st.x;
+                               lds_at<double>(LY.s1[KIND_SP])[i] = v.st.y;
+                               lds_at<double>(LY.s2[KIND_SP])[i] = v.st.z;
+                               lds_at<double>(LY.s3[KIND_SP])[i] = v.st.w;
+                               lds_at<double>(LY.iv[KIND_SP])[i] = v.iv;
+                           });
+        __syncthreads();
+    }
     if (r >= T.end) return;
     double t0 = 0, t1 = 0, t2 = 0, t3 = 0, tv = 0, norm = 0;
     auto body = [&](auto kt, const auto &A, int j, float wf, float, float) __attribute__((always_inline)) {
@@ -1254,14 +1355,12 @@ __global__ __launch_bounds__(TILE) void k_spu_nodes(StateBufs S, SearchView V, P
         tv = tv + A.iv(kt, j) * h1;
         norm = norm + (w * ms) / rs;
     };
-    const int key_r = V.cell[r];
-    if (key_r < C.g.ndivt) {
+    if (inside) {
         if (lds) {
-            walk_list<MASK_SP, true, false>(PL.ia, PL.wa, PL.gxa, PL.gya, tile, PL.deg_a[r],
-                                            !C.any_fwd && C.any_rev, LdsFields<LY>{}, body);
+            lw.run(LdsFields<LY>{}, body);
         } else {
             int cx, cy;
-            cell_xy(key_r, C.g, cx, cy);
+            cell_xy(V.cell[r], C.g, cx, cy);
             const SplitClass cls{check, check ? V.proc[r] : 0, V.proc, V.cnt, V.off, C.sp};
             walk_stencil<MASK_SP>(GlobalCells{V.kstart, C.g, C.L}, GlobalFields{S}, C.g, cx, cy, S.x[r],
                                   S.hsml[r], KIND_NODE, r, C.any_fwd ? check : C.any_rev, C.any_fwd, V.pi, cls,
@@ -1290,43 +1389,66 @@ __global__ __launch_bounds__(TILE) void k_spu_nodes(StateBufs S, SearchView V, P
 __global__ __launch_bounds__(TILE) void k_sp_update(StateBufs S, Work W, SearchView V, PairLists PL,
                                                     DevParams P, double time_sph, double f2, int deriv,
                                                     int adapt, int nn, int nt) {
-    __shared__ StageScratch sc;
+    __shared__ StagePlan pl[2];
     constexpr const LdsLayout &LY = LAY_SP_UPDATE;
     const WalkCtx C = walk_ctx(V);
     const Tile T = make_tile(nn, nt - nn, KIND_SP, V, C);
     if (T.first >= T.end) return;
     const int r = T.first + (int)threadIdx.x;
-    const int tile = xcd_block();
+    const int tile = tile_index();
     const int ltile = PL.ntile_n + tile;
+    const bool inside = r < T.end && V.cell[r] < C.g.ndivt;
+    const int deg = inside ? PL.deg_a[r] : 0;
+    const bool rev_l = !C.any_fwd && C.any_rev;
+    ListWalk<MASK_NODE, true, false> lw;
+    lw.start(PL.ia, PL.wa, PL.gxa, PL.gya, ltile, deg, rev_l);
     bool check;
     bool lds = tile_uses_lds(T, V, C, r, check) && PL.ok[ltile];
-    if (lds)
-        lds = stage_kind<KIND_NODE>(T, V, C, CAP, nullptr, sc, [&](int g, int i) {
-            lds_at<double2>(LY.x[KIND_NODE])[i] = S.x[g];
-            lds_at<double2>(LY.v[KIND_NODE])[i] = S.vel[g];
-            lds_at<double>(LY.m[KIND_NODE])[i] = S.mass[g];
-            lds_at<double>(LY.rho[KIND_NODE])[i] = S.rho[g];
-            lds_at<double>(LY.mr[KIND_NODE])[i] = S.mass[g] / S.rho[g];
-        });
-    if (lds && deriv)
-        lds = stage_kind<KIND_DUMMY>(T, V, C, CAP_D, nullptr, sc, [&](int g, int i) {
-            lds_at<double2>(LY.x[KIND_DUMMY])[i] = S.x[g];
-            lds_at<double>(LY.m[KIND_DUMMY])[i] = S.mass[g];
-            lds_at<double>(LY.rho[KIND_DUMMY])[i] = S.rho[g];
-            lds_at<float>(LY.wall)[i] = S.wallpos[g];
-            lds_at<float>(LY.horiz)[i] = S.horiz[g];
-        });
+    if (lds) {
+        plan_kind<KIND_NODE>(T, V, C, 0, nullptr, pl[0]);
+        if (deriv) plan_kind<KIND_DUMMY>(T, V, C, 1, nullptr, pl[1]);
+        __syncthreads();
+        lds = pl[0].total <= CAP && (!deriv || pl[1].total <= CAP_D);
+    }
+    if (lds) {
+        struct NItem {
+            double2 x, v;
+            double m, rho;
+        };
+        copy_kind<KIND_NODE>(T, V, C, nullptr, pl[0], [&](int g) { return NItem{S.x[g], S.vel[g], S.mass[g], S.rho[g]}; },
+                             [&](int i, const NItem &n) {
+                                 lds_at<double2>(LY.x[KIND_NODE])[i] = n.x;
+                                 lds_at<double2>(LY.v[KIND_NODE])[i] = n.v;
+                                 lds_at<double>(LY.m[KIND_NODE])[i] = n.m;
+                                 lds_at<double>(LY.rho[KIND_NODE])[i] = n.rho;
+                                 lds_at<double>(LY.mr[KIND_NODE])[i] = n.m / n.rho;
+                             });
+        if (deriv) {
+            struct DItem {
+                double2 x;
+                double m, rho;
+                float wall, horiz;
+            };
+            copy_kind<KIND_DUMMY>(T, V, C, nullptr, pl[1],
+                                  [&](int g) { return DItem{S.x[g], S.mass[g], S.rho[g], S.wallpos[g], S.horiz[g]}; },
+                                  [&](int i, const DItem &d) {
+                                      lds_at<double2>(LY.x[KIND_DUMMY])[i] = d.x;
+                                      lds_at<double>(LY.m[KIND_DUMMY])[i] = d.m;
+                                      lds_at<double>(LY.rho[KIND_DUMMY])[i] = d.rho;
+                                      lds_at<float>(LY.wall)[i] = d.wall;
+                                      lds_at<float>(LY.horiz)[i] = d.horiz;
+                                  });
+        }
+        __syncthreads();
+    }
     if (r >= T.end) return;
-    const int key_r = V.cell[r];
-    const bool inside = key_r < C.g.ndivt;
     int cx = 0, cy = 0;
-    if (inside && !lds) cell_xy(key_r, C.g, cx, cy);
+    if (inside && !lds) cell_xy(V.cell[r], C.g, cx, cy);
     const double2 xr = S.x[r];
     const double hr = S.hsml[r];
     const LdsFields<LY> LF{};
-    const int deg = lds && inside ? PL.deg_a[r] : 0;
     const SplitClass cls{check, check ? V.proc[r] : 0, V.proc, V.cnt, V.off, C.sp};
-    const bool rev_l = !C.any_fwd && C.any_rev, rev_g = C.any_fwd ? check : C.any_rev;
+    const bool rev_g = C.any_fwd ? check : C.any_rev;
 
     // stress_point_update
     double v0 = 0, v1 = 0, norm = 0;
@@ -1341,7 +1463,7 @@ __global__ __launch_bounds__(TILE) void k_sp_update(StateBufs S, Work W, SearchV
     };
     if (inside) {
         if (lds)
-            walk_list<MASK_NODE, true, false>(PL.ia, PL.wa, PL.gxa, PL.gya, ltile, deg, rev_l, LF, spu);
+            lw.run(LF, spu);
         else
             walk_stencil<MASK_NODE>(GlobalCells{V.kstart, C.g, C.L}, GlobalFields{S}, C.g, cx, cy, xr, hr,
                                     KIND_SP, r, rev_g, C.any_fwd, V.pi, cls, spu);
@@ -1405,10 +1527,11 @@ __global__ __launch_bounds__(TILE) void k_sp_update(StateBufs S, Work W, SearchV
         }
     };
     if (inside) {
-        if (lds)
-            walk_list<MASK_NODE | MASK_DUMMY, false, true>(PL.ia, PL.wa, PL.gxa, PL.gya, ltile, deg, rev_l, LF,
-                                                           grad);
-        else
+        if (lds) {
+            ListWalk<MASK_NODE | MASK_DUMMY, false, true> lg;
+            lg.start(PL.ia, PL.wa, PL.gxa, PL.gya, ltile, deg, rev_l);
+            lg.run(LF, grad);
+        } else
             walk_stencil<MASK_NODE | MASK_DUMMY>(GlobalCells{V.kstart, C.g, C.L}, GlobalFields{S}, C.g, cx, cy,
                                                  xr, hr, KIND_SP, r, rev_g, C.any_fwd, V.pi, cls, grad);
     }
@@ -1511,18 +1634,21 @@ __device__ __forceinline__ double2 wall_force(const StateBufs &S, const SearchVi
 __global__ __launch_bounds__(TILE) void k_deriv_nodes(StateBufs S, Work W, SearchView V, PairLists PL,
                                                       DevParams P, double gx_grav, double gy_grav, double f2,
                                                       int *__restrict__ err, int nn) {
-    __shared__ StageScratch sc;
+    __shared__ StagePlan pl[2];
     constexpr const LdsLayout &LY = LAY_DERIV_NODES;
     const WalkCtx C = walk_ctx(V);
     const Tile T = make_tile(0, nn, KIND_NODE, V, C);
     if (T.first >= T.end) return;
     const int r = T.first + (int)threadIdx.x;
-    const int tile = xcd_block();
-    bool check;
-    const bool tiled = tile_uses_lds(T, V, C, r, check) && PL.ok[tile];
+    const int tile = tile_index();
     const bool active = r < T.end;
     const int key_r = active ? V.cell[r] : C.g.ndivt;
     const bool inside = key_r < C.g.ndivt;
+    const bool rev_l = !C.any_fwd && C.any_rev;
+    ListWalk<MASK_SP | MASK_DUMMY, false, true> la;
+    la.start(PL.ia, PL.wa, PL.gxa, PL.gya, tile, inside ? PL.deg_a[r] : 0, rev_l);
+    bool check;
+    const bool tiled = tile_uses_lds(T, V, C, r, check) && PL.ok[tile];
     int cx = 0, cy = 0;
     if (inside) cell_xy(key_r, C.g, cx, cy);
     const int ra = active ? r : T.first;   // a valid slot for the idle lanes' loads
@@ -1532,27 +1658,40 @@ __global__ __launch_bounds__(TILE) void k_deriv_nodes(StateBufs S, Work W, Searc
     const double rr2 = rhor * rhor;
     const double or0 = sr.x / rr2, or1 = sr.y / rr2, or2 = sr.z / rr2;   // own sigma / rho^2
     const SplitClass cls{check, check ? V.proc[r] : 0, V.proc, V.cnt, V.off, C.sp};
-    const bool rev_l = !C.any_fwd && C.any_rev, rev_g = C.any_fwd ? check : C.any_rev;
+    const bool rev_g = C.any_fwd ? check : C.any_rev;
     const LdsFields<LY> LF{};
 
     // phase 1: stress gradient over stress points (type 1) and dummies (type 6)
     bool lds = tiled;
-    if (lds)
-        lds = stage_kind<KIND_SP>(T, V, C, CAP, nullptr, sc, [&](int g, int i) {
-            lds_at<double2>(LY.x[KIND_SP])[i] = S.x[g];
-            lds_at<double>(LY.m[KIND_SP])[i] = S.mass[g];
-            const double rs = S.rho[g], rs2 = rs * rs;
-            lds_at<double>(LY.rho[KIND_SP])[i] = rs;
-            const double4 st = S.stress[g];
-            lds_at<double>(LY.p0[KIND_SP])[i] = st.x / rs2;
-            lds_at<double>(LY.p1[KIND_SP])[i] = st.y / rs2;
-            lds_at<double>(LY.p2[KIND_SP])[i] = st.z / rs2;
-        });
-    if (lds)
-        lds = stage_kind<KIND_DUMMY>(T, V, C, CAP_D, nullptr, sc, [&](int g, int i) {
-            lds_at<double>(LY.m[KIND_DUMMY])[i] = S.mass[g];
-            lds_at<double>(LY.rho[KIND_DUMMY])[i] = S.rho[g];
-        });
+    if (lds) {
+        plan_kind<KIND_SP>(T, V, C, 0, nullptr, pl[0]);
+        plan_kind<KIND_DUMMY>(T, V, C, 1, nullptr, pl[1]);
+        __syncthreads();
+        lds = pl[0].total <= CAP && pl[1].total <= CAP_D;
+    }
+    if (lds) {
+        struct SItem {
+            double2 x;
+            double m, rho;
+            double4 st;
+        };
+        copy_kind<KIND_SP>(T, V, C, nullptr, pl[0], [&](int g) { return SItem{S.x[g], S.mass[g], S.rho[g], S.stress[g]}; },
+                           [&](int i, const SItem &v) {
+                               lds_at<double2>(LY.x[KIND_SP])[i] = v.x;
+                               lds_at<double>(LY.m[KIND_SP])[i] = v.m;
+                               lds_at<double>(LY.rho[KIND_SP])[i] = v.rho;
+                               const double rs2 = v.rho * v.rho;
+                               lds_at<double>(LY.p0[KIND_SP])[i] = v.st.x / rs2;
+                               lds_at<double>(LY.p1[KIND_SP])[i] = v.st.y / rs2;
+                               lds_at<double>(LY.p2[KIND_SP])[i] = v.st.z / rs2;
+                           });
+        copy_kind<KIND_DUMMY>(T, V, C, nullptr, pl[1], [&](int g) { return make_double2(S.mass[g], S.rho[g]); },
+                              [&](int i, const double2 &v) {
+                                  lds_at<double>(LY.m[KIND_DUMMY])[i] = v.x;
+                                  lds_at<double>(LY.rho[KIND_DUMMY])[i] = v.y;
+                              });
+        __syncthreads();
+    }
     double q00 = 0, q01 = 0, q10 = 0, q11 = 0, q20 = 0, q21 = 0;        // grad2_tmp(s,b)
     double A0 = 0, A1 = 0, A2 = 0, A3 = 0;
     int ndum = 0;
@@ -1588,8 +1727,7 @@ __global__ __launch_bounds__(TILE) void k_deriv_nodes(StateBufs S, Work W, Searc
     };
     if (inside) {
         if (lds)
-            walk_list<MASK_SP | MASK_DUMMY, false, true>(PL.ia, PL.wa, PL.gxa, PL.gya, tile, PL.deg_a[r], rev_l,
-                                                         LF, grad);
+            la.run(LF, grad);
         else
             walk_stencil<MASK_SP | MASK_DUMMY>(GlobalCells{V.kstart, C.g, C.L}, GlobalFields{S}, C.g, cx, cy,
                                                xr, hr, KIND_NODE, r, rev_g, C.any_fwd, V.pi, cls, grad);
@@ -1598,16 +1736,29 @@ __global__ __launch_bounds__(TILE) void k_deriv_nodes(StateBufs S, Work W, Searc
     // phase 2: artificial viscosity over node partners (type 3), single precision
     float ax = 0.0f, ay = 0.0f, visc = 0.0f;
     if (P.do_av) {
+        ListWalk<MASK_NODE, false, true> lb;
+        lb.start(PL.ib, nullptr, PL.gxb, PL.gyb, tile, inside ? PL.deg_b[r] : 0, rev_l);
         bool lds2 = tiled;
         if (lds2) {
-            __syncthreads();   // phase-1 walks are done with the shared bytes
-            lds2 = stage_kind<KIND_NODE>(T, V, C, CAP, nullptr, sc, [&](int g, int i) {
-                lds_at<double2>(LY.x[KIND_NODE])[i] = S.x[g];
-                lds_at<double2>(LY.v[KIND_NODE])[i] = S.vel[g];
-                lds_at<double>(LY.h[KIND_NODE])[i] = S.hsml[g];
-                lds_at<double>(LY.m[KIND_NODE])[i] = S.mass[g];
-                lds_at<double>(LY.rho[KIND_NODE])[i] = S.rho[g];
-            });
+            plan_kind<KIND_NODE>(T, V, C, 0, nullptr, pl[0]);
+            __syncthreads();   // also: the phase-1 walks are done with the shared bytes
+            lds2 = pl[0].total <= CAP;
+        }
+        if (lds2) {
+            struct NItem {
+                double2 x, v;
+                double h, m, rho;
+            };
+            copy_kind<KIND_NODE>(T, V, C, nullptr, pl[0],
+                                 [&](int g) { return NItem{S.x[g], S.vel[g], S.hsml[g], S.mass[g], S.rho[g]}; },
+                                 [&](int i, const NItem &n) {
+                                     lds_at<double2>(LY.x[KIND_NODE])[i] = n.x;
+                                     lds_at<double2>(LY.v[KIND_NODE])[i] = n.v;
+                                     lds_at<double>(LY.h[KIND_NODE])[i] = n.h;
+                                     lds_at<double>(LY.m[KIND_NODE])[i] = n.m;
+                                     lds_at<double>(LY.rho[KIND_NODE])[i] = n.rho;
+                                 });
+            __syncthreads();
         }
         auto av = [&](auto kt, const auto &A, int j, float, float gxf, float gyf) __attribute__((always_inline)) {
             double2 xs = A.x(kt, j), vs = A.vel(kt, j);
@@ -1631,7 +1782,7 @@ __global__ __launch_bounds__(TILE) void k_deriv_nodes(StateBufs S, Work W, Searc
         };
         if (inside) {
             if (lds2)
-                walk_list<MASK_NODE, false, true>(PL.ib, nullptr, PL.gxb, PL.gyb, tile, PL.deg_b[r], rev_l, LF, av);
+                lb.run(LF, av);
             else
                 walk_stencil<MASK_NODE>(GlobalCells{V.kstart, C.g, C.L}, GlobalFields{S}, C.g, cx, cy, xr, hr,
                                         KIND_NODE, r, rev_g, C.any_fwd, V.pi, cls, av);

@@ -17,6 +17,7 @@ if ROOT not in sys.path:
 
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 ORACLE_LIB = os.path.join(ROOT, "oracle", "libsps_oracle.so")
+PRODUCT_LIB = os.path.join(ROOT, "stress-particle-sph_amd", "libsps_hip.so")
 
 
 def pytest_configure(config):
@@ -62,3 +63,19 @@ def describe_diff(a, b):
     d = np.abs(a - b)
     scale = max(np.abs(a).max(), 1e-300)
     return f"max|diff| {d.max():.3e}, rel {d.max() / scale:.3e}, n differing {(a != b).sum()}"
+
+
+def open_varied_h(sps, deck, lib_path, factor=1.05):
+    """A deck's initial state with every third particle's smoothing length
+    scaled by `factor`: a case where pairs have different h, which the
+    reference handles through mhsml = (h_i + h_j)/2."""
+    import importlib
+    import numpy as np
+    setup = importlib.import_module("stress-particle-sph_amd.setup")
+    parts = setup.build_particles(deck)
+    st = sps.initial_state(parts)
+    st["hsml"] = st["hsml"] * np.where(np.arange(len(st["hsml"])) % 3 == 0, factor, 1.0)
+    sim = sps.Simulation(sps.make_params(deck, parts), lib_path)
+    sim.set_state(st)
+    sim.deck, sim.parts = deck, parts
+    return sim

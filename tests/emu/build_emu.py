@@ -1,0 +1,116 @@
+"""CPU build of the product's HIP sources (test infrastructure).
+
+Compiles stress-particle-sph_amd/csrc/sps_api.hip (which includes the
+kernels) with g++ against the host stand-in in tests/emu/hip/, after three
+textual rewrites of a copy of the sources:
+
+* each launch `kern<<<grid, block, lds, stream>>>(args)` becomes
+  `emu::launch(grid, block, lds, [&] { kern(args); }, sync)`;
+* the dynamic LDS array becomes the launch's buffer of exactly the requested
+  size, so an access past it is reported;
+* a kernel's local layout reference used as a template argument is replaced
+  by the layout it names (g++ does not take a local reference there).
+
+The library implements include/sps.h like the product and the oracle, so the
+Python package drives it the same way.  Built with AddressSanitizer by
+default: run the Python process with LD_PRELOAD of libasan.
+
+    python tests/emu/build_emu.py [out.so] [--no-asan]
+"""
+import os
+import re
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+SRC = os.path.join(ROOT, "stress-particle-sph_amd", "csrc")
+BUILD = os.path.join(HERE, "_build")
+
+# kernels that use barriers or wave exchanges (their threads run as fibers)
+SYNC = ("k_bounds_partial", "k_bounds_final", "k_count", "k_spu_nodes", "k_sp_update", "k_deriv_nodes",
+        "k_scan_local", "k_scan_sums")
+
+
+def split_top(s):
+    out, depth, cur = [], 0, ""
+    for ch in s:
+        depth += ch in "([{"
+        depth -= ch in ")]}"
+        if ch == "," and depth == 0:
+            out.append(cur)
+            cur = ""
+        else:
+            cur += ch
+    out.append(cur)
+    return [c.strip() for c in out]
+
+
+def rewrite_launches(s):
+    out, i = [], 0
+    while True:
+        k = s.find("<<<", i)
+        if k < 0:
+            out.append(s[i:])
+            return "".join(out)
+        m = re.search(r"([A-Za-z_][A-Za-z_0-9:]*(?:<[^<>]*>)?)\s*$", s[:k])
+        name, start = m.group(1), m.start(1)
+        e = s.find(">>>", k)
+        cfg = split_top(s[k + 3:e])
+        p = s.find("(", e)
+        depth, q = 0, p
+        while True:
+            depth += s[q] == "("
+            depth -= s[q] == ")"
+            if depth == 0:
+                break
+            q += 1
+        args = s[p + 1:q]
+        if name == "KERN":          # the launch macro: decide by the kernel's name
+            sync = "emu::needs_sync(#KERN)"
+        else:
+            sync = "true" if name.split("<")[0].startswith(SYNC) else "false"
+        out.append(s[i:start])
+        out.append(f"emu::launch({cfg[0]}, {cfg[1]}, {cfg[2]}, [&] {{ {name}({args}); }}, {sync})")
+        i = q + 1
+
+
+def generate(gen):
+    os.makedirs(gen, exist_ok=True)
+    api = open(os.path.join(SRC, "sps_api.hip")).read()
+    api = api.replace('#include "../../include/sps.h"', '#include "sps.h"')
+    api = '#include "hip/hip_runtime.h"\n#define sps_lds (::emu::lds_base)\n' + rewrite_launches(api)
+    open(os.path.join(gen, "sps_api_emu.cpp"), "w").write(api)
+    ker = open(os.path.join(SRC, "sps_kernels.hip")).read()
+    decl = "extern __shared__ __align__(16) unsigned char sps_lds[];"
+    if decl not in ker:
+        raise RuntimeError("dynamic LDS declaration not found")
+    lines, cur = [], None
+    for ln in ker.replace(decl, "").split("\n"):
+        m = re.search(r"constexpr const LdsLayout &LY = (LAY_\w+);", ln)
+        if m:
+            cur = m.group(1)
+        elif cur:
+            ln = ln.replace("<LY>", "<" + cur + ">")
+        lines.append(ln)
+    open(os.path.join(gen, "sps_kernels.hip"), "w").write("\n".join(lines))
+    open(os.path.join(gen, "sps_device.h"), "w").write(open(os.path.join(SRC, "sps_device.h")).read())
+
+
+def build(out=None, asan=True):
+    gen = os.path.join(BUILD, "gen")
+    generate(gen)
+    out = out or os.path.join(BUILD, "libsps_emu.so")
+    flags = ["-std=c++17", "-O1", "-g", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math",
+             "-U_FORTIFY_SOURCE", "-D_FORTIFY_SOURCE=0", "-Wno-unknown-pragmas", "-Wno-unused-result",
+             "-I" + HERE, "-I" + os.path.join(ROOT, "include"), "-I" + gen]
+    if asan:
+        flags += ["-fsanitize=address", "-fno-omit-frame-pointer"]
+    subprocess.run(["g++"] + flags + ["-o", out, os.path.join(gen, "sps_api_emu.cpp"),
+                                      os.path.join(HERE, "emu_rt.cpp")], check=True)
+    return out
+
+
+if __name__ == "__main__":
+    args = [a for a in sys.argv[1:] if not a.startswith("--")]
+    print(build(args[0] if args else None, asan="--no-asan" not in sys.argv))

@@ -1,0 +1,104 @@
+// Test infrastructure: host stand-in for the HIP runtime calls and device
+// intrinsics that stress-particle-sph_amd/csrc uses, so that its kernels can
+// be compiled by g++ (with AddressSanitizer) and executed on the CPU, one
+// block at a time (emu_rt.cpp).  Device memory is host memory; streams,
+// events and copies are immediate.  Not part of the product.
+#pragma once
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <vector>
+#include <algorithm>
+#include <ucontext.h>
+
+#define __global__
+#define __device__
+#define __host__
+#define __forceinline__ inline
+#define __launch_bounds__(...)
+#define __shared__ static
+#define __align__(n) alignas(n)
+
+using std::min;
+using std::max;
+
+struct dim3e { unsigned x = 0, y = 0, z = 0; };
+extern dim3e threadIdx, blockIdx, blockDim, gridDim;
+
+struct double2 { double x, y; };
+struct double3 { double x, y, z; };
+struct double4 { double x, y, z, w; };
+struct int2 { int x, y; };
+struct float4 { float x, y, z, w; };
+inline double2 make_double2(double a, double b) { return {a, b}; }
+inline double3 make_double3(double a, double b, double c) { return {a, b, c}; }
+inline double4 make_double4(double a, double b, double c, double d) { return {a, b, c, d}; }
+inline int2 make_int2(int a, int b) { return {a, b}; }
+inline float4 make_float4(float a, float b, float c, float d) { return {a, b, c, d}; }
+
+namespace emu {
+bool needs_sync(const char *name);
+void sync_block();
+int sync_block_or(int p);
+void wave_put(const void *v, int bytes);
+void wave_get(int lane, void *v, int bytes);
+// run body for every thread of grid x block; sync: the kernel uses barriers
+// or wave exchanges (its threads then run as fibers)
+void launch(unsigned grid, unsigned block, size_t lds, const std::function<void()> &body, bool sync);
+extern unsigned char *lds_base;   // the launch's dynamic LDS
+extern size_t lds_bytes;
+}  // namespace emu
+
+inline void __syncthreads() { emu::sync_block(); }
+inline int __syncthreads_or(int p) { return emu::sync_block_or(p); }
+template <typename T> T __shfl_up(T v, unsigned d, int w = 64) {
+    emu::wave_put(&v, sizeof(T));
+    int lane = threadIdx.x & 63, seg = lane & ~(w - 1);
+    T r = v;
+    if ((int)(lane - d) >= seg) emu::wave_get(lane - d, &r, sizeof(T));
+    else emu::wave_get(lane, &r, sizeof(T));
+    emu::wave_put(nullptr, 0);
+    return r;
+}
+template <typename T> T __shfl_down(T v, unsigned d, int w = 64) {
+    emu::wave_put(&v, sizeof(T));
+    int lane = threadIdx.x & 63, seg = lane & ~(w - 1);
+    T r = v;
+    if ((int)(lane + d) < seg + w) emu::wave_get(lane + d, &r, sizeof(T));
+    else emu::wave_get(lane, &r, sizeof(T));
+    emu::wave_put(nullptr, 0);
+    return r;
+}
+template <typename T> T atomicAdd(T *p, T v) { T o = *p; *p = o + v; return o; }
+inline int atomicOr(int *p, int v) { int o = *p; *p = o | v; return o; }
+inline int atomicMin(int *p, int v) { int o = *p; *p = std::min(o, v); return o; }
+
+typedef int hipError_t;
+enum { hipSuccess = 0, hipErrorUnknown = 999 };
+enum hipMemcpyKind { hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice };
+typedef void *hipStream_t;
+typedef void *hipEvent_t;
+#define hipStreamNonBlocking 1
+#define hipHostMallocDefault 0
+inline hipError_t hipMalloc(void **p, size_t n) { *p = malloc(n ? n : 1); return *p ? hipSuccess : hipErrorUnknown; }
+inline hipError_t hipFree(void *p) { free(p); return hipSuccess; }
+inline hipError_t hipHostMalloc(void **p, size_t n, unsigned) { *p = malloc(n); return hipSuccess; }
+inline hipError_t hipHostFree(void *p) { free(p); return hipSuccess; }
+inline hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, hipMemcpyKind, hipStream_t) { memmove(d, s, n); return hipSuccess; }
+inline hipError_t hipMemsetAsync(void *d, int v, size_t n, hipStream_t) { memset(d, v, n); return hipSuccess; }
+inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+inline hipError_t hipGetLastError() { return hipSuccess; }
+inline hipError_t hipSetDevice(int) { return hipSuccess; }
+inline hipError_t hipGetDevice(int *d) { *d = 0; return hipSuccess; }
+inline hipError_t hipGetDeviceCount(int *n) { *n = 1; return hipSuccess; }
+inline hipError_t hipStreamCreate(hipStream_t *s) { *s = nullptr; return hipSuccess; }
+inline hipError_t hipStreamCreateWithFlags(hipStream_t *s, unsigned) { *s = nullptr; return hipSuccess; }
+inline hipError_t hipStreamDestroy(hipStream_t) { return hipSuccess; }
+inline hipError_t hipEventCreate(hipEvent_t *e) { *e = (void *)1; return hipSuccess; }
+inline hipError_t hipEventDestroy(hipEvent_t) { return hipSuccess; }
+inline hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
+inline hipError_t hipEventElapsedTime(float *ms, hipEvent_t, hipEvent_t) { *ms = 0; return hipSuccess; }
+inline const char *hipGetErrorString(hipError_t) { return "emulated"; }

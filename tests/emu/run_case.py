@@ -1,0 +1,38 @@
+"""Run one deck on the CPU build of the HIP kernels and on the oracle and
+compare state and pair list bit for bit (run under AddressSanitizer by
+tests/test_emu.py).  Prints one line; exit status 0 when equal."""
+import importlib
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sps = importlib.import_module("stress-particle-sph_amd")
+
+
+def main(emu_lib, oracle_lib, case, steps):
+    deck_name, _, variant = case.partition(":")
+    deck = sps.read_deck(os.path.join(ROOT, "stress-particle-sph_amd", "decks", deck_name))
+    if variant == "hvar":       # smoothing lengths that differ between particles
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from conftest import open_varied_h
+        emu = open_varied_h(sps, deck, emu_lib)
+        ora = open_varied_h(sps, deck, oracle_lib)
+    else:
+        emu = sps.Simulation.from_deck(deck, lib_path=emu_lib)
+        ora = sps.Simulation.from_deck(deck, lib_path=oracle_lib)
+    emu.step(steps)
+    ora.step(steps)
+    a, b = emu.get_state(), ora.get_state()
+    bad = [k for k in a if not np.array_equal(a[k].view(np.uint8), b[k].view(np.uint8))]
+    pa, pb = emu.get_pairs(), ora.get_pairs()
+    bad += [f"pairs.{k}" for k in pa if not np.array_equal(pa[k].view(np.uint8), pb[k].view(np.uint8))]
+    print(f"{case} {steps} steps: " + ("bitwise equal" if not bad else "differ in " + ", ".join(bad)))
+    return 1 if bad else 0
+
+
+if __name__ == "__main__":
+    sys.exit(main(sys.argv[1], sys.argv[2], sys.argv[3], int(sys.argv[4])))

@@ -1,0 +1,48 @@
+"""The product's HIP kernels, compiled for the CPU with AddressSanitizer
+(tests/emu: a host stand-in for the HIP runtime and a fiber scheduler for
+barriers and wave exchanges), against the oracle.
+
+This checks on the CPU what the GPU parity tests check on MI355X -- the
+kernel sources reproduce the reference bit for bit -- and, through the
+sanitizer, that no kernel reads or writes outside its buffers, its LDS or
+its stack on these decks.
+"""
+import os
+import subprocess
+import sys
+
+import pytest
+
+from conftest import ORACLE_LIB, ROOT
+
+sys.path.insert(0, os.path.join(ROOT, "tests", "emu"))
+import build_emu  # noqa: E402
+
+
+def _asan_runtime():
+    try:
+        p = subprocess.run(["g++", "-print-file-name=libasan.so"], capture_output=True, text=True, check=True)
+    except (OSError, subprocess.CalledProcessError):
+        return None
+    path = p.stdout.strip()
+    return path if os.path.isabs(path) and os.path.exists(path) else None
+
+
+@pytest.fixture(scope="module")
+def emu_lib():
+    if _asan_runtime() is None:
+        pytest.skip("g++ with AddressSanitizer runtime not available")
+    if not os.path.exists(ORACLE_LIB):
+        pytest.skip("oracle library not built")
+    return build_emu.build()
+
+
+@pytest.mark.parametrize("deck_name,steps", [("t1_tiny", 60), ("t2_cspm", 60), ("t3_elastic", 60),
+                                             ("c1_collapse", 3), ("t2_cspm:hvar", 40)])
+def test_cpu_build_of_kernels_matches_oracle(emu_lib, deck_name, steps):
+    env = dict(os.environ, LD_PRELOAD=_asan_runtime(),
+               ASAN_OPTIONS="detect_leaks=0:detect_stack_use_after_return=0:halt_on_error=1")
+    p = subprocess.run([sys.executable, os.path.join(ROOT, "tests", "emu", "run_case.py"), emu_lib, ORACLE_LIB,
+                        deck_name, str(steps)], env=env, capture_output=True, text=True, timeout=900)
+    print(p.stdout[-2000:], p.stderr[-4000:])
+    assert p.returncode == 0, p.stderr[-4000:] or p.stdout

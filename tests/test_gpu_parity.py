@@ -12,7 +12,7 @@ import sys
 import numpy as np
 import pytest
 
-from conftest import ORACLE_LIB, bits_equal, deck_dir, describe_diff, golden, load_pkg
+from conftest import ORACLE_LIB, PRODUCT_LIB, bits_equal, deck_dir, describe_diff, golden, load_pkg, open_varied_h
 
 pytestmark = pytest.mark.gpu
 
@@ -71,6 +71,23 @@ def test_small_decks_bitwise(deck_name, checkpoints):
         done = st
         compare_pairs(gpu, ora, f"{deck_name} step {st}")
         compare_states(gpu, ora, f"{deck_name} step {st}")
+    gpu.close()
+    ora.close()
+
+
+def test_varied_smoothing_lengths_bitwise():
+    """Smoothing lengths that differ between particles (every third one 5 %
+    longer): the search's bounds and the kernel's mhsml then vary per pair."""
+    say("[gpu parity] t2_cspm with varied h")
+    sps = load_pkg()
+    deck = sps.read_deck(deck_dir("t2_cspm"))
+    gpu = open_varied_h(sps, deck, PRODUCT_LIB)
+    ora = open_varied_h(sps, deck, ORACLE_LIB)
+    for st, n in ((1, 1), (20, 19), (60, 40)):
+        gpu.step(n)
+        ora.step(n)
+        compare_pairs(gpu, ora, f"varied h step {st}")
+        compare_states(gpu, ora, f"varied h step {st}")
     gpu.close()
     ora.close()
 
