@@ -15,6 +15,15 @@ position update.  The metric is pair interactions/s = (pairs found by each
 step's search, summed over the timed steps) / wall time.  Inputs are resident
 on the GPU before the timed region; nothing is copied to the host inside it.
 
+Beside it, the mixed-precision form (single-precision pair arithmetic with
+compensated sums, double-precision state and return map) is timed on the
+same workload, and its error against the double-precision form (whose
+results are the reference's bit for bit) is measured after --acc-steps steps
+from the same initial state: displacement, stress and failure-surface
+location, next to the spread of the double-precision form itself under a
+single-precision perturbation of the initial positions (the conditioning of
+the problem).
+
 The CPU baseline is the oracle (a C restatement of the reference, 1 thread)
 on a bounded sample of the same workload.
 """
@@ -82,9 +91,9 @@ def measured_traffic(deck_name, kernel):
     return None if k is None else k["bytes"]
 
 
-def run_gpu(args, sps, deck):
+def run_gpu(args, sps, deck, mode=0):
     import torch
-    sim = sps.Simulation.from_deck(deck)
+    sim = sps.Simulation.from_deck(deck, mode=mode)
     counts = (sim.parts.nnode, sim.parts.nstress, sim.parts.ndummy)
     print(f"[bench] deck {args.deck}: {counts[0]} nodes, {counts[1]} stress points, {counts[2]} dummies",
           file=sys.stderr, flush=True)
@@ -107,6 +116,44 @@ def run_gpu(args, sps, deck):
     return sim, counts, pairs, elapsed, prof, st
 
 
+def form_errors(args, sps, deck):
+    """Errors of the mixed-precision form against the double-precision one
+    after args.acc_steps steps from rest, and the double-precision form's own
+    spread under a 1e-7 relative perturbation of the initial positions."""
+    import importlib
+    import numpy as np
+    acc = importlib.import_module("stress-particle-sph_amd.accuracy")
+    setup = importlib.import_module("stress-particle-sph_amd.setup")
+    parts = setup.build_particles(deck)
+
+    def run(mode, perturb=0.0):
+        st = sps.initial_state(parts)
+        if perturb:
+            n = parts.nnode + parts.nstress
+            rng = np.random.default_rng(2024)
+            st["x"][:n] = st["x"][:n] * (1 + perturb * rng.standard_normal((n, 2)))
+        sim = sps.Simulation(sps.make_params(deck, parts, mode=mode), sps.PRODUCT_LIB)
+        sim.set_state(st)
+        sim.deck, sim.parts = deck, parts
+        done = 0
+        while done < args.acc_steps:
+            k = min(500, args.acc_steps - done)
+            sim.step(k)
+            done += k
+            print(f"[bench] accuracy run mode {mode} perturb {perturb:g}: {done} steps", file=sys.stderr,
+                  flush=True)
+        out = sim.get_state()
+        sim.close()
+        return out
+
+    x0 = sps.initial_state(parts)["x"]
+    ref = run(sps.MODE_ORDERED)
+    mixed = acc.compare_states(run(sps.MODE_FAST), ref, x0)
+    spread = acc.compare_states(run(sps.MODE_ORDERED, 1e-7), ref, x0)
+    rnd = lambda d: {k: float(f"{v:.4g}") for k, v in d.items()}
+    return {"steps": args.acc_steps, "mixed_vs_fp64": rnd(mixed), "fp64_perturbed_1e-7_vs_fp64": rnd(spread)}
+
+
 def run_cpu(args, sps, deck, oracle_path):
     sim = sps.Simulation.from_deck(deck, lib_path=oracle_path)
     pairs = 0
@@ -127,6 +174,9 @@ def main():
     ap.add_argument("--deck", default="c2_slope_dx01")
     ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps timed for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision form")
+    ap.add_argument("--acc-steps", type=int, default=2000,
+                    help="steps of the mixed-vs-double accuracy runs (0: skip them)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (5 times) and exit; its "
                          "FETCH_SIZE / WRITE_SIZE counters calibrate the HBM traffic of the kernels")
@@ -168,6 +218,18 @@ def main():
     print("[bench] kernel breakdown (ms per step): " +
           ", ".join(f"{k} {v['ms_per_step']}" for k, v in breakdown.items()), file=sys.stderr, flush=True)
     sim.close()
+    forms = {"fp64": {"ms_per_step": round(ms_per_step, 4), "pairs_per_s": round(value, 1)}}
+    if not args.no_mixed:
+        msim, _, mpairs, melapsed, mprof, _ = run_gpu(args, sps, deck, mode=sps.MODE_FAST)
+        msim.close()
+        forms["mixed"] = {
+            "ms_per_step": round(1e3 * melapsed / args.steps, 4), "pairs_per_s": round(mpairs / melapsed, 1),
+            "speedup_vs_fp64": round(elapsed / melapsed, 3),
+            "kernels": {k: round(ms / args.steps, 4) for k, (n, ms) in sorted(mprof.items(), key=lambda kv: -kv[1][1])}}
+        print(f"[bench] mixed form: {forms['mixed']['ms_per_step']} ms/step "
+              f"({forms['mixed']['speedup_vs_fp64']}x)", file=sys.stderr, flush=True)
+        if args.acc_steps > 0:
+            forms["mixed"]["accuracy"] = form_errors(args, sps, deck)
     cpu = None
     if not args.no_cpu:
         oracle = os.path.join(ROOT, "oracle", "libsps_oracle.so")
@@ -197,6 +259,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": breakdown,
+        "forms": forms,
     }
     print(json.dumps(out), flush=True)
 

@@ -49,10 +49,17 @@ enum {
     SPS_ENUMERIC = -6     /* non-finite value detected in the state */
 };
 
-/* summation/ordering mode of the pair loops */
+/* Arithmetic of the pair loops.  SPS_MODE_FAST has no counterpart in the
+ * reference: single-precision pair arithmetic (kernel values computed in
+ * single precision from window-relative positions) with compensated (Kahan)
+ * sums, in the pair lists' forward order; positions, velocities, the stress
+ * state, the CSPM solve, the Drucker-Prager terms and the return map stay in
+ * double precision.  Its results differ from the reference's by rounding
+ * (tolerance-checked against the oracle); the oracle runs every mode as
+ * SPS_MODE_ORDERED. */
 enum {
     SPS_MODE_ORDERED = 0, /* reference list order, bitwise parity target */
-    SPS_MODE_FAST = 1     /* cell order, reassociated sums (tolerance-checked) */
+    SPS_MODE_FAST = 1     /* mixed precision, as above */
 };
 
 #define SPS_MAX_TCURVE 64

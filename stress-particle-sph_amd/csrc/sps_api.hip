@@ -22,6 +22,7 @@
 
 #include "../../include/sps.h"
 #include "sps_kernels.hip"
+#include "sps_mixed.hip"
 
 using namespace sps;
 
@@ -139,6 +140,7 @@ struct sps_handle {
     DBuf<unsigned char> tile_ok;
     int ntile_n = 0, ntile_s = 0;
     double h_uniform = 0.0;   // the smoothing length of every particle, or 0 when they differ
+    bool mixed = false;       // SPS_MODE_FAST: the mixed-precision stage kernels
     DBuf<int> err;
     // host-visible summary of the last search
     unsigned long long n_pairs = 0, m_pairs = 0, m_prev = 0;
@@ -257,7 +259,7 @@ static int check_params(const sps_params *p) {
     if (ntype_eco > 1 && ncrit <= 5) return fail(SPS_EINVAL, "Perzyna models (ncrit<=5) are not supported");
     if (p->ic_grav && (p->tcurve_npts < 2 || p->tcurve_npts > SPS_MAX_TCURVE))
         return fail(SPS_EINVAL, "gravity time curve needs 2..%d points", SPS_MAX_TCURVE);
-    if (p->mode != SPS_MODE_ORDERED) return fail(SPS_EINVAL, "only SPS_MODE_ORDERED is available in this library");
+    if (p->mode != SPS_MODE_ORDERED && p->mode != SPS_MODE_FAST) return fail(SPS_EINVAL, "unknown mode %d", p->mode);
     return 0;
 }
 
@@ -300,6 +302,7 @@ extern "C" int sps_create(const sps_params *params, sps_handle **out) {
     h->nt = h->nn + h->ns;
     h->nt2 = h->nt + h->nd;
     h->niv = params->nint_vars;
+    h->mixed = params->mode == SPS_MODE_FAST;
     (void)hipGetDevice(&h->device);
     DevParams &d = h->dp;
     d.nn = h->nn; d.ns = h->ns; d.nd = h->nd; d.nd2 = h->nd2; d.nt = h->nt; d.nt2 = h->nt2;
@@ -496,11 +499,18 @@ static int search(sps_handle *h, bool fresh_list) {
     {
         const SearchView V{h->cell.p, h->kstart.p, h->proc.p, h->cnt.p, h->off.p, h->grid.p, h->split.p, h->p.pi};
         const PairLists PL = h->lists();
-        if (nn > 0)
-            LAUNCH_LDS(K_COUNT_PAIRS, nblk8(nn), TILE, LAY_COUNT.bytes, k_count_lists<KIND_NODE>, V, C.x.p, C.hsml.p,
-                       h->cnt.p, h->split.p, PL, h->h_uniform, 0, nn);
-        if (nt > nn)
-            LAUNCH_LDS(K_COUNT_PAIRS, nblk8(nt - nn), TILE, LAY_COUNT.bytes, k_count_lists<KIND_SP>, V, C.x.p,
+        // the mixed-precision stage kernels compute the kernel values themselves
+        if (nn > 0 && !h->mixed)
+            LAUNCH_LDS(K_COUNT_PAIRS, nblk8(nn), TILE, LAY_COUNT.bytes, (k_count_lists<KIND_NODE, true>), V, C.x.p,
+                       C.hsml.p, h->cnt.p, h->split.p, PL, h->h_uniform, 0, nn);
+        if (nn > 0 && h->mixed)
+            LAUNCH_LDS(K_COUNT_PAIRS, nblk8(nn), TILE, LAY_COUNT.bytes, (k_count_lists<KIND_NODE, false>), V, C.x.p,
+                       C.hsml.p, h->cnt.p, h->split.p, PL, h->h_uniform, 0, nn);
+        if (nt > nn && !h->mixed)
+            LAUNCH_LDS(K_COUNT_PAIRS, nblk8(nt - nn), TILE, LAY_COUNT.bytes, (k_count_lists<KIND_SP, true>), V, C.x.p,
+                       C.hsml.p, h->cnt.p, h->split.p, PL, h->h_uniform, nn, nt - nn);
+        if (nt > nn && h->mixed)
+            LAUNCH_LDS(K_COUNT_PAIRS, nblk8(nt - nn), TILE, LAY_COUNT.bytes, (k_count_lists<KIND_SP, false>), V, C.x.p,
                        C.hsml.p, h->cnt.p, h->split.p, PL, h->h_uniform, nn, nt - nn);
         if (nt2 > nt)
             LAUNCH(K_COUNT_PAIRS, nblk8(nt2 - nt), BLOCK, k_count, C.x.p, C.hsml.p, h->cell.p, h->kstart.p, h->proc.p,
@@ -542,9 +552,24 @@ static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
     const double gxg = h->p.ic_grav ? factg * h->p.cgrav[0] : 0.0;
     const double gyg = h->p.ic_grav ? factg * h->p.cgrav[1] : 0.0;
     LAUNCH(K_RK_INIT, nblk(nt2), BLOCK, k_rk_init, SB, W, nn, nt, nt2);
+    const float hu = (float)h->h_uniform;
     for (int st = 0; st < 4; ++st) {
         const double c1 = f1rk[st] * (dt);
         LAUNCH(K_STAGE_BEGIN, nblk(nt), BLOCK, k_stage_begin, SB, W, c1, adapt, tanfi, coh, nn, nt);
+        if (h->mixed) {
+            if (nn > 0)
+                LAUNCH_LDS(K_SPU_NODES, nblk8(nn), TILE, LAY_SPU_NODES_MIX.bytes, k_spu_nodes_mix, SB, V, PL, adapt,
+                           tanfi, coh, hu, nn);
+            if (nt > nn)
+                LAUNCH_LDS(K_SP_UPDATE, nblk8(nt - nn), TILE, LAY_SP_UPDATE_MIX.bytes, k_sp_update_mix, SB, W, V, PL,
+                           h->dp, time_sph, f2rk[st], 1, adapt, hu, nn, nt);
+            if (nn > 0)
+                LAUNCH_LDS(K_DERIV_NODES, nblk8(nn), TILE, LAY_DERIV_NODES_MIX.bytes, k_deriv_nodes_mix, SB, W, V, PL,
+                           h->dp, gxg, gyg, f2rk[st], h->err.p, hu, nn);
+            if (st == 3 && nt2 > nt)
+                LAUNCH(K_DUMMY_STRESS, nblk(nt2 - nt), BLOCK, k_dummy_stress, SB, V, nt, nt2);
+            continue;
+        }
         if (nn > 0)
             LAUNCH_LDS(K_SPU_NODES, nblk8(nn), TILE, LAY_SPU_NODES.bytes, k_spu_nodes, SB, V, PL, adapt, tanfi, coh, nn);
         if (nt > nn)
@@ -557,6 +582,17 @@ static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
             LAUNCH(K_DUMMY_STRESS, nblk(nt2 - nt), BLOCK, k_dummy_stress, SB, V, nt, nt2);
     }
     LAUNCH(K_RK_FINAL, nblk(nt), BLOCK, k_rk_final, SB, W, dt, adapt, tanfi, coh, nn, nt);
+    if (h->mixed) {
+        if (nn > 0)
+            LAUNCH_LDS(K_SPU_NODES, nblk8(nn), TILE, LAY_SPU_NODES_MIX.bytes, k_spu_nodes_mix, SB, V, PL, adapt, tanfi,
+                       coh, hu, nn);
+        if (nt > nn)
+            LAUNCH_LDS(K_SP_UPDATE, nblk8(nt - nn), TILE, LAY_SP_UPDATE_MIX.bytes, k_sp_update_mix, SB, W, V, PL, h->dp,
+                       time_sph, 0.0, 0, adapt, hu, nn, nt);
+        LAUNCH(K_POSITION, nblk(nt), BLOCK, k_position, SB, W, dt, h->p.update_x, nn, nt);
+        HIP_TRY(hipGetLastError());
+        return 0;
+    }
     if (nn > 0)
         LAUNCH_LDS(K_SPU_NODES, nblk8(nn), TILE, LAY_SPU_NODES.bytes, k_spu_nodes, SB, V, PL, adapt, tanfi, coh, nn);
     if (nt > nn)

@@ -325,4 +325,57 @@ __device__ __forceinline__ void wall_force_term(double ri0, double ri1, double h
     t1 = scale * f2 * f * (r1 / (r2 * r2));
 }
 
+// ------------------------------------------------------- mixed precision
+// The mixed-precision form sums pair terms in single precision with
+// compensation (Kahan 1965: the rounding error of every addition is carried
+// into the next), keeping positions, the stress state and the return map in
+// double precision.
+struct KSum {
+    float s = 0.0f, c = 0.0f;   // sum; minus the low part lost so far
+    __device__ __forceinline__ void add(float v) {
+        const float y = v - c;
+        const float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+    __device__ __forceinline__ double get() const { return (double)s - (double)c; }
+};
+
+// 15 / (7 pi) of the 2-D cubic spline's normalisation.
+constexpr float CUBIC_2D_NORM = 0.682092613f;
+
+// The cubic spline (the formula of kernel(), 2_...:1468-1492) evaluated in
+// single precision from a relative position dx = x_r - x_s and the pair's
+// smoothing length, with the hardware reciprocal and square root: w and
+// d W / d x_r.  Zero beyond 2h.
+__device__ __forceinline__ void cubic_kernel_mix(float dx0, float dx1, float mh, float &w, float &gx,
+                                                 float &gy) {
+    const float r2 = dx0 * dx0 + dx1 * dx1;
+    const float ih = __builtin_amdgcn_rcpf(mh);
+    const float r = __builtin_amdgcn_sqrtf(r2);
+    const float q = r * ih;
+    const float fac = CUBIC_2D_NORM * (ih * ih);
+    if (q <= 1.0f) {
+        w = fac * ((2.0f / 3.0f) - q * q + 0.5f * (q * (q * q)));
+        const float c = fac * (-2.0f + 1.5f * q) * (ih * ih);
+        gx = c * dx0;
+        gy = c * dx1;
+    } else if (q < 2.0f) {
+        const float t = 2.0f - q;
+        w = fac * (1.0f / 6.0f) * (t * (t * t));
+        const float c = -(fac * 0.5f * (t * t) * ih) * __builtin_amdgcn_rcpf(r);
+        gx = c * dx0;
+        gy = c * dx1;
+    } else {
+        w = gx = gy = 0.0f;
+    }
+}
+
+// A partner position relative to the tile origin o: staged windows hold it
+// already in single precision; global-state positions are reduced here.
+__device__ __forceinline__ float2 rel_pos(float2 x, double2) { return x; }
+__device__ __forceinline__ float2 rel_pos(double2 x, double2 o) {
+    return make_float2((float)(x.x - o.x), (float)(x.y - o.y));
+}
+
 }  // namespace sps

@@ -1081,8 +1081,9 @@ __device__ __forceinline__ bool in_list_a(int kr, int ks) {
 
 // Counting of k_count for the node (KR = 0) or stress-point (KR = 1)
 // receivers [base, base + count), tiled as the stage kernels are, and the
-// tile's pair lists.
-template <int KR>
+// tile's pair lists; with FILL, also the lists' kernel values (the
+// mixed-precision stage kernels compute them as they go).
+template <int KR, bool FILL>
 __global__ __launch_bounds__(TILE) void k_count_lists(SearchView V, const double2 *__restrict__ x,
                                                       const double *__restrict__ hsml,
                                                       unsigned *__restrict__ cnt, Split *__restrict__ split,
@@ -1193,7 +1194,7 @@ __global__ __launch_bounds__(TILE) void k_count_lists(SearchView V, const double
     const bool over = la.n > LMAX || lb.n > LMAX;
     const bool any_over = lds ? __syncthreads_or(over) : true;
     if (threadIdx.x == 0) PL.ok[ltile] = !any_over;
-    if (!any_over && active) {
+    if (FILL && !any_over && active) {
         const double2 xr = x[r];
         const double hr = hsml[r];
         KernelFactor kf;
@@ -1378,6 +1379,57 @@ __global__ __launch_bounds__(TILE) void k_spu_nodes(StateBufs S, SearchView V, P
     S.stress[r] = s;
 }
 
+// The end of get_derivatives for stress point r (CSPM correction, div1,
+// grad_u; 2_...:596-644) from its velocity-gradient sums and CSPM moments,
+// then plastic_terms, the Jaumann spin, the stress right-hand side and the RK
+// accumulation of the stage (2_...:730-777).  st is its updated stress.
+__device__ __forceinline__ void sp_stage_tail(const DevParams &P, const StateBufs &S, const Work &W, int r,
+                                              const double4 &st, double g00, double g01, double g10, double g11,
+                                              double A0, double A1, double A2, double A3, double time_sph,
+                                              double f2) {
+    if (P.cspm) {
+        double A4 = A0 * A3 - A1 * A2;
+        if (fabs(A4) < P.cspm_tol) {
+            A4 = 1; A0 = 1; A1 = 0; A2 = 0; A3 = 1;
+        } else {
+            A4 = 1 / A4;
+        }
+        g00 = A4 * (A0 * g00 + A1 * g01);
+        g01 = A4 * (A2 * g00 + A3 * g01);
+        g10 = A4 * (A0 * g10 + A1 * g11);
+        g11 = A4 * (A2 * g10 + A3 * g11);
+    }
+    double4 d1;
+    d1.x = -(P.D11 * g00 + P.D12 * g11);
+    d1.y = -(P.D12 * g00 + P.D22 * g11);
+    d1.z = -(P.D33 * g10 + P.D33 * g01);
+    d1.w = -(P.D41 * g00 + P.D42 * g11);
+    double4 gu = make_double4(g00, g10, g01, g11);
+    W.grad_u[r] = gu;
+    double4 Gs;
+    double der1;
+    double fdr = S.fdr[r];
+    plastic_terms(P, st, gu, fdr, time_sph, Gs, der1);
+    S.fdr[r] = fdr;
+    double4 sp = make_double4(0, 0, 0, 0);
+    if (P.update_x) {
+        double om1 = 0.5 * (g01 - g10);
+        double om2 = -(0.5 * (g01 - g10));
+        sp.x = 2 * om1 * st.z;
+        sp.y = 2 * om2 * st.z;
+        sp.z = om2 * st.x + om1 * st.y;
+    }
+    double4 rh;
+    rh.x = -d1.x + sp.x + Gs.x;
+    rh.y = -d1.y + sp.y + Gs.y;
+    rh.z = -d1.z + sp.z + Gs.z;
+    rh.w = -d1.w + sp.w + Gs.w;
+    W.rhs1[r] = rh;
+    double4 rk = W.rk_stress[r];
+    W.rk_stress[r] = make_double4(rk.x + f2 * rh.x, rk.y + f2 * rh.y, rk.z + f2 * rh.z, rk.w + f2 * rh.w);
+    W.rk_dev[r] = W.rk_dev[r] + der1 * f2;
+}
+
 // Stress-point receivers: stress_point_update (CSPM-normalised velocity from
 // the nodes, then adapt_stress2) and, when deriv is set, get_derivatives
 // (2_...:487-648) fused with plastic_terms (loop 2_...:730-734), the Jaumann
@@ -1535,47 +1587,7 @@ __global__ __launch_bounds__(TILE) void k_sp_update(StateBufs S, Work W, SearchV
             walk_stencil<MASK_NODE | MASK_DUMMY>(GlobalCells{V.kstart, C.g, C.L}, GlobalFields{S}, C.g, cx, cy,
                                                  xr, hr, KIND_SP, r, rev_g, C.any_fwd, V.pi, cls, grad);
     }
-    if (P.cspm) {
-        double A4 = A0 * A3 - A1 * A2;
-        if (fabs(A4) < P.cspm_tol) {
-            A4 = 1; A0 = 1; A1 = 0; A2 = 0; A3 = 1;
-        } else {
-            A4 = 1 / A4;
-        }
-        g00 = A4 * (A0 * g00 + A1 * g01);
-        g01 = A4 * (A2 * g00 + A3 * g01);
-        g10 = A4 * (A0 * g10 + A1 * g11);
-        g11 = A4 * (A2 * g10 + A3 * g11);
-    }
-    double4 d1;
-    d1.x = -(P.D11 * g00 + P.D12 * g11);
-    d1.y = -(P.D12 * g00 + P.D22 * g11);
-    d1.z = -(P.D33 * g10 + P.D33 * g01);
-    d1.w = -(P.D41 * g00 + P.D42 * g11);
-    double4 gu = make_double4(g00, g10, g01, g11);
-    W.grad_u[r] = gu;
-    double4 Gs;
-    double der1;
-    double fdr = S.fdr[r];
-    plastic_terms(P, st, gu, fdr, time_sph, Gs, der1);
-    S.fdr[r] = fdr;
-    double4 sp = make_double4(0, 0, 0, 0);
-    if (P.update_x) {
-        double om1 = 0.5 * (g01 - g10);
-        double om2 = -(0.5 * (g01 - g10));
-        sp.x = 2 * om1 * st.z;
-        sp.y = 2 * om2 * st.z;
-        sp.z = om2 * st.x + om1 * st.y;
-    }
-    double4 rh;
-    rh.x = -d1.x + sp.x + Gs.x;
-    rh.y = -d1.y + sp.y + Gs.y;
-    rh.z = -d1.z + sp.z + Gs.z;
-    rh.w = -d1.w + sp.w + Gs.w;
-    W.rhs1[r] = rh;
-    double4 rk = W.rk_stress[r];
-    W.rk_stress[r] = make_double4(rk.x + f2 * rh.x, rk.y + f2 * rh.y, rk.z + f2 * rh.z, rk.w + f2 * rh.w);
-    W.rk_dev[r] = W.rk_dev[r] + der1 * f2;
+    sp_stage_tail(P, S, W, r, st, g00, g01, g10, g11, A0, A1, A2, A3, time_sph, f2);
 }
 
 // boundary_forces for one node: the nonzero terms of its inner-layer dummies
@@ -1631,6 +1643,54 @@ __device__ __forceinline__ double2 wall_force(const StateBufs &S, const SearchVi
 // partners (list A), the viscosity over node partners (list B); each sum
 // keeps list order, so the window is staged in two phases (stress points
 // and dummies, then nodes) through the same LDS.
+// The end of get_derivatives for node r (CSPM correction, div2; 2_...:617-642)
+// from its stress-divergence sums and CSPM moments, then gravity_force,
+// boundary_forces (when a dummy is among its pairs), the artificial
+// viscosity sums ax, ay, the velocity right-hand side and the RK accumulation
+// of the stage (2_...:737-776).
+__device__ __forceinline__ void node_stage_tail(const StateBufs &S, const Work &W, const SearchView &V,
+                                                const WalkCtx &C, const DevParams &P, int r, int cx, int cy,
+                                                double2 xr, double2 vr, double hr, double q00, double q01,
+                                                double q10, double q11, double q20, double q21, double A0,
+                                                double A1, double A2, double A3, int ndum, float ax, float ay,
+                                                double gx_grav, double gy_grav, double f2,
+                                                int *__restrict__ err) {
+    if (P.cspm) {
+        double A4 = A0 * A3 - A1 * A2;
+        if (fabs(A4) < P.cspm_tol) {
+            A4 = 1; A0 = 1; A1 = 0; A2 = 0; A3 = 1;
+        } else {
+            A4 = 1 / A4;
+        }
+        q00 = A4 * (A0 * q00 + A1 * q01);
+        q01 = A4 * (A2 * q00 + A3 * q01);
+        q10 = A4 * (A0 * q10 + A1 * q11);
+        q11 = A4 * (A2 * q10 + A3 * q11);
+    }
+    const double d2x = -(q00 + q21);
+    const double d2y = -(q20 + q11);
+    // gravity_force: G = factg*cgrav - DampingTG*v
+    const double Gx = gx_grav - P.damping * vr.x;
+    const double Gy = gy_grav - P.damping * vr.y;
+    // boundary_forces
+    double2 fb = W.f_bound[r];
+    if (P.do_bforce) {
+        fb = ndum ? wall_force(S, V, C, cx, cy, xr, hr, P, err) : make_double2(0.0, 0.0);
+        W.f_bound[r] = fb;
+    }
+    double2 avf = make_double2(0.0, 0.0);
+    if (P.do_av) {
+        avf = make_double2(-(double)ax, -(double)ay);
+        W.art_visc[r] = avf;
+    }
+    double2 rh;
+    rh.x = -d2x + Gx + avf.x + fb.x + 0.0;
+    rh.y = -d2y + Gy + avf.y + fb.y + 0.0;
+    W.rhs2[r] = rh;
+    double2 rk = W.rk_vel[r];
+    W.rk_vel[r] = make_double2(rk.x + f2 * rh.x, rk.y + f2 * rh.y);
+}
+
 __global__ __launch_bounds__(TILE) void k_deriv_nodes(StateBufs S, Work W, SearchView V, PairLists PL,
                                                       DevParams P, double gx_grav, double gy_grav, double f2,
                                                       int *__restrict__ err, int nn) {
@@ -1789,40 +1849,8 @@ __global__ __launch_bounds__(TILE) void k_deriv_nodes(StateBufs S, Work W, Searc
         }
     }
     if (!active) return;
-    if (P.cspm) {
-        double A4 = A0 * A3 - A1 * A2;
-        if (fabs(A4) < P.cspm_tol) {
-            A4 = 1; A0 = 1; A1 = 0; A2 = 0; A3 = 1;
-        } else {
-            A4 = 1 / A4;
-        }
-        q00 = A4 * (A0 * q00 + A1 * q01);
-        q01 = A4 * (A2 * q00 + A3 * q01);
-        q10 = A4 * (A0 * q10 + A1 * q11);
-        q11 = A4 * (A2 * q10 + A3 * q11);
-    }
-    const double d2x = -(q00 + q21);
-    const double d2y = -(q20 + q11);
-    // gravity_force: G = factg*cgrav - DampingTG*v
-    const double Gx = gx_grav - P.damping * vr.x;
-    const double Gy = gy_grav - P.damping * vr.y;
-    // boundary_forces
-    double2 fb = W.f_bound[r];
-    if (P.do_bforce) {
-        fb = ndum ? wall_force(S, V, C, cx, cy, xr, hr, P, err) : make_double2(0.0, 0.0);
-        W.f_bound[r] = fb;
-    }
-    double2 avf = make_double2(0.0, 0.0);
-    if (P.do_av) {
-        avf = make_double2(-(double)ax, -(double)ay);
-        W.art_visc[r] = avf;
-    }
-    double2 rh;
-    rh.x = -d2x + Gx + avf.x + fb.x + 0.0;
-    rh.y = -d2y + Gy + avf.y + fb.y + 0.0;
-    W.rhs2[r] = rh;
-    double2 rk = W.rk_vel[r];
-    W.rk_vel[r] = make_double2(rk.x + f2 * rh.x, rk.y + f2 * rh.y);
+    node_stage_tail(S, W, V, C, P, r, cx, cy, xr, vr, hr, q00, q01, q10, q11, q20, q21, A0, A1, A2, A3, ndum,
+                    ax, ay, gx_grav, gy_grav, f2, err);
 }
 
 // The type-6 loop of get_derivatives overwrites each dummy's stress with

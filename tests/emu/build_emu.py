@@ -5,7 +5,8 @@ kernels) with g++ against the host stand-in in tests/emu/hip/, after three
 textual rewrites of a copy of the sources:
 
 * each launch `kern<<<grid, block, lds, stream>>>(args)` becomes
-  `emu::launch(grid, block, lds, [&] { kern(args); }, sync)`;
+  `emu::launch_k(grid, block, lds, sync, kern, args)`, which evaluates the
+  arguments once, as a launch does;
 * the dynamic LDS array becomes the launch's buffer of exactly the requested
   size, so an access past it is reported;
 * a kernel's local layout reference used as a template argument is replaced
@@ -71,7 +72,7 @@ def rewrite_launches(s):
         else:
             sync = "true" if name.split("<")[0].startswith(SYNC) else "false"
         out.append(s[i:start])
-        out.append(f"emu::launch({cfg[0]}, {cfg[1]}, {cfg[2]}, [&] {{ {name}({args}); }}, {sync})")
+        out.append(f"emu::launch_k({cfg[0]}, {cfg[1]}, {cfg[2]}, {sync}, {name}, {args})")
         i = q + 1
 
 
@@ -81,19 +82,20 @@ def generate(gen):
     api = api.replace('#include "../../include/sps.h"', '#include "sps.h"')
     api = '#include "hip/hip_runtime.h"\n#define sps_lds (::emu::lds_base)\n' + rewrite_launches(api)
     open(os.path.join(gen, "sps_api_emu.cpp"), "w").write(api)
-    ker = open(os.path.join(SRC, "sps_kernels.hip")).read()
     decl = "extern __shared__ __align__(16) unsigned char sps_lds[];"
-    if decl not in ker:
-        raise RuntimeError("dynamic LDS declaration not found")
-    lines, cur = [], None
-    for ln in ker.replace(decl, "").split("\n"):
-        m = re.search(r"constexpr const LdsLayout &LY = (LAY_\w+);", ln)
-        if m:
-            cur = m.group(1)
-        elif cur:
-            ln = ln.replace("<LY>", "<" + cur + ">")
-        lines.append(ln)
-    open(os.path.join(gen, "sps_kernels.hip"), "w").write("\n".join(lines))
+    for name in ("sps_kernels.hip", "sps_mixed.hip"):
+        ker = open(os.path.join(SRC, name)).read()
+        if name == "sps_kernels.hip" and decl not in ker:
+            raise RuntimeError("dynamic LDS declaration not found")
+        lines, cur = [], None
+        for ln in ker.replace(decl, "").split("\n"):
+            m = re.search(r"constexpr const LdsLayout &LY = (LAY_\w+);", ln)
+            if m:
+                cur = m.group(1)
+            elif cur:
+                ln = ln.replace("<LY>", "<" + cur + ">")
+            lines.append(ln)
+        open(os.path.join(gen, name), "w").write("\n".join(lines))
     open(os.path.join(gen, "sps_device.h"), "w").write(open(os.path.join(SRC, "sps_device.h")).read())
 
 

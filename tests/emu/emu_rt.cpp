@@ -17,6 +17,7 @@ dim3e threadIdx, blockIdx, blockDim, gridDim;
 namespace emu {
 
 bool needs_sync(const char *name) {
+    while (*name == '(') ++name;   // a kernel name in parentheses (template arguments with commas)
     static const char *const sync[] = {"k_bounds_partial", "k_bounds_final", "k_count", "k_spu_nodes",
                                        "k_sp_update",      "k_deriv_nodes",  "k_scan_local", "k_scan_sums"};
     for (const char *s : sync)

@@ -32,11 +32,15 @@ struct double2 { double x, y; };
 struct double3 { double x, y, z; };
 struct double4 { double x, y, z, w; };
 struct int2 { int x, y; };
+struct float2 { float x, y; };
+struct float3 { float x, y, z; };
 struct float4 { float x, y, z, w; };
 inline double2 make_double2(double a, double b) { return {a, b}; }
 inline double3 make_double3(double a, double b, double c) { return {a, b, c}; }
 inline double4 make_double4(double a, double b, double c, double d) { return {a, b, c, d}; }
 inline int2 make_int2(int a, int b) { return {a, b}; }
+inline float2 make_float2(float a, float b) { return {a, b}; }
+inline float3 make_float3(float a, float b, float c) { return {a, b, c}; }
 inline float4 make_float4(float a, float b, float c, float d) { return {a, b, c, d}; }
 
 namespace emu {
@@ -48,6 +52,12 @@ void wave_get(int lane, void *v, int bytes);
 // run body for every thread of grid x block; sync: the kernel uses barriers
 // or wave exchanges (its threads then run as fibers)
 void launch(unsigned grid, unsigned block, size_t lds, const std::function<void()> &body, bool sync);
+// A launch as `kern<<<grid, block, lds>>>(args...)`: the arguments are
+// evaluated once, on the host, and every thread gets the same copies.
+template <typename... P, typename... A>
+void launch_k(unsigned grid, unsigned block, size_t lds, bool sync, void (*kern)(P...), A... args) {
+    launch(grid, block, lds, [&] { kern(args...); }, sync);
+}
 extern unsigned char *lds_base;   // the launch's dynamic LDS
 extern size_t lds_bytes;
 }  // namespace emu
@@ -73,6 +83,11 @@ template <typename T> T __shfl_down(T v, unsigned d, int w = 64) {
     return r;
 }
 template <typename T> T atomicAdd(T *p, T v) { T o = *p; *p = o + v; return o; }
+// hardware reciprocal and square root (within 1 ulp on the device)
+inline float __builtin_amdgcn_rcpf(float x) { return 1.0f / x; }
+inline float __builtin_amdgcn_sqrtf(float x) { return sqrtf(x); }
+#define __HIP_MEMORY_SCOPE_AGENT 0
+template <typename T> T __hip_atomic_load(const T *p, int, int) { return *p; }
 inline int atomicOr(int *p, int v) { int o = *p; *p = o | v; return o; }
 inline int atomicMin(int *p, int v) { int o = *p; *p = std::min(o, v); return o; }
 
@@ -92,6 +107,13 @@ inline hipError_t hipMemsetAsync(void *d, int v, size_t n, hipStream_t) { memset
 inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
 inline hipError_t hipGetLastError() { return hipSuccess; }
 inline hipError_t hipSetDevice(int) { return hipSuccess; }
+enum hipDeviceAttribute_t { hipDeviceAttributeMultiprocessorCount };
+// a small device: a work-queue launch then has few workgroups, each taking many tiles
+inline hipError_t hipDeviceGetAttribute(int *v, hipDeviceAttribute_t, int) { *v = 2; return hipSuccess; }
+inline hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int *nb, const void *, int, size_t) {
+    *nb = 4;
+    return hipSuccess;
+}
 inline hipError_t hipGetDevice(int *d) { *d = 0; return hipSuccess; }
 inline hipError_t hipGetDeviceCount(int *n) { *n = 1; return hipSuccess; }
 inline hipError_t hipStreamCreate(hipStream_t *s) { *s = nullptr; return hipSuccess; }

@@ -1,6 +1,8 @@
 """Run one deck on the CPU build of the HIP kernels and on the oracle and
 compare state and pair list bit for bit (run under AddressSanitizer by
-tests/test_emu.py).  Prints one line; exit status 0 when equal."""
+tests/test_emu.py); for a case "deck:mixed", run the mixed-precision form
+and compare its state with the oracle's within a relative tolerance.  Prints
+one line; exit status 0 when equal (within the tolerance)."""
 import importlib
 import os
 import sys
@@ -13,7 +15,7 @@ sys.path.insert(0, ROOT)
 sps = importlib.import_module("stress-particle-sph_amd")
 
 
-def main(emu_lib, oracle_lib, case, steps):
+def main(emu_lib, oracle_lib, case, steps, tol="1e-4"):
     deck_name, _, variant = case.partition(":")
     deck = sps.read_deck(os.path.join(ROOT, "stress-particle-sph_amd", "decks", deck_name))
     if variant == "hvar":       # smoothing lengths that differ between particles
@@ -21,6 +23,18 @@ def main(emu_lib, oracle_lib, case, steps):
         from conftest import open_varied_h
         emu = open_varied_h(sps, deck, emu_lib)
         ora = open_varied_h(sps, deck, oracle_lib)
+    elif variant == "mixed":    # the mixed-precision form against the double-precision oracle
+        emu = sps.Simulation.from_deck(deck, lib_path=emu_lib, mode=sps.MODE_FAST)
+        ora = sps.Simulation.from_deck(deck, lib_path=oracle_lib)
+        x0 = ora.get_state(["x"])["x"]
+        emu.step(steps)
+        ora.step(steps)
+        acc = importlib.import_module("stress-particle-sph_amd.accuracy")
+        err = acc.compare_states(emu.get_state(), ora.get_state(), x0)
+        ok = err["stress_rel_max"] <= float(tol) and err["disp_rel_max"] <= float(tol)
+        print(f"{case} {steps} steps: " + " ".join(f"{k} {v:.3e}" for k, v in err.items()) +
+              (" within " if ok else " BEYOND ") + str(tol))
+        return 0 if ok else 1
     else:
         emu = sps.Simulation.from_deck(deck, lib_path=emu_lib)
         ora = sps.Simulation.from_deck(deck, lib_path=oracle_lib)
@@ -35,4 +49,4 @@ def main(emu_lib, oracle_lib, case, steps):
 
 
 if __name__ == "__main__":
-    sys.exit(main(sys.argv[1], sys.argv[2], sys.argv[3], int(sys.argv[4])))
+    sys.exit(main(sys.argv[1], sys.argv[2], sys.argv[3], int(sys.argv[4]), *sys.argv[5:]))

@@ -46,3 +46,17 @@ def test_cpu_build_of_kernels_matches_oracle(emu_lib, deck_name, steps):
                         deck_name, str(steps)], env=env, capture_output=True, text=True, timeout=900)
     print(p.stdout[-2000:], p.stderr[-4000:])
     assert p.returncode == 0, p.stderr[-4000:] or p.stdout
+
+
+# the mixed-precision kernels: in bounds (AddressSanitizer) and within single-
+# precision rounding of the oracle where the problem does not amplify it (an
+# elastic deck; a yielding deck before yield starts)
+@pytest.mark.parametrize("case,steps,tol", [("t3_elastic:mixed", 60, "1e-5"), ("t2_cspm:mixed", 1, "1e-5"),
+                                            ("c1_collapse:mixed", 1, "5e-5")])
+def test_cpu_build_of_mixed_kernels_near_oracle(emu_lib, case, steps, tol):
+    env = dict(os.environ, LD_PRELOAD=_asan_runtime(),
+               ASAN_OPTIONS="detect_leaks=0:detect_stack_use_after_return=0:halt_on_error=1")
+    p = subprocess.run([sys.executable, os.path.join(ROOT, "tests", "emu", "run_case.py"), emu_lib, ORACLE_LIB,
+                        case, str(steps), tol], env=env, capture_output=True, text=True, timeout=900)
+    print(p.stdout[-2000:], p.stderr[-4000:])
+    assert p.returncode == 0, p.stderr[-4000:] or p.stdout
