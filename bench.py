@@ -91,9 +91,11 @@ def measured_traffic(deck_name, kernel):
     return None if k is None else k["bytes"]
 
 
-def run_gpu(args, sps, deck, mode=0):
+def run_gpu(args, sps, deck, mode=0, skin=0.0):
     import torch
     sim = sps.Simulation.from_deck(deck, mode=mode)
+    if skin:
+        sim.set_list_skin(skin)
     counts = (sim.parts.nnode, sim.parts.nstress, sim.parts.ndummy)
     print(f"[bench] deck {args.deck}: {counts[0]} nodes, {counts[1]} stress points, {counts[2]} dummies",
           file=sys.stderr, flush=True)
@@ -126,7 +128,7 @@ def form_errors(args, sps, deck):
     setup = importlib.import_module("stress-particle-sph_amd.setup")
     parts = setup.build_particles(deck)
 
-    def run(mode, perturb=0.0):
+    def run(mode, perturb=0.0, skin=0.0):
         st = sps.initial_state(parts)
         if perturb:
             n = parts.nnode + parts.nstress
@@ -135,6 +137,8 @@ def form_errors(args, sps, deck):
         sim = sps.Simulation(sps.make_params(deck, parts, mode=mode), sps.PRODUCT_LIB)
         sim.set_state(st)
         sim.deck, sim.parts = deck, parts
+        if skin:
+            sim.set_list_skin(skin)
         done = 0
         while done < args.acc_steps:
             k = min(500, args.acc_steps - done)
@@ -149,9 +153,17 @@ def form_errors(args, sps, deck):
     x0 = sps.initial_state(parts)["x"]
     ref = run(sps.MODE_ORDERED)
     mixed = acc.compare_states(run(sps.MODE_FAST), ref, x0)
+    skinned = acc.compare_states(run(sps.MODE_FAST, skin=args.skin * deck_dx(sps, deck)), ref, x0)
     spread = acc.compare_states(run(sps.MODE_ORDERED, 1e-7), ref, x0)
     rnd = lambda d: {k: float(f"{v:.4g}") for k, v in d.items()}
-    return {"steps": args.acc_steps, "mixed_vs_fp64": rnd(mixed), "fp64_perturbed_1e-7_vs_fp64": rnd(spread)}
+    return {"steps": args.acc_steps, "mixed_vs_fp64": rnd(mixed), "mixed_skin_vs_fp64": rnd(skinned),
+            "fp64_perturbed_1e-7_vs_fp64": rnd(spread)}
+
+
+def deck_dx(sps, deck):
+    import importlib
+    setup = importlib.import_module("stress-particle-sph_amd.setup")
+    return sps.make_params(deck, setup.build_particles(deck)).dx
 
 
 def run_cpu(args, sps, deck, oracle_path):
@@ -175,6 +187,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps timed for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision form")
+    ap.add_argument("--skin", type=float, default=0.25,
+                    help="list skin of the mixed form's list-reuse leg, in lattice spacings")
     ap.add_argument("--acc-steps", type=int, default=2000,
                     help="steps of the mixed-vs-double accuracy runs (0: skip them)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -228,6 +242,18 @@ def main():
             "kernels": {k: round(ms / args.steps, 4) for k, (n, ms) in sorted(mprof.items(), key=lambda kv: -kv[1][1])}}
         print(f"[bench] mixed form: {forms['mixed']['ms_per_step']} ms/step "
               f"({forms['mixed']['speedup_vs_fp64']}x)", file=sys.stderr, flush=True)
+        skin = args.skin * sim.params.dx
+        ssim, _, spairs, selapsed, sprof, _ = run_gpu(args, sps, deck, mode=sps.MODE_FAST, skin=skin)
+        searches = ssim.search_count()
+        ssim.close()
+        forms["mixed_skin"] = {
+            "ms_per_step": round(1e3 * selapsed / args.steps, 4), "pairs_per_s": round(spairs / selapsed, 1),
+            "speedup_vs_fp64": round(elapsed / selapsed, 3), "skin": skin,
+            "searches": f"{searches} in {args.warmup + args.steps} steps (pairs/s counts each step's pairs as "
+                        "those of the last search)",
+            "kernels": {k: round(ms / args.steps, 4) for k, (n, ms) in sorted(sprof.items(), key=lambda kv: -kv[1][1])}}
+        print(f"[bench] mixed form with list reuse: {forms['mixed_skin']['ms_per_step']} ms/step "
+              f"({forms['mixed_skin']['speedup_vs_fp64']}x, {searches} searches)", file=sys.stderr, flush=True)
         if args.acc_steps > 0:
             forms["mixed"]["accuracy"] = form_errors(args, sps, deck)
     cpu = None

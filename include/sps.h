@@ -177,6 +177,16 @@ int sps_get_pairs(sps_handle *h, int64_t cap, int32_t *pair_i, int32_t *pair_j,
 
 int sps_get_stats(sps_handle *h, sps_stats *out);
 
+/* Pair lists kept over steps.  No counterpart in the reference, which
+ * searches every step; SPS_MODE_FAST only.  With skin > 0 the search also
+ * lists the partners within 2h + skin (cells 2 max h + skin wide) and later
+ * steps reuse those lists while no moving particle has moved skin / 2 since
+ * the search (pairs beyond 2h contribute zero); the out-of-domain check runs
+ * with each search.  0 (the default) searches every step.
+ * sps_get_search_count returns the searches made since sps_create. */
+int sps_set_list_skin(sps_handle *h, double skin);
+int sps_get_search_count(sps_handle *h, int64_t *n);
+
 /* Kernel timing.  No counterpart in the reference (which has no timers);
  * for performance reports.  While enabled, every kernel launch of sps_step is
  * bracketed by HIP events on the handle's stream and the elapsed times are

@@ -997,6 +997,22 @@ int sps_get_stats(sps_handle *h, sps_stats *out)
     return 0;
 }
 
+/* The oracle searches every step whatever the skin (it runs every mode as
+ * SPS_MODE_ORDERED); it accepts a skin so that the same driver runs on it. */
+int sps_set_list_skin(sps_handle *h, double skin)
+{
+    if (!h) return fail(SPS_EINVAL, "null argument");
+    if (!(skin >= 0.0)) return fail(SPS_EINVAL, "a list skin must be >= 0");
+    return 0;
+}
+
+int sps_get_search_count(sps_handle *h, int64_t *n)
+{
+    if (!h || !n) return fail(SPS_EINVAL, "null argument");
+    *n = h->steps;
+    return 0;
+}
+
 int sps_set_profiling(sps_handle *h, int32_t enable)
 {
     if (!h) return fail(SPS_EINVAL, "null argument");

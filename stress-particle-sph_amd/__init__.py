@@ -100,12 +100,15 @@ def load_library(path):
     lib.sps_set_profiling.argtypes = [h, ctypes.c_int32]
     lib.sps_get_profile.argtypes = [h, ctypes.c_int32, ctypes.POINTER(KernelTime),
                                     ctypes.POINTER(ctypes.c_int32), ctypes.c_int32]
+    lib.sps_set_list_skin.argtypes = [h, ctypes.c_double]
+    lib.sps_get_search_count.argtypes = [h, ctypes.POINTER(ctypes.c_int64)]
     lib.sps_destroy.argtypes = [h]
     lib.sps_destroy.restype = None
     lib.sps_last_error.restype = ctypes.c_char_p
     lib.sps_backend.restype = ctypes.c_char_p
     for f in ("sps_create", "sps_set_state", "sps_step", "sps_get_state", "sps_get_pairs",
-              "sps_get_stats", "sps_set_profiling", "sps_get_profile"):
+              "sps_get_stats", "sps_set_profiling", "sps_get_profile", "sps_set_list_skin",
+              "sps_get_search_count"):
         getattr(lib, f).restype = ctypes.c_int
     return lib
 
@@ -260,6 +263,16 @@ class Simulation:
         self._check(self.lib.sps_get_stats(self.handle, ctypes.byref(s)))
         return dict(n_pairs=s.n_pairs, m_pairs=s.m_pairs, n_by_type=list(s.n_by_type),
                     ndivx=s.ndivx, ndivy=s.ndivy, steps=s.steps)
+
+    def set_list_skin(self, skin):
+        """Keep the pair lists over steps, rebuilt when a particle has moved
+        skin / 2 (SPS_MODE_FAST only; 0 searches every step)."""
+        self._check(self.lib.sps_set_list_skin(self.handle, float(skin)))
+
+    def search_count(self):
+        n = ctypes.c_int64(0)
+        self._check(self.lib.sps_get_search_count(self.handle, ctypes.byref(n)))
+        return int(n.value)
 
     def set_profiling(self, on=True):
         self._check(self.lib.sps_set_profiling(self.handle, int(bool(on))))

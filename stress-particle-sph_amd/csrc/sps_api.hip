@@ -141,6 +141,14 @@ struct sps_handle {
     int ntile_n = 0, ntile_s = 0;
     double h_uniform = 0.0;   // the smoothing length of every particle, or 0 when they differ
     bool mixed = false;       // SPS_MODE_FAST: the mixed-precision stage kernels
+    // pair lists kept over steps (mixed form): built with this skin, reused
+    // while no moving particle has gone skin / 2 from where it was at the build
+    double skin = 0.0;
+    bool lists_valid = false;
+    long long searches = 0;
+    DBuf<double2> x_build;
+    DBuf<unsigned long long> dmax;
+    hipEvent_t ev_disp = nullptr;
     DBuf<int> err;
     // host-visible summary of the last search
     unsigned long long n_pairs = 0, m_pairs = 0, m_prev = 0;
@@ -353,9 +361,12 @@ extern "C" int sps_create(const sps_params *params, sps_handle **out) {
     }
     rc |= dalloc(h->deg_a, nt); rc |= dalloc(h->deg_b, nn);
     rc |= dalloc(h->tile_ok, h->ntile_n + h->ntile_s + 1);
+    rc |= dalloc(h->x_build, nt); rc |= dalloc(h->dmax, 1);
     if (rc) return bail(rc);
     if (hipHostMalloc((void **)&h->pinned, 64 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
         return bail(fail(SPS_ENOMEM, "pinned staging allocation"));
+    if (hipEventCreateWithFlags(&h->ev_disp, hipEventDisableTiming) != hipSuccess)
+        return bail(fail(SPS_EDEVICE, "event creation failed"));
     if (hipMemsetAsync(h->f_bound.p, 0, nn * sizeof(double2), h->stream) != hipSuccess ||
         hipMemsetAsync(h->art_visc.p, 0, nn * sizeof(double2), h->stream) != hipSuccess ||
         hipMemsetAsync(h->grad_u.p, 0, nt * sizeof(double4), h->stream) != hipSuccess ||
@@ -443,6 +454,7 @@ extern "C" int sps_set_state(sps_handle *h, const sps_state *st) {
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(s));
     h->have_state = true;
+    h->lists_valid = false;
     h->poisoned = false;
     return 0;
 }
@@ -458,7 +470,7 @@ static int search(sps_handle *h, bool fresh_list) {
     LAUNCH(K_CHECK_OOD, nblk(nt2), BLOCK, k_check_ood, A.x.p, A.ood.p, nt2, h->p.xmin_domain[0], h->p.xmin_domain[1],
                                            h->p.xmax_domain[0], h->p.xmax_domain[1]);
     LAUNCH(K_BOUNDS, h->nbparts, BLOCK, k_bounds_partial, A.x.p, A.hsml.p, A.ood.p, nt2, h->bparts.p);
-    LAUNCH(K_BOUNDS, 1, BLOCK, k_bounds_final, h->bparts.p, h->nbparts, h->grid.p, h->cell_cap);
+    LAUNCH(K_BOUNDS, 1, BLOCK, k_bounds_final, h->bparts.p, h->nbparts, h->grid.p, h->cell_cap, h->skin);
     // the cell count is needed on the host for the bucket arrays' extent
     HIP_TRY(hipMemcpyAsync(h->pinned, h->grid.p, sizeof(Grid), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
@@ -502,16 +514,16 @@ static int search(sps_handle *h, bool fresh_list) {
         // the mixed-precision stage kernels compute the kernel values themselves
         if (nn > 0 && !h->mixed)
             LAUNCH_LDS(K_COUNT_PAIRS, nblk8(nn), TILE, LAY_COUNT.bytes, (k_count_lists<KIND_NODE, true>), V, C.x.p,
-                       C.hsml.p, h->cnt.p, h->split.p, PL, h->h_uniform, 0, nn);
+                       C.hsml.p, h->cnt.p, h->split.p, PL, h->h_uniform, 0, nn, h->skin);
         if (nn > 0 && h->mixed)
             LAUNCH_LDS(K_COUNT_PAIRS, nblk8(nn), TILE, LAY_COUNT.bytes, (k_count_lists<KIND_NODE, false>), V, C.x.p,
-                       C.hsml.p, h->cnt.p, h->split.p, PL, h->h_uniform, 0, nn);
+                       C.hsml.p, h->cnt.p, h->split.p, PL, h->h_uniform, 0, nn, h->skin);
         if (nt > nn && !h->mixed)
             LAUNCH_LDS(K_COUNT_PAIRS, nblk8(nt - nn), TILE, LAY_COUNT.bytes, (k_count_lists<KIND_SP, true>), V, C.x.p,
-                       C.hsml.p, h->cnt.p, h->split.p, PL, h->h_uniform, nn, nt - nn);
+                       C.hsml.p, h->cnt.p, h->split.p, PL, h->h_uniform, nn, nt - nn, h->skin);
         if (nt > nn && h->mixed)
             LAUNCH_LDS(K_COUNT_PAIRS, nblk8(nt - nn), TILE, LAY_COUNT.bytes, (k_count_lists<KIND_SP, false>), V, C.x.p,
-                       C.hsml.p, h->cnt.p, h->split.p, PL, h->h_uniform, nn, nt - nn);
+                       C.hsml.p, h->cnt.p, h->split.p, PL, h->h_uniform, nn, nt - nn, h->skin);
         if (nt2 > nt)
             LAUNCH(K_COUNT_PAIRS, nblk8(nt2 - nt), BLOCK, k_count, C.x.p, C.hsml.p, h->cell.p, h->kstart.p, h->proc.p,
                    h->grid.p, h->cnt.p, h->split.p, nn, nt, nt, nt2);
@@ -535,11 +547,35 @@ static int search(sps_handle *h, bool fresh_list) {
     return 0;
 }
 
+// Whether the mixed form's lists of the last search still hold every pair:
+// no moving particle has gone skin / 2 since (read from the check that the
+// previous step queued).
+static int lists_still_valid(sps_handle *h, bool &valid) {
+    valid = false;
+    if (!h->mixed || h->skin <= 0 || !h->lists_valid) return 0;
+    HIP_TRY(hipEventSynchronize(h->ev_disp));
+    double d2;
+    std::memcpy(&d2, h->pinned + 32, sizeof d2);
+    valid = 2.0 * std::sqrt(d2) < h->skin;
+    return 0;
+}
+
 static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
-    int rc = search(h, itimestep == 1);
+    bool reuse;
+    int rc = lists_still_valid(h, reuse);
     if (rc) return rc;
     hipStream_t s = h->stream;
     const int nn = h->nn, nt = h->nt, nt2 = h->nt2;
+    if (!reuse) {
+        rc = search(h, itimestep == 1);
+        if (rc) return rc;
+        ++h->searches;
+        if (h->mixed && h->skin > 0) {
+            HIP_TRY(hipMemcpyAsync(h->x_build.p, h->S[h->cur].x.p, (size_t)nt * sizeof(double2),
+                                   hipMemcpyDeviceToDevice, s));
+            h->lists_valid = true;
+        }
+    }
     StateBufs SB = h->S[h->cur].view();
     Work W{h->vx0.p, h->x0.p, h->vel0.p, h->rk_vel.p, h->rhs2.p, h->f_bound.p, h->art_visc.p,
            h->stress0.p, h->rk_stress.p, h->rhs1.p, h->grad_u.p, h->rk_dev.p, h->ddev.p};
@@ -590,6 +626,12 @@ static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
             LAUNCH_LDS(K_SP_UPDATE, nblk8(nt - nn), TILE, LAY_SP_UPDATE_MIX.bytes, k_sp_update_mix, SB, W, V, PL, h->dp,
                        time_sph, 0.0, 0, adapt, hu, nn, nt);
         LAUNCH(K_POSITION, nblk(nt), BLOCK, k_position, SB, W, dt, h->p.update_x, nn, nt);
+        if (h->skin > 0) {
+            HIP_TRY(hipMemsetAsync(h->dmax.p, 0, sizeof(unsigned long long), s));
+            LAUNCH(K_POSITION, std::min(1024u, nblk(nt)), BLOCK, k_max_disp, SB.x, h->x_build.p, nt, h->dmax.p);
+            HIP_TRY(hipMemcpyAsync(h->pinned + 32, h->dmax.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipEventRecord(h->ev_disp, s));
+        }
         HIP_TRY(hipGetLastError());
         return 0;
     }
@@ -747,6 +789,21 @@ extern "C" int sps_get_stats(sps_handle *h, sps_stats *out) {
     return 0;
 }
 
+extern "C" int sps_set_list_skin(sps_handle *h, double skin) {
+    if (!h) return fail(SPS_EINVAL, "null argument");
+    if (!(skin >= 0.0) || (skin > 0.0 && !h->mixed))
+        return fail(SPS_EINVAL, "a list skin (%g) needs SPS_MODE_FAST and must be >= 0", skin);
+    h->skin = skin;
+    h->lists_valid = false;
+    return 0;
+}
+
+extern "C" int sps_get_search_count(sps_handle *h, int64_t *n) {
+    if (!h || !n) return fail(SPS_EINVAL, "null argument");
+    *n = h->searches;
+    return 0;
+}
+
 extern "C" int sps_set_profiling(sps_handle *h, int32_t enable) {
     if (!h) return fail(SPS_EINVAL, "null argument");
     h->prof = enable != 0;
@@ -783,6 +840,7 @@ extern "C" void sps_destroy(sps_handle *h) {
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     prof_collect(h);
     for (auto e : h->ev_pool) (void)hipEventDestroy(e);
+    if (h->ev_disp) (void)hipEventDestroy(h->ev_disp);
     h->S[0].release();
     h->S[1].release();
     dfree(h->vx0); dfree(h->x0); dfree(h->vel0); dfree(h->rk_vel); dfree(h->rhs2); dfree(h->f_bound);

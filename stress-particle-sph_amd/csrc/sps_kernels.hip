@@ -98,10 +98,11 @@ __global__ void k_bounds_partial(const double2 *__restrict__ x, const double *__
 }
 
 // Grid parameters with the reference's double-precision expressions
-// (one block: reduce the partials, then thread 0 derives the grid).
-// Launch with exactly BLOCK threads.
+// (one block: reduce the partials, then thread 0 derives the grid).  skin
+// widens the cells beyond 2 max h for pair lists that are kept over several
+// steps (0: the reference's grid).  Launch with exactly BLOCK threads.
 __global__ void __launch_bounds__(BLOCK) k_bounds_final(const double *__restrict__ part, int nparts,
-                                                        Grid *__restrict__ g, int cell_cap) {
+                                                        Grid *__restrict__ g, int cell_cap, double skin) {
     __shared__ double s[5][BLOCK];
     int t = threadIdx.x;
     double mn0 = (double)1.e+10f, mn1 = (double)1.e+10f, mx0 = -(double)1.e+10f,
@@ -128,6 +129,7 @@ __global__ void __launch_bounds__(BLOCK) k_bounds_final(const double *__restrict
     if (t != 0) return;
     double mn[2] = {s[0][0], s[1][0]}, mx[2] = {s[2][0], s[3][0]};
     double deltx = s[4][0] * 2;
+    if (skin > 0) deltx = deltx + skin;
     int nd[2];
     for (int d = 0; d < 2; ++d) {
         double length = mx[d] - mn[d];
@@ -931,6 +933,29 @@ __device__ __forceinline__ void copy_kind(const Tile &T, const SearchView &V, co
         if (t + k * TILE < total) put(t + k * TILE, it[k]);
 }
 
+// ------------------------------------------------------- list reuse check
+// Largest squared displacement of the moving particles [0, n) since the lists
+// were built, as the bits of a non-negative double (which order as the
+// doubles do): one atomic per block.
+__global__ void k_max_disp(const double2 *__restrict__ x, const double2 *__restrict__ xb, int n,
+                           unsigned long long *__restrict__ out) {
+    __shared__ double s[BLOCK];
+    double m = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double2 a = x[i], b = xb[i];
+        const double d0 = a.x - b.x, d1 = a.y - b.y;
+        m = fmax(m, d0 * d0 + d1 * d1);
+    }
+    const int t = threadIdx.x;
+    s[t] = m;
+    __syncthreads();
+    for (int w = BLOCK / 2; w > 0; w >>= 1) {
+        if (t < w) s[t] = fmax(s[t], s[t + w]);
+        __syncthreads();
+    }
+    if (t == 0) atomicMax(out, (unsigned long long)__double_as_longlong(s[0]));
+}
+
 // ------------------------------------------------------------ step kernels
 struct Work {
     double2 *vx0, *x0, *vel0, *rk_vel, *rhs2, *f_bound, *art_visc;
@@ -1087,7 +1112,8 @@ template <int KR, bool FILL>
 __global__ __launch_bounds__(TILE) void k_count_lists(SearchView V, const double2 *__restrict__ x,
                                                       const double *__restrict__ hsml,
                                                       unsigned *__restrict__ cnt, Split *__restrict__ split,
-                                                      PairLists PL, double h_uniform, int base, int count) {
+                                                      PairLists PL, double h_uniform, int base, int count,
+                                                      double skin) {
     __shared__ StagePlan pl[3];
     constexpr const LdsLayout &LY = LAY_COUNT;
     const WalkCtx C = walk_ctx(V);
@@ -1151,12 +1177,25 @@ __global__ __launch_bounds__(TILE) void k_count_lists(SearchView V, const double
                         for (int j = rg.x; j < rg.x + rg.y; ++j) {
                             if (ks == KR && j == self) continue;
                             const double2 xs = A.xk(ks, j);
-                            if constexpr (decltype(uni)::value) {
-                                if (!accepted_by(xr.x - xs.x, xr.y - xs.y, ub)) continue;
+                            if (!FILL && skin > 0) {
+                                // lists kept over several steps: also the partners within the
+                                // skin; only the pairs themselves are counted
+                                const double dx0 = xr.x - xs.x, dx1 = xr.y - xs.y;
+                                const double hs = decltype(uni)::value ? h_uniform : A.hk(ks, j);
+                                const double ls = (hr + hs) + skin;
+                                if (dx0 * dx0 + dx1 * dx1 >= ls * ls) continue;
+                                const bool acc = decltype(uni)::value ? accepted_by(dx0, dx1, ub)
+                                                                      : accepted_by(dx0, dx1, accept_bounds(hr, hs));
+                                if (acc) fk[ks] += (kind_after || (slot_after && j > self)) ? 1u : 0u;
                             } else {
-                                if (!accepted_by(xr.x - xs.x, xr.y - xs.y, accept_bounds(hr, A.hk(ks, j)))) continue;
+                                if constexpr (decltype(uni)::value) {
+                                    if (!accepted_by(xr.x - xs.x, xr.y - xs.y, ub)) continue;
+                                } else {
+                                    if (!accepted_by(xr.x - xs.x, xr.y - xs.y, accept_bounds(hr, A.hk(ks, j))))
+                                        continue;
+                                }
+                                fk[ks] += (kind_after || (slot_after && j > self)) ? 1u : 0u;
                             }
-                            fk[ks] += (kind_after || (slot_after && j > self)) ? 1u : 0u;
                             if (!lists) continue;
                             const unsigned e = (unsigned)((ks << 14) | j);
                             if (in_list_a(KR, ks))

@@ -19,7 +19,8 @@ namespace emu {
 bool needs_sync(const char *name) {
     while (*name == '(') ++name;   // a kernel name in parentheses (template arguments with commas)
     static const char *const sync[] = {"k_bounds_partial", "k_bounds_final", "k_count", "k_spu_nodes",
-                                       "k_sp_update",      "k_deriv_nodes",  "k_scan_local", "k_scan_sums"};
+                                       "k_sp_update",      "k_deriv_nodes",  "k_scan_local", "k_scan_sums",
+                                       "k_max_disp"};
     for (const char *s : sync)
         if (strncmp(name, s, strlen(s)) == 0) return true;   // k_count also covers k_count_lists
     return false;

@@ -90,6 +90,10 @@ inline float __builtin_amdgcn_sqrtf(float x) { return sqrtf(x); }
 template <typename T> T __hip_atomic_load(const T *p, int, int) { return *p; }
 inline int atomicOr(int *p, int v) { int o = *p; *p = o | v; return o; }
 inline int atomicMin(int *p, int v) { int o = *p; *p = std::min(o, v); return o; }
+inline unsigned long long atomicMax(unsigned long long *p, unsigned long long v) {
+    unsigned long long o = *p; *p = std::max(o, v); return o;
+}
+inline long long __double_as_longlong(double d) { long long r; memcpy(&r, &d, sizeof r); return r; }
 
 typedef int hipError_t;
 enum { hipSuccess = 0, hipErrorUnknown = 999 };
@@ -122,5 +126,8 @@ inline hipError_t hipStreamDestroy(hipStream_t) { return hipSuccess; }
 inline hipError_t hipEventCreate(hipEvent_t *e) { *e = (void *)1; return hipSuccess; }
 inline hipError_t hipEventDestroy(hipEvent_t) { return hipSuccess; }
 inline hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
+#define hipEventDisableTiming 2
+inline hipError_t hipEventCreateWithFlags(hipEvent_t *e, unsigned) { *e = (void *)1; return hipSuccess; }
+inline hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
 inline hipError_t hipEventElapsedTime(float *ms, hipEvent_t, hipEvent_t) { *ms = 0; return hipSuccess; }
 inline const char *hipGetErrorString(hipError_t) { return "emulated"; }

@@ -13,7 +13,9 @@ reference by rounding only; these tests bound that difference:
   form's error is bounded by ten times the double-precision oracle's spread
   under such a perturbation (the conditioning of the problem, not of the
   form);
-* the pair set and statistics are those of the double-precision form.
+* the pair set and statistics are those of the double-precision form;
+* with the pair lists kept over steps (a skin), fewer searches than steps and
+  the same bounds.
 """
 import importlib
 import sys
@@ -31,11 +33,13 @@ def say(msg):
     sys.stdout.flush()
 
 
-def run_pair(deck_name, steps):
+def run_pair(deck_name, steps, skin=0.0):
     sps = load_pkg()
     acc = importlib.import_module("stress-particle-sph_amd.accuracy")
     deck = sps.read_deck(deck_dir(deck_name))
     mix = sps.Simulation.from_deck(deck, mode=sps.MODE_FAST)
+    if skin:
+        mix.set_list_skin(skin * mix.params.dx)
     ora = sps.Simulation.from_deck(deck, lib_path=ORACLE_LIB)
     assert mix.backend == "hip-gfx950", mix.backend
     x0 = ora.get_state(["x"])["x"]
@@ -65,12 +69,30 @@ def test_mixed_pairs_and_stats_equal_double_form():
         assert sm[k] == so[k], f"stats {k}: {sm[k]} vs {so[k]}"
 
 
-@pytest.mark.parametrize("deck_name,steps", [("t2_cspm", 60), ("t1_tiny", 3), ("t1_tiny", 60)])
-def test_mixed_error_within_conditioning_of_yielding_deck(deck_name, steps):
+@pytest.mark.parametrize("deck_name,steps,tol", [("t3_elastic", 200, 1e-5), ("t2_cspm", 1, 1e-5)])
+def test_mixed_with_list_skin_close_to_oracle(deck_name, steps, tol):
+    err, mix, _ = run_pair(deck_name, steps, skin=0.25)
+    n = mix.search_count()
+    say(f"  {n} searches: {err}")
+    assert 1 <= n < max(steps, 2), f"{n} searches in {steps} steps"
+    for k in ("disp_rel_max", "vel_rel_max", "stress_rel_max"):
+        assert err[k] <= tol, f"{deck_name}: {k} {err[k]:.3e} > {tol}"
+
+
+def test_list_skin_needs_the_mixed_form():
+    sps = load_pkg()
+    sim = sps.Simulation.from_deck(sps.read_deck(deck_dir("t1_tiny")))
+    with pytest.raises(sps.SpsError):
+        sim.set_list_skin(0.001)
+
+
+@pytest.mark.parametrize("deck_name,steps,skin", [("t2_cspm", 60, 0.0), ("t1_tiny", 3, 0.0), ("t1_tiny", 60, 0.0),
+                                                   ("t2_cspm", 60, 0.25), ("t1_tiny", 60, 0.25)])
+def test_mixed_error_within_conditioning_of_yielding_deck(deck_name, steps, skin):
     sps = load_pkg()
     acc = importlib.import_module("stress-particle-sph_amd.accuracy")
     setup = importlib.import_module("stress-particle-sph_amd.setup")
-    err, _, ora = run_pair(deck_name, steps)
+    err, _, ora = run_pair(deck_name, steps, skin)
     ref = ora.get_state()
     deck = sps.read_deck(deck_dir(deck_name))
     parts = setup.build_particles(deck)
