@@ -187,6 +187,54 @@ int sps_get_stats(sps_handle *h, sps_stats *out);
 int sps_set_list_skin(sps_handle *h, double skin);
 int sps_get_search_count(sps_handle *h, int64_t *n);
 
+/* Learned Drucker-Prager stress update.  No counterpart in the reference.
+ * A multilayer perceptron 10 -> 128 -> 128 -> 128 -> 4 (SiLU; bfloat16
+ * weights and activations, single-precision accumulation on the matrix
+ * cores) maps features of a plastic trial state to the correction the exact
+ * return map (adapt_stress2, 3_...:2066-2140) makes:
+ *   x[0:4] = sigma_n / S, x[4:8] = (sigma_trial - sigma_n) / S,
+ *   x[8] = c / Sc, x[9] = tan(phi);
+ *   y[0:4] = (R(sigma_trial) - sigma_trial) / F(sigma_trial)
+ * with F = sqrt(J2) + 3 alpha p - k_c the yield function (positive at every
+ * plastic trial state)
+ * (stress components xx, yy, xy, zz; S, Sc the stress and cohesion scales).
+ * Training (forward, backward, Adam) and inference run on the GPU.
+ * sps_mlp_create: Glorot-uniform weights from `seed`; batch_rows sizes the
+ *   work buffers (grown as needed).
+ * sps_mlp_get_params / set_params: the sps_mlp_param_count() single-precision
+ *   parameters, blocks W1 [16][128], b1 [128], W2 [128][128], b2, W3, b3,
+ *   W4 [128][16], b4 [16] (inputs past 10 and outputs past 4 are padding).
+ * sps_mlp_forward: y (n x 4) for x (n x 10).
+ * sps_mlp_gradient: mean squared error over n <= batch_rows rows and its
+ *   gradient (all parameters), without an update.
+ * sps_mlp_train: `epochs` shuffled passes of Adam (lr; beta 0.9, 0.999) in
+ *   batches of `batch` rows (a multiple of 64); the mean loss of each epoch.
+ * sps_return_map: the exact return map of n stresses (n x 4) on the GPU, each
+ *   with its own tan(phi) and c (n each).
+ * sps_set_surrogate (SPS_MODE_FAST; NULL detaches): each step's final stress
+ *   update sends the stress points' plastic trial states through the network
+ *   and keeps a result where |F| <= tol * (sqrt(J2) + |3 alpha p| + k_c) of
+ *   the trial state (F the yield function); elsewhere the exact return map's
+ *   result stays.  sps_get_surrogate_stats: states evaluated and kept.
+ * sps_set_plastic_log / sps_get_plastic_log: the plain method's tuples
+ *   (sigma_n, sigma_trial, R(sigma_trial)), 12 doubles each, of every plastic
+ *   trial state of the final stress updates, up to cap (training data). */
+typedef struct sps_mlp sps_mlp;
+int64_t sps_mlp_param_count(void);
+int sps_mlp_create(int32_t batch_rows, uint64_t seed, sps_mlp **out);
+void sps_mlp_destroy(sps_mlp *ml);
+int sps_mlp_get_params(sps_mlp *ml, float *out);
+int sps_mlp_set_params(sps_mlp *ml, const float *in);
+int sps_mlp_forward(sps_mlp *ml, int64_t n, const float *x, float *y);
+int sps_mlp_gradient(sps_mlp *ml, int64_t n, const float *x, const float *t, float *grad, float *loss);
+int sps_mlp_train(sps_mlp *ml, int64_t n, const float *x, const float *t, int32_t epochs, int32_t batch,
+                  float lr, uint64_t seed, float *epoch_loss);
+int sps_return_map(int64_t n, const double *in, const double *tanfi, const double *coh, double *out);
+int sps_set_surrogate(sps_handle *h, sps_mlp *ml, double stress_scale, double coh_scale, double tol);
+int sps_get_surrogate_stats(sps_handle *h, int64_t *evaluated, int64_t *kept);
+int sps_set_plastic_log(sps_handle *h, int64_t cap);
+int sps_get_plastic_log(sps_handle *h, int64_t cap, double *out, int64_t *n);
+
 /* Kernel timing.  No counterpart in the reference (which has no timers);
  * for performance reports.  While enabled, every kernel launch of sps_step is
  * bracketed by HIP events on the handle's stream and the elapsed times are

@@ -73,6 +73,9 @@ struct sps_handle {
     double *vel0, *stress0, *rk_vel, *rk_stress, *rk_dev, *rhs1, *rhs2;
     double *divf1, *divf2, *source_stress, *g_grav, *spin;
     double *grad1, *grad2, *ae, *dummy_vel, *cspm_norm, *vel_tmp, *stress_tmp, *ivar_tmp;
+    /* plastic trial-state log (sps_set_plastic_log) */
+    int64_t log_cap, log_n;
+    double *plog;
     float *av_tmp;
 };
 
@@ -322,14 +325,12 @@ static void pint_update(struct sps_handle *h)
 }
 
 /* ---------------------------------------------------------- adapt_stress2 */
-/* code/3_SPH_material_2018.f90:2066-2140, all of 1..ntotal, all DP */
-static void adapt_stress2(struct sps_handle *h)
+/* code/3_SPH_material_2018.f90:2066-2140 for one stress, all DP */
+static void adapt_one(double *s, double tanfi, double coh)
 {
-    double tanfi = h->p.props[12], coh = h->p.props[13];
     double alpha2 = tanfi / (sqrt(9 + 12 * (tanfi * tanfi)));
     double kc = (3 * coh) / (sqrt(9 + 12 * (tanfi * tanfi)));
-    for (int i = 0; i < h->nt; ++i) {
-        double *s = h->stress + 4 * i;
+    {
         double smean = (s[0] + s[1] + s[3]) / 3.0;
         double dv[4] = {s[0] - smean, s[1] - smean, s[2], s[3] - smean};
         double varj2 = dv[2] * dv[2] + 0.5 * (dv[0] * dv[0] + dv[1] * dv[1] + dv[3] * dv[3]);
@@ -355,6 +356,12 @@ static void adapt_stress2(struct sps_handle *h)
             s[2] = rn * dv[2];
         }
     }
+}
+
+/* all of 1..ntotal */
+static void adapt_stress2(struct sps_handle *h)
+{
+    for (int i = 0; i < h->nt; ++i) adapt_one(h->stress + 4 * i, h->p.props[12], h->p.props[13]);
 }
 
 /* ---------------------------------------------------- stress_point_update */
@@ -822,6 +829,21 @@ static void rk4(struct sps_handle *h, double dt_sph, double time_sph)
     for (int i = nn; i < nt; ++i)
         for (int q = 0; q < 4; ++q)
             h->stress[4 * i + q] = h->stress0[4 * i + q] + (dt_sph / 6) * h->rk_stress[4 * i + q];
+    if (ncrit == 12 && h->log_cap > 0) {
+        /* the plain method's tuples of the plastic trial states (training data) */
+        for (int i = nn; i < nt; ++i) {
+            double ret[4];
+            const double *tr = h->stress + 4 * i;
+            memcpy(ret, tr, sizeof ret);
+            adapt_one(ret, h->p.props[12], h->p.props[13]);   /* a copy: adapt_stress2 below does it */
+            if (memcmp(tr, ret, sizeof ret) != 0 && h->log_n < h->log_cap) {
+                double *o = h->plog + 12 * h->log_n++;
+                memcpy(o, h->stress0 + 4 * i, 4 * sizeof(double));
+                memcpy(o + 4, tr, 4 * sizeof(double));
+                memcpy(o + 8, ret, sizeof ret);
+            }
+        }
+    }
     if (ncrit == 12) adapt_stress2(h);
     for (int i = 0; i < nt; ++i) h->ddev[i] = h->rk_dev[i] / 6;
 }
@@ -997,6 +1019,77 @@ int sps_get_stats(sps_handle *h, sps_stats *out)
     return 0;
 }
 
+int sps_set_plastic_log(sps_handle *h, int64_t cap)
+{
+    if (!h || cap < 0 || cap > ((int64_t)1 << 30)) return fail(SPS_EINVAL, "bad argument");
+    free(h->plog);
+    h->plog = cap ? (double *)malloc((size_t)cap * 12 * sizeof(double)) : NULL;
+    if (cap && !h->plog) return fail(SPS_ENOMEM, "log allocation");
+    h->log_cap = cap;
+    h->log_n = 0;
+    return 0;
+}
+
+int sps_get_plastic_log(sps_handle *h, int64_t cap, double *out, int64_t *n)
+{
+    if (!h || !n || cap < 0 || (cap && !out)) return fail(SPS_EINVAL, "bad argument");
+    int64_t k = h->log_n < cap ? h->log_n : cap;
+    if (k) memcpy(out, h->plog, (size_t)k * 12 * sizeof(double));
+    *n = k;
+    return 0;
+}
+
+int sps_return_map(int64_t n, const double *in, const double *tanfi, const double *coh, double *out)
+{
+    if (n < 0 || (n && (!in || !out || !tanfi || !coh))) return fail(SPS_EINVAL, "bad argument");
+    for (int64_t i = 0; i < n; ++i) {
+        memcpy(out + 4 * i, in + 4 * i, 4 * sizeof(double));
+        adapt_one(out + 4 * i, tanfi[i], coh[i]);
+    }
+    return 0;
+}
+
+/* The learned stress update runs on the GPU only; the oracle has no network
+ * (its CPU check is oracle/mlp_oracle.py). */
+int64_t sps_mlp_param_count(void) { return 37264; }
+int sps_mlp_create(int32_t batch_rows, uint64_t seed, sps_mlp **out)
+{
+    (void)batch_rows; (void)seed;
+    if (out) *out = NULL;
+    return fail(SPS_EINVAL, "the oracle has no network");
+}
+void sps_mlp_destroy(sps_mlp *ml) { (void)ml; }
+int sps_mlp_get_params(sps_mlp *ml, float *o) { (void)ml; (void)o; return fail(SPS_EINVAL, "the oracle has no network"); }
+int sps_mlp_set_params(sps_mlp *ml, const float *i) { (void)ml; (void)i; return fail(SPS_EINVAL, "the oracle has no network"); }
+int sps_mlp_forward(sps_mlp *ml, int64_t n, const float *x, float *y)
+{
+    (void)ml; (void)n; (void)x; (void)y;
+    return fail(SPS_EINVAL, "the oracle has no network");
+}
+int sps_mlp_gradient(sps_mlp *ml, int64_t n, const float *x, const float *t, float *g, float *l)
+{
+    (void)ml; (void)n; (void)x; (void)t; (void)g; (void)l;
+    return fail(SPS_EINVAL, "the oracle has no network");
+}
+int sps_mlp_train(sps_mlp *ml, int64_t n, const float *x, const float *t, int32_t e, int32_t b, float lr,
+                  uint64_t seed, float *el)
+{
+    (void)ml; (void)n; (void)x; (void)t; (void)e; (void)b; (void)lr; (void)seed; (void)el;
+    return fail(SPS_EINVAL, "the oracle has no network");
+}
+int sps_set_surrogate(sps_handle *h, sps_mlp *ml, double s, double c, double t)
+{
+    (void)s; (void)c; (void)t;
+    if (!h) return fail(SPS_EINVAL, "null handle");
+    return ml ? fail(SPS_EINVAL, "the oracle has no network") : 0;
+}
+int sps_get_surrogate_stats(sps_handle *h, int64_t *ev, int64_t *kept)
+{
+    if (!h || !ev || !kept) return fail(SPS_EINVAL, "null argument");
+    *ev = *kept = 0;
+    return 0;
+}
+
 /* The oracle searches every step whatever the skin (it runs every mode as
  * SPS_MODE_ORDERED); it accepts a skin so that the same driver runs on it. */
 int sps_set_list_skin(sps_handle *h, double skin)
@@ -1030,5 +1123,6 @@ void sps_destroy(sps_handle *h)
 {
     if (!h) return;
     free_all(h);
+    free(h->plog);
     free(h);
 }

@@ -23,6 +23,7 @@
 #include "../../include/sps.h"
 #include "sps_kernels.hip"
 #include "sps_mixed.hip"
+#include "sps_surrogate.hip"
 
 using namespace sps;
 
@@ -149,6 +150,16 @@ struct sps_handle {
     DBuf<double2> x_build;
     DBuf<unsigned long long> dmax;
     hipEvent_t ev_disp = nullptr;
+    // learned stress update (mixed form) and the plain method's training log
+    sps_mlp *mlp = nullptr;
+    int surr_cap = 0, log_cap = 0;
+    double surr_S = 1.0, surr_Sc = 1.0, surr_tol = 0.0;
+    DBuf<unsigned> surr_count;   // [0] plastic trial states this step, [1] tuples logged
+    DBuf<int> surr_slot;
+    DBuf<double4> surr_sn, surr_str;
+    DBuf<float> surr_feat;
+    DBuf<unsigned long long> surr_stats;
+    DBuf<double> plog;
     DBuf<int> err;
     // host-visible summary of the last search
     unsigned long long n_pairs = 0, m_pairs = 0, m_prev = 0;
@@ -176,13 +187,13 @@ struct sps_handle {
 enum KernelId {
     K_CHECK_OOD, K_BOUNDS, K_CELL_COUNT, K_CELL_SCATTER, K_CELL_SORT, K_PERMUTE, K_PROC,
     K_COUNT_PAIRS, K_SCAN, K_SPLIT, K_RK_INIT, K_STAGE_BEGIN, K_SPU_NODES,
-    K_SP_UPDATE, K_DERIV_NODES, K_DUMMY_STRESS, K_RK_FINAL, K_POSITION, K_NKERNELS
+    K_SP_UPDATE, K_DERIV_NODES, K_DUMMY_STRESS, K_RK_FINAL, K_POSITION, K_SURROGATE, K_NKERNELS
 };
 static const char *const kKernelNames[K_NKERNELS] = {
     "k_check_ood", "k_bounds", "k_cell_count", "k_cell_scatter", "k_cell_sort", "k_permute",
     "k_proc_slot", "k_count", "k_scan", "k_split", "k_rk_init",
     "k_stage_begin", "k_spu_nodes", "k_sp_update", "k_deriv_nodes",
-    "k_dummy_stress", "k_rk_final", "k_position"};
+    "k_dummy_stress", "k_rk_final", "k_position", "k_surrogate"};
 
 static hipEvent_t prof_begin(sps_handle *h) {
     if (!h->prof) return nullptr;
@@ -234,6 +245,8 @@ static void prof_collect(sps_handle *h) {
 #define LAUNCH(KID, GRID, BLK, KERN, ...) LAUNCH_LDS(KID, GRID, BLK, 0, KERN, __VA_ARGS__)
 
 extern "C" const char *sps_last_error(void) { return g_err; }
+
+#include "sps_surrogate_host.hip"
 extern "C" const char *sps_backend(void) { return "hip-gfx950"; }
 
 // ------------------------------------------------------------------ scans
@@ -617,7 +630,14 @@ static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
         if (st == 3 && nt2 > nt)
             LAUNCH(K_DUMMY_STRESS, nblk(nt2 - nt), BLOCK, k_dummy_stress, SB, V, nt, nt2);
     }
-    LAUNCH(K_RK_FINAL, nblk(nt), BLOCK, k_rk_final, SB, W, dt, adapt, tanfi, coh, nn, nt);
+    const bool hook = h->mlp || h->log_cap > 0;
+    if (h->mlp) HIP_TRY(hipMemsetAsync(h->surr_count.p, 0, sizeof(unsigned), s));
+    LAUNCH(K_RK_FINAL, nblk(nt), BLOCK, k_rk_final, SB, W, dt, adapt, tanfi, coh, nn, nt, surrogate_dev(h),
+           hook ? 1 : 0);
+    if (h->mlp && adapt) {
+        rc = surrogate_correct(h, SB);
+        if (rc) return rc;
+    }
     if (h->mixed) {
         if (nn > 0)
             LAUNCH_LDS(K_SPU_NODES, nblk8(nn), TILE, LAY_SPU_NODES_MIX.bytes, k_spu_nodes_mix, SB, V, PL, adapt, tanfi,
@@ -851,6 +871,8 @@ extern "C" void sps_destroy(sps_handle *h) {
     dfree(h->cnt); dfree(h->off); dfree(h->scan_tmp); dfree(h->ctot); dfree(h->cstart_rm); dfree(h->err);
     dfree(h->list_ia); dfree(h->list_wa); dfree(h->list_gxa); dfree(h->list_gya); dfree(h->list_ib);
     dfree(h->list_gxb); dfree(h->list_gyb); dfree(h->deg_a); dfree(h->deg_b); dfree(h->tile_ok);
+    dfree(h->x_build); dfree(h->dmax); dfree(h->surr_count); dfree(h->surr_slot); dfree(h->surr_sn);
+    dfree(h->surr_str); dfree(h->surr_feat); dfree(h->surr_stats); dfree(h->plog);
     if (h->pinned) (void)hipHostFree(h->pinned);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;

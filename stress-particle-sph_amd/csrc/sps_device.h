@@ -325,6 +325,57 @@ __device__ __forceinline__ void wall_force_term(double ri0, double ri1, double h
     t1 = scale * f2 * f * (r1 / (r2 * r2));
 }
 
+// ------------------------------------------------ learned stress update hook
+// The final stress update of a step records every stress point whose trial
+// stress the return map changes (a plastic trial state): for the network
+// that replaces the return map (features as the network takes them) and, when
+// a log is kept, as a training tuple of the plain method.
+constexpr int SURR_NIN = 10;
+
+struct SurrogateDev {
+    unsigned *count;         // plastic trial states logged this step
+    int cap;
+    int *slot;               // their storage slots
+    double4 *s_n, *s_tr;     // stress before the step and trial stress
+    float *feat;             // their features (cap x 10)
+    float stress_scale, coh_scale;
+    unsigned long long *stats;   // [0] evaluated, [1] kept
+    // plain-method log (training tuples), when log_cap > 0
+    unsigned *log_count;
+    int log_cap;
+    double *log;             // log_cap x 12: sigma_n, sigma_trial, R(sigma_trial)
+};
+
+// Record a stress point's plastic trial state for the network (and the
+// plain method's tuple for training).
+__device__ __forceinline__ void surrogate_record(const SurrogateDev &G, int i, const double4 &s_n,
+                                                 const double4 &s_tr, const double4 &s_ret, double tanfi,
+                                                 double coh) {
+    if (G.log_cap > 0) {
+        const unsigned k = atomicAdd(G.log_count, 1u);
+        if ((int)k < G.log_cap) {
+            double *o = G.log + 12ll * k;
+            o[0] = s_n.x; o[1] = s_n.y; o[2] = s_n.z; o[3] = s_n.w;
+            o[4] = s_tr.x; o[5] = s_tr.y; o[6] = s_tr.z; o[7] = s_tr.w;
+            o[8] = s_ret.x; o[9] = s_ret.y; o[10] = s_ret.z; o[11] = s_ret.w;
+        }
+    }
+    if (G.cap <= 0) return;
+    const unsigned k = atomicAdd(G.count, 1u);
+    if ((int)k >= G.cap) return;          // beyond the batch: the exact result stays
+    G.slot[k] = i;
+    G.s_n[k] = s_n;
+    G.s_tr[k] = s_tr;
+    const float is = 1.0f / G.stress_scale;
+    float *f = G.feat + (long long)k * SURR_NIN;
+    f[0] = (float)s_n.x * is; f[1] = (float)s_n.y * is; f[2] = (float)s_n.z * is; f[3] = (float)s_n.w * is;
+    f[4] = (float)(s_tr.x - s_n.x) * is; f[5] = (float)(s_tr.y - s_n.y) * is;
+    f[6] = (float)(s_tr.z - s_n.z) * is; f[7] = (float)(s_tr.w - s_n.w) * is;
+    f[8] = (float)coh / G.coh_scale;
+    f[9] = (float)tanfi;
+}
+
+
 // ------------------------------------------------------- mixed precision
 // The mixed-precision form sums pair terms in single precision with
 // compensation (Kahan 1965: the rounding error of every addition is carried

@@ -1912,8 +1912,9 @@ __global__ void k_dummy_stress(StateBufs S, SearchView V, int nt, int nt2) {
 }
 
 // End of RK4 (2_...:784-799) and update_strain (3_...:1847-1863).
+// hook: record the stress points' plastic trial states (SurrogateDev).
 __global__ void k_rk_final(StateBufs S, Work W, double dt, int adapt, double tanfi, double coh,
-                           int nn, int nt) {
+                           int nn, int nt, SurrogateDev G, int hook) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nt) return;
     double c = dt / 6;
@@ -1929,7 +1930,14 @@ __global__ void k_rk_final(StateBufs S, Work W, double dt, int adapt, double tan
     } else {
         double4 s0 = W.stress0[i], rk = W.rk_stress[i];
         double4 s = make_double4(s0.x + c * rk.x, s0.y + c * rk.y, s0.z + c * rk.z, s0.w + c * rk.w);
-        if (adapt) adapt_stress(s, tanfi, coh);
+        if (adapt && hook) {
+            const double4 tr = s;
+            adapt_stress(s, tanfi, coh);
+            if (s.x != tr.x || s.y != tr.y || s.z != tr.z || s.w != tr.w)
+                surrogate_record(G, i, s0, tr, s, tanfi, coh);
+        } else if (adapt) {
+            adapt_stress(s, tanfi, coh);
+        }
         S.stress[i] = s;
         double dd = W.rk_dev[i] / 6;
         W.ddev[i] = dd;

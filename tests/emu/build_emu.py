@@ -30,7 +30,8 @@ BUILD = os.path.join(HERE, "_build")
 
 # kernels that use barriers or wave exchanges (their threads run as fibers)
 SYNC = ("k_bounds_partial", "k_bounds_final", "k_count", "k_spu_nodes", "k_sp_update", "k_deriv_nodes",
-        "k_scan_local", "k_scan_sums", "k_max_disp")
+        "k_scan_local", "k_scan_sums", "k_max_disp", "k_gemm_bf16", "k_mlp_loss", "k_mlp_colsum",
+        "k_surrogate_apply")
 
 
 def split_top(s):
@@ -82,8 +83,10 @@ def generate(gen):
     api = api.replace('#include "../../include/sps.h"', '#include "sps.h"')
     api = '#include "hip/hip_runtime.h"\n#define sps_lds (::emu::lds_base)\n' + rewrite_launches(api)
     open(os.path.join(gen, "sps_api_emu.cpp"), "w").write(api)
+    host = rewrite_launches(open(os.path.join(SRC, "sps_surrogate_host.hip")).read())
+    open(os.path.join(gen, "sps_surrogate_host.hip"), "w").write(host)
     decl = "extern __shared__ __align__(16) unsigned char sps_lds[];"
-    for name in ("sps_kernels.hip", "sps_mixed.hip"):
+    for name in ("sps_kernels.hip", "sps_mixed.hip", "sps_surrogate.hip"):
         ker = open(os.path.join(SRC, name)).read()
         if name == "sps_kernels.hip" and decl not in ker:
             raise RuntimeError("dynamic LDS declaration not found")
@@ -95,7 +98,11 @@ def generate(gen):
             elif cur:
                 ln = ln.replace("<LY>", "<" + cur + ">")
             lines.append(ln)
-        open(os.path.join(gen, name), "w").write("\n".join(lines))
+        text = "\n".join(lines)
+        # g++ spells the device vector types with vector_size
+        text = re.sub(r"typedef (short|float) (\w+) __attribute__\(\(ext_vector_type\(4\)\)\);",
+                      r"typedef \1 \2 __attribute__((vector_size(4 * sizeof(\1))));", text)
+        open(os.path.join(gen, name), "w").write(text)
     open(os.path.join(gen, "sps_device.h"), "w").write(open(os.path.join(SRC, "sps_device.h")).read())
 
 

@@ -20,7 +20,8 @@ bool needs_sync(const char *name) {
     while (*name == '(') ++name;   // a kernel name in parentheses (template arguments with commas)
     static const char *const sync[] = {"k_bounds_partial", "k_bounds_final", "k_count", "k_spu_nodes",
                                        "k_sp_update",      "k_deriv_nodes",  "k_scan_local", "k_scan_sums",
-                                       "k_max_disp"};
+                                       "k_max_disp",       "k_gemm_bf16",    "k_mlp_loss",   "k_mlp_colsum",
+                                       "k_surrogate_apply"};
     for (const char *s : sync)
         if (strncmp(name, s, strlen(s)) == 0) return true;   // k_count also covers k_count_lists
     return false;
@@ -46,7 +47,7 @@ jmp_buf sched_jb;
 int cur = -1;
 bool fiber_mode = false;
 const std::function<void()> *body_fn = nullptr;
-unsigned char wave_buf[MAX_BLOCK / 64][64][8];
+unsigned char wave_buf[MAX_BLOCK / 64][64][16];
 int or_acc = 0;
 
 void yield_as(State s) {
@@ -92,7 +93,7 @@ bool release(unsigned block) {
     }
     if (live == 0) return false;
     if (at != live) {
-        fprintf(stderr, "emu: threads of block %u wait at different barriers\n", blockIdx.x);
+        fprintf(stderr, "emu: threads of block %u wait at different barriers\n", (unsigned)blockIdx.x);
         abort();
     }
     for (unsigned t = 0; t < block; ++t)
@@ -121,7 +122,7 @@ void wave_put(const void *v, int bytes) {
 
 void wave_get(int lane, void *v, int bytes) { memcpy(v, wave_buf[threadIdx.x >> 6][lane], bytes); }
 
-void launch(unsigned grid, unsigned block, size_t lds, const std::function<void()> &body, bool sync) {
+void launch(dim3 grid, unsigned block, size_t lds, const std::function<void()> &body, bool sync) {
     if (block > MAX_BLOCK) {
         fprintf(stderr, "emu: block of %u threads\n", block);
         abort();
@@ -130,12 +131,17 @@ void launch(unsigned grid, unsigned block, size_t lds, const std::function<void(
     std::vector<unsigned char> ldsmem(lds ? lds : 1);
     lds_base = ldsmem.data();
     lds_bytes = lds;
-    gridDim.x = grid;
+    gridDim.x = grid.x;
+    gridDim.y = grid.y;
+    gridDim.z = grid.z;
     blockDim.x = block;
     body_fn = &body;
     fiber_mode = sync;
-    for (unsigned b = 0; b < grid; ++b) {
-        blockIdx.x = b;
+    const unsigned long long nblocks = (unsigned long long)grid.x * grid.y * grid.z;
+    for (unsigned long long b = 0; b < nblocks; ++b) {
+        blockIdx.x = (unsigned)(b % grid.x);
+        blockIdx.y = (unsigned)(b / grid.x % grid.y);
+        blockIdx.z = (unsigned)(b / grid.x / grid.y);
         if (!sync) {
             for (unsigned t = 0; t < block; ++t) {
                 threadIdx.x = t;

@@ -27,6 +27,10 @@ using std::max;
 
 struct dim3e { unsigned x = 0, y = 0, z = 0; };
 extern dim3e threadIdx, blockIdx, blockDim, gridDim;
+struct dim3 {
+    unsigned x, y, z;
+    dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {}
+};
 
 struct double2 { double x, y; };
 struct double3 { double x, y, z; };
@@ -51,11 +55,11 @@ void wave_put(const void *v, int bytes);
 void wave_get(int lane, void *v, int bytes);
 // run body for every thread of grid x block; sync: the kernel uses barriers
 // or wave exchanges (its threads then run as fibers)
-void launch(unsigned grid, unsigned block, size_t lds, const std::function<void()> &body, bool sync);
+void launch(dim3 grid, unsigned block, size_t lds, const std::function<void()> &body, bool sync);
 // A launch as `kern<<<grid, block, lds>>>(args...)`: the arguments are
 // evaluated once, on the host, and every thread gets the same copies.
 template <typename... P, typename... A>
-void launch_k(unsigned grid, unsigned block, size_t lds, bool sync, void (*kern)(P...), A... args) {
+void launch_k(dim3 grid, unsigned block, size_t lds, bool sync, void (*kern)(P...), A... args) {
     launch(grid, block, lds, [&] { kern(args...); }, sync);
 }
 extern unsigned char *lds_base;   // the launch's dynamic LDS
@@ -87,6 +91,51 @@ template <typename T> T atomicAdd(T *p, T v) { T o = *p; *p = o + v; return o; }
 inline float __builtin_amdgcn_rcpf(float x) { return 1.0f / x; }
 inline float __builtin_amdgcn_sqrtf(float x) { return sqrtf(x); }
 #define __HIP_MEMORY_SCOPE_AGENT 0
+inline unsigned __float_as_uint(float f) { unsigned u; memcpy(&u, &f, 4); return u; }
+inline float __uint_as_float(unsigned u) { float f; memcpy(&f, &u, 4); return f; }
+inline float __expf(float x) { return expf(x); }
+inline int __popcll(unsigned long long v) { return __builtin_popcountll(v); }
+// bit l of the result: lane l of the wave has p set (every live lane takes part)
+inline unsigned long long __ballot(int p) {
+    const int mine = p != 0;
+    emu::wave_put(&mine, sizeof mine);
+    unsigned long long m = 0;
+    const unsigned w0 = threadIdx.x & ~63u;
+    for (unsigned l = 0; l < 64 && w0 + l < blockDim.x; ++l) {
+        int v = 0;
+        emu::wave_get((int)l, &v, sizeof v);
+        if (v) m |= 1ull << l;
+    }
+    emu::wave_put(nullptr, 0);
+    return m;
+}
+// v_mfma_f32_16x16x16_bf16 of a wave: lane l holds A[l % 16][4 (l / 16) + q]
+// and B[4 (l / 16) + q][l % 16] (bfloat16 bits), and gets C[4 (l / 16) + q]
+// [l % 16] plus the products, summed in single precision.
+typedef short emu_v4s __attribute__((vector_size(8)));
+typedef float emu_v4f __attribute__((vector_size(16)));
+inline emu_v4f __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(emu_v4s a, emu_v4s b, emu_v4f c, int, int, int) {
+    struct AB {
+        emu_v4s a, b;
+    } mine{a, b};
+    emu::wave_put(&mine, sizeof mine);
+    const int l = threadIdx.x & 63;
+    auto bf = [](short v) { return __uint_as_float((unsigned)(unsigned short)v << 16); };
+    emu_v4f d = c;
+    for (int q = 0; q < 4; ++q) {
+        const int i = 4 * (l / 16) + q, j = l % 16;
+        float acc = 0.0f;
+        for (int k = 0; k < 16; ++k) {
+            AB la, lb;
+            emu::wave_get(i + 16 * (k / 4), &la, sizeof la);
+            emu::wave_get(j + 16 * (k / 4), &lb, sizeof lb);
+            acc += bf(la.a[k % 4]) * bf(lb.b[k % 4]);
+        }
+        d[q] += acc;
+    }
+    emu::wave_put(nullptr, 0);
+    return d;
+}
 template <typename T> T __hip_atomic_load(const T *p, int, int) { return *p; }
 inline int atomicOr(int *p, int v) { int o = *p; *p = o | v; return o; }
 inline int atomicMin(int *p, int v) { int o = *p; *p = std::min(o, v); return o; }
@@ -107,6 +156,7 @@ inline hipError_t hipFree(void *p) { free(p); return hipSuccess; }
 inline hipError_t hipHostMalloc(void **p, size_t n, unsigned) { *p = malloc(n); return hipSuccess; }
 inline hipError_t hipHostFree(void *p) { free(p); return hipSuccess; }
 inline hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, hipMemcpyKind, hipStream_t) { memmove(d, s, n); return hipSuccess; }
+inline hipError_t hipMemcpy(void *d, const void *s, size_t n, hipMemcpyKind) { memmove(d, s, n); return hipSuccess; }
 inline hipError_t hipMemsetAsync(void *d, int v, size_t n, hipStream_t) { memset(d, v, n); return hipSuccess; }
 inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
 inline hipError_t hipGetLastError() { return hipSuccess; }

@@ -60,3 +60,14 @@ def test_cpu_build_of_mixed_kernels_near_oracle(emu_lib, case, steps, tol):
                         case, str(steps), tol], env=env, capture_output=True, text=True, timeout=900)
     print(p.stdout[-2000:], p.stderr[-4000:])
     assert p.returncode == 0, p.stderr[-4000:] or p.stdout
+
+
+# the network kernels (matrix-core GEMM emulated from its lane layout, loss,
+# backward, Adam) against the numpy restatement
+def test_cpu_build_of_network_kernels_matches_numpy_oracle(emu_lib):
+    env = dict(os.environ, LD_PRELOAD=_asan_runtime(),
+               ASAN_OPTIONS="detect_leaks=0:detect_stack_use_after_return=0:halt_on_error=1")
+    p = subprocess.run([sys.executable, os.path.join(ROOT, "tests", "emu", "run_mlp.py"), emu_lib], env=env,
+                       capture_output=True, text=True, timeout=900)
+    print(p.stdout[-2000:], p.stderr[-4000:])
+    assert p.returncode == 0, p.stderr[-4000:] or p.stdout

@@ -1,0 +1,181 @@
+"""The learned Drucker-Prager stress update on the GPU.
+
+* The network's forward pass and gradient (matrix-core bfloat16 GEMMs) equal
+  the numpy restatement (oracle/mlp_oracle.py, itself pinned to finite
+  differences) to bfloat16 rounding: this also checks the matrix
+  instruction's lane layout the GEMM kernel assumes.
+* Training lowers the loss.
+* The exact return map and the plain method's logged plastic tuples equal
+  the oracle's bit for bit.
+* In a step of the mixed form the network corrects plastic trial states,
+  results that fail the yield-function check fall back to the exact return
+  map, and the trained network captures most of the exact correction on
+  held-out states (the gain over the method without it: the uncorrected
+  elastic trial, whose error is the whole correction).
+"""
+import ctypes
+import importlib
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import ORACLE_LIB, ROOT, deck_dir, load_pkg
+
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+import mlp_oracle as mo  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def surrogate():
+    load_pkg()
+    return importlib.import_module("stress-particle-sph_amd.surrogate")
+
+
+def product_lib():
+    return ctypes.CDLL(load_pkg().PRODUCT_LIB)
+
+
+def test_forward_and_gradient_match_numpy_oracle():
+    sur = surrogate()
+    say("network forward / gradient vs numpy")
+    net = sur.Network(product_lib(), seed=11, batch_rows=1024)
+    theta = net.params()
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((1000, 10)).astype(np.float32)
+    t = (0.1 * rng.standard_normal((1000, 4))).astype(np.float32)
+    y = net.forward(x)
+    yo, _ = mo.forward(theta, x)
+    say(f"  forward max|diff| {np.abs(y - yo).max():.3e} of {np.abs(yo).max():.3e}")
+    assert np.abs(y - yo).max() <= 1e-2 * np.abs(yo).max()
+    assert np.mean(np.abs(y - yo)) <= 2e-3 * np.mean(np.abs(yo))
+    loss, g = net.gradient(x, t)
+    lo, go = mo.loss_and_grad(theta, x, t)
+    rel = np.linalg.norm(g - go) / np.linalg.norm(go)
+    say(f"  loss {loss:.6e} vs {lo:.6e}, gradient rel {rel:.3e}")
+    assert abs(loss - lo) <= 1e-3 * lo
+    assert rel <= 2e-2
+    # every parameter block, not just the norm
+    offs = np.cumsum([0] + [int(np.prod(s)) for _, s in mo.SHAPES])
+    for b, (name, _) in enumerate(mo.SHAPES):
+        sl = slice(offs[b], offs[b + 1])
+        if np.linalg.norm(go[sl]) > 0:
+            assert np.linalg.norm(g[sl] - go[sl]) <= 5e-2 * np.linalg.norm(go[sl]), name
+
+
+def test_training_lowers_the_loss():
+    sur = surrogate()
+    lib = product_lib()
+    rng = np.random.default_rng(9)
+    # a regression task of the same shape: the exact return map's corrections
+    n = 20000
+    tanfi, coh, S = np.tan(np.radians(25.0)), 5e3, 1e5
+    s_n = -S * rng.uniform(0.0, 1.0, size=(n, 4))
+    s_n[:, 2] = S * rng.uniform(-0.3, 0.3, size=n)
+    s_tr = s_n + 0.3 * S * rng.standard_normal((n, 4))
+    ret = sur.return_map(lib, s_tr, tanfi, coh)
+    x = sur.features(s_n, s_tr, coh, tanfi, S, 1e4)
+    keep = np.any(ret != s_tr, axis=1) & sur.correctable(s_tr, tanfi, coh)
+    x, t = x[keep], sur.targets(s_tr[keep], ret[keep], tanfi, coh)
+    net = sur.Network(lib, seed=2, batch_rows=2048)
+    losses = net.train(x, t, epochs=15, batch=1024, lr=2e-3)
+    say(f"  losses {losses}")
+    assert losses[-1] < 0.2 * losses[0]
+
+
+def test_return_map_equals_oracle():
+    sur = surrogate()
+    rng = np.random.default_rng(4)
+    n = 5000
+    s = 1e5 * rng.standard_normal((n, 4))
+    tf = np.tan(np.radians(rng.uniform(10, 40, size=n)))
+    c = rng.uniform(0, 2e4, size=n)
+    gpu = sur.return_map(product_lib(), s, tf, c)
+    cpu = sur.return_map(ctypes.CDLL(ORACLE_LIB), s, tf, c)
+    assert np.array_equal(gpu.view(np.uint64), cpu.view(np.uint64))
+    assert np.any(gpu != s)
+
+
+def sorted_rows(*cols):
+    a = np.hstack(cols)
+    return a[np.lexsort(a.T[::-1])]
+
+
+@pytest.mark.parametrize("deck_name,steps", [("t1_tiny", 30), ("c1_collapse", 10)])
+def test_plastic_log_equals_oracle(deck_name, steps):
+    sps = load_pkg()
+    sur = surrogate()
+    deck = sps.read_deck(deck_dir(deck_name))
+    logs = []
+    for lib in (sps.PRODUCT_LIB, ORACLE_LIB):
+        sim = sps.Simulation.from_deck(deck, lib_path=lib)
+        sur.log_plastic(sim, 1 << 20)
+        sim.step(steps)
+        logs.append(sorted_rows(*sur.plastic_log(sim, 1 << 20)))
+    say(f"  {deck_name}: {len(logs[0])} tuples")
+    assert len(logs[0]) > 0
+    assert logs[0].shape == logs[1].shape and np.array_equal(logs[0].view(np.uint64), logs[1].view(np.uint64))
+
+
+def trained_network(sps, sur, deck, steps, epochs=40):
+    """A network trained on the plain method's tuples of `steps` steps of
+    `deck` (and their augmentation); returns it, the scales and a held-out
+    part of the tuples."""
+    lib = product_lib()
+    plain = sps.Simulation.from_deck(deck)
+    sur.log_plastic(plain, 1 << 21)
+    plain.step(steps)
+    sn, st, sr = sur.plastic_log(plain, 1 << 21)
+    p = plain.params
+    tanfi, coh = p.props[12], p.props[13]
+    S = float(np.abs(st).max())
+    Sc = max(float(coh), 1e3)
+    ok = sur.correctable(st, tanfi, coh)
+    sn, st, sr = sn[ok], st[ok], sr[ok]
+    rng = np.random.default_rng(0)
+    hold = rng.random(len(sn)) < 0.2
+    a = sur.augment(lib, sn[~hold], st[~hold], tanfi, coh, rng, copies=4, scale_max=10.0)
+    x = np.vstack([sur.features(sn[~hold], st[~hold], coh, tanfi, S, Sc), sur.features(a[0], a[1], a[3], a[4], S, Sc)])
+    t = np.vstack([sur.targets(st[~hold], sr[~hold], tanfi, coh), sur.targets(a[1], a[2], a[4], a[3])])
+    net = sur.Network(lib, seed=5, batch_rows=4096)
+    losses = net.train(x, t, epochs=epochs, batch=512, lr=2e-3)
+    held = (sn[hold], st[hold], sr[hold])
+    return net, S, Sc, tanfi, coh, held, losses
+
+
+def test_trained_network_gain_and_fallback_in_step():
+    sps = load_pkg()
+    sur = surrogate()
+    acc = importlib.import_module("stress-particle-sph_amd.accuracy")
+    deck = sps.read_deck(deck_dir("c1_collapse"))
+    net, S, Sc, tanfi, coh, (hn, ht, hr), losses = trained_network(sps, sur, deck, 20)
+    say(f"  training losses {losses[0]:.3e} -> {losses[-1]:.3e}; held-out tuples {len(hn)}")
+    # gain on held-out plastic states: error of the corrected stress against the
+    # exact return map, relative to the correction itself (= the error of the
+    # uncorrected elastic trial)
+    y = net.forward(sur.features(hn, ht, coh, tanfi, S, Sc)).astype(np.float64)
+    pred = sur.corrected(ht, y, tanfi, coh)
+    err = np.linalg.norm(pred - hr) / np.linalg.norm(ht - hr)
+    say(f"  held-out error relative to the correction: {err:.3e}")
+    assert err < 0.2
+    # in a step: plastic trial states are evaluated, inconsistent ones fall back
+    steps = 20
+    mix = sps.Simulation.from_deck(deck, mode=sps.MODE_FAST)
+    ref = sps.Simulation.from_deck(deck, mode=sps.MODE_FAST)
+    x0 = ref.get_state(["x"])["x"]
+    sur.attach(mix, net, S, Sc, tol=1e-4)
+    mix.step(steps)
+    ref.step(steps)
+    ev, kept = sur.stats(mix)
+    e = acc.compare_states(mix.get_state(), ref.get_state(), x0)
+    say(f"  evaluated {ev}, kept {kept} ({kept / max(ev, 1):.1%}); against the exact return map: {e}")
+    assert ev > 0
+    assert 0 <= kept <= ev
+    assert e["stress_rel_rms"] < 1e-2
