@@ -177,6 +177,16 @@ int sps_get_pairs(sps_handle *h, int64_t cap, int32_t *pair_i, int32_t *pair_j,
 
 int sps_get_stats(sps_handle *h, sps_stats *out);
 
+/* A material per particle.  No counterpart in the reference, which has one
+ * material (props(1,:)); for an ensemble of instances concatenated into one
+ * particle set.  table: nmat rows of 10 doubles: tan(phi), c, E, nu, D11, D22,
+ * D12, D33, D41, D42 (the elastic matrix of 3_...:719-730); mat: each
+ * particle's row (ntotal2, 0-based, in sps_set_state's order).  Call after
+ * sps_set_state; it then replaces props(1,3:4), props(1,13:14) and D11..D42
+ * of sps_params for the pair loops, the Drucker-Prager terms and the return
+ * map, particle by particle. */
+int sps_set_materials(sps_handle *h, int32_t nmat, const double *table, const int32_t *mat);
+
 /* Pair lists kept over steps.  No counterpart in the reference, which
  * searches every step; SPS_MODE_FAST only.  With skin > 0 the search also
  * lists the partners within 2h + skin (cells 2 max h + skin wide) and later
@@ -185,6 +195,13 @@ int sps_get_stats(sps_handle *h, sps_stats *out);
  * with each search.  0 (the default) searches every step.
  * sps_get_search_count returns the searches made since sps_create. */
 int sps_set_list_skin(sps_handle *h, double skin);
+
+/* Replay of the mixed form's step (SPS_MODE_FAST; no counterpart in the
+ * reference): with enable != 0, the launches of a step that reuses the pair
+ * lists are captured once into a HIP graph (one per state buffer, again when
+ * dt changes) and replayed; steps that search, and steps while kernel timing
+ * is on, launch directly. */
+int sps_set_graph(sps_handle *h, int32_t enable);
 int sps_get_search_count(sps_handle *h, int64_t *n);
 
 /* Learned Drucker-Prager stress update.  No counterpart in the reference.

@@ -76,6 +76,10 @@ struct sps_handle {
     /* plastic trial-state log (sps_set_plastic_log) */
     int64_t log_cap, log_n;
     double *plog;
+    /* a material per particle (sps_set_materials): rows of 10 doubles
+     * tan(phi), c, E, nu, D11, D22, D12, D33, D41, D42 */
+    int32_t *mat;
+    double *mtab;
     float *av_tmp;
 };
 
@@ -358,10 +362,26 @@ static void adapt_one(double *s, double tanfi, double coh)
     }
 }
 
+/* Material constants of particle i: its row of the material table when one
+ * is set, else props(1,:) and the elastic matrix of sps_params. */
+static double mat_tanfi(const struct sps_handle *h, int i) { return h->mat ? h->mtab[10 * h->mat[i]] : h->p.props[12]; }
+static double mat_coh(const struct sps_handle *h, int i) { return h->mat ? h->mtab[10 * h->mat[i] + 1] : h->p.props[13]; }
+static double mat_young(const struct sps_handle *h, int i) { return h->mat ? h->mtab[10 * h->mat[i] + 2] : h->p.props[2]; }
+static double mat_poiss(const struct sps_handle *h, int i) { return h->mat ? h->mtab[10 * h->mat[i] + 3] : h->p.props[3]; }
+/* D11, D22, D12, D33, D41, D42 of particle i */
+static void mat_D(const struct sps_handle *h, int i, double D[6])
+{
+    if (h->mat) {
+        memcpy(D, h->mtab + 10 * h->mat[i] + 4, 6 * sizeof(double));
+    } else {
+        D[0] = h->p.D11; D[1] = h->p.D22; D[2] = h->p.D12; D[3] = h->p.D33; D[4] = h->p.D41; D[5] = h->p.D42;
+    }
+}
+
 /* all of 1..ntotal */
 static void adapt_stress2(struct sps_handle *h)
 {
-    for (int i = 0; i < h->nt; ++i) adapt_one(h->stress + 4 * i, h->p.props[12], h->p.props[13]);
+    for (int i = 0; i < h->nt; ++i) adapt_one(h->stress + 4 * i, mat_tanfi(h, i), mat_coh(h, i));
 }
 
 /* ---------------------------------------------------- stress_point_update */
@@ -501,12 +521,13 @@ static void get_derivatives(struct sps_handle *h)
             for (int p = 0; p < nn; ++p) G2(a, 1, p) = AE(4, p) * (AE(2, p) * G2(a, 0, p) + AE(3, p) * G2(a, 1, p));
         }
     }
-    const sps_params *P = &h->p;
     for (int p = nn; p < nt; ++p) { /* 2_...:633-636 */
-        h->divf1[4 * p + 0] = -(P->D11 * G1(0, 0, p) + P->D12 * G1(1, 1, p));
-        h->divf1[4 * p + 1] = -(P->D12 * G1(0, 0, p) + P->D22 * G1(1, 1, p));
-        h->divf1[4 * p + 2] = -(P->D33 * G1(1, 0, p) + P->D33 * G1(0, 1, p));
-        h->divf1[4 * p + 3] = -(P->D41 * G1(0, 0, p) + P->D42 * G1(1, 1, p));
+        double D[6]; /* D11 D22 D12 D33 D41 D42 */
+        mat_D(h, p, D);
+        h->divf1[4 * p + 0] = -(D[0] * G1(0, 0, p) + D[2] * G1(1, 1, p));
+        h->divf1[4 * p + 1] = -(D[2] * G1(0, 0, p) + D[1] * G1(1, 1, p));
+        h->divf1[4 * p + 2] = -(D[3] * G1(1, 0, p) + D[3] * G1(0, 1, p));
+        h->divf1[4 * p + 3] = -(D[4] * G1(0, 0, p) + D[5] * G1(1, 1, p));
     }
     for (int p = 0; p < nn; ++p) { /* 2_...:641-642 */
         h->divf2[2 * p + 0] = -(G2(0, 0, p) + G2(2, 1, p));
@@ -533,8 +554,8 @@ static void plastic_terms(struct sps_handle *h, int ie, double time_sph, double 
             const double *st = h->stress + 4 * ie;
             double stress2[4] = {st[0], st[1], st[2], st[3]};
             double f0 = h->fdr[ie];
-            float tanfi = (float)props[12], coh = (float)props[13];
-            float young = (float)props[2], poiss = (float)props[3];
+            float tanfi = (float)mat_tanfi(h, ie), coh = (float)mat_coh(h, ie);
+            float young = (float)mat_young(h, ie), poiss = (float)mat_poiss(h, ie);
             float smean = (float)((stress2[0] + stress2[1] + stress2[3]) / 3.0);
             float dv[4];
             dv[0] = (float)(stress2[0] - (double)smean);
@@ -835,7 +856,7 @@ static void rk4(struct sps_handle *h, double dt_sph, double time_sph)
             double ret[4];
             const double *tr = h->stress + 4 * i;
             memcpy(ret, tr, sizeof ret);
-            adapt_one(ret, h->p.props[12], h->p.props[13]);   /* a copy: adapt_stress2 below does it */
+            adapt_one(ret, mat_tanfi(h, i), mat_coh(h, i));   /* a copy: adapt_stress2 below does it */
             if (memcmp(tr, ret, sizeof ret) != 0 && h->log_n < h->log_cap) {
                 double *o = h->plog + 12 * h->log_n++;
                 memcpy(o, h->stress0 + 4 * i, 4 * sizeof(double));
@@ -1019,6 +1040,28 @@ int sps_get_stats(sps_handle *h, sps_stats *out)
     return 0;
 }
 
+int sps_set_materials(sps_handle *h, int32_t nmat, const double *table, const int32_t *mat)
+{
+    if (!h || nmat < 1 || !table || !mat) return fail(SPS_EINVAL, "bad argument");
+    for (int i = 0; i < h->nt2; ++i)
+        if (mat[i] < 0 || mat[i] >= nmat) return fail(SPS_EINVAL, "particle %d: material %d of %d", i + 1, mat[i], nmat);
+    free(h->mat);
+    free(h->mtab);
+    h->mat = (int32_t *)malloc((size_t)h->nt2 * sizeof(int32_t));
+    h->mtab = (double *)malloc((size_t)nmat * 10 * sizeof(double));
+    if (!h->mat || !h->mtab) return fail(SPS_ENOMEM, "material allocation");
+    memcpy(h->mat, mat, (size_t)h->nt2 * sizeof(int32_t));
+    memcpy(h->mtab, table, (size_t)nmat * 10 * sizeof(double));
+    return 0;
+}
+
+/* The oracle launches nothing: graph replay is accepted and changes nothing. */
+int sps_set_graph(sps_handle *h, int32_t enable)
+{
+    (void)enable;
+    return h ? 0 : fail(SPS_EINVAL, "null argument");
+}
+
 int sps_set_plastic_log(sps_handle *h, int64_t cap)
 {
     if (!h || cap < 0 || cap > ((int64_t)1 << 30)) return fail(SPS_EINVAL, "bad argument");
@@ -1124,5 +1167,7 @@ void sps_destroy(sps_handle *h)
     if (!h) return;
     free_all(h);
     free(h->plog);
+    free(h->mat);
+    free(h->mtab);
     free(h);
 }

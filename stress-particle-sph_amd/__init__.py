@@ -101,13 +101,15 @@ def load_library(path):
     lib.sps_get_profile.argtypes = [h, ctypes.c_int32, ctypes.POINTER(KernelTime),
                                     ctypes.POINTER(ctypes.c_int32), ctypes.c_int32]
     lib.sps_set_list_skin.argtypes = [h, ctypes.c_double]
+    lib.sps_set_materials.argtypes = [h, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p]
+    lib.sps_set_graph.argtypes = [h, ctypes.c_int32]
     lib.sps_get_search_count.argtypes = [h, ctypes.POINTER(ctypes.c_int64)]
     lib.sps_destroy.argtypes = [h]
     lib.sps_destroy.restype = None
     lib.sps_last_error.restype = ctypes.c_char_p
     lib.sps_backend.restype = ctypes.c_char_p
     for f in ("sps_create", "sps_set_state", "sps_step", "sps_get_state", "sps_get_pairs",
-              "sps_get_stats", "sps_set_profiling", "sps_get_profile", "sps_set_list_skin",
+              "sps_get_stats", "sps_set_profiling", "sps_get_profile", "sps_set_list_skin", "sps_set_materials", "sps_set_graph",
               "sps_get_search_count"):
         getattr(lib, f).restype = ctypes.c_int
     return lib
@@ -263,6 +265,18 @@ class Simulation:
         self._check(self.lib.sps_get_stats(self.handle, ctypes.byref(s)))
         return dict(n_pairs=s.n_pairs, m_pairs=s.m_pairs, n_by_type=list(s.n_by_type),
                     ndivx=s.ndivx, ndivy=s.ndivy, steps=s.steps)
+
+    def set_materials(self, table, mat):
+        """A material per particle: table rows (tan phi, c, E, nu, D11, D22,
+        D12, D33, D41, D42), mat the row of each particle (after set_state)."""
+        table = np.ascontiguousarray(table, dtype=np.float64)
+        mat = np.ascontiguousarray(mat, dtype=np.int32)
+        assert table.ndim == 2 and table.shape[1] == 10 and mat.shape == (self.n2,)
+        self._check(self.lib.sps_set_materials(self.handle, table.shape[0], table.ctypes.data, mat.ctypes.data))
+
+    def set_graph(self, on=True):
+        """Replay the mixed form's list-reusing steps as a captured graph."""
+        self._check(self.lib.sps_set_graph(self.handle, int(bool(on))))
 
     def set_list_skin(self, skin):
         """Keep the pair lists over steps, rebuilt when a particle has moved

@@ -85,12 +85,16 @@ struct StateSet {
     DBuf<double> mass, rho, hsml, ivar1, fdr;
     DBuf<int> orig, ood;
     DBuf<float> wallpos, horiz;
+    DBuf<int> mat;                 // per-slot materials, when a table is set
+    const Material *mtab = nullptr;
 
     StateBufs view() {
         StateBufs s;
         s.x = x.p; s.vel = vel.p; s.x00 = x00.p; s.displ = displ.p; s.stress = stress.p;
         s.mass = mass.p; s.rho = rho.p; s.hsml = hsml.p; s.ivar1 = ivar1.p; s.fdr = fdr.p;
         s.orig = orig.p; s.ood = ood.p; s.wallpos = wallpos.p; s.horiz = horiz.p;
+        s.mat = mtab ? mat.p : nullptr;
+        s.mtab = mtab;
         return s;
     }
     int alloc(int nn, int nt, int nt2) {
@@ -102,9 +106,11 @@ struct StateSet {
         rc |= dalloc(horiz, nt2);
         return rc;
     }
+    int mat_alloc(int nt2) { return dalloc(mat, nt2); }
     void release() {
         dfree(x); dfree(vel); dfree(x00); dfree(displ); dfree(stress); dfree(mass); dfree(rho);
         dfree(hsml); dfree(ivar1); dfree(fdr); dfree(orig); dfree(ood); dfree(wallpos); dfree(horiz);
+        dfree(mat);
     }
 };
 
@@ -157,6 +163,16 @@ struct sps_handle {
     DBuf<unsigned> surr_count;   // [0] plastic trial states this step, [1] tuples logged
     DBuf<int> surr_slot;
     DBuf<double4> surr_sn, surr_str;
+    DBuf<double2> surr_mat;
+    // materials (an ensemble): table and each original particle's entry
+    DBuf<Material> mtab;
+    DBuf<int> mat_orig;
+    // the mixed form's step as a graph (per state set)
+    bool use_graph = false;
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    double graph_dt[2] = {0.0, 0.0};
+    long long graph_steps = 0;
+    DBuf<double> stepv;
     DBuf<float> surr_feat;
     DBuf<unsigned long long> surr_stats;
     DBuf<double> plog;
@@ -245,6 +261,8 @@ static void prof_collect(sps_handle *h) {
 #define LAUNCH(KID, GRID, BLK, KERN, ...) LAUNCH_LDS(KID, GRID, BLK, 0, KERN, __VA_ARGS__)
 
 extern "C" const char *sps_last_error(void) { return g_err; }
+
+static void drop_graphs(sps_handle *h);
 
 #include "sps_surrogate_host.hip"
 extern "C" const char *sps_backend(void) { return "hip-gfx950"; }
@@ -374,7 +392,7 @@ extern "C" int sps_create(const sps_params *params, sps_handle **out) {
     }
     rc |= dalloc(h->deg_a, nt); rc |= dalloc(h->deg_b, nn);
     rc |= dalloc(h->tile_ok, h->ntile_n + h->ntile_s + 1);
-    rc |= dalloc(h->x_build, nt); rc |= dalloc(h->dmax, 1);
+    rc |= dalloc(h->x_build, nt); rc |= dalloc(h->dmax, 1); rc |= dalloc(h->stepv, 4);
     if (rc) return bail(rc);
     if (hipHostMalloc((void **)&h->pinned, 64 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
         return bail(fail(SPS_ENOMEM, "pinned staging allocation"));
@@ -468,6 +486,12 @@ extern "C" int sps_set_state(sps_handle *h, const sps_state *st) {
     HIP_TRY(hipStreamSynchronize(s));
     h->have_state = true;
     h->lists_valid = false;
+    drop_graphs(h);
+    if (h->mtab.p) {   // the slots are in original order again: their materials too
+        StateSet &C = h->S[h->cur];   // (the other set gets them from k_permute)
+        LAUNCH(K_PERMUTE, nblk(h->nt2), BLOCK, k_material_slots, C.orig.p, h->mat_orig.p, C.mat.p, h->nt2);
+        HIP_TRY(hipStreamSynchronize(h->stream));
+    }
     h->poisoned = false;
     return 0;
 }
@@ -573,6 +597,102 @@ static int lists_still_valid(sps_handle *h, bool &valid) {
     return 0;
 }
 
+// The mixed form's step after the search: every launch, no host
+// synchronisation and no memset, so that it can be captured in a graph.
+// stepv: the scalars of a replayed step (k_set_step), or null.
+static int mixed_step_kernels(sps_handle *h, double dt, double time_sph, double gxg, double gyg,
+                              const double *stepv) {
+    const int nn = h->nn, nt = h->nt, nt2 = h->nt2;
+    StateBufs SB = h->S[h->cur].view();
+    Work W{h->vx0.p, h->x0.p, h->vel0.p, h->rk_vel.p, h->rhs2.p, h->f_bound.p, h->art_visc.p,
+           h->stress0.p, h->rk_stress.p, h->rhs1.p, h->grad_u.p, h->rk_dev.p, h->ddev.p};
+    const SearchView V{h->cell.p, h->kstart.p, h->proc.p, h->cnt.p, h->off.p, h->grid.p, h->split.p, h->p.pi};
+    const PairLists PL = h->lists();
+    const int adapt = h->dp.ncrit == 12;
+    const double tanfi = h->dp.tanfi, coh = h->dp.coh;
+    const double f1rk[4] = {0., 0.5, 0.5, 1.0}, f2rk[4] = {1., 2., 2., 1.0};
+    const float hu = (float)h->h_uniform;
+    LAUNCH(K_RK_INIT, nblk(nt2), BLOCK, k_rk_init, SB, W, nn, nt, nt2);
+    for (int st = 0; st < 4; ++st) {
+        const double c1 = f1rk[st] * (dt);
+        LAUNCH(K_STAGE_BEGIN, nblk(nt), BLOCK, k_stage_begin, SB, W, c1, adapt, tanfi, coh, nn, nt);
+        if (nn > 0)
+            LAUNCH_LDS(K_SPU_NODES, nblk8(nn), TILE, LAY_SPU_NODES_MIX.bytes, k_spu_nodes_mix, SB, V, PL, adapt,
+                       tanfi, coh, hu, nn);
+        if (nt > nn)
+            LAUNCH_LDS(K_SP_UPDATE, nblk8(nt - nn), TILE, LAY_SP_UPDATE_MIX.bytes, k_sp_update_mix, SB, W, V, PL,
+                       h->dp, time_sph, f2rk[st], 1, adapt, hu, nn, nt, stepv);
+        if (nn > 0)
+            LAUNCH_LDS(K_DERIV_NODES, nblk8(nn), TILE, LAY_DERIV_NODES_MIX.bytes, k_deriv_nodes_mix, SB, W, V, PL,
+                       h->dp, gxg, gyg, f2rk[st], h->err.p, hu, nn, stepv);
+        if (st == 3 && nt2 > nt)
+            LAUNCH(K_DUMMY_STRESS, nblk(nt2 - nt), BLOCK, k_dummy_stress, SB, V, nt, nt2);
+    }
+    const bool hook = h->mlp || h->log_cap > 0;
+    if (h->mlp) LAUNCH(K_SURROGATE, 1, 64, k_zero_u32, h->surr_count.p, 1);
+    LAUNCH(K_RK_FINAL, nblk(nt), BLOCK, k_rk_final, SB, W, dt, adapt, tanfi, coh, nn, nt, surrogate_dev(h),
+           hook ? 1 : 0);
+    if (h->mlp && adapt) {
+        int rc = surrogate_correct(h, SB);
+        if (rc) return rc;
+    }
+    if (nn > 0)
+        LAUNCH_LDS(K_SPU_NODES, nblk8(nn), TILE, LAY_SPU_NODES_MIX.bytes, k_spu_nodes_mix, SB, V, PL, adapt, tanfi,
+                   coh, hu, nn);
+    if (nt > nn)
+        LAUNCH_LDS(K_SP_UPDATE, nblk8(nt - nn), TILE, LAY_SP_UPDATE_MIX.bytes, k_sp_update_mix, SB, W, V, PL, h->dp,
+                   time_sph, 0.0, 0, adapt, hu, nn, nt, stepv);
+    LAUNCH(K_POSITION, nblk(nt), BLOCK, k_position, SB, W, dt, h->p.update_x, nn, nt);
+    if (h->skin > 0) {
+        LAUNCH(K_POSITION, 1, 64, k_zero_u64, h->dmax.p, 1);
+        LAUNCH(K_POSITION, std::min(1024u, nblk(nt)), BLOCK, k_max_disp, SB.x, h->x_build.p, nt, h->dmax.p);
+    }
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+static void drop_graphs(sps_handle *h) {
+    for (int k = 0; k < 2; ++k)
+        if (h->gexec[k]) {
+            (void)hipGraphExecDestroy(h->gexec[k]);
+            h->gexec[k] = nullptr;
+        }
+}
+
+// The mixed form's step as a replayed graph (one per state set, captured on
+// first use and whenever dt changes).
+static int mixed_step_graph(sps_handle *h, double dt, double time_sph, double gxg, double gyg) {
+    hipStream_t s = h->stream;
+    const int c = h->cur;
+    if (h->gexec[c] && h->graph_dt[c] != dt) {
+        (void)hipGraphExecDestroy(h->gexec[c]);
+        h->gexec[c] = nullptr;
+    }
+    if (!h->gexec[c]) {
+        hipGraph_t g = nullptr;
+        HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        int rc = mixed_step_kernels(h, dt, 0.0, 0.0, 0.0, h->stepv.p);
+        hipError_t e = hipStreamEndCapture(s, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(SPS_EDEVICE, "step capture failed: %s", hipGetErrorString(e));
+        e = hipGraphInstantiate(&h->gexec[c], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            h->gexec[c] = nullptr;
+            return fail(SPS_EDEVICE, "step graph instantiation failed: %s", hipGetErrorString(e));
+        }
+        h->graph_dt[c] = dt;
+    }
+    k_set_step<<<1, 64, 0, s>>>(h->stepv.p, time_sph, gxg, gyg);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipGraphLaunch(h->gexec[c], s));
+    ++h->graph_steps;
+    return 0;
+}
+
 static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
     bool reuse;
     int rc = lists_still_valid(h, reuse);
@@ -589,6 +709,20 @@ static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
             h->lists_valid = true;
         }
     }
+    const double factg = gravity_factor(&h->p, time_sph);
+    const double gxg = h->p.ic_grav ? factg * h->p.cgrav[0] : 0.0;
+    const double gyg = h->p.ic_grav ? factg * h->p.cgrav[1] : 0.0;
+    if (h->mixed) {
+        // replayed as a graph when enabled and no kernel timing is asked for
+        rc = (h->use_graph && !h->prof) ? mixed_step_graph(h, dt, time_sph, gxg, gyg)
+                                        : mixed_step_kernels(h, dt, time_sph, gxg, gyg, nullptr);
+        if (rc) return rc;
+        if (h->skin > 0) {
+            HIP_TRY(hipMemcpyAsync(h->pinned + 32, h->dmax.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipEventRecord(h->ev_disp, s));
+        }
+        return 0;
+    }
     StateBufs SB = h->S[h->cur].view();
     Work W{h->vx0.p, h->x0.p, h->vel0.p, h->rk_vel.p, h->rhs2.p, h->f_bound.p, h->art_visc.p,
            h->stress0.p, h->rk_stress.p, h->rhs1.p, h->grad_u.p, h->rk_dev.p, h->ddev.p};
@@ -597,28 +731,10 @@ static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
     const int adapt = h->dp.ncrit == 12;
     const double tanfi = h->dp.tanfi, coh = h->dp.coh;
     const double f1rk[4] = {0., 0.5, 0.5, 1.0}, f2rk[4] = {1., 2., 2., 1.0};
-    const double factg = gravity_factor(&h->p, time_sph);
-    const double gxg = h->p.ic_grav ? factg * h->p.cgrav[0] : 0.0;
-    const double gyg = h->p.ic_grav ? factg * h->p.cgrav[1] : 0.0;
     LAUNCH(K_RK_INIT, nblk(nt2), BLOCK, k_rk_init, SB, W, nn, nt, nt2);
-    const float hu = (float)h->h_uniform;
     for (int st = 0; st < 4; ++st) {
         const double c1 = f1rk[st] * (dt);
         LAUNCH(K_STAGE_BEGIN, nblk(nt), BLOCK, k_stage_begin, SB, W, c1, adapt, tanfi, coh, nn, nt);
-        if (h->mixed) {
-            if (nn > 0)
-                LAUNCH_LDS(K_SPU_NODES, nblk8(nn), TILE, LAY_SPU_NODES_MIX.bytes, k_spu_nodes_mix, SB, V, PL, adapt,
-                           tanfi, coh, hu, nn);
-            if (nt > nn)
-                LAUNCH_LDS(K_SP_UPDATE, nblk8(nt - nn), TILE, LAY_SP_UPDATE_MIX.bytes, k_sp_update_mix, SB, W, V, PL,
-                           h->dp, time_sph, f2rk[st], 1, adapt, hu, nn, nt);
-            if (nn > 0)
-                LAUNCH_LDS(K_DERIV_NODES, nblk8(nn), TILE, LAY_DERIV_NODES_MIX.bytes, k_deriv_nodes_mix, SB, W, V, PL,
-                           h->dp, gxg, gyg, f2rk[st], h->err.p, hu, nn);
-            if (st == 3 && nt2 > nt)
-                LAUNCH(K_DUMMY_STRESS, nblk(nt2 - nt), BLOCK, k_dummy_stress, SB, V, nt, nt2);
-            continue;
-        }
         if (nn > 0)
             LAUNCH_LDS(K_SPU_NODES, nblk8(nn), TILE, LAY_SPU_NODES.bytes, k_spu_nodes, SB, V, PL, adapt, tanfi, coh, nn);
         if (nt > nn)
@@ -630,31 +746,9 @@ static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
         if (st == 3 && nt2 > nt)
             LAUNCH(K_DUMMY_STRESS, nblk(nt2 - nt), BLOCK, k_dummy_stress, SB, V, nt, nt2);
     }
-    const bool hook = h->mlp || h->log_cap > 0;
-    if (h->mlp) HIP_TRY(hipMemsetAsync(h->surr_count.p, 0, sizeof(unsigned), s));
+    const bool hook = h->log_cap > 0;
     LAUNCH(K_RK_FINAL, nblk(nt), BLOCK, k_rk_final, SB, W, dt, adapt, tanfi, coh, nn, nt, surrogate_dev(h),
            hook ? 1 : 0);
-    if (h->mlp && adapt) {
-        rc = surrogate_correct(h, SB);
-        if (rc) return rc;
-    }
-    if (h->mixed) {
-        if (nn > 0)
-            LAUNCH_LDS(K_SPU_NODES, nblk8(nn), TILE, LAY_SPU_NODES_MIX.bytes, k_spu_nodes_mix, SB, V, PL, adapt, tanfi,
-                       coh, hu, nn);
-        if (nt > nn)
-            LAUNCH_LDS(K_SP_UPDATE, nblk8(nt - nn), TILE, LAY_SP_UPDATE_MIX.bytes, k_sp_update_mix, SB, W, V, PL, h->dp,
-                       time_sph, 0.0, 0, adapt, hu, nn, nt);
-        LAUNCH(K_POSITION, nblk(nt), BLOCK, k_position, SB, W, dt, h->p.update_x, nn, nt);
-        if (h->skin > 0) {
-            HIP_TRY(hipMemsetAsync(h->dmax.p, 0, sizeof(unsigned long long), s));
-            LAUNCH(K_POSITION, std::min(1024u, nblk(nt)), BLOCK, k_max_disp, SB.x, h->x_build.p, nt, h->dmax.p);
-            HIP_TRY(hipMemcpyAsync(h->pinned + 32, h->dmax.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
-            HIP_TRY(hipEventRecord(h->ev_disp, s));
-        }
-        HIP_TRY(hipGetLastError());
-        return 0;
-    }
     if (nn > 0)
         LAUNCH_LDS(K_SPU_NODES, nblk8(nn), TILE, LAY_SPU_NODES.bytes, k_spu_nodes, SB, V, PL, adapt, tanfi, coh, nn);
     if (nt > nn)
@@ -809,12 +903,47 @@ extern "C" int sps_get_stats(sps_handle *h, sps_stats *out) {
     return 0;
 }
 
+extern "C" int sps_set_materials(sps_handle *h, int32_t nmat, const double *table, const int32_t *mat) {
+    if (!h || nmat < 1 || !table || !mat) return fail(SPS_EINVAL, "bad argument");
+    if (!h->have_state) return fail(SPS_ESTATE, "sps_set_materials before sps_set_state");
+    for (int i = 0; i < h->nt2; ++i)
+        if (mat[i] < 0 || mat[i] >= nmat) return fail(SPS_EINVAL, "particle %d: material %d of %d", i + 1, mat[i], nmat);
+    HIP_TRY(hipSetDevice(h->device));
+    std::vector<Material> tab((size_t)nmat);
+    for (int k = 0; k < nmat; ++k) {
+        const double *t = table + 10 * k;
+        tab[k] = Material{t[0], t[1], t[2], t[3], t[4], t[5], t[6], t[7], t[8], t[9]};
+    }
+    int rc = dalloc(h->mtab, (size_t)nmat) | dalloc(h->mat_orig, (size_t)h->nt2) | h->S[0].mat_alloc(h->nt2) |
+             h->S[1].mat_alloc(h->nt2);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(h->mtab.p, tab.data(), (size_t)nmat * sizeof(Material), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(h->mat_orig.p, mat, (size_t)h->nt2 * sizeof(int), hipMemcpyHostToDevice));
+    // the slots' materials through the current order (the other state set gets
+    // them from k_permute, which copies them with the rest of the state)
+    h->S[0].mtab = h->S[1].mtab = h->mtab.p;
+    drop_graphs(h);
+    StateSet &C = h->S[h->cur];
+    LAUNCH(K_PERMUTE, nblk(h->nt2), BLOCK, k_material_slots, C.orig.p, h->mat_orig.p, C.mat.p, h->nt2);
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
 extern "C" int sps_set_list_skin(sps_handle *h, double skin) {
     if (!h) return fail(SPS_EINVAL, "null argument");
     if (!(skin >= 0.0) || (skin > 0.0 && !h->mixed))
         return fail(SPS_EINVAL, "a list skin (%g) needs SPS_MODE_FAST and must be >= 0", skin);
     h->skin = skin;
     h->lists_valid = false;
+    drop_graphs(h);
+    return 0;
+}
+
+extern "C" int sps_set_graph(sps_handle *h, int32_t enable) {
+    if (!h) return fail(SPS_EINVAL, "null argument");
+    if (enable && !h->mixed) return fail(SPS_EINVAL, "graph replay of a step needs SPS_MODE_FAST");
+    h->use_graph = enable != 0;
+    drop_graphs(h);
     return 0;
 }
 
@@ -861,6 +990,8 @@ extern "C" void sps_destroy(sps_handle *h) {
     prof_collect(h);
     for (auto e : h->ev_pool) (void)hipEventDestroy(e);
     if (h->ev_disp) (void)hipEventDestroy(h->ev_disp);
+    drop_graphs(h);
+    dfree(h->stepv);
     h->S[0].release();
     h->S[1].release();
     dfree(h->vx0); dfree(h->x0); dfree(h->vel0); dfree(h->rk_vel); dfree(h->rhs2); dfree(h->f_bound);
@@ -872,7 +1003,8 @@ extern "C" void sps_destroy(sps_handle *h) {
     dfree(h->list_ia); dfree(h->list_wa); dfree(h->list_gxa); dfree(h->list_gya); dfree(h->list_ib);
     dfree(h->list_gxb); dfree(h->list_gyb); dfree(h->deg_a); dfree(h->deg_b); dfree(h->tile_ok);
     dfree(h->x_build); dfree(h->dmax); dfree(h->surr_count); dfree(h->surr_slot); dfree(h->surr_sn);
-    dfree(h->surr_str); dfree(h->surr_feat); dfree(h->surr_stats); dfree(h->plog);
+    dfree(h->surr_str); dfree(h->surr_feat); dfree(h->surr_stats); dfree(h->plog); dfree(h->surr_mat);
+    dfree(h->mtab); dfree(h->mat_orig);
     if (h->pinned) (void)hipHostFree(h->pinned);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;

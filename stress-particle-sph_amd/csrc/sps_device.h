@@ -26,6 +26,13 @@ struct DevParams {
     double cgrav[2];
 };
 
+// One material's constants (an ensemble of instances has one per instance;
+// the reference has one, props(1,:)): tan(phi), c (props(1,13:14)), E, nu
+// (props(1,3:4)) and the elastic matrix (3_...:719-730).
+struct Material {
+    double tanfi, coh, young, poiss, D11, D22, D12, D33, D41, D42;
+};
+
 // Kind of a storage slot: the storage keeps the reference's index ranges
 // (nodes, then stress points, then dummies), only reordered inside each.
 enum { KIND_NODE = 0, KIND_SP = 1, KIND_DUMMY = 2 };
@@ -337,6 +344,7 @@ struct SurrogateDev {
     int cap;
     int *slot;               // their storage slots
     double4 *s_n, *s_tr;     // stress before the step and trial stress
+    double2 *mat;            // their tan(phi), c
     float *feat;             // their features (cap x 10)
     float stress_scale, coh_scale;
     unsigned long long *stats;   // [0] evaluated, [1] kept
@@ -364,6 +372,7 @@ __device__ __forceinline__ void surrogate_record(const SurrogateDev &G, int i, c
     const unsigned k = atomicAdd(G.count, 1u);
     if ((int)k >= G.cap) return;          // beyond the batch: the exact result stays
     G.slot[k] = i;
+    G.mat[k] = make_double2(tanfi, coh);
     G.s_n[k] = s_n;
     G.s_tr[k] = s_tr;
     const float is = 1.0f / G.stress_scale;

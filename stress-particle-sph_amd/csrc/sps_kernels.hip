@@ -285,7 +285,29 @@ struct StateBufs {
     double *mass, *rho, *hsml, *ivar1, *fdr;
     int *orig, *ood;
     float *wallpos, *horiz;
+    int *mat;                  // material of each slot (null: every particle has DevParams')
+    const Material *mtab;      // the materials
 };
+
+// The friction / cohesion (and, for material_params, every material
+// constant) of slot i: its own material's where a table is set, else the
+// DevParams values, unchanged.
+__device__ __forceinline__ void material_tc(const StateBufs &S, int i, double &tanfi, double &coh) {
+    if (S.mat) {
+        const Material &m = S.mtab[S.mat[i]];
+        tanfi = m.tanfi;
+        coh = m.coh;
+    }
+}
+__device__ __forceinline__ DevParams material_params(const DevParams &P, const StateBufs &S, int i) {
+    DevParams q = P;
+    if (S.mat) {
+        const Material &m = S.mtab[S.mat[i]];
+        q.tanfi = m.tanfi; q.coh = m.coh; q.young = m.young; q.poiss = m.poiss;
+        q.D11 = m.D11; q.D22 = m.D22; q.D12 = m.D12; q.D33 = m.D33; q.D41 = m.D41; q.D42 = m.D42;
+    }
+    return q;
+}
 
 __global__ void k_permute(StateBufs dst, StateBufs src, const int *__restrict__ perm, int nn,
                           int nt, int nt2) {
@@ -308,6 +330,14 @@ __global__ void k_permute(StateBufs dst, StateBufs src, const int *__restrict__ 
         dst.ivar1[t] = src.ivar1[s];
         dst.fdr[t] = src.fdr[s];
     }
+    if (src.mat) dst.mat[t] = src.mat[s];
+}
+
+// Materials of the slots from the materials of the original particles.
+__global__ void k_material_slots(const int *__restrict__ orig, const int *__restrict__ mat_orig,
+                                 int *__restrict__ mat, int nt2) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nt2) mat[t] = mat_orig[orig[t]];
 }
 
 // --------------------------------------------------------- stencil helpers
@@ -990,6 +1020,7 @@ __global__ void k_stage_begin(StateBufs S, Work W, double c1, int adapt, double 
                               int nn, int nt) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nt) return;
+    material_tc(S, i, tanfi, coh);
     if (i < nn) {
         double2 v0 = W.vel0[i], r2 = W.rhs2[i];
         S.vel[i] = make_double2(v0.x + c1 * r2.x, v0.y + c1 * r2.y);
@@ -1414,6 +1445,7 @@ __global__ __launch_bounds__(TILE) void k_spu_nodes(StateBufs S, SearchView V, P
     } else {
         S.vel[r] = make_double2(0, 0);
     }
+    material_tc(S, r, tanfi, coh);
     if (adapt) adapt_stress(s, tanfi, coh);
     S.stress[r] = s;
 }
@@ -1565,8 +1597,9 @@ __global__ __launch_bounds__(TILE) void k_sp_update(StateBufs S, Work W, SearchV
         S.vel[r] = vr;
     }
     double4 st = S.stress[r];
+    const DevParams Pm = material_params(P, S, r);
     if (adapt) {
-        adapt_stress(st, P.tanfi, P.coh);
+        adapt_stress(st, Pm.tanfi, Pm.coh);
         S.stress[r] = st;
     }
     if (!deriv) return;
@@ -1626,7 +1659,7 @@ __global__ __launch_bounds__(TILE) void k_sp_update(StateBufs S, Work W, SearchV
             walk_stencil<MASK_NODE | MASK_DUMMY>(GlobalCells{V.kstart, C.g, C.L}, GlobalFields{S}, C.g, cx, cy,
                                                  xr, hr, KIND_SP, r, rev_g, C.any_fwd, V.pi, cls, grad);
     }
-    sp_stage_tail(P, S, W, r, st, g00, g01, g10, g11, A0, A1, A2, A3, time_sph, f2);
+    sp_stage_tail(Pm, S, W, r, st, g00, g01, g10, g11, A0, A1, A2, A3, time_sph, f2);
 }
 
 // boundary_forces for one node: the nonzero terms of its inner-layer dummies
@@ -1918,6 +1951,7 @@ __global__ void k_rk_final(StateBufs S, Work W, double dt, int adapt, double tan
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nt) return;
     double c = dt / 6;
+    material_tc(S, i, tanfi, coh);
     if (i < nn) {
         double2 v0 = W.vel0[i], rk = W.rk_vel[i];
         S.vel[i] = make_double2(v0.x + c * rk.x, v0.y + c * rk.y);

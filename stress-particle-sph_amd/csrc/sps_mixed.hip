@@ -239,14 +239,18 @@ __global__ __launch_bounds__(TILE) void k_spu_nodes_mix(StateBufs S, SearchView 
     } else {
         S.vel[r] = make_double2(0, 0);
     }
+    material_tc(S, r, tanfi, coh);
     if (adapt) adapt_stress(s, tanfi, coh);
     S.stress[r] = s;
 }
 
 // Stress-point receivers (as k_sp_update).
+// stepv (a captured step): time_sph from stepv[0] instead of the argument.
 __global__ __launch_bounds__(TILE) void k_sp_update_mix(StateBufs S, Work W, SearchView V, PairLists PL,
                                                         DevParams P, double time_sph, double f2, int deriv,
-                                                        int adapt, float hu, int nn, int nt) {
+                                                        int adapt, float hu, int nn, int nt,
+                                                        const double *__restrict__ stepv) {
+    if (stepv) time_sph = stepv[0];
     __shared__ StagePlan pl[2];
     constexpr const LdsLayout &LY = LAY_SP_UPDATE_MIX;
     const WalkCtx C = walk_ctx(V);
@@ -331,8 +335,9 @@ __global__ __launch_bounds__(TILE) void k_sp_update_mix(StateBufs S, Work W, Sea
         S.vel[r] = vr;
     }
     double4 st = S.stress[r];
+    const DevParams Pm = material_params(P, S, r);
     if (adapt) {
-        adapt_stress(st, P.tanfi, P.coh);
+        adapt_stress(st, Pm.tanfi, Pm.coh);
         S.stress[r] = st;
     }
     if (!deriv) return;
@@ -391,14 +396,20 @@ __global__ __launch_bounds__(TILE) void k_sp_update_mix(StateBufs S, Work W, Sea
             walk_stencil<MASK_NODE | MASK_DUMMY>(GlobalCells{V.kstart, C.g, C.L}, GF, C.g, cx, cy, xr, hr, KIND_SP,
                                                  r, false, true, V.pi, NoClass{}, grad);
     }
-    sp_stage_tail(P, S, W, r, st, g00.get(), g01.get(), g10.get(), g11.get(), A0.get(), A1.get(), A2.get(),
+    sp_stage_tail(Pm, S, W, r, st, g00.get(), g01.get(), g10.get(), g11.get(), A0.get(), A1.get(), A2.get(),
                   A3.get(), time_sph, f2);
 }
 
 // Node receivers (as k_deriv_nodes).
+// stepv (a captured step): the gravity terms from stepv[1:3].
 __global__ __launch_bounds__(TILE) void k_deriv_nodes_mix(StateBufs S, Work W, SearchView V, PairLists PL,
                                                           DevParams P, double gx_grav, double gy_grav, double f2,
-                                                          int *__restrict__ err, float hu, int nn) {
+                                                          int *__restrict__ err, float hu, int nn,
+                                                          const double *__restrict__ stepv) {
+    if (stepv) {
+        gx_grav = stepv[1];
+        gy_grav = stepv[2];
+    }
     __shared__ StagePlan pl[2];
     constexpr const LdsLayout &LY = LAY_DERIV_NODES_MIX;
     const WalkCtx C = walk_ctx(V);
@@ -561,6 +572,28 @@ __global__ __launch_bounds__(TILE) void k_deriv_nodes_mix(StateBufs S, Work W, S
     node_stage_tail(S, W, V, C, P, r, cx, cy, xr, vr, hr, q00.get(), q01.get(), q10.get(), q11.get(), q20.get(),
                     q21.get(), A0.get(), A1.get(), A2.get(), A3.get(), ndum, (float)ax.get(), (float)ay.get(),
                     gx_grav, gy_grav, f2, err);
+}
+
+// ------------------------------------------------------ captured steps
+// The step's changing scalars for a replayed graph: time_sph and the gravity
+// terms (a launch outside the graph, before each replay).
+__global__ void k_set_step(double *__restrict__ stepv, double time_sph, double gx, double gy) {
+    if (threadIdx.x == 0) {
+        stepv[0] = time_sph;
+        stepv[1] = gx;
+        stepv[2] = gy;
+    }
+}
+
+// Clears inside a captured step are kernels (a memset recorded into a graph
+// is avoided).
+__global__ void k_zero_u32(unsigned *__restrict__ p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+__global__ void k_zero_u64(unsigned long long *__restrict__ p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
 }
 
 }  // namespace sps

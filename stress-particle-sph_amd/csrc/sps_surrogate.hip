@@ -233,13 +233,14 @@ __device__ __forceinline__ double dp_yield(const double4 &s, double alpha2, doub
 
 // The network's corrections of the recorded states, kept where consistent.
 __global__ void k_surrogate_apply(SurrogateDev G, const float *__restrict__ Y, double4 *__restrict__ stress,
-                                  double tanfi, double coh, double tol) {
+                                  double tol) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = min((int)*G.count, G.cap);
     bool kept = false;
     if (k < n) {
         const double sc = G.stress_scale;
         const double4 t = G.s_tr[k];
+        const double tanfi = G.mat[k].x, coh = G.mat[k].y;
         const float *y = Y + (long long)k * MLP_OUT;
         const double alpha2 = tanfi / (sqrt(9 + 12 * (tanfi * tanfi)));
         const double kc = (3 * coh) / (sqrt(9 + 12 * (tanfi * tanfi)));

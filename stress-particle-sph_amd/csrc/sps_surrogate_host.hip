@@ -336,6 +336,7 @@ static SurrogateDev surrogate_dev(sps_handle *h) {
     G.slot = h->surr_slot.p;
     G.s_n = h->surr_sn.p;
     G.s_tr = h->surr_str.p;
+    G.mat = h->surr_mat.p;
     G.feat = h->surr_feat.p;
     G.stress_scale = (float)h->surr_S;
     G.coh_scale = (float)h->surr_Sc;
@@ -356,8 +357,7 @@ static int surrogate_correct(sps_handle *h, const StateBufs &SB) {
     MLAUNCH((unsigned)((rows + 255) / 256), 256, k_mlp_batch, h->surr_feat.p, nullptr, nullptr, cap, rows, ml->X.p,
             nullptr, h->surr_count.p);
     mlp_forward(ml, st, h, rows, (const int *)h->surr_count.p, false);
-    MLAUNCH((unsigned)((cap + 255) / 256), 256, k_surrogate_apply, G, ml->Y.p, SB.stress, h->dp.tanfi, h->dp.coh,
-            h->surr_tol);
+    MLAUNCH((unsigned)((cap + 255) / 256), 256, k_surrogate_apply, G, ml->Y.p, SB.stress, h->surr_tol);
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -370,11 +370,13 @@ extern "C" int sps_set_surrogate(sps_handle *h, sps_mlp *ml, double stress_scale
     if (ml && ml->device != h->device) return fail(SPS_EINVAL, "network and handle on different devices");
     HIP_TRY(hipSetDevice(h->device));
     h->mlp = ml;
+    drop_graphs(h);
     if (!ml) return 0;
     const int cap = h->ns > 0 ? h->ns : 1;
     int rc = mlp_reserve(ml, cap);
     rc |= dalloc(h->surr_count, 2) | dalloc(h->surr_slot, (size_t)cap) | dalloc(h->surr_sn, (size_t)cap) |
-          dalloc(h->surr_str, (size_t)cap) | dalloc(h->surr_feat, (size_t)cap * MLP_NIN) | dalloc(h->surr_stats, 2);
+          dalloc(h->surr_str, (size_t)cap) | dalloc(h->surr_feat, (size_t)cap * MLP_NIN) | dalloc(h->surr_stats, 2) |
+          dalloc(h->surr_mat, (size_t)cap);
     if (rc) return rc;
     h->surr_cap = cap;
     h->surr_S = stress_scale;

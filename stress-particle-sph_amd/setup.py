@@ -62,6 +62,46 @@ def elastic_matrix(young, poiss):
     return dict(D11=D11, D22=D11, D12=D12, D33=G_mod, D41=D12, D42=D12)
 
 
+def wall_dummies(walls, dx, dy):
+    """Dummy particles of the walls (set_up_dummy_nodes, 3_...:736-878) in
+    default-real arithmetic: three rows per wall, the innermost rows of all
+    walls first.  Returns x, y, horizontal_or_not, wall_position (float32)
+    and the innermost-row count ndummy2."""
+    nrow = 3
+    dx2, dy2 = F32(dx), F32(dy)
+    ndiv_wall = [int(F32((w.x2 - w.x1) / dx2 + F32(1))) for w in walls]
+    ndummy2 = sum(ndiv_wall)
+    dxs, dys, hor, wpos = [], [], [], []
+
+    def layer(w, nd, m):
+        j = np.arange(nd, dtype=np.float32)
+        if w.wall_id in (1, 3):
+            xs_ = np.full(nd, F32(w.position - F32(m - 1) * dx2), dtype=np.float32) if m > 1 else \
+                np.full(nd, w.position, dtype=np.float32)
+            ys_ = (w.x1 + j * dy2).astype(np.float32)
+            h = 2.0 if w.wall_id == 1 else 22.0
+        elif w.wall_id == 2:
+            xs_ = (w.x1 + j * dx2).astype(np.float32)
+            ys_ = np.full(nd, F32(w.position - F32(m - 1) * dy2), dtype=np.float32) if m > 1 else \
+                np.full(nd, w.position, dtype=np.float32)
+            h = 1.0
+        else:
+            raise DeckError(f"unknown wall id {w.wall_id}")
+        dxs.append(xs_); dys.append(ys_)
+        hor.append(np.full(nd, h, dtype=np.float32))
+        wpos.append(np.full(nd, w.position, dtype=np.float32))
+
+    for w, nd in zip(walls, ndiv_wall):
+        layer(w, nd, 1)
+    for w, nd in zip(walls, ndiv_wall):
+        for m in range(2, nrow + 1):
+            layer(w, nd, m)
+    if not dxs:
+        e = np.zeros(0, dtype=np.float32)
+        return e, e, e, e, 0
+    return np.concatenate(dxs), np.concatenate(dys), np.concatenate(hor), np.concatenate(wpos), ndummy2
+
+
 def build_particles(deck: Deck) -> Particles:
     if deck.geom_type != 1:
         raise DeckError("only geom_type 1 (rectangle) is implemented, as in the reference (3_...:388-390)")
@@ -100,36 +140,9 @@ def build_particles(deck: Deck) -> Particles:
 
     # dummy walls, default-real arithmetic (3_...:736-878)
     walls = deck.walls if deck.dummy_nodes else []
-    nrow = 3
-    dx2, dy2 = F32(dx), F32(dy)
-    ndiv_wall = [int(F32((w.x2 - w.x1) / dx2 + F32(1))) for w in walls]
-    ndummy2 = sum(ndiv_wall)
-    ndummy = nrow * ndummy2
-    dxs, dys, hor, wpos = [], [], [], []
-
-    def layer(w, nd, m):
-        j = np.arange(nd, dtype=np.float32)
-        if w.wall_id in (1, 3):
-            xs_ = np.full(nd, F32(w.position - F32(m - 1) * dx2), dtype=np.float32) if m > 1 else \
-                np.full(nd, w.position, dtype=np.float32)
-            ys_ = (w.x1 + j * dy2).astype(np.float32)
-            h = 2.0 if w.wall_id == 1 else 22.0
-        elif w.wall_id == 2:
-            xs_ = (w.x1 + j * dx2).astype(np.float32)
-            ys_ = np.full(nd, F32(w.position - F32(m - 1) * dy2), dtype=np.float32) if m > 1 else \
-                np.full(nd, w.position, dtype=np.float32)
-            h = 1.0
-        else:
-            raise DeckError(f"unknown wall id {w.wall_id}")
-        dxs.append(xs_); dys.append(ys_)
-        hor.append(np.full(nd, h, dtype=np.float32))
-        wpos.append(np.full(nd, w.position, dtype=np.float32))
-
-    for w, nd in zip(walls, ndiv_wall):
-        layer(w, nd, 1)
-    for w, nd in zip(walls, ndiv_wall):
-        for m in range(2, nrow + 1):
-            layer(w, nd, m)
+    wx, wy, hor_all, wpos_all, ndummy2 = wall_dummies(walls, dx, dy)
+    ndummy = len(wx)
+    dx2 = F32(dx)
 
     ntotal = nnode + nstress
     ntotal2 = ntotal + ndummy
@@ -137,8 +150,8 @@ def build_particles(deck: Deck) -> Particles:
     x[:nnode, 0], x[:nnode, 1] = xn, yn
     x[nnode:ntotal, 0], x[nnode:ntotal, 1] = xs, ys
     if ndummy:
-        x[ntotal:, 0] = np.concatenate(dxs).astype(np.float64)
-        x[ntotal:, 1] = np.concatenate(dys).astype(np.float64)
+        x[ntotal:, 0] = wx.astype(np.float64)
+        x[ntotal:, 1] = wy.astype(np.float64)
     mass = np.empty(ntotal2)
     mass[:nnode] = mass_n
     mass[nnode:ntotal] = mass_s
@@ -150,8 +163,8 @@ def build_particles(deck: Deck) -> Particles:
     wall_position = np.zeros(ntotal2, dtype=np.float32)
     horizontal_or_not = np.zeros(ntotal2, dtype=np.float32)
     if ndummy:
-        wall_position[ntotal:] = np.concatenate(wpos)
-        horizontal_or_not[ntotal:] = np.concatenate(hor)
+        wall_position[ntotal:] = wpos_all
+        horizontal_or_not[ntotal:] = hor_all
     return Particles(nnode=nnode, nstress=nstress, ndummy=ndummy, ndummy2=ndummy2,
                      ndivx=ndivx, ndivy=ndivy, x=x, mass=mass, rho=rho, hsml=hsml,
                      itype=itype, wall_position=wall_position,

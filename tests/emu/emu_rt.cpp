@@ -30,6 +30,39 @@ bool needs_sync(const char *name) {
 unsigned char *lds_base = nullptr;
 size_t lds_bytes = 0;
 
+bool capturing = false;
+struct Launch {
+    dim3 grid;
+    unsigned block;
+    size_t lds;
+    std::function<void()> body;
+    bool sync;
+};
+struct Graph {
+    std::vector<Launch> launches;
+};
+static Graph *recording = nullptr;
+
+void check_not_capturing(const char *what) {
+    if (capturing) {
+        fprintf(stderr, "emu: %s while capturing a graph\n", what);
+        abort();
+    }
+}
+
+Graph *end_capture() {
+    capturing = false;
+    Graph *g = recording ? recording : new Graph();
+    recording = nullptr;
+    return g;
+}
+
+void replay(const Graph *g) {
+    for (const Launch &l : g->launches) launch(l.grid, l.block, l.lds, l.body, l.sync);
+}
+
+void drop(Graph *g) { delete g; }
+
 namespace {
 enum State { RUN, AT_BLOCK, AT_WAVE, DONE };
 struct Fiber {
@@ -123,6 +156,11 @@ void wave_put(const void *v, int bytes) {
 void wave_get(int lane, void *v, int bytes) { memcpy(v, wave_buf[threadIdx.x >> 6][lane], bytes); }
 
 void launch(dim3 grid, unsigned block, size_t lds, const std::function<void()> &body, bool sync) {
+    if (capturing) {
+        if (!recording) recording = new Graph();
+        recording->launches.push_back(Launch{grid, block, lds, body, sync});
+        return;
+    }
     if (block > MAX_BLOCK) {
         fprintf(stderr, "emu: block of %u threads\n", block);
         abort();

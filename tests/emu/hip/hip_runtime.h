@@ -60,8 +60,17 @@ void launch(dim3 grid, unsigned block, size_t lds, const std::function<void()> &
 // evaluated once, on the host, and every thread gets the same copies.
 template <typename... P, typename... A>
 void launch_k(dim3 grid, unsigned block, size_t lds, bool sync, void (*kern)(P...), A... args) {
-    launch(grid, block, lds, [&] { kern(args...); }, sync);
+    launch(grid, block, lds, [=] { kern(args...); }, sync);
 }
+// Stream capture: launches are recorded (with their argument values) instead
+// of run, and replayed by hipGraphLaunch; a memset or copy while capturing
+// aborts (none is meant to be recorded).
+extern bool capturing;
+void check_not_capturing(const char *what);
+struct Graph;
+Graph *end_capture();
+void replay(const Graph *g);
+void drop(Graph *g);
 extern unsigned char *lds_base;   // the launch's dynamic LDS
 extern size_t lds_bytes;
 }  // namespace emu
@@ -155,9 +164,45 @@ inline hipError_t hipMalloc(void **p, size_t n) { *p = malloc(n ? n : 1); return
 inline hipError_t hipFree(void *p) { free(p); return hipSuccess; }
 inline hipError_t hipHostMalloc(void **p, size_t n, unsigned) { *p = malloc(n); return hipSuccess; }
 inline hipError_t hipHostFree(void *p) { free(p); return hipSuccess; }
-inline hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, hipMemcpyKind, hipStream_t) { memmove(d, s, n); return hipSuccess; }
-inline hipError_t hipMemcpy(void *d, const void *s, size_t n, hipMemcpyKind) { memmove(d, s, n); return hipSuccess; }
-inline hipError_t hipMemsetAsync(void *d, int v, size_t n, hipStream_t) { memset(d, v, n); return hipSuccess; }
+inline hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, hipMemcpyKind, hipStream_t) {
+    emu::check_not_capturing("hipMemcpyAsync");
+    memmove(d, s, n);
+    return hipSuccess;
+}
+inline hipError_t hipMemcpy(void *d, const void *s, size_t n, hipMemcpyKind) {
+    emu::check_not_capturing("hipMemcpy");
+    memmove(d, s, n);
+    return hipSuccess;
+}
+typedef emu::Graph *hipGraph_t;
+typedef emu::Graph *hipGraphExec_t;
+enum hipStreamCaptureMode { hipStreamCaptureModeGlobal, hipStreamCaptureModeThreadLocal, hipStreamCaptureModeRelaxed };
+inline hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) {
+    emu::capturing = true;
+    return hipSuccess;
+}
+inline hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t *g) {
+    *g = emu::end_capture();
+    return hipSuccess;
+}
+inline hipError_t hipGraphInstantiate(hipGraphExec_t *e, hipGraph_t g, void *, void *, size_t) {
+    *e = g;   // the recorded launches themselves; hipGraphDestroy of g keeps them
+    return hipSuccess;
+}
+inline hipError_t hipGraphDestroy(hipGraph_t) { return hipSuccess; }
+inline hipError_t hipGraphExecDestroy(hipGraphExec_t e) {
+    emu::drop(e);
+    return hipSuccess;
+}
+inline hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t) {
+    emu::replay(e);
+    return hipSuccess;
+}
+inline hipError_t hipMemsetAsync(void *d, int v, size_t n, hipStream_t) {
+    emu::check_not_capturing("hipMemsetAsync");
+    memset(d, v, n);
+    return hipSuccess;
+}
 inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
 inline hipError_t hipGetLastError() { return hipSuccess; }
 inline hipError_t hipSetDevice(int) { return hipSuccess; }

@@ -23,6 +23,19 @@ def main(emu_lib, oracle_lib, case, steps, tol="1e-4"):
         from conftest import open_varied_h
         emu = open_varied_h(sps, deck, emu_lib)
         ora = open_varied_h(sps, deck, oracle_lib)
+    elif variant == "graph":    # mixed form with list reuse: graph replay equals direct launches bit for bit
+        sims = []
+        for graph in (False, True):
+            sim = sps.Simulation.from_deck(deck, lib_path=emu_lib, mode=sps.MODE_FAST)
+            sim.set_list_skin(0.25 * sim.params.dx)
+            sim.set_graph(graph)
+            sim.step(steps)
+            sims.append(sim)
+        a, b = sims[0].get_state(), sims[1].get_state()
+        bad = [k for k in a if not np.array_equal(a[k].view(np.uint8), b[k].view(np.uint8))]
+        print(f"{case} {steps} steps ({sims[1].search_count()} searches): " +
+              ("graph replay bitwise equal to direct launches" if not bad else "differ in " + ", ".join(bad)))
+        return 1 if bad else 0
     elif variant in ("mixed", "skin"):   # the mixed-precision form against the double-precision oracle
         emu = sps.Simulation.from_deck(deck, lib_path=emu_lib, mode=sps.MODE_FAST)
         if variant == "skin":           # lists kept over steps

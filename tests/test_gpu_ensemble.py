@@ -1,0 +1,119 @@
+"""The batched form: instances concatenated into one particle set, each with
+its own material, and the mixed form's step replayed as a graph.
+
+* An ensemble of slope instances (cohesive and cohesionless, vertical and
+  inclined) in the double-precision form equals the oracle running the same
+  ensemble (same per-particle materials) bit for bit, through yield.
+* Graph replay of the mixed form's list-reusing step equals direct launches
+  bit for bit (a deck, and an ensemble).
+* Each instance of a mixed-form ensemble stays within single-precision
+  rounding of the same instance run alone (the concatenation adds no
+  interaction).
+* The autotuner returns one of its candidates.
+"""
+import importlib
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import ORACLE_LIB, deck_dir, load_pkg
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def ensemble_mod():
+    load_pkg()
+    return importlib.import_module("stress-particle-sph_amd.ensemble")
+
+
+def small_instances(ens):
+    return [ens.Instance(6.0, 60.0, 20.0, 0.0, 1e7), ens.Instance(5.0, 45.0, 35.0, 500.0, 3e7),
+            ens.Instance(7.0, 90.0, 25.0, 0.0, 5e6), ens.Instance(5.5, 75.0, 30.0, 2e3, 2e7)]
+
+
+def bits_equal_states(a, b):
+    return [k for k in a if not np.array_equal(a[k].view(np.uint8), b[k].view(np.uint8))]
+
+
+def test_ensemble_double_form_equals_oracle():
+    sps = load_pkg()
+    ens = ensemble_mod()
+    base = sps.read_deck(deck_dir("c2_slope_dx01"))
+    inst = small_instances(ens)
+    gpu = ens.open_ensemble(sps, base, inst, dx=0.25, mode=sps.MODE_ORDERED)
+    ora = ens.open_ensemble(sps, base, inst, dx=0.25, mode=sps.MODE_ORDERED, lib_path=ORACLE_LIB)
+    say(f"ensemble of {len(inst)} instances, {gpu.n2} particles")
+    gpu.step(60)
+    ora.step(60)
+    a, b = gpu.get_state(), ora.get_state()
+    bad = bits_equal_states(a, b)
+    io = gpu.ensemble[1][:gpu.n]
+    plastic = [int(np.count_nonzero(b["internal_vars"][io == k, 0])) for k in range(len(inst))]
+    say(f"  plastic stress points per instance {plastic}")
+    assert not bad, bad
+    assert sum(plastic) > 0
+
+
+@pytest.mark.parametrize("which", ["deck", "ensemble"])
+def test_graph_replay_equals_direct_launches(which):
+    sps = load_pkg()
+    ens = ensemble_mod()
+    states = []
+    for graph in (False, True):
+        if which == "deck":
+            sim = sps.Simulation.from_deck(sps.read_deck(deck_dir("t2_cspm")), mode=sps.MODE_FAST)
+            sim.set_list_skin(0.25 * sim.params.dx)
+        else:
+            base = sps.read_deck(deck_dir("c2_slope_dx01"))
+            sim = ens.open_ensemble(sps, base, small_instances(ens), dx=0.25, skin=0.25 * 0.25)
+        sim.set_graph(graph)
+        sim.step(40)
+        states.append(sim.get_state())
+        say(f"  {which} graph={graph}: {sim.search_count()} searches")
+    assert not bits_equal_states(*states)
+
+
+def test_ensemble_instances_match_single_runs():
+    sps = load_pkg()
+    ens = ensemble_mod()
+    acc = importlib.import_module("stress-particle-sph_amd.accuracy")
+    base = sps.read_deck(deck_dir("c2_slope_dx01"))
+    inst = small_instances(ens)[:2]
+    steps = 30
+    batch = ens.open_ensemble(sps, base, inst, dx=0.25)
+    batch.step(steps)
+    B = batch.get_state()
+    _, inst_of, offs = batch.ensemble
+    for k in range(len(inst)):
+        one = ens.open_ensemble(sps, base, [inst[k]], dx=0.25)
+        x0 = one.get_state(["x"])["x"]
+        one.step(steps)
+        S = one.get_state()
+        # the instance's particles keep their order within each kind range
+        sel = np.nonzero(inst_of == k)[0]
+        sub = {name: B[name][sel[sel < len(B[name])]] for name in ("x", "vel", "stress", "internal_vars")}
+        sub["x"] = sub["x"] - np.array([offs[k], 0.0])
+        ref = {name: S[name] for name in ("x", "vel", "stress", "internal_vars")}
+        ref["x"] = ref["x"][:len(sub["x"])]
+        e = acc.compare_states(sub, ref, x0 - 0.0)
+        say(f"  instance {k}: {e}")
+        assert e["stress_rel_max"] < 1e-4 and e["vel_rel_max"] < 1e-4
+
+
+def test_autotuner_picks_a_candidate():
+    sps = load_pkg()
+    ens = ensemble_mod()
+    tune = importlib.import_module("stress-particle-sph_amd.autotune")
+    base = sps.read_deck(deck_dir("c2_slope_dx01"))
+    inst = small_instances(ens)
+    best, table = tune.tune(lambda: ens.open_ensemble(sps, base, inst, dx=0.25), 0.25, skins=(0.1, 0.3),
+                            warmup=2, steps=4)
+    say(f"  {best}; {table}")
+    assert len(table) == 4
+    assert (best["skin_dx"], best["graph"]) in {(r["skin_dx"], r["graph"]) for r in table}
