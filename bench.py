@@ -216,7 +216,7 @@ def train_surrogate(args, sps, sur, log, parts, deck):
     Sc = max(float(coh), 1e3)
     rng = np.random.default_rng(0)
     hold = rng.random(len(sn)) < 0.1
-    a = sur.augment(lib, sn[~hold], st[~hold], tanfi, coh, rng, copies=2, scale_max=10.0)
+    a = sur.augment(lib, sn[~hold], st[~hold], tanfi, coh, rng, copies=8, scale_max=10.0)
     x = np.vstack([sur.features(sn[~hold], st[~hold], coh, tanfi, S, Sc), sur.features(a[0], a[1], a[3], a[4], S, Sc)])
     t = np.vstack([sur.targets(st[~hold], sr[~hold], tanfi, coh), sur.targets(a[1], a[2], a[4], a[3])])
     net = sur.Network(lib, seed=1, batch_rows=8192)
@@ -317,7 +317,7 @@ def main():
     ap.add_argument("--skin", type=float, default=0.25,
                     help="list skin of the mixed form's list-reuse leg, in lattice spacings")
     ap.add_argument("--log-cap", type=int, default=4_000_000, help="plastic tuples logged for the surrogate")
-    ap.add_argument("--epochs", type=int, default=10, help="surrogate training epochs")
+    ap.add_argument("--epochs", type=int, default=200, help="surrogate training epochs")
     ap.add_argument("--surrogate-tol", type=float, default=1e-4, help="yield-function residual kept")
     ap.add_argument("--instances", type=int, default=64, help="slope instances of the batched form (0: skip)")
     ap.add_argument("--acc-steps", type=int, default=2000,

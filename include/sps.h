@@ -230,9 +230,12 @@ int sps_get_search_count(sps_handle *h, int64_t *n);
  *   with its own tan(phi) and c (n each).
  * sps_set_surrogate (SPS_MODE_FAST; NULL detaches): each step's final stress
  *   update sends the stress points' plastic trial states through the network
- *   and keeps a result where |F| <= tol * (sqrt(J2) + |3 alpha p| + k_c) of
- *   the trial state (F the yield function); elsewhere the exact return map's
- *   result stays.  sps_get_surrogate_stats: states evaluated and kept.
+ *   and keeps a result where it is consistent: |F| <= tol * (sqrt(J2) +
+ *   |3 alpha p| + k_c) of the trial state (F the yield function), its mean
+ *   stress that of the trial state (the apex beyond the tension cutoff) and
+ *   its deviator parallel to the trial one, within the same tolerance;
+ *   elsewhere the exact return map's result stays.  sps_get_surrogate_stats:
+ *   states evaluated and kept.
  * sps_set_plastic_log / sps_get_plastic_log: the plain method's tuples
  *   (sigma_n, sigma_trial, R(sigma_trial)), 12 doubles each, of every plastic
  *   trial state of the final stress updates, up to cap (training data). */

@@ -22,10 +22,12 @@
 // In a step (SPS_MODE_FAST with a network attached) the final stress update
 // of the stress points (k_rk_final) logs every plastic trial state; the
 // network corrects them in one batch and each result is kept only if it is
-// consistent: its yield-function residual |F| relative to the trial state's
-// size is at most the tolerance.  Otherwise the exact return map's result
-// stays.  Elastic trial states never reach the network (R is the identity
-// there).
+// consistent, each residual at most the tolerance relative to the trial
+// state's size (sqrt(J2) + |3 alpha p| + k_c): the yield function |F|, and
+// the flow rule of the return map -- the mean stress unchanged (or at the
+// apex beyond the tension cutoff) and the deviator parallel to the trial
+// one.  Otherwise the exact return map's result stays.  Elastic trial states
+// never reach the network (R is the identity there).
 namespace sps {
 
 typedef unsigned short bf16_t;
@@ -250,10 +252,20 @@ __global__ void k_surrogate_apply(SurrogateDev G, const float *__restrict__ Y, d
         const double f = dp_yield(s, alpha2, kc, sj2);
         const double size = sj2_t + fabs(3 * alpha2 * (t.x + t.y + t.w) / 3.0) + kc;
         (void)sc;
+        // consistency with the return map's flow rule: the returned deviator
+        // is the trial one scaled down, at the trial mean stress, or at the
+        // apex kc / (3 alpha) where the trial state is beyond the tension cutoff
+        const double pt = (t.x + t.y + t.w) / 3.0, po = (s.x + s.y + s.w) / 3.0;
+        const double p_exp = kc - 3 * alpha2 * pt < 0 ? kc / (3 * alpha2) : pt;
+        const double a0 = t.x - pt, a1 = t.y - pt, a3 = t.w - pt;
+        const double b0 = s.x - po, b1 = s.y - po, b3 = s.w - po;
+        const double dot = t.z * s.z + 0.5 * (a0 * b0 + a1 * b1 + a3 * b3);   // the J2 inner product
+        const double cross = sj2_t * sj2 - dot;                               // >= 0, 0 when parallel
         // left to the return map: an overstress at rounding level (it moved a few
         // bits), and sqrt(J2) <= 1e-5, where it zeroes the deviator whatever
         // the overstress (3_...:2122)
-        kept = ft > OVERSTRESS_MIN * size && sj2_t > (double)10e-06f && fabs(f) <= tol * size;
+        kept = ft > OVERSTRESS_MIN * size && sj2_t > (double)10e-06f && fabs(f) <= tol * size &&
+               fabs(po - p_exp) <= tol * size && cross <= tol * sj2_t * sj2_t;
         if (kept) stress[G.slot[k]] = s;
     }
     // per-wave counts, one atomic per wave
