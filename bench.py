@@ -362,8 +362,13 @@ def main():
                  sorted(prof.items(), key=lambda kv: -kv[1][1])}
     print("[bench] kernel breakdown (ms per step): " +
           ", ".join(f"{k} {v['ms_per_step']}" for k, v in breakdown.items()), file=sys.stderr, flush=True)
+    # what a driver that takes the whole state back every step pays on top (PCIe)
+    t0 = time.perf_counter()
+    sim.get_state()
+    host_copy_ms = 1e3 * (time.perf_counter() - t0)
     sim.close()
-    forms = {"fp64": {"ms_per_step": round(ms_per_step, 4), "pairs_per_s": round(value, 1)}}
+    forms = {"fp64": {"ms_per_step": round(ms_per_step, 4), "pairs_per_s": round(value, 1),
+                      "host_copy_ms": round(host_copy_ms, 3)}}
     if not args.no_mixed:
         msim, _, mpairs, melapsed, mprof, _ = run_gpu(args, sps, deck, mode=sps.MODE_FAST)
         msim.close()
