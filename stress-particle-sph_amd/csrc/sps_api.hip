@@ -145,6 +145,8 @@ struct sps_handle {
     DBuf<float4> list_wa, list_gxa, list_gya, list_gxb, list_gyb;
     DBuf<unsigned short> deg_a, deg_b;
     DBuf<unsigned char> tile_ok;
+    DBuf<Tile> tiles;
+    DBuf<StagePlan> plans;
     int ntile_n = 0, ntile_s = 0;
     double h_uniform = 0.0;   // the smoothing length of every particle, or 0 when they differ
     bool mixed = false;       // SPS_MODE_FAST: the mixed-precision stage kernels
@@ -195,7 +197,7 @@ struct sps_handle {
 
     PairLists lists() {
         return PairLists{list_ia.p, list_ib.p, list_wa.p, list_gxa.p, list_gya.p, list_gxb.p, list_gyb.p,
-                         deg_a.p, deg_b.p, tile_ok.p, ntile_n};
+                         deg_a.p, deg_b.p, tile_ok.p, ntile_n, tiles.p, plans.p};
     }
 };
 
@@ -392,6 +394,8 @@ extern "C" int sps_create(const sps_params *params, sps_handle **out) {
     }
     rc |= dalloc(h->deg_a, nt); rc |= dalloc(h->deg_b, nn);
     rc |= dalloc(h->tile_ok, h->ntile_n + h->ntile_s + 1);
+    rc |= dalloc(h->tiles, h->ntile_n + h->ntile_s + 1);
+    rc |= dalloc(h->plans, 3 * (size_t)(h->ntile_n + h->ntile_s + 1));
     rc |= dalloc(h->x_build, nt); rc |= dalloc(h->dmax, 1); rc |= dalloc(h->stepv, 4);
     if (rc) return bail(rc);
     if (hipHostMalloc((void **)&h->pinned, 64 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
@@ -1001,7 +1005,7 @@ extern "C" void sps_destroy(sps_handle *h) {
     dfree(h->kfill); dfree(h->tmp_orig); dfree(h->tmp_src); dfree(h->proc); dfree(h->proc_to_slot);
     dfree(h->cnt); dfree(h->off); dfree(h->scan_tmp); dfree(h->ctot); dfree(h->cstart_rm); dfree(h->err);
     dfree(h->list_ia); dfree(h->list_wa); dfree(h->list_gxa); dfree(h->list_gya); dfree(h->list_ib);
-    dfree(h->list_gxb); dfree(h->list_gyb); dfree(h->deg_a); dfree(h->deg_b); dfree(h->tile_ok);
+    dfree(h->list_gxb); dfree(h->list_gyb); dfree(h->deg_a); dfree(h->deg_b); dfree(h->tile_ok); dfree(h->tiles); dfree(h->plans);
     dfree(h->x_build); dfree(h->dmax); dfree(h->surr_count); dfree(h->surr_slot); dfree(h->surr_sn);
     dfree(h->surr_str); dfree(h->surr_feat); dfree(h->surr_stats); dfree(h->plog); dfree(h->surr_mat);
     dfree(h->mtab); dfree(h->mat_orig);

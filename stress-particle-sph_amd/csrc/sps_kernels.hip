@@ -1058,7 +1058,23 @@ struct PairLists {
     unsigned short *deg_a, *deg_b;  // per slot: nodes and stress points / nodes
     unsigned char *ok;              // per tile (node tiles, then SP tiles): lists usable
     int ntile_n;                    // node tiles
+    // each tile's extent and (when it fits) the staging plan of every partner
+    // kind, as the search computed them: the stage kernels load them in one
+    // round trip instead of deriving them again through dependent loads
+    Tile *tiles;                    // per tile
+    StagePlan *plans;               // per tile and kind
 };
+
+// Wave `wave` of the block copies the search's plan of kind K for tile
+// ltile into the shared plan (the other waves return at once); the caller
+// puts a barrier after it, as after plan_kind.
+template <int K>
+__device__ __forceinline__ void load_plan(const PairLists &PL, int ltile, int wave, StagePlan &sc) {
+    if ((int)(threadIdx.x >> 6) != wave) return;
+    const int *src = reinterpret_cast<const int *>(PL.plans + (size_t)ltile * 3 + K);
+    int *dst = reinterpret_cast<int *>(&sc);
+    for (int i = (int)(threadIdx.x & 63); i < (int)(sizeof(StagePlan) / sizeof(int)); i += 64) dst[i] = src[i];
+}
 
 __device__ __forceinline__ size_t word_at(int tile, int g) {
     return ((size_t)tile * LGROUPS + g) * TILE + threadIdx.x;
@@ -1149,9 +1165,10 @@ __global__ __launch_bounds__(TILE) void k_count_lists(SearchView V, const double
     constexpr const LdsLayout &LY = LAY_COUNT;
     const WalkCtx C = walk_ctx(V);
     const Tile T = make_tile(base, count, KR, V, C);
-    if (T.first >= T.end) return;
     const int tile = tile_index();
     const int ltile = KR == KIND_NODE ? tile : PL.ntile_n + tile;
+    if (threadIdx.x == 0) PL.tiles[ltile] = T;
+    if (T.first >= T.end) return;
     const int r = T.first + (int)threadIdx.x;
     int self_base = 0;
     bool lds = T.fits;
@@ -1160,6 +1177,15 @@ __global__ __launch_bounds__(TILE) void k_count_lists(SearchView V, const double
         plan_kind<KIND_SP>(T, V, C, 1, lds_at<int>(LY.tab[KIND_SP]), pl[KIND_SP]);
         plan_kind<KIND_DUMMY>(T, V, C, 2, lds_at<int>(LY.tab[KIND_DUMMY]), pl[KIND_DUMMY]);
         __syncthreads();
+        {   // keep the plans for the stage kernels: wave k stores kind k's
+            const int wave = (int)(threadIdx.x >> 6);
+            if (wave < 3) {
+                const int *src = reinterpret_cast<const int *>(&pl[wave]);
+                int *dst = reinterpret_cast<int *>(PL.plans + (size_t)ltile * 3 + wave);
+                for (int i = (int)(threadIdx.x & 63); i < (int)(sizeof(StagePlan) / sizeof(int)); i += 64)
+                    dst[i] = src[i];
+            }
+        }
         lds = pl[KIND_NODE].total <= CAP && pl[KIND_SP].total <= CAP && pl[KIND_DUMMY].total <= CAP_D;
         self_base = pl[KR].mainlo;
     }
@@ -1381,17 +1407,17 @@ __global__ __launch_bounds__(TILE) void k_spu_nodes(StateBufs S, SearchView V, P
     __shared__ StagePlan pl;
     constexpr const LdsLayout &LY = LAY_SPU_NODES;
     const WalkCtx C = walk_ctx(V);
-    const Tile T = make_tile(0, nn, KIND_NODE, V, C);
+    const int tile = tile_index();
+    const Tile T = PL.tiles[tile];
     if (T.first >= T.end) return;
     const int r = T.first + (int)threadIdx.x;
-    const int tile = tile_index();
     const bool inside = r < T.end && V.cell[r] < C.g.ndivt;
     ListWalk<MASK_SP, true, false> lw;
     lw.start(PL.ia, PL.wa, PL.gxa, PL.gya, tile, inside ? PL.deg_a[r] : 0, !C.any_fwd && C.any_rev);
     bool check;
     bool lds = tile_uses_lds(T, V, C, r, check) && PL.ok[tile];
     if (lds) {
-        plan_kind<KIND_SP>(T, V, C, 0, nullptr, pl);
+        load_plan<KIND_SP>(PL, tile, 0, pl);
         __syncthreads();
         lds = pl.total <= CAP;
     }
@@ -1515,11 +1541,11 @@ __global__ __launch_bounds__(TILE) void k_sp_update(StateBufs S, Work W, SearchV
     __shared__ StagePlan pl[2];
     constexpr const LdsLayout &LY = LAY_SP_UPDATE;
     const WalkCtx C = walk_ctx(V);
-    const Tile T = make_tile(nn, nt - nn, KIND_SP, V, C);
-    if (T.first >= T.end) return;
-    const int r = T.first + (int)threadIdx.x;
     const int tile = tile_index();
     const int ltile = PL.ntile_n + tile;
+    const Tile T = PL.tiles[ltile];
+    if (T.first >= T.end) return;
+    const int r = T.first + (int)threadIdx.x;
     const bool inside = r < T.end && V.cell[r] < C.g.ndivt;
     const int deg = inside ? PL.deg_a[r] : 0;
     const bool rev_l = !C.any_fwd && C.any_rev;
@@ -1528,8 +1554,8 @@ __global__ __launch_bounds__(TILE) void k_sp_update(StateBufs S, Work W, SearchV
     bool check;
     bool lds = tile_uses_lds(T, V, C, r, check) && PL.ok[ltile];
     if (lds) {
-        plan_kind<KIND_NODE>(T, V, C, 0, nullptr, pl[0]);
-        if (deriv) plan_kind<KIND_DUMMY>(T, V, C, 1, nullptr, pl[1]);
+        load_plan<KIND_NODE>(PL, ltile, 0, pl[0]);
+        if (deriv) load_plan<KIND_DUMMY>(PL, ltile, 1, pl[1]);
         __syncthreads();
         lds = pl[0].total <= CAP && (!deriv || pl[1].total <= CAP_D);
     }
@@ -1769,10 +1795,10 @@ __global__ __launch_bounds__(TILE) void k_deriv_nodes(StateBufs S, Work W, Searc
     __shared__ StagePlan pl[2];
     constexpr const LdsLayout &LY = LAY_DERIV_NODES;
     const WalkCtx C = walk_ctx(V);
-    const Tile T = make_tile(0, nn, KIND_NODE, V, C);
+    const int tile = tile_index();
+    const Tile T = PL.tiles[tile];
     if (T.first >= T.end) return;
     const int r = T.first + (int)threadIdx.x;
-    const int tile = tile_index();
     const bool active = r < T.end;
     const int key_r = active ? V.cell[r] : C.g.ndivt;
     const bool inside = key_r < C.g.ndivt;
@@ -1796,8 +1822,8 @@ __global__ __launch_bounds__(TILE) void k_deriv_nodes(StateBufs S, Work W, Searc
     // phase 1: stress gradient over stress points (type 1) and dummies (type 6)
     bool lds = tiled;
     if (lds) {
-        plan_kind<KIND_SP>(T, V, C, 0, nullptr, pl[0]);
-        plan_kind<KIND_DUMMY>(T, V, C, 1, nullptr, pl[1]);
+        load_plan<KIND_SP>(PL, tile, 0, pl[0]);
+        load_plan<KIND_DUMMY>(PL, tile, 1, pl[1]);
         __syncthreads();
         lds = pl[0].total <= CAP && pl[1].total <= CAP_D;
     }
@@ -1872,7 +1898,7 @@ __global__ __launch_bounds__(TILE) void k_deriv_nodes(StateBufs S, Work W, Searc
         lb.start(PL.ib, nullptr, PL.gxb, PL.gyb, tile, inside ? PL.deg_b[r] : 0, rev_l);
         bool lds2 = tiled;
         if (lds2) {
-            plan_kind<KIND_NODE>(T, V, C, 0, nullptr, pl[0]);
+            load_plan<KIND_NODE>(PL, tile, 0, pl[0]);
             __syncthreads();   // also: the phase-1 walks are done with the shared bytes
             lds2 = pl[0].total <= CAP;
         }

@@ -173,17 +173,17 @@ __global__ __launch_bounds__(TILE) void k_spu_nodes_mix(StateBufs S, SearchView 
     __shared__ StagePlan pl;
     constexpr const LdsLayout &LY = LAY_SPU_NODES_MIX;
     const WalkCtx C = walk_ctx(V);
-    const Tile T = make_tile(0, nn, KIND_NODE, V, C);
+    const int tile = tile_index();
+    const Tile T = PL.tiles[tile];
     if (T.first >= T.end) return;
     const int r = T.first + (int)threadIdx.x;
-    const int tile = tile_index();
     const bool inside = r < T.end && V.cell[r] < C.g.ndivt;
     ListWalk<MASK_SP, false, false> lw;
     lw.start(PL.ia, nullptr, nullptr, nullptr, tile, inside ? PL.deg_a[r] : 0, false);
     const double2 o = S.x[T.first];
     bool lds = T.fits && PL.ok[tile];
     if (lds) {
-        plan_kind<KIND_SP>(T, V, C, 0, nullptr, pl);
+        load_plan<KIND_SP>(PL, tile, 0, pl);
         __syncthreads();
         lds = pl.total <= CAP;
     }
@@ -254,11 +254,11 @@ __global__ __launch_bounds__(TILE) void k_sp_update_mix(StateBufs S, Work W, Sea
     __shared__ StagePlan pl[2];
     constexpr const LdsLayout &LY = LAY_SP_UPDATE_MIX;
     const WalkCtx C = walk_ctx(V);
-    const Tile T = make_tile(nn, nt - nn, KIND_SP, V, C);
-    if (T.first >= T.end) return;
-    const int r = T.first + (int)threadIdx.x;
     const int tile = tile_index();
     const int ltile = PL.ntile_n + tile;
+    const Tile T = PL.tiles[ltile];
+    if (T.first >= T.end) return;
+    const int r = T.first + (int)threadIdx.x;
     const bool inside = r < T.end && V.cell[r] < C.g.ndivt;
     const int deg = inside ? PL.deg_a[r] : 0;
     ListWalk<MASK_NODE, false, false> lw;
@@ -266,8 +266,8 @@ __global__ __launch_bounds__(TILE) void k_sp_update_mix(StateBufs S, Work W, Sea
     const double2 o = S.x[T.first];
     bool lds = T.fits && PL.ok[ltile];
     if (lds) {
-        plan_kind<KIND_NODE>(T, V, C, 0, nullptr, pl[0]);
-        if (deriv) plan_kind<KIND_DUMMY>(T, V, C, 1, nullptr, pl[1]);
+        load_plan<KIND_NODE>(PL, ltile, 0, pl[0]);
+        if (deriv) load_plan<KIND_DUMMY>(PL, ltile, 1, pl[1]);
         __syncthreads();
         lds = pl[0].total <= CAP && (!deriv || pl[1].total <= CAP_D);
     }
@@ -413,10 +413,10 @@ __global__ __launch_bounds__(TILE) void k_deriv_nodes_mix(StateBufs S, Work W, S
     __shared__ StagePlan pl[2];
     constexpr const LdsLayout &LY = LAY_DERIV_NODES_MIX;
     const WalkCtx C = walk_ctx(V);
-    const Tile T = make_tile(0, nn, KIND_NODE, V, C);
+    const int tile = tile_index();
+    const Tile T = PL.tiles[tile];
     if (T.first >= T.end) return;
     const int r = T.first + (int)threadIdx.x;
-    const int tile = tile_index();
     const bool active = r < T.end;
     const int key_r = active ? V.cell[r] : C.g.ndivt;
     const bool inside = key_r < C.g.ndivt;
@@ -440,8 +440,8 @@ __global__ __launch_bounds__(TILE) void k_deriv_nodes_mix(StateBufs S, Work W, S
     // phase 1: stress gradient over stress points (type 1) and dummies (type 6)
     bool lds = tiled;
     if (lds) {
-        plan_kind<KIND_SP>(T, V, C, 0, nullptr, pl[0]);
-        plan_kind<KIND_DUMMY>(T, V, C, 1, nullptr, pl[1]);
+        load_plan<KIND_SP>(PL, tile, 0, pl[0]);
+        load_plan<KIND_DUMMY>(PL, tile, 1, pl[1]);
         __syncthreads();
         lds = pl[0].total <= CAP && pl[1].total <= CAP_D;
     }
@@ -526,7 +526,7 @@ __global__ __launch_bounds__(TILE) void k_deriv_nodes_mix(StateBufs S, Work W, S
         lb.start(PL.ib, nullptr, nullptr, nullptr, tile, inside ? PL.deg_b[r] : 0, false);
         bool lds2 = tiled;
         if (lds2) {
-            plan_kind<KIND_NODE>(T, V, C, 0, nullptr, pl[0]);
+            load_plan<KIND_NODE>(PL, tile, 0, pl[0]);
             __syncthreads();   // also: the phase-1 walks are done with the shared bytes
             lds2 = pl[0].total <= CAP;
         }
