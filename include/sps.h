@@ -83,7 +83,8 @@ typedef struct sps_params {
     int32_t inside_approach; /* must be 1 */
     int32_t cspm;            /* CSPM gradient correction (2_...:539) */
     int32_t update_x;        /* position update + Jaumann spin (2_...:140, 751) */
-    int32_t xsph;            /* must be 0 */
+    int32_t xsph;            /* XSPH position update (2_...:142-143, 189-239);
+                                SPS_MODE_ORDERED only */
     int32_t cont_density;    /* must be 0 */
     int32_t art_stress;      /* must be 0 */
     int32_t dummy_nodes;     /* wall dummies present (.pts) */

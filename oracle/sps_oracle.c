@@ -871,6 +871,35 @@ static void rk4(struct sps_handle *h, double dt_sph, double time_sph)
 
 /* ------------------------------------------------------- time_integration */
 /* code/2_SPH_main_2018.f90:78-184 for the supported flag set */
+/* ----------------------------------------------------------- XSPH_update */
+/* code/2_SPH_main_2018.f90:189-239: positions moved with the velocity plus
+ * eps = 0.5 times the XSPH average over stress-point pairs (type 2) and node
+ * pairs (type 3), each pair added to both ends in list order.  Its n_int and
+ * bc_or_not bookkeeping feed only the boundary conditions and the outside
+ * approach, neither on the supported path. */
+static void xsph_update(struct sps_handle *h, double dt)
+{
+    const int nt = h->nt;
+    double *vs = h->vel_tmp; /* free again after stress_point_update */
+    memset(vs, 0, 2 * (size_t)nt * sizeof(double));
+    for (int64_t k = 0; k < h->n_pairs; ++k) {
+        const int64_t d = h->order[k];
+        const int t = h->pt[d];
+        if (t != 2 && t != 3) continue;
+        const int i = h->pi[d], j = h->pj[d];
+        const double w = (double)h->pw[d];
+        for (int a = 0; a < 2; ++a) {
+            const double vi = h->vel[2 * i + a], vj = h->vel[2 * j + a];
+            vs[2 * i + a] = vs[2 * i + a] + ((h->mass[j] / h->rho[j]) * (vj - vi)) * w;
+            vs[2 * j + a] = vs[2 * j + a] + ((h->mass[i] / h->rho[i]) * (vi - vj)) * w;
+        }
+    }
+    const double eps = 0.5;
+    for (int i = 0; i < nt; ++i)
+        for (int a = 0; a < 2; ++a)
+            h->x[2 * i + a] = h->x0[2 * i + a] + dt * (h->vel[2 * i + a] + eps * vs[2 * i + a]);
+}
+
 static int time_integration(struct sps_handle *h, int itimestep, double time_sph, double dt)
 {
     const int nn = h->nn, nt = h->nt;
@@ -886,7 +915,11 @@ static int time_integration(struct sps_handle *h, int itimestep, double time_sph
         h->ivars[(size_t)h->niv * i] = h->ivars[(size_t)h->niv * i] + dt * h->ddev[i];
     stress_point_update(h);
     if (ncrit == 12) adapt_stress2(h);
-    if (h->p.update_x) {
+    if (h->p.update_x && h->p.xsph) {
+        xsph_update(h, dt);
+        for (int i = 0; i < nn; ++i)
+            for (int a = 0; a < 2; ++a) h->displ[2 * i + a] = h->x[2 * i + a] - h->x00[2 * i + a];
+    } else if (h->p.update_x) {
         for (int i = 0; i < nt; ++i)
             for (int a = 0; a < 2; ++a) {
                 float vel_half = (float)(0.5 * (h->vx0[2 * i + a] + h->vel[2 * i + a]));
@@ -913,8 +946,9 @@ static int check_params(const sps_params *p)
     if (p->ndimn != 2 || p->nstre != 4) return fail(SPS_EINVAL, "only ndimn=2, nstre=4 (plane strain) are supported");
     if (p->skf != 1) return fail(SPS_EINVAL, "only the cubic spline kernel (skf=1) is supported");
     if (!p->sp_sph || !p->inside_approach) return fail(SPS_EINVAL, "only stress-particle SPH with the inside approach is supported");
-    if (p->xsph || p->cont_density || p->art_stress || p->sph_shift || p->no_bcs)
-        return fail(SPS_EINVAL, "XSPH, continuity density, artificial stress, stress-point shifting and prescribed BCs are not supported");
+    if (p->cont_density || p->art_stress || p->sph_shift || p->no_bcs)
+        return fail(SPS_EINVAL, "continuity density, artificial stress, stress-point shifting and prescribed BCs are not supported");
+    if (p->xsph && p->mode != SPS_MODE_ORDERED) return fail(SPS_EINVAL, "XSPH runs in SPS_MODE_ORDERED");
     int ntype_eco = (int)p->props[0], ncrit = (int)p->props[1];
     if (ntype_eco > 1 && ncrit <= 5) return fail(SPS_EINVAL, "Perzyna models (ncrit<=5) are not supported");
     if (p->ic_grav && (p->tcurve_npts < 2 || p->tcurve_npts > SPS_MAX_TCURVE))

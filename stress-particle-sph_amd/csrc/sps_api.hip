@@ -142,7 +142,9 @@ struct sps_handle {
     DBuf<unsigned> cnt, off, scan_tmp, ctot, cstart_rm;
     // per-step pair lists of the tiled stage kernels
     DBuf<unsigned long long> list_ia, list_ib;
-    DBuf<float4> list_wa, list_gxa, list_gya, list_gxb, list_gyb;
+    DBuf<float4> list_wa, list_gxa, list_gya, list_gxb, list_gyb, list_wb;
+    bool xsph = false;            // XSPH position update (list B also for stress points, with w)
+    DBuf<double2> xsph_v;
     DBuf<unsigned short> deg_a, deg_b;
     DBuf<unsigned char> tile_ok;
     DBuf<Tile> tiles;
@@ -196,8 +198,9 @@ struct sps_handle {
     long long prof_n[32] = {0};
 
     PairLists lists() {
-        return PairLists{list_ia.p, list_ib.p, list_wa.p, list_gxa.p, list_gya.p, list_gxb.p, list_gyb.p,
-                         deg_a.p, deg_b.p, tile_ok.p, ntile_n, tiles.p, plans.p};
+        return PairLists{list_ia.p,  list_ib.p,  list_wa.p, list_gxa.p,          list_gya.p,
+                         list_gxb.p, list_gyb.p, xsph ? list_wb.p : nullptr, deg_a.p, deg_b.p,
+                         tile_ok.p,  ntile_n,    tiles.p,   plans.p};
     }
 };
 
@@ -205,13 +208,13 @@ struct sps_handle {
 enum KernelId {
     K_CHECK_OOD, K_BOUNDS, K_CELL_COUNT, K_CELL_SCATTER, K_CELL_SORT, K_PERMUTE, K_PROC,
     K_COUNT_PAIRS, K_SCAN, K_SPLIT, K_RK_INIT, K_STAGE_BEGIN, K_SPU_NODES,
-    K_SP_UPDATE, K_DERIV_NODES, K_DUMMY_STRESS, K_RK_FINAL, K_POSITION, K_SURROGATE, K_NKERNELS
+    K_SP_UPDATE, K_DERIV_NODES, K_DUMMY_STRESS, K_RK_FINAL, K_POSITION, K_SURROGATE, K_XSPH, K_NKERNELS
 };
 static const char *const kKernelNames[K_NKERNELS] = {
     "k_check_ood", "k_bounds", "k_cell_count", "k_cell_scatter", "k_cell_sort", "k_permute",
     "k_proc_slot", "k_count", "k_scan", "k_split", "k_rk_init",
     "k_stage_begin", "k_spu_nodes", "k_sp_update", "k_deriv_nodes",
-    "k_dummy_stress", "k_rk_final", "k_position", "k_surrogate"};
+    "k_dummy_stress", "k_rk_final", "k_position", "k_surrogate", "k_xsph"};
 
 static hipEvent_t prof_begin(sps_handle *h) {
     if (!h->prof) return nullptr;
@@ -294,8 +297,9 @@ static int check_params(const sps_params *p) {
     if (p->ndimn != 2 || p->nstre != 4) return fail(SPS_EINVAL, "only ndimn=2, nstre=4 (plane strain) are supported");
     if (p->skf != 1) return fail(SPS_EINVAL, "only the cubic spline kernel (skf=1) is supported");
     if (!p->sp_sph || !p->inside_approach) return fail(SPS_EINVAL, "only stress-particle SPH with the inside approach is supported");
-    if (p->xsph || p->cont_density || p->art_stress || p->sph_shift || p->no_bcs)
-        return fail(SPS_EINVAL, "XSPH, continuity density, artificial stress, stress-point shifting and prescribed BCs are not supported");
+    if (p->cont_density || p->art_stress || p->sph_shift || p->no_bcs)
+        return fail(SPS_EINVAL, "continuity density, artificial stress, stress-point shifting and prescribed BCs are not supported");
+    if (p->xsph && p->mode != SPS_MODE_ORDERED) return fail(SPS_EINVAL, "XSPH runs in SPS_MODE_ORDERED");
     int ntype_eco = (int)p->props[0], ncrit = (int)p->props[1];
     if (ntype_eco > 1 && ncrit <= 5) return fail(SPS_EINVAL, "Perzyna models (ncrit<=5) are not supported");
     if (p->ic_grav && (p->tcurve_npts < 2 || p->tcurve_npts > SPS_MAX_TCURVE))
@@ -344,6 +348,7 @@ extern "C" int sps_create(const sps_params *params, sps_handle **out) {
     h->nt2 = h->nt + h->nd;
     h->niv = params->nint_vars;
     h->mixed = params->mode == SPS_MODE_FAST;
+    h->xsph = params->xsph != 0;
     (void)hipGetDevice(&h->device);
     DevParams &d = h->dp;
     d.nn = h->nn; d.ns = h->ns; d.nd = h->nd; d.nd2 = h->nd2; d.nt = h->nt; d.nt2 = h->nt2;
@@ -390,9 +395,11 @@ extern "C" int sps_create(const sps_params *params, sps_handle **out) {
         const size_t wa = (size_t)(h->ntile_n + h->ntile_s) * LGROUPS * TILE, wb = (size_t)h->ntile_n * LGROUPS * TILE;
         rc |= dalloc(h->list_ia, wa); rc |= dalloc(h->list_wa, wa); rc |= dalloc(h->list_gxa, wa);
         rc |= dalloc(h->list_gya, wa);
-        rc |= dalloc(h->list_ib, wb); rc |= dalloc(h->list_gxb, wb); rc |= dalloc(h->list_gyb, wb);
+        // list B: node tiles; with XSPH stress-point tiles too, and its w
+        rc |= dalloc(h->list_ib, h->xsph ? wa : wb); rc |= dalloc(h->list_gxb, wb); rc |= dalloc(h->list_gyb, wb);
+        if (h->xsph) rc |= dalloc(h->list_wb, wa) | dalloc(h->xsph_v, (size_t)nt);
     }
-    rc |= dalloc(h->deg_a, nt); rc |= dalloc(h->deg_b, nn);
+    rc |= dalloc(h->deg_a, nt); rc |= dalloc(h->deg_b, h->xsph ? nt : nn);
     rc |= dalloc(h->tile_ok, h->ntile_n + h->ntile_s + 1);
     rc |= dalloc(h->tiles, h->ntile_n + h->ntile_s + 1);
     rc |= dalloc(h->plans, 3 * (size_t)(h->ntile_n + h->ntile_s + 1));
@@ -646,7 +653,7 @@ static int mixed_step_kernels(sps_handle *h, double dt, double time_sph, double 
     if (nt > nn)
         LAUNCH_LDS(K_SP_UPDATE, nblk8(nt - nn), TILE, LAY_SP_UPDATE_MIX.bytes, k_sp_update_mix, SB, W, V, PL, h->dp,
                    time_sph, 0.0, 0, adapt, hu, nn, nt, stepv);
-    LAUNCH(K_POSITION, nblk(nt), BLOCK, k_position, SB, W, dt, h->p.update_x, nn, nt);
+    LAUNCH(K_POSITION, nblk(nt), BLOCK, k_position, SB, W, dt, h->p.update_x, nn, nt, (const double2 *)nullptr);
     if (h->skin > 0) {
         LAUNCH(K_POSITION, 1, 64, k_zero_u64, h->dmax.p, 1);
         LAUNCH(K_POSITION, std::min(1024u, nblk(nt)), BLOCK, k_max_disp, SB.x, h->x_build.p, nt, h->dmax.p);
@@ -758,7 +765,12 @@ static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
     if (nt > nn)
         LAUNCH_LDS(K_SP_UPDATE, nblk8(nt - nn), TILE, LAY_SP_UPDATE.bytes, k_sp_update, SB, W, V, PL, h->dp, time_sph, 0.0,
                    0, adapt, nn, nt);
-    LAUNCH(K_POSITION, nblk(nt), BLOCK, k_position, SB, W, dt, h->p.update_x, nn, nt);
+    const bool xsph = h->xsph && h->p.update_x;
+    if (xsph && nn > 0)
+        LAUNCH_LDS(K_XSPH, nblk8(nn), TILE, LAY_XSPH.bytes, (k_xsph<KIND_NODE>), SB, V, PL, h->xsph_v.p);
+    if (xsph && nt > nn)
+        LAUNCH_LDS(K_XSPH, nblk8(nt - nn), TILE, LAY_XSPH.bytes, (k_xsph<KIND_SP>), SB, V, PL, h->xsph_v.p);
+    LAUNCH(K_POSITION, nblk(nt), BLOCK, k_position, SB, W, dt, h->p.update_x, nn, nt, xsph ? h->xsph_v.p : nullptr);
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -1006,6 +1018,7 @@ extern "C" void sps_destroy(sps_handle *h) {
     dfree(h->cnt); dfree(h->off); dfree(h->scan_tmp); dfree(h->ctot); dfree(h->cstart_rm); dfree(h->err);
     dfree(h->list_ia); dfree(h->list_wa); dfree(h->list_gxa); dfree(h->list_gya); dfree(h->list_ib);
     dfree(h->list_gxb); dfree(h->list_gyb); dfree(h->deg_a); dfree(h->deg_b); dfree(h->tile_ok); dfree(h->tiles); dfree(h->plans);
+    dfree(h->list_wb); dfree(h->xsph_v);
     dfree(h->x_build); dfree(h->dmax); dfree(h->surr_count); dfree(h->surr_slot); dfree(h->surr_sn);
     dfree(h->surr_str); dfree(h->surr_feat); dfree(h->surr_stats); dfree(h->plog); dfree(h->surr_mat);
     dfree(h->mtab); dfree(h->mat_orig);

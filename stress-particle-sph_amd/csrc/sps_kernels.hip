@@ -1054,7 +1054,9 @@ constexpr int LGROUPS = LMAX / 4;
 struct PairLists {
     unsigned long long *ia, *ib;    // [tile][LGROUPS][TILE]: four entries per word
     float4 *wa, *gxa, *gya;         // list A: w, dW/dx_r, dW/dy_r of those entries
-    float4 *gxb, *gyb;              // list B: dW/dx_r, dW/dy_r
+    float4 *gxb, *gyb;              // list B (nodes): dW/dx_r, dW/dy_r
+    float4 *wb;                     // list B: w (XSPH only; null otherwise, and then stress
+                                    // points have no list B)
     unsigned short *deg_a, *deg_b;  // per slot: nodes and stress points / nodes
     unsigned char *ok;              // per tile (node tiles, then SP tiles): lists usable
     int ntile_n;                    // node tiles
@@ -1134,8 +1136,8 @@ __device__ __forceinline__ void fill_kernel_values(const unsigned long long *ip,
             }
         }
         if (wp) wp[o] = make_float4(wv[0], wv[1], wv[2], wv[3]);
-        gxp[o] = make_float4(gxv[0], gxv[1], gxv[2], gxv[3]);
-        gyp[o] = make_float4(gyv[0], gyv[1], gyv[2], gyv[3]);
+        if (gxp) gxp[o] = make_float4(gxv[0], gxv[1], gxv[2], gxv[3]);
+        if (gyp) gyp[o] = make_float4(gyv[0], gyv[1], gyv[2], gyv[3]);
     }
 }
 
@@ -1170,6 +1172,8 @@ __global__ __launch_bounds__(TILE) void k_count_lists(SearchView V, const double
     if (threadIdx.x == 0) PL.tiles[ltile] = T;
     if (T.first >= T.end) return;
     const int r = T.first + (int)threadIdx.x;
+    // list B: a node's node partners; with XSPH also a stress point's stress-point partners
+    const bool list_b = KR == KIND_NODE || PL.wb != nullptr;
     int self_base = 0;
     bool lds = T.fits;
     if (lds) {
@@ -1257,7 +1261,7 @@ __global__ __launch_bounds__(TILE) void k_count_lists(SearchView V, const double
                             const unsigned e = (unsigned)((ks << 14) | j);
                             if (in_list_a(KR, ks))
                                 la.put(ltile, PL.ia, e);
-                            else if (KR == KIND_NODE && ks == KIND_NODE)
+                            else if (ks == KR && list_b)
                                 lb.put(ltile, PL.ib, e);
                         }
                     }
@@ -1275,7 +1279,7 @@ __global__ __launch_bounds__(TILE) void k_count_lists(SearchView V, const double
         }
         if (lds) {
             la.finish(ltile, PL.ia);
-            if (KR == KIND_NODE) lb.finish(ltile, PL.ib);
+            if (list_b) lb.finish(ltile, PL.ib);
         }
         cnt[V.proc[r]] = fk[0] + fk[1] + fk[2];
     }
@@ -1295,12 +1299,13 @@ __global__ __launch_bounds__(TILE) void k_count_lists(SearchView V, const double
         const double hr = hsml[r];
         KernelFactor kf;
         fill_kernel_values(PL.ia, PL.wa, PL.gxa, PL.gya, ltile, la.n, LdsFields<LY>{}, xr, hr, V.pi, kf);
-        if (KR == KIND_NODE)
-            fill_kernel_values(PL.ib, nullptr, PL.gxb, PL.gyb, ltile, lb.n, LdsFields<LY>{}, xr, hr, V.pi, kf);
+        if (list_b)
+            fill_kernel_values(PL.ib, PL.wb, KR == KIND_NODE ? PL.gxb : nullptr, KR == KIND_NODE ? PL.gyb : nullptr,
+                               ltile, lb.n, LdsFields<LY>{}, xr, hr, V.pi, kf);
     }
     if (r < T.end) {
         PL.deg_a[r] = (unsigned short)(la.n < LMAX ? la.n : LMAX);
-        if (KR == KIND_NODE) PL.deg_b[r] = (unsigned short)(lb.n < LMAX ? lb.n : LMAX);
+        if (list_b) PL.deg_b[r] = (unsigned short)(lb.n < LMAX ? lb.n : LMAX);
     }
 }
 
@@ -1951,6 +1956,80 @@ __global__ __launch_bounds__(TILE) void k_deriv_nodes(StateBufs S, Work W, Searc
                     ax, ay, gx_grav, gy_grav, f2, err);
 }
 
+// XSPH_update (2_...:189-239) for the receivers of kind KR: the list-order
+// sum of (m_s / rho_s) (v_s - v_r) w over the same-kind partners (type-3 pairs
+// for nodes, type 2 for stress points), written to xv; k_position then moves
+// the particles.  The sum reads only velocities and masses, which no kernel
+// of the step changes after the last stress_point_update.
+constexpr LdsLayout layout_xsph() {
+    LdsLayout L = layout_none();
+    for (int k = 0; k < 2; ++k) {   // one receiver kind per launch: both share the bytes
+        L.v[k] = 0;
+        L.mr[k] = 16 * CAP;
+    }
+    L.bytes = 24 * CAP;
+    return L;
+}
+constexpr LdsLayout LAY_XSPH = layout_xsph();
+
+template <int KR>
+__global__ __launch_bounds__(TILE) void k_xsph(StateBufs S, SearchView V, PairLists PL,
+                                               double2 *__restrict__ xv) {
+    __shared__ StagePlan pl;
+    constexpr const LdsLayout &LY = LAY_XSPH;
+    const WalkCtx C = walk_ctx(V);
+    const int tile = tile_index();
+    const int ltile = KR == KIND_NODE ? tile : PL.ntile_n + tile;
+    const Tile T = PL.tiles[ltile];
+    if (T.first >= T.end) return;
+    const int r = T.first + (int)threadIdx.x;
+    const bool inside = r < T.end && V.cell[r] < C.g.ndivt;
+    const bool rev_l = !C.any_fwd && C.any_rev;
+    ListWalk<(1 << KR), true, false> lw;
+    lw.start(PL.ib, PL.wb, nullptr, nullptr, ltile, inside ? PL.deg_b[r] : 0, rev_l);
+    bool check;
+    bool lds = tile_uses_lds(T, V, C, r, check) && PL.ok[ltile];
+    if (lds) {
+        load_plan<KR>(PL, ltile, 0, pl);
+        __syncthreads();
+        lds = pl.total <= CAP;
+    }
+    if (lds) {
+        struct Item {
+            double2 v;
+            double m, rho;
+        };
+        copy_kind<KR>(T, V, C, nullptr, pl, [&](int g) { return Item{S.vel[g], S.mass[g], S.rho[g]}; },
+                      [&](int i, const Item &v) {
+                          lds_at<double2>(LY.v[KR])[i] = v.v;
+                          lds_at<double>(LY.mr[KR])[i] = v.m / v.rho;
+                      });
+        __syncthreads();
+    }
+    if (r >= T.end) return;
+    const double2 vr = S.vel[r];
+    double s0 = 0.0, s1 = 0.0;
+    auto body = [&](auto kt, const auto &A, int j, float wf, float, float) __attribute__((always_inline)) {
+        const double w = (double)wf;
+        const double mr = A.mr(kt, j);
+        const double2 v = A.vel(kt, j);
+        s0 = s0 + (mr * (v.x - vr.x)) * w;
+        s1 = s1 + (mr * (v.y - vr.y)) * w;
+    };
+    if (inside) {
+        if (lds) {
+            lw.run(LdsFields<LY>{}, body);
+        } else {
+            int cx, cy;
+            cell_xy(V.cell[r], C.g, cx, cy);
+            const SplitClass cls{check, check ? V.proc[r] : 0, V.proc, V.cnt, V.off, C.sp};
+            walk_stencil<(1 << KR)>(GlobalCells{V.kstart, C.g, C.L}, GlobalFields{S}, C.g, cx, cy, S.x[r],
+                                    S.hsml[r], KR, r, C.any_fwd ? check : C.any_rev, C.any_fwd, V.pi, cls, body);
+        }
+    }
+    xv[r] = make_double2(s0, s1);
+}
+
 // The type-6 loop of get_derivatives overwrites each dummy's stress with
 // that of its node partners in list order (2_...:580); the last one stays.
 __global__ void k_dummy_stress(StateBufs S, SearchView V, int nt, int nt2) {
@@ -2007,11 +2086,23 @@ __global__ void k_rk_final(StateBufs S, Work W, double dt, int adapt, double tan
 
 // Position update (2_...:140-147) through the single-precision vel_half, and
 // the displacement of the nodes (2_...:171, 179).
-__global__ void k_position(StateBufs S, Work W, double dt, int update_x, int nn, int nt) {
+// xv (XSPH): the XSPH sums of k_xsph; the positions then follow
+// XSPH_update (2_...:234-235), x = x0 + dt (v + eps sum), eps = 0.5.
+__global__ void k_position(StateBufs S, Work W, double dt, int update_x, int nn, int nt,
+                           const double2 *__restrict__ xv) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nt) return;
     double2 v0 = W.vx0[i], v = S.vel[i];
-    if (update_x) {
+    if (update_x && xv) {
+        const double eps = 0.5;
+        const double2 x0 = W.x0[i], s = xv[i];
+        const double2 xn = make_double2(x0.x + dt * (v.x + eps * s.x), x0.y + dt * (v.y + eps * s.y));
+        S.x[i] = xn;
+        if (i < nn) {
+            double2 x00 = S.x00[i];
+            S.displ[i] = make_double2(xn.x - x00.x, xn.y - x00.y);
+        }
+    } else if (update_x) {
         float hx = (float)(0.5 * (v0.x + v.x));
         float hy = (float)(0.5 * (v0.y + v.y));
         double2 x0 = W.x0[i];

@@ -36,6 +36,7 @@ PLAN = {
     "t2_cspm": ([1, 2, 50, 200], "all", True),
     "t3_elastic": ([1, 50, 200], "all", True),
     "c1_collapse": ([1, 2, 10, 100, 1000], [1000], False),
+    "t4_xsph": ([1, 2, 3, 10, 50, 200], "all", True),
 }
 
 STATE_FIELDS = ("x", "vel", "stress", "ivar1", "f_drucker", "ddev_strn", "grad_u", "displ",

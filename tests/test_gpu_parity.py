@@ -60,6 +60,7 @@ def compare_pairs(gpu, ora, tag):
     ("t1_tiny", [1, 2, 3, 10, 50, 200]),
     ("t2_cspm", [1, 2, 50, 200]),
     ("t3_elastic", [1, 50, 200]),
+    ("t4_xsph", [1, 2, 3, 10, 50, 200]),
 ])
 def test_small_decks_bitwise(deck_name, checkpoints):
     say(f"[gpu parity] deck={deck_name} checkpoints={checkpoints}")
@@ -92,7 +93,7 @@ def test_varied_smoothing_lengths_bitwise():
     ora.close()
 
 
-@pytest.mark.parametrize("deck_name", ["t1_tiny", "t2_cspm", "t3_elastic"])
+@pytest.mark.parametrize("deck_name", ["t1_tiny", "t2_cspm", "t3_elastic", "t4_xsph"])
 def test_small_decks_vs_reference(deck_name):
     """The GPU state against the reference Fortran's own dumps."""
     say(f"[gpu vs reference fixture] deck={deck_name}")

@@ -12,7 +12,8 @@ import pytest
 
 from conftest import bits_equal, deck_dir, describe_diff, golden, load_pkg
 
-SMALL = {"t1_tiny": [1, 2, 3, 10, 50, 200], "t2_cspm": [1, 2, 50, 200], "t3_elastic": [1, 50, 200]}
+SMALL = {"t1_tiny": [1, 2, 3, 10, 50, 200], "t2_cspm": [1, 2, 50, 200], "t3_elastic": [1, 50, 200],
+         "t4_xsph": [1, 2, 3, 10, 50, 200]}
 
 
 def _sim(deck_name, oracle_lib):

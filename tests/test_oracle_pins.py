@@ -216,11 +216,12 @@ def test_error_behaviour(oracle_lib):
     sps = load_pkg()
     deck = sps.read_deck(deck_dir("t1_tiny"))
     parts = sps.build_particles(deck)
-    prm = sps.make_params(deck, parts)
-    prm.xsph = 1
-    with pytest.raises(sps.SpsError) as e:
-        sps.Simulation(prm, oracle_lib)
-    assert e.value.code == -1
+    for name, mode in (("xsph", sps.MODE_FAST), ("art_stress", sps.MODE_ORDERED)):
+        prm = sps.make_params(deck, parts, mode=mode)
+        setattr(prm, name, 1)
+        with pytest.raises(sps.SpsError) as e:
+            sps.Simulation(prm, oracle_lib)
+        assert e.value.code == -1
     prm = sps.make_params(deck, parts)
     sim = sps.Simulation(prm, oracle_lib)
     sim.deck = deck
