@@ -52,6 +52,16 @@ class ListReader:
             out.extend(self._record())
         return out[:n]
 
+    def peek(self):
+        """The tokens of the next non-blank record, without consuming it."""
+        k = self.pos
+        while k < len(self.lines):
+            toks = _TOKEN.findall(self.lines[k])
+            if toks:
+                return toks
+            k += 1
+        return []
+
     def text(self):
         """READ(u,*) text: a character item (first token; a blank record
         supplies nothing and the read moves on, as for numbers)."""
@@ -170,14 +180,17 @@ class Deck:
     walls: list = None
 
 
-def read_deck(directory, layout="code"):
+def read_deck(directory, layout="auto"):
     """Read a deck directory the way the reference does.
 
-    layout: "code" (code/ and the vertical-slope / strain-localisation forks)
-    or "bui" (the Bui et al. 2008 fork with its extra ntype_solid field).
+    layout: "code" (code/ and the vertical-slope / strain-localisation forks),
+    "bui" (the Bui et al. 2008 fork with its extra ntype_solid field), or
+    "auto": "bui" when the material-type record holds three values, else
+    "code".  The layout also selects the fork's hot-path variant
+    (make_params: wall-force gate and CSPM determinant threshold).
     """
-    if layout not in ("code", "bui"):
-        raise DeckError("layout must be 'code' or 'bui'")
+    if layout not in ("code", "bui", "auto"):
+        raise DeckError("layout must be 'code', 'bui' or 'auto'")
     inp = ListReader(os.path.join(directory, "input.txt"))
     inp.text()                                   # 3_...:62
     name = inp.text()[:15]                       # character(15) problem_name
@@ -192,6 +205,8 @@ def read_deck(directory, layout="code"):
     dat.text()
     d.ndimn = dat.int1()
     dat.text()
+    if layout == "auto":
+        layout = d.layout = "bui" if len(dat.peek()) >= 3 else "code"
     if layout == "code":
         d.nstre, d.nmats = dat.ints(2)
     else:

@@ -102,11 +102,50 @@ def wall_dummies(walls, dx, dy):
     return np.concatenate(dxs), np.concatenate(dys), np.concatenate(hor), np.concatenate(wpos), ndummy2
 
 
+def inside_stress_points(xn, yn, ndivx, ndivy, npoints, x1, y1, dx, dy):
+    """Stress-point positions of the inside approach (3_...:541-605).
+
+    One point per cell sits at the cell centre.  With 2 to 4 points, each is
+    the mean of three corners of the cell (a lower-left triangle and its
+    complements), summed left to right and divided by 3 in double precision;
+    cells run x column outer, y row inner, and a cell's points are
+    consecutive.  The corner choices are the reference's, including the
+    repeated corner of the second point of 2 and of the first point of 4.
+    """
+    if npoints == 1:
+        si = np.repeat(np.arange(1, ndivx, dtype=np.float64), ndivy - 1)
+        sj = np.tile(np.arange(1, ndivy, dtype=np.float64), ndivx - 1)
+        return x1 - dx / 2. + si * dx, y1 - dx / 2. + sj * dy
+    X = xn.reshape(ndivx, ndivy)
+    Y = yn[:ndivy]                     # y of a node depends on its row only
+    a, b = X[:-1, :-1], X[:-1, 1:]     # column j: rows i, i + 1
+    c, d = X[1:, :-1], X[1:, 1:]       # column j + 1: rows i, i + 1
+    ya = np.broadcast_to(Y[:-1], a.shape)
+    yb = np.broadcast_to(Y[1:], a.shape)
+    corners = {
+        2: [((a, b, c), (ya, yb, ya)), ((b, c, c), (yb, ya, yb))],
+        3: [((a, b, c), (ya, yb, ya)), ((a, c, d), (ya, ya, yb)), None],
+        4: [((a, a, d), (ya, yb, ya)), ((b, c, d), (yb, ya, yb)), ((a, b, d), (ya, yb, yb)),
+            ((a, c, d), (ya, ya, yb))],
+    }[npoints]
+    xs = np.empty((ndivx - 1, ndivy - 1, npoints))
+    ys = np.empty_like(xs)
+    for p, cor in enumerate(corners):
+        if cor is None:                # third point of 3: two corners and the left edge's midpoint
+            xs[:, :, p] = (b + d + (c + a) / 2.) / 3.
+            ys[:, :, p] = (yb + yb + (ya + yb) / 2.) / 3.
+            continue
+        (p0, p1, p2), (q0, q1, q2) = cor
+        xs[:, :, p] = (p0 + p1 + p2) / 3.
+        ys[:, :, p] = (q0 + q1 + q2) / 3.
+    return xs.ravel(), ys.ravel()
+
+
 def build_particles(deck: Deck) -> Particles:
     if deck.geom_type != 1:
         raise DeckError("only geom_type 1 (rectangle) is implemented, as in the reference (3_...:388-390)")
-    if not (deck.sp_sph and deck.inside_approach and deck.npoints == 1):
-        raise DeckError("only the inside approach with one stress point per cell is supported")
+    if not (deck.sp_sph and deck.inside_approach and 1 <= deck.npoints <= 4):
+        raise DeckError("only the inside approach with 1 to 4 stress points per cell is supported")
     x1, _x2, _x3, x4 = deck.xs
     y1, _y2, _y3, y4 = deck.ys
     dx, dy = deck.dx, deck.dy
@@ -115,7 +154,7 @@ def build_particles(deck: Deck) -> Particles:
     ndivx = int(lx / dx + 1)          # integer truncation, 3_...:415-416
     ndivy = int(ly / dy + 1)
     nnode = ndivx * ndivy
-    nstress = (ndivx - 1) * (ndivy - 1)
+    nstress = deck.npoints * (ndivx - 1) * (ndivy - 1)      # 3_...:424-433
 
     # nodes: i (x) outer, j (y) inner, 3_...:498-514
     ii = np.repeat(np.arange(ndivx, dtype=np.float64), ndivy)
@@ -126,11 +165,7 @@ def build_particles(deck: Deck) -> Particles:
     corner = ((xn == x1) & ((yn == y1) | (yn == y4))) | ((xn == x4) & ((yn == y1) | (yn == y4)))
     no_int = np.where(corner, 2, no_int)
 
-    # stress points, inside approach npoints = 1: 3_...:541-549
-    si = np.repeat(np.arange(1, ndivx, dtype=np.float64), ndivy - 1)
-    sj = np.tile(np.arange(1, ndivy, dtype=np.float64), ndivx - 1)
-    xs = x1 - dx / 2. + si * dx
-    ys = y1 - dx / 2. + sj * dy
+    xs, ys = inside_stress_points(xn, yn, ndivx, ndivy, deck.npoints, x1, y1, dx, dy)
 
     rho0 = deck.props[5]
     hs = deck.sml * dx

@@ -1,7 +1,8 @@
 """Write the golden fixtures of the hot path from the compiled reference.
 
 Runs oracle/_ref/code/sps_ref_dump (the reference Fortran built from its own
-sources by `make -C oracle ref`) on the decks under
+sources by `make -C oracle ref`; oracle/_ref/bui/sps_ref_dump, the Bui et al.
+(2008) fork, for the decks in that fork's layout) on the decks under
 stress-particle-sph_amd/decks and reduces its dumps to small .npz fixtures:
 
   <deck>.npz  per dumped step s:
@@ -28,6 +29,7 @@ sys.path.insert(0, HERE)
 from refdump import read_pairs, read_state  # noqa: E402
 
 REF_BIN = os.path.join(ROOT, "oracle", "_ref", "code", "sps_ref_dump")
+REF_BIN_BUI = os.path.join(ROOT, "oracle", "_ref", "bui", "sps_ref_dump")
 DECKS = os.path.join(ROOT, "stress-particle-sph_amd", "decks")
 
 # deck: (steps to dump, steps stored in full, store full pair lists)
@@ -37,6 +39,10 @@ PLAN = {
     "t3_elastic": ([1, 50, 200], "all", True),
     "c1_collapse": ([1, 2, 10, 100, 1000], [1000], False),
     "t4_xsph": ([1, 2, 3, 10, 50, 200], "all", True),
+    # Bui et al. (2008) slope, inside approach, XSPH, 1-3 stress points per cell
+    "b1_bui_sp1": ([1, 2, 10, 100, 1000], [1, 10, 100], False, "bui"),
+    "b2_bui_sp2": ([1, 10, 200], [1, 200], False, "bui"),
+    "b3_bui_sp3": ([1, 10, 200], [1, 200], False, "bui"),
 }
 
 STATE_FIELDS = ("x", "vel", "stress", "ivar1", "f_drucker", "ddev_strn", "grad_u", "displ",
@@ -53,14 +59,15 @@ def pair_records(p):
 
 
 def run(deck):
-    steps, full, pairs_full = PLAN[deck]
+    steps, full, pairs_full = PLAN[deck][:3]
+    ref_bin = REF_BIN_BUI if PLAN[deck][3:] == ("bui",) else REF_BIN
     work = tempfile.mkdtemp(prefix=f"golden_{deck}_")
     try:
         for f in os.listdir(os.path.join(DECKS, deck)):
             shutil.copy(os.path.join(DECKS, deck, f), work)
         out = os.path.join(work, "out")
         os.makedirs(out)
-        cmd = f"ulimit -s unlimited && {REF_BIN} {out} {max(steps)} " + " ".join(map(str, steps))
+        cmd = f"ulimit -s unlimited && {ref_bin} {out} {max(steps)} " + " ".join(map(str, steps))
         subprocess.run(["bash", "-c", cmd], cwd=work, check=True, stdout=subprocess.DEVNULL)
         fx = {}
         s0 = read_state(os.path.join(out, "state_000000.bin"))

@@ -117,16 +117,19 @@ def test_small_decks_vs_reference(deck_name):
     gpu.close()
 
 
-def test_c1_collapse_1000_steps_vs_reference():
-    """granular column collapse, 1000 steps of dt = 1e-5,
-    GPU state digests against the reference Fortran's dumps."""
+@pytest.mark.parametrize("deck_name", ["c1_collapse", "b1_bui_sp1", "b2_bui_sp2", "b3_bui_sp3"])
+def test_digest_decks_vs_reference(deck_name):
+    """GPU state digests against the reference Fortran's dumps: the granular
+    column collapse (1000 steps of dt = 1e-5) and the Bui
+    et al. (2008) slope of that fork (XSPH, 1 to 3 stress points per cell)."""
     import hashlib
-    say("[gpu vs reference fixture] deck=c1_collapse 1000 steps")
-    g = golden("c1_collapse")
+    g = golden(deck_name)
+    steps = sorted(int(k[1:].split("_")[0]) for k in g.files if k.endswith("_sha_x") and k != "s0_sha_x")
+    say(f"[gpu vs reference fixture] deck={deck_name} steps={steps}")
     sps = load_pkg()
-    gpu = sps.Simulation.from_deck(sps.read_deck(deck_dir("c1_collapse")))
+    gpu = sps.Simulation.from_deck(sps.read_deck(deck_dir(deck_name)))
     done = 0
-    for st in (1, 2, 10, 100, 1000):
+    for st in steps:
         gpu.step(st - done)
         done = st
         s = gpu.get_state()
@@ -139,7 +142,13 @@ def test_c1_collapse_1000_steps_vs_reference():
                 if f"s{st}_{k}" in g.files:
                     msg += ": " + describe_diff(g[f"s{st}_{k}"], s[k])
                 pytest.fail(msg)
-        assert gpu.stats()["n_pairs"] == int(g[f"s{st}_npairs"])
+        p = gpu.get_pairs()
+        assert len(p["i"]) == int(g[f"s{st}_npairs"])
+        rec = np.zeros(len(p["i"]), dtype=[("i", "<i4"), ("j", "<i4"), ("type", "<i4"), ("w", "<f4"),
+                                           ("dwdx", "<f4"), ("dwdy", "<f4")])
+        for k in rec.dtype.names:
+            rec[k] = p[k]
+        assert hashlib.sha256(rec.tobytes()).hexdigest() == str(g[f"s{st}_sha_pairs"]), f"step {st} pairs"
     gpu.close()
 
 

@@ -56,17 +56,43 @@ def test_oracle_bitwise_small_decks(deck_name, oracle_lib):
     sim.close()
 
 
-def test_oracle_c1_collapse_1000_steps(oracle_lib):
-    """BASELINE config 1 (5,151 + 5,000 particles, 1000 steps of 1e-5 s)."""
-    g = golden("c1_collapse")
+# decks checked through per-field SHA-256 digests at every dumped step
+DIGEST = {"c1_collapse": "BASELINE config 1 (5,151 + 5,000 particles, 1000 steps of 1e-5 s)",
+          "b1_bui_sp1": "Bui et al. (2008) slope, XSPH, one stress point per cell, 1000 steps",
+          "b2_bui_sp2": "the same slope with two stress points per cell, 200 steps",
+          "b3_bui_sp3": "the same slope with three stress points per cell, 200 steps"}
+
+
+def fixture_steps(g):
+    return sorted(int(k[1:].split("_")[0]) for k in g.files if k.endswith("_sha_x") and k != "s0_sha_x")
+
+
+def pair_digest(p):
+    rec = np.zeros(len(p["i"]), dtype=[("i", "<i4"), ("j", "<i4"), ("type", "<i4"), ("w", "<f4"),
+                                       ("dwdx", "<f4"), ("dwdy", "<f4")])
+    for k in rec.dtype.names:
+        rec[k] = p[k]
+    return hashlib.sha256(rec.tobytes()).hexdigest()
+
+
+@pytest.mark.parametrize("deck_name", sorted(n for n in DIGEST if n != "c1_collapse"))
+def test_setup_matches_reference_other_forks(deck_name):
+    """The Bui fork's decks: its .dat layout, 1 to 3 stress points per cell."""
+    test_setup_matches_reference(deck_name)
+
+
+@pytest.mark.parametrize("deck_name", sorted(DIGEST))
+def test_oracle_digest_decks(deck_name, oracle_lib):
+    g = golden(deck_name)
     sps = load_pkg()
-    parts = sps.build_particles(sps.read_deck(deck_dir("c1_collapse")))
-    for k in ("x", "mass", "rho", "hsml", "itype", "wall_position", "horizontal_or_not"):
-        digest = hashlib.sha256(np.ascontiguousarray(getattr(parts, k)).tobytes()).hexdigest()
-        assert digest == str(g[f"s0_sha_{k}"]), f"setup {k}"
-    sim = _sim("c1_collapse", oracle_lib)
+    if "s0_sha_x" in g.files:
+        parts = sps.build_particles(sps.read_deck(deck_dir(deck_name)))
+        for k in ("x", "mass", "rho", "hsml", "itype", "wall_position", "horizontal_or_not"):
+            digest = hashlib.sha256(np.ascontiguousarray(getattr(parts, k)).tobytes()).hexdigest()
+            assert digest == str(g[f"s0_sha_{k}"]), f"setup {k}"
+    sim = _sim(deck_name, oracle_lib)
     done = 0
-    for st in (1, 2, 10, 100, 1000):
+    for st in fixture_steps(g):
         sim.step(st - done)
         done = st
         s = sim.get_state()
@@ -74,11 +100,12 @@ def test_oracle_c1_collapse_1000_steps(oracle_lib):
         for k in ("x", "vel", "stress", "ivar1", "f_drucker", "ddev_strn", "grad_u", "displ", "f_bound",
                   "art_visc"):
             digest = hashlib.sha256(np.ascontiguousarray(s[k]).tobytes()).hexdigest()
-            assert digest == str(g[f"s{st}_sha_{k}"]), f"step {st} field {k}"
+            if digest != str(g[f"s{st}_sha_{k}"]):
+                msg = f"step {st} field {k}"
+                if f"s{st}_{k}" in g.files:
+                    msg += ": " + describe_diff(g[f"s{st}_{k}"], s[k])
+                pytest.fail(msg)
         p = sim.get_pairs()
-        rec = np.zeros(len(p["i"]), dtype=[("i", "<i4"), ("j", "<i4"), ("type", "<i4"), ("w", "<f4"),
-                                           ("dwdx", "<f4"), ("dwdy", "<f4")])
-        for k in rec.dtype.names:
-            rec[k] = p[k]
-        assert hashlib.sha256(rec.tobytes()).hexdigest() == str(g[f"s{st}_sha_pairs"]), f"step {st} pairs"
+        assert len(p["i"]) == int(g[f"s{st}_npairs"])
+        assert pair_digest(p) == str(g[f"s{st}_sha_pairs"]), f"step {st} pairs"
     sim.close()
