@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define SPS_ABI_VERSION 1
+#define SPS_ABI_VERSION 2
 
 enum {
     SPS_OK = 0,
@@ -80,7 +80,9 @@ typedef struct sps_params {
     int32_t skf;             /* kernel: must be 1, cubic spline (2_...:1468) */
     /* control logicals (0/1), read at 3_...:286-299, 364-375 */
     int32_t sp_sph;          /* must be 1 */
-    int32_t inside_approach; /* must be 1 */
+    int32_t inside_approach; /* 1: inside approach; 0: outside approach, the
+                                stress points attached to their node and
+                                shifted with it (2_...:156-160, 244-370) */
     int32_t cspm;            /* CSPM gradient correction (2_...:539) */
     int32_t update_x;        /* position update + Jaumann spin (2_...:140, 751) */
     int32_t xsph;            /* XSPH position update (2_...:142-143, 189-239);
@@ -88,7 +90,9 @@ typedef struct sps_params {
     int32_t cont_density;    /* must be 0 */
     int32_t art_stress;      /* must be 0 */
     int32_t dummy_nodes;     /* wall dummies present (.pts) */
-    int32_t sph_shift;       /* must be 0 (outside approach only) */
+    int32_t sph_shift;       /* SPH_shift (3_...:373), outside approach only:
+                                stress_point_update + adapt_stress2 after the
+                                search on shift steps (2_...:99-109) */
     int32_t no_bcs;          /* must be 0 */
     int32_t bforce_gate;     /* 0: wall force when inside_approach .and.
                                 dummy_nodes (code/2_...:742); 1: when
@@ -109,6 +113,17 @@ typedef struct sps_params {
     double ft_grav;          /* gravity factor (3_...:308) */
     double tcurve_t[SPS_MAX_TCURVE];  /* ttcurves(tcurve_grav,:), real(DP) */
     float  tcurve_f[SPS_MAX_TCURVE];  /* ftcurves(tcurve_grav,:), default real */
+    /* the outside approach (read at 3_...:370-378, derived at 3_...:609);
+       ignored with the inside approach */
+    int32_t npoints;         /* stress points per node, 1..4 (2 with vel_vector):
+                                stress point nnode + npoints*(i-1) + p, p = 1..npoints,
+                                belongs to node i (3_...:611-660) */
+    int32_t vel_vector;      /* orient the two points along the node velocity
+                                (2_...:261-289) */
+    int32_t shift_update;    /* steps between shifts, >= 1 (3_...:373) */
+    int32_t reserved0;       /* must be 0 */
+    double r_x, r_y;         /* dx*rx_factor, dx*ry_factor (3_...:609) */
+    double disp_tol;         /* displacement tolerance in dx (vel_vector) */
 } sps_params;
 
 /*
@@ -136,6 +151,14 @@ typedef struct sps_state {
     double  *ddev_strn;         /* (ntotal)     out: Ddev_strn of the last step */
     double  *f_bound;           /* (2,nnode)    out: last stage wall force */
     double  *art_visc;          /* (2,nnode)    out: last stage art. viscosity */
+    double  *x_10;              /* (2,nnode)    in/out, outside approach with
+                                   vel_vector: node positions at the last shift
+                                   (NULL at sps_set_state: the nodes' x, as
+                                   3_...:1032 sets them) */
+    double  *disp_10;           /* (nnode)      in/out, the same: displacement
+                                   over the last shift interval in dx (NULL at
+                                   sps_set_state: 0).  With the inside approach
+                                   both are ignored and left untouched. */
 } sps_state;
 
 typedef struct sps_stats {

@@ -51,6 +51,9 @@ struct sps_handle {
     /* state (Fortran layouts, 0-based) */
     double *x, *vel, *stress, *mass, *rho, *hsml, *ivars, *fdr, *x00, *displ;
     double *grad_u, *ddev, *f_bound, *art_visc;
+    double *x10, *disp10;   /* outside approach, vel_vector (x_10, disp_10) */
+    float *n_int;           /* node-node pairs of each node (outside approach) */
+    int32_t *bc_int;        /* node has a node-dummy pair (outside approach) */
     int32_t *itype, *ood;
     float *wallpos, *horiz;
     /* start-of-step copies (2_...:114) */
@@ -113,6 +116,7 @@ static int alloc_all(struct sps_handle *h)
     A(dummy_vel, 2 * nt2, double); A(cspm_norm, nt, double);
     A(vel_tmp, 2 * nt, double); A(stress_tmp, 4 * nt, double); A(ivar_tmp, nt, double);
     A(av_tmp, 2 * nn, float);
+    A(x10, 2 * nn, double); A(disp10, nn, double); A(n_int, nn, float); A(bc_int, nn, int32_t);
 #undef A
     return 0;
 }
@@ -127,7 +131,7 @@ static void free_all(struct sps_handle *h)
                     h->vel0, h->stress0, h->rk_vel, h->rk_stress, h->rk_dev, h->rhs1,
                     h->rhs2, h->divf1, h->divf2, h->source_stress, h->g_grav, h->spin,
                     h->grad1, h->grad2, h->ae, h->dummy_vel, h->cspm_norm, h->vel_tmp,
-                    h->stress_tmp, h->ivar_tmp, h->av_tmp};
+                    h->stress_tmp, h->ivar_tmp, h->av_tmp, h->x10, h->disp10, h->n_int, h->bc_int};
     for (size_t k = 0; k < sizeof ptrs / sizeof ptrs[0]; ++k) free(ptrs[k]);
 }
 
@@ -900,6 +904,89 @@ static void xsph_update(struct sps_handle *h, double dt)
             h->x[2 * i + a] = h->x0[2 * i + a] + dt * (h->vel[2 * i + a] + eps * vs[2 * i + a]);
 }
 
+/* ---------------------------------------------------- shift_stress_points */
+/* code/2_SPH_main_2018.f90:244-370, the outside approach after the position
+ * update.  On shift steps the stress points of each node are placed again
+ * relative to it: along its velocity (vel_vector: two points at +-dx/2 in
+ * the direction of travel, components in (0, dx/5) pushed out by dx/3, and
+ * only when the node moved more than disp_tol*dx since the last shift), or
+ * at the fixed offsets r_x, r_y of 2_...:293-326 (whose second point of two
+ * takes r_x for its y offset).  Every step, the points of a node that has
+ * a wall-dummy pair and moves faster than 0.4, or has fewer than two node
+ * pairs, are put on the node (the first three points: 2_...:352-366).
+ * n_int and bc_int count the current list's type-3 and type-6 pairs, as
+ * artificial_viscosity / XSPH_update / isolated_nodes and get_derivatives
+ * leave them (2_...:221-222, 392-393, 579, 870-871). */
+static void shift_stress_points(struct sps_handle *h, int itimestep)
+{
+    const int nn = h->nn, np = h->p.npoints;
+    const double dx = h->p.dx;
+    double *x = h->x;
+    if (itimestep % h->p.shift_update == 0) {
+        if (h->p.vel_vector) {
+            for (int i = 0; i < nn; ++i) {
+                const double d0 = x[2 * i] - h->x10[2 * i], d1 = x[2 * i + 1] - h->x10[2 * i + 1];
+                h->disp10[i] = sqrt(d0 * d0 + d1 * d1);
+                h->disp10[i] = h->disp10[i] / dx;
+            }
+            memcpy(h->x10, x, 2 * (size_t)nn * sizeof(double));
+            for (int i = 0; i < nn; ++i) {
+                const double v0 = h->vel[2 * i], v1 = h->vel[2 * i + 1];
+                const float ct = (float)(v0 / sqrt(v0 * v0 + v1 * v1));
+                const float sn = (float)(v1 / sqrt(v0 * v0 + v1 * v1));
+                float r[2] = {(float)((dx / 2) * (double)ct), (float)((dx / 2) * (double)sn)};
+                for (int a = 0; a < 2; ++a)
+                    if (r[a] > 0.0f && (double)r[a] < dx / 5) r[a] = (float)((double)r[a] + dx / 3.0);
+                for (int a = 0; a < 2; ++a)
+                    if (r[a] < 0.0f && (double)r[a] > -(dx / 5)) r[a] = (float)((double)r[a] - dx / 3.0);
+                if (h->disp10[i] > h->p.disp_tol) {
+                    const int k = nn + 2 * i;
+                    for (int a = 0; a < 2; ++a) {
+                        x[2 * k + a] = x[2 * i + a] + (double)r[a];
+                        x[2 * (k + 1) + a] = x[2 * i + a] - (double)r[a];
+                    }
+                }
+            }
+        } else {
+            const double rx = h->p.r_x, ry = h->p.r_y;
+            for (int i = 0; i < nn && np <= 3; ++i) {
+                double *s = x + 2 * (size_t)(nn + np * i);
+                const double xi = x[2 * i], yi = x[2 * i + 1];
+                if (np == 1) {
+                    s[0] = xi + rx; s[1] = yi + ry;
+                } else if (np == 2) {
+                    s[0] = xi + rx; s[1] = yi + ry;
+                    s[2] = xi - rx; s[3] = yi - rx;
+                } else {
+                    s[0] = xi; s[1] = yi + ry;
+                    s[2] = xi - rx; s[3] = yi - ry;
+                    s[4] = xi + rx; s[5] = yi - ry;
+                }
+            }
+        }
+    }
+    memset(h->n_int, 0, (size_t)nn * sizeof(float));
+    memset(h->bc_int, 0, (size_t)nn * sizeof(int32_t));
+    for (int64_t k = 0; k < h->n_pairs; ++k) {
+        if (h->pt[k] == 3) {
+            h->n_int[h->pi[k]] = h->n_int[h->pi[k]] + 1.0f;
+            h->n_int[h->pj[k]] = h->n_int[h->pj[k]] + 1.0f;
+        } else if (h->pt[k] == 6) {
+            h->bc_int[h->pj[k]] = 1; /* pair_j: the node (pint_update) */
+        }
+    }
+    for (int i = 0; i < nn; ++i) {
+        const double v0 = h->vel[2 * i], v1 = h->vel[2 * i + 1];
+        const float abs_vel = (float)sqrt(v0 * v0 + v1 * v1);
+        const int onto = (h->bc_int[i] == 1 && abs_vel > 0.4f) || h->n_int[i] < 2.0f;
+        for (int p = 0; onto && p < np && p < 3; ++p) {
+            const int k = nn + np * i + p;
+            x[2 * k] = x[2 * i];
+            x[2 * k + 1] = x[2 * i + 1];
+        }
+    }
+}
+
 static int time_integration(struct sps_handle *h, int itimestep, double time_sph, double dt)
 {
     const int nn = h->nn, nt = h->nt;
@@ -908,6 +995,10 @@ static int time_integration(struct sps_handle *h, int itimestep, double time_sph
     int rc = grid_find(h, itimestep);
     if (rc) return rc;
     pint_update(h);
+    if (!h->p.inside_approach && h->p.sph_shift && itimestep > 1 && (itimestep - 1) % h->p.shift_update == 0) {
+        stress_point_update(h); /* 2_...:99-109 */
+        if (ncrit == 12) adapt_stress2(h);
+    }
     memcpy(h->vx0, h->vel, 2 * (size_t)nt * sizeof(double));
     memcpy(h->x0, h->x, 2 * (size_t)nt * sizeof(double));
     rk4(h, dt, time_sph);
@@ -917,6 +1008,7 @@ static int time_integration(struct sps_handle *h, int itimestep, double time_sph
     if (ncrit == 12) adapt_stress2(h);
     if (h->p.update_x && h->p.xsph) {
         xsph_update(h, dt);
+        if (!h->p.inside_approach) shift_stress_points(h, itimestep);
         for (int i = 0; i < nn; ++i)
             for (int a = 0; a < 2; ++a) h->displ[2 * i + a] = h->x[2 * i + a] - h->x00[2 * i + a];
     } else if (h->p.update_x) {
@@ -925,6 +1017,7 @@ static int time_integration(struct sps_handle *h, int itimestep, double time_sph
                 float vel_half = (float)(0.5 * (h->vx0[2 * i + a] + h->vel[2 * i + a]));
                 h->x[2 * i + a] = h->x0[2 * i + a] + (double)vel_half * dt;
             }
+        if (!h->p.inside_approach) shift_stress_points(h, itimestep);
         for (int i = 0; i < nn; ++i)
             for (int a = 0; a < 2; ++a) h->displ[2 * i + a] = h->x[2 * i + a] - h->x00[2 * i + a];
     } else {
@@ -945,9 +1038,18 @@ static int check_params(const sps_params *p)
     if (p->nint_vars < 1) return fail(SPS_EINVAL, "nint_vars must be >= 1");
     if (p->ndimn != 2 || p->nstre != 4) return fail(SPS_EINVAL, "only ndimn=2, nstre=4 (plane strain) are supported");
     if (p->skf != 1) return fail(SPS_EINVAL, "only the cubic spline kernel (skf=1) is supported");
-    if (!p->sp_sph || !p->inside_approach) return fail(SPS_EINVAL, "only stress-particle SPH with the inside approach is supported");
-    if (p->cont_density || p->art_stress || p->sph_shift || p->no_bcs)
-        return fail(SPS_EINVAL, "continuity density, artificial stress, stress-point shifting and prescribed BCs are not supported");
+    if (!p->sp_sph) return fail(SPS_EINVAL, "only stress-particle SPH (sp_sph = 1) is supported");
+    if (p->inside_approach && p->sph_shift) return fail(SPS_EINVAL, "sph_shift belongs to the outside approach");
+    if (!p->inside_approach) {
+        if (p->npoints < 1 || p->npoints > 4 || (long long)p->nstress != (long long)p->npoints * p->nnode)
+            return fail(SPS_EINVAL, "outside approach: npoints must be 1..4 and nstress = npoints * nnode");
+        if (p->vel_vector && p->npoints != 2) return fail(SPS_EINVAL, "vel_vector needs npoints = 2");
+        if (p->shift_update < 1) return fail(SPS_EINVAL, "shift_update must be >= 1");
+        if (p->mode != SPS_MODE_ORDERED) return fail(SPS_EINVAL, "the outside approach runs in SPS_MODE_ORDERED");
+    }
+    if (p->reserved0 != 0) return fail(SPS_EINVAL, "reserved0 must be 0");
+    if (p->cont_density || p->art_stress || p->no_bcs)
+        return fail(SPS_EINVAL, "continuity density, artificial stress and prescribed BCs are not supported");
     if (p->xsph && p->mode != SPS_MODE_ORDERED) return fail(SPS_EINVAL, "XSPH runs in SPS_MODE_ORDERED");
     int ntype_eco = (int)p->props[0], ncrit = (int)p->props[1];
     if (ntype_eco > 1 && ncrit <= 5) return fail(SPS_EINVAL, "Perzyna models (ncrit<=5) are not supported");
@@ -1006,6 +1108,10 @@ int sps_set_state(sps_handle *h, const sps_state *st)
         memcpy(h->wallpos, st->wall_position, n2 * sizeof(float));
         memcpy(h->horiz, st->horizontal_or_not, n2 * sizeof(float));
     }
+    if (st->x_10) memcpy(h->x10, st->x_10, 2 * (size_t)h->nn * sizeof(double));
+    else memcpy(h->x10, st->x, 2 * (size_t)h->nn * sizeof(double));
+    if (st->disp_10) memcpy(h->disp10, st->disp_10, (size_t)h->nn * sizeof(double));
+    else memset(h->disp10, 0, (size_t)h->nn * sizeof(double));
     if (bw_setup(h)) return fail(SPS_ENOMEM, "wall-dummy grid");
     h->have_state = 1;
     return 0;
@@ -1038,6 +1144,9 @@ int sps_get_state(sps_handle *h, sps_state *st)
     OUT(wall_position, wallpos, n2); OUT(horizontal_or_not, horiz, n2);
     OUT(grad_u, grad_u, 4 * n); OUT(ddev_strn, ddev, n);
     OUT(f_bound, f_bound, 2 * nn); OUT(art_visc, art_visc, 2 * nn);
+    if (!h->p.inside_approach) {
+        OUT(x_10, x10, 2 * nn); OUT(disp_10, disp10, nn);
+    }
 #undef OUT
     return 0;
 }

@@ -27,7 +27,7 @@ PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 PRODUCT_LIB = os.path.join(PKG_DIR, "libsps_hip.so")
 DECK_DIR = os.path.join(PKG_DIR, "decks")
 
-SPS_ABI_VERSION = 1
+SPS_ABI_VERSION = 2
 SPS_MAX_TCURVE = 64
 MODE_ORDERED = 0
 MODE_FAST = 1
@@ -61,7 +61,10 @@ class Params(ctypes.Structure):
                 ("xmin_domain", ctypes.c_double * 2), ("xmax_domain", ctypes.c_double * 2),
                 ("cgrav", ctypes.c_double * 2), ("ft_grav", ctypes.c_double),
                 ("tcurve_t", ctypes.c_double * SPS_MAX_TCURVE),
-                ("tcurve_f", ctypes.c_float * SPS_MAX_TCURVE)]
+                ("tcurve_f", ctypes.c_float * SPS_MAX_TCURVE),
+                ("npoints", ctypes.c_int32), ("vel_vector", ctypes.c_int32),
+                ("shift_update", ctypes.c_int32), ("reserved0", ctypes.c_int32),
+                ("r_x", ctypes.c_double), ("r_y", ctypes.c_double), ("disp_tol", ctypes.c_double)]
 
 
 _P = ctypes.c_void_p
@@ -71,7 +74,7 @@ class State(ctypes.Structure):
     _fields_ = [(n, _P) for n in ("x", "vel", "stress", "mass", "rho", "hsml", "itype",
                                    "if_out_domain", "internal_vars", "f_drucker", "x00",
                                    "displ", "wall_position", "horizontal_or_not", "grad_u",
-                                   "ddev_strn", "f_bound", "art_visc")]
+                                   "ddev_strn", "f_bound", "art_visc", "x_10", "disp_10")]
 
 
 class Stats(ctypes.Structure):
@@ -126,7 +129,11 @@ def make_params(deck: Deck, parts: Particles, mode=MODE_ORDERED, bforce_gate=Non
     p.sp_sph, p.inside_approach = int(deck.sp_sph), int(deck.inside_approach)
     p.cspm, p.update_x, p.xsph = int(deck.cspm), int(deck.update_x), int(deck.xsph)
     p.cont_density, p.art_stress = int(deck.cont_density), int(deck.art_stress)
-    p.dummy_nodes, p.sph_shift, p.no_bcs = int(deck.dummy_nodes), 0, deck.no_bcs
+    p.dummy_nodes, p.no_bcs = int(deck.dummy_nodes), deck.no_bcs
+    p.npoints = deck.npoints
+    if not deck.inside_approach:                # 3_...:371-378, 609
+        p.sph_shift, p.vel_vector, p.shift_update = int(deck.sph_shift), int(deck.vel_vector), deck.shift_update
+        p.r_x, p.r_y, p.disp_tol = deck.dx * deck.rx_factor, deck.dx * deck.ry_factor, deck.disp_tol
     bui = deck.layout == "bui"
     p.bforce_gate = int(bui) if bforce_gate is None else int(bforce_gate)
     p.mode = mode
@@ -167,7 +174,8 @@ def initial_state(parts: Particles, nint_vars=10):
         internal_vars=np.zeros((n, nint_vars)), f_drucker=np.zeros(n),
         x00=parts.x.copy(), displ=np.zeros((parts.nnode, 2)),
         wall_position=parts.wall_position.copy(),
-        horizontal_or_not=parts.horizontal_or_not.copy())
+        horizontal_or_not=parts.horizontal_or_not.copy(),
+        x_10=parts.x[:parts.nnode].copy(), disp_10=np.zeros(parts.nnode))
 
 
 _STATE_SHAPES = {  # name: (dtype, per-particle shape, count key)
@@ -180,6 +188,7 @@ _STATE_SHAPES = {  # name: (dtype, per-particle shape, count key)
     "wall_position": (np.float32, (), "n2"), "horizontal_or_not": (np.float32, (), "n2"),
     "grad_u": (np.float64, (4,), "n"), "ddev_strn": (np.float64, (), "n"),
     "f_bound": (np.float64, (2,), "nn"), "art_visc": (np.float64, (2,), "nn"),
+    "x_10": (np.float64, (2,), "nn"), "disp_10": (np.float64, (), "nn"),
 }
 
 

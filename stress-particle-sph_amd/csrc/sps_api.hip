@@ -145,6 +145,11 @@ struct sps_handle {
     DBuf<float4> list_wa, list_gxa, list_gya, list_gxb, list_gyb, list_wb;
     bool xsph = false;            // XSPH position update (list B also for stress points, with w)
     DBuf<double2> xsph_v;
+    // the outside approach: each node's stress points follow it
+    bool outside = false;
+    DBuf<int> inv, nodeinfo;      // slot of each original particle; per node slot n_int, bc_int
+    DBuf<double2> x10;            // x_10 and disp_10 by original node index
+    DBuf<double> disp10;
     DBuf<unsigned short> deg_a, deg_b;
     DBuf<unsigned char> tile_ok;
     DBuf<Tile> tiles;
@@ -200,7 +205,7 @@ struct sps_handle {
     PairLists lists() {
         return PairLists{list_ia.p,  list_ib.p,  list_wa.p, list_gxa.p,          list_gya.p,
                          list_gxb.p, list_gyb.p, xsph ? list_wb.p : nullptr, deg_a.p, deg_b.p,
-                         tile_ok.p,  ntile_n,    tiles.p,   plans.p};
+                         tile_ok.p,  ntile_n,    tiles.p,   plans.p, outside ? nodeinfo.p : nullptr};
     }
 };
 
@@ -208,13 +213,13 @@ struct sps_handle {
 enum KernelId {
     K_CHECK_OOD, K_BOUNDS, K_CELL_COUNT, K_CELL_SCATTER, K_CELL_SORT, K_PERMUTE, K_PROC,
     K_COUNT_PAIRS, K_SCAN, K_SPLIT, K_RK_INIT, K_STAGE_BEGIN, K_SPU_NODES,
-    K_SP_UPDATE, K_DERIV_NODES, K_DUMMY_STRESS, K_RK_FINAL, K_POSITION, K_SURROGATE, K_XSPH, K_NKERNELS
+    K_SP_UPDATE, K_DERIV_NODES, K_DUMMY_STRESS, K_RK_FINAL, K_POSITION, K_SURROGATE, K_XSPH, K_SHIFT, K_NKERNELS
 };
 static const char *const kKernelNames[K_NKERNELS] = {
     "k_check_ood", "k_bounds", "k_cell_count", "k_cell_scatter", "k_cell_sort", "k_permute",
     "k_proc_slot", "k_count", "k_scan", "k_split", "k_rk_init",
     "k_stage_begin", "k_spu_nodes", "k_sp_update", "k_deriv_nodes",
-    "k_dummy_stress", "k_rk_final", "k_position", "k_surrogate", "k_xsph"};
+    "k_dummy_stress", "k_rk_final", "k_position", "k_surrogate", "k_xsph", "k_shift_sp"};
 
 static hipEvent_t prof_begin(sps_handle *h) {
     if (!h->prof) return nullptr;
@@ -296,9 +301,18 @@ static int check_params(const sps_params *p) {
     if (p->nint_vars < 1) return fail(SPS_EINVAL, "nint_vars must be >= 1");
     if (p->ndimn != 2 || p->nstre != 4) return fail(SPS_EINVAL, "only ndimn=2, nstre=4 (plane strain) are supported");
     if (p->skf != 1) return fail(SPS_EINVAL, "only the cubic spline kernel (skf=1) is supported");
-    if (!p->sp_sph || !p->inside_approach) return fail(SPS_EINVAL, "only stress-particle SPH with the inside approach is supported");
-    if (p->cont_density || p->art_stress || p->sph_shift || p->no_bcs)
-        return fail(SPS_EINVAL, "continuity density, artificial stress, stress-point shifting and prescribed BCs are not supported");
+    if (!p->sp_sph) return fail(SPS_EINVAL, "only stress-particle SPH (sp_sph = 1) is supported");
+    if (p->inside_approach && p->sph_shift) return fail(SPS_EINVAL, "sph_shift belongs to the outside approach");
+    if (!p->inside_approach) {
+        if (p->npoints < 1 || p->npoints > 4 || (long long)p->nstress != (long long)p->npoints * p->nnode)
+            return fail(SPS_EINVAL, "outside approach: npoints must be 1..4 and nstress = npoints * nnode");
+        if (p->vel_vector && p->npoints != 2) return fail(SPS_EINVAL, "vel_vector needs npoints = 2");
+        if (p->shift_update < 1) return fail(SPS_EINVAL, "shift_update must be >= 1");
+        if (p->mode != SPS_MODE_ORDERED) return fail(SPS_EINVAL, "the outside approach runs in SPS_MODE_ORDERED");
+    }
+    if (p->reserved0 != 0) return fail(SPS_EINVAL, "reserved0 must be 0");
+    if (p->cont_density || p->art_stress || p->no_bcs)
+        return fail(SPS_EINVAL, "continuity density, artificial stress and prescribed BCs are not supported");
     if (p->xsph && p->mode != SPS_MODE_ORDERED) return fail(SPS_EINVAL, "XSPH runs in SPS_MODE_ORDERED");
     int ntype_eco = (int)p->props[0], ncrit = (int)p->props[1];
     if (ntype_eco > 1 && ncrit <= 5) return fail(SPS_EINVAL, "Perzyna models (ncrit<=5) are not supported");
@@ -349,6 +363,7 @@ extern "C" int sps_create(const sps_params *params, sps_handle **out) {
     h->niv = params->nint_vars;
     h->mixed = params->mode == SPS_MODE_FAST;
     h->xsph = params->xsph != 0;
+    h->outside = params->inside_approach == 0;
     (void)hipGetDevice(&h->device);
     DevParams &d = h->dp;
     d.nn = h->nn; d.ns = h->ns; d.nd = h->nd; d.nd2 = h->nd2; d.nt = h->nt; d.nt2 = h->nt2;
@@ -404,6 +419,10 @@ extern "C" int sps_create(const sps_params *params, sps_handle **out) {
     rc |= dalloc(h->tiles, h->ntile_n + h->ntile_s + 1);
     rc |= dalloc(h->plans, 3 * (size_t)(h->ntile_n + h->ntile_s + 1));
     rc |= dalloc(h->x_build, nt); rc |= dalloc(h->dmax, 1); rc |= dalloc(h->stepv, 4);
+    if (h->outside) {
+        rc |= dalloc(h->inv, nt2); rc |= dalloc(h->nodeinfo, nn);
+        rc |= dalloc(h->x10, nn); rc |= dalloc(h->disp10, nn);
+    }
     if (rc) return bail(rc);
     if (hipHostMalloc((void **)&h->pinned, 64 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
         return bail(fail(SPS_ENOMEM, "pinned staging allocation"));
@@ -481,6 +500,13 @@ extern "C" int sps_set_state(sps_handle *h, const sps_state *st) {
     HIP_TRY(hipMemcpyAsync(S.fdr.p, st->f_drucker, nt * sizeof(double), hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(S.x00.p, st->x00, nt2 * sizeof(double2), hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(S.displ.p, st->displ, nn * sizeof(double2), hipMemcpyHostToDevice, s));
+    if (h->outside) {   // x_10 and disp_10, by original node index
+        HIP_TRY(hipMemcpyAsync(h->x10.p, st->x_10 ? st->x_10 : st->x, nn * sizeof(double2), hipMemcpyHostToDevice, s));
+        if (st->disp_10)
+            HIP_TRY(hipMemcpyAsync(h->disp10.p, st->disp_10, nn * sizeof(double), hipMemcpyHostToDevice, s));
+        else
+            HIP_TRY(hipMemsetAsync(h->disp10.p, 0, nn * sizeof(double), s));
+    }
     h->ivars_host.assign(st->internal_vars, st->internal_vars + (size_t)h->niv * nt);
     for (int i = 0; i < nt; ++i) tmp[i] = st->internal_vars[(size_t)h->niv * i];
     HIP_TRY(hipMemcpyAsync(S.ivar1.p, tmp.data(), nt * sizeof(double), hipMemcpyHostToDevice, s));
@@ -546,6 +572,7 @@ static int search(sps_handle *h, bool fresh_list) {
     HIP_TRY(hipGetLastError());
     h->cur = 1 - h->cur;
     StateSet &C = h->S[h->cur];
+    if (h->outside) LAUNCH(K_PERMUTE, nblk(nt2), BLOCK, k_inv_slots, C.orig.p, h->inv.p, nt2);
     LAUNCH(K_PROC, nblk(ndivt + 1), BLOCK, k_cell_totals, h->kstart.p, h->grid.p, h->ctot.p);
     rc = scan_u32(h, h->ctot.p, h->cstart_rm.p, ndivt + 1, nullptr);
     if (rc) return rc;
@@ -742,6 +769,14 @@ static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
     const int adapt = h->dp.ncrit == 12;
     const double tanfi = h->dp.tanfi, coh = h->dp.coh;
     const double f1rk[4] = {0., 0.5, 0.5, 1.0}, f2rk[4] = {1., 2., 2., 1.0};
+    if (h->outside && h->p.sph_shift && itimestep > 1 && (itimestep - 1) % h->p.shift_update == 0) {
+        // stress_point_update + adapt_stress2 on the new lists (2_...:99-109)
+        if (nn > 0)
+            LAUNCH_LDS(K_SPU_NODES, nblk8(nn), TILE, LAY_SPU_NODES.bytes, k_spu_nodes, SB, V, PL, adapt, tanfi, coh, nn);
+        if (nt > nn)
+            LAUNCH_LDS(K_SP_UPDATE, nblk8(nt - nn), TILE, LAY_SP_UPDATE.bytes, k_sp_update, SB, W, V, PL, h->dp,
+                       time_sph, 0.0, 0, adapt, nn, nt);
+    }
     LAUNCH(K_RK_INIT, nblk(nt2), BLOCK, k_rk_init, SB, W, nn, nt, nt2);
     for (int st = 0; st < 4; ++st) {
         const double c1 = f1rk[st] * (dt);
@@ -771,6 +806,11 @@ static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
     if (xsph && nt > nn)
         LAUNCH_LDS(K_XSPH, nblk8(nt - nn), TILE, LAY_XSPH.bytes, (k_xsph<KIND_SP>), SB, V, PL, h->xsph_v.p);
     LAUNCH(K_POSITION, nblk(nt), BLOCK, k_position, SB, W, dt, h->p.update_x, nn, nt, xsph ? h->xsph_v.p : nullptr);
+    if (h->outside && h->p.update_x && nn > 0) {
+        const ShiftParams sp{h->p.npoints, h->p.vel_vector, itimestep % h->p.shift_update == 0 ? 1 : 0,
+                             h->p.r_x, h->p.r_y, h->p.disp_tol, h->p.dx};
+        LAUNCH(K_SHIFT, nblk(nn), BLOCK, k_shift_sp, SB, h->inv.p, h->nodeinfo.p, h->x10.p, h->disp10.p, sp, nn);
+    }
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -838,6 +878,13 @@ extern "C" int sps_get_state(sps_handle *h, sps_state *st) {
     if (st->f_bound) rc = rc ? rc : fetch(h, h->f_bound.p, orig, nn, (double2 *)st->f_bound, k_gather_out2);
     if (st->art_visc) rc = rc ? rc : fetch(h, h->art_visc.p, orig, nn, (double2 *)st->art_visc, k_gather_out2);
     if (rc) return rc;
+    if (h->outside && (st->x_10 || st->disp_10)) {
+        if (st->x_10)
+            HIP_TRY(hipMemcpyAsync(st->x_10, h->x10.p, nn * sizeof(double2), hipMemcpyDeviceToHost, h->stream));
+        if (st->disp_10)
+            HIP_TRY(hipMemcpyAsync(st->disp_10, h->disp10.p, nn * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+    }
     if (st->internal_vars) {
         std::vector<double> iv(nt);
         rc = fetch(h, S.ivar1.p, orig, nt, iv.data(), k_gather_out1);

@@ -1065,6 +1065,10 @@ struct PairLists {
     // round trip instead of deriving them again through dependent loads
     Tile *tiles;                    // per tile
     StagePlan *plans;               // per tile and kind
+    // per node slot (the outside approach; null otherwise): 2 * its node
+    // partners + 1 if it has a wall-dummy partner, i.e. n_int and bc_int of
+    // the reference (2_...:579, 870-871), from a search without skin
+    int *nodeinfo;
 };
 
 // Wave `wave` of the block copies the search's plan of kind K for tile
@@ -1209,6 +1213,8 @@ __global__ __launch_bounds__(TILE) void k_count_lists(SearchView V, const double
         __syncthreads();
     }
     unsigned fk[3] = {0, 0, 0};   // pairs this receiver discovers, by partner kind
+    unsigned n_node = 0;          // node receivers: node partners (all, not only discovered)
+    bool any_dummy = false;       // node receivers: a wall-dummy partner
     IdxWriter la, lb;
     const bool active = r < T.end && V.cell[r] < C.g.ndivt;
     if (active) {
@@ -1256,6 +1262,10 @@ __global__ __launch_bounds__(TILE) void k_count_lists(SearchView V, const double
                                         continue;
                                 }
                                 fk[ks] += (kind_after || (slot_after && j > self)) ? 1u : 0u;
+                                if constexpr (KR == KIND_NODE) {
+                                    n_node += ks == KIND_NODE ? 1u : 0u;
+                                    any_dummy = any_dummy || ks == KIND_DUMMY;
+                                }
                             }
                             if (!lists) continue;
                             const unsigned e = (unsigned)((ks << 14) | j);
@@ -1306,6 +1316,7 @@ __global__ __launch_bounds__(TILE) void k_count_lists(SearchView V, const double
     if (r < T.end) {
         PL.deg_a[r] = (unsigned short)(la.n < LMAX ? la.n : LMAX);
         if (list_b) PL.deg_b[r] = (unsigned short)(lb.n < LMAX ? lb.n : LMAX);
+        if (KR == KIND_NODE && PL.nodeinfo) PL.nodeinfo[r] = (int)(2u * n_node + (any_dummy ? 1u : 0u));
     }
 }
 
@@ -2116,6 +2127,90 @@ __global__ void k_position(StateBufs S, Work W, double dt, int update_x, int nn,
         double2 d = S.displ[i];
         S.displ[i] = make_double2(d.x + 0.5 * (v0.x + v.x) * dt, d.y + 0.5 * (v0.y + v.y) * dt);
     }
+}
+
+// Slot of each original particle: the outside approach writes a node's
+// stress points by their original indices.
+__global__ void k_inv_slots(const int *__restrict__ orig, int *__restrict__ inv, int nt2) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nt2) inv[orig[t]] = t;
+}
+
+struct ShiftParams {
+    int npoints, vel_vector, shift_now;
+    double r_x, r_y, disp_tol, dx;
+};
+
+// A velocity-vector offset component in (0, dx/5) or (-dx/5, 0) pushed out
+// by dx/3, in the reference's mixed single / double precision (2_...:279-282).
+__device__ __forceinline__ float push_out(float r, double dx) {
+    if (r > 0.0f && (double)r < dx / 5) r = (float)((double)r + dx / 3.0);
+    if (r < 0.0f && (double)r > -(dx / 5)) r = (float)((double)r - dx / 3.0);
+    return r;
+}
+
+// shift_stress_points (2_...:244-370), the outside approach after the
+// position update; one lane per node slot, writing the node's stress points
+// nn + npoints * i + p (i its original index) through inv.  On shift steps
+// (shift_now) the points are placed along the node velocity (vel_vector:
+// +-dx/2 (cos, sin) in single precision, pushed out of (-dx/5, dx/5), only
+// when the node moved more than disp_tol dx since the last shift; x_10 and
+// disp_10 are kept by original index) or at the fixed offsets of 2_...:293-326
+// (the second of two points takes r_x for its y offset, npoints 4 has no
+// pattern there).  Every step, the first three points of a node with a
+// wall-dummy partner moving faster than 0.4 (speed in single precision), or
+// with fewer than two node partners, go onto the node (2_...:346-366).
+__global__ void k_shift_sp(StateBufs S, const int *__restrict__ inv, const int *__restrict__ nodeinfo,
+                           double2 *__restrict__ x10, double *__restrict__ disp10, ShiftParams P, int nn) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nn) return;
+    const int i = S.orig[r];
+    const double2 xn = S.x[r], v = S.vel[r];
+    const int np = P.npoints;
+    double2 q0 = xn, q1 = xn, q2 = xn;
+    bool put = false;
+    if (P.shift_now) {
+        if (P.vel_vector) {
+            const double2 o = x10[i];
+            const double d0 = xn.x - o.x, d1 = xn.y - o.y;
+            double d10 = sqrt(d0 * d0 + d1 * d1);
+            d10 = d10 / P.dx;
+            disp10[i] = d10;
+            x10[i] = xn;
+            const float ct = (float)(v.x / sqrt(v.x * v.x + v.y * v.y));
+            const float sn = (float)(v.y / sqrt(v.x * v.x + v.y * v.y));
+            const float rx = push_out((float)((P.dx / 2) * (double)ct), P.dx);
+            const float ry = push_out((float)((P.dx / 2) * (double)sn), P.dx);
+            if (d10 > P.disp_tol) {
+                q0 = make_double2(xn.x + (double)rx, xn.y + (double)ry);
+                q1 = make_double2(xn.x - (double)rx, xn.y - (double)ry);
+                put = true;
+            }
+        } else if (np == 1) {
+            q0 = make_double2(xn.x + P.r_x, xn.y + P.r_y);
+            put = true;
+        } else if (np == 2) {
+            q0 = make_double2(xn.x + P.r_x, xn.y + P.r_y);
+            q1 = make_double2(xn.x - P.r_x, xn.y - P.r_x);
+            put = true;
+        } else if (np == 3) {
+            q0 = make_double2(xn.x, xn.y + P.r_y);
+            q1 = make_double2(xn.x - P.r_x, xn.y - P.r_y);
+            q2 = make_double2(xn.x + P.r_x, xn.y - P.r_y);
+            put = true;
+        }
+    }
+    const int info = nodeinfo[r];
+    const float abs_vel = (float)sqrt(v.x * v.x + v.y * v.y);
+    if (((info & 1) && abs_vel > 0.4f) || (float)(info >> 1) < 2.0f) {
+        q0 = q1 = q2 = xn;
+        put = true;
+    }
+    if (!put) return;
+    const int base = nn + np * i;
+    S.x[inv[base]] = q0;
+    if (np > 1) S.x[inv[base + 1]] = q1;
+    if (np > 2) S.x[inv[base + 2]] = q2;
 }
 
 // --------------------------------------------------------------- pair dump

@@ -130,6 +130,13 @@ class Deck:
     particle_shift: bool = False
     inside_approach: bool = True
     npoints: int = 1
+    # the outside approach (3_...:371-373)
+    sph_shift: bool = False
+    vel_vector: bool = False
+    shift_update: int = 1
+    rx_factor: float = 0.0
+    ry_factor: float = 0.0
+    disp_tol: float = 0.0
     sml: float = 1.2
     dt: float = 0.0
     time_end: float = 0.0
@@ -193,7 +200,9 @@ def read_deck(directory, layout="auto"):
         raise DeckError("layout must be 'code', 'bui' or 'auto'")
     inp = ListReader(os.path.join(directory, "input.txt"))
     inp.text()                                   # 3_...:62
-    name = inp.text()[:15]                       # character(15) problem_name
+    # character(15) problem_name; character(10) in the Bui fork (its 5_...:115),
+    # whose decks' names "auto" therefore also needs within 10 characters
+    name = inp.text()[:10 if layout == "bui" else 15]
     d = Deck(directory=directory, problem_name=name, layout=layout)
     dat = ListReader(os.path.join(directory, name + ".dat"))
     pts = ListReader(os.path.join(directory, name + ".pts"))
@@ -233,9 +242,15 @@ def read_deck(directory, layout="auto"):
         d.inside_approach = inp.logicals(1)[0]
         inp.text()
         d.npoints = inp.int1()
-        if not d.inside_approach:
-            raise DeckError("the outside approach (stress-point shifting) is not supported")
-    else:
+        if not d.inside_approach:                # 3_...:371-374
+            inp.text()
+            v = inp.values(6)
+            d.sph_shift, d.vel_vector = _logical(v[0], inp.path), _logical(v[1], inp.path)
+            d.shift_update = int(v[2])
+            d.rx_factor, d.ry_factor, d.disp_tol = _real(v[3]), _real(v[4]), _real(v[5])
+    if d.vel_vector:                             # 3_...:377-378
+        d.npoints = 2
+    if not d.sp_sph:
         d.npoints = 1
     # .pts geometry (3_...:383-405)
     pts.text()

@@ -10,7 +10,7 @@ module sps_binding
   use, intrinsic :: iso_c_binding
   implicit none
 
-  integer(c_int32_t), parameter :: SPS_ABI_VERSION = 1
+  integer(c_int32_t), parameter :: SPS_ABI_VERSION = 2
   integer, parameter :: SPS_MAX_TCURVE = 64
   integer(c_int), parameter :: SPS_OK = 0
   integer(c_int32_t), parameter :: SPS_MODE_ORDERED = 0, SPS_MODE_FAST = 1
@@ -26,6 +26,8 @@ module sps_binding
      real(c_double) :: xmin_domain(2), xmax_domain(2), cgrav(2), ft_grav
      real(c_double) :: tcurve_t(SPS_MAX_TCURVE)
      real(c_float) :: tcurve_f(SPS_MAX_TCURVE)
+     integer(c_int32_t) :: npoints, vel_vector, shift_update, reserved0
+     real(c_double) :: r_x, r_y, disp_tol
   end type sps_params_t
 
   type, bind(C) :: sps_state_t
@@ -37,6 +39,7 @@ module sps_binding
      type(c_ptr) :: wall_position = c_null_ptr, horizontal_or_not = c_null_ptr
      type(c_ptr) :: grad_u = c_null_ptr, ddev_strn = c_null_ptr
      type(c_ptr) :: f_bound = c_null_ptr, art_visc = c_null_ptr
+     type(c_ptr) :: x_10 = c_null_ptr, disp_10 = c_null_ptr
   end type sps_state_t
 
   type, bind(C) :: sps_stats_t

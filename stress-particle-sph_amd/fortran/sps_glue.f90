@@ -50,7 +50,7 @@ contains
   subroutine sps_glue_sync
     if (.not. c_associated(handle)) return
     call get_arrays(x, vel, stress, internal_vars, f_drucker, displ, If_Out_Domain, grad_u, &
-                    Ddev_strn, f_bound, art_visc)
+                    Ddev_strn, f_bound, art_visc, x_10, disp_10, .not. inside_approach)
   end subroutine sps_glue_sync
 
   subroutine sps_glue_finish
@@ -71,7 +71,14 @@ contains
     p%sp_sph = merge(1, 0, SP_SPH); p%inside_approach = merge(1, 0, inside_approach)
     p%cspm = merge(1, 0, cspm); p%update_x = merge(1, 0, update_x); p%xsph = merge(1, 0, XSPH)
     p%cont_density = merge(1, 0, cont_density); p%art_stress = merge(1, 0, art_stress)
-    p%dummy_nodes = merge(1, 0, dummy_nodes); p%sph_shift = merge(1, 0, SPH_shift)
+    p%dummy_nodes = merge(1, 0, dummy_nodes)
+    ! the outside approach's controls are read only for it (3_...:371-378)
+    p%sph_shift = 0; p%npoints = npoints; p%vel_vector = 0; p%shift_update = 1; p%reserved0 = 0
+    p%r_x = 0.0_c_double; p%r_y = 0.0_c_double; p%disp_tol = 0.0_c_double
+    if (.not. inside_approach) then
+       p%sph_shift = merge(1, 0, SPH_shift); p%vel_vector = merge(1, 0, vel_vector)
+       p%shift_update = shift_update; p%r_x = r_x; p%r_y = r_y; p%disp_tol = disp_tol
+    end if
     p%no_bcs = no_bcs
     p%bforce_gate = sps_glue_fork
     p%mode = SPS_MODE_ORDERED
@@ -100,22 +107,24 @@ contains
     call sps_check(sps_create(p, handle), 'sps_create')
     if (dummy_nodes) then
        call set_arrays(x, vel, stress, mass, rho, hsml, itype, If_Out_Domain, internal_vars, &
-                       f_drucker, x00, displ, wall_position, horizontal_or_not, .true.)
+                       f_drucker, x00, displ, wall_position, horizontal_or_not, .true., &
+                       x_10, disp_10, .not. inside_approach)
     else
        call set_arrays(x, vel, stress, mass, rho, hsml, itype, If_Out_Domain, internal_vars, &
-                       f_drucker, x00, displ, no_walls, no_walls, .false.)
+                       f_drucker, x00, displ, no_walls, no_walls, .false., &
+                       x_10, disp_10, .not. inside_approach)
     end if
   end subroutine open_handle
 
   ! The module arrays arrive as assumed-size TARGET dummies, so their
   ! addresses are valid for the duration of the C call made here.
   subroutine set_arrays(x_, vel_, stress_, mass_, rho_, hsml_, itype_, ood_, ivars_, fdr_, x00_, &
-                        displ_, wall_, horiz_, with_walls)
+                        displ_, wall_, horiz_, with_walls, x10_, d10_, outside)
     real(DP), target, intent(in) :: x_(*), vel_(*), stress_(*), mass_(*), rho_(*), hsml_(*)
-    real(DP), target, intent(in) :: ivars_(*), fdr_(*), x00_(*), displ_(*)
+    real(DP), target, intent(in) :: ivars_(*), fdr_(*), x00_(*), displ_(*), x10_(*), d10_(*)
     integer, target, intent(in) :: itype_(*), ood_(*)
     real, target, intent(in) :: wall_(*), horiz_(*)
-    logical, intent(in) :: with_walls
+    logical, intent(in) :: with_walls, outside
     type(sps_state_t) :: st
     st%x = c_loc(x_); st%vel = c_loc(vel_); st%stress = c_loc(stress_)
     st%mass = c_loc(mass_); st%rho = c_loc(rho_); st%hsml = c_loc(hsml_)
@@ -125,18 +134,26 @@ contains
     if (with_walls) then
        st%wall_position = c_loc(wall_); st%horizontal_or_not = c_loc(horiz_)
     end if
+    if (outside) then
+       st%x_10 = c_loc(x10_); st%disp_10 = c_loc(d10_)
+    end if
     call sps_check(sps_set_state(handle, st), 'sps_set_state')
   end subroutine set_arrays
 
-  subroutine get_arrays(x_, vel_, stress_, ivars_, fdr_, displ_, ood_, gradu_, ddev_, fb_, av_)
+  subroutine get_arrays(x_, vel_, stress_, ivars_, fdr_, displ_, ood_, gradu_, ddev_, fb_, av_, &
+                        x10_, d10_, outside)
     real(DP), target, intent(inout) :: x_(*), vel_(*), stress_(*), ivars_(*), fdr_(*), displ_(*)
-    real(DP), target, intent(inout) :: gradu_(*), ddev_(*), fb_(*), av_(*)
+    real(DP), target, intent(inout) :: gradu_(*), ddev_(*), fb_(*), av_(*), x10_(*), d10_(*)
+    logical, intent(in) :: outside
     integer, target, intent(inout) :: ood_(*)
     type(sps_state_t) :: st
     st%x = c_loc(x_); st%vel = c_loc(vel_); st%stress = c_loc(stress_)
     st%internal_vars = c_loc(ivars_); st%f_drucker = c_loc(fdr_); st%displ = c_loc(displ_)
     st%if_out_domain = c_loc(ood_); st%grad_u = c_loc(gradu_); st%ddev_strn = c_loc(ddev_)
     st%f_bound = c_loc(fb_); st%art_visc = c_loc(av_)
+    if (outside) then
+       st%x_10 = c_loc(x10_); st%disp_10 = c_loc(d10_)
+    end if
     call sps_check(sps_get_state(handle, st), 'sps_get_state')
   end subroutine get_arrays
 

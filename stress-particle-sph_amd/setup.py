@@ -141,11 +141,25 @@ def inside_stress_points(xn, yn, ndivx, ndivy, npoints, x1, y1, dx, dy):
     return xs.ravel(), ys.ravel()
 
 
+def outside_stress_points(xn, yn, npoints, r_x, r_y):
+    """Stress points of the outside approach (3_...:607-660): npoints per
+    node at the offsets (+-r_x, +-r_y) of the reference's patterns, a node's
+    points consecutive."""
+    pat = {1: [(1, 1)], 2: [(1, 1), (-1, -1)], 3: [(0, 1), (-1, -1), (1, -1)],
+           4: [(-1, -1), (-1, 1), (1, 1), (1, -1)]}[npoints]
+    xs = np.empty((len(xn), npoints))
+    ys = np.empty_like(xs)
+    for p, (sx, sy) in enumerate(pat):
+        xs[:, p] = xn if sx == 0 else (xn + r_x if sx > 0 else xn - r_x)
+        ys[:, p] = yn + r_y if sy > 0 else yn - r_y
+    return xs.ravel(), ys.ravel()
+
+
 def build_particles(deck: Deck) -> Particles:
     if deck.geom_type != 1:
         raise DeckError("only geom_type 1 (rectangle) is implemented, as in the reference (3_...:388-390)")
-    if not (deck.sp_sph and deck.inside_approach and 1 <= deck.npoints <= 4):
-        raise DeckError("only the inside approach with 1 to 4 stress points per cell is supported")
+    if not (deck.sp_sph and 1 <= deck.npoints <= 4):
+        raise DeckError("only stress-particle SPH with 1 to 4 stress points per cell or node is supported")
     x1, _x2, _x3, x4 = deck.xs
     y1, _y2, _y3, y4 = deck.ys
     dx, dy = deck.dx, deck.dy
@@ -154,7 +168,10 @@ def build_particles(deck: Deck) -> Particles:
     ndivx = int(lx / dx + 1)          # integer truncation, 3_...:415-416
     ndivy = int(ly / dy + 1)
     nnode = ndivx * ndivy
-    nstress = deck.npoints * (ndivx - 1) * (ndivy - 1)      # 3_...:424-433
+    if deck.inside_approach:                    # 3_...:424-438
+        nstress = deck.npoints * (ndivx - 1) * (ndivy - 1)
+    else:
+        nstress = deck.npoints * nnode
 
     # nodes: i (x) outer, j (y) inner, 3_...:498-514
     ii = np.repeat(np.arange(ndivx, dtype=np.float64), ndivy)
@@ -165,13 +182,19 @@ def build_particles(deck: Deck) -> Particles:
     corner = ((xn == x1) & ((yn == y1) | (yn == y4))) | ((xn == x4) & ((yn == y1) | (yn == y4)))
     no_int = np.where(corner, 2, no_int)
 
-    xs, ys = inside_stress_points(xn, yn, ndivx, ndivy, deck.npoints, x1, y1, dx, dy)
+    if deck.inside_approach:
+        xs, ys = inside_stress_points(xn, yn, ndivx, ndivy, deck.npoints, x1, y1, dx, dy)
+    else:
+        xs, ys = outside_stress_points(xn, yn, deck.npoints, dx * deck.rx_factor, dx * deck.ry_factor)
 
     rho0 = deck.props[5]
     hs = deck.sml * dx
     area = (rho0 * dx * dy) / 8
     mass_n = no_int * area
-    mass_s = np.full(nstress, (8 * area) / deck.npoints)
+    if deck.inside_approach:                    # 3_...:680-705
+        mass_s = np.full(nstress, (8 * area) / deck.npoints)
+    else:
+        mass_s = np.repeat(mass_n / deck.npoints, deck.npoints)
 
     # dummy walls, default-real arithmetic (3_...:736-878)
     walls = deck.walls if deck.dummy_nodes else []

@@ -43,6 +43,12 @@ PLAN = {
     "b1_bui_sp1": ([1, 2, 10, 100, 1000], [1, 10, 100], False, "bui"),
     "b2_bui_sp2": ([1, 10, 200], [1, 200], False, "bui"),
     "b3_bui_sp3": ([1, 10, 200], [1, 200], False, "bui"),
+    # the same slope with the outside approach: two stress points per node,
+    # shifted every step at fixed offsets or along the node velocity
+    "b4_bui_outside": ([1, 2, 10, 100, 500], [1, 2, 100], False, "bui"),
+    "b5_bui_velvec": ([1, 2, 10, 100, 500], [1, 2, 100], False, "bui"),
+    # velocity-vector shifts on every step (displacement tolerance 1e-5 dx)
+    "b6_bui_velvec_shift": ([1, 2, 10, 100, 500], [1, 2, 100], False, "bui"),
 }
 
 STATE_FIELDS = ("x", "vel", "stress", "ivar1", "f_drucker", "ddev_strn", "grad_u", "displ",

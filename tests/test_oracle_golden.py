@@ -60,7 +60,10 @@ def test_oracle_bitwise_small_decks(deck_name, oracle_lib):
 DIGEST = {"c1_collapse": "BASELINE config 1 (5,151 + 5,000 particles, 1000 steps of 1e-5 s)",
           "b1_bui_sp1": "Bui et al. (2008) slope, XSPH, one stress point per cell, 1000 steps",
           "b2_bui_sp2": "the same slope with two stress points per cell, 200 steps",
-          "b3_bui_sp3": "the same slope with three stress points per cell, 200 steps"}
+          "b3_bui_sp3": "the same slope with three stress points per cell, 200 steps",
+          "b4_bui_outside": "outside approach, two points per node at fixed offsets, 500 steps",
+          "b5_bui_velvec": "outside approach, velocity-vector orientation (no shift reaches disp_tol), 500 steps",
+          "b6_bui_velvec_shift": "velocity-vector shifts on every step (disp_tol 1e-5), 500 steps"}
 
 
 def fixture_steps(g):
