@@ -93,7 +93,8 @@ typedef struct sps_params {
     int32_t sph_shift;       /* SPH_shift (3_...:373), outside approach only:
                                 stress_point_update + adapt_stress2 after the
                                 search on shift steps (2_...:99-109) */
-    int32_t no_bcs;          /* must be 0 */
+    int32_t no_bcs;          /* prescribed BCs (3_...:165): 0, or > 0 with
+                                sps_set_bcs (update_x = 0 only) */
     int32_t bforce_gate;     /* 0: wall force when inside_approach .and.
                                 dummy_nodes (code/2_...:742); 1: when
                                 inside_approach (Bui fork 2_...:742) */
@@ -210,6 +211,39 @@ int sps_get_stats(sps_handle *h, sps_stats *out);
  * of sps_params for the pair loops, the Drucker-Prager terms and the return
  * map, particle by particle. */
 int sps_set_materials(sps_handle *h, int32_t nmat, const double *table, const int32_t *mat);
+
+/* Prescribed velocity / stress values (Normal_BCs, 3_...:1644-1753) at the
+ * points where the reference calls BCs: after each RK stage's update and
+ * after its stress_point_update (2_...:715-723), after the final RK update
+ * (2_...:788) and after the step's stress_point_update (2_...:134).  Replaces
+ * the globals bc_list, bc_info, BC_or_not and the time curves
+ * (3_...:160-260, 1030-1113).  Required before sps_step when
+ * sps_params.no_bcs > 0, which sps_create accepts only with update_x = 0
+ * (with update_x the reference's free-surface detection, 3_...:1116-1411,
+ * changes BC_or_not every step) in SPS_MODE_ORDERED.  The BC time is
+ * time_sph: the reference adds ic_time*dt_sph, ic_time being a local it never
+ * sets (3_...:1622-1627).  Values are applied to the nodes 1..nnode with
+ * BC_or_not = 1, as code/ does (the vertical-slope fork loops over ntotal;
+ * its stress points never lie on a BC segment with the inside approach). */
+#define SPS_MAX_BCS 32
+#define SPS_MAX_CURVES 16
+typedef struct sps_bcs {
+    int32_t no_bcs;            /* rows of bc_list, 1..SPS_MAX_BCS, = sps_params.no_bcs */
+    int32_t ifsigman;          /* must be 0 (apply_stress_free needs the free surface) */
+    int32_t ncurves;           /* time curves, 0..SPS_MAX_CURVES */
+    int32_t reserved0;         /* must be 0 */
+    const double *bc_list;     /* (8,no_bcs), 3_...:176-180: number; variable (1 sxx,
+                                  2 syy, 3 sxy, 5 vx, 6 vy; others act on nothing);
+                                  curve (0: (a0 + a1 sin(w t - phi)) (1 - exp(-t/Tf)),
+                                  k > 0: a1 times time curve k); a1; a0; w; phi
+                                  (already times twopi); Tf */
+    const int32_t *bc_info;    /* (8,nnode): Get_BCs_on_node's rows of the nodes */
+    const int32_t *bc_or_not;  /* (nnode) */
+    const int32_t *curve_npts; /* (ncurves) nptstcurves, 2..SPS_MAX_TCURVE */
+    const double *curve_t;     /* (SPS_MAX_TCURVE,ncurves) ttcurves */
+    const float *curve_f;      /* (SPS_MAX_TCURVE,ncurves) ftcurves */
+} sps_bcs;
+int sps_set_bcs(sps_handle *h, const sps_bcs *bcs);
 
 /* Pair lists kept over steps.  No counterpart in the reference, which
  * searches every step; SPS_MODE_FAST only.  With skin > 0 the search also

@@ -54,6 +54,13 @@ struct sps_handle {
     double *x10, *disp10;   /* outside approach, vel_vector (x_10, disp_10) */
     float *n_int;           /* node-node pairs of each node (outside approach) */
     int32_t *bc_int;        /* node has a node-dummy pair (outside approach) */
+    /* prescribed BCs (sps_set_bcs) */
+    int bcs_set, nbc, ncurves;
+    double bc_list[8 * SPS_MAX_BCS];
+    int32_t *bc_info, *bc_or_not;
+    int32_t curve_npts[SPS_MAX_CURVES];
+    double curve_t[SPS_MAX_TCURVE * SPS_MAX_CURVES];
+    float curve_f[SPS_MAX_TCURVE * SPS_MAX_CURVES];
     int32_t *itype, *ood;
     float *wallpos, *horiz;
     /* start-of-step copies (2_...:114) */
@@ -117,6 +124,7 @@ static int alloc_all(struct sps_handle *h)
     A(vel_tmp, 2 * nt, double); A(stress_tmp, 4 * nt, double); A(ivar_tmp, nt, double);
     A(av_tmp, 2 * nn, float);
     A(x10, 2 * nn, double); A(disp10, nn, double); A(n_int, nn, float); A(bc_int, nn, int32_t);
+    A(bc_info, 8 * nn, int32_t); A(bc_or_not, nn, int32_t);
 #undef A
     return 0;
 }
@@ -131,7 +139,8 @@ static void free_all(struct sps_handle *h)
                     h->vel0, h->stress0, h->rk_vel, h->rk_stress, h->rk_dev, h->rhs1,
                     h->rhs2, h->divf1, h->divf2, h->source_stress, h->g_grav, h->spin,
                     h->grad1, h->grad2, h->ae, h->dummy_vel, h->cspm_norm, h->vel_tmp,
-                    h->stress_tmp, h->ivar_tmp, h->av_tmp, h->x10, h->disp10, h->n_int, h->bc_int};
+                    h->stress_tmp, h->ivar_tmp, h->av_tmp, h->x10, h->disp10, h->n_int, h->bc_int,
+                    h->bc_info, h->bc_or_not};
     for (size_t k = 0; k < sizeof ptrs / sizeof ptrs[0]; ++k) free(ptrs[k]);
 }
 
@@ -783,6 +792,8 @@ static void artificial_viscosity(struct sps_handle *h)
 
 /* -------------------------------------------------------------------- RK4 */
 /* code/2_SPH_main_2018.f90:653-802 */
+static void normal_bcs(struct sps_handle *h, double time_sph);
+
 static void rk4(struct sps_handle *h, double dt_sph, double time_sph)
 {
     const int nn = h->nn, nt = h->nt;
@@ -811,8 +822,10 @@ static void rk4(struct sps_handle *h, double dt_sph, double time_sph)
             for (int q = 0; q < 4; ++q)
                 h->stress[4 * i + q] = h->stress0[4 * i + q] + c1 * h->rhs1[4 * i + q];
         if (ncrit == 12) adapt_stress2(h);
+        normal_bcs(h, time_sph);
         stress_point_update(h);
         if (ncrit == 12) adapt_stress2(h);
+        normal_bcs(h, time_sph);
         get_derivatives(h);
         for (int j = nn; j < nt; ++j) {
             double Gs[4], der1;
@@ -870,7 +883,63 @@ static void rk4(struct sps_handle *h, double dt_sph, double time_sph)
         }
     }
     if (ncrit == 12) adapt_stress2(h);
+    normal_bcs(h, time_sph);
     for (int i = 0; i < nt; ++i) h->ddev[i] = h->rk_dev[i] / 6;
+}
+
+/* ------------------------------------------------------------ Normal_BCs */
+/* 3_SPH_material_2018.f90:1644-1753 (code/ as it compiles): the value of BC
+ * row b at time t, a0 + a1 sin(w t - phi) ramped by 1 - exp(-t/Tf), or a1
+ * times the piecewise-linear time curve (0 outside it) ... */
+static double bc_value(const struct sps_handle *h, int b, double t)
+{
+    const double *L = h->bc_list + 8 * b;
+    const int it_curves = (int)L[2];
+    if (it_curves == 0) {
+        const double a0 = L[4], a1 = L[3], w = L[5], phi = L[6], tt = L[7];
+        const double fact = 1.0 - exp(-t / tt);
+        const double argum = w * t - phi;
+        return (a0 + a1 * sin(argum)) * fact;
+    }
+    const int c = it_curves - 1, npts = h->curve_npts[c];
+    const double *T = h->curve_t + SPS_MAX_TCURVE * c;
+    const float *F = h->curve_f + SPS_MAX_TCURVE * c;
+    double tt0 = -1000.0, tt1 = 0.0, v = 0.0;
+    int ip;
+    for (ip = 0; ip < npts - 1; ++ip) {
+        tt0 = T[ip];
+        tt1 = T[ip + 1];
+        if (t >= tt0 && t <= tt1) break;
+        tt0 = -1000.0;
+    }
+    if (tt0 >= 0.0) {
+        const double xi = (t - tt0) / (tt1 - tt0);
+        v = (1.0 - xi) * (double)F[ip] + xi * (double)F[ip + 1];
+    }
+    return v * L[3];
+}
+
+/* ... applied to the nodes with BC_or_not = 1, up to five BCs each from
+ * bc_info(3:7,:) in order: variable 5/6 set vx/vy, 1/2/3 set sxx/syy/sxy. */
+static void normal_bcs(struct sps_handle *h, double time_sph)
+{
+    if (h->p.no_bcs <= 0) return;
+    double val[SPS_MAX_BCS];
+    for (int b = 0; b < h->nbc; ++b) val[b] = bc_value(h, b, time_sph);
+    for (int i = 0; i < h->nn; ++i) {
+        const int32_t *info = h->bc_info + 8 * i;
+        if (h->bc_or_not[i] != 1 || info[1] == 0) continue;
+        for (int k = 0; k < 5; ++k) {
+            const int b = info[2 + k];
+            if (b == 0) continue;
+            const double v = val[b - 1], var = h->bc_list[8 * (b - 1) + 1];
+            if (var == 5) h->vel[2 * i] = v;
+            else if (var == 6) h->vel[2 * i + 1] = v;
+            else if (var == 1) h->stress[4 * i] = v;
+            else if (var == 3) h->stress[4 * i + 2] = v;
+            else if (var == 2) h->stress[4 * i + 1] = v;
+        }
+    }
 }
 
 /* ------------------------------------------------------- time_integration */
@@ -1006,6 +1075,7 @@ static int time_integration(struct sps_handle *h, int itimestep, double time_sph
         h->ivars[(size_t)h->niv * i] = h->ivars[(size_t)h->niv * i] + dt * h->ddev[i];
     stress_point_update(h);
     if (ncrit == 12) adapt_stress2(h);
+    normal_bcs(h, time_sph);
     if (h->p.update_x && h->p.xsph) {
         xsph_update(h, dt);
         if (!h->p.inside_approach) shift_stress_points(h, itimestep);
@@ -1048,8 +1118,11 @@ static int check_params(const sps_params *p)
         if (p->mode != SPS_MODE_ORDERED) return fail(SPS_EINVAL, "the outside approach runs in SPS_MODE_ORDERED");
     }
     if (p->reserved0 != 0) return fail(SPS_EINVAL, "reserved0 must be 0");
-    if (p->cont_density || p->art_stress || p->no_bcs)
-        return fail(SPS_EINVAL, "continuity density, artificial stress and prescribed BCs are not supported");
+    if (p->cont_density || p->art_stress)
+        return fail(SPS_EINVAL, "continuity density and artificial stress are not supported");
+    if (p->no_bcs < 0 || p->no_bcs > SPS_MAX_BCS) return fail(SPS_EINVAL, "no_bcs must be 0..%d", SPS_MAX_BCS);
+    if (p->no_bcs > 0 && (p->update_x || p->mode != SPS_MODE_ORDERED))
+        return fail(SPS_EINVAL, "prescribed BCs need update_x = 0 (no free-surface detection) and SPS_MODE_ORDERED");
     if (p->xsph && p->mode != SPS_MODE_ORDERED) return fail(SPS_EINVAL, "XSPH runs in SPS_MODE_ORDERED");
     int ntype_eco = (int)p->props[0], ncrit = (int)p->props[1];
     if (ntype_eco > 1 && ncrit <= 5) return fail(SPS_EINVAL, "Perzyna models (ncrit<=5) are not supported");
@@ -1117,10 +1190,42 @@ int sps_set_state(sps_handle *h, const sps_state *st)
     return 0;
 }
 
+int sps_set_bcs(sps_handle *h, const sps_bcs *b)
+{
+    if (!h || !b || !b->bc_list || !b->bc_info || !b->bc_or_not) return fail(SPS_EINVAL, "null argument");
+    if (b->no_bcs != h->p.no_bcs || b->no_bcs < 1) return fail(SPS_EINVAL, "no_bcs %d, sps_params says %d", b->no_bcs, h->p.no_bcs);
+    if (b->ifsigman != 0 || b->reserved0 != 0) return fail(SPS_EINVAL, "ifsigman and reserved0 must be 0");
+    if (b->ncurves < 0 || b->ncurves > SPS_MAX_CURVES || (b->ncurves > 0 && (!b->curve_npts || !b->curve_t || !b->curve_f)))
+        return fail(SPS_EINVAL, "bad time curves");
+    for (int c = 0; c < b->ncurves; ++c)
+        if (b->curve_npts[c] < 2 || b->curve_npts[c] > SPS_MAX_TCURVE) return fail(SPS_EINVAL, "curve %d: bad point count", c + 1);
+    for (int k = 0; k < b->no_bcs; ++k) {
+        const double cv = b->bc_list[8 * k + 2];
+        if (!(cv >= 0 && (int)cv <= b->ncurves)) return fail(SPS_EINVAL, "BC %d: time curve %g of %d", k + 1, cv, b->ncurves);
+    }
+    for (int i = 0; i < h->nn; ++i)
+        for (int k = 2; k < 7; ++k)
+            if (b->bc_info[8 * i + k] < 0 || b->bc_info[8 * i + k] > b->no_bcs)
+                return fail(SPS_EINVAL, "node %d: BC number %d of %d", i + 1, b->bc_info[8 * i + k], b->no_bcs);
+    h->nbc = b->no_bcs;
+    h->ncurves = b->ncurves;
+    memcpy(h->bc_list, b->bc_list, 8 * (size_t)b->no_bcs * sizeof(double));
+    memcpy(h->bc_info, b->bc_info, 8 * (size_t)h->nn * sizeof(int32_t));
+    memcpy(h->bc_or_not, b->bc_or_not, (size_t)h->nn * sizeof(int32_t));
+    for (int c = 0; c < b->ncurves; ++c) {
+        h->curve_npts[c] = b->curve_npts[c];
+        memcpy(h->curve_t + SPS_MAX_TCURVE * c, b->curve_t + SPS_MAX_TCURVE * c, SPS_MAX_TCURVE * sizeof(double));
+        memcpy(h->curve_f + SPS_MAX_TCURVE * c, b->curve_f + SPS_MAX_TCURVE * c, SPS_MAX_TCURVE * sizeof(float));
+    }
+    h->bcs_set = 1;
+    return 0;
+}
+
 int sps_step(sps_handle *h, int32_t nsteps, double dt, double *time_sph, int32_t *itimestep_sph)
 {
     if (!h || !time_sph || !itimestep_sph) return fail(SPS_EINVAL, "null argument");
     if (!h->have_state) return fail(SPS_ESTATE, "sps_step before sps_set_state");
+    if (h->p.no_bcs > 0 && !h->bcs_set) return fail(SPS_ESTATE, "no_bcs > 0: sps_set_bcs before sps_step");
     for (int32_t s = 0; s < nsteps; ++s) {
         *itimestep_sph += 1;
         int rc = time_integration(h, *itimestep_sph, *time_sph, dt);

@@ -21,7 +21,7 @@ import os
 import numpy as np
 
 from .deck import Deck, DeckError, read_deck  # noqa: F401
-from .setup import Particles, build_particles, elastic_matrix, reference_pi  # noqa: F401
+from .setup import Particles, bcs_on_particles, build_particles, elastic_matrix, reference_pi  # noqa: F401
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 PRODUCT_LIB = os.path.join(PKG_DIR, "libsps_hip.so")
@@ -29,6 +29,8 @@ DECK_DIR = os.path.join(PKG_DIR, "decks")
 
 SPS_ABI_VERSION = 2
 SPS_MAX_TCURVE = 64
+SPS_MAX_BCS = 32
+SPS_MAX_CURVES = 16
 MODE_ORDERED = 0
 MODE_FAST = 1
 
@@ -77,6 +79,12 @@ class State(ctypes.Structure):
                                    "ddev_strn", "f_bound", "art_visc", "x_10", "disp_10")]
 
 
+class Bcs(ctypes.Structure):
+    _fields_ = [("no_bcs", ctypes.c_int32), ("ifsigman", ctypes.c_int32), ("ncurves", ctypes.c_int32),
+                ("reserved0", ctypes.c_int32)] + [(n, _P) for n in ("bc_list", "bc_info", "bc_or_not", "curve_npts",
+                                                                     "curve_t", "curve_f")]
+
+
 class Stats(ctypes.Structure):
     _fields_ = [("n_pairs", ctypes.c_int64), ("m_pairs", ctypes.c_int64),
                 ("n_by_type", ctypes.c_int64 * 10), ("ndivx", ctypes.c_int32),
@@ -107,13 +115,14 @@ def load_library(path):
     lib.sps_set_materials.argtypes = [h, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p]
     lib.sps_set_graph.argtypes = [h, ctypes.c_int32]
     lib.sps_get_search_count.argtypes = [h, ctypes.POINTER(ctypes.c_int64)]
+    lib.sps_set_bcs.argtypes = [h, ctypes.POINTER(Bcs)]
     lib.sps_destroy.argtypes = [h]
     lib.sps_destroy.restype = None
     lib.sps_last_error.restype = ctypes.c_char_p
     lib.sps_backend.restype = ctypes.c_char_p
     for f in ("sps_create", "sps_set_state", "sps_step", "sps_get_state", "sps_get_pairs",
               "sps_get_stats", "sps_set_profiling", "sps_get_profile", "sps_set_list_skin", "sps_set_materials", "sps_set_graph",
-              "sps_get_search_count"):
+              "sps_get_search_count", "sps_set_bcs"):
         getattr(lib, f).restype = ctypes.c_int
     return lib
 
@@ -213,7 +222,29 @@ class Simulation:
         sim = cls(make_params(deck, parts, mode=mode, **kw), lib_path)
         sim.set_state(initial_state(parts, sim.niv))
         sim.deck, sim.parts = deck, parts
+        if deck.no_bcs > 0:
+            sim.set_bcs(deck, parts)
         return sim
+
+    def set_bcs(self, deck: Deck, parts: Particles):
+        """The deck's prescribed BCs (sps_set_bcs): bc_list, the nodes' rows of
+        Get_BCs_on_node at the initial positions, every time curve."""
+        info, bc_or_not = bcs_on_particles(deck, parts.x[:parts.ntotal])
+        nc = len(deck.ttcurves)
+        if nc > SPS_MAX_CURVES:
+            raise DeckError(f"more than {SPS_MAX_CURVES} time curves")
+        ct = np.zeros((max(nc, 1), SPS_MAX_TCURVE))
+        cf = np.zeros((max(nc, 1), SPS_MAX_TCURVE), dtype=np.float32)
+        npts = np.zeros(max(nc, 1), dtype=np.int32)
+        for c in range(nc):
+            npts[c] = len(deck.ttcurves[c])
+            ct[c, :npts[c]], cf[c, :npts[c]] = deck.ttcurves[c], deck.ftcurves[c]
+        keep = [np.ascontiguousarray(deck.bc_list, dtype=np.float64), np.ascontiguousarray(info[:self.nn]),
+                np.ascontiguousarray(bc_or_not[:self.nn]), npts, ct, cf]
+        b = Bcs(no_bcs=deck.no_bcs, ifsigman=deck.ifsigman, ncurves=nc, reserved0=0)
+        for name, a in zip(("bc_list", "bc_info", "bc_or_not", "curve_npts", "curve_t", "curve_f"), keep):
+            setattr(b, name, a.ctypes.data)
+        self._check(self.lib.sps_set_bcs(self.handle, ctypes.byref(b)))
 
     def _check(self, rc):
         if rc != 0:

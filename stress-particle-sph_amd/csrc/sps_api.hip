@@ -150,6 +150,14 @@ struct sps_handle {
     DBuf<int> inv, nodeinfo;      // slot of each original particle; per node slot n_int, bc_int
     DBuf<double2> x10;            // x_10 and disp_10 by original node index
     DBuf<double> disp10;
+    // prescribed BCs (sps_set_bcs): tables on the host, node rows on the device
+    bool bcs_set = false;
+    int nbc = 0, ncurves = 0;
+    double bc_list[8 * SPS_MAX_BCS];
+    int curve_npts[SPS_MAX_CURVES];
+    double curve_t[SPS_MAX_TCURVE * SPS_MAX_CURVES];
+    float curve_f[SPS_MAX_TCURVE * SPS_MAX_CURVES];
+    DBuf<int> bc_info, bc_or_not;
     DBuf<unsigned short> deg_a, deg_b;
     DBuf<unsigned char> tile_ok;
     DBuf<Tile> tiles;
@@ -213,13 +221,14 @@ struct sps_handle {
 enum KernelId {
     K_CHECK_OOD, K_BOUNDS, K_CELL_COUNT, K_CELL_SCATTER, K_CELL_SORT, K_PERMUTE, K_PROC,
     K_COUNT_PAIRS, K_SCAN, K_SPLIT, K_RK_INIT, K_STAGE_BEGIN, K_SPU_NODES,
-    K_SP_UPDATE, K_DERIV_NODES, K_DUMMY_STRESS, K_RK_FINAL, K_POSITION, K_SURROGATE, K_XSPH, K_SHIFT, K_NKERNELS
+    K_SP_UPDATE, K_DERIV_NODES, K_DUMMY_STRESS, K_RK_FINAL, K_POSITION, K_SURROGATE, K_XSPH, K_SHIFT, K_BCS,
+    K_NKERNELS
 };
 static const char *const kKernelNames[K_NKERNELS] = {
     "k_check_ood", "k_bounds", "k_cell_count", "k_cell_scatter", "k_cell_sort", "k_permute",
     "k_proc_slot", "k_count", "k_scan", "k_split", "k_rk_init",
     "k_stage_begin", "k_spu_nodes", "k_sp_update", "k_deriv_nodes",
-    "k_dummy_stress", "k_rk_final", "k_position", "k_surrogate", "k_xsph", "k_shift_sp"};
+    "k_dummy_stress", "k_rk_final", "k_position", "k_surrogate", "k_xsph", "k_shift_sp", "k_bcs"};
 
 static hipEvent_t prof_begin(sps_handle *h) {
     if (!h->prof) return nullptr;
@@ -311,8 +320,11 @@ static int check_params(const sps_params *p) {
         if (p->mode != SPS_MODE_ORDERED) return fail(SPS_EINVAL, "the outside approach runs in SPS_MODE_ORDERED");
     }
     if (p->reserved0 != 0) return fail(SPS_EINVAL, "reserved0 must be 0");
-    if (p->cont_density || p->art_stress || p->no_bcs)
-        return fail(SPS_EINVAL, "continuity density, artificial stress and prescribed BCs are not supported");
+    if (p->cont_density || p->art_stress)
+        return fail(SPS_EINVAL, "continuity density and artificial stress are not supported");
+    if (p->no_bcs < 0 || p->no_bcs > SPS_MAX_BCS) return fail(SPS_EINVAL, "no_bcs must be 0..%d", SPS_MAX_BCS);
+    if (p->no_bcs > 0 && (p->update_x || p->mode != SPS_MODE_ORDERED))
+        return fail(SPS_EINVAL, "prescribed BCs need update_x = 0 (no free-surface detection) and SPS_MODE_ORDERED");
     if (p->xsph && p->mode != SPS_MODE_ORDERED) return fail(SPS_EINVAL, "XSPH runs in SPS_MODE_ORDERED");
     int ntype_eco = (int)p->props[0], ncrit = (int)p->props[1];
     if (ntype_eco > 1 && ncrit <= 5) return fail(SPS_EINVAL, "Perzyna models (ncrit<=5) are not supported");
@@ -320,6 +332,48 @@ static int check_params(const sps_params *p) {
         return fail(SPS_EINVAL, "gravity time curve needs 2..%d points", SPS_MAX_TCURVE);
     if (p->mode != SPS_MODE_ORDERED && p->mode != SPS_MODE_FAST) return fail(SPS_EINVAL, "unknown mode %d", p->mode);
     return 0;
+}
+
+// Normal_BCs' value of BC row b at time t (3_...:1671-1700): a0 + a1 sin(w t
+// - phi) ramped by 1 - exp(-t/Tf), or a1 times the piecewise-linear time
+// curve (0 outside it).  Computed on the host once per step: the time does
+// not change inside a step.
+static double bc_value(const sps_handle *h, int b, double t) {
+    const double *L = h->bc_list + 8 * b;
+    const int it_curves = (int)L[2];
+    if (it_curves == 0) {
+        const double a0 = L[4], a1 = L[3], w = L[5], phi = L[6], tt = L[7];
+        const double fact = 1.0 - std::exp(-t / tt);
+        const double argum = w * t - phi;
+        return (a0 + a1 * std::sin(argum)) * fact;
+    }
+    const int c = it_curves - 1, npts = h->curve_npts[c];
+    const double *T = h->curve_t + SPS_MAX_TCURVE * c;
+    const float *F = h->curve_f + SPS_MAX_TCURVE * c;
+    double tt0 = -1000.0, tt1 = 0.0, v = 0.0;
+    int ip;
+    for (ip = 0; ip < npts - 1; ++ip) {
+        tt0 = T[ip];
+        tt1 = T[ip + 1];
+        if (t >= tt0 && t <= tt1) break;
+        tt0 = -1000.0;
+    }
+    if (tt0 >= 0.0) {
+        const double xi = (t - tt0) / (tt1 - tt0);
+        v = (1.0 - xi) * (double)F[ip] + xi * (double)F[ip + 1];
+    }
+    return v * L[3];
+}
+
+static BcValues bc_values(const sps_handle *h, double t) {
+    BcValues V{};
+    V.nbc = h->nbc;
+    for (int b = 0; b < h->nbc; ++b) {
+        V.v[b] = bc_value(h, b, t);
+        const double var = h->bc_list[8 * b + 1];
+        V.var[b] = (signed char)((var == 5 || var == 6 || var == 1 || var == 2 || var == 3) ? (int)var : 0);
+    }
+    return V;
 }
 
 // gravity_force time factor at step time (3_...:2655-2680)
@@ -419,6 +473,7 @@ extern "C" int sps_create(const sps_params *params, sps_handle **out) {
     rc |= dalloc(h->tiles, h->ntile_n + h->ntile_s + 1);
     rc |= dalloc(h->plans, 3 * (size_t)(h->ntile_n + h->ntile_s + 1));
     rc |= dalloc(h->x_build, nt); rc |= dalloc(h->dmax, 1); rc |= dalloc(h->stepv, 4);
+    if (params->no_bcs > 0) rc |= dalloc(h->bc_info, 8 * (size_t)nn) | dalloc(h->bc_or_not, nn);
     if (h->outside) {
         rc |= dalloc(h->inv, nt2); rc |= dalloc(h->nodeinfo, nn);
         rc |= dalloc(h->x10, nn); rc |= dalloc(h->disp10, nn);
@@ -769,6 +824,8 @@ static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
     const int adapt = h->dp.ncrit == 12;
     const double tanfi = h->dp.tanfi, coh = h->dp.coh;
     const double f1rk[4] = {0., 0.5, 0.5, 1.0}, f2rk[4] = {1., 2., 2., 1.0};
+    const bool bcs = h->p.no_bcs > 0 && nn > 0;
+    const BcValues BV = bcs ? bc_values(h, time_sph) : BcValues{};
     if (h->outside && h->p.sph_shift && itimestep > 1 && (itimestep - 1) % h->p.shift_update == 0) {
         // stress_point_update + adapt_stress2 on the new lists (2_...:99-109)
         if (nn > 0)
@@ -781,8 +838,10 @@ static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
     for (int st = 0; st < 4; ++st) {
         const double c1 = f1rk[st] * (dt);
         LAUNCH(K_STAGE_BEGIN, nblk(nt), BLOCK, k_stage_begin, SB, W, c1, adapt, tanfi, coh, nn, nt);
+        if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, nn);
         if (nn > 0)
             LAUNCH_LDS(K_SPU_NODES, nblk8(nn), TILE, LAY_SPU_NODES.bytes, k_spu_nodes, SB, V, PL, adapt, tanfi, coh, nn);
+        if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, nn);
         if (nt > nn)
             LAUNCH_LDS(K_SP_UPDATE, nblk8(nt - nn), TILE, LAY_SP_UPDATE.bytes, k_sp_update, SB, W, V, PL, h->dp, time_sph,
                        f2rk[st], 1, adapt, nn, nt);
@@ -795,11 +854,13 @@ static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
     const bool hook = h->log_cap > 0;
     LAUNCH(K_RK_FINAL, nblk(nt), BLOCK, k_rk_final, SB, W, dt, adapt, tanfi, coh, nn, nt, surrogate_dev(h),
            hook ? 1 : 0);
+    if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, nn);
     if (nn > 0)
         LAUNCH_LDS(K_SPU_NODES, nblk8(nn), TILE, LAY_SPU_NODES.bytes, k_spu_nodes, SB, V, PL, adapt, tanfi, coh, nn);
     if (nt > nn)
         LAUNCH_LDS(K_SP_UPDATE, nblk8(nt - nn), TILE, LAY_SP_UPDATE.bytes, k_sp_update, SB, W, V, PL, h->dp, time_sph, 0.0,
                    0, adapt, nn, nt);
+    if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, nn);
     const bool xsph = h->xsph && h->p.update_x;
     if (xsph && nn > 0)
         LAUNCH_LDS(K_XSPH, nblk8(nn), TILE, LAY_XSPH.bytes, (k_xsph<KIND_NODE>), SB, V, PL, h->xsph_v.p);
@@ -815,9 +876,46 @@ static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
     return 0;
 }
 
+extern "C" int sps_set_bcs(sps_handle *h, const sps_bcs *b) {
+    if (!h || !b || !b->bc_list || !b->bc_info || !b->bc_or_not) return fail(SPS_EINVAL, "null argument");
+    if (b->no_bcs != h->p.no_bcs || b->no_bcs < 1)
+        return fail(SPS_EINVAL, "no_bcs %d, sps_params says %d", b->no_bcs, h->p.no_bcs);
+    if (b->ifsigman != 0 || b->reserved0 != 0) return fail(SPS_EINVAL, "ifsigman and reserved0 must be 0");
+    if (b->ncurves < 0 || b->ncurves > SPS_MAX_CURVES || (b->ncurves > 0 && (!b->curve_npts || !b->curve_t || !b->curve_f)))
+        return fail(SPS_EINVAL, "bad time curves");
+    for (int c = 0; c < b->ncurves; ++c)
+        if (b->curve_npts[c] < 2 || b->curve_npts[c] > SPS_MAX_TCURVE)
+            return fail(SPS_EINVAL, "curve %d: bad point count", c + 1);
+    for (int k = 0; k < b->no_bcs; ++k) {
+        const double cv = b->bc_list[8 * k + 2];
+        if (!(cv >= 0 && (int)cv <= b->ncurves))
+            return fail(SPS_EINVAL, "BC %d: time curve %g of %d", k + 1, cv, b->ncurves);
+    }
+    const int nn = h->nn;
+    for (int i = 0; i < nn; ++i)
+        for (int k = 2; k < 7; ++k)
+            if (b->bc_info[8 * i + k] < 0 || b->bc_info[8 * i + k] > b->no_bcs)
+                return fail(SPS_EINVAL, "node %d: BC number %d of %d", i + 1, b->bc_info[8 * i + k], b->no_bcs);
+    HIP_TRY(hipSetDevice(h->device));
+    h->nbc = b->no_bcs;
+    h->ncurves = b->ncurves;
+    std::memcpy(h->bc_list, b->bc_list, 8 * (size_t)b->no_bcs * sizeof(double));
+    for (int c = 0; c < b->ncurves; ++c) {
+        h->curve_npts[c] = b->curve_npts[c];
+        std::memcpy(h->curve_t + SPS_MAX_TCURVE * c, b->curve_t + SPS_MAX_TCURVE * c, SPS_MAX_TCURVE * sizeof(double));
+        std::memcpy(h->curve_f + SPS_MAX_TCURVE * c, b->curve_f + SPS_MAX_TCURVE * c, SPS_MAX_TCURVE * sizeof(float));
+    }
+    HIP_TRY(hipMemcpyAsync(h->bc_info.p, b->bc_info, 8 * (size_t)nn * sizeof(int), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->bc_or_not.p, b->bc_or_not, (size_t)nn * sizeof(int), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->bcs_set = true;
+    return 0;
+}
+
 extern "C" int sps_step(sps_handle *h, int32_t nsteps, double dt, double *time_sph, int32_t *itimestep_sph) {
     if (!h || !time_sph || !itimestep_sph) return fail(SPS_EINVAL, "null argument");
     if (!h->have_state) return fail(SPS_ESTATE, "sps_step before sps_set_state");
+    if (h->p.no_bcs > 0 && !h->bcs_set) return fail(SPS_ESTATE, "no_bcs > 0: sps_set_bcs before sps_step");
     if (h->poisoned) return fail(SPS_ESTATE, "a previous step failed; set the state again");
     HIP_TRY(hipSetDevice(h->device));
     for (int32_t k = 0; k < nsteps; ++k) {

@@ -2129,6 +2129,44 @@ __global__ void k_position(StateBufs S, Work W, double dt, int update_x, int nn,
     }
 }
 
+// Prescribed values of one step: BC row b's value and its variable (5 vx,
+// 6 vy, 1 sxx, 2 syy, 3 sxy, 0 none).
+struct BcValues {
+    int nbc;
+    double v[SPS_MAX_BCS];
+    signed char var[SPS_MAX_BCS];
+};
+
+// Normal_BCs (3_...:1644-1753) on the node slots: a node with BC_or_not = 1
+// takes the values of the BCs listed in bc_info(3:7) (by original index), in
+// that order.
+__global__ void k_bcs(StateBufs S, const int *__restrict__ bc_info, const int *__restrict__ bc_or_not, BcValues V,
+                      int nn) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nn) return;
+    const int i = S.orig[r];
+    const int *info = bc_info + 8 * (size_t)i;
+    if (bc_or_not[i] != 1 || info[1] == 0) return;
+    double2 v = S.vel[r];
+    double4 s = S.stress[r];
+    bool dv = false, ds = false;
+    for (int k = 0; k < 5; ++k) {
+        const int b = info[2 + k];
+        if (b <= 0 || b > V.nbc) continue;
+        const double val = V.v[b - 1];
+        switch (V.var[b - 1]) {
+        case 5: v.x = val; dv = true; break;
+        case 6: v.y = val; dv = true; break;
+        case 1: s.x = val; ds = true; break;
+        case 3: s.z = val; ds = true; break;
+        case 2: s.y = val; ds = true; break;
+        default: break;
+        }
+    }
+    if (dv) S.vel[r] = v;
+    if (ds) S.stress[r] = s;
+}
+
 // Slot of each original particle: the outside approach writes a node's
 // stress points by their original indices.
 __global__ void k_inv_slots(const int *__restrict__ orig, int *__restrict__ inv, int nt2) {

@@ -155,6 +155,9 @@ class Deck:
     ifsigman: int = 0
     no_segments_bc: int = 0
     no_nodes_bc: int = 0
+    bc_list: np.ndarray = None        # (no_bcs, 8): number, variable, curve, a1, a0, w, phi*twopi, Tf
+    segment_bcs: np.ndarray = None    # (no_segments_bc, 5): x1 y1 x2 y2 bc
+    nodal_bcs: np.ndarray = None      # (no_nodes_bc, 2): node, bc
     ic_tcurve: int = 0
     ttcurves: list = None             # per curve: float64 array
     ftcurves: list = None             # per curve: float32 array
@@ -278,16 +281,28 @@ def read_deck(directory, layout="auto"):
     # .dat remainder (3_...:160-317)
     dat.text()
     d.no_bcs, d.ifsigman = dat.ints(2)
-    if d.no_bcs > 0:
-        raise DeckError("prescribed boundary conditions (no_bcs > 0) are not supported")
+    d.bc_list = np.zeros((max(d.no_bcs, 0), 8))
+    if d.no_bcs > 0:                             # 3_...:168-181
+        # twopi = acos(0.0)*4.0/360. in default real: degrees to radians
+        twopi = np.float32(np.float32(np.float32(np.pi / 2) * np.float32(4.0)) / np.float32(360.0))
+        dat.text()
+        for i in range(d.no_bcs):
+            d.bc_list[i] = dat.reals(8)
+            d.bc_list[i, 6] = d.bc_list[i, 6] * float(twopi)
     dat.text()
     d.no_segments_bc = dat.int1()
-    if d.no_segments_bc > 0:
-        raise DeckError("segment boundary conditions are not supported")
+    d.segment_bcs = np.zeros((max(d.no_segments_bc, 0), 5))
+    if d.no_segments_bc > 0:                     # 3_...:185-196
+        dat.text()
+        for i in range(d.no_segments_bc):
+            d.segment_bcs[i] = dat.reals(5)
     dat.text()
     d.no_nodes_bc = dat.int1()
-    if d.no_nodes_bc > 0:
-        raise DeckError("nodal boundary conditions are not supported")
+    d.nodal_bcs = np.zeros((max(d.no_nodes_bc, 0), 2))
+    if d.no_nodes_bc > 0:                        # 3_...:199-210
+        dat.text()
+        for i in range(d.no_nodes_bc):
+            d.nodal_bcs[i] = dat.reals(2)
     dat.text()
     d.ic_tcurve = dat.int1()
     d.ttcurves, d.ftcurves = [], []

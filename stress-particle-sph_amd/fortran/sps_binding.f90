@@ -12,6 +12,7 @@ module sps_binding
 
   integer(c_int32_t), parameter :: SPS_ABI_VERSION = 2
   integer, parameter :: SPS_MAX_TCURVE = 64
+  integer, parameter :: SPS_MAX_BCS = 32, SPS_MAX_CURVES = 16
   integer(c_int), parameter :: SPS_OK = 0
   integer(c_int32_t), parameter :: SPS_MODE_ORDERED = 0, SPS_MODE_FAST = 1
 
@@ -42,6 +43,12 @@ module sps_binding
      type(c_ptr) :: x_10 = c_null_ptr, disp_10 = c_null_ptr
   end type sps_state_t
 
+  type, bind(C) :: sps_bcs_t
+     integer(c_int32_t) :: no_bcs, ifsigman, ncurves, reserved0
+     type(c_ptr) :: bc_list = c_null_ptr, bc_info = c_null_ptr, bc_or_not = c_null_ptr
+     type(c_ptr) :: curve_npts = c_null_ptr, curve_t = c_null_ptr, curve_f = c_null_ptr
+  end type sps_bcs_t
+
   type, bind(C) :: sps_stats_t
      integer(c_int64_t) :: n_pairs, m_pairs, n_by_type(10)
      integer(c_int32_t) :: ndivx, ndivy
@@ -60,6 +67,12 @@ module sps_binding
        type(c_ptr), value :: h
        type(sps_state_t), intent(in) :: st
      end function sps_set_state
+
+     integer(c_int) function sps_set_bcs(h, b) bind(C, name="sps_set_bcs")
+       import :: c_int, c_ptr, sps_bcs_t
+       type(c_ptr), value :: h
+       type(sps_bcs_t), intent(in) :: b
+     end function sps_set_bcs
 
      integer(c_int) function sps_step(h, nsteps, dt, time_sph, itimestep_sph) bind(C, name="sps_step")
        import :: c_int, c_int32_t, c_double, c_ptr

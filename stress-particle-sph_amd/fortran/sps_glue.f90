@@ -22,7 +22,7 @@ module sps_glue
   use variable_types
   use SPH_time_vars_2018
   use SPH_global_vars_2018
-  use SPH_material_vars_2018, only: ic_grav, tcurve_grav, cgrav, ft_grav, dampingTG
+  use SPH_material_vars_2018, only: ic_grav, tcurve_grav, cgrav, ft_grav, dampingTG, bc_list, ic_tcurve
   implicit none
   private
   public :: sps_time_integration, sps_glue_sync, sps_glue_finish
@@ -114,7 +114,38 @@ contains
                        f_drucker, x00, displ, no_walls, no_walls, .false., &
                        x_10, disp_10, .not. inside_approach)
     end if
+    if (no_bcs > 0) call set_bcs(bc_info, BC_or_not)
   end subroutine open_handle
+
+  ! Prescribed BCs: bc_list as read, the nodes' columns of bc_info and
+  ! BC_or_not, and the time curves repacked one after another
+  ! (ttcurves(ntcurves, mpts) holds a curve's points with stride ntcurves).
+  subroutine set_bcs(info_, bcn_)
+    integer, target, intent(in) :: info_(*), bcn_(*)
+    type(sps_bcs_t) :: b
+    integer(c_int32_t), target :: npts(SPS_MAX_CURVES)
+    real(c_double), target :: ct(SPS_MAX_TCURVE, SPS_MAX_CURVES)
+    real(c_float), target :: cf(SPS_MAX_TCURVE, SPS_MAX_CURVES)
+    real(c_double), allocatable, target :: bl(:,:)
+    integer :: c, k, nc
+    npts = 0; ct = 0.0_c_double; cf = 0.0_c_float
+    nc = 0
+    if (ic_tcurve == 1) nc = min(ntcurves, SPS_MAX_CURVES)
+    do c = 1, nc
+       npts(c) = min(nptstcurves(c), SPS_MAX_TCURVE)
+       do k = 1, npts(c)
+          ct(k, c) = ttcurves(c, k)
+          cf(k, c) = ftcurves(c, k)
+       end do
+    end do
+    allocate(bl(8, no_bcs))
+    bl = bc_list(1:8, 1:no_bcs)
+    b%no_bcs = no_bcs; b%ifsigman = ifsigman; b%ncurves = nc; b%reserved0 = 0
+    b%bc_list = c_loc(bl); b%bc_info = c_loc(info_); b%bc_or_not = c_loc(bcn_)
+    b%curve_npts = c_loc(npts); b%curve_t = c_loc(ct); b%curve_f = c_loc(cf)
+    call sps_check(sps_set_bcs(handle, b), 'sps_set_bcs')
+    deallocate(bl)
+  end subroutine set_bcs
 
   ! The module arrays arrive as assumed-size TARGET dummies, so their
   ! addresses are valid for the duration of the C call made here.

@@ -155,6 +155,41 @@ def outside_stress_points(xn, yn, npoints, r_x, r_y):
     return xs.ravel(), ys.ravel()
 
 
+def bcs_on_particles(deck: Deck, x):
+    """Get_BCs_on_node (3_...:1030-1113) over the particles 1..ntotal at
+    their positions x (ntotal, 2): bc_info (ntotal, 8) int32 (column 0 the
+    1-based particle number, 1 how many BCs, 2.. the BC numbers) and
+    bc_or_not (ntotal) int32 (1 where a BC applies).  A particle is on a
+    segment when (y1-y2)(x-x1) + (x2-x1)(y-y1) is exactly zero in double
+    precision and it lies in the segment's box; segments add their BC in the
+    next column, nodal BCs put theirs in the column of its variable (the
+    reference asks on standard input when that column already holds another
+    BC: here that is an error)."""
+    n = len(x)
+    info = np.zeros((n, 8), dtype=np.int32)
+    info[:, 0] = np.arange(1, n + 1)
+    bc_or_not = np.zeros(n, dtype=np.int32)
+    xx, yy = x[:, 0], x[:, 1]
+    for x1, y1, x2, y2, nb in deck.segment_bcs:
+        z = (y1 - y2) * (xx - x1) + (x2 - x1) * (yy - y1)
+        on = (z == 0.0) & (xx >= min(x1, x2)) & (xx <= max(x1, x2)) & (yy >= min(y1, y2)) & (yy <= max(y1, y2))
+        idx = np.nonzero(on)[0]
+        info[idx, 1] += 1
+        if np.any(info[idx, 1] > 6):
+            raise DeckError("more than 6 boundary conditions on one particle")
+        info[idx, info[idx, 1] + 1] = int(nb)
+        bc_or_not[idx] = 1
+    for node, nb in deck.nodal_bcs:
+        i, nb = int(node) - 1, int(nb)
+        k = int(deck.bc_list[nb - 1, 1]) + 1
+        info[i, 1] += 1
+        if info[i, k] not in (0, nb):
+            raise DeckError(f"node {i + 1}: two boundary conditions for variable {k - 1}")
+        info[i, k] = nb
+        bc_or_not[i] = 1
+    return info, bc_or_not
+
+
 def build_particles(deck: Deck) -> Particles:
     if deck.geom_type != 1:
         raise DeckError("only geom_type 1 (rectangle) is implemented, as in the reference (3_...:388-390)")

@@ -49,6 +49,10 @@ PLAN = {
     "b5_bui_velvec": ([1, 2, 10, 100, 500], [1, 2, 100], False, "bui"),
     # velocity-vector shifts on every step (displacement tolerance 1e-5 dx)
     "b6_bui_velvec_shift": ([1, 2, 10, 100, 500], [1, 2, 100], False, "bui"),
+    # the reference's elastic vertical cut with prescribed velocities and
+    # stresses on its sides (no position update), 1 and 3 stress points per cell
+    "v1_vslope_sp1": ([1, 2, 10, 100, 1000, 2000], [1, 2, 10, 100], False),
+    "v3_vslope_sp3": ([1, 2, 10, 100, 1000], [1, 2, 100], False),
 }
 
 STATE_FIELDS = ("x", "vel", "stress", "ivar1", "f_drucker", "ddev_strn", "grad_u", "displ",

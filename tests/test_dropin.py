@@ -7,11 +7,13 @@ in the build container (they contain the reference's compiled modules and
 travel with the tree as built files).  Their state dumps must equal the
 reference's own, bit for bit.
 """
+import hashlib
 import os
 import shutil
 import subprocess
 import sys
 
+import numpy as np
 import pytest
 
 from conftest import ROOT, bits_equal, deck_dir, golden
@@ -42,16 +44,21 @@ def check(states, deck_name):
     g = golden(deck_name)
     for st, s in states.items():
         for k in FIELDS:
-            assert bits_equal(g[f"s{st}_{k}"], s[k]), f"step {st} {k}"
+            if f"s{st}_{k}" in g.files:
+                assert bits_equal(g[f"s{st}_{k}"], s[k]), f"step {st} {k}"
+            else:                   # fixtures that keep only a digest of this field
+                digest = hashlib.sha256(np.ascontiguousarray(s[k]).tobytes()).hexdigest()
+                assert digest == str(g[f"s{st}_sha_{k}"]), f"step {st} {k}"
 
 
-@pytest.mark.parametrize("deck_name,steps", [("t1_tiny", [1, 50, 200]), ("t2_cspm", [1, 50, 200])])
+@pytest.mark.parametrize("deck_name,steps", [("t1_tiny", [1, 50, 200]), ("t2_cspm", [1, 50, 200]),
+                                             ("v1_vslope_sp1", [1, 10, 100])])
 def test_fortran_dropin_oracle(deck_name, steps, tmp_path):
     check(run_dropin("sps_dropin_oracle", deck_name, steps, tmp_path), deck_name)
 
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("deck_name,steps", [("t1_tiny", [1, 50, 200]), ("t2_cspm", [1, 50, 200]),
-                                             ("t3_elastic", [1, 50, 200])])
+                                             ("t3_elastic", [1, 50, 200]), ("v1_vslope_sp1", [1, 10, 100])])
 def test_fortran_dropin_hip(deck_name, steps, tmp_path):
     check(run_dropin("sps_dropin_hip", deck_name, steps, tmp_path), deck_name)

@@ -118,12 +118,13 @@ def test_small_decks_vs_reference(deck_name):
 
 
 @pytest.mark.parametrize("deck_name", ["c1_collapse", "b1_bui_sp1", "b2_bui_sp2", "b3_bui_sp3", "b4_bui_outside",
-                                       "b5_bui_velvec", "b6_bui_velvec_shift"])
+                                       "b5_bui_velvec", "b6_bui_velvec_shift", "v1_vslope_sp1", "v3_vslope_sp3"])
 def test_digest_decks_vs_reference(deck_name):
     """GPU state digests against the reference Fortran's dumps: the granular
     column collapse (1000 steps of dt = 1e-5) and the Bui
     et al. (2008) slope of that fork (XSPH, 1 to 3 stress points per cell; the
-    outside approach with fixed and velocity-vector shifts)."""
+    outside approach with fixed and velocity-vector shifts) and the elastic
+    vertical cut with prescribed velocities and stresses."""
     import hashlib
     g = golden(deck_name)
     steps = sorted(int(k[1:].split("_")[0]) for k in g.files if k.endswith("_sha_x") and k != "s0_sha_x")
