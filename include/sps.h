@@ -211,6 +211,7 @@ int sps_get_stats(sps_handle *h, sps_stats *out);
  * of sps_params for the pair loops, the Drucker-Prager terms and the return
  * map, particle by particle. */
 int sps_set_materials(sps_handle *h, int32_t nmat, const double *table, const int32_t *mat);
+/* (SPS_EINVAL with the Perzyna models: they read props(1,:) only.) */
 
 /* Prescribed velocity / stress values (Normal_BCs, 3_...:1644-1753) at the
  * points where the reference calls BCs: after each RK stage's update and

@@ -145,14 +145,17 @@ static void free_all(struct sps_handle *h)
 }
 
 /* ------------------------------------------------------- Check_Out_Domain */
-/* code/2_SPH_main_2018.f90:1170-1194 (sticky flag) */
+/* code/2_SPH_main_2018.f90:1170-1194 (sticky flag).  A non-finite
+ * coordinate also flags the particle (dxx > 0 is false for NaN, and the
+ * reference's grid_find then takes int() of NaN, which Fortran leaves
+ * undefined): the search never sees it. */
 static void check_out_domain(struct sps_handle *h)
 {
     for (int i = 0; i < h->nt2; ++i)
         for (int d = 0; d < 2; ++d) {
             double dxx = (h->x[2 * i + d] - h->p.xmin_domain[d]) *
                          (h->x[2 * i + d] - h->p.xmax_domain[d]);
-            if (dxx > 0.0) h->ood[i] = 1;
+            if (!(dxx <= 0.0)) h->ood[i] = 1;
         }
 }
 
@@ -555,6 +558,99 @@ static void get_derivatives(struct sps_handle *h)
 /* --------------------------------------------- plastic_terms / drucker_prager */
 /* code/3_SPH_material_2018.f90:1867-1933 -> 1937-2062 -> 2557-2635.
  * Returns the source Gs(4) = -G and der_intvars(1). */
+/* ------------------------------------------------- Perzyna viscoplasticity */
+/* plastic_terms' ncrit <= 4 branch (3_...:1912-1923 of the strain-
+ * localisation fork, whose sources the reference builds from): Get_Vivel
+ * (2168-2190) -> invar09 (2196-2336): invariants, yield measure and the
+ * current yield size fdatm0 + H evpstn limited below by 0.1 fdatm0;
+ * yieldf09 (2340-2472): the flow vector a = c1 a1 + c2 a2 + c3 a3;
+ * flowvp09 (2476-2525): vivel = gamma <F/fdatm>^delta a (or gamma (exp(delta
+ * F/fdatm) - 1) a when n != 1) past the 0.01 threshold.  sqrt(3.0) and the
+ * literals 0.017453292, 57.29577951308, 0.1, 0.01 are default reals, as the
+ * reference writes them.  Writes vivel (zero below yield). */
+static void perzyna_vivel(const struct sps_handle *h, const double s[4], double evpstn, double vivel[4])
+{
+    const double *pr = h->p.props;
+    const int ncrit = (int)pr[1];
+    const double root3 = (double)sqrtf(3.0f);
+    const double smean = (s[0] + s[1] + s[3]) / 3.0;
+    const double dv[4] = {s[0] - smean, s[1] - smean, s[2], s[3] - smean};
+    const double varj2 = dv[2] * dv[2] + 0.5 * (dv[0] * dv[0] + dv[1] * dv[1] + dv[3] * dv[3]);
+    const double varj3 = dv[3] * (dv[3] * dv[3] - varj2);
+    const double steff = sqrt(varj2);
+    double sint3;
+    if (steff != 0.0) {
+        sint3 = -(3.0 * root3 * varj3 / (2.0 * varj2 * steff));
+        if (sint3 > 1.0) sint3 = 1.0;
+    } else {
+        sint3 = 0.0;
+    }
+    if (sint3 < -1.0) sint3 = -1.0;
+    if (sint3 > 1.0) sint3 = 1.0;
+    const double theta = asin(sint3) / 3.0;
+    const double snphi = sin(pr[8] * (double)0.017453292f);
+    double yield = 0.0;
+    if (ncrit == 1) yield = 2.0 * cos(theta) * steff;
+    else if (ncrit == 2) yield = root3 * steff;
+    else if (ncrit == 3) yield = smean * snphi + steff * (cos(theta) - sin(theta) * snphi / root3);
+    else if (ncrit == 4) yield = 6.0 * smean * snphi / (root3 * (3.0 - snphi)) + steff;
+    const double fdatm0 = pr[6], hards = pr[7];
+    double fdatm = fdatm0 + hards * evpstn;
+    double fact = fabs(fdatm) / fabs(fdatm0);
+    if (fact < (double)0.1f) fact = (double)0.1f;
+    fdatm = fdatm0 * fact;
+    vivel[0] = vivel[1] = vivel[2] = vivel[3] = 0.0;
+    if (!(yield > fdatm)) return;
+    /* yieldf09 */
+    const double a1[4] = {1.0, 1.0, 0.0, 1.0};
+    double a2[4] = {0.0, 0.0, 0.0, 0.0}, a3[4];
+    if (steff > 0) {
+        for (int i = 0; i < 4; ++i) a2[i] = dv[i] / (2.0 * steff);
+        a2[2] = dv[2] / steff;
+    }
+    a3[0] = dv[1] * dv[3] + varj2 / 3.0;
+    a3[1] = dv[0] * dv[3] + varj2 / 3.0;
+    a3[2] = -(2.0 * dv[2] * dv[3]);
+    a3[3] = dv[0] * dv[1] - dv[2] * dv[2] + varj2 / 3.0;
+    const double tanth = tan(theta), tant3 = tan(3.0 * theta), sinth = sin(theta), costh = cos(theta);
+    const double cost3 = cos(3.0 * theta);
+    double c1 = 0.0, c2 = 0.0, c3 = 0.0;
+    if (ncrit == 1) {
+        const double abthe = fabs(theta * (double)57.29577951308f);
+        if (abthe >= 29.0) {
+            c2 = root3;
+            c3 = 0.0;
+        } else {
+            c2 = 2.0 * (costh + sinth * tant3);
+            c3 = root3 * sinth / (varj2 * cost3);
+        }
+    } else if (ncrit == 2) {
+        c2 = root3;
+    } else if (ncrit == 3) {
+        c1 = sin(pr[8] * (double)0.017453292f) / 3.0;
+        const double abthe = fabs(theta * (double)57.29577951308f);
+        if (abthe >= 29.0) {
+            const double plumi = theta > 0.0 ? -1.0 : 1.0;
+            c2 = 0.5 * (root3 + plumi * c1 * root3);
+        } else {
+            c2 = costh * ((1.0 + tanth * tant3) + c1 * (tant3 - tanth) * root3);
+            c3 = (root3 * sinth + 3.0 * c1 * costh) / (2.0 * varj2 * cost3);
+        }
+    } else if (ncrit == 4) {
+        c1 = 2.0 * snphi / (root3 * (3.0 - snphi));
+        c2 = 1.0;
+    }
+    double av[4];
+    for (int i = 0; i < 4; ++i) av[i] = c1 * a1[i] + c2 * a2[i] + c3 * a3[i];
+    /* flowvp09 */
+    const double gamma = pr[9], delta = pr[10], nflow = pr[11];
+    const double fnorm = (yield - fdatm) / fdatm;
+    if (fnorm >= (double)0.01f) {
+        const double cmult = nflow != 1 ? gamma * (exp(delta * fnorm) - 1.0) : gamma * pow(fnorm, delta);
+        for (int i = 0; i < 4; ++i) vivel[i] = cmult * av[i];
+    }
+}
+
 static void plastic_terms(struct sps_handle *h, int ie, double time_sph, double Gs[4], double *der1)
 {
     const double *props = h->p.props;
@@ -606,6 +702,15 @@ static void plastic_terms(struct sps_handle *h, int ie, double time_sph, double 
                 for (int i = 0; i < 4; ++i) vivel[i] = vivel[i] * (double)G2;
             }
             for (int i = 0; i < 4; ++i) Gs[i] = -G[i];
+        } else if (ncrit >= 1 && ncrit <= 4) {
+            /* Gs = -Dmatx vivel, Get_Dmatx (3_...:2530-2561) */
+            perzyna_vivel(h, h->stress + 4 * ie, h->ivars[(size_t)h->niv * ie], vivel);
+            const double young = props[2], poiss = props[3];
+            const double c = young * (1.0 - poiss) / ((1.0 + poiss) * (1.0 - 2.0 * poiss));
+            const double c12 = c * poiss / (1.0 - poiss), c33 = (1.0 - 2.0 * poiss) * c / (2.0 * (1.0 - poiss));
+            const double D[4][4] = {{c, c12, 0.0, c12}, {c12, c, 0.0, c12}, {0.0, 0.0, c33, 0.0}, {c12, c12, 0.0, c}};
+            for (int i = 0; i < 4; ++i)
+                for (int j = 0; j < 4; ++j) Gs[i] = Gs[i] - D[i][j] * vivel[j];
         }
         /* Get_derivative_intvars (3_...:2614-2617) */
         *der1 = sqrt((2.0 * (vivel[0] * vivel[0] + vivel[1] * vivel[1] + vivel[3] * vivel[3]) +
@@ -1125,7 +1230,11 @@ static int check_params(const sps_params *p)
         return fail(SPS_EINVAL, "prescribed BCs need update_x = 0 (no free-surface detection) and SPS_MODE_ORDERED");
     if (p->xsph && p->mode != SPS_MODE_ORDERED) return fail(SPS_EINVAL, "XSPH runs in SPS_MODE_ORDERED");
     int ntype_eco = (int)p->props[0], ncrit = (int)p->props[1];
-    if (ntype_eco > 1 && ncrit <= 5) return fail(SPS_EINVAL, "Perzyna models (ncrit<=5) are not supported");
+    if (ntype_eco > 1 && ncrit == 5) return fail(SPS_EINVAL, "the Modified Cam Clay model (ncrit 5) is not supported");
+    if (ntype_eco > 1 && ncrit <= 4 && (ncrit < 1 || !(p->props[6] > (double)0.001f)))
+        return fail(SPS_EINVAL, "Perzyna models need ncrit 1..4 and an initial yield size props(7) > 0.001");
+    if (ntype_eco > 1 && ncrit <= 4 && p->mode != SPS_MODE_ORDERED)
+        return fail(SPS_EINVAL, "Perzyna models run in SPS_MODE_ORDERED");
     if (p->ic_grav && (p->tcurve_npts < 2 || p->tcurve_npts > SPS_MAX_TCURVE))
         return fail(SPS_EINVAL, "gravity time curve needs 2..%d points", SPS_MAX_TCURVE);
     if (p->mode != SPS_MODE_ORDERED && p->mode != SPS_MODE_FAST) return fail(SPS_EINVAL, "unknown mode");
@@ -1291,6 +1400,8 @@ int sps_get_stats(sps_handle *h, sps_stats *out)
 int sps_set_materials(sps_handle *h, int32_t nmat, const double *table, const int32_t *mat)
 {
     if (!h || nmat < 1 || !table || !mat) return fail(SPS_EINVAL, "bad argument");
+    if ((int)h->p.props[0] > 1 && (int)h->p.props[1] <= 4)
+        return fail(SPS_EINVAL, "a material per particle is for the Drucker-Prager return map, not the Perzyna models");
     for (int i = 0; i < h->nt2; ++i)
         if (mat[i] < 0 || mat[i] >= nmat) return fail(SPS_EINVAL, "particle %d: material %d of %d", i + 1, mat[i], nmat);
     free(h->mat);

@@ -327,7 +327,11 @@ static int check_params(const sps_params *p) {
         return fail(SPS_EINVAL, "prescribed BCs need update_x = 0 (no free-surface detection) and SPS_MODE_ORDERED");
     if (p->xsph && p->mode != SPS_MODE_ORDERED) return fail(SPS_EINVAL, "XSPH runs in SPS_MODE_ORDERED");
     int ntype_eco = (int)p->props[0], ncrit = (int)p->props[1];
-    if (ntype_eco > 1 && ncrit <= 5) return fail(SPS_EINVAL, "Perzyna models (ncrit<=5) are not supported");
+    if (ntype_eco > 1 && ncrit == 5) return fail(SPS_EINVAL, "the Modified Cam Clay model (ncrit 5) is not supported");
+    if (ntype_eco > 1 && ncrit <= 4 && (ncrit < 1 || !(p->props[6] > (double)0.001f)))
+        return fail(SPS_EINVAL, "Perzyna models need ncrit 1..4 and an initial yield size props(7) > 0.001");
+    if (ntype_eco > 1 && ncrit <= 4 && p->mode != SPS_MODE_ORDERED)
+        return fail(SPS_EINVAL, "Perzyna models run in SPS_MODE_ORDERED");
     if (p->ic_grav && (p->tcurve_npts < 2 || p->tcurve_npts > SPS_MAX_TCURVE))
         return fail(SPS_EINVAL, "gravity time curve needs 2..%d points", SPS_MAX_TCURVE);
     if (p->mode != SPS_MODE_ORDERED && p->mode != SPS_MODE_FAST) return fail(SPS_EINVAL, "unknown mode %d", p->mode);
@@ -432,6 +436,10 @@ extern "C" int sps_create(const sps_params *params, sps_handle **out) {
     d.D41 = params->D41; d.D42 = params->D42;
     d.tanfi = params->props[12]; d.coh = params->props[13];
     d.young = params->props[2]; d.poiss = params->props[3];
+    d.pz_fdatm0 = params->props[6]; d.pz_hards = params->props[7]; d.pz_gamma = params->props[9];
+    d.pz_delta = params->props[10]; d.pz_nflow = params->props[11];
+    d.pz_root3 = (double)sqrtf(3.0f);
+    d.pz_snphi = std::sin(params->props[8] * (double)0.017453292f);
     for (int a = 0; a < 2; ++a) {
         d.xmin_dom[a] = params->xmin_domain[a];
         d.xmax_dom[a] = params->xmax_domain[a];
@@ -1067,6 +1075,8 @@ extern "C" int sps_get_stats(sps_handle *h, sps_stats *out) {
 extern "C" int sps_set_materials(sps_handle *h, int32_t nmat, const double *table, const int32_t *mat) {
     if (!h || nmat < 1 || !table || !mat) return fail(SPS_EINVAL, "bad argument");
     if (!h->have_state) return fail(SPS_ESTATE, "sps_set_materials before sps_set_state");
+    if (h->dp.ntype_eco > 1 && h->dp.ncrit <= 4)
+        return fail(SPS_EINVAL, "a material per particle is for the Drucker-Prager return map, not the Perzyna models");
     for (int i = 0; i < h->nt2; ++i)
         if (mat[i] < 0 || mat[i] >= nmat) return fail(SPS_EINVAL, "particle %d: material %d of %d", i + 1, mat[i], nmat);
     HIP_TRY(hipSetDevice(h->device));

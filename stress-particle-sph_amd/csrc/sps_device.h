@@ -24,6 +24,9 @@ struct DevParams {
     double young, poiss;               // props(1,3), props(1,4)
     double xmin_dom[2], xmax_dom[2];
     double cgrav[2];
+    // Perzyna models (ntype_eco > 1, ncrit 1..4): props(1,7:12), sqrt(3.0) in
+    // single precision, sin(props(1,9) * 0.017453292) from the host's libm
+    double pz_fdatm0, pz_hards, pz_gamma, pz_delta, pz_nflow, pz_root3, pz_snphi;
 };
 
 // One material's constants (an ensemble of instances has one per instance;
@@ -247,13 +250,102 @@ __device__ __forceinline__ void adapt_stress(double4 &s, double tanfi, double co
     }
 }
 
+// Perzyna viscoplastic strain rate of one stress point (ncrit 1 Tresca, 2 von
+// Mises, 3 Mohr-Coulomb, 4 Drucker-Prager; 3_...:2168-2525 of the strain-
+// localisation fork): invar09's invariants and yield measure, the current
+// yield size fdatm0 + H evpstn (at least 0.1 fdatm0), yieldf09's flow vector
+// and flowvp09's overstress function, with the reference's single-precision
+// literals.  Zero below the yield size or under the 0.01 threshold.
+// <F>^delta with delta = 1 is F itself (what a correctly rounded pow gives).
+__device__ __forceinline__ void perzyna_vivel(const DevParams &P, const double4 &st, double evpstn, double v[4]) {
+    const int ncrit = P.ncrit;
+    const double root3 = P.pz_root3;
+    const double smean = (st.x + st.y + st.w) / 3.0;
+    const double dv0 = st.x - smean, dv1 = st.y - smean, dv2 = st.z, dv3 = st.w - smean;
+    const double varj2 = dv2 * dv2 + 0.5 * (dv0 * dv0 + dv1 * dv1 + dv3 * dv3);
+    const double varj3 = dv3 * (dv3 * dv3 - varj2);
+    const double steff = sqrt(varj2);
+    const double snphi = P.pz_snphi;
+    double theta = 0.0;
+    if (ncrit == 1 || ncrit == 3) {   // the other criteria never use the Lode angle
+        double sint3;
+        if (steff != 0.0) {
+            sint3 = -(3.0 * root3 * varj3 / (2.0 * varj2 * steff));
+            if (sint3 > 1.0) sint3 = 1.0;
+        } else {
+            sint3 = 0.0;
+        }
+        if (sint3 < -1.0) sint3 = -1.0;
+        if (sint3 > 1.0) sint3 = 1.0;
+        theta = asin(sint3) / 3.0;
+    }
+    double yield = 0.0;
+    if (ncrit == 1) yield = 2.0 * cos(theta) * steff;
+    else if (ncrit == 2) yield = root3 * steff;
+    else if (ncrit == 3) yield = smean * snphi + steff * (cos(theta) - sin(theta) * snphi / root3);
+    else if (ncrit == 4) yield = 6.0 * smean * snphi / (root3 * (3.0 - snphi)) + steff;
+    double fdatm = P.pz_fdatm0 + P.pz_hards * evpstn;
+    double fact = fabs(fdatm) / fabs(P.pz_fdatm0);
+    if (fact < (double)0.1f) fact = (double)0.1f;
+    fdatm = P.pz_fdatm0 * fact;
+    v[0] = v[1] = v[2] = v[3] = 0.0;
+    if (!(yield > fdatm)) return;
+    double a2[4] = {0.0, 0.0, 0.0, 0.0};
+    if (steff > 0) {
+        a2[0] = dv0 / (2.0 * steff);
+        a2[1] = dv1 / (2.0 * steff);
+        a2[3] = dv3 / (2.0 * steff);
+        a2[2] = dv2 / steff;
+    }
+    const double a3[4] = {dv1 * dv3 + varj2 / 3.0, dv0 * dv3 + varj2 / 3.0, -(2.0 * dv2 * dv3),
+                          dv0 * dv1 - dv2 * dv2 + varj2 / 3.0};
+    double c1 = 0.0, c2 = 0.0, c3 = 0.0;
+    if (ncrit == 1 || ncrit == 3) {
+        const double tanth = tan(theta), tant3 = tan(3.0 * theta), sinth = sin(theta), costh = cos(theta);
+        const double cost3 = cos(3.0 * theta);
+        const double abthe = fabs(theta * (double)57.29577951308f);
+        if (ncrit == 1) {
+            if (abthe >= 29.0) {
+                c2 = root3;
+            } else {
+                c2 = 2.0 * (costh + sinth * tant3);
+                c3 = root3 * sinth / (varj2 * cost3);
+            }
+        } else {
+            c1 = snphi / 3.0;
+            if (abthe >= 29.0) {
+                const double plumi = theta > 0.0 ? -1.0 : 1.0;
+                c2 = 0.5 * (root3 + plumi * c1 * root3);
+            } else {
+                c2 = costh * ((1.0 + tanth * tant3) + c1 * (tant3 - tanth) * root3);
+                c3 = (root3 * sinth + 3.0 * c1 * costh) / (2.0 * varj2 * cost3);
+            }
+        }
+    } else if (ncrit == 2) {
+        c2 = root3;
+    } else {
+        c1 = 2.0 * snphi / (root3 * (3.0 - snphi));
+        c2 = 1.0;
+    }
+    const double a1[4] = {1.0, 1.0, 0.0, 1.0};
+    double av[4];
+    for (int i = 0; i < 4; ++i) av[i] = c1 * a1[i] + c2 * a2[i] + c3 * a3[i];
+    const double fnorm = (yield - fdatm) / fdatm;
+    if (fnorm >= (double)0.01f) {
+        double cmult;
+        if (P.pz_nflow != 1) cmult = P.pz_gamma * (exp(P.pz_delta * fnorm) - 1.0);
+        else cmult = P.pz_gamma * (P.pz_delta == 1.0 ? fnorm : pow(fnorm, P.pz_delta));
+        for (int i = 0; i < 4; ++i) v[i] = cmult * av[i];
+    }
+}
+
 // plastic_terms -> drucker_prager -> Get_derivative_intvars for one stress
 // point (code/3_SPH_material_2018.f90:1867-1933, 1937-2062, 2557-2635).
 // st: stress, gu: grad_u as [g11, g21, g12, g22]; fdr: f_drucker in/out.
 // Gs receives the stress source (-G), der1 the deviatoric plastic strain rate.
 __device__ __forceinline__ void plastic_terms(const DevParams &P, const double4 &st,
                                               const double4 &gu, double &fdr, double time_sph,
-                                              double4 &Gs, double &der1) {
+                                              double evpstn, double4 &Gs, double &der1) {
     double G0 = 0.0, G1 = 0.0, G2a = 0.0, G3 = 0.0;
     double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
     Gs = make_double4(0.0, 0.0, 0.0, 0.0);
@@ -304,6 +396,21 @@ __device__ __forceinline__ void plastic_terms(const DevParams &P, const double4 
                 v3 = v3 * (double)G2;
             }
             Gs = make_double4(-G0, -G1, -G2a, -G3);
+        } else if (P.ncrit >= 1 && P.ncrit <= 4) {
+            // Gs = -Dmatx vivel with Get_Dmatx (3_...:2530-2561), row by row
+            double vv[4];
+            perzyna_vivel(P, st, evpstn, vv);
+            const double c = P.young * (1.0 - P.poiss) / ((1.0 + P.poiss) * (1.0 - 2.0 * P.poiss));
+            const double c12 = c * P.poiss / (1.0 - P.poiss);
+            const double c33 = (1.0 - 2.0 * P.poiss) * c / (2.0 * (1.0 - P.poiss));
+            double g[4] = {0.0, 0.0, 0.0, 0.0};
+            const double D[4][4] = {{c, c12, 0.0, c12}, {c12, c, 0.0, c12}, {0.0, 0.0, c33, 0.0}, {c12, c12, 0.0, c}};
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) g[i] = g[i] - D[i][j] * vv[j];
+            Gs = make_double4(g[0], g[1], g[2], g[3]);
+            v0 = vv[0]; v1 = vv[1]; v2 = vv[2]; v3 = vv[3];
         }
         der1 = sqrt((2.0 * (v0 * v0 + v1 * v1 + v3 * v3) + v2 * v2) / 3.0);
     } else {

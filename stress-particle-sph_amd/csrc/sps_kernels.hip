@@ -59,7 +59,9 @@ __global__ void k_check_ood(const double2 *__restrict__ x, int *__restrict__ ood
     double2 p = x[i];
     double d0 = (p.x - xmin0) * (p.x - xmax0);
     double d1 = (p.y - xmin1) * (p.y - xmax1);
-    if (d0 > 0.0 || d1 > 0.0) ood[i] = 1;
+    // d > 0 as the reference tests it; a NaN coordinate is flagged too, so
+    // that no cell index is ever derived from it
+    if (!(d0 <= 0.0) || !(d1 <= 0.0)) ood[i] = 1;
 }
 
 // ------------------------------------------------------------- grid bounds
@@ -1522,7 +1524,7 @@ __device__ __forceinline__ void sp_stage_tail(const DevParams &P, const StateBuf
     double4 Gs;
     double der1;
     double fdr = S.fdr[r];
-    plastic_terms(P, st, gu, fdr, time_sph, Gs, der1);
+    plastic_terms(P, st, gu, fdr, time_sph, S.ivar1[r], Gs, der1);
     S.fdr[r] = fdr;
     double4 sp = make_double4(0, 0, 0, 0);
     if (P.update_x) {

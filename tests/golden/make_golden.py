@@ -53,6 +53,12 @@ PLAN = {
     # stresses on its sides (no position update), 1 and 3 stress points per cell
     "v1_vslope_sp1": ([1, 2, 10, 100, 1000, 2000], [1, 2, 10, 100], False),
     "v3_vslope_sp3": ([1, 2, 10, 100, 1000], [1, 2, 100], False),
+    # Perzyna viscoplasticity on the tiny block: von Mises with softening,
+    # Drucker-Prager (delta 2), Tresca (exponential flow), Mohr-Coulomb
+    "p1_vm": ([1, 2, 10, 50, 100, 200], "all", True),
+    "p2_dp": ([1, 2, 10, 50, 100, 200], "all", True),
+    "p3_tresca": ([1, 2, 10, 50, 100, 200], "all", True),
+    "p4_mc": ([1, 2, 10, 50, 100, 200], "all", True),
 }
 
 STATE_FIELDS = ("x", "vel", "stress", "ivar1", "f_drucker", "ddev_strn", "grad_u", "displ",

@@ -61,6 +61,7 @@ def compare_pairs(gpu, ora, tag):
     ("t2_cspm", [1, 2, 50, 200]),
     ("t3_elastic", [1, 50, 200]),
     ("t4_xsph", [1, 2, 3, 10, 50, 200]),
+    ("p1_vm", [1, 2, 10, 50, 100, 200]),
 ])
 def test_small_decks_bitwise(deck_name, checkpoints):
     say(f"[gpu parity] deck={deck_name} checkpoints={checkpoints}")
@@ -93,7 +94,44 @@ def test_varied_smoothing_lengths_bitwise():
     ora.close()
 
 
-@pytest.mark.parametrize("deck_name", ["t1_tiny", "t2_cspm", "t3_elastic", "t4_xsph"])
+@pytest.mark.parametrize("deck_name", ["p2_dp", "p3_tresca", "p4_mc"])
+def test_perzyna_decks_near_oracle(deck_name):
+    """Perzyna models whose flow goes through pow with delta != 1, exp or the
+    Lode angle's asin / sin / cos / tan: the device's math library may round
+    these differently from the host's in the last bit, so the state is held to
+    a relative tolerance (field max-norm) against the oracle instead of bits:
+    1e-10 through step 100, 1e-8 at step 200 (the yielding block amplifies
+    last-bit differences, as it does a 1e-7 perturbation of the double-
+    precision run).  The pair set (indices and types, in list order) stays
+    exact, its single-precision kernel values within the same tolerance."""
+    say(f"[gpu parity, tolerance] deck={deck_name}")
+    gpu, ora = open_pair(deck_name)
+    done = 0
+    for st, tol in ((1, 1e-10), (10, 1e-10), (50, 1e-10), (100, 1e-10), (200, 1e-8)):
+        gpu.step(st - done)
+        ora.step(st - done)
+        done = st
+        pg, po = gpu.get_pairs(), ora.get_pairs()
+        for k in ("i", "j", "type"):
+            assert bits_equal(po[k], pg[k]), f"{deck_name} step {st}: pairs.{k} differ"
+        g, o = gpu.get_state(), ora.get_state()
+        g.update({"pair_" + k: pg[k] for k in ("w", "dwdx", "dwdy")})
+        o.update({"pair_" + k: po[k] for k in ("w", "dwdx", "dwdy")})
+        for k in FIELDS + ("internal_vars", "pair_w", "pair_dwdx", "pair_dwdy"):
+            a, b = np.asarray(o[k], dtype=np.float64), np.asarray(g[k], dtype=np.float64)
+            scale = max(np.abs(a).max(), 1e-300)
+            # fields the reference computes in single precision move by whole
+            # single-precision units once the positions differ
+            t = max(tol, 1e-6) if k in ("art_visc", "f_bound", "pair_w", "pair_dwdx", "pair_dwdy") else tol
+            assert np.abs(a - b).max() <= t * scale, f"{deck_name} step {st} {k}: {describe_diff(a, b)}"
+        yielded = int(np.count_nonzero(o["internal_vars"][gpu.nn:, 0]))
+        say(f"  step {st}: within {tol}; {yielded} stress points with viscoplastic strain")
+    assert yielded > 0
+    gpu.close()
+    ora.close()
+
+
+@pytest.mark.parametrize("deck_name", ["t1_tiny", "t2_cspm", "t3_elastic", "t4_xsph", "p1_vm"])
 def test_small_decks_vs_reference(deck_name):
     """The GPU state against the reference Fortran's own dumps."""
     say(f"[gpu vs reference fixture] deck={deck_name}")
