@@ -94,7 +94,7 @@ typedef struct sps_params {
                                 stress_point_update + adapt_stress2 after the
                                 search on shift steps (2_...:99-109) */
     int32_t no_bcs;          /* prescribed BCs (3_...:165): 0, or > 0 with
-                                sps_set_bcs (update_x = 0 only) */
+                                sps_set_bcs (not with xsph and update_x) */
     int32_t bforce_gate;     /* 0: wall force when inside_approach .and.
                                 dummy_nodes (code/2_...:742); 1: when
                                 inside_approach (Bui fork 2_...:742) */
@@ -219,9 +219,12 @@ int sps_set_materials(sps_handle *h, int32_t nmat, const double *table, const in
  * (2_...:788) and after the step's stress_point_update (2_...:134).  Replaces
  * the globals bc_list, bc_info, BC_or_not and the time curves
  * (3_...:160-260, 1030-1113).  Required before sps_step when
- * sps_params.no_bcs > 0, which sps_create accepts only with update_x = 0
- * (with update_x the reference's free-surface detection, 3_...:1116-1411,
- * changes BC_or_not every step) in SPS_MODE_ORDERED.  The BC time is
+ * sps_params.no_bcs > 0, which sps_create accepts in SPS_MODE_ORDERED, not
+ * with XSPH and update_x together.  The reference's free-surface detection
+ * (3_...:1116-1411, every step with update_x) never changes a node whose
+ * BC_or_not is 1, the only nodes Normal_BCs acts on, so it is not run; with
+ * XSPH, XSPH_update would mark such nodes next to the surface 3
+ * (2_...:224-230) and the detection then 0.  The BC time is
  * time_sph: the reference adds ic_time*dt_sph, ic_time being a local it never
  * sets (3_...:1622-1627).  Values are applied to the nodes 1..nnode with
  * BC_or_not = 1, as code/ does (the vertical-slope fork loops over ntotal;

@@ -1226,8 +1226,9 @@ static int check_params(const sps_params *p)
     if (p->cont_density || p->art_stress)
         return fail(SPS_EINVAL, "continuity density and artificial stress are not supported");
     if (p->no_bcs < 0 || p->no_bcs > SPS_MAX_BCS) return fail(SPS_EINVAL, "no_bcs must be 0..%d", SPS_MAX_BCS);
-    if (p->no_bcs > 0 && (p->update_x || p->mode != SPS_MODE_ORDERED))
-        return fail(SPS_EINVAL, "prescribed BCs need update_x = 0 (no free-surface detection) and SPS_MODE_ORDERED");
+    if (p->no_bcs > 0 && ((p->update_x && p->xsph) || p->mode != SPS_MODE_ORDERED))
+        return fail(SPS_EINVAL, "prescribed BCs run in SPS_MODE_ORDERED, and not with XSPH and the position update "
+                                "together (XSPH_update then re-marks BC nodes next to the free surface)");
     if (p->xsph && p->mode != SPS_MODE_ORDERED) return fail(SPS_EINVAL, "XSPH runs in SPS_MODE_ORDERED");
     int ntype_eco = (int)p->props[0], ncrit = (int)p->props[1];
     if (ntype_eco > 1 && ncrit == 5) return fail(SPS_EINVAL, "the Modified Cam Clay model (ncrit 5) is not supported");

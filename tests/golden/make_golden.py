@@ -59,6 +59,9 @@ PLAN = {
     "p2_dp": ([1, 2, 10, 50, 100, 200], "all", True),
     "p3_tresca": ([1, 2, 10, 50, 100, 200], "all", True),
     "p4_mc": ([1, 2, 10, 50, 100, 200], "all", True),
+    # the reference's biaxial strain-localisation test: von Mises Perzyna with
+    # softening, prescribed velocities with the position update, 2100 steps
+    "l1_localisation": ([1, 2, 10, 100, 1000, 2100], [1, 2, 10, 100], False),
 }
 
 STATE_FIELDS = ("x", "vel", "stress", "ivar1", "f_drucker", "ddev_strn", "grad_u", "displ",

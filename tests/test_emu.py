@@ -41,7 +41,7 @@ def emu_lib():
                                              ("c1_collapse", 3), ("t2_cspm:hvar", 40), ("t2_cspm:graph", 30),
                                              ("t4_xsph", 60), ("b3_bui_sp3", 20), ("b4_bui_outside", 20),
                                              ("b6_bui_velvec_shift:kick", 20), ("v3_vslope_sp3", 30), ("p1_vm", 60), ("p3_tresca", 60),
-                                             ("p4_mc", 60)])
+                                             ("p4_mc", 60), ("l1_localisation", 15)])
 def test_cpu_build_of_kernels_matches_oracle(emu_lib, deck_name, steps):
     env = dict(os.environ, LD_PRELOAD=_asan_runtime(),
                ASAN_OPTIONS="detect_leaks=0:detect_stack_use_after_return=0:halt_on_error=1")

@@ -156,7 +156,8 @@ def test_small_decks_vs_reference(deck_name):
 
 
 @pytest.mark.parametrize("deck_name", ["c1_collapse", "b1_bui_sp1", "b2_bui_sp2", "b3_bui_sp3", "b4_bui_outside",
-                                       "b5_bui_velvec", "b6_bui_velvec_shift", "v1_vslope_sp1", "v3_vslope_sp3"])
+                                       "b5_bui_velvec", "b6_bui_velvec_shift", "v1_vslope_sp1", "v3_vslope_sp3",
+                                       "l1_localisation"])
 def test_digest_decks_vs_reference(deck_name):
     """GPU state digests against the reference Fortran's dumps: the granular
     column collapse (1000 steps of dt = 1e-5) and the Bui
@@ -177,6 +178,12 @@ def test_digest_decks_vs_reference(deck_name):
         s["ivar1"] = np.ascontiguousarray(s["internal_vars"][:, 0])
         for k in ("x", "vel", "stress", "ivar1", "f_drucker", "ddev_strn", "grad_u", "displ", "f_bound",
                   "art_visc"):
+            if deck_name == "l1_localisation" and st <= 2:
+                # the reference's BC time adds an uninitialised local (a
+                # subnormal there): its first steps differ below 1e-300 only
+                if f"s{st}_{k}" in g.files:
+                    assert np.abs(g[f"s{st}_{k}"] - s[k]).max() < 1e-300, f"step {st} field {k}"
+                continue
             digest = hashlib.sha256(np.ascontiguousarray(s[k]).tobytes()).hexdigest()
             if digest != str(g[f"s{st}_sha_{k}"]):
                 msg = f"step {st} field {k} differs from the reference"

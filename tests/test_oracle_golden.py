@@ -77,7 +77,14 @@ DIGEST = {"c1_collapse": "BASELINE config 1 (5,151 + 5,000 particles, 1000 steps
           "b5_bui_velvec": "outside approach, velocity-vector orientation (no shift reaches disp_tol), 500 steps",
           "b6_bui_velvec_shift": "velocity-vector shifts on every step (disp_tol 1e-5), 500 steps",
           "v1_vslope_sp1": "elastic vertical cut with prescribed velocities and stresses, 2000 steps",
-          "v3_vslope_sp3": "the same cut with three stress points per cell, 1000 steps"}
+          "v3_vslope_sp3": "the same cut with three stress points per cell, 1000 steps",
+          "l1_localisation": "biaxial strain localisation: Perzyna von Mises, BCs with the position update"}
+# (deck, step) where the reference reads an uninitialised local: its BCs
+# take t = time_sph + ic_time*dt_sph with ic_time never set (3_...:1622-1627;
+# the compiled reference finds about 6.95e-310 there, a stack address), so
+# at step 1 its top velocity is 1.4e-311 where the restatement (ic_time = 0)
+# has 0; the subnormal differences are gone by step 10 (bitwise from there)
+SUBNORMAL = {("l1_localisation", 1), ("l1_localisation", 2)}
 
 
 def fixture_steps(g):
@@ -116,6 +123,10 @@ def test_oracle_digest_decks(deck_name, oracle_lib):
         s["ivar1"] = np.ascontiguousarray(s["internal_vars"][:, 0])
         for k in ("x", "vel", "stress", "ivar1", "f_drucker", "ddev_strn", "grad_u", "displ", "f_bound",
                   "art_visc"):
+            if (deck_name, st) in SUBNORMAL:   # fields kept only as digests are not checked there
+                if f"s{st}_{k}" in g.files:
+                    assert np.abs(g[f"s{st}_{k}"] - s[k]).max() < 1e-300, f"step {st} field {k}"
+                continue
             digest = hashlib.sha256(np.ascontiguousarray(s[k]).tobytes()).hexdigest()
             if digest != str(g[f"s{st}_sha_{k}"]):
                 msg = f"step {st} field {k}"
