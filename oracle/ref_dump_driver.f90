@@ -19,7 +19,13 @@
 ! (stress-particle-sph_amd/fortran/sps_glue.f90) instead of the reference's
 ! time_integration: the reference's readers set the problem up, the library
 ! behind include/sps.h advances it, and the state dumps can be compared with
-! the reference's own (pair lists are not dumped in that variant).
+! the reference's own (pair lists are not dumped in that variant).  With the
+! environment variable SPS_SYNC_AT_OUTPUT=1 that variant keeps the state on
+! the device (sps_glue_sync_at_output), advances the reference driver's
+! single-precision output timers as code/1_SPH_2018.f90:67-100 does, and
+! dumps a listed step only when the timers say the driver writes after it
+! (state_NNNNNN.bin); other listed steps dump the host's copy as it stands
+! (stale_NNNNNN.bin: the state of the last output step).
 !-------------------------------------------------------------------------------
 program sps_ref_dump
   use variable_types
@@ -28,12 +34,13 @@ program sps_ref_dump
   use SPH_material_vars_2018, only: cgrav, ft_grav, dampingTG
   use SPH_main_2018, only: Init_SPH, time_integration
 #ifdef SPS_DROPIN
-  use sps_glue, only: sps_time_integration, sps_glue_finish
+  use sps_glue, only: sps_time_integration, sps_glue_finish, sps_glue_sync_every_step, sps_glue_sync_at_output
 #endif
   implicit none
 
   character(len=512) :: outdir, arg
   integer :: nsteps, nargs, ia, istep, ndump
+  logical :: at_output, writes
   integer, allocatable :: dump_steps(:)
 
   nargs = command_argument_count()
@@ -66,11 +73,25 @@ program sps_ref_dump
   read(input_file, *) text
   read(input_file, *) print_step, save_step, plot_step
   dt_sph = dt
+  at_output = .false.
+#ifdef SPS_DROPIN
+  call get_environment_variable('SPS_SYNC_AT_OUTPUT', arg)
+  at_output = trim(arg) == '1'
+  if (at_output) then
+     sps_glue_sync_every_step = .false.
+     sps_glue_sync_at_output = .true.
+  end if
+#endif
+  time_print = print_step*dt; time_save = save_step*dt; time_plot = plot_step*dt
+  t_print_reset = 0.; t_save_reset = 0.; t_plot_reset = 0.
 
-  call dump_state(0)
+  call dump_state(0, 'state_')
 
   do istep = 1, nsteps
      current_ts = current_ts + 1
+     t_print_reset = t_print_reset + dt
+     t_plot_reset = t_plot_reset + dt
+     t_save_reset = t_save_reset + dt
      itimestep_sph = itimestep_sph + 1
 #ifdef SPS_DROPIN
      call sps_time_integration
@@ -79,8 +100,15 @@ program sps_ref_dump
 #endif
      time_sph = time_sph + dt_sph
      time = time + dt
+     writes = t_plot_reset >= time_plot .or. t_print_reset >= time_print
+     if (t_plot_reset >= time_plot) t_plot_reset = 0.0
+     if (t_print_reset >= time_print) t_print_reset = 0.0
+     if (at_output .and. .not. writes) then
+        if (any(dump_steps(1:max(ndump,1)) == istep)) call dump_state(istep, 'stale_')
+        cycle
+     end if
      if (any(dump_steps(1:max(ndump,1)) == istep)) then
-        call dump_state(istep)
+        call dump_state(istep, 'state_')
 #ifndef SPS_DROPIN
         call dump_pairs(istep)
 #endif
@@ -92,13 +120,14 @@ program sps_ref_dump
 
 contains
 
-  subroutine dump_state(step)
+  subroutine dump_state(step, prefix)
     integer, intent(in) :: step
+    character(len=*), intent(in) :: prefix
     character(len=640) :: fname
     integer :: u, hdr(8), i
     real(kind=DP) :: scal(16)
 
-    write(fname, '(A,A,I6.6,A)') trim(outdir), '/state_', step, '.bin'
+    write(fname, '(A,A,A,I6.6,A)') trim(outdir), '/', prefix, step, '.bin'
     open(newunit=u, file=trim(fname), access='stream', form='unformatted', status='replace')
     hdr = (/ int(z'53505331'), nnode, nstress, ndummy, ntotal, ntotal2, itimestep_sph, ndummy2 /)
     write(u) hdr

@@ -14,7 +14,12 @@
 ! step on the device and downloads the state back into the same globals, so
 ! the writers and the driver see what time_integration would have left.
 ! With sps_glue_sync_every_step set to .false. the download is skipped and a
-! driver that needs the state only at output times calls sps_glue_sync.
+! driver that needs the state only at output times calls sps_glue_sync; with
+! sps_glue_sync_at_output also set, the glue downloads on exactly the steps
+! after which the reference driver writes: those whose single-precision
+! timers, which the driver advances before calling time_integration, reach
+! their periods (code/1_SPH_2018.f90:72-100, 6_SPH_time_vars_2018.f90:31-36).
+! The state stays on the device in between.
 !-------------------------------------------------------------------------------
 module sps_glue
   use, intrinsic :: iso_c_binding
@@ -32,6 +37,7 @@ module sps_glue
   ! al. (2008) fork (1) -- wall-force gate and CSPM determinant threshold
   integer, save, public :: sps_glue_fork = 0
   logical, save, public :: sps_glue_sync_every_step = .true.
+  logical, save, public :: sps_glue_sync_at_output = .false.
 
 contains
 
@@ -44,7 +50,11 @@ contains
     t = time_sph
     it = int(itimestep_sph - 1, c_int32_t)
     call sps_check(sps_step(handle, 1_c_int32_t, real(dt_sph, c_double), t, it), 'sps_step')
-    if (sps_glue_sync_every_step) call sps_glue_sync()
+    if (sps_glue_sync_every_step) then
+       call sps_glue_sync()
+    else if (sps_glue_sync_at_output) then
+       if (t_plot_reset >= time_plot .or. t_print_reset >= time_print) call sps_glue_sync()
+    end if
   end subroutine sps_time_integration
 
   subroutine sps_glue_sync

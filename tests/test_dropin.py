@@ -25,19 +25,47 @@ BIN = os.path.join(ROOT, "oracle", "_ref", "code")
 FIELDS = ("x", "vel", "stress", "ivar1", "f_drucker", "ddev_strn", "grad_u", "displ", "f_bound", "art_visc")
 
 
-def run_dropin(binary, deck_name, steps, tmp_path):
+def run_dropin(binary, deck_name, steps, tmp_path, plot_step=None):
     exe = os.path.join(BIN, binary)
     if not os.path.exists(exe):
         pytest.skip(f"{exe} not built (needs the reference sources at build time)")
     work = tmp_path / deck_name
     shutil.copytree(deck_dir(deck_name), work)
+    env = dict(os.environ)
+    if plot_step is not None:
+        # the state stays on the device; the glue downloads it on the steps
+        # after which the driver's output timers fire
+        lines = (work / "input.txt").read_text().splitlines()
+        k = next(i for i, ln in enumerate(lines) if ln.lower().startswith("print, save, plot"))
+        lines[k + 1] = f"1000000,1000000,{plot_step}"
+        (work / "input.txt").write_text("\n".join(lines) + "\n")
+        env["SPS_SYNC_AT_OUTPUT"] = "1"
     out = work / "out"
     out.mkdir()
-    print(f"[dropin] {binary} deck={deck_name} steps={steps}", flush=True)
+    print(f"[dropin] {binary} deck={deck_name} steps={steps} plot_step={plot_step}", flush=True)
     cmd = f"ulimit -s unlimited && '{exe}' out {max(steps)} " + " ".join(map(str, steps))
     subprocess.run(["bash", "-c", cmd], cwd=work, check=True, timeout=240,
-                   stdout=subprocess.DEVNULL)
-    return {st: read_state(str(out / f"state_{st:06d}.bin")) for st in steps}
+                   stdout=subprocess.DEVNULL, env=env)
+    states = {st: read_state(str(out / f"state_{st:06d}.bin")) for st in steps
+              if (out / f"state_{st:06d}.bin").exists()}
+    if plot_step is not None:
+        states["stale"] = {st: read_state(str(out / f"stale_{st:06d}.bin")) for st in steps
+                           if (out / f"stale_{st:06d}.bin").exists()}
+    return states
+
+
+def output_steps(dt, plot_step, n):
+    """Steps after which the reference driver writes with this plot period:
+    its single-precision timer gains dt each step and fires when it reaches
+    plot_step*dt rounded to single precision (code/1_SPH_2018.f90:67-90)."""
+    period = np.float32(plot_step * dt)
+    t, out = np.float32(0.0), []
+    for i in range(1, n + 1):
+        t = np.float32(np.float64(t) + dt)
+        if t >= period:
+            out.append(i)
+            t = np.float32(0.0)
+    return out
 
 
 def check(states, deck_name):
@@ -55,6 +83,39 @@ def check(states, deck_name):
                                              ("v1_vslope_sp1", [1, 10, 100])])
 def test_fortran_dropin_oracle(deck_name, steps, tmp_path):
     check(run_dropin("sps_dropin_oracle", deck_name, steps, tmp_path), deck_name)
+
+
+def check_at_output(binary, tmp_path):
+    """t1_tiny with a plot period of 2 steps (the timers fire after steps
+    2, 4, ..., 200): only the listed steps that are output steps are dumped,
+    and their states, downloaded by the glue on those steps alone, equal the
+    reference's."""
+    steps = [1, 2, 3, 10, 50, 200]
+    dt = 5e-5
+    expect = [s for s in steps if s in set(output_steps(dt, 2, 200))]
+    assert expect == [2, 10, 50, 200]
+    assert output_steps(dt, 5, 20) == [6, 12, 18]          # single-precision drift of the timer
+    states = run_dropin(binary, "t1_tiny", steps, tmp_path, plot_step=2)
+    stale = states.pop("stale")
+    assert sorted(states) == expect
+    check(states, "t1_tiny")
+    # between output steps the host's copy is the last download's: the state
+    # stayed on the device (step 3 still shows step 2; step 1 the initial state)
+    g = golden("t1_tiny")
+    assert sorted(stale) == [1, 3]
+    for k in ("x", "vel", "stress"):
+        assert bits_equal(g[f"s2_{k}"], stale[3][k]), k
+    assert bits_equal(g["s0_x"], stale[1]["x"])
+    assert not bits_equal(g["s3_x"], stale[3]["x"])
+
+
+def test_fortran_dropin_oracle_sync_at_output(tmp_path):
+    check_at_output("sps_dropin_oracle", tmp_path)
+
+
+@pytest.mark.gpu
+def test_fortran_dropin_hip_sync_at_output(tmp_path):
+    check_at_output("sps_dropin_hip", tmp_path)
 
 
 @pytest.mark.gpu
