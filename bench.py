@@ -31,7 +31,13 @@ mixed-precision form with list reuse; its speed, its error after
 check are reported.  The batched form runs --instances slope instances drawn
 from a seed (heights 5-20 m, angles 30-90 deg, dx 0.05 m, each with its own
 material) as one particle set, with the configuration the autotuner picks,
-beside the same instances run one after the other.
+beside the same instances run one after the other.  The combined form runs
+that ensemble mixed, batched, with list reuse and the learned stress update
+(trained on the plain method's tuples of the ensemble, with their
+materials), against the plain FP64 form one instance at a time and as one
+particle set; its stress error is measured after 500 and --ens-acc-steps
+steps, beside the mixed form without the network and the FP64 form's own
+spread under a 1e-7 perturbation.
 
 The CPU baseline is the oracle (a C restatement of the reference, 1 thread)
 on a bounded sample of the same workload.
@@ -202,7 +208,7 @@ def train_surrogate(args, sps, sur, log, parts, deck):
     run (and an augmentation of them), with its training record."""
     import ctypes
     import numpy as np
-    sn, st, sr = log
+    sn, st, sr, _ = log
     p = sps.make_params(deck, parts)
     tanfi, coh = p.props[12], p.props[13]
     ok = sur.correctable(st, tanfi, coh)
@@ -293,6 +299,130 @@ def run_batched(args, sps, base_deck):
             "autotuned": best, "autotune_table": table, "autotune_s": round(tune_s, 2)}
 
 
+def run_combined(args, sps, base_deck, tuned):
+    """The combined form on the slope ensemble: mixed precision, the
+    instances batched into one particle set, list reuse and graph replay as
+    autotuned, and the learned stress update trained on the plain method's
+    tuples of the same ensemble (each with its point's material) -- against
+    the plain FP64 form on the same instances, one at a time (as one would
+    run the reference) and as one particle set.  Error of the combined run
+    against the FP64 run after args.ens_acc_steps steps from rest."""
+    import importlib
+    import torch
+    ens = importlib.import_module("stress-particle-sph_amd.ensemble")
+    sur = importlib.import_module("stress-particle-sph_amd.surrogate")
+    acc = importlib.import_module("stress-particle-sph_amd.accuracy")
+    dx, sync = 0.05, torch.cuda.synchronize
+    inst = ens.draw_instances(args.instances, 2024)
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    # plain FP64, one instance at a time
+    seq = 0.0
+    for k in range(len(inst)):
+        one = ens.open_ensemble(sps, base_deck, [inst[k]], dx=dx, mode=sps.MODE_ORDERED)
+        one.step(1)
+        sync()
+        t1 = time.perf_counter()
+        one.step(3)
+        sync()
+        seq += (time.perf_counter() - t1) / 3
+        one.close()
+    log(f"[bench] combined: FP64 one instance at a time {1e3 * seq:.3f} ms per step of all")
+    # plain FP64 on the ensemble as one particle set: timing, plastic tuples, reference state
+    nacc, ntime = args.ens_acc_steps, 10
+    ref = ens.open_ensemble(sps, base_deck, inst, dx=dx, mode=sps.MODE_ORDERED)
+    x0 = ref.get_state(["x"])["x"]
+    sur.log_plastic(ref, args.log_cap)
+    ref.step(5)
+    sync()
+    t1 = time.perf_counter()
+    ref.step(ntime)
+    sync()
+    fp64_batched = (time.perf_counter() - t1) / ntime
+    early = min(500, nacc)
+    ref.step(early - 5 - ntime)
+    S_ref_early = ref.get_state()
+    ref.step(nacc - early)
+    S_ref = ref.get_state()
+    sn, st, sr, mat = sur.plastic_log(ref, args.log_cap)
+    ref.close()
+    log(f"[bench] combined: FP64 batched {1e3 * fp64_batched:.3f} ms/step, {len(sn)} plastic tuples logged")
+    out = {"instances": len(inst), "seed": 2024, "steps": nacc,
+           "fp64_one_at_a_time_ms_per_step": round(1e3 * seq, 4),
+           "fp64_batched_ms_per_step": round(1e3 * fp64_batched, 4), "plastic_tuples_logged": int(len(sn))}
+    tf, coh = mat[:, 0], mat[:, 1]
+    ok = sur.correctable(st, tf, coh)
+    if ok.sum() < 256:
+        out["note"] = "too few plastic states in the logged steps to train on"
+        return out
+    sn, st, sr, tf, coh = sn[ok], st[ok], sr[ok], tf[ok], coh[ok]
+    lib = __import__("ctypes").CDLL(sps.PRODUCT_LIB)
+    S, Sc = float(np.abs(st).max()), max(float(coh.max()), 1e3)
+    rng = np.random.default_rng(0)
+    a = sur.augment(lib, sn, st, tf, coh, rng, copies=4, scale_max=10.0, material=((0.0, 20e3), (15.0, 40.0)))
+    x = np.vstack([sur.features(sn, st, coh, tf, S, Sc), sur.features(a[0], a[1], a[3], a[4], S, Sc)])
+    t = np.vstack([sur.targets(st, sr, tf, coh), sur.targets(a[1], a[2], a[4], a[3])])
+    net = sur.Network(lib, seed=1, batch_rows=8192)
+    t1 = time.perf_counter()
+    losses = net.train(x, t, epochs=max(args.epochs // 2, 1), batch=4096, lr=2e-3)
+    train_s = time.perf_counter() - t1
+    log(f"[bench] combined: surrogate trained on {len(x)} rows in {train_s:.1f} s")
+    def ens_run(mode, skin=0.0, graph=False, net_=None, perturb=0.0):
+        """A run of the ensemble from rest over nacc steps, timed over a window."""
+        sim = ens.open_ensemble(sps, base_deck, inst, dx=dx, mode=mode, skin=skin)
+        if perturb:
+            st0 = sim.get_state()
+            n = sim.n
+            st0["x"][:n] = st0["x"][:n] * (1 + perturb * np.random.default_rng(2024).standard_normal((n, 2)))
+            sim.set_state({k: st0[k] for k in ("x", "vel", "stress", "mass", "rho", "hsml", "itype",
+                                               "if_out_domain", "internal_vars", "f_drucker", "x00", "displ",
+                                               "wall_position", "horizontal_or_not")})
+            sim.set_materials(sim.ensemble[0], sim.ensemble[1])
+        sim.set_graph(graph)
+        if net_ is not None:
+            sur.attach(sim, net_, S, Sc, args.surrogate_tol)
+        sim.step(5)
+        sync()
+        t1 = time.perf_counter()
+        sim.step(ntime)
+        sync()
+        ms = (time.perf_counter() - t1) / ntime
+        sim.step(early - 5 - ntime)
+        state_early = sim.get_state()
+        sim.step(nacc - early)
+        state = sim.get_state()
+        stats = sur.stats(sim) if net_ is not None else None
+        sim.close()
+        return (state_early, state), ms, stats
+
+    skin = tuned["skin_dx"] * dx
+    S_comb, comb_ms, (ev, kept) = ens_run(sps.MODE_FAST, skin, tuned["graph"], net)
+    e, e_early = acc.compare_states(S_comb[1], S_ref, x0), acc.compare_states(S_comb[0], S_ref_early, x0)
+    net.close()
+    S_mix, _, _ = ens_run(sps.MODE_FAST, skin, tuned["graph"])
+    e_mix, e_mix_early = acc.compare_states(S_mix[1], S_ref, x0), acc.compare_states(S_mix[0], S_ref_early, x0)
+    S_pert, _, _ = ens_run(sps.MODE_ORDERED, perturb=1e-7)
+    e_pert, e_pert_early = (acc.compare_states(S_pert[1], S_ref, x0),
+                            acc.compare_states(S_pert[0], S_ref_early, x0))
+    rnd = lambda d: {k: float(f"{v:.4g}") for k, v in d.items()}
+    out.update({"ms_per_step": round(1e3 * comb_ms, 4), "steps_per_s": round(1.0 / comb_ms, 3),
+                "speedup_vs_fp64_one_at_a_time": round(seq / comb_ms, 3),
+                "speedup_vs_fp64_batched": round(fp64_batched / comb_ms, 3),
+                "autotuned": tuned, "training_rows": int(len(x)), "train_s": round(train_s, 2),
+                "loss_first_last": [float(f"{losses[0]:.4g}"), float(f"{losses[-1]:.4g}")],
+                "plastic_states_evaluated": ev, "kept": kept, "accuracy_vs_fp64": rnd(e),
+                "mixed_batched_no_surrogate_vs_fp64": rnd(e_mix),
+                "fp64_perturbed_1e-7_vs_fp64": rnd(e_pert),
+                f"at_step_{early}": {"accuracy_vs_fp64": rnd(e_early),
+                                     "mixed_batched_no_surrogate_vs_fp64": rnd(e_mix_early),
+                                     "fp64_perturbed_1e-7_vs_fp64": rnd(e_pert_early)}})
+    log(f"[bench] combined: {1e3 * comb_ms:.3f} ms/step, {out['speedup_vs_fp64_one_at_a_time']}x FP64 one at a "
+        f"time, {out['speedup_vs_fp64_batched']}x FP64 batched; stress error {e['stress_rel_max']:.3g} max, "
+        f"{e['stress_rel_rms']:.3g} rms after {nacc} steps (mixed without the network "
+        f"{e_mix['stress_rel_max']:.3g} max; FP64 perturbed 1e-7 {e_pert['stress_rel_max']:.3g} max); after "
+        f"{early} steps {e_early['stress_rel_max']:.3g} max (FP64 perturbed {e_pert_early['stress_rel_max']:.3g})")
+    return out
+
+
 def run_cpu(args, sps, deck, oracle_path):
     sim = sps.Simulation.from_deck(deck, lib_path=oracle_path)
     pairs = 0
@@ -320,6 +450,8 @@ def main():
     ap.add_argument("--epochs", type=int, default=200, help="surrogate training epochs")
     ap.add_argument("--surrogate-tol", type=float, default=1e-4, help="yield-function residual kept")
     ap.add_argument("--instances", type=int, default=64, help="slope instances of the batched form (0: skip)")
+    ap.add_argument("--ens-acc-steps", type=int, default=2000,
+                    help="steps of the combined form's accuracy run on the ensemble (0: skip the combined form)")
     ap.add_argument("--acc-steps", type=int, default=2000,
                     help="steps of the mixed-vs-double accuracy runs (0: skip them)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -402,6 +534,8 @@ def main():
             forms["mixed_skin_surrogate"] = sinfo
     if args.instances > 0:
         forms["batched"] = run_batched(args, sps, deck)
+        if args.ens_acc_steps > 0:
+            forms["combined"] = run_combined(args, sps, deck, forms["batched"]["autotuned"])
     cpu = None
     if not args.no_cpu:
         oracle = os.path.join(ROOT, "oracle", "libsps_oracle.so")

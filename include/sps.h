@@ -299,8 +299,10 @@ int sps_get_search_count(sps_handle *h, int64_t *n);
  *   elsewhere the exact return map's result stays.  sps_get_surrogate_stats:
  *   states evaluated and kept.
  * sps_set_plastic_log / sps_get_plastic_log: the plain method's tuples
- *   (sigma_n, sigma_trial, R(sigma_trial)), 12 doubles each, of every plastic
- *   trial state of the final stress updates, up to cap (training data). */
+ *   (sigma_n, sigma_trial, R(sigma_trial), tan(phi), c), SPS_PLOG_ROW doubles
+ *   each, of every plastic trial state of the final stress updates, up to cap
+ *   (training data; the material is the point's own with sps_set_materials). */
+#define SPS_PLOG_ROW 14
 typedef struct sps_mlp sps_mlp;
 int64_t sps_mlp_param_count(void);
 int sps_mlp_create(int32_t batch_rows, uint64_t seed, sps_mlp **out);

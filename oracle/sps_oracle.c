@@ -980,10 +980,12 @@ static void rk4(struct sps_handle *h, double dt_sph, double time_sph)
             memcpy(ret, tr, sizeof ret);
             adapt_one(ret, mat_tanfi(h, i), mat_coh(h, i));   /* a copy: adapt_stress2 below does it */
             if (memcmp(tr, ret, sizeof ret) != 0 && h->log_n < h->log_cap) {
-                double *o = h->plog + 12 * h->log_n++;
+                double *o = h->plog + SPS_PLOG_ROW * h->log_n++;
                 memcpy(o, h->stress0 + 4 * i, 4 * sizeof(double));
                 memcpy(o + 4, tr, 4 * sizeof(double));
                 memcpy(o + 8, ret, sizeof ret);
+                o[12] = mat_tanfi(h, i);
+                o[13] = mat_coh(h, i);
             }
         }
     }
@@ -1426,7 +1428,7 @@ int sps_set_plastic_log(sps_handle *h, int64_t cap)
 {
     if (!h || cap < 0 || cap > ((int64_t)1 << 30)) return fail(SPS_EINVAL, "bad argument");
     free(h->plog);
-    h->plog = cap ? (double *)malloc((size_t)cap * 12 * sizeof(double)) : NULL;
+    h->plog = cap ? (double *)malloc((size_t)cap * SPS_PLOG_ROW * sizeof(double)) : NULL;
     if (cap && !h->plog) return fail(SPS_ENOMEM, "log allocation");
     h->log_cap = cap;
     h->log_n = 0;
@@ -1437,7 +1439,7 @@ int sps_get_plastic_log(sps_handle *h, int64_t cap, double *out, int64_t *n)
 {
     if (!h || !n || cap < 0 || (cap && !out)) return fail(SPS_EINVAL, "bad argument");
     int64_t k = h->log_n < cap ? h->log_n : cap;
-    if (k) memcpy(out, h->plog, (size_t)k * 12 * sizeof(double));
+    if (k) memcpy(out, h->plog, (size_t)k * SPS_PLOG_ROW * sizeof(double));
     *n = k;
     return 0;
 }

@@ -469,10 +469,11 @@ __device__ __forceinline__ void surrogate_record(const SurrogateDev &G, int i, c
     if (G.log_cap > 0) {
         const unsigned k = atomicAdd(G.log_count, 1u);
         if ((int)k < G.log_cap) {
-            double *o = G.log + 12ll * k;
+            double *o = G.log + (long long)SPS_PLOG_ROW * k;
             o[0] = s_n.x; o[1] = s_n.y; o[2] = s_n.z; o[3] = s_n.w;
             o[4] = s_tr.x; o[5] = s_tr.y; o[6] = s_tr.z; o[7] = s_tr.w;
             o[8] = s_ret.x; o[9] = s_ret.y; o[10] = s_ret.z; o[11] = s_ret.w;
+            o[12] = tanfi; o[13] = coh;
         }
     }
     if (G.cap <= 0) return;

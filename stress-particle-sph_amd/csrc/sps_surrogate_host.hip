@@ -403,7 +403,7 @@ extern "C" int sps_get_surrogate_stats(sps_handle *h, int64_t *evaluated, int64_
 extern "C" int sps_set_plastic_log(sps_handle *h, int64_t cap) {
     if (!h || cap < 0 || cap > (1ll << 30)) return fail(SPS_EINVAL, "bad argument");
     HIP_TRY(hipSetDevice(h->device));
-    int rc = dalloc(h->surr_count, 2) | dalloc(h->plog, (size_t)(cap ? cap : 1) * 12);
+    int rc = dalloc(h->surr_count, 2) | dalloc(h->plog, (size_t)(cap ? cap : 1) * SPS_PLOG_ROW);
     if (rc) return rc;
     h->log_cap = (int)cap;
     HIP_TRY(hipMemsetAsync(h->surr_count.p + 1, 0, sizeof(unsigned), h->stream));
@@ -419,7 +419,7 @@ extern "C" int sps_get_plastic_log(sps_handle *h, int64_t cap, double *out, int6
     HIP_TRY(hipMemcpyAsync(&c, h->surr_count.p + 1, sizeof c, hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     const int64_t k = std::min<int64_t>(std::min<int64_t>((int64_t)c, h->log_cap), cap);
-    if (k > 0) HIP_TRY(hipMemcpy(out, h->plog.p, (size_t)k * 12 * sizeof(double), hipMemcpyDeviceToHost));
+    if (k > 0) HIP_TRY(hipMemcpy(out, h->plog.p, (size_t)k * SPS_PLOG_ROW * sizeof(double), hipMemcpyDeviceToHost));
     *n = k;
     return 0;
 }

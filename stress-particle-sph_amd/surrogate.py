@@ -192,14 +192,18 @@ def log_plastic(sim, cap):
     sim._check(lib.sps_set_plastic_log(sim.handle, int(cap)))
 
 
+PLOG_ROW = 14
+
+
 def plastic_log(sim, cap):
-    """The logged tuples: sigma_n, sigma_trial, R(sigma_trial), each n x 4."""
+    """The logged tuples: sigma_n, sigma_trial, R(sigma_trial) (each n x 4)
+    and the material (tan phi, c) of each (n x 2)."""
     lib = _declare(sim.lib)
-    out = np.zeros((cap, 12), dtype=np.float64)
+    out = np.zeros((cap, PLOG_ROW), dtype=np.float64)
     n = ctypes.c_int64(0)
     sim._check(lib.sps_get_plastic_log(sim.handle, int(cap), _ptr(out), ctypes.byref(n)))
     out = out[:n.value]
-    return out[:, 0:4], out[:, 4:8], out[:, 8:12]
+    return out[:, 0:4], out[:, 4:8], out[:, 8:12], out[:, 12:14]
 
 
 def augment(lib, s_n, s_tr, tanfi, coh, rng, copies=4, scale_max=100.0, material=None):

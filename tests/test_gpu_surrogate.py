@@ -132,7 +132,7 @@ def trained_network(sps, sur, deck, steps, epochs=40):
     plain = sps.Simulation.from_deck(deck)
     sur.log_plastic(plain, 1 << 21)
     plain.step(steps)
-    sn, st, sr = sur.plastic_log(plain, 1 << 21)
+    sn, st, sr, _ = sur.plastic_log(plain, 1 << 21)
     p = plain.params
     tanfi, coh = p.props[12], p.props[13]
     S = float(np.abs(st).max())
