@@ -62,6 +62,11 @@ PLAN = {
     # the reference's biaxial strain-localisation test: von Mises Perzyna with
     # softening, prescribed velocities with the position update, 2100 steps
     "l1_localisation": ([1, 2, 10, 100, 1000, 2100], [1, 2, 10, 100], False),
+    # the outside approach with one and three points per node (fixed offsets)
+    "b7_bui_out1": ([1, 2, 10, 100, 500], [1, 2, 100], False, "bui"),
+    "b8_bui_out3": ([1, 2, 10, 100, 500], [1, 2, 100], False, "bui"),
+    # the inside approach with four points per cell
+    "t5_sp4": ([1, 2, 3, 10, 50, 200], "all", True),
 }
 
 STATE_FIELDS = ("x", "vel", "stress", "ivar1", "f_drucker", "ddev_strn", "grad_u", "displ",

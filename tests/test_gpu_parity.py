@@ -62,6 +62,7 @@ def compare_pairs(gpu, ora, tag):
     ("t3_elastic", [1, 50, 200]),
     ("t4_xsph", [1, 2, 3, 10, 50, 200]),
     ("p1_vm", [1, 2, 10, 50, 100, 200]),
+    ("t5_sp4", [1, 2, 3, 10, 50, 200]),
 ])
 def test_small_decks_bitwise(deck_name, checkpoints):
     say(f"[gpu parity] deck={deck_name} checkpoints={checkpoints}")
@@ -131,7 +132,7 @@ def test_perzyna_decks_near_oracle(deck_name):
     ora.close()
 
 
-@pytest.mark.parametrize("deck_name", ["t1_tiny", "t2_cspm", "t3_elastic", "t4_xsph", "p1_vm"])
+@pytest.mark.parametrize("deck_name", ["t1_tiny", "t2_cspm", "t3_elastic", "t4_xsph", "p1_vm", "t5_sp4"])
 def test_small_decks_vs_reference(deck_name):
     """The GPU state against the reference Fortran's own dumps."""
     say(f"[gpu vs reference fixture] deck={deck_name}")
@@ -157,7 +158,7 @@ def test_small_decks_vs_reference(deck_name):
 
 @pytest.mark.parametrize("deck_name", ["c1_collapse", "b1_bui_sp1", "b2_bui_sp2", "b3_bui_sp3", "b4_bui_outside",
                                        "b5_bui_velvec", "b6_bui_velvec_shift", "v1_vslope_sp1", "v3_vslope_sp3",
-                                       "l1_localisation"])
+                                       "l1_localisation", "b7_bui_out1", "b8_bui_out3"])
 def test_digest_decks_vs_reference(deck_name):
     """GPU state digests against the reference Fortran's dumps: the granular
     column collapse (1000 steps of dt = 1e-5) and the Bui

@@ -14,7 +14,8 @@ from conftest import bits_equal, deck_dir, describe_diff, golden, load_pkg
 
 SMALL = {"t1_tiny": [1, 2, 3, 10, 50, 200], "t2_cspm": [1, 2, 50, 200], "t3_elastic": [1, 50, 200],
          "t4_xsph": [1, 2, 3, 10, 50, 200], "p1_vm": [1, 2, 10, 50, 100, 200], "p2_dp": [1, 2, 10, 50, 100, 200],
-         "p3_tresca": [1, 2, 10, 50, 100, 200], "p4_mc": [1, 2, 10, 50, 100, 200]}
+         "p3_tresca": [1, 2, 10, 50, 100, 200], "p4_mc": [1, 2, 10, 50, 100, 200],
+         "t5_sp4": [1, 2, 3, 10, 50, 200]}
 # (deck, step) -> relative tolerance where the oracle is not bitwise: the
 # Mohr-Coulomb Perzyna deck's one internal_vars value differs from the
 # reference's by one unit in the last place at step 200 (every field is
@@ -78,7 +79,9 @@ DIGEST = {"c1_collapse": "BASELINE config 1 (5,151 + 5,000 particles, 1000 steps
           "b6_bui_velvec_shift": "velocity-vector shifts on every step (disp_tol 1e-5), 500 steps",
           "v1_vslope_sp1": "elastic vertical cut with prescribed velocities and stresses, 2000 steps",
           "v3_vslope_sp3": "the same cut with three stress points per cell, 1000 steps",
-          "l1_localisation": "biaxial strain localisation: Perzyna von Mises, BCs with the position update"}
+          "l1_localisation": "biaxial strain localisation: Perzyna von Mises, BCs with the position update",
+          "b7_bui_out1": "outside approach, one point per node, 500 steps",
+          "b8_bui_out3": "outside approach, three points per node, 500 steps"}
 # (deck, step) where the reference reads an uninitialised local: its BCs
 # take t = time_sph + ic_time*dt_sph with ic_time never set (3_...:1622-1627;
 # the compiled reference finds about 6.95e-310 there, a stack address), so
