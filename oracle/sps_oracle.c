@@ -1228,16 +1228,15 @@ static int check_params(const sps_params *p)
     if (p->cont_density || p->art_stress)
         return fail(SPS_EINVAL, "continuity density and artificial stress are not supported");
     if (p->no_bcs < 0 || p->no_bcs > SPS_MAX_BCS) return fail(SPS_EINVAL, "no_bcs must be 0..%d", SPS_MAX_BCS);
-    if (p->no_bcs > 0 && ((p->update_x && p->xsph) || p->mode != SPS_MODE_ORDERED))
-        return fail(SPS_EINVAL, "prescribed BCs run in SPS_MODE_ORDERED, and not with XSPH and the position update "
-                                "together (XSPH_update then re-marks BC nodes next to the free surface)");
+    if (p->no_bcs > 0 && p->update_x && p->xsph)
+        return fail(SPS_EINVAL, "prescribed BCs do not run with XSPH and the position update together "
+                                "(XSPH_update then re-marks BC nodes next to the free surface)");
     if (p->xsph && p->mode != SPS_MODE_ORDERED) return fail(SPS_EINVAL, "XSPH runs in SPS_MODE_ORDERED");
     int ntype_eco = (int)p->props[0], ncrit = (int)p->props[1];
     if (ntype_eco > 1 && ncrit == 5) return fail(SPS_EINVAL, "the Modified Cam Clay model (ncrit 5) is not supported");
     if (ntype_eco > 1 && ncrit <= 4 && (ncrit < 1 || !(p->props[6] > (double)0.001f)))
         return fail(SPS_EINVAL, "Perzyna models need ncrit 1..4 and an initial yield size props(7) > 0.001");
-    if (ntype_eco > 1 && ncrit <= 4 && p->mode != SPS_MODE_ORDERED)
-        return fail(SPS_EINVAL, "Perzyna models run in SPS_MODE_ORDERED");
+
     if (p->ic_grav && (p->tcurve_npts < 2 || p->tcurve_npts > SPS_MAX_TCURVE))
         return fail(SPS_EINVAL, "gravity time curve needs 2..%d points", SPS_MAX_TCURVE);
     if (p->mode != SPS_MODE_ORDERED && p->mode != SPS_MODE_FAST) return fail(SPS_EINVAL, "unknown mode");

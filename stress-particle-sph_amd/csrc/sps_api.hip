@@ -158,6 +158,8 @@ struct sps_handle {
     double curve_t[SPS_MAX_TCURVE * SPS_MAX_CURVES];
     float curve_f[SPS_MAX_TCURVE * SPS_MAX_CURVES];
     DBuf<int> bc_info, bc_or_not;
+    DBuf<double> bc_vals;         // the step's values (k_bc_values)
+    BcValues bc_vars{};           // the BC rows' variables (values filled per step)
     DBuf<unsigned short> deg_a, deg_b;
     DBuf<unsigned char> tile_ok;
     DBuf<Tile> tiles;
@@ -323,16 +325,15 @@ static int check_params(const sps_params *p) {
     if (p->cont_density || p->art_stress)
         return fail(SPS_EINVAL, "continuity density and artificial stress are not supported");
     if (p->no_bcs < 0 || p->no_bcs > SPS_MAX_BCS) return fail(SPS_EINVAL, "no_bcs must be 0..%d", SPS_MAX_BCS);
-    if (p->no_bcs > 0 && ((p->update_x && p->xsph) || p->mode != SPS_MODE_ORDERED))
-        return fail(SPS_EINVAL, "prescribed BCs run in SPS_MODE_ORDERED, and not with XSPH and the position update "
-                                "together (XSPH_update then re-marks BC nodes next to the free surface)");
+    if (p->no_bcs > 0 && p->update_x && p->xsph)
+        return fail(SPS_EINVAL, "prescribed BCs do not run with XSPH and the position update together "
+                                "(XSPH_update then re-marks BC nodes next to the free surface)");
     if (p->xsph && p->mode != SPS_MODE_ORDERED) return fail(SPS_EINVAL, "XSPH runs in SPS_MODE_ORDERED");
     int ntype_eco = (int)p->props[0], ncrit = (int)p->props[1];
     if (ntype_eco > 1 && ncrit == 5) return fail(SPS_EINVAL, "the Modified Cam Clay model (ncrit 5) is not supported");
     if (ntype_eco > 1 && ncrit <= 4 && (ncrit < 1 || !(p->props[6] > (double)0.001f)))
         return fail(SPS_EINVAL, "Perzyna models need ncrit 1..4 and an initial yield size props(7) > 0.001");
-    if (ntype_eco > 1 && ncrit <= 4 && p->mode != SPS_MODE_ORDERED)
-        return fail(SPS_EINVAL, "Perzyna models run in SPS_MODE_ORDERED");
+
     if (p->ic_grav && (p->tcurve_npts < 2 || p->tcurve_npts > SPS_MAX_TCURVE))
         return fail(SPS_EINVAL, "gravity time curve needs 2..%d points", SPS_MAX_TCURVE);
     if (p->mode != SPS_MODE_ORDERED && p->mode != SPS_MODE_FAST) return fail(SPS_EINVAL, "unknown mode %d", p->mode);
@@ -482,7 +483,8 @@ extern "C" int sps_create(const sps_params *params, sps_handle **out) {
     rc |= dalloc(h->tiles, h->ntile_n + h->ntile_s + 1);
     rc |= dalloc(h->plans, 3 * (size_t)(h->ntile_n + h->ntile_s + 1));
     rc |= dalloc(h->x_build, nt); rc |= dalloc(h->dmax, 1); rc |= dalloc(h->stepv, 4);
-    if (params->no_bcs > 0) rc |= dalloc(h->bc_info, 8 * (size_t)nn) | dalloc(h->bc_or_not, nn);
+    if (params->no_bcs > 0)
+        rc |= dalloc(h->bc_info, 8 * (size_t)nn) | dalloc(h->bc_or_not, nn) | dalloc(h->bc_vals, SPS_MAX_BCS);
     if (h->outside) {
         rc |= dalloc(h->inv, nt2); rc |= dalloc(h->nodeinfo, nn);
         rc |= dalloc(h->x10, nn); rc |= dalloc(h->disp10, nn);
@@ -714,13 +716,19 @@ static int mixed_step_kernels(sps_handle *h, double dt, double time_sph, double 
     const double tanfi = h->dp.tanfi, coh = h->dp.coh;
     const double f1rk[4] = {0., 0.5, 0.5, 1.0}, f2rk[4] = {1., 2., 2., 1.0};
     const float hu = (float)h->h_uniform;
+    // prescribed BCs where the reference applies them (as in the FP64 step);
+    // a captured step reads the values k_bc_values leaves before each replay
+    const bool bcs = h->p.no_bcs > 0 && nn > 0;
+    if (bcs && !stepv) LAUNCH(K_BCS, 1, 64, k_bc_values, bc_values(h, time_sph), h->bc_vals.p);
     LAUNCH(K_RK_INIT, nblk(nt2), BLOCK, k_rk_init, SB, W, nn, nt, nt2);
     for (int st = 0; st < 4; ++st) {
         const double c1 = f1rk[st] * (dt);
         LAUNCH(K_STAGE_BEGIN, nblk(nt), BLOCK, k_stage_begin, SB, W, c1, adapt, tanfi, coh, nn, nt);
+        if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, h->bc_vars, h->bc_vals.p, nn);
         if (nn > 0)
             LAUNCH_LDS(K_SPU_NODES, nblk8(nn), TILE, LAY_SPU_NODES_MIX.bytes, k_spu_nodes_mix, SB, V, PL, adapt,
                        tanfi, coh, hu, nn);
+        if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, h->bc_vars, h->bc_vals.p, nn);
         if (nt > nn)
             LAUNCH_LDS(K_SP_UPDATE, nblk8(nt - nn), TILE, LAY_SP_UPDATE_MIX.bytes, k_sp_update_mix, SB, W, V, PL,
                        h->dp, time_sph, f2rk[st], 1, adapt, hu, nn, nt, stepv);
@@ -738,12 +746,14 @@ static int mixed_step_kernels(sps_handle *h, double dt, double time_sph, double 
         int rc = surrogate_correct(h, SB);
         if (rc) return rc;
     }
+    if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, h->bc_vars, h->bc_vals.p, nn);
     if (nn > 0)
         LAUNCH_LDS(K_SPU_NODES, nblk8(nn), TILE, LAY_SPU_NODES_MIX.bytes, k_spu_nodes_mix, SB, V, PL, adapt, tanfi,
                    coh, hu, nn);
     if (nt > nn)
         LAUNCH_LDS(K_SP_UPDATE, nblk8(nt - nn), TILE, LAY_SP_UPDATE_MIX.bytes, k_sp_update_mix, SB, W, V, PL, h->dp,
                    time_sph, 0.0, 0, adapt, hu, nn, nt, stepv);
+    if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, h->bc_vars, h->bc_vals.p, nn);
     LAUNCH(K_POSITION, nblk(nt), BLOCK, k_position, SB, W, dt, h->p.update_x, nn, nt, (const double2 *)nullptr);
     if (h->skin > 0) {
         LAUNCH(K_POSITION, 1, 64, k_zero_u64, h->dmax.p, 1);
@@ -789,6 +799,7 @@ static int mixed_step_graph(sps_handle *h, double dt, double time_sph, double gx
         h->graph_dt[c] = dt;
     }
     k_set_step<<<1, 64, 0, s>>>(h->stepv.p, time_sph, gxg, gyg);
+    if (h->p.no_bcs > 0 && h->nn > 0) k_bc_values<<<1, 64, 0, s>>>(bc_values(h, time_sph), h->bc_vals.p);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipGraphLaunch(h->gexec[c], s));
     ++h->graph_steps;
@@ -834,7 +845,8 @@ static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
     const double tanfi = h->dp.tanfi, coh = h->dp.coh;
     const double f1rk[4] = {0., 0.5, 0.5, 1.0}, f2rk[4] = {1., 2., 2., 1.0};
     const bool bcs = h->p.no_bcs > 0 && nn > 0;
-    const BcValues BV = bcs ? bc_values(h, time_sph) : BcValues{};
+    const BcValues &BV = h->bc_vars;
+    if (bcs) LAUNCH(K_BCS, 1, 64, k_bc_values, bc_values(h, time_sph), h->bc_vals.p);
     if (h->outside && h->p.sph_shift && itimestep > 1 && (itimestep - 1) % h->p.shift_update == 0) {
         // stress_point_update + adapt_stress2 on the new lists (2_...:99-109)
         if (nn > 0)
@@ -847,10 +859,10 @@ static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
     for (int st = 0; st < 4; ++st) {
         const double c1 = f1rk[st] * (dt);
         LAUNCH(K_STAGE_BEGIN, nblk(nt), BLOCK, k_stage_begin, SB, W, c1, adapt, tanfi, coh, nn, nt);
-        if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, nn);
+        if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, nn);
         if (nn > 0)
             LAUNCH_LDS(K_SPU_NODES, nblk8(nn), TILE, LAY_SPU_NODES.bytes, k_spu_nodes, SB, V, PL, adapt, tanfi, coh, nn);
-        if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, nn);
+        if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, nn);
         if (nt > nn)
             LAUNCH_LDS(K_SP_UPDATE, nblk8(nt - nn), TILE, LAY_SP_UPDATE.bytes, k_sp_update, SB, W, V, PL, h->dp, time_sph,
                        f2rk[st], 1, adapt, nn, nt);
@@ -863,13 +875,13 @@ static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
     const bool hook = h->log_cap > 0;
     LAUNCH(K_RK_FINAL, nblk(nt), BLOCK, k_rk_final, SB, W, dt, adapt, tanfi, coh, nn, nt, surrogate_dev(h),
            hook ? 1 : 0);
-    if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, nn);
+    if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, nn);
     if (nn > 0)
         LAUNCH_LDS(K_SPU_NODES, nblk8(nn), TILE, LAY_SPU_NODES.bytes, k_spu_nodes, SB, V, PL, adapt, tanfi, coh, nn);
     if (nt > nn)
         LAUNCH_LDS(K_SP_UPDATE, nblk8(nt - nn), TILE, LAY_SP_UPDATE.bytes, k_sp_update, SB, W, V, PL, h->dp, time_sph, 0.0,
                    0, adapt, nn, nt);
-    if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, nn);
+    if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, nn);
     const bool xsph = h->xsph && h->p.update_x;
     if (xsph && nn > 0)
         LAUNCH_LDS(K_XSPH, nblk8(nn), TILE, LAY_XSPH.bytes, (k_xsph<KIND_NODE>), SB, V, PL, h->xsph_v.p);
@@ -917,6 +929,7 @@ extern "C" int sps_set_bcs(sps_handle *h, const sps_bcs *b) {
     HIP_TRY(hipMemcpyAsync(h->bc_info.p, b->bc_info, 8 * (size_t)nn * sizeof(int), hipMemcpyHostToDevice, h->stream));
     HIP_TRY(hipMemcpyAsync(h->bc_or_not.p, b->bc_or_not, (size_t)nn * sizeof(int), hipMemcpyHostToDevice, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
+    h->bc_vars = bc_values(h, 0.0);   // the variables (the values are the step's, k_bc_values)
     h->bcs_set = true;
     return 0;
 }

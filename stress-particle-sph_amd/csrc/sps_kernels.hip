@@ -2139,11 +2139,18 @@ struct BcValues {
     signed char var[SPS_MAX_BCS];
 };
 
+// The step's BC values into device memory, read by k_bcs: a replayed graph
+// holds k_bcs with its arguments fixed, the values change with time.
+__global__ void k_bc_values(BcValues V, double *__restrict__ vals) {
+    const int b = (int)threadIdx.x;
+    if (b < V.nbc) vals[b] = V.v[b];
+}
+
 // Normal_BCs (3_...:1644-1753) on the node slots: a node with BC_or_not = 1
 // takes the values of the BCs listed in bc_info(3:7) (by original index), in
-// that order.
+// that order; vals holds the step's values (k_bc_values), V the variables.
 __global__ void k_bcs(StateBufs S, const int *__restrict__ bc_info, const int *__restrict__ bc_or_not, BcValues V,
-                      int nn) {
+                      const double *__restrict__ vals, int nn) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nn) return;
     const int i = S.orig[r];
@@ -2155,7 +2162,7 @@ __global__ void k_bcs(StateBufs S, const int *__restrict__ bc_info, const int *_
     for (int k = 0; k < 5; ++k) {
         const int b = info[2 + k];
         if (b <= 0 || b > V.nbc) continue;
-        const double val = V.v[b - 1];
+        const double val = vals[b - 1];
         switch (V.var[b - 1]) {
         case 5: v.x = val; dv = true; break;
         case 6: v.y = val; dv = true; break;

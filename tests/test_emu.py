@@ -41,7 +41,8 @@ def emu_lib():
                                              ("c1_collapse", 3), ("t2_cspm:hvar", 30), ("t2_cspm:graph", 30),
                                              ("t4_xsph", 30), ("b3_bui_sp3", 15), ("b4_bui_outside", 15),
                                              ("b6_bui_velvec_shift:kick", 15), ("v3_vslope_sp3", 20),
-                                             ("p1_vm", 30), ("p4_mc", 30), ("l1_localisation", 10)])
+                                             ("p1_vm", 30), ("p4_mc", 30), ("l1_localisation", 10),
+                                             ("l1_localisation:graph", 10)])
 def test_cpu_build_of_kernels_matches_oracle(emu_lib, deck_name, steps):
     env = dict(os.environ, LD_PRELOAD=_asan_runtime(),
                ASAN_OPTIONS="detect_leaks=0:detect_stack_use_after_return=0:halt_on_error=1")
@@ -55,7 +56,8 @@ def test_cpu_build_of_kernels_matches_oracle(emu_lib, deck_name, steps):
 # precision rounding of the oracle where the problem does not amplify it (an
 # elastic deck; a yielding deck before yield starts)
 @pytest.mark.parametrize("case,steps,tol", [("t3_elastic:mixed", 30, "1e-5"), ("t2_cspm:mixed", 1, "1e-5"),
-                                            ("c1_collapse:mixed", 1, "5e-5")])
+                                            ("c1_collapse:mixed", 1, "5e-5"),
+                                            ("v3_vslope_sp3:mixed", 20, "1e-5"), ("l1_localisation:mixed", 10, "1e-5")])
 def test_cpu_build_of_mixed_kernels_near_oracle(emu_lib, case, steps, tol):
     env = dict(os.environ, LD_PRELOAD=_asan_runtime(),
                ASAN_OPTIONS="detect_leaks=0:detect_stack_use_after_return=0:halt_on_error=1")

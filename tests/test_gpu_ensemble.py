@@ -60,14 +60,17 @@ def test_ensemble_double_form_equals_oracle():
     assert sum(plastic) > 0
 
 
-@pytest.mark.parametrize("which", ["deck", "ensemble"])
+@pytest.mark.parametrize("which", ["deck", "ensemble", "bcs"])
 def test_graph_replay_equals_direct_launches(which):
     sps = load_pkg()
     ens = ensemble_mod()
     states = []
     for graph in (False, True):
-        if which == "deck":
-            sim = sps.Simulation.from_deck(sps.read_deck(deck_dir("t2_cspm")), mode=sps.MODE_FAST)
+        if which in ("deck", "bcs"):
+            # "bcs": prescribed velocities ramped in time, whose values a
+            # replayed step must take from the step, not from its capture
+            name = "t2_cspm" if which == "deck" else "l1_localisation"
+            sim = sps.Simulation.from_deck(sps.read_deck(deck_dir(name)), mode=sps.MODE_FAST)
             sim.set_list_skin(0.25 * sim.params.dx)
         else:
             base = sps.read_deck(deck_dir("c2_slope_dx01"))

@@ -54,6 +54,8 @@ def run_pair(deck_name, steps, skin=0.0):
     ("t2_cspm", 1, 1e-5),          # yielding decks, before yield starts
     ("t1_tiny", 1, 1e-5),
     ("c1_collapse", 1, 5e-5),      # stresses start at zero: first-step sums cancel
+    ("v1_vslope_sp1", 200, 1e-5),  # prescribed velocities and stresses, elastic
+    ("l1_localisation", 50, 1e-5),  # BCs with a time ramp, Perzyna von Mises, before yield
 ])
 def test_mixed_close_to_oracle(deck_name, steps, tol):
     err, _, _ = run_pair(deck_name, steps)
