@@ -85,8 +85,7 @@ typedef struct sps_params {
                                 shifted with it (2_...:156-160, 244-370) */
     int32_t cspm;            /* CSPM gradient correction (2_...:539) */
     int32_t update_x;        /* position update + Jaumann spin (2_...:140, 751) */
-    int32_t xsph;            /* XSPH position update (2_...:142-143, 189-239);
-                                SPS_MODE_ORDERED only */
+    int32_t xsph;            /* XSPH position update (2_...:142-143, 189-239) */
     int32_t cont_density;    /* must be 0 */
     int32_t art_stress;      /* must be 0 */
     int32_t dummy_nodes;     /* wall dummies present (.pts) */

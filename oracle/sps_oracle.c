@@ -1231,7 +1231,6 @@ static int check_params(const sps_params *p)
     if (p->no_bcs > 0 && p->update_x && p->xsph)
         return fail(SPS_EINVAL, "prescribed BCs do not run with XSPH and the position update together "
                                 "(XSPH_update then re-marks BC nodes next to the free surface)");
-    if (p->xsph && p->mode != SPS_MODE_ORDERED) return fail(SPS_EINVAL, "XSPH runs in SPS_MODE_ORDERED");
     int ntype_eco = (int)p->props[0], ncrit = (int)p->props[1];
     if (ntype_eco > 1 && ncrit == 5) return fail(SPS_EINVAL, "the Modified Cam Clay model (ncrit 5) is not supported");
     if (ntype_eco > 1 && ncrit <= 4 && (ncrit < 1 || !(p->props[6] > (double)0.001f)))

@@ -328,7 +328,6 @@ static int check_params(const sps_params *p) {
     if (p->no_bcs > 0 && p->update_x && p->xsph)
         return fail(SPS_EINVAL, "prescribed BCs do not run with XSPH and the position update together "
                                 "(XSPH_update then re-marks BC nodes next to the free surface)");
-    if (p->xsph && p->mode != SPS_MODE_ORDERED) return fail(SPS_EINVAL, "XSPH runs in SPS_MODE_ORDERED");
     int ntype_eco = (int)p->props[0], ncrit = (int)p->props[1];
     if (ntype_eco > 1 && ncrit == 5) return fail(SPS_EINVAL, "the Modified Cam Clay model (ncrit 5) is not supported");
     if (ntype_eco > 1 && ncrit <= 4 && (ncrit < 1 || !(p->props[6] > (double)0.001f)))
@@ -754,7 +753,13 @@ static int mixed_step_kernels(sps_handle *h, double dt, double time_sph, double 
         LAUNCH_LDS(K_SP_UPDATE, nblk8(nt - nn), TILE, LAY_SP_UPDATE_MIX.bytes, k_sp_update_mix, SB, W, V, PL, h->dp,
                    time_sph, 0.0, 0, adapt, hu, nn, nt, stepv);
     if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, h->bc_vars, h->bc_vals.p, nn);
-    LAUNCH(K_POSITION, nblk(nt), BLOCK, k_position, SB, W, dt, h->p.update_x, nn, nt, (const double2 *)nullptr);
+    const bool xsph = h->xsph && h->p.update_x;
+    if (xsph && nn > 0)
+        LAUNCH_LDS(K_XSPH, nblk8(nn), TILE, LAY_XSPH_MIX.bytes, (k_xsph_mix<KIND_NODE>), SB, V, PL, hu, h->xsph_v.p);
+    if (xsph && nt > nn)
+        LAUNCH_LDS(K_XSPH, nblk8(nt - nn), TILE, LAY_XSPH_MIX.bytes, (k_xsph_mix<KIND_SP>), SB, V, PL, hu, h->xsph_v.p);
+    LAUNCH(K_POSITION, nblk(nt), BLOCK, k_position, SB, W, dt, h->p.update_x, nn, nt,
+           xsph ? (const double2 *)h->xsph_v.p : (const double2 *)nullptr);
     if (h->skin > 0) {
         LAUNCH(K_POSITION, 1, 64, k_zero_u64, h->dmax.p, 1);
         LAUNCH(K_POSITION, std::min(1024u, nblk(nt)), BLOCK, k_max_disp, SB.x, h->x_build.p, nt, h->dmax.p);

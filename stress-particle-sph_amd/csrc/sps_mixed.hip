@@ -577,6 +577,84 @@ __global__ __launch_bounds__(TILE) void k_deriv_nodes_mix(StateBufs S, Work W, S
 // ------------------------------------------------------ captured steps
 // The step's changing scalars for a replayed graph: time_sph and the gravity
 // terms (a launch outside the graph, before each replay).
+// XSPH_update (2_...:189-239) in the mixed form: a receiver's same-kind
+// partners (list B) with kernel values computed in single precision from
+// window-relative positions, velocity differences in single precision and
+// compensated sums.
+constexpr LdsLayout layout_xsph_mix() {
+    LdsLayout L = layout_none();
+    for (int k = 0; k < 2; ++k) {   // one receiver kind per launch: both share the bytes
+        L.x[k] = 0;
+        L.v[k] = 8 * CAP;
+        L.h[k] = 16 * CAP;
+        L.mr[k] = 20 * CAP;
+    }
+    L.bytes = 24 * CAP;
+    return L;
+}
+constexpr LdsLayout LAY_XSPH_MIX = layout_xsph_mix();
+
+template <int KR>
+__global__ __launch_bounds__(TILE) void k_xsph_mix(StateBufs S, SearchView V, PairLists PL, float hu,
+                                                   double2 *__restrict__ xv) {
+    __shared__ StagePlan pl;
+    constexpr const LdsLayout &LY = LAY_XSPH_MIX;
+    const WalkCtx C = walk_ctx(V);
+    const int tile = tile_index();
+    const int ltile = KR == KIND_NODE ? tile : PL.ntile_n + tile;
+    const Tile T = PL.tiles[ltile];
+    if (T.first >= T.end) return;
+    const int r = T.first + (int)threadIdx.x;
+    const bool inside = r < T.end && V.cell[r] < C.g.ndivt;
+    ListWalk<(1 << KR), false, false> lw;
+    lw.start(PL.ib, nullptr, nullptr, nullptr, ltile, inside ? PL.deg_b[r] : 0, false);
+    const double2 o = S.x[T.first];
+    bool lds = T.fits && PL.ok[ltile];
+    if (lds) {
+        load_plan<KR>(PL, ltile, 0, pl);
+        __syncthreads();
+        lds = pl.total <= CAP;
+    }
+    if (lds) {
+        struct Item {
+            double2 x, v;
+            double h, m, rho;
+        };
+        copy_kind<KR>(T, V, C, nullptr, pl,
+                      [&](int g) { return Item{S.x[g], S.vel[g], S.hsml[g], S.mass[g], S.rho[g]}; },
+                      [&](int i, const Item &v) {
+                          lds_at<float2>(LY.x[KR])[i] = to_rel(v.x, o);
+                          lds_at<float2>(LY.v[KR])[i] = make_float2((float)v.v.x, (float)v.v.y);
+                          lds_at<float>(LY.h[KR])[i] = (float)v.h;
+                          lds_at<float>(LY.mr[KR])[i] = (float)(v.m / v.rho);
+                      });
+        __syncthreads();
+    }
+    if (r >= T.end) return;
+    const double2 vr = S.vel[r];
+    const float vx = (float)vr.x, vy = (float)vr.y;
+    KSum s0, s1;
+    auto body = [&](auto kt, const auto &A, int j, float w, float, float) __attribute__((always_inline)) {
+        const float mr = A.mr(kt, j);
+        const float2 v = A.vel(kt, j);
+        s0.add((mr * (v.x - vx)) * w);
+        s1.add((mr * (v.y - vy)) * w);
+    };
+    if (inside) {
+        const double2 xr = S.x[r];
+        const double hr = S.hsml[r];
+        if (lds) {
+            lw.run(LdsFieldsMix<LY>{}, on_the_fly(body, to_rel(xr, o), (float)hr, hu));
+        } else {
+            int cx, cy;
+            cell_xy(V.cell[r], C.g, cx, cy);
+            walk_stencil<(1 << KR)>(GlobalCells{V.kstart, C.g, C.L}, GlobalFieldsMix{S, o}, C.g, cx, cy, xr, hr, KR,
+                                    r, false, true, V.pi, NoClass{}, body);
+        }
+    }
+    xv[r] = make_double2(s0.get(), s1.get());
+}
+
 __global__ void k_set_step(double *__restrict__ stepv, double time_sph, double gx, double gy) {
     if (threadIdx.x == 0) {
         stepv[0] = time_sph;

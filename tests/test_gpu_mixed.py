@@ -56,6 +56,8 @@ def run_pair(deck_name, steps, skin=0.0):
     ("c1_collapse", 1, 5e-5),      # stresses start at zero: first-step sums cancel
     ("v1_vslope_sp1", 200, 1e-5),  # prescribed velocities and stresses, elastic
     ("l1_localisation", 50, 1e-5),  # BCs with a time ramp, Perzyna von Mises, before yield
+    ("b1_bui_sp1", 50, 1e-5),      # XSPH position update (Bui fork slope)
+    ("t4_xsph", 1, 1e-5),
 ])
 def test_mixed_close_to_oracle(deck_name, steps, tol):
     err, _, _ = run_pair(deck_name, steps)
@@ -71,7 +73,8 @@ def test_mixed_pairs_and_stats_equal_double_form():
         assert sm[k] == so[k], f"stats {k}: {sm[k]} vs {so[k]}"
 
 
-@pytest.mark.parametrize("deck_name,steps,tol", [("t3_elastic", 200, 1e-5), ("t2_cspm", 1, 1e-5)])
+@pytest.mark.parametrize("deck_name,steps,tol", [("t3_elastic", 200, 1e-5), ("t2_cspm", 1, 1e-5),
+                                                ("b1_bui_sp1", 50, 1e-5)])
 def test_mixed_with_list_skin_close_to_oracle(deck_name, steps, tol):
     err, mix, _ = run_pair(deck_name, steps, skin=0.25)
     n = mix.search_count()

@@ -216,7 +216,7 @@ def test_error_behaviour(oracle_lib):
     sps = load_pkg()
     deck = sps.read_deck(deck_dir("t1_tiny"))
     parts = sps.build_particles(deck)
-    for name, mode in (("xsph", sps.MODE_FAST), ("art_stress", sps.MODE_ORDERED)):
+    for name, mode in (("cont_density", sps.MODE_FAST), ("art_stress", sps.MODE_ORDERED)):
         prm = sps.make_params(deck, parts, mode=mode)
         setattr(prm, name, 1)
         with pytest.raises(sps.SpsError) as e:
