@@ -233,3 +233,24 @@ def test_error_behaviour(oracle_lib):
     n = ctypes.c_int64(0)
     rc = sim.lib.sps_get_pairs(sim.handle, 10, None, None, None, None, None, None, ctypes.byref(n))
     assert rc == -4 and n.value == sim.stats()["n_pairs"]
+
+
+def test_non_finite_position_is_flagged_out_of_domain(oracle_lib):
+    """A particle whose position is not finite is flagged out of domain (the
+    reference's dxx > 0 test misses NaN) and never enters the search: the
+    step completes and the particle has no pairs."""
+    sps = load_pkg()
+    deck = sps.read_deck(deck_dir("t1_tiny"))
+    parts = sps.build_particles(deck)
+    sim = sps.Simulation(sps.make_params(deck, parts), oracle_lib)
+    st = sps.initial_state(parts, sim.niv)
+    k = parts.nnode + 3                       # a stress point
+    st["x"][k] = (np.nan, 0.005)
+    sim.set_state(st)
+    sim.deck = deck
+    sim.step(2)
+    s = sim.get_state(["if_out_domain"])
+    assert s["if_out_domain"][k] == 1
+    assert int(s["if_out_domain"].sum()) == 1
+    p = sim.get_pairs()
+    assert not np.any(p["i"] == k + 1) and not np.any(p["j"] == k + 1)
