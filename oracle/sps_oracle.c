@@ -1221,8 +1221,11 @@ static int check_params(const sps_params *p)
         if (p->npoints < 1 || p->npoints > 4 || (long long)p->nstress != (long long)p->npoints * p->nnode)
             return fail(SPS_EINVAL, "outside approach: npoints must be 1..4 and nstress = npoints * nnode");
         if (p->vel_vector && p->npoints != 2) return fail(SPS_EINVAL, "vel_vector needs npoints = 2");
+        if (p->vel_vector && p->mode != SPS_MODE_ORDERED)
+            return fail(SPS_EINVAL, "the velocity-vector shift runs in SPS_MODE_ORDERED: it puts a node's stress "
+                                    "points on the side of the sign of each velocity component, which single "
+                                    "precision flips for components near zero");
         if (p->shift_update < 1) return fail(SPS_EINVAL, "shift_update must be >= 1");
-        if (p->mode != SPS_MODE_ORDERED) return fail(SPS_EINVAL, "the outside approach runs in SPS_MODE_ORDERED");
     }
     if (p->reserved0 != 0) return fail(SPS_EINVAL, "reserved0 must be 0");
     if (p->cont_density || p->art_stress)

@@ -318,8 +318,11 @@ static int check_params(const sps_params *p) {
         if (p->npoints < 1 || p->npoints > 4 || (long long)p->nstress != (long long)p->npoints * p->nnode)
             return fail(SPS_EINVAL, "outside approach: npoints must be 1..4 and nstress = npoints * nnode");
         if (p->vel_vector && p->npoints != 2) return fail(SPS_EINVAL, "vel_vector needs npoints = 2");
+        if (p->vel_vector && p->mode != SPS_MODE_ORDERED)
+            return fail(SPS_EINVAL, "the velocity-vector shift runs in SPS_MODE_ORDERED: it puts a node's stress "
+                                    "points on the side of the sign of each velocity component, which single "
+                                    "precision flips for components near zero");
         if (p->shift_update < 1) return fail(SPS_EINVAL, "shift_update must be >= 1");
-        if (p->mode != SPS_MODE_ORDERED) return fail(SPS_EINVAL, "the outside approach runs in SPS_MODE_ORDERED");
     }
     if (p->reserved0 != 0) return fail(SPS_EINVAL, "reserved0 must be 0");
     if (p->cont_density || p->art_stress)
@@ -704,7 +707,7 @@ static int lists_still_valid(sps_handle *h, bool &valid) {
 // synchronisation and no memset, so that it can be captured in a graph.
 // stepv: the scalars of a replayed step (k_set_step), or null.
 static int mixed_step_kernels(sps_handle *h, double dt, double time_sph, double gxg, double gyg,
-                              const double *stepv) {
+                              const double *stepv, bool pre_shift = false, int shift = -1) {
     const int nn = h->nn, nt = h->nt, nt2 = h->nt2;
     StateBufs SB = h->S[h->cur].view();
     Work W{h->vx0.p, h->x0.p, h->vel0.p, h->rk_vel.p, h->rhs2.p, h->f_bound.p, h->art_visc.p,
@@ -719,6 +722,16 @@ static int mixed_step_kernels(sps_handle *h, double dt, double time_sph, double 
     // a captured step reads the values k_bc_values leaves before each replay
     const bool bcs = h->p.no_bcs > 0 && nn > 0;
     if (bcs && !stepv) LAUNCH(K_BCS, 1, 64, k_bc_values, bc_values(h, time_sph), h->bc_vals.p);
+    if (pre_shift) {
+        // the outside approach's stress_point_update + adapt_stress2 on the
+        // step's lists (2_...:99-109)
+        if (nn > 0)
+            LAUNCH_LDS(K_SPU_NODES, nblk8(nn), TILE, LAY_SPU_NODES_MIX.bytes, k_spu_nodes_mix, SB, V, PL, adapt,
+                       tanfi, coh, hu, nn);
+        if (nt > nn)
+            LAUNCH_LDS(K_SP_UPDATE, nblk8(nt - nn), TILE, LAY_SP_UPDATE_MIX.bytes, k_sp_update_mix, SB, W, V, PL,
+                       h->dp, time_sph, 0.0, 0, adapt, hu, nn, nt, stepv);
+    }
     LAUNCH(K_RK_INIT, nblk(nt2), BLOCK, k_rk_init, SB, W, nn, nt, nt2);
     for (int st = 0; st < 4; ++st) {
         const double c1 = f1rk[st] * (dt);
@@ -760,6 +773,10 @@ static int mixed_step_kernels(sps_handle *h, double dt, double time_sph, double 
         LAUNCH_LDS(K_XSPH, nblk8(nt - nn), TILE, LAY_XSPH_MIX.bytes, (k_xsph_mix<KIND_SP>), SB, V, PL, hu, h->xsph_v.p);
     LAUNCH(K_POSITION, nblk(nt), BLOCK, k_position, SB, W, dt, h->p.update_x, nn, nt,
            xsph ? (const double2 *)h->xsph_v.p : (const double2 *)nullptr);
+    if (shift >= 0 && h->p.update_x && nn > 0) {
+        const ShiftParams sp{h->p.npoints, h->p.vel_vector, shift, h->p.r_x, h->p.r_y, h->p.disp_tol, h->p.dx};
+        LAUNCH(K_SHIFT, nblk(nn), BLOCK, k_shift_sp, SB, h->inv.p, h->nodeinfo.p, h->x10.p, h->disp10.p, sp, nn);
+    }
     if (h->skin > 0) {
         LAUNCH(K_POSITION, 1, 64, k_zero_u64, h->dmax.p, 1);
         LAUNCH(K_POSITION, std::min(1024u, nblk(nt)), BLOCK, k_max_disp, SB.x, h->x_build.p, nt, h->dmax.p);
@@ -831,9 +848,14 @@ static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
     const double gxg = h->p.ic_grav ? factg * h->p.cgrav[0] : 0.0;
     const double gyg = h->p.ic_grav ? factg * h->p.cgrav[1] : 0.0;
     if (h->mixed) {
-        // replayed as a graph when enabled and no kernel timing is asked for
-        rc = (h->use_graph && !h->prof) ? mixed_step_graph(h, dt, time_sph, gxg, gyg)
-                                        : mixed_step_kernels(h, dt, time_sph, gxg, gyg, nullptr);
+        // the outside approach's per-step work depends on the step number
+        const bool pre = h->outside && h->p.sph_shift && itimestep > 1 && (itimestep - 1) % h->p.shift_update == 0;
+        const int shift = h->outside ? (itimestep % h->p.shift_update == 0 ? 1 : 0) : -1;
+        // replayed as a graph when enabled, no kernel timing is asked for and
+        // the step's launches do not depend on its number
+        rc = (h->use_graph && !h->prof && !h->outside)
+                 ? mixed_step_graph(h, dt, time_sph, gxg, gyg)
+                 : mixed_step_kernels(h, dt, time_sph, gxg, gyg, nullptr, pre, shift);
         if (rc) return rc;
         if (h->skin > 0) {
             HIP_TRY(hipMemcpyAsync(h->pinned + 32, h->dmax.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));

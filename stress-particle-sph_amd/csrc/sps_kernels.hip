@@ -1069,7 +1069,7 @@ struct PairLists {
     StagePlan *plans;               // per tile and kind
     // per node slot (the outside approach; null otherwise): 2 * its node
     // partners + 1 if it has a wall-dummy partner, i.e. n_int and bc_int of
-    // the reference (2_...:579, 870-871), from a search without skin
+    // the reference (2_...:579, 870-871), counted within 2h at the search
     int *nodeinfo;
 };
 
@@ -1256,6 +1256,10 @@ __global__ __launch_bounds__(TILE) void k_count_lists(SearchView V, const double
                                 const bool acc = decltype(uni)::value ? accepted_by(dx0, dx1, ub)
                                                                       : accepted_by(dx0, dx1, accept_bounds(hr, hs));
                                 if (acc) fk[ks] += (kind_after || (slot_after && j > self)) ? 1u : 0u;
+                                if constexpr (KR == KIND_NODE) {
+                                    n_node += (acc && ks == KIND_NODE) ? 1u : 0u;
+                                    any_dummy = any_dummy || (acc && ks == KIND_DUMMY);
+                                }
                             } else {
                                 if constexpr (decltype(uni)::value) {
                                     if (!accepted_by(xr.x - xs.x, xr.y - xs.y, ub)) continue;

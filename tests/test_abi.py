@@ -141,11 +141,11 @@ def outside_params(sps, **change):
     return prm
 
 
-@pytest.mark.parametrize("change", [dict(mode=1), dict(vel_vector=1, npoints=3), dict(shift_update=0),
+@pytest.mark.parametrize("change", [dict(vel_vector=1, mode=1), dict(vel_vector=1, npoints=3), dict(shift_update=0),
                                     dict(npoints=3), dict(inside_approach=1, sph_shift=1), dict(reserved0=1)])
 def test_outside_approach_rejections(change):
     """Both libraries reject the same outside-approach settings at sps_create
-    (before looking for a device): the mixed form, vel_vector without two
+    (before looking for a device): vel_vector in the mixed form or without two
     points per node, shift_update < 1, nstress != npoints * nnode, sph_shift
     with the inside approach."""
     sps = load_pkg()

@@ -58,7 +58,8 @@ def test_cpu_build_of_kernels_matches_oracle(emu_lib, deck_name, steps):
 @pytest.mark.parametrize("case,steps,tol", [("t3_elastic:mixed", 30, "1e-5"), ("t2_cspm:mixed", 1, "1e-5"),
                                             ("c1_collapse:mixed", 1, "5e-5"),
                                             ("v3_vslope_sp3:mixed", 20, "1e-5"), ("l1_localisation:mixed", 10, "1e-5"),
-                                            ("t4_xsph:mixed", 1, "1e-5"), ("b1_bui_sp1:skin", 15, "1e-5")])
+                                            ("t4_xsph:mixed", 1, "1e-5"), ("b1_bui_sp1:skin", 15, "1e-5"),
+                                            ("b8_bui_out3:skin", 15, "1e-5")])
 def test_cpu_build_of_mixed_kernels_near_oracle(emu_lib, case, steps, tol):
     env = dict(os.environ, LD_PRELOAD=_asan_runtime(),
                ASAN_OPTIONS="detect_leaks=0:detect_stack_use_after_return=0:halt_on_error=1")

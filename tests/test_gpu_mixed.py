@@ -57,6 +57,8 @@ def run_pair(deck_name, steps, skin=0.0):
     ("v1_vslope_sp1", 200, 1e-5),  # prescribed velocities and stresses, elastic
     ("l1_localisation", 50, 1e-5),  # BCs with a time ramp, Perzyna von Mises, before yield
     ("b1_bui_sp1", 50, 1e-5),      # XSPH position update (Bui fork slope)
+    ("b4_bui_outside", 50, 1e-5),  # the outside approach, fixed shifts
+    ("b8_bui_out3", 50, 1e-5),
     ("t4_xsph", 1, 1e-5),
 ])
 def test_mixed_close_to_oracle(deck_name, steps, tol):
@@ -74,7 +76,7 @@ def test_mixed_pairs_and_stats_equal_double_form():
 
 
 @pytest.mark.parametrize("deck_name,steps,tol", [("t3_elastic", 200, 1e-5), ("t2_cspm", 1, 1e-5),
-                                                ("b1_bui_sp1", 50, 1e-5)])
+                                                ("b1_bui_sp1", 50, 1e-5), ("b8_bui_out3", 50, 1e-5)])
 def test_mixed_with_list_skin_close_to_oracle(deck_name, steps, tol):
     err, mix, _ = run_pair(deck_name, steps, skin=0.25)
     n = mix.search_count()
