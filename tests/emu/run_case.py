@@ -71,4 +71,15 @@ def main(emu_lib, oracle_lib, case, steps, tol="1e-4"):
 
 
 if __name__ == "__main__":
-    sys.exit(main(sys.argv[1], sys.argv[2], sys.argv[3], int(sys.argv[4]), *sys.argv[5:]))
+    # run_case.py EMU_LIB ORACLE_LIB CASE STEPS [TOL] [CASE STEPS [TOL] ...]: several
+    # cases in one process (a TOL is recognised by its decimal exponent)
+    args, rc = sys.argv[3:], 0
+    while args:
+        case, steps = args[0], int(args[1])
+        rest = args[2:]
+        tol = []
+        if rest and "e-" in rest[0]:
+            tol, rest = [rest[0]], rest[1:]
+        rc |= main(sys.argv[1], sys.argv[2], case, steps, *tol)
+        args = rest
+    sys.exit(rc)

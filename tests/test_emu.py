@@ -37,35 +37,42 @@ def emu_lib():
     return build_emu.build()
 
 
-@pytest.mark.parametrize("deck_name,steps", [("t1_tiny", 30), ("t2_cspm", 30), ("t3_elastic", 30),
-                                             ("c1_collapse", 3), ("t2_cspm:hvar", 30), ("t2_cspm:graph", 30),
-                                             ("t4_xsph", 30), ("b3_bui_sp3", 15), ("b4_bui_outside", 15),
-                                             ("b6_bui_velvec_shift:kick", 15), ("v3_vslope_sp3", 20),
-                                             ("p1_vm", 30), ("p4_mc", 30), ("l1_localisation", 10),
-                                             ("l1_localisation:graph", 10)])
-def test_cpu_build_of_kernels_matches_oracle(emu_lib, deck_name, steps):
+# several cases per process: each process pays the start-up of the
+# AddressSanitizer build once
+EXACT_GROUPS = {
+    "tiny": [("t1_tiny", 20), ("t2_cspm", 20), ("t3_elastic", 20), ("t2_cspm:hvar", 20), ("t2_cspm:graph", 20),
+             ("t4_xsph", 20), ("p1_vm", 20), ("p4_mc", 20)],
+    "bui": [("b3_bui_sp3", 8), ("b4_bui_outside", 8), ("b6_bui_velvec_shift:kick", 8)],
+    "examples": [("c1_collapse", 2), ("v3_vslope_sp3", 10), ("l1_localisation", 6), ("l1_localisation:graph", 6)],
+}
+
+
+def run_cases(emu_lib, cases):
     env = dict(os.environ, LD_PRELOAD=_asan_runtime(),
                ASAN_OPTIONS="detect_leaks=0:detect_stack_use_after_return=0:halt_on_error=1")
-    p = subprocess.run([sys.executable, os.path.join(ROOT, "tests", "emu", "run_case.py"), emu_lib, ORACLE_LIB,
-                        deck_name, str(steps)], env=env, capture_output=True, text=True, timeout=900)
-    print(p.stdout[-2000:], p.stderr[-4000:])
+    argv = [a for c in cases for a in map(str, c)]
+    return subprocess.run([sys.executable, os.path.join(ROOT, "tests", "emu", "run_case.py"), emu_lib,
+                           ORACLE_LIB] + argv, env=env, capture_output=True, text=True, timeout=1500)
+
+
+@pytest.mark.parametrize("group", sorted(EXACT_GROUPS))
+def test_cpu_build_of_kernels_matches_oracle(emu_lib, group):
+    p = run_cases(emu_lib, EXACT_GROUPS[group])
+    print(p.stdout[-3000:], p.stderr[-4000:])
     assert p.returncode == 0, p.stderr[-4000:] or p.stdout
 
 
 # the mixed-precision kernels: in bounds (AddressSanitizer) and within single-
 # precision rounding of the oracle where the problem does not amplify it (an
 # elastic deck; a yielding deck before yield starts)
-@pytest.mark.parametrize("case,steps,tol", [("t3_elastic:mixed", 30, "1e-5"), ("t2_cspm:mixed", 1, "1e-5"),
-                                            ("c1_collapse:mixed", 1, "5e-5"),
-                                            ("v3_vslope_sp3:mixed", 20, "1e-5"), ("l1_localisation:mixed", 10, "1e-5"),
-                                            ("t4_xsph:mixed", 1, "1e-5"), ("b1_bui_sp1:skin", 15, "1e-5"),
-                                            ("b8_bui_out3:skin", 15, "1e-5")])
-def test_cpu_build_of_mixed_kernels_near_oracle(emu_lib, case, steps, tol):
-    env = dict(os.environ, LD_PRELOAD=_asan_runtime(),
-               ASAN_OPTIONS="detect_leaks=0:detect_stack_use_after_return=0:halt_on_error=1")
-    p = subprocess.run([sys.executable, os.path.join(ROOT, "tests", "emu", "run_case.py"), emu_lib, ORACLE_LIB,
-                        case, str(steps), tol], env=env, capture_output=True, text=True, timeout=900)
-    print(p.stdout[-2000:], p.stderr[-4000:])
+MIXED_CASES = [("t3_elastic:mixed", 20, "1e-5"), ("t2_cspm:mixed", 1, "1e-5"), ("c1_collapse:mixed", 1, "5e-5"),
+               ("v3_vslope_sp3:mixed", 10, "1e-5"), ("l1_localisation:mixed", 6, "1e-5"),
+               ("t4_xsph:mixed", 1, "1e-5"), ("b1_bui_sp1:skin", 8, "1e-5"), ("b8_bui_out3:skin", 8, "1e-5")]
+
+
+def test_cpu_build_of_mixed_kernels_near_oracle(emu_lib):
+    p = run_cases(emu_lib, MIXED_CASES)
+    print(p.stdout[-3000:], p.stderr[-4000:])
     assert p.returncode == 0, p.stderr[-4000:] or p.stdout
 
 
