@@ -448,7 +448,7 @@ def main():
                     help="list skin of the mixed form's list-reuse leg, in lattice spacings")
     ap.add_argument("--log-cap", type=int, default=4_000_000, help="plastic tuples logged for the surrogate")
     ap.add_argument("--epochs", type=int, default=200, help="surrogate training epochs")
-    ap.add_argument("--surrogate-tol", type=float, default=1e-4, help="yield-function residual kept")
+    ap.add_argument("--surrogate-tol", type=float, default=1e-5, help="yield-function residual kept")
     ap.add_argument("--instances", type=int, default=64, help="slope instances of the batched form (0: skip)")
     ap.add_argument("--ens-acc-steps", type=int, default=2000,
                     help="steps of the combined form's accuracy run on the ensemble (0: skip the combined form)")
