@@ -260,8 +260,9 @@ int sps_set_list_skin(sps_handle *h, double skin);
 /* Replay of the mixed form's step (SPS_MODE_FAST; no counterpart in the
  * reference): with enable != 0, the launches of a step that reuses the pair
  * lists are captured once into a HIP graph (one per state buffer, again when
- * dt changes) and replayed; steps that search, and steps while kernel timing
- * is on, launch directly. */
+ * dt changes) and replayed; steps that search, steps while kernel timing is
+ * on and every step of the outside approach (whose shifts depend on the step
+ * number) launch directly. */
 int sps_set_graph(sps_handle *h, int32_t enable);
 int sps_get_search_count(sps_handle *h, int64_t *n);
 
