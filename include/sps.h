@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define SPS_ABI_VERSION 2
+#define SPS_ABI_VERSION 3
 
 enum {
     SPS_OK = 0,
@@ -219,11 +219,17 @@ int sps_set_materials(sps_handle *h, int32_t nmat, const double *table, const in
  * the globals bc_list, bc_info, BC_or_not and the time curves
  * (3_...:160-260, 1030-1113).  Required before sps_step when
  * sps_params.no_bcs > 0, which sps_create accepts in SPS_MODE_ORDERED, not
- * with XSPH and update_x together.  The reference's free-surface detection
- * (3_...:1116-1411, every step with update_x) never changes a node whose
- * BC_or_not is 1, the only nodes Normal_BCs acts on, so it is not run; with
- * XSPH, XSPH_update would mark such nodes next to the surface 3
- * (2_...:224-230) and the detection then 0.  The BC time is
+ * with XSPH and update_x together (XSPH_update marks nodes next to the free
+ * surface 3, 2_...:224-230, and the detection then resets even a prescribed
+ * node to 0).  With update_x, every step runs the free-surface detection
+ * (get_nodes_on_free_surface, 3_...:1116-1411) after the position update:
+ * BC_or_not becomes 2 on the surface and 0 inside (1 stays 1), with the
+ * surface normals; ifsigman = 1 then applies apply_stress_free
+ * (3_...:1758-1841) after Normal_BCs at every BC call: each surface node's
+ * stress keeps only its tangential part, sigma_tt in the normal frame
+ * (not with wall dummies, whose node pairs set the bc_int it reads).  The
+ * mixed form (SPS_MODE_FAST) does not run the detection and rejects
+ * ifsigman = 1; in it BC_or_not stays as set.  The BC time is
  * time_sph: the reference adds ic_time*dt_sph, ic_time being a local it never
  * sets (3_...:1622-1627).  Values are applied to the nodes 1..nnode with
  * BC_or_not = 1, as code/ does (the vertical-slope fork loops over ntotal;
@@ -232,7 +238,7 @@ int sps_set_materials(sps_handle *h, int32_t nmat, const double *table, const in
 #define SPS_MAX_CURVES 16
 typedef struct sps_bcs {
     int32_t no_bcs;            /* rows of bc_list, 1..SPS_MAX_BCS, = sps_params.no_bcs */
-    int32_t ifsigman;          /* must be 0 (apply_stress_free needs the free surface) */
+    int32_t ifsigman;          /* 0, or 1: apply_stress_free on the free-surface nodes */
     int32_t ncurves;           /* time curves, 0..SPS_MAX_CURVES */
     int32_t reserved0;         /* must be 0 */
     const double *bc_list;     /* (8,no_bcs), 3_...:176-180: number; variable (1 sxx,
@@ -241,12 +247,23 @@ typedef struct sps_bcs {
                                   k > 0: a1 times time curve k); a1; a0; w; phi
                                   (already times twopi); Tf */
     const int32_t *bc_info;    /* (8,nnode): Get_BCs_on_node's rows of the nodes */
-    const int32_t *bc_or_not;  /* (nnode) */
+    const int32_t *bc_or_not;  /* (ntotal) Get_BCs_on_node's BC_or_not of every
+                                  particle, nodes then stress points: 0 or 1 */
     const int32_t *curve_npts; /* (ncurves) nptstcurves, 2..SPS_MAX_TCURVE */
     const double *curve_t;     /* (SPS_MAX_TCURVE,ncurves) ttcurves */
     const float *curve_f;      /* (SPS_MAX_TCURVE,ncurves) ftcurves */
 } sps_bcs;
 int sps_set_bcs(sps_handle *h, const sps_bcs *bcs);
+
+/* The free surface as the last step's detection left it (SPS_MODE_ORDERED,
+ * after sps_set_bcs): bc_or_not (ntotal) the reference's BC_or_not (0
+ * inside, 1 prescribed, 2 on the free surface), normal (2,ntotal) its
+ * global normal: the unit outward normals of the surface particles, 0
+ * elsewhere, NaN where the reference divides zero by zero.  Either pointer
+ * may be null.  Replaces reading BC_or_not and normal after
+ * get_nodes_on_free_surface (3_...:1116-1411; the reference's
+ * surface_points output, 3_...:2910-2919, lists the nodes with 2). */
+int sps_get_surface(sps_handle *h, int32_t *bc_or_not, double *normal);
 
 /* Pair lists kept over steps.  No counterpart in the reference, which
  * searches every step; SPS_MODE_FAST only.  With skin > 0 the search also

@@ -109,6 +109,7 @@ program sps_ref_dump
      end if
      if (any(dump_steps(1:max(ndump,1)) == istep)) then
         call dump_state(istep, 'state_')
+        call dump_surface(istep)
 #ifndef SPS_DROPIN
         call dump_pairs(istep)
 #endif
@@ -151,6 +152,21 @@ contains
     end if
     close(u)
   end subroutine dump_state
+
+  ! BC_or_not and normal of every particle (free-surface detection,
+  ! 3_...:1116-1411), when the BCs allocated them
+  subroutine dump_surface(step)
+    integer, intent(in) :: step
+    character(len=640) :: fname
+    integer :: u
+    if (.not. (allocated(BC_or_not) .and. allocated(normal))) return
+    write(fname, '(A,A,I6.6,A)') trim(outdir), '/surf_', step, '.bin'
+    open(newunit=u, file=trim(fname), access='stream', form='unformatted', status='replace')
+    write(u) ntotal
+    write(u) BC_or_not(1:ntotal)
+    write(u) normal(1:2, 1:ntotal)
+    close(u)
+  end subroutine dump_surface
 
   subroutine dump_pairs(step)
     integer, intent(in) :: step

@@ -57,7 +57,12 @@ struct sps_handle {
     /* prescribed BCs (sps_set_bcs) */
     int bcs_set, nbc, ncurves;
     double bc_list[8 * SPS_MAX_BCS];
-    int32_t *bc_info, *bc_or_not;
+    int32_t *bc_info, *bc_or_not;   /* bc_or_not: every particle (Get_BCs_on_node) */
+    int ifsigman;
+    /* free-surface detection (get_nodes_on_free_surface): the unit normals
+     * (double, as the reference's global) and the subsets of every particle */
+    double *normal;
+    int32_t *subset;
     int32_t curve_npts[SPS_MAX_CURVES];
     double curve_t[SPS_MAX_TCURVE * SPS_MAX_CURVES];
     float curve_f[SPS_MAX_TCURVE * SPS_MAX_CURVES];
@@ -124,7 +129,8 @@ static int alloc_all(struct sps_handle *h)
     A(vel_tmp, 2 * nt, double); A(stress_tmp, 4 * nt, double); A(ivar_tmp, nt, double);
     A(av_tmp, 2 * nn, float);
     A(x10, 2 * nn, double); A(disp10, nn, double); A(n_int, nn, float); A(bc_int, nn, int32_t);
-    A(bc_info, 8 * nn, int32_t); A(bc_or_not, nn, int32_t);
+    A(bc_info, 8 * nn, int32_t); A(bc_or_not, nt, int32_t);
+    A(normal, 2 * nt, double); A(subset, nt, int32_t);
 #undef A
     return 0;
 }
@@ -140,7 +146,7 @@ static void free_all(struct sps_handle *h)
                     h->rhs2, h->divf1, h->divf2, h->source_stress, h->g_grav, h->spin,
                     h->grad1, h->grad2, h->ae, h->dummy_vel, h->cspm_norm, h->vel_tmp,
                     h->stress_tmp, h->ivar_tmp, h->av_tmp, h->x10, h->disp10, h->n_int, h->bc_int,
-                    h->bc_info, h->bc_or_not};
+                    h->bc_info, h->bc_or_not, h->normal, h->subset};
     for (size_t k = 0; k < sizeof ptrs / sizeof ptrs[0]; ++k) free(ptrs[k]);
 }
 
@@ -1047,6 +1053,171 @@ static void normal_bcs(struct sps_handle *h, double time_sph)
             else if (var == 2) h->stress[4 * i + 1] = v;
         }
     }
+    /* apply_stress_free (3_...:1758-1841, BCs 3_...:1633-1636): the free-
+     * surface nodes without a wall-dummy partner keep only the tangential
+     * stress of their normal frame; nodes whose normal is NaN are skipped */
+    if (h->ifsigman != 1) return;
+    for (int i = 0; i < h->nn; ++i) {
+        if (h->bc_or_not[i] != 2 || h->bc_int[i] == 1) continue;
+        const double costh = h->normal[2 * i], sinth = h->normal[2 * i + 1];
+        if (isnan(costh) || isnan(sinth)) continue;
+        double *st = h->stress + 4 * (size_t)i;
+        const double s2 = sinth * sinth, c2 = costh * costh, sc = sinth * costh;
+        const double sigmatt = s2 * st[0] - 2 * sc * st[2] + c2 * st[1];
+        st[0] = s2 * sigmatt;
+        st[1] = c2 * sigmatt;
+        st[2] = -sc * sigmatt;
+        st[3] = c2 * sigmatt;
+    }
+}
+
+/* ------------------------------------------------ get_nodes_on_free_surface */
+/* code/3_SPH_material_2018.f90:1116-1411, called after the position update
+ * (2_...:150-154).  Its locals are single precision, normal and subset are
+ * the reference's double-precision globals; every mixed expression below is
+ * promoted and rounded where the Fortran assignment does it.  x**0.5 is a
+ * square root there (the compiler lowers it to sqrtss / sqrtsd) and 2**0.5
+ * the single-precision sqrt(2). */
+static int fs_hit(const struct sps_handle *h, const float *tt, const float *tau, int r, int p)
+{
+    /* does partner p fall in the scan region of particle r (3_...:1250-1278) */
+    const double *xr = h->x + 2 * (size_t)r, *xp = h->x + 2 * (size_t)p;
+    const double d0 = xp[0] - xr[0], d1 = xp[1] - xr[1];
+    const float dist = (float)sqrt(d0 * d0 + d1 * d1);
+    const float v0 = (float)(xp[0] - (double)tt[2 * r]), v1 = (float)(xp[1] - (double)tt[2 * r + 1]);
+    const float vnorm = sqrtf(v0 * v0 + v1 * v1);
+    const float prod = (float)(fabs(h->normal[2 * r] * (double)v0 + h->normal[2 * r + 1] * (double)v1) +
+                               (double)fabsf(tau[2 * r] * v0 + tau[2 * r + 1] * v1));
+    const double hr = h->hsml[r];
+    const float limit = (float)((double)1.41421354f * hr);
+    if (dist >= limit && (double)vnorm < hr) return 1;   /* set 1 */
+    if (dist < limit && (double)prod < hr) return 1;     /* set 2 */
+    return 0;
+}
+
+static void fs_hf(const struct sps_handle *h, int64_t k, float *h1i, float *h1j, float *h2i, float *h2j)
+{
+    const int i = h->pi[k], j = h->pj[k];
+    *h1i = (float)(h->mass[j] * (double)h->pgx[k] / h->rho[j]);
+    *h1j = (float)(-(h->mass[i] * (double)h->pgx[k] / h->rho[i]));
+    *h2i = (float)(h->mass[j] * (double)h->pgy[k] / h->rho[j]);
+    *h2j = (float)(-(h->mass[i] * (double)h->pgy[k] / h->rho[i]));
+}
+
+static int free_surface(struct sps_handle *h)
+{
+    const int nt = h->nt;
+    const double *x = h->x;
+    double *nrm = h->normal;
+    int32_t *sub = h->subset, *bc = h->bc_or_not;
+    float *A = calloc(4 * (size_t)nt, sizeof(float)), *ff = calloc(2 * (size_t)nt, sizeof(float));
+    float *tt = calloc(2 * (size_t)nt, sizeof(float)), *tau = calloc(2 * (size_t)nt, sizeof(float));
+    float *fint = calloc((size_t)nt, sizeof(float)), *gf = calloc(2 * (size_t)nt, sizeof(float));
+    float *nb = calloc((size_t)nt, sizeof(float));
+    if (!A || !ff || !tt || !tau || !fint || !gf || !nb) {
+        free(A); free(ff); free(tt); free(tau); free(fint); free(gf); free(nb);
+        return fail(SPS_ENOMEM, "free-surface work arrays");
+    }
+    memset(sub, 0, (size_t)nt * sizeof(int32_t));
+    /* step 1 (3_...:1160-1225): renormalisation matrix and sum of m/rho grad W
+     * over the node-stress point, stress point and node pairs, in list order */
+    for (int64_t k = 0; k < h->n_pairs; ++k) {
+        const int t = h->pt[k];
+        if (t != 1 && t != 2 && t != 3) continue;
+        const int i = h->pi[k], j = h->pj[k];
+        const double *xi = x + 2 * (size_t)i, *xj = x + 2 * (size_t)j;
+        float h1i, h1j, h2i, h2j;
+        fs_hf(h, k, &h1i, &h1j, &h2i, &h2j);
+        float *ai = A + 4 * (size_t)i, *aj = A + 4 * (size_t)j;
+        ai[0] = (float)((double)ai[0] + (xj[0] - xi[0]) * (double)h1i);
+        aj[0] = (float)((double)aj[0] + (xi[0] - xj[0]) * (double)h1j);
+        ff[2 * i] = ff[2 * i] + h1i;
+        ff[2 * j] = ff[2 * j] + h1j;
+        ai[1] = (float)((double)ai[1] + (xj[1] - xi[1]) * (double)h1i);
+        ai[2] = (float)((double)ai[2] + (xj[0] - xi[0]) * (double)h2i);
+        ai[3] = (float)((double)ai[3] + (xj[1] - xi[1]) * (double)h2i);
+        aj[1] = (float)((double)aj[1] + (xi[1] - xj[1]) * (double)h1j);
+        aj[2] = (float)((double)aj[2] + (xi[0] - xj[0]) * (double)h2j);
+        aj[3] = (float)((double)aj[3] + (xi[1] - xj[1]) * (double)h2j);
+        ff[2 * i + 1] = ff[2 * i + 1] + h2i;
+        ff[2 * j + 1] = ff[2 * j + 1] + h2j;
+    }
+    /* step 2 (3_...:1229-1245): first normal estimate; step 3's scan frame */
+    for (int i = 0; i < nt; ++i) {
+        float *a = A + 4 * (size_t)i;
+        for (int q = 0; q < 4; ++q)
+            if (fabsf(a[q]) <= 1.e-8f) a[q] = 0.0f;
+        const float v1 = -(a[0] * ff[2 * i] + a[1] * ff[2 * i + 1]);
+        const float v2 = -(a[2] * ff[2 * i] + a[3] * ff[2 * i + 1]);
+        const float v3 = sqrtf(v1 * v1 + v2 * v2);
+        nrm[2 * i] = (double)(v1 / v3);
+        nrm[2 * i + 1] = (double)(v2 / v3);
+        tt[2 * i] = (float)(x[2 * i] + h->hsml[i] * nrm[2 * i]);
+        tt[2 * i + 1] = (float)(x[2 * i + 1] + h->hsml[i] * nrm[2 * i + 1]);
+        tau[2 * i] = (float)(-nrm[2 * i + 1]);
+        tau[2 * i + 1] = (float)nrm[2 * i];
+    }
+    /* step 3 (3_...:1247-1319): a particle with a partner in its scan region
+     * is inside (subset 2, f_int -1); the rest are on the surface */
+    for (int64_t k = 0; k < h->n_pairs; ++k) {
+        const int t = h->pt[k], i = h->pi[k], j = h->pj[k];
+        if (t == 2 || t == 3) {
+            if (sub[i] == 0 && fs_hit(h, tt, tau, i, j)) { sub[i] = 2; fint[i] = -1; }
+            if (sub[j] == 0 && fs_hit(h, tt, tau, j, i)) { sub[j] = 2; fint[j] = -1; }
+        }
+        if ((t == 6 || t == 9) && sub[j] == 0 && fs_hit(h, tt, tau, j, i)) { sub[j] = 2; fint[j] = -1; }
+    }
+    for (int i = 0; i < nt; ++i)
+        if (sub[i] == 0) sub[i] = 1;
+    /* 3_...:1329-1349: BC_or_not 2 on the surface, 0 inside; prescribed
+     * particles (1) keep it and leave the surface set */
+    for (int i = 0; i < nt; ++i)
+        if (bc[i] != 1) bc[i] = 0;
+    for (int i = 0; i < nt; ++i) {
+        if (sub[i] == 2 && bc[i] != 1) bc[i] = 0;
+        else if (sub[i] == 1 && bc[i] != 1) bc[i] = 2;
+        else if (sub[i] == 1 && bc[i] == 1) sub[i] = 2;
+    }
+    /* step 4 (3_...:1353-1389): the normal from the surface neighbours'
+     * edges, oriented against the gradient of f_int */
+    memset(nrm, 0, 2 * (size_t)nt * sizeof(double));
+    for (int64_t k = 0; k < h->n_pairs; ++k) {
+        const int t = h->pt[k];
+        if (t != 1 && t != 2 && t != 3) continue;
+        const int i = h->pi[k], j = h->pj[k];
+        if (sub[i] == 1 && sub[j] == 1) {
+            const float x1 = (float)x[2 * i], y1 = (float)x[2 * i + 1];
+            const float x2 = (float)x[2 * j], y2 = (float)x[2 * j + 1];
+            const float xv = x2 - x1, yv = y2 - y1;
+            nrm[2 * i] = nrm[2 * i] - (double)yv;
+            nrm[2 * i + 1] = nrm[2 * i + 1] + (double)xv;
+            nrm[2 * j] = nrm[2 * j] - (double)yv;
+            nrm[2 * j + 1] = nrm[2 * j + 1] + (double)xv;
+            nb[i] = nb[i] + 1.0f;
+            nb[j] = nb[j] + 1.0f;
+        }
+        float h1i, h1j, h2i, h2j;
+        fs_hf(h, k, &h1i, &h1j, &h2i, &h2j);
+        gf[2 * i] = gf[2 * i] + (fint[j] - fint[i]) * h1i;
+        gf[2 * j] = gf[2 * j] + (fint[i] - fint[j]) * h1j;
+        gf[2 * i + 1] = gf[2 * i + 1] + (fint[j] - fint[i]) * h2i;
+        gf[2 * j + 1] = gf[2 * j + 1] + (fint[i] - fint[j]) * h2j;
+    }
+    for (int i = 0; i < nt; ++i) {   /* 3_...:1391-1405 */
+        if (sub[i] != 1) continue;
+        nrm[2 * i] = nrm[2 * i] / (double)nb[i];
+        nrm[2 * i + 1] = nrm[2 * i + 1] / (double)nb[i];
+        const float nv = (float)sqrt(nrm[2 * i] * nrm[2 * i] + nrm[2 * i + 1] * nrm[2 * i + 1]);
+        nrm[2 * i] = nrm[2 * i] / (double)nv;
+        nrm[2 * i + 1] = nrm[2 * i + 1] / (double)nv;
+        const float ps = (float)((double)gf[2 * i] * nrm[2 * i] + (double)gf[2 * i + 1] * nrm[2 * i + 1]);
+        if (ps < 0) {
+            nrm[2 * i] = -nrm[2 * i];
+            nrm[2 * i + 1] = -nrm[2 * i + 1];
+        }
+    }
+    free(A); free(ff); free(tt); free(tau); free(fint); free(gf); free(nb);
+    return 0;
 }
 
 /* ------------------------------------------------------- time_integration */
@@ -1185,6 +1356,7 @@ static int time_integration(struct sps_handle *h, int itimestep, double time_sph
     normal_bcs(h, time_sph);
     if (h->p.update_x && h->p.xsph) {
         xsph_update(h, dt);
+        if (h->p.no_bcs > 0 && (rc = free_surface(h))) return rc;
         if (!h->p.inside_approach) shift_stress_points(h, itimestep);
         for (int i = 0; i < nn; ++i)
             for (int a = 0; a < 2; ++a) h->displ[2 * i + a] = h->x[2 * i + a] - h->x00[2 * i + a];
@@ -1194,6 +1366,7 @@ static int time_integration(struct sps_handle *h, int itimestep, double time_sph
                 float vel_half = (float)(0.5 * (h->vx0[2 * i + a] + h->vel[2 * i + a]));
                 h->x[2 * i + a] = h->x0[2 * i + a] + (double)vel_half * dt;
             }
+        if (h->p.no_bcs > 0 && (rc = free_surface(h))) return rc;
         if (!h->p.inside_approach) shift_stress_points(h, itimestep);
         for (int i = 0; i < nn; ++i)
             for (int a = 0; a < 2; ++a) h->displ[2 * i + a] = h->x[2 * i + a] - h->x00[2 * i + a];
@@ -1307,7 +1480,9 @@ int sps_set_bcs(sps_handle *h, const sps_bcs *b)
 {
     if (!h || !b || !b->bc_list || !b->bc_info || !b->bc_or_not) return fail(SPS_EINVAL, "null argument");
     if (b->no_bcs != h->p.no_bcs || b->no_bcs < 1) return fail(SPS_EINVAL, "no_bcs %d, sps_params says %d", b->no_bcs, h->p.no_bcs);
-    if (b->ifsigman != 0 || b->reserved0 != 0) return fail(SPS_EINVAL, "ifsigman and reserved0 must be 0");
+    if ((b->ifsigman != 0 && b->ifsigman != 1) || b->reserved0 != 0) return fail(SPS_EINVAL, "ifsigman must be 0 or 1, reserved0 0");
+    if (b->ifsigman == 1 && h->nd > 0)
+        return fail(SPS_EINVAL, "ifsigman = 1 with wall dummies is not supported (apply_stress_free reads bc_int)");
     if (b->ncurves < 0 || b->ncurves > SPS_MAX_CURVES || (b->ncurves > 0 && (!b->curve_npts || !b->curve_t || !b->curve_f)))
         return fail(SPS_EINVAL, "bad time curves");
     for (int c = 0; c < b->ncurves; ++c)
@@ -1324,7 +1499,10 @@ int sps_set_bcs(sps_handle *h, const sps_bcs *b)
     h->ncurves = b->ncurves;
     memcpy(h->bc_list, b->bc_list, 8 * (size_t)b->no_bcs * sizeof(double));
     memcpy(h->bc_info, b->bc_info, 8 * (size_t)h->nn * sizeof(int32_t));
-    memcpy(h->bc_or_not, b->bc_or_not, (size_t)h->nn * sizeof(int32_t));
+    memcpy(h->bc_or_not, b->bc_or_not, (size_t)h->nt * sizeof(int32_t));
+    for (int i = 0; i < h->nt; ++i)
+        if (b->bc_or_not[i] < 0 || b->bc_or_not[i] > 3) return fail(SPS_EINVAL, "bc_or_not[%d] = %d", i, b->bc_or_not[i]);
+    h->ifsigman = b->ifsigman;
     for (int c = 0; c < b->ncurves; ++c) {
         h->curve_npts[c] = b->curve_npts[c];
         memcpy(h->curve_t + SPS_MAX_TCURVE * c, b->curve_t + SPS_MAX_TCURVE * c, SPS_MAX_TCURVE * sizeof(double));
@@ -1533,4 +1711,13 @@ void sps_destroy(sps_handle *h)
     free(h->mat);
     free(h->mtab);
     free(h);
+}
+
+int sps_get_surface(sps_handle *h, int32_t *bc_or_not, double *normal)
+{
+    if (!h) return fail(SPS_EINVAL, "null argument");
+    if (h->p.no_bcs <= 0 || !h->bcs_set) return fail(SPS_ESTATE, "no BCs set: the free surface is not tracked");
+    if (bc_or_not) memcpy(bc_or_not, h->bc_or_not, (size_t)h->nt * sizeof(int32_t));
+    if (normal) memcpy(normal, h->normal, 2 * (size_t)h->nt * sizeof(double));
+    return 0;
 }

@@ -27,7 +27,7 @@ PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 PRODUCT_LIB = os.path.join(PKG_DIR, "libsps_hip.so")
 DECK_DIR = os.path.join(PKG_DIR, "decks")
 
-SPS_ABI_VERSION = 2
+SPS_ABI_VERSION = 3
 SPS_MAX_TCURVE = 64
 SPS_MAX_BCS = 32
 SPS_MAX_CURVES = 16
@@ -108,6 +108,7 @@ def load_library(path):
     lib.sps_get_state.argtypes = [h, ctypes.POINTER(State)]
     lib.sps_get_pairs.argtypes = [h, ctypes.c_int64] + [_P] * 6 + [ctypes.POINTER(ctypes.c_int64)]
     lib.sps_get_stats.argtypes = [h, ctypes.POINTER(Stats)]
+    lib.sps_get_surface.argtypes = [h, _P, _P]
     lib.sps_set_profiling.argtypes = [h, ctypes.c_int32]
     lib.sps_get_profile.argtypes = [h, ctypes.c_int32, ctypes.POINTER(KernelTime),
                                     ctypes.POINTER(ctypes.c_int32), ctypes.c_int32]
@@ -228,7 +229,8 @@ class Simulation:
 
     def set_bcs(self, deck: Deck, parts: Particles):
         """The deck's prescribed BCs (sps_set_bcs): bc_list, the nodes' rows of
-        Get_BCs_on_node at the initial positions, every time curve."""
+        Get_BCs_on_node at the initial positions, BC_or_not of every particle,
+        every time curve, ifsigman."""
         info, bc_or_not = bcs_on_particles(deck, parts.x[:parts.ntotal])
         nc = len(deck.ttcurves)
         if nc > SPS_MAX_CURVES:
@@ -240,7 +242,7 @@ class Simulation:
             npts[c] = len(deck.ttcurves[c])
             ct[c, :npts[c]], cf[c, :npts[c]] = deck.ttcurves[c], deck.ftcurves[c]
         keep = [np.ascontiguousarray(deck.bc_list, dtype=np.float64), np.ascontiguousarray(info[:self.nn]),
-                np.ascontiguousarray(bc_or_not[:self.nn]), npts, ct, cf]
+                np.ascontiguousarray(bc_or_not[:self.n]), npts, ct, cf]
         b = Bcs(no_bcs=deck.no_bcs, ifsigman=deck.ifsigman, ncurves=nc, reserved0=0)
         for name, a in zip(("bc_list", "bc_info", "bc_or_not", "curve_npts", "curve_t", "curve_f"), keep):
             setattr(b, name, a.ctypes.data)
@@ -287,6 +289,14 @@ class Simulation:
             dt = self.deck.dt
         self._check(self.lib.sps_step(self.handle, nsteps, dt, ctypes.byref(self.time_sph),
                                       ctypes.byref(self.itimestep_sph)))
+
+    def get_surface(self):
+        """BC_or_not (ntotal; 2 on the free surface) and the unit normals
+        (ntotal, 2) of the last step's free-surface detection."""
+        bc = np.zeros(self.n, dtype=np.int32)
+        nrm = np.zeros((self.n, 2))
+        self._check(self.lib.sps_get_surface(self.handle, bc.ctypes.data, nrm.ctypes.data))
+        return bc, nrm
 
     def get_pairs(self):
         n = ctypes.c_int64(0)

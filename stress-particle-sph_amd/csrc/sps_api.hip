@@ -157,7 +157,15 @@ struct sps_handle {
     int curve_npts[SPS_MAX_CURVES];
     double curve_t[SPS_MAX_TCURVE * SPS_MAX_CURVES];
     float curve_f[SPS_MAX_TCURVE * SPS_MAX_CURVES];
-    DBuf<int> bc_info, bc_or_not;
+    DBuf<int> bc_info, bc_or_not;  // bc_or_not: every particle, by original index
+    int ifsigman = 0;
+    // free-surface detection: normals by original index; the pair list in
+    // traversal order and each particle's entries (free_surface)
+    DBuf<double2> normal, surf_nrm0;
+    DBuf<int> surf_pi, surf_pj, surf_pt, surf_sub;
+    DBuf<float> surf_gx, surf_gy;
+    DBuf<unsigned> surf_cnt, surf_start, surf_cur, surf_ent, surf_flag;
+    DBuf<float2> surf_tt, surf_tau;
     DBuf<double> bc_vals;         // the step's values (k_bc_values)
     BcValues bc_vars{};           // the BC rows' variables (values filled per step)
     DBuf<unsigned short> deg_a, deg_b;
@@ -223,14 +231,14 @@ struct sps_handle {
 enum KernelId {
     K_CHECK_OOD, K_BOUNDS, K_CELL_COUNT, K_CELL_SCATTER, K_CELL_SORT, K_PERMUTE, K_PROC,
     K_COUNT_PAIRS, K_SCAN, K_SPLIT, K_RK_INIT, K_STAGE_BEGIN, K_SPU_NODES,
-    K_SP_UPDATE, K_DERIV_NODES, K_DUMMY_STRESS, K_RK_FINAL, K_POSITION, K_SURROGATE, K_XSPH, K_SHIFT, K_BCS,
+    K_SP_UPDATE, K_DERIV_NODES, K_DUMMY_STRESS, K_RK_FINAL, K_POSITION, K_SURROGATE, K_XSPH, K_SHIFT, K_BCS, K_SURF,
     K_NKERNELS
 };
 static const char *const kKernelNames[K_NKERNELS] = {
     "k_check_ood", "k_bounds", "k_cell_count", "k_cell_scatter", "k_cell_sort", "k_permute",
     "k_proc_slot", "k_count", "k_scan", "k_split", "k_rk_init",
     "k_stage_begin", "k_spu_nodes", "k_sp_update", "k_deriv_nodes",
-    "k_dummy_stress", "k_rk_final", "k_position", "k_surrogate", "k_xsph", "k_shift_sp", "k_bcs"};
+    "k_dummy_stress", "k_rk_final", "k_position", "k_surrogate", "k_xsph", "k_shift_sp", "k_bcs", "k_surf"};
 
 static hipEvent_t prof_begin(sps_handle *h) {
     if (!h->prof) return nullptr;
@@ -486,7 +494,8 @@ extern "C" int sps_create(const sps_params *params, sps_handle **out) {
     rc |= dalloc(h->plans, 3 * (size_t)(h->ntile_n + h->ntile_s + 1));
     rc |= dalloc(h->x_build, nt); rc |= dalloc(h->dmax, 1); rc |= dalloc(h->stepv, 4);
     if (params->no_bcs > 0)
-        rc |= dalloc(h->bc_info, 8 * (size_t)nn) | dalloc(h->bc_or_not, nn) | dalloc(h->bc_vals, SPS_MAX_BCS);
+        rc |= dalloc(h->bc_info, 8 * (size_t)nn) | dalloc(h->bc_or_not, nt) | dalloc(h->bc_vals, SPS_MAX_BCS) |
+              dalloc(h->normal, nt);
     if (h->outside) {
         rc |= dalloc(h->inv, nt2); rc |= dalloc(h->nodeinfo, nn);
         rc |= dalloc(h->x10, nn); rc |= dalloc(h->disp10, nn);
@@ -736,11 +745,11 @@ static int mixed_step_kernels(sps_handle *h, double dt, double time_sph, double 
     for (int st = 0; st < 4; ++st) {
         const double c1 = f1rk[st] * (dt);
         LAUNCH(K_STAGE_BEGIN, nblk(nt), BLOCK, k_stage_begin, SB, W, c1, adapt, tanfi, coh, nn, nt);
-        if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, h->bc_vars, h->bc_vals.p, nn);
+        if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, h->bc_vars, h->bc_vals.p, h->ifsigman, h->normal.p, nn);
         if (nn > 0)
             LAUNCH_LDS(K_SPU_NODES, nblk8(nn), TILE, LAY_SPU_NODES_MIX.bytes, k_spu_nodes_mix, SB, V, PL, adapt,
                        tanfi, coh, hu, nn);
-        if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, h->bc_vars, h->bc_vals.p, nn);
+        if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, h->bc_vars, h->bc_vals.p, h->ifsigman, h->normal.p, nn);
         if (nt > nn)
             LAUNCH_LDS(K_SP_UPDATE, nblk8(nt - nn), TILE, LAY_SP_UPDATE_MIX.bytes, k_sp_update_mix, SB, W, V, PL,
                        h->dp, time_sph, f2rk[st], 1, adapt, hu, nn, nt, stepv);
@@ -758,14 +767,14 @@ static int mixed_step_kernels(sps_handle *h, double dt, double time_sph, double 
         int rc = surrogate_correct(h, SB);
         if (rc) return rc;
     }
-    if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, h->bc_vars, h->bc_vals.p, nn);
+    if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, h->bc_vars, h->bc_vals.p, h->ifsigman, h->normal.p, nn);
     if (nn > 0)
         LAUNCH_LDS(K_SPU_NODES, nblk8(nn), TILE, LAY_SPU_NODES_MIX.bytes, k_spu_nodes_mix, SB, V, PL, adapt, tanfi,
                    coh, hu, nn);
     if (nt > nn)
         LAUNCH_LDS(K_SP_UPDATE, nblk8(nt - nn), TILE, LAY_SP_UPDATE_MIX.bytes, k_sp_update_mix, SB, W, V, PL, h->dp,
                    time_sph, 0.0, 0, adapt, hu, nn, nt, stepv);
-    if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, h->bc_vars, h->bc_vals.p, nn);
+    if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, h->bc_vars, h->bc_vals.p, h->ifsigman, h->normal.p, nn);
     const bool xsph = h->xsph && h->p.update_x;
     if (xsph && nn > 0)
         LAUNCH_LDS(K_XSPH, nblk8(nn), TILE, LAY_XSPH_MIX.bytes, (k_xsph_mix<KIND_NODE>), SB, V, PL, hu, h->xsph_v.p);
@@ -828,6 +837,47 @@ static int mixed_step_graph(sps_handle *h, double dt, double time_sph, double gx
     return 0;
 }
 
+// get_nodes_on_free_surface (3_...:1116-1411) after the position update:
+// the step's pair list laid out in traversal order by storage slot, each
+// particle's pairs of types 1-3 gathered in that order, then the four steps
+// of the detection; BC_or_not and the normals by original index.
+static int free_surface(sps_handle *h, const StateBufs &SB) {
+    const long long np = (long long)h->n_pairs;
+    const int nn = h->nn, nt = h->nt, nt2 = h->nt2;
+    int rc = 0;
+    const size_t npa = (size_t)std::max(np, 1LL);
+    rc |= dalloc(h->surf_pi, npa) | dalloc(h->surf_pj, npa) | dalloc(h->surf_pt, npa);
+    rc |= dalloc(h->surf_gx, npa) | dalloc(h->surf_gy, npa) | dalloc(h->surf_ent, 2 * npa);
+    rc |= dalloc(h->surf_cnt, (size_t)nt + 1) | dalloc(h->surf_start, (size_t)nt + 1) | dalloc(h->surf_cur, nt);
+    rc |= dalloc(h->surf_flag, nt) | dalloc(h->surf_sub, nt) | dalloc(h->surf_nrm0, nt);
+    rc |= dalloc(h->surf_tt, nt) | dalloc(h->surf_tau, nt);
+    if (rc) return rc;
+    if (np >= (1LL << 31)) return fail(SPS_ECAPACITY, "free surface: %lld pairs", np);
+    hipStream_t s = h->stream;
+    StateSet &S = h->S[h->cur];
+    const SurfBufs B{h->surf_pi.p, h->surf_pj.p, h->surf_pt.p, h->surf_gx.p, h->surf_gy.p, h->surf_start.p,
+                     h->surf_ent.p, h->surf_nrm0.p, h->surf_tt.p, h->surf_tau.p, h->surf_flag.p, h->surf_sub.p};
+    LAUNCH(K_SURF, nblk(nt + 1), BLOCK, k_surf_zero, h->surf_cnt.p, (long long)nt + 1);
+    LAUNCH(K_SURF, nblk(nt), BLOCK, k_surf_zero, h->surf_cur.p, (long long)nt);
+    LAUNCH(K_SURF, nblk(nt), BLOCK, k_surf_zero, h->surf_flag.p, (long long)nt);
+    if (np > 0) {
+        LAUNCH(K_SURF, nblk(nt2), BLOCK, k_dump_pairs, S.x.p, h->x0.p, S.hsml.p, h->cell.p, h->kstart.p, S.orig.p,
+               h->off.p, h->proc.p, h->grid.p, h->p.pi, h->surf_pi.p, h->surf_pj.p, h->surf_pt.p, (float *)nullptr,
+               h->surf_gx.p, h->surf_gy.p, nn, nt, nt2, 1, (unsigned long long)np, h->m_prev);
+        LAUNCH(K_SURF, nblk(np), BLOCK, k_surf_count, B, h->surf_cnt.p, np);
+    }
+    rc = scan_u32(h, h->surf_cnt.p, h->surf_start.p, (long long)nt + 1, nullptr);
+    if (rc) return rc;
+    if (np > 0) LAUNCH(K_SURF, nblk(np), BLOCK, k_surf_fill, B, h->surf_cur.p, h->surf_ent.p, np);
+    LAUNCH(K_SURF, nblk(nt), BLOCK, k_surf_sort, h->surf_start.p, h->surf_ent.p, nt);
+    LAUNCH(K_SURF, nblk(nt), BLOCK, k_surf_a, SB, B, nt);
+    if (np > 0) LAUNCH(K_SURF, nblk(np), BLOCK, k_surf_hit, SB, B, np);
+    LAUNCH(K_SURF, nblk(nt), BLOCK, k_surf_sub, SB, B, h->bc_or_not.p, nt);
+    LAUNCH(K_SURF, nblk(nt), BLOCK, k_surf_b, SB, B, h->normal.p, nt);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
 static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
     bool reuse;
     int rc = lists_still_valid(h, reuse);
@@ -886,10 +936,10 @@ static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
     for (int st = 0; st < 4; ++st) {
         const double c1 = f1rk[st] * (dt);
         LAUNCH(K_STAGE_BEGIN, nblk(nt), BLOCK, k_stage_begin, SB, W, c1, adapt, tanfi, coh, nn, nt);
-        if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, nn);
+        if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, h->ifsigman, h->normal.p, nn);
         if (nn > 0)
             LAUNCH_LDS(K_SPU_NODES, nblk8(nn), TILE, LAY_SPU_NODES.bytes, k_spu_nodes, SB, V, PL, adapt, tanfi, coh, nn);
-        if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, nn);
+        if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, h->ifsigman, h->normal.p, nn);
         if (nt > nn)
             LAUNCH_LDS(K_SP_UPDATE, nblk8(nt - nn), TILE, LAY_SP_UPDATE.bytes, k_sp_update, SB, W, V, PL, h->dp, time_sph,
                        f2rk[st], 1, adapt, nn, nt);
@@ -902,19 +952,23 @@ static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
     const bool hook = h->log_cap > 0;
     LAUNCH(K_RK_FINAL, nblk(nt), BLOCK, k_rk_final, SB, W, dt, adapt, tanfi, coh, nn, nt, surrogate_dev(h),
            hook ? 1 : 0);
-    if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, nn);
+    if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, h->ifsigman, h->normal.p, nn);
     if (nn > 0)
         LAUNCH_LDS(K_SPU_NODES, nblk8(nn), TILE, LAY_SPU_NODES.bytes, k_spu_nodes, SB, V, PL, adapt, tanfi, coh, nn);
     if (nt > nn)
         LAUNCH_LDS(K_SP_UPDATE, nblk8(nt - nn), TILE, LAY_SP_UPDATE.bytes, k_sp_update, SB, W, V, PL, h->dp, time_sph, 0.0,
                    0, adapt, nn, nt);
-    if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, nn);
+    if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, h->ifsigman, h->normal.p, nn);
     const bool xsph = h->xsph && h->p.update_x;
     if (xsph && nn > 0)
         LAUNCH_LDS(K_XSPH, nblk8(nn), TILE, LAY_XSPH.bytes, (k_xsph<KIND_NODE>), SB, V, PL, h->xsph_v.p);
     if (xsph && nt > nn)
         LAUNCH_LDS(K_XSPH, nblk8(nt - nn), TILE, LAY_XSPH.bytes, (k_xsph<KIND_SP>), SB, V, PL, h->xsph_v.p);
     LAUNCH(K_POSITION, nblk(nt), BLOCK, k_position, SB, W, dt, h->p.update_x, nn, nt, xsph ? h->xsph_v.p : nullptr);
+    if (h->p.update_x && h->p.no_bcs > 0) {   // 2_...:150-154
+        rc = free_surface(h, SB);
+        if (rc) return rc;
+    }
     if (h->outside && h->p.update_x && nn > 0) {
         const ShiftParams sp{h->p.npoints, h->p.vel_vector, itimestep % h->p.shift_update == 0 ? 1 : 0,
                              h->p.r_x, h->p.r_y, h->p.disp_tol, h->p.dx};
@@ -928,7 +982,14 @@ extern "C" int sps_set_bcs(sps_handle *h, const sps_bcs *b) {
     if (!h || !b || !b->bc_list || !b->bc_info || !b->bc_or_not) return fail(SPS_EINVAL, "null argument");
     if (b->no_bcs != h->p.no_bcs || b->no_bcs < 1)
         return fail(SPS_EINVAL, "no_bcs %d, sps_params says %d", b->no_bcs, h->p.no_bcs);
-    if (b->ifsigman != 0 || b->reserved0 != 0) return fail(SPS_EINVAL, "ifsigman and reserved0 must be 0");
+    if ((b->ifsigman != 0 && b->ifsigman != 1) || b->reserved0 != 0)
+        return fail(SPS_EINVAL, "ifsigman must be 0 or 1, reserved0 0");
+    if (b->ifsigman == 1 && h->nt2 > h->nt)
+        return fail(SPS_EINVAL, "ifsigman = 1 with wall dummies is not supported (apply_stress_free reads bc_int)");
+    if (b->ifsigman == 1 && h->mixed)
+        return fail(SPS_EINVAL, "ifsigman = 1: SPS_MODE_ORDERED only (the mixed form has no free-surface detection)");
+    for (int i = 0; i < h->nt; ++i)
+        if (b->bc_or_not[i] < 0 || b->bc_or_not[i] > 3) return fail(SPS_EINVAL, "bc_or_not[%d] = %d", i, b->bc_or_not[i]);
     if (b->ncurves < 0 || b->ncurves > SPS_MAX_CURVES || (b->ncurves > 0 && (!b->curve_npts || !b->curve_t || !b->curve_f)))
         return fail(SPS_EINVAL, "bad time curves");
     for (int c = 0; c < b->ncurves; ++c)
@@ -954,7 +1015,9 @@ extern "C" int sps_set_bcs(sps_handle *h, const sps_bcs *b) {
         std::memcpy(h->curve_f + SPS_MAX_TCURVE * c, b->curve_f + SPS_MAX_TCURVE * c, SPS_MAX_TCURVE * sizeof(float));
     }
     HIP_TRY(hipMemcpyAsync(h->bc_info.p, b->bc_info, 8 * (size_t)nn * sizeof(int), hipMemcpyHostToDevice, h->stream));
-    HIP_TRY(hipMemcpyAsync(h->bc_or_not.p, b->bc_or_not, (size_t)nn * sizeof(int), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->bc_or_not.p, b->bc_or_not, (size_t)h->nt * sizeof(int), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemsetAsync(h->normal.p, 0, (size_t)h->nt * sizeof(double2), h->stream));
+    h->ifsigman = b->ifsigman;
     HIP_TRY(hipStreamSynchronize(h->stream));
     h->bc_vars = bc_values(h, 0.0);   // the variables (the values are the step's, k_bc_values)
     h->bcs_set = true;
@@ -1067,7 +1130,7 @@ extern "C" int sps_get_pairs(sps_handle *h, int64_t cap, int32_t *pair_i, int32_
     StateSet &S = h->S[h->cur];
     k_dump_pairs<<<nblk(h->nt2), BLOCK, 0, h->stream>>>(S.x.p, h->x0.p, S.hsml.p, h->cell.p, h->kstart.p, S.orig.p,
                                                      h->off.p, h->proc.p, h->grid.p, h->p.pi, di.p, dj.p, dt.p,
-                                                     dw.p, dgx.p, dgy.p, h->nn, h->nt, h->nt2);
+                                                     dw.p, dgx.p, dgy.p, h->nn, h->nt, h->nt2, 0, 0ull, 0ull);
     std::vector<int> hi(np), hj(np), ht(np);
     std::vector<float> hw(np), hgx(np), hgy(np);
     hipError_t e = hipGetLastError();
@@ -1221,4 +1284,19 @@ extern "C" void sps_destroy(sps_handle *h) {
     if (h->pinned) (void)hipHostFree(h->pinned);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
+}
+
+extern "C" int sps_get_surface(sps_handle *h, int32_t *bc_or_not, double *normal) {
+    if (!h) return fail(SPS_EINVAL, "null argument");
+    if (h->p.no_bcs <= 0 || !h->bcs_set) return fail(SPS_ESTATE, "no BCs set: the free surface is not tracked");
+    if (h->mixed) return fail(SPS_EINVAL, "SPS_MODE_FAST does not track the free surface");
+    HIP_TRY(hipSetDevice(h->device));
+    if (bc_or_not)
+        HIP_TRY(hipMemcpyAsync(bc_or_not, h->bc_or_not.p, (size_t)h->nt * sizeof(int), hipMemcpyDeviceToHost,
+                               h->stream));
+    if (normal)
+        HIP_TRY(hipMemcpyAsync(normal, h->normal.p, (size_t)h->nt * sizeof(double2), hipMemcpyDeviceToHost,
+                               h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return 0;
 }

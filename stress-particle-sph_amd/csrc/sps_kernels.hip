@@ -2153,12 +2153,25 @@ __global__ void k_bc_values(BcValues V, double *__restrict__ vals) {
 // Normal_BCs (3_...:1644-1753) on the node slots: a node with BC_or_not = 1
 // takes the values of the BCs listed in bc_info(3:7) (by original index), in
 // that order; vals holds the step's values (k_bc_values), V the variables.
+// ifsigman: apply_stress_free (3_...:1758-1841) on the nodes with BC_or_not
+// 2, which Normal_BCs leaves alone, so one thread does either: the stress
+// keeps sigma_tt of the normal frame.  Its bc_int test always passes: the
+// option is refused with wall dummies, the only source of bc_int = 1.
 __global__ void k_bcs(StateBufs S, const int *__restrict__ bc_info, const int *__restrict__ bc_or_not, BcValues V,
-                      const double *__restrict__ vals, int nn) {
+                      const double *__restrict__ vals, int ifsigman, const double2 *__restrict__ normal, int nn) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nn) return;
     const int i = S.orig[r];
     const int *info = bc_info + 8 * (size_t)i;
+    if (ifsigman && bc_or_not[i] == 2) {
+        const double2 n = normal[i];
+        if (__builtin_isnan(n.x) || __builtin_isnan(n.y)) return;
+        const double s2 = n.y * n.y, c2 = n.x * n.x, sc = n.y * n.x;
+        double4 s = S.stress[r];
+        const double stt = s2 * s.x - 2 * sc * s.z + c2 * s.y;
+        S.stress[r] = make_double4(s2 * stt, c2 * stt, -sc * stt, c2 * stt);
+        return;
+    }
     if (bc_or_not[i] != 1 || info[1] == 0) return;
     double2 v = S.vel[r];
     double4 s = S.stress[r];
@@ -2178,6 +2191,191 @@ __global__ void k_bcs(StateBufs S, const int *__restrict__ bc_info, const int *_
     }
     if (dv) S.vel[r] = v;
     if (ds) S.stress[r] = s;
+}
+
+// ------------------------------------------------------------ free surface
+// get_nodes_on_free_surface (3_...:1116-1411) over the step's pair list, at
+// the updated positions.  k_dump_pairs (as_slots) lays the list out in the
+// reference's traversal order; every particle's pairs of types 1-3 are then
+// gathered in that order (count, scan, fill, sort), so the single-precision
+// sums of steps 1 and 4 add in the reference's order.  Its locals are
+// single precision and normal / subset double, as in the reference; x**0.5
+// is a square root there (the compiler lowers it so) and 2**0.5 the single-
+// precision sqrt(2).
+struct SurfBufs {
+    const int *pi, *pj, *pt;          // per pair, in traversal order: slots and type
+    const float *gx, *gy;             // stored dW/dx, dW/dy of pair_i
+    const unsigned *start, *ent;      // per particle: its entries 2 k + (0: pair_i, 1: pair_j)
+    double2 *nrm0;                    // first normal estimate (step 2)
+    float2 *tt, *tau;                 // scan-region centre and tangent (step 3)
+    unsigned *flag;                   // step 3: a partner in the scan region (f_int = -1)
+    int *sub;                         // subset after 3_...:1341-1349
+};
+
+__global__ void k_surf_zero(unsigned *__restrict__ a, long long n) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) a[t] = 0u;
+}
+
+__device__ __forceinline__ bool surf_kind(int t) { return t == 1 || t == 2 || t == 3; }
+
+__global__ void k_surf_count(SurfBufs B, unsigned *__restrict__ cnt, long long np) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= np || !surf_kind(B.pt[k])) return;
+    atomicAdd(cnt + B.pi[k], 1u);
+    atomicAdd(cnt + B.pj[k], 1u);
+}
+
+__global__ void k_surf_fill(SurfBufs B, unsigned *__restrict__ cur, unsigned *__restrict__ ent, long long np) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= np || !surf_kind(B.pt[k])) return;
+    const int i = B.pi[k], j = B.pj[k];
+    ent[B.start[i] + atomicAdd(cur + i, 1u)] = 2u * (unsigned)k;
+    ent[B.start[j] + atomicAdd(cur + j, 1u)] = 2u * (unsigned)k + 1u;
+}
+
+// each particle's entries into traversal order (insertion sort: a few
+// dozen per particle)
+__global__ void k_surf_sort(const unsigned *__restrict__ start, unsigned *__restrict__ ent, int nt) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nt) return;
+    const unsigned b = start[r], e = start[r + 1];
+    for (unsigned a = b + 1; a < e; ++a) {
+        const unsigned v = ent[a];
+        unsigned c = a;
+        while (c > b && ent[c - 1] > v) {
+            ent[c] = ent[c - 1];
+            --c;
+        }
+        ent[c] = v;
+    }
+}
+
+// m_p / rho_p dW of one entry, as the reference's hf1 / hf2 of that end
+__device__ __forceinline__ void surf_hf(const StateBufs &S, const SurfBufs &B, unsigned e, int &p, float &h1,
+                                        float &h2) {
+    const unsigned k = e >> 1;
+    const bool as_j = e & 1u;
+    p = as_j ? B.pi[k] : B.pj[k];
+    const double m = S.mass[p], rho = S.rho[p];
+    const double g1 = (double)B.gx[k], g2 = (double)B.gy[k];
+    h1 = as_j ? (float)(-(m * g1 / rho)) : (float)(m * g1 / rho);
+    h2 = as_j ? (float)(-(m * g2 / rho)) : (float)(m * g2 / rho);
+}
+
+// steps 1-2 (3_...:1160-1245): renormalisation matrix, first normal, and
+// the scan frame of step 3
+__global__ void k_surf_a(StateBufs S, SurfBufs B, int nt) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nt) return;
+    const double2 xr = S.x[r];
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f, f0 = 0.f, f1 = 0.f;
+    for (unsigned q = B.start[r]; q < B.start[r + 1]; ++q) {
+        int p;
+        float h1, h2;
+        surf_hf(S, B, B.ent[q], p, h1, h2);
+        const double2 xp = S.x[p];
+        const double d0 = xp.x - xr.x, d1 = xp.y - xr.y;
+        a0 = (float)((double)a0 + d0 * (double)h1);
+        f0 = f0 + h1;
+        a1 = (float)((double)a1 + d1 * (double)h1);
+        a2 = (float)((double)a2 + d0 * (double)h2);
+        a3 = (float)((double)a3 + d1 * (double)h2);
+        f1 = f1 + h2;
+    }
+    if (fabsf(a0) <= 1.e-8f) a0 = 0.f;
+    if (fabsf(a1) <= 1.e-8f) a1 = 0.f;
+    if (fabsf(a2) <= 1.e-8f) a2 = 0.f;
+    if (fabsf(a3) <= 1.e-8f) a3 = 0.f;
+    const float v1 = -(a0 * f0 + a1 * f1), v2 = -(a2 * f0 + a3 * f1);
+    const float v3 = sqrtf(v1 * v1 + v2 * v2);
+    const double n0 = (double)(v1 / v3), n1 = (double)(v2 / v3);
+    const double hr = S.hsml[r];
+    B.nrm0[r] = make_double2(n0, n1);
+    B.tt[r] = make_float2((float)(xr.x + hr * n0), (float)(xr.y + hr * n1));
+    B.tau[r] = make_float2((float)(-n1), (float)n0);
+}
+
+// step 3 (3_...:1250-1278): does partner p lie in particle r's scan region
+__device__ __forceinline__ bool surf_hit(const StateBufs &S, const SurfBufs &B, int r, int p) {
+    const double2 xr = S.x[r], xp = S.x[p];
+    const double d0 = xp.x - xr.x, d1 = xp.y - xr.y;
+    const float dist = (float)sqrt(d0 * d0 + d1 * d1);
+    const float2 t = B.tt[r], u = B.tau[r];
+    const double2 n = B.nrm0[r];
+    const float v0 = (float)(xp.x - (double)t.x), v1 = (float)(xp.y - (double)t.y);
+    const float vnorm = sqrtf(v0 * v0 + v1 * v1);
+    const float prod = (float)(fabs(n.x * (double)v0 + n.y * (double)v1) + (double)fabsf(u.x * v0 + u.y * v1));
+    const double hr = S.hsml[r];
+    const float limit = (float)((double)1.41421354f * hr);
+    return (dist >= limit && (double)vnorm < hr) || (dist < limit && (double)prod < hr);
+}
+
+// step 3 per pair: node and stress-point pairs test both ends, dummy pairs
+// their non-dummy end (pair_j); the outcome does not depend on the order
+__global__ void k_surf_hit(StateBufs S, SurfBufs B, long long np) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= np) return;
+    const int t = B.pt[k], i = B.pi[k], j = B.pj[k];
+    if (t == 2 || t == 3) {
+        if (surf_hit(S, B, i, j)) B.flag[i] = 1u;
+        if (surf_hit(S, B, j, i)) B.flag[j] = 1u;
+    } else if (t == 6 || t == 9) {
+        if (surf_hit(S, B, j, i)) B.flag[j] = 1u;
+    }
+}
+
+// 3_...:1321-1349: subsets and BC_or_not (2 on the surface, 0 inside, 1 kept)
+__global__ void k_surf_sub(StateBufs S, SurfBufs B, int *__restrict__ bc_or_not, int nt) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nt) return;
+    const int o = S.orig[r];
+    int sub = B.flag[r] ? 2 : 1, bc = bc_or_not[o];
+    if (bc != 1) bc = 0;
+    if (sub == 2 && bc != 1) bc = 0;
+    else if (sub == 1 && bc != 1) bc = 2;
+    else if (sub == 1 && bc == 1) sub = 2;
+    bc_or_not[o] = bc;
+    B.sub[r] = sub;
+}
+
+// step 4 (3_...:1353-1405): the surface normal from the edges to surface
+// neighbours, oriented against the gradient of f_int; by original index
+__global__ void k_surf_b(StateBufs S, SurfBufs B, double2 *__restrict__ normal, int nt) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nt) return;
+    const bool on = B.sub[r] == 1;
+    const float fr = B.flag[r] ? -1.f : 0.f;
+    double n0 = 0.0, n1 = 0.0;
+    float nb = 0.f, g0 = 0.f, g1 = 0.f;
+    for (unsigned q = B.start[r]; q < B.start[r + 1]; ++q) {
+        const unsigned e = B.ent[q];
+        int p;
+        float h1, h2;
+        surf_hf(S, B, e, p, h1, h2);
+        if (on && B.sub[p] == 1) {
+            const unsigned k = e >> 1;
+            const double2 xi = S.x[B.pi[k]], xj = S.x[B.pj[k]];
+            const float xv = (float)xj.x - (float)xi.x, yv = (float)xj.y - (float)xi.y;
+            n0 = n0 - (double)yv;
+            n1 = n1 + (double)xv;
+            nb = nb + 1.f;
+        }
+        const float df = (B.flag[p] ? -1.f : 0.f) - fr;
+        g0 = g0 + df * h1;
+        g1 = g1 + df * h2;
+    }
+    double2 out = make_double2(0.0, 0.0);
+    if (on) {
+        n0 = n0 / (double)nb;
+        n1 = n1 / (double)nb;
+        const float nv = (float)sqrt(n0 * n0 + n1 * n1);
+        n0 = n0 / (double)nv;
+        n1 = n1 / (double)nv;
+        const float ps = (float)((double)g0 * n0 + (double)g1 * n1);
+        out = ps < 0 ? make_double2(-n0, -n1) : make_double2(n0, n1);
+    }
+    normal[S.orig[r]] = out;
 }
 
 // Slot of each original particle: the outside approach writes a node's
@@ -2275,7 +2473,8 @@ __global__ void k_dump_pairs(const double2 *__restrict__ x, const double2 *__res
                              const Grid *__restrict__ g, double pi, int *__restrict__ pi_out,
                              int *__restrict__ pj_out, int *__restrict__ pt_out,
                              float *__restrict__ w_out, float *__restrict__ gx_out,
-                             float *__restrict__ gy_out, int nn, int nt, int nt2) {
+                             float *__restrict__ gy_out, int nn, int nt, int nt2, int as_slots,
+                             unsigned long long npairs, unsigned long long mprev) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nt2) return;
     int ndivt = g->ndivt;
@@ -2301,19 +2500,23 @@ __global__ void k_dump_pairs(const double2 *__restrict__ x, const double2 *__res
                     if (rr < 0.0) continue;
                     float w, gx, gy;
                     cubic_kernel(rr, dx0, dx1, mh, pi, w, gx, gy);
-                    int a = orig[r] + 1, b = orig[s] + 1;
+                    int a = as_slots ? r : orig[r] + 1, b = as_slots ? s : orig[s] + 1;
                     // Pint_Update: stress point or dummy first; gradient negated on swap
                     bool swap = (kr == KIND_NODE && ks != KIND_NODE) || (kr == KIND_SP && ks == KIND_DUMMY);
                     if (swap) {
                         int t = a; a = b; b = t;
                         gx = -gx; gy = -gy;
                     }
-                    pi_out[d] = a;
-                    pj_out[d] = b;
-                    pt_out[d] = pair_type(kr, ks);
-                    w_out[d] = w;
-                    gx_out[d] = gx;
-                    gy_out[d] = gy;
+                    // as_slots: at the pair's place in the traversal order,
+                    // the discovery indices [n-1 .. M_prev] then [0 .. M_prev-1]
+                    const unsigned long long q =
+                        !as_slots || npairs <= mprev ? d : (d >= mprev ? npairs - 1 - d : npairs - mprev + d);
+                    pi_out[q] = a;
+                    pj_out[q] = b;
+                    pt_out[q] = pair_type(kr, ks);
+                    if (w_out) w_out[q] = w;
+                    gx_out[q] = gx;
+                    gy_out[q] = gy;
                     d++;
                 }
         }

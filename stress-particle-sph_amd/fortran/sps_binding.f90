@@ -10,7 +10,7 @@ module sps_binding
   use, intrinsic :: iso_c_binding
   implicit none
 
-  integer(c_int32_t), parameter :: SPS_ABI_VERSION = 2
+  integer(c_int32_t), parameter :: SPS_ABI_VERSION = 3
   integer, parameter :: SPS_MAX_TCURVE = 64
   integer, parameter :: SPS_MAX_BCS = 32, SPS_MAX_CURVES = 16
   integer(c_int), parameter :: SPS_OK = 0
@@ -73,6 +73,11 @@ module sps_binding
        type(c_ptr), value :: h
        type(sps_bcs_t), intent(in) :: b
      end function sps_set_bcs
+
+     integer(c_int) function sps_get_surface(h, bc_or_not, normal) bind(C, name="sps_get_surface")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h, bc_or_not, normal
+     end function sps_get_surface
 
      integer(c_int) function sps_step(h, nsteps, dt, time_sph, itimestep_sph) bind(C, name="sps_step")
        import :: c_int, c_int32_t, c_double, c_ptr

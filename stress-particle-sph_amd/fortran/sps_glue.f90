@@ -61,6 +61,7 @@ contains
     if (.not. c_associated(handle)) return
     call get_arrays(x, vel, stress, internal_vars, f_drucker, displ, If_Out_Domain, grad_u, &
                     Ddev_strn, f_bound, art_visc, x_10, disp_10, .not. inside_approach)
+    if (no_bcs > 0) call get_surface(BC_or_not, normal)
   end subroutine sps_glue_sync
 
   subroutine sps_glue_finish
@@ -127,8 +128,8 @@ contains
     if (no_bcs > 0) call set_bcs(bc_info, BC_or_not)
   end subroutine open_handle
 
-  ! Prescribed BCs: bc_list as read, the nodes' columns of bc_info and
-  ! BC_or_not, and the time curves repacked one after another
+  ! Prescribed BCs: bc_list as read, the nodes' columns of bc_info,
+  ! BC_or_not of every particle, and the time curves repacked one after another
   ! (ttcurves(ntcurves, mpts) holds a curve's points with stride ntcurves).
   subroutine set_bcs(info_, bcn_)
     integer, target, intent(in) :: info_(*), bcn_(*)
@@ -197,5 +198,13 @@ contains
     end if
     call sps_check(sps_get_state(handle, st), 'sps_get_state')
   end subroutine get_arrays
+
+  ! BC_or_not (ntotal) and normal (2,ntotal) as the last free-surface
+  ! detection left them, for the driver's surface_points writer.
+  subroutine get_surface(bcn_, nrm_)
+    integer, target, intent(inout) :: bcn_(*)
+    real(DP), target, intent(inout) :: nrm_(*)
+    call sps_check(sps_get_surface(handle, c_loc(bcn_), c_loc(nrm_)), 'sps_get_surface')
+  end subroutine get_surface
 
 end module sps_glue

@@ -66,6 +66,10 @@ def main(emu_lib, oracle_lib, case, steps, tol="1e-4"):
     bad = [k for k in a if not np.array_equal(a[k].view(np.uint8), b[k].view(np.uint8))]
     pa, pb = emu.get_pairs(), ora.get_pairs()
     bad += [f"pairs.{k}" for k in pa if not np.array_equal(pa[k].view(np.uint8), pb[k].view(np.uint8))]
+    if deck.no_bcs > 0 and deck.update_x:   # the free-surface detection
+        for k, u, v in zip(("bc_or_not", "normal"), emu.get_surface(), ora.get_surface()):
+            if not np.array_equal(u.view(np.uint8), v.view(np.uint8)):
+                bad.append(k)
     print(f"{case} {steps} steps: " + ("bitwise equal" if not bad else "differ in " + ", ".join(bad)))
     return 1 if bad else 0
 

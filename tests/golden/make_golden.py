@@ -67,11 +67,23 @@ PLAN = {
     "b8_bui_out3": ([1, 2, 10, 100, 500], [1, 2, 100], False, "bui"),
     # the inside approach with four points per cell
     "t5_sp4": ([1, 2, 3, 10, 50, 200], "all", True),
+    # the same test with the traction-free surface (ifsigman 1): free-surface
+    # detection every step, apply_stress_free at every BC call
+    "l2_localisation_free": ([1, 2, 3, 10, 100, 500], [1, 2, 3, 10, 100, 500], False),
 }
 
 STATE_FIELDS = ("x", "vel", "stress", "ivar1", "f_drucker", "ddev_strn", "grad_u", "displ",
                 "f_bound", "art_visc")
 FULL_FIELDS_BIG = ("x", "vel", "stress", "ivar1", "f_drucker", "displ")
+
+
+def read_surface(path):
+    """BC_or_not (ntotal) and normal (ntotal, 2) of a surf_ dump."""
+    with open(path, "rb") as f:
+        n = int(np.fromfile(f, dtype="<i4", count=1)[0])
+        bc = np.fromfile(f, dtype="<i4", count=n)
+        nrm = np.fromfile(f, dtype="<f8", count=2 * n).reshape(n, 2)
+    return bc, nrm
 
 
 def sha(a):
@@ -111,6 +123,13 @@ def run(deck):
                 fx[f"s{st}_sha_{k}"] = np.array(sha(s[k]))
                 if keep_full and (full == "all" or k in FULL_FIELDS_BIG):
                     fx[f"s{st}_{k}"] = s[k]
+            surf = os.path.join(out, f"surf_{st:06d}.bin")
+            if os.path.exists(surf):
+                bc, nrm = read_surface(surf)
+                fx[f"s{st}_sha_bc_or_not"] = np.array(sha(bc))
+                fx[f"s{st}_sha_normal"] = np.array(sha(nrm))
+                if keep_full:
+                    fx[f"s{st}_bc_or_not"], fx[f"s{st}_normal"] = bc, nrm
             p = read_pairs(os.path.join(out, f"pairs_{st:06d}.bin"))
             fx[f"s{st}_npairs"] = np.array(len(p))
             fx[f"s{st}_sha_pairs"] = np.array(hashlib.sha256(pair_records(p)).hexdigest())

@@ -6,6 +6,8 @@ import os
 import re
 import subprocess
 
+import numpy as np
+
 import pytest
 
 from conftest import ORACLE_LIB, ROOT, load_pkg
@@ -131,6 +133,29 @@ def test_bcs_rejections_and_order():
     assert e.value.code == -1
     sim.set_bcs(deck, parts)
     sim.step(1)
+
+
+def test_free_surface_options():
+    """ifsigman must be 0 or 1 (the oracle, which runs here); sps_get_surface
+    needs the BCs set and reports the initial BC_or_not before any step."""
+    sps = load_pkg()
+    oracle = os.path.join(ROOT, "oracle", "libsps_oracle.so")
+    deck = sps.read_deck(os.path.join(ROOT, "stress-particle-sph_amd", "decks", "l2_localisation_free"))
+    parts = sps.build_particles(deck)
+    sim = sps.Simulation(sps.make_params(deck, parts), oracle)
+    sim.set_state(sps.initial_state(parts, sim.niv))
+    with pytest.raises(sps.SpsError) as e:
+        sim.get_surface()
+    assert e.value.code == -5
+    deck.ifsigman = 2
+    with pytest.raises(sps.SpsError) as e:
+        sim.set_bcs(deck, parts)
+    assert e.value.code == -1
+    deck.ifsigman = 1
+    sim.set_bcs(deck, parts)
+    bc, nrm = sim.get_surface()
+    assert np.count_nonzero(bc == 1) > 0 and not nrm.any()
+    sim.close()
 
 
 def outside_params(sps, **change):

@@ -156,9 +156,14 @@ def test_small_decks_vs_reference(deck_name):
     gpu.close()
 
 
+# the reference's BC time adds an uninitialised local (a subnormal there)
+SUBNORMAL = {("l1_localisation", 1), ("l1_localisation", 2), ("l2_localisation_free", 1),
+             ("l2_localisation_free", 2), ("l2_localisation_free", 3)}
+
+
 @pytest.mark.parametrize("deck_name", ["c1_collapse", "b1_bui_sp1", "b2_bui_sp2", "b3_bui_sp3", "b4_bui_outside",
                                        "b5_bui_velvec", "b6_bui_velvec_shift", "v1_vslope_sp1", "v3_vslope_sp3",
-                                       "l1_localisation", "b7_bui_out1", "b8_bui_out3"])
+                                       "l1_localisation", "b7_bui_out1", "b8_bui_out3", "l2_localisation_free"])
 def test_digest_decks_vs_reference(deck_name):
     """GPU state digests against the reference Fortran's dumps: the granular
     column collapse (1000 steps of dt = 1e-5) and the Bui
@@ -179,7 +184,7 @@ def test_digest_decks_vs_reference(deck_name):
         s["ivar1"] = np.ascontiguousarray(s["internal_vars"][:, 0])
         for k in ("x", "vel", "stress", "ivar1", "f_drucker", "ddev_strn", "grad_u", "displ", "f_bound",
                   "art_visc"):
-            if deck_name == "l1_localisation" and st <= 2:
+            if (deck_name, st) in SUBNORMAL:
                 # the reference's BC time adds an uninitialised local (a
                 # subnormal there): its first steps differ below 1e-300 only
                 if f"s{st}_{k}" in g.files:
@@ -215,3 +220,26 @@ def test_scale_bitwise_vs_oracle(deck_name, nsteps):
         compare_states(gpu, ora, f"{deck_name} step {st}")
     gpu.close()
     ora.close()
+
+
+def test_free_surface_vs_reference():
+    """Free-surface detection (3_...:1116-1411) on the GPU: BC_or_not of
+    every particle and the double-precision normals bitwise against the
+    reference's arrays; with ifsigman = 1 apply_stress_free then shapes the
+    node stresses (the digest test of this deck)."""
+    g = golden("l2_localisation_free")
+    steps = sorted(int(k[1:].split("_")[0]) for k in g.files if k.endswith("_sha_x") and k != "s0_sha_x")
+    say(f"[gpu free surface] deck=l2_localisation_free steps={steps}")
+    sps = load_pkg()
+    deck = sps.read_deck(deck_dir("l2_localisation_free"))
+    gpu = sps.Simulation.from_deck(deck)
+    done = 0
+    for st in steps:
+        gpu.step(st - done)
+        done = st
+        bc, nrm = gpu.get_surface()
+        assert np.array_equal(bc, g[f"s{st}_bc_or_not"]), f"step {st} BC_or_not"
+        assert bits_equal(g[f"s{st}_normal"], nrm), f"step {st} normal: {describe_diff(g[f's{st}_normal'], nrm)}"
+    gpu.close()
+    with pytest.raises(sps.SpsError):   # the mixed form has no detection
+        sps.Simulation.from_deck(deck, mode=sps.MODE_FAST)
