@@ -1121,7 +1121,8 @@ static int free_surface(struct sps_handle *h)
     memset(sub, 0, (size_t)nt * sizeof(int32_t));
     /* step 1 (3_...:1160-1225): renormalisation matrix and sum of m/rho grad W
      * over the node-stress point, stress point and node pairs, in list order */
-    for (int64_t k = 0; k < h->n_pairs; ++k) {
+    for (int64_t q = 0; q < h->n_pairs; ++q) {
+        const int64_t k = h->order[q];
         const int t = h->pt[k];
         if (t != 1 && t != 2 && t != 3) continue;
         const int i = h->pi[k], j = h->pj[k];
@@ -1159,7 +1160,8 @@ static int free_surface(struct sps_handle *h)
     }
     /* step 3 (3_...:1247-1319): a particle with a partner in its scan region
      * is inside (subset 2, f_int -1); the rest are on the surface */
-    for (int64_t k = 0; k < h->n_pairs; ++k) {
+    for (int64_t q = 0; q < h->n_pairs; ++q) {
+        const int64_t k = h->order[q];
         const int t = h->pt[k], i = h->pi[k], j = h->pj[k];
         if (t == 2 || t == 3) {
             if (sub[i] == 0 && fs_hit(h, tt, tau, i, j)) { sub[i] = 2; fint[i] = -1; }
@@ -1181,7 +1183,8 @@ static int free_surface(struct sps_handle *h)
     /* step 4 (3_...:1353-1389): the normal from the surface neighbours'
      * edges, oriented against the gradient of f_int */
     memset(nrm, 0, 2 * (size_t)nt * sizeof(double));
-    for (int64_t k = 0; k < h->n_pairs; ++k) {
+    for (int64_t q = 0; q < h->n_pairs; ++q) {
+        const int64_t k = h->order[q];
         const int t = h->pt[k];
         if (t != 1 && t != 2 && t != 3) continue;
         const int i = h->pi[k], j = h->pj[k];
@@ -1238,6 +1241,10 @@ static void xsph_update(struct sps_handle *h, double dt)
         const int t = h->pt[d];
         if (t != 2 && t != 3) continue;
         const int i = h->pi[d], j = h->pj[d];
+        if (t == 3) {   /* 2_...:224-230: nodes next to the free surface */
+            if (h->bc_or_not[i] == 2 && h->bc_or_not[j] != 2) h->bc_or_not[j] = 3;
+            if (h->bc_or_not[j] == 2 && h->bc_or_not[i] != 2) h->bc_or_not[i] = 3;
+        }
         const double w = (double)h->pw[d];
         for (int a = 0; a < 2; ++a) {
             const double vi = h->vel[2 * i + a], vj = h->vel[2 * j + a];
@@ -1404,9 +1411,6 @@ static int check_params(const sps_params *p)
     if (p->cont_density || p->art_stress)
         return fail(SPS_EINVAL, "continuity density and artificial stress are not supported");
     if (p->no_bcs < 0 || p->no_bcs > SPS_MAX_BCS) return fail(SPS_EINVAL, "no_bcs must be 0..%d", SPS_MAX_BCS);
-    if (p->no_bcs > 0 && p->update_x && p->xsph)
-        return fail(SPS_EINVAL, "prescribed BCs do not run with XSPH and the position update together "
-                                "(XSPH_update then re-marks BC nodes next to the free surface)");
     int ntype_eco = (int)p->props[0], ncrit = (int)p->props[1];
     if (ntype_eco > 1 && ncrit == 5) return fail(SPS_EINVAL, "the Modified Cam Clay model (ncrit 5) is not supported");
     if (ntype_eco > 1 && ncrit <= 4 && (ncrit < 1 || !(p->props[6] > (double)0.001f)))

@@ -336,9 +336,10 @@ static int check_params(const sps_params *p) {
     if (p->cont_density || p->art_stress)
         return fail(SPS_EINVAL, "continuity density and artificial stress are not supported");
     if (p->no_bcs < 0 || p->no_bcs > SPS_MAX_BCS) return fail(SPS_EINVAL, "no_bcs must be 0..%d", SPS_MAX_BCS);
-    if (p->no_bcs > 0 && p->update_x && p->xsph)
-        return fail(SPS_EINVAL, "prescribed BCs do not run with XSPH and the position update together "
-                                "(XSPH_update then re-marks BC nodes next to the free surface)");
+    if (p->no_bcs > 0 && p->update_x && p->xsph && p->mode == SPS_MODE_FAST)
+        return fail(SPS_EINVAL, "prescribed BCs with XSPH and the position update: SPS_MODE_ORDERED only "
+                                "(XSPH_update re-marks BC nodes next to the free surface, which the mixed "
+                                "form does not detect)");
     int ntype_eco = (int)p->props[0], ncrit = (int)p->props[1];
     if (ntype_eco > 1 && ncrit == 5) return fail(SPS_EINVAL, "the Modified Cam Clay model (ncrit 5) is not supported");
     if (ntype_eco > 1 && ncrit <= 4 && (ncrit < 1 || !(p->props[6] > (double)0.001f)))
@@ -872,6 +873,8 @@ static int free_surface(sps_handle *h, const StateBufs &SB) {
     LAUNCH(K_SURF, nblk(nt), BLOCK, k_surf_sort, h->surf_start.p, h->surf_ent.p, nt);
     LAUNCH(K_SURF, nblk(nt), BLOCK, k_surf_a, SB, B, nt);
     if (np > 0) LAUNCH(K_SURF, nblk(np), BLOCK, k_surf_hit, SB, B, np);
+    if (h->xsph && np > 0)   // XSPH_update's marks (2_...:224-230) before the detection
+        LAUNCH(K_SURF, nblk(np), BLOCK, k_surf_mark, SB, B, h->bc_or_not.p, np);
     LAUNCH(K_SURF, nblk(nt), BLOCK, k_surf_sub, SB, B, h->bc_or_not.p, nt);
     LAUNCH(K_SURF, nblk(nt), BLOCK, k_surf_b, SB, B, h->normal.p, nt);
     HIP_TRY(hipGetLastError());

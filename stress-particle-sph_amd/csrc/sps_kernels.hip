@@ -2325,6 +2325,19 @@ __global__ void k_surf_hit(StateBufs S, SurfBufs B, long long np) {
     }
 }
 
+// XSPH_update's bookkeeping (2_...:224-230): a node paired (type 3) with a
+// free-surface node becomes 3, so the detection that follows resets even a
+// prescribed node to 0.  Concurrent marks cannot change any test's outcome:
+// they only turn a value other than 2 into 3.
+__global__ void k_surf_mark(StateBufs S, SurfBufs B, int *__restrict__ bc_or_not, long long np) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= np || B.pt[k] != 3) return;
+    const int oi = S.orig[B.pi[k]], oj = S.orig[B.pj[k]];
+    const int bi = bc_or_not[oi], bj = bc_or_not[oj];
+    if (bi == 2 && bj != 2) bc_or_not[oj] = 3;
+    if (bj == 2 && bi != 2) bc_or_not[oi] = 3;
+}
+
 // 3_...:1321-1349: subsets and BC_or_not (2 on the surface, 0 inside, 1 kept)
 __global__ void k_surf_sub(StateBufs S, SurfBufs B, int *__restrict__ bc_or_not, int nt) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;

@@ -70,6 +70,8 @@ PLAN = {
     # the same test with the traction-free surface (ifsigman 1): free-surface
     # detection every step, apply_stress_free at every BC call
     "l2_localisation_free": ([1, 2, 3, 10, 100, 500], [1, 2, 3, 10, 100, 500], False),
+    # and with XSPH: its marks next to the surface drop prescribed nodes
+    "l3_localisation_xsph": ([1, 2, 3, 10, 100, 300], [1, 2, 3, 10, 100, 300], False),
 }
 
 STATE_FIELDS = ("x", "vel", "stress", "ivar1", "f_drucker", "ddev_strn", "grad_u", "displ",

@@ -44,7 +44,7 @@ EXACT_GROUPS = {
              ("t4_xsph", 20), ("p1_vm", 20), ("p4_mc", 20)],
     "bui": [("b3_bui_sp3", 8), ("b4_bui_outside", 8), ("b6_bui_velvec_shift:kick", 8)],
     "examples": [("c1_collapse", 2), ("v3_vslope_sp3", 10), ("l1_localisation", 6), ("l1_localisation:graph", 6)],
-    "surface": [("l2_localisation_free", 4)],
+    "surface": [("l2_localisation_free", 4), ("l3_localisation_xsph", 4)],
 }
 
 

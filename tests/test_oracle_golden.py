@@ -82,14 +82,18 @@ DIGEST = {"c1_collapse": "BASELINE config 1 (5,151 + 5,000 particles, 1000 steps
           "l1_localisation": "biaxial strain localisation: Perzyna von Mises, BCs with the position update",
           "b7_bui_out1": "outside approach, one point per node, 500 steps",
           "b8_bui_out3": "outside approach, three points per node, 500 steps",
-          "l2_localisation_free": "the localisation test with the traction-free surface (ifsigman 1), 500 steps"}
+          "l2_localisation_free": "the localisation test with the traction-free surface (ifsigman 1), 500 steps",
+          "l3_localisation_xsph": "the same with XSPH (its marks next to the surface), 300 steps"}
 # (deck, step) where the reference reads an uninitialised local: its BCs
 # take t = time_sph + ic_time*dt_sph with ic_time never set (3_...:1622-1627;
 # the compiled reference finds about 6.95e-310 there, a stack address), so
 # at step 1 its top velocity is 1.4e-311 where the restatement (ic_time = 0)
-# has 0; the subnormal differences are gone by step 10 (bitwise from there)
+# has 0; the subnormal differences are gone by step 10 (bitwise from there);
+# with XSPH (l3) a node at coordinate 0 also moves by a subnormal there, so
+# the reference's own pair order at those steps depends on the garbage value
 SUBNORMAL = {("l1_localisation", 1), ("l1_localisation", 2), ("l2_localisation_free", 1),
-             ("l2_localisation_free", 2), ("l2_localisation_free", 3)}
+             ("l2_localisation_free", 2), ("l2_localisation_free", 3), ("l3_localisation_xsph", 1),
+             ("l3_localisation_xsph", 2), ("l3_localisation_xsph", 3)}
 
 
 def fixture_steps(g):
@@ -140,17 +144,23 @@ def test_oracle_digest_decks(deck_name, oracle_lib):
                 pytest.fail(msg)
         p = sim.get_pairs()
         assert len(p["i"]) == int(g[f"s{st}_npairs"])
+        if (deck_name, st) in SUBNORMAL:
+            # a node at coordinate 0 sits at +-1e-321 there, by the garbage
+            # value, and may fall in either cell: the reference's own list
+            # order at these steps changes with its command line
+            continue
         assert pair_digest(p) == str(g[f"s{st}_sha_pairs"]), f"step {st} pairs"
     sim.close()
 
 
-def test_oracle_free_surface_matches_reference(oracle_lib):
+@pytest.mark.parametrize("deck_name", ["l2_localisation_free", "l3_localisation_xsph"])
+def test_oracle_free_surface_matches_reference(deck_name, oracle_lib):
     """get_nodes_on_free_surface (3_...:1116-1411): BC_or_not of every
     particle and the double-precision normals, bitwise against the
     reference's own arrays after each dumped step; with ifsigman = 1
     apply_stress_free then acts on the 2s (the stresses above)."""
-    g = golden("l2_localisation_free")
-    sim = _sim("l2_localisation_free", oracle_lib)
+    g = golden(deck_name)
+    sim = _sim(deck_name, oracle_lib)
     done = 0
     for st in fixture_steps(g):
         sim.step(st - done)
