@@ -218,10 +218,10 @@ int sps_set_materials(sps_handle *h, int32_t nmat, const double *table, const in
  * (2_...:788) and after the step's stress_point_update (2_...:134).  Replaces
  * the globals bc_list, bc_info, BC_or_not and the time curves
  * (3_...:160-260, 1030-1113).  Required before sps_step when
- * sps_params.no_bcs > 0, which sps_create accepts in SPS_MODE_ORDERED, not
- * with XSPH and update_x together (XSPH_update marks nodes next to the free
+ * sps_params.no_bcs > 0 (with XSPH and update_x together in
+ * SPS_MODE_ORDERED only: XSPH_update marks the nodes next to the free
  * surface 3, 2_...:224-230, and the detection then resets even a prescribed
- * node to 0).  With update_x, every step runs the free-surface detection
+ * one to 0).  With update_x, every step runs the free-surface detection
  * (get_nodes_on_free_surface, 3_...:1116-1411) after the position update:
  * BC_or_not becomes 2 on the surface and 0 inside (1 stays 1), with the
  * surface normals; ifsigman = 1 then applies apply_stress_free
