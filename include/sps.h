@@ -46,7 +46,10 @@ enum {
     SPS_EDEVICE = -3,     /* HIP runtime error */
     SPS_ECAPACITY = -4,   /* a grid or pair capacity was exceeded */
     SPS_ESTATE = -5,      /* call out of order (e.g. step before set_state) */
-    SPS_ENUMERIC = -6     /* non-finite value detected in the state */
+    SPS_ENUMERIC = -6     /* a particle in the domain box had a non-finite
+                             position, velocity or stress at the end of a
+                             step (checked once per sps_step call; the HIP
+                             library then needs sps_set_state again) */
 };
 
 /* Arithmetic of the pair loops.  SPS_MODE_FAST has no counterpart in the

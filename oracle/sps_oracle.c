@@ -1516,18 +1516,38 @@ int sps_set_bcs(sps_handle *h, const sps_bcs *b)
     return 0;
 }
 
+/* The watchdog (no counterpart in the reference, which runs on with NaNs):
+ * whether every particle in the domain box has a finite position, velocity
+ * and stress.  Particles that left the box interact with nothing. */
+static int state_finite(const struct sps_handle *h)
+{
+    for (int i = 0; i < h->nt; ++i) {
+        if (h->ood[i]) continue;
+        for (int a = 0; a < 2; ++a)
+            if (!isfinite(h->x[2 * i + a]) || !isfinite(h->vel[2 * i + a])) return 0;
+        for (int c = 0; c < 4; ++c)
+            if (!isfinite(h->stress[4 * i + c])) return 0;
+    }
+    return 1;
+}
+
 int sps_step(sps_handle *h, int32_t nsteps, double dt, double *time_sph, int32_t *itimestep_sph)
 {
     if (!h || !time_sph || !itimestep_sph) return fail(SPS_EINVAL, "null argument");
     if (!h->have_state) return fail(SPS_ESTATE, "sps_step before sps_set_state");
     if (h->p.no_bcs > 0 && !h->bcs_set) return fail(SPS_ESTATE, "no_bcs > 0: sps_set_bcs before sps_step");
+    int finite = 1;
     for (int32_t s = 0; s < nsteps; ++s) {
         *itimestep_sph += 1;
         int rc = time_integration(h, *itimestep_sph, *time_sph, dt);
         if (rc) return rc;
         *time_sph = *time_sph + dt;
         h->steps++;
+        finite = finite && state_finite(h);
     }
+    if (!finite)
+        return fail(SPS_ENUMERIC, "non-finite position, velocity or stress of a particle in the domain box "
+                    "after step %d (checked at the end of each sps_step call)", *itimestep_sph);
     return 0;
 }
 

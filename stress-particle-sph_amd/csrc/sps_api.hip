@@ -598,6 +598,7 @@ extern "C" int sps_set_state(sps_handle *h, const sps_state *st) {
     HIP_TRY(hipMemcpyAsync(S.horiz.p, hz.data(), nt2 * sizeof(float), hipMemcpyHostToDevice, s));
     k_scatter_in_identity<<<nblk(nt2), BLOCK, 0, s>>>(S.orig.p, nt2);
     HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemsetAsync(h->err.p, 0, sizeof(int), s));   // a new state starts with a clear flag
     HIP_TRY(hipStreamSynchronize(s));
     h->have_state = true;
     h->lists_valid = false;
@@ -763,7 +764,7 @@ static int mixed_step_kernels(sps_handle *h, double dt, double time_sph, double 
     const bool hook = h->mlp || h->log_cap > 0;
     if (h->mlp) LAUNCH(K_SURROGATE, 1, 64, k_zero_u32, h->surr_count.p, 1);
     LAUNCH(K_RK_FINAL, nblk(nt), BLOCK, k_rk_final, SB, W, dt, adapt, tanfi, coh, nn, nt, surrogate_dev(h),
-           hook ? 1 : 0);
+           hook ? 1 : 0, h->err.p);
     if (h->mlp && adapt) {
         int rc = surrogate_correct(h, SB);
         if (rc) return rc;
@@ -782,7 +783,7 @@ static int mixed_step_kernels(sps_handle *h, double dt, double time_sph, double 
     if (xsph && nt > nn)
         LAUNCH_LDS(K_XSPH, nblk8(nt - nn), TILE, LAY_XSPH_MIX.bytes, (k_xsph_mix<KIND_SP>), SB, V, PL, hu, h->xsph_v.p);
     LAUNCH(K_POSITION, nblk(nt), BLOCK, k_position, SB, W, dt, h->p.update_x, nn, nt,
-           xsph ? (const double2 *)h->xsph_v.p : (const double2 *)nullptr);
+           xsph ? (const double2 *)h->xsph_v.p : (const double2 *)nullptr, h->err.p);
     if (shift >= 0 && h->p.update_x && nn > 0) {
         const ShiftParams sp{h->p.npoints, h->p.vel_vector, shift, h->p.r_x, h->p.r_y, h->p.disp_tol, h->p.dx};
         LAUNCH(K_SHIFT, nblk(nn), BLOCK, k_shift_sp, SB, h->inv.p, h->nodeinfo.p, h->x10.p, h->disp10.p, sp, nn);
@@ -954,7 +955,7 @@ static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
     }
     const bool hook = h->log_cap > 0;
     LAUNCH(K_RK_FINAL, nblk(nt), BLOCK, k_rk_final, SB, W, dt, adapt, tanfi, coh, nn, nt, surrogate_dev(h),
-           hook ? 1 : 0);
+           hook ? 1 : 0, h->err.p);
     if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, h->ifsigman, h->normal.p, nn);
     if (nn > 0)
         LAUNCH_LDS(K_SPU_NODES, nblk8(nn), TILE, LAY_SPU_NODES.bytes, k_spu_nodes, SB, V, PL, adapt, tanfi, coh, nn);
@@ -967,7 +968,8 @@ static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
         LAUNCH_LDS(K_XSPH, nblk8(nn), TILE, LAY_XSPH.bytes, (k_xsph<KIND_NODE>), SB, V, PL, h->xsph_v.p);
     if (xsph && nt > nn)
         LAUNCH_LDS(K_XSPH, nblk8(nt - nn), TILE, LAY_XSPH.bytes, (k_xsph<KIND_SP>), SB, V, PL, h->xsph_v.p);
-    LAUNCH(K_POSITION, nblk(nt), BLOCK, k_position, SB, W, dt, h->p.update_x, nn, nt, xsph ? h->xsph_v.p : nullptr);
+    LAUNCH(K_POSITION, nblk(nt), BLOCK, k_position, SB, W, dt, h->p.update_x, nn, nt, xsph ? h->xsph_v.p : nullptr,
+           h->err.p);
     if (h->p.update_x && h->p.no_bcs > 0) {   // 2_...:150-154
         rc = free_surface(h, SB);
         if (rc) return rc;
@@ -1046,10 +1048,16 @@ extern "C" int sps_step(sps_handle *h, int32_t nsteps, double dt, double *time_s
     HIP_TRY(hipMemcpyAsync(h->pinned, h->err.p, sizeof(int), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     prof_collect(h);
-    if (*(int *)h->pinned) {
+    const int flag = *(int *)h->pinned;
+    if (flag & ERR_NONFINITE) {
+        h->poisoned = true;
+        return fail(SPS_ENUMERIC, "non-finite position, velocity or stress of a particle in the domain box "
+                    "after step %d (checked at the end of each sps_step call)", *itimestep_sph);
+    }
+    if (flag) {
         h->poisoned = true;
         return fail(SPS_ECAPACITY, "device capacity flag %d: more than %d wall-force terms on one node (1) "
-                    "or more than %d particles of one kind in one cell (2)", *(int *)h->pinned, MAX_WALL_TERMS,
+                    "or more than %d particles of one kind in one cell (2)", flag, MAX_WALL_TERMS,
                     MAX_BUCKET_SORT);
     }
     return 0;

@@ -31,6 +31,16 @@ namespace sps {
 
 constexpr int BLOCK = 256;
 
+// Bits of the device error flag (read back at the end of sps_step).
+constexpr int ERR_WALL_TERMS = 1;   // more wall-force terms on one node than MAX_WALL_TERMS
+constexpr int ERR_BUCKET = 2;       // more particles of one kind in one cell than MAX_BUCKET_SORT
+constexpr int ERR_NONFINITE = 4;    // a non-finite position, velocity or stress in the domain box
+
+__device__ __forceinline__ bool finite2(double2 v) { return isfinite(v.x) && isfinite(v.y); }
+__device__ __forceinline__ bool finite4(double4 v) {
+    return isfinite(v.x) && isfinite(v.y) && isfinite(v.z) && isfinite(v.w);
+}
+
 // Grid of the current search (device-resident).
 struct Grid {
     double xmin[2];
@@ -224,7 +234,7 @@ __global__ void k_cell_sort(const int *__restrict__ kstart, int *__restrict__ tm
     int lo = kstart[b], hi = kstart[b + 1];
     if (hi - lo < 2) return;
     if (hi - lo > MAX_BUCKET_SORT) {
-        if (b % (ndivt + 1) != ndivt) atomicOr(err, 2);   // an in-domain cell
+        if (b % (ndivt + 1) != ndivt) atomicOr(err, ERR_BUCKET);   // an in-domain cell
         return;
     }
     for (int a = lo + 1; a < hi; ++a) {
@@ -1746,7 +1756,7 @@ __device__ __forceinline__ double2 wall_force(const StateBufs &S, const SearchVi
             }
         if (best == 0x7fffffff) return fb;
         if (it == MAX_WALL_TERMS) {
-            atomicOr(err, 1);
+            atomicOr(err, ERR_WALL_TERMS);
             return fb;
         }
         fb.x = fb.x + (double)b0;
@@ -2069,19 +2079,22 @@ __global__ void k_dummy_stress(StateBufs S, SearchView V, int nt, int nt2) {
 // End of RK4 (2_...:784-799) and update_strain (3_...:1847-1863).
 // hook: record the stress points' plastic trial states (SurrogateDev).
 __global__ void k_rk_final(StateBufs S, Work W, double dt, int adapt, double tanfi, double coh,
-                           int nn, int nt, SurrogateDev G, int hook) {
+                           int nn, int nt, SurrogateDev G, int hook, int *__restrict__ err) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nt) return;
     double c = dt / 6;
     material_tc(S, i, tanfi, coh);
+    const bool watched = S.ood[i] == 0;
     if (i < nn) {
         double2 v0 = W.vel0[i], rk = W.rk_vel[i];
-        S.vel[i] = make_double2(v0.x + c * rk.x, v0.y + c * rk.y);
+        const double2 v = make_double2(v0.x + c * rk.x, v0.y + c * rk.y);
+        S.vel[i] = v;
+        double4 s = S.stress[i];
         if (adapt) {
-            double4 s = S.stress[i];
             adapt_stress(s, tanfi, coh);
             S.stress[i] = s;
         }
+        if (watched && !(finite4(s) && finite2(v))) atomicOr(err, ERR_NONFINITE);
         W.ddev[i] = W.rk_dev[i] / 6;
     } else {
         double4 s0 = W.stress0[i], rk = W.rk_stress[i];
@@ -2095,6 +2108,7 @@ __global__ void k_rk_final(StateBufs S, Work W, double dt, int adapt, double tan
             adapt_stress(s, tanfi, coh);
         }
         S.stress[i] = s;
+        if (watched && !finite4(s)) atomicOr(err, ERR_NONFINITE);
         double dd = W.rk_dev[i] / 6;
         W.ddev[i] = dd;
         S.ivar1[i] = S.ivar1[i] + dt * dd;
@@ -2106,14 +2120,15 @@ __global__ void k_rk_final(StateBufs S, Work W, double dt, int adapt, double tan
 // xv (XSPH): the XSPH sums of k_xsph; the positions then follow
 // XSPH_update (2_...:234-235), x = x0 + dt (v + eps sum), eps = 0.5.
 __global__ void k_position(StateBufs S, Work W, double dt, int update_x, int nn, int nt,
-                           const double2 *__restrict__ xv) {
+                           const double2 *__restrict__ xv, int *__restrict__ err) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nt) return;
     double2 v0 = W.vx0[i], v = S.vel[i];
+    double2 xn = make_double2(0.0, 0.0);
     if (update_x && xv) {
         const double eps = 0.5;
         const double2 x0 = W.x0[i], s = xv[i];
-        const double2 xn = make_double2(x0.x + dt * (v.x + eps * s.x), x0.y + dt * (v.y + eps * s.y));
+        xn = make_double2(x0.x + dt * (v.x + eps * s.x), x0.y + dt * (v.y + eps * s.y));
         S.x[i] = xn;
         if (i < nn) {
             double2 x00 = S.x00[i];
@@ -2123,7 +2138,7 @@ __global__ void k_position(StateBufs S, Work W, double dt, int update_x, int nn,
         float hx = (float)(0.5 * (v0.x + v.x));
         float hy = (float)(0.5 * (v0.y + v.y));
         double2 x0 = W.x0[i];
-        double2 xn = make_double2(x0.x + (double)hx * dt, x0.y + (double)hy * dt);
+        xn = make_double2(x0.x + (double)hx * dt, x0.y + (double)hy * dt);
         S.x[i] = xn;
         if (i < nn) {
             double2 x00 = S.x00[i];
@@ -2133,6 +2148,9 @@ __global__ void k_position(StateBufs S, Work W, double dt, int update_x, int nn,
         double2 d = S.displ[i];
         S.displ[i] = make_double2(d.x + 0.5 * (v0.x + v.x) * dt, d.y + 0.5 * (v0.y + v.y) * dt);
     }
+    // the watchdog: the step's velocities and positions of the particles in
+    // the domain box (one that has left it interacts with nothing)
+    if (S.ood[i] == 0 && !(finite2(v) && finite2(xn))) atomicOr(err, ERR_NONFINITE);
 }
 
 // Prescribed values of one step: BC row b's value and its variable (5 vx,

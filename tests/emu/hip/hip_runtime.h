@@ -24,6 +24,7 @@
 
 using std::min;
 using std::max;
+using std::isfinite;
 
 struct dim3e { unsigned x = 0, y = 0, z = 0; };
 extern dim3e threadIdx, blockIdx, blockDim, gridDim;

@@ -43,6 +43,21 @@ def main(emu_lib, oracle_lib, case, steps, tol="1e-4"):
         print(f"{case} {steps} steps ({sims[1].search_count()} searches): " +
               ("graph replay bitwise equal to direct launches" if not bad else "differ in " + ", ".join(bad)))
         return 1 if bad else 0
+    elif variant == "nan":      # the watchdog: a stress point's stress set to NaN
+        codes = []
+        for lib in (emu_lib, oracle_lib):
+            sim = sps.Simulation.from_deck(deck, lib_path=lib)
+            st = sps.initial_state(sim.parts, sim.niv)
+            st["stress"][sim.parts.nnode + 7, 1] = np.nan
+            sim.set_state(st)
+            try:
+                sim.step(steps)
+                codes.append(0)
+            except sps.SpsError as e:
+                codes.append(e.code)
+        ok = codes == [-6, -6]
+        print(f"{case} {steps} steps: return codes {codes}" + (" (both SPS_ENUMERIC)" if ok else " MISMATCH"))
+        return 0 if ok else 1
     elif variant in ("mixed", "skin"):   # the mixed-precision form against the double-precision oracle
         emu = sps.Simulation.from_deck(deck, lib_path=emu_lib, mode=sps.MODE_FAST)
         if variant == "skin":           # lists kept over steps

@@ -41,7 +41,7 @@ def emu_lib():
 # AddressSanitizer build once
 EXACT_GROUPS = {
     "tiny": [("t1_tiny", 20), ("t2_cspm", 20), ("t3_elastic", 20), ("t2_cspm:hvar", 20), ("t2_cspm:graph", 20),
-             ("t4_xsph", 20), ("p1_vm", 20), ("p4_mc", 20)],
+             ("t4_xsph", 20), ("p1_vm", 20), ("p4_mc", 20), ("t2_cspm:nan", 2)],
     "bui": [("b3_bui_sp3", 8), ("b4_bui_outside", 8), ("b6_bui_velvec_shift:kick", 8)],
     "examples": [("c1_collapse", 2), ("v3_vslope_sp3", 10), ("l1_localisation", 6), ("l1_localisation:graph", 6)],
     "surface": [("l2_localisation_free", 4), ("l3_localisation_xsph", 4)],
