@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define SPS_ABI_VERSION 3
+#define SPS_ABI_VERSION 4
 
 enum {
     SPS_OK = 0,
@@ -170,6 +170,9 @@ typedef struct sps_stats {
     int64_t n_by_type[10];  /* pint_type histogram 0..9 (3_...:1598-1608) */
     int32_t ndivx, ndivy;   /* cell grid of the last search (2_...:1250) */
     int64_t steps;          /* steps taken by this handle */
+    int64_t pairs_total;    /* pairs found by all searches of this handle (ABI 4;
+                               no counterpart in the reference: for throughput
+                               over many steps in one sps_step call) */
 } sps_stats;
 
 typedef struct sps_handle sps_handle;

@@ -71,7 +71,7 @@ struct sps_handle {
     /* start-of-step copies (2_...:114) */
     double *x0, *vx0;
     /* pair list in discovery order + traversal order */
-    int64_t pcap, n_pairs, m_pairs;
+    int64_t pcap, n_pairs, m_pairs, pairs_total;
     int32_t *pi, *pj, *pt;
     float *pw, *pgx, *pgy;
     int64_t *order;
@@ -1347,6 +1347,7 @@ static int time_integration(struct sps_handle *h, int itimestep, double time_sph
     const int ncrit = (int)h->p.props[1];
     check_out_domain(h);
     int rc = grid_find(h, itimestep);
+    h->pairs_total += h->n_pairs;
     if (rc) return rc;
     pint_update(h);
     if (!h->p.inside_approach && h->p.sph_shift && itimestep > 1 && (itimestep - 1) % h->p.shift_update == 0) {
@@ -1363,7 +1364,7 @@ static int time_integration(struct sps_handle *h, int itimestep, double time_sph
     normal_bcs(h, time_sph);
     if (h->p.update_x && h->p.xsph) {
         xsph_update(h, dt);
-        if (h->p.no_bcs > 0 && (rc = free_surface(h))) return rc;
+        if ((rc = free_surface(h))) return rc;   /* 2_...:150-154, every step with update_x */
         if (!h->p.inside_approach) shift_stress_points(h, itimestep);
         for (int i = 0; i < nn; ++i)
             for (int a = 0; a < 2; ++a) h->displ[2 * i + a] = h->x[2 * i + a] - h->x00[2 * i + a];
@@ -1373,7 +1374,7 @@ static int time_integration(struct sps_handle *h, int itimestep, double time_sph
                 float vel_half = (float)(0.5 * (h->vx0[2 * i + a] + h->vel[2 * i + a]));
                 h->x[2 * i + a] = h->x0[2 * i + a] + (double)vel_half * dt;
             }
-        if (h->p.no_bcs > 0 && (rc = free_surface(h))) return rc;
+        if ((rc = free_surface(h))) return rc;   /* 2_...:150-154, every step with update_x */
         if (!h->p.inside_approach) shift_stress_points(h, itimestep);
         for (int i = 0; i < nn; ++i)
             for (int a = 0; a < 2; ++a) h->displ[2 * i + a] = h->x[2 * i + a] - h->x00[2 * i + a];
@@ -1600,6 +1601,7 @@ int sps_get_stats(sps_handle *h, sps_stats *out)
     out->ndivx = h->ndivx;
     out->ndivy = h->ndivy;
     out->steps = h->steps;
+    out->pairs_total = h->pairs_total;
     return 0;
 }
 
@@ -1740,7 +1742,9 @@ void sps_destroy(sps_handle *h)
 int sps_get_surface(sps_handle *h, int32_t *bc_or_not, double *normal)
 {
     if (!h) return fail(SPS_EINVAL, "null argument");
-    if (h->p.no_bcs <= 0 || !h->bcs_set) return fail(SPS_ESTATE, "no BCs set: the free surface is not tracked");
+    if (h->p.no_bcs > 0 && !h->bcs_set) return fail(SPS_ESTATE, "no_bcs > 0: sps_set_bcs before sps_get_surface");
+    if (h->p.no_bcs <= 0 && !h->p.update_x)
+        return fail(SPS_ESTATE, "the free surface is tracked with BCs or the position update only");
     if (bc_or_not) memcpy(bc_or_not, h->bc_or_not, (size_t)h->nt * sizeof(int32_t));
     if (normal) memcpy(normal, h->normal, 2 * (size_t)h->nt * sizeof(double));
     return 0;

@@ -1,0 +1,101 @@
+"""Wall-clock timing of the reference as written (CPU baseline cross-check).
+
+Runs oracle/_ref/code/sps_ref_dump -- the reference's own Fortran sources
+compiled by `make -C oracle ref` (flang -O2 -ffp-contract=off, one thread) --
+on BASELINE config 1 (decks/c1_collapse) and on config 2's slope at coarse
+lattice spacings (the c2_slope_dx02 deck with dx 0.2, 0.1 and 0.05 m and its
+walls moved with dx), and the oracle (the C restatement, one thread) on the
+same decks.  Each run dumps state only at its last step; the time reported
+is the wall time of the run minus the same run with 0 steps (start-up and
+particle setup), divided by the step count.  Test infrastructure: it reads
+the reference build, so it runs only in the build container.
+
+Usage: python oracle/time_reference.py [--steps-c1 N] [--quick]
+"""
+import argparse
+import importlib
+import os
+import re
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+REF_BIN = os.path.join(HERE, "_ref", "code", "sps_ref_dump")
+DECKS = os.path.join(ROOT, "stress-particle-sph_amd", "decks")
+
+
+def coarse_slope(dst, dx):
+    """The c2 slope deck with lattice spacing dx (walls at -dx, the bottom
+    wall from -3 dx), in directory dst."""
+    src = os.path.join(DECKS, "c2_slope_dx02")
+    os.makedirs(dst, exist_ok=True)
+    for f in os.listdir(src):
+        shutil.copy(os.path.join(src, f), dst)
+    pts = os.path.join(dst, "c2_slope_dx02.pts")
+    text = open(pts).read()
+    text = text.replace("0.0 10  0.0 10  0.02", f"0.0 10  0.0 10  {dx:g}")
+    text = text.replace("0  0  10  10   0.02", f"0  0  10  10   {dx:g}")
+    text = text.replace("1      -0.02   -0.02   10.5", f"1      {-dx:g}   {-dx:g}   10.5")
+    text = text.replace("2      -0.02   -0.06   30", f"2      {-dx:g}   {-3 * dx:g}   30")
+    open(pts, "w").write(text)
+    return dst
+
+
+def run_ref(deck_dir, steps):
+    work = tempfile.mkdtemp(prefix="tref_")
+    try:
+        for f in os.listdir(deck_dir):
+            shutil.copy(os.path.join(deck_dir, f), work)
+        out = os.path.join(work, "out")
+        os.makedirs(out)
+        last = max(steps, 1)
+        cmd = f"ulimit -s unlimited && {REF_BIN} {out} {steps} {last}"
+        t0 = time.perf_counter()
+        subprocess.run(["bash", "-c", cmd], cwd=work, check=True, stdout=subprocess.DEVNULL)
+        return time.perf_counter() - t0
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
+def run_oracle(deck_dir, steps):
+    sps = importlib.import_module("stress-particle-sph_amd")
+    sim = sps.Simulation.from_deck(sps.read_deck(deck_dir), lib_path=os.path.join(HERE, "libsps_oracle.so"))
+    t0 = time.perf_counter()
+    sim.step(steps)
+    el = time.perf_counter() - t0
+    n = sim.n2
+    sim.close()
+    return el, n
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--steps-c1", type=int, default=20)
+    ap.add_argument("--quick", action="store_true", help="skip dx 0.05")
+    args = ap.parse_args()
+    if not os.path.exists(REF_BIN):
+        sys.exit("build the reference first: make -C oracle ref")
+    sys.path.insert(0, ROOT)
+    tmp = tempfile.mkdtemp(prefix="tref_decks_")
+    cases = [("c1_collapse", os.path.join(DECKS, "c1_collapse"), args.steps_c1)]
+    for dx, steps in ((0.2, 20), (0.1, 5), (0.05, 2)):
+        if args.quick and dx < 0.1:
+            continue
+        cases.append((f"c2 slope dx {dx:g}", coarse_slope(os.path.join(tmp, f"dx{dx:g}"), dx), steps))
+    print("| case | particles | steps timed | reference as written, s/step | oracle (C), s/step | ratio |")
+    print("|---|---|---|---|---|---|")
+    for name, d, steps in cases:
+        base = run_ref(d, 0)
+        ref = (run_ref(d, steps) - base) / steps
+        ora, n = run_oracle(d, steps)
+        ora /= steps
+        print(f"| {name} | {n} | {steps} | {ref:.4g} | {ora:.4g} | {ref / ora:.1f}x |", flush=True)
+    shutil.rmtree(tmp, ignore_errors=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -27,7 +27,7 @@ PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 PRODUCT_LIB = os.path.join(PKG_DIR, "libsps_hip.so")
 DECK_DIR = os.path.join(PKG_DIR, "decks")
 
-SPS_ABI_VERSION = 3
+SPS_ABI_VERSION = 4
 SPS_MAX_TCURVE = 64
 SPS_MAX_BCS = 32
 SPS_MAX_CURVES = 16
@@ -88,7 +88,7 @@ class Bcs(ctypes.Structure):
 class Stats(ctypes.Structure):
     _fields_ = [("n_pairs", ctypes.c_int64), ("m_pairs", ctypes.c_int64),
                 ("n_by_type", ctypes.c_int64 * 10), ("ndivx", ctypes.c_int32),
-                ("ndivy", ctypes.c_int32), ("steps", ctypes.c_int64)]
+                ("ndivy", ctypes.c_int32), ("steps", ctypes.c_int64), ("pairs_total", ctypes.c_int64)]
 
 
 class KernelTime(ctypes.Structure):
@@ -314,7 +314,7 @@ class Simulation:
         s = Stats()
         self._check(self.lib.sps_get_stats(self.handle, ctypes.byref(s)))
         return dict(n_pairs=s.n_pairs, m_pairs=s.m_pairs, n_by_type=list(s.n_by_type),
-                    ndivx=s.ndivx, ndivy=s.ndivy, steps=s.steps)
+                    ndivx=s.ndivx, ndivy=s.ndivy, steps=s.steps, pairs_total=s.pairs_total)
 
     def set_materials(self, table, mat):
         """A material per particle: table rows (tan phi, c, E, nu, D11, D22,

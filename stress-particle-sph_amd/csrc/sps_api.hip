@@ -4,7 +4,7 @@
 // One step is the reference's time_integration (code/2_SPH_main_2018.f90
 // :78-184) for the supported flag set:
 //   Check_Out_Domain, grid_find_NEW + Pint_Update  -> k_check_ood .. k_split_partner
-//   RK4 (2_...:653-802)                            -> k_rk_init, 4 x stage, k_rk_final
+//   RK4 (2_...:653-802)                            -> k_stage_first, 4 x stage, k_rk_final
 //   update_strain, stress_point_update, adapt      -> k_rk_final, k_spu_*
 //   position update and displacement               -> k_position
 // The search bins the particles and fixes the order of the pair list; every
@@ -129,7 +129,7 @@ struct sps_handle {
     StateSet S[2];
     int cur = 0;
     // step work arrays
-    DBuf<double2> vx0, x0, vel0, rk_vel, rhs2, f_bound, art_visc;
+    DBuf<double2> x0, vel0, rk_vel, rhs2, f_bound, art_visc;   // vel0 is also the step's vx0
     DBuf<double4> stress0, rk_stress, rhs1, grad_u;
     DBuf<double> rk_dev, ddev;
     // search
@@ -139,6 +139,7 @@ struct sps_handle {
     DBuf<double> bparts;
     int nbparts = 0;
     DBuf<int> cellkey, new_cell, cell, kcount, kstart, kfill, tmp_orig, tmp_src, proc, proc_to_slot;
+    DBuf<unsigned char> moved;    // per block of slots: the permutation moves a particle of it
     DBuf<unsigned> cnt, off, scan_tmp, ctot, cstart_rm;
     // per-step pair lists of the tiled stage kernels
     DBuf<unsigned long long> list_ia, list_ib;
@@ -152,6 +153,7 @@ struct sps_handle {
     DBuf<double> disp10;
     // prescribed BCs (sps_set_bcs): tables on the host, node rows on the device
     bool bcs_set = false;
+    bool surface_pending = false;  // the last step's free-surface detection is still to run
     int nbc = 0, ncurves = 0;
     double bc_list[8 * SPS_MAX_BCS];
     int curve_npts[SPS_MAX_CURVES];
@@ -206,6 +208,9 @@ struct sps_handle {
     DBuf<int> err;
     // host-visible summary of the last search
     unsigned long long n_pairs = 0, m_pairs = 0, m_prev = 0;
+    bool search_stale = false;         // the device holds a search the host has not read (sync_search)
+    DBuf<unsigned long long> pairs_total;
+    unsigned long long pairs_total_host = 0;
     unsigned long long n_by_type[10] = {0};
     int ndivx = 0, ndivy = 0;
     unsigned long long *pinned = nullptr;   // readback staging
@@ -236,7 +241,7 @@ enum KernelId {
 };
 static const char *const kKernelNames[K_NKERNELS] = {
     "k_check_ood", "k_bounds", "k_cell_count", "k_cell_scatter", "k_cell_sort", "k_permute",
-    "k_proc_slot", "k_count", "k_scan", "k_split", "k_rk_init",
+    "k_proc_slot", "k_count", "k_scan", "k_split", "k_stage_first",
     "k_stage_begin", "k_spu_nodes", "k_sp_update", "k_deriv_nodes",
     "k_dummy_stress", "k_rk_final", "k_position", "k_surrogate", "k_xsph", "k_shift_sp", "k_bcs", "k_surf"};
 
@@ -297,18 +302,26 @@ static void drop_graphs(sps_handle *h);
 extern "C" const char *sps_backend(void) { return "hip-gfx950"; }
 
 // ------------------------------------------------------------------ scans
-// Exclusive scan of n unsigned values; optional device total.
+// Exclusive scan of n unsigned values; optional device total.  With a
+// device count (sc.g set) n is only the launch bound: the kernels scan the
+// first sc.get(n) values, as many as the current grid has.
 static int scan_u32(sps_handle *h, const unsigned *in, unsigned *out, long long n,
-                    unsigned long long *dtotal) {
+                    unsigned long long *dtotal, ScanCount sc = ScanCount{nullptr, 0, 0}) {
     long long nb = (n + BLOCK * 4 - 1) / (BLOCK * 4);
     if (nb == 0) nb = 1;
     int rc = dalloc(h->scan_tmp, (size_t)nb);
     if (rc) return rc;
-    LAUNCH(K_SCAN, (unsigned)nb, BLOCK, k_scan_local, in, out, h->scan_tmp.p, n);
-    LAUNCH(K_SCAN, 1, BLOCK, k_scan_sums, h->scan_tmp.p, (int)nb, dtotal);
-    LAUNCH(K_SCAN, (unsigned)nb, BLOCK, k_scan_add, out, h->scan_tmp.p, n);
+    LAUNCH(K_SCAN, (unsigned)nb, BLOCK, k_scan_local, in, out, h->scan_tmp.p, n, sc);
+    LAUNCH(K_SCAN, 1, BLOCK, k_scan_sums, h->scan_tmp.p, (int)nb, dtotal, n, sc);
+    LAUNCH(K_SCAN, (unsigned)nb, BLOCK, k_scan_add, out, h->scan_tmp.p, n, sc);
     HIP_TRY(hipGetLastError());
     return 0;
+}
+
+// Blocks of a grid-stride kernel over at most n items.
+static inline unsigned nstride(long long n) {
+    const unsigned b = nblk(n);
+    return b < MAX_STRIDE_BLOCKS ? (b ? b : 1u) : MAX_STRIDE_BLOCKS;
 }
 
 // --------------------------------------------------------------- params
@@ -468,16 +481,17 @@ extern "C" int sps_create(const sps_params *params, sps_handle **out) {
     rc = 0;
     rc |= h->S[0].alloc(nn, nt, nt2);
     rc |= h->S[1].alloc(nn, nt, nt2);
-    rc |= dalloc(h->vx0, nt); rc |= dalloc(h->x0, nt); rc |= dalloc(h->vel0, nt);
+    rc |= dalloc(h->x0, nt); rc |= dalloc(h->vel0, nt);
     rc |= dalloc(h->rk_vel, nn); rc |= dalloc(h->rhs2, nn); rc |= dalloc(h->f_bound, nn);
     rc |= dalloc(h->art_visc, nn); rc |= dalloc(h->stress0, nt); rc |= dalloc(h->rk_stress, nt);
     rc |= dalloc(h->rhs1, nt); rc |= dalloc(h->grad_u, nt); rc |= dalloc(h->rk_dev, nt);
     rc |= dalloc(h->ddev, nt);
-    rc |= dalloc(h->grid, 1); rc |= dalloc(h->split, 1); rc |= dalloc(h->err, 1);
+    rc |= dalloc(h->grid, 1); rc |= dalloc(h->split, 1); rc |= dalloc(h->err, 1); rc |= dalloc(h->pairs_total, 1);
     h->nbparts = std::min(1024, (int)nblk(nt2));
     rc |= dalloc(h->bparts, 5 * (size_t)h->nbparts);
     rc |= dalloc(h->cellkey, nt2); rc |= dalloc(h->new_cell, nt2); rc |= dalloc(h->cell, nt2);
     rc |= dalloc(h->tmp_orig, nt2); rc |= dalloc(h->tmp_src, nt2); rc |= dalloc(h->proc, nt2);
+    rc |= dalloc(h->moved, nblk(nt2));
     rc |= dalloc(h->proc_to_slot, nt2); rc |= dalloc(h->cnt, nt2); rc |= dalloc(h->off, nt2);
     h->ntile_n = (int)nblk8(nn);
     h->ntile_s = (int)nblk8(nt - nn);
@@ -495,8 +509,10 @@ extern "C" int sps_create(const sps_params *params, sps_handle **out) {
     rc |= dalloc(h->plans, 3 * (size_t)(h->ntile_n + h->ntile_s + 1));
     rc |= dalloc(h->x_build, nt); rc |= dalloc(h->dmax, 1); rc |= dalloc(h->stepv, 4);
     if (params->no_bcs > 0)
-        rc |= dalloc(h->bc_info, 8 * (size_t)nn) | dalloc(h->bc_or_not, nt) | dalloc(h->bc_vals, SPS_MAX_BCS) |
-              dalloc(h->normal, nt);
+        rc |= dalloc(h->bc_info, 8 * (size_t)nn) | dalloc(h->bc_vals, SPS_MAX_BCS);
+    // BC_or_not and the surface normals exist with BCs and with the position
+    // update, which runs the free-surface detection (3_...:1045-1047, 918)
+    if (params->no_bcs > 0 || params->update_x) rc |= dalloc(h->bc_or_not, nt) | dalloc(h->normal, nt);
     if (h->outside) {
         rc |= dalloc(h->inv, nt2); rc |= dalloc(h->nodeinfo, nn);
         rc |= dalloc(h->x10, nn); rc |= dalloc(h->disp10, nn);
@@ -506,11 +522,15 @@ extern "C" int sps_create(const sps_params *params, sps_handle **out) {
         return bail(fail(SPS_ENOMEM, "pinned staging allocation"));
     if (hipEventCreateWithFlags(&h->ev_disp, hipEventDisableTiming) != hipSuccess)
         return bail(fail(SPS_EDEVICE, "event creation failed"));
-    if (hipMemsetAsync(h->f_bound.p, 0, nn * sizeof(double2), h->stream) != hipSuccess ||
+    if ((h->bc_or_not.p && (hipMemsetAsync(h->bc_or_not.p, 0, nt * sizeof(int), h->stream) != hipSuccess ||
+                            hipMemsetAsync(h->normal.p, 0, nt * sizeof(double2), h->stream) != hipSuccess)) ||
+        hipMemsetAsync(h->f_bound.p, 0, nn * sizeof(double2), h->stream) != hipSuccess ||
         hipMemsetAsync(h->art_visc.p, 0, nn * sizeof(double2), h->stream) != hipSuccess ||
         hipMemsetAsync(h->grad_u.p, 0, nt * sizeof(double4), h->stream) != hipSuccess ||
         hipMemsetAsync(h->ddev.p, 0, nt * sizeof(double), h->stream) != hipSuccess ||
         hipMemsetAsync(h->err.p, 0, sizeof(int), h->stream) != hipSuccess ||
+        hipMemsetAsync(h->split.p, 0, sizeof(Split), h->stream) != hipSuccess ||
+        hipMemsetAsync(h->pairs_total.p, 0, sizeof(unsigned long long), h->stream) != hipSuccess ||
         hipStreamSynchronize(h->stream) != hipSuccess)
         return bail(fail(SPS_EDEVICE, "initial clears failed"));
     *out = h;
@@ -533,6 +553,8 @@ static int size_cells(sps_handle *h, const double *hsml) {
     size_t L = 3 * ((size_t)cap + 1) + 1;
     rc |= dalloc(h->kcount, L); rc |= dalloc(h->kstart, L); rc |= dalloc(h->kfill, L);
     rc |= dalloc(h->ctot, (size_t)cap + 2); rc |= dalloc(h->cstart_rm, (size_t)cap + 2);
+    // block sums of the largest scan of a step (the bucket counts or the slots)
+    rc |= dalloc(h->scan_tmp, std::max(L, (size_t)h->nt2 + 1) / (BLOCK * 4) + 2);
     if (rc) return rc;
     h->cell_cap = cap;
     return 0;
@@ -601,6 +623,7 @@ extern "C" int sps_set_state(sps_handle *h, const sps_state *st) {
     HIP_TRY(hipMemsetAsync(h->err.p, 0, sizeof(int), s));   // a new state starts with a clear flag
     HIP_TRY(hipStreamSynchronize(s));
     h->have_state = true;
+    h->surface_pending = false;
     h->lists_valid = false;
     drop_graphs(h);
     if (h->mtab.p) {   // the slots are in original order again: their materials too
@@ -613,9 +636,11 @@ extern "C" int sps_set_state(sps_handle *h, const sps_state *st) {
 }
 
 // Neighbour search of one step: binning, processing order, pair counts, the
-// list split and the ordered lists.
+// list split and the ordered lists.  Every launch is sized from host bounds
+// (the cell capacity of the domain box, the particle counts) and the kernels
+// read the current grid on the device, so the search queues without a host
+// round trip; its counts reach the host when asked (sync_search).
 static int search(sps_handle *h, bool fresh_list) {
-    hipStream_t s = h->stream;
     const int nn = h->nn, nt = h->nt, nt2 = h->nt2;
     StateSet &A = h->S[h->cur];
     StateSet &B = h->S[1 - h->cur];
@@ -623,45 +648,35 @@ static int search(sps_handle *h, bool fresh_list) {
     LAUNCH(K_CHECK_OOD, nblk(nt2), BLOCK, k_check_ood, A.x.p, A.ood.p, nt2, h->p.xmin_domain[0], h->p.xmin_domain[1],
                                            h->p.xmax_domain[0], h->p.xmax_domain[1]);
     LAUNCH(K_BOUNDS, h->nbparts, BLOCK, k_bounds_partial, A.x.p, A.hsml.p, A.ood.p, nt2, h->bparts.p);
-    LAUNCH(K_BOUNDS, 1, BLOCK, k_bounds_final, h->bparts.p, h->nbparts, h->grid.p, h->cell_cap, h->skin);
-    // the cell count is needed on the host for the bucket arrays' extent
-    HIP_TRY(hipMemcpyAsync(h->pinned, h->grid.p, sizeof(Grid), hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    Grid gh;
-    std::memcpy(&gh, h->pinned, sizeof(Grid));
-    if (gh.overflow) {
-        h->poisoned = true;
-        return fail(SPS_ECAPACITY, "cell grid %d x %d exceeds the capacity %d sized from the domain box",
-                    gh.ndivx, gh.ndivy, h->cell_cap);
-    }
-    h->ndivx = gh.ndivx;
-    h->ndivy = gh.ndivy;
-    const int ndivt = gh.ndivt;
-    const long long L3 = 3LL * (ndivt + 1);
-    HIP_TRY(hipMemsetAsync(h->kcount.p, 0, (L3 + 1) * sizeof(int), s));
-    HIP_TRY(hipMemsetAsync(h->kfill.p, 0, (L3 + 1) * sizeof(int), s));
+    LAUNCH(K_BOUNDS, 1, BLOCK, k_bounds_final, h->bparts.p, h->nbparts, h->grid.p, h->cell_cap, h->skin, h->err.p);
+    // bucket arrays: 3 (ndivt + 1) + 1 entries of the current grid, at most
+    // those of the capacity
+    const long long L3max = 3LL * (h->cell_cap + 1);
+    LAUNCH(K_CELL_COUNT, nstride(L3max + 1), BLOCK, k_zero_cells, h->kcount.p, h->kfill.p, h->grid.p);
     LAUNCH(K_CELL_COUNT, nblk(nt2), BLOCK, k_cell_count, A.x.p, A.ood.p, h->grid.p, h->cellkey.p, h->kcount.p, nn, nt, nt2);
-    int rc = scan_u32(h, (const unsigned *)h->kcount.p, (unsigned *)h->kstart.p, L3 + 1, nullptr);
+    int rc = scan_u32(h, (const unsigned *)h->kcount.p, (unsigned *)h->kstart.p, L3max + 1, nullptr,
+                      ScanCount{h->grid.p, 3, 4});
     if (rc) return rc;
     LAUNCH(K_CELL_SCATTER, nblk(nt2), BLOCK, k_cell_scatter, h->cellkey.p, A.orig.p, h->kstart.p, h->kfill.p, h->grid.p,
                                               h->tmp_orig.p, h->tmp_src.p, h->cell.p, nn, nt, nt2);
-    LAUNCH(K_CELL_SORT, nblk(L3), BLOCK, k_cell_sort, h->kstart.p, h->tmp_orig.p, h->tmp_src.p, h->grid.p, h->err.p);
-    // permute the state into (kind, cell, original index) order
-    LAUNCH(K_PERMUTE, nblk(nt2), BLOCK, k_permute, B.view(), A.view(), h->tmp_src.p, nn, nt, nt2);
+    LAUNCH(K_CELL_SORT, nstride(L3max), BLOCK, k_cell_sort, h->kstart.p, h->tmp_orig.p, h->tmp_src.p, h->grid.p,
+           h->err.p);
+    // permute the state into (kind, cell, original index) order: the blocks
+    // whose particles moved go through the scratch set and back
+    LAUNCH(K_PERMUTE, nblk(nt2), BLOCK, k_permute, B.view(), A.view(), h->tmp_src.p, nn, nt, nt2, h->moved.p);
+    LAUNCH(K_PERMUTE, nblk(nt2), BLOCK, k_permute_commit, A.view(), B.view(), h->moved.p, nn, nt, nt2);
     HIP_TRY(hipGetLastError());
-    h->cur = 1 - h->cur;
-    StateSet &C = h->S[h->cur];
+    StateSet &C = A;
     if (h->outside) LAUNCH(K_PERMUTE, nblk(nt2), BLOCK, k_inv_slots, C.orig.p, h->inv.p, nt2);
-    LAUNCH(K_PROC, nblk(ndivt + 1), BLOCK, k_cell_totals, h->kstart.p, h->grid.p, h->ctot.p);
-    rc = scan_u32(h, h->ctot.p, h->cstart_rm.p, ndivt + 1, nullptr);
+    LAUNCH(K_PROC, nstride((long long)h->cell_cap + 1), BLOCK, k_cell_totals, h->kstart.p, h->grid.p, h->ctot.p);
+    rc = scan_u32(h, h->ctot.p, h->cstart_rm.p, (long long)h->cell_cap + 1, nullptr, ScanCount{h->grid.p, 1, 1});
     if (rc) return rc;
     LAUNCH(K_PROC, nblk(nt2), BLOCK, k_proc_slot, h->cell.p, h->kstart.p, h->cstart_rm.p, h->grid.p, h->proc.p,
            nn, nt, nt2);
     LAUNCH(K_PROC, nblk(nt2), BLOCK, k_proc_to_slot, h->proc.p, h->proc_to_slot.p, nt2);
     // counts, offsets, split
-    HIP_TRY(hipMemsetAsync(h->cnt.p, 0, nt2 * sizeof(unsigned), s));
-    const unsigned long long M = fresh_list ? 0ull : h->m_pairs;
-    LAUNCH(K_SPLIT, 1, 64, k_split_reset, h->split.p, M);
+    LAUNCH(K_SPLIT, nblk(nt2), BLOCK, k_surf_zero, h->cnt.p, (long long)nt2);
+    LAUNCH(K_SPLIT, 1, 64, k_split_reset, h->split.p, fresh_list ? 1 : 0);
     {
         const SearchView V{h->cell.p, h->kstart.p, h->proc.p, h->cnt.p, h->off.p, h->grid.p, h->split.p, h->p.pi};
         const PairLists PL = h->lists();
@@ -688,16 +703,33 @@ static int search(sps_handle *h, bool fresh_list) {
     if (rc) return rc;
     LAUNCH(K_SPLIT, nblk(nt2), BLOCK, k_split, h->cnt.p, h->off.p, nt2, h->split.p);
     LAUNCH(K_SPLIT, 1, 64, k_split_partner, C.x.p, C.hsml.p, h->cell.p, h->kstart.p, h->proc.p, h->off.p,
-                                     h->proc_to_slot.p, h->grid.p, h->split.p, nn, nt);
+                                     h->proc_to_slot.p, h->grid.p, h->split.p, nn, nt, h->pairs_total.p);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(h->pinned, h->split.p, sizeof(Split), hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
+    h->search_stale = true;
+    return 0;
+}
+
+// The last search's counts on the host (n, M, the type histogram, the grid):
+// one copy and a wait, only when a caller asks for them.
+static int sync_search(sps_handle *h) {
+    if (!h->search_stale) return 0;
+    HIP_TRY(hipMemcpyAsync(h->pinned, h->split.p, sizeof(Split), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->pinned + 16, h->grid.p, sizeof(Grid), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->pinned + 24, h->pairs_total.p, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                           h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
     Split sh;
+    Grid gh;
     std::memcpy(&sh, h->pinned, sizeof(Split));
+    std::memcpy(&gh, h->pinned + 16, sizeof(Grid));
     h->n_pairs = sh.n;
     h->m_prev = sh.M;
     h->m_pairs = std::max(sh.M, sh.n);
     for (int k = 0; k < 10; ++k) h->n_by_type[k] = sh.n_by_type[k];
+    h->ndivx = gh.ndivx;
+    h->ndivy = gh.ndivy;
+    std::memcpy(&h->pairs_total_host, h->pinned + 24, sizeof(unsigned long long));
+    h->search_stale = false;
     return 0;
 }
 
@@ -721,7 +753,7 @@ static int mixed_step_kernels(sps_handle *h, double dt, double time_sph, double 
                               const double *stepv, bool pre_shift = false, int shift = -1) {
     const int nn = h->nn, nt = h->nt, nt2 = h->nt2;
     StateBufs SB = h->S[h->cur].view();
-    Work W{h->vx0.p, h->x0.p, h->vel0.p, h->rk_vel.p, h->rhs2.p, h->f_bound.p, h->art_visc.p,
+    Work W{h->vel0.p, h->x0.p, h->vel0.p, h->rk_vel.p, h->rhs2.p, h->f_bound.p, h->art_visc.p,
            h->stress0.p, h->rk_stress.p, h->rhs1.p, h->grad_u.p, h->rk_dev.p, h->ddev.p};
     const SearchView V{h->cell.p, h->kstart.p, h->proc.p, h->cnt.p, h->off.p, h->grid.p, h->split.p, h->p.pi};
     const PairLists PL = h->lists();
@@ -743,10 +775,10 @@ static int mixed_step_kernels(sps_handle *h, double dt, double time_sph, double 
             LAUNCH_LDS(K_SP_UPDATE, nblk8(nt - nn), TILE, LAY_SP_UPDATE_MIX.bytes, k_sp_update_mix, SB, W, V, PL,
                        h->dp, time_sph, 0.0, 0, adapt, hu, nn, nt, stepv);
     }
-    LAUNCH(K_RK_INIT, nblk(nt2), BLOCK, k_rk_init, SB, W, nn, nt, nt2);
+    LAUNCH(K_RK_INIT, nblk(nt2), BLOCK, k_stage_first, SB, W, adapt, tanfi, coh, nn, nt, nt2);
     for (int st = 0; st < 4; ++st) {
         const double c1 = f1rk[st] * (dt);
-        LAUNCH(K_STAGE_BEGIN, nblk(nt), BLOCK, k_stage_begin, SB, W, c1, adapt, tanfi, coh, nn, nt);
+        if (st > 0) LAUNCH(K_STAGE_BEGIN, nblk(nt), BLOCK, k_stage_begin, SB, W, c1, adapt, tanfi, coh, nn, nt);
         if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, h->bc_vars, h->bc_vals.p, h->ifsigman, h->normal.p, nn);
         if (nn > 0)
             LAUNCH_LDS(K_SPU_NODES, nblk8(nn), TILE, LAY_SPU_NODES_MIX.bytes, k_spu_nodes_mix, SB, V, PL, adapt,
@@ -844,6 +876,8 @@ static int mixed_step_graph(sps_handle *h, double dt, double time_sph, double gx
 // particle's pairs of types 1-3 gathered in that order, then the four steps
 // of the detection; BC_or_not and the normals by original index.
 static int free_surface(sps_handle *h, const StateBufs &SB) {
+    int rc0 = sync_search(h);   // the pair count sizes the detection's arrays
+    if (rc0) return rc0;
     const long long np = (long long)h->n_pairs;
     const int nn = h->nn, nt = h->nt, nt2 = h->nt2;
     int rc = 0;
@@ -918,7 +952,7 @@ static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
         return 0;
     }
     StateBufs SB = h->S[h->cur].view();
-    Work W{h->vx0.p, h->x0.p, h->vel0.p, h->rk_vel.p, h->rhs2.p, h->f_bound.p, h->art_visc.p,
+    Work W{h->vel0.p, h->x0.p, h->vel0.p, h->rk_vel.p, h->rhs2.p, h->f_bound.p, h->art_visc.p,
            h->stress0.p, h->rk_stress.p, h->rhs1.p, h->grad_u.p, h->rk_dev.p, h->ddev.p};
     const SearchView V{h->cell.p, h->kstart.p, h->proc.p, h->cnt.p, h->off.p, h->grid.p, h->split.p, h->p.pi};
     const PairLists PL = h->lists();
@@ -936,10 +970,10 @@ static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
             LAUNCH_LDS(K_SP_UPDATE, nblk8(nt - nn), TILE, LAY_SP_UPDATE.bytes, k_sp_update, SB, W, V, PL, h->dp,
                        time_sph, 0.0, 0, adapt, nn, nt);
     }
-    LAUNCH(K_RK_INIT, nblk(nt2), BLOCK, k_rk_init, SB, W, nn, nt, nt2);
+    LAUNCH(K_RK_INIT, nblk(nt2), BLOCK, k_stage_first, SB, W, adapt, tanfi, coh, nn, nt, nt2);
     for (int st = 0; st < 4; ++st) {
         const double c1 = f1rk[st] * (dt);
-        LAUNCH(K_STAGE_BEGIN, nblk(nt), BLOCK, k_stage_begin, SB, W, c1, adapt, tanfi, coh, nn, nt);
+        if (st > 0) LAUNCH(K_STAGE_BEGIN, nblk(nt), BLOCK, k_stage_begin, SB, W, c1, adapt, tanfi, coh, nn, nt);
         if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, h->ifsigman, h->normal.p, nn);
         if (nn > 0)
             LAUNCH_LDS(K_SPU_NODES, nblk8(nn), TILE, LAY_SPU_NODES.bytes, k_spu_nodes, SB, V, PL, adapt, tanfi, coh, nn);
@@ -970,9 +1004,21 @@ static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
         LAUNCH_LDS(K_XSPH, nblk8(nt - nn), TILE, LAY_XSPH.bytes, (k_xsph<KIND_SP>), SB, V, PL, h->xsph_v.p);
     LAUNCH(K_POSITION, nblk(nt), BLOCK, k_position, SB, W, dt, h->p.update_x, nn, nt, xsph ? h->xsph_v.p : nullptr,
            h->err.p);
-    if (h->p.update_x && h->p.no_bcs > 0) {   // 2_...:150-154
-        rc = free_surface(h, SB);
-        if (rc) return rc;
+    if (h->p.update_x) {   // 2_...:150-154
+        // The detection's results (BC_or_not, the normals) change the next
+        // step only through apply_stress_free (ifsigman = 1) and through
+        // XSPH_update's marks on prescribed nodes (which it then drops), and
+        // the outside approach's shift follows it; otherwise only the output
+        // reads them (prescribed nodes keep BC_or_not = 1 through it), and
+        // sps_get_surface runs the detection on the step's pairs and
+        // positions when asked: its result depends on nothing else then.
+        h->surface_pending = false;
+        if (h->outside || (h->p.no_bcs > 0 && (h->ifsigman || h->xsph))) {
+            rc = free_surface(h, SB);
+            if (rc) return rc;
+        } else {
+            h->surface_pending = true;
+        }
     }
     if (h->outside && h->p.update_x && nn > 0) {
         const ShiftParams sp{h->p.npoints, h->p.vel_vector, itimestep % h->p.shift_update == 0 ? 1 : 0,
@@ -1049,6 +1095,12 @@ extern "C" int sps_step(sps_handle *h, int32_t nsteps, double dt, double *time_s
     HIP_TRY(hipStreamSynchronize(h->stream));
     prof_collect(h);
     const int flag = *(int *)h->pinned;
+    if (flag & ERR_GRID) {
+        h->poisoned = true;
+        (void)sync_search(h);
+        return fail(SPS_ECAPACITY, "cell grid %d x %d exceeds the capacity %d sized from the domain box",
+                    h->ndivx, h->ndivy, h->cell_cap);
+    }
     if (flag & ERR_NONFINITE) {
         h->poisoned = true;
         return fail(SPS_ENUMERIC, "non-finite position, velocity or stress of a particle in the domain box "
@@ -1123,6 +1175,9 @@ extern "C" int sps_get_state(sps_handle *h, sps_state *st) {
 extern "C" int sps_get_pairs(sps_handle *h, int64_t cap, int32_t *pair_i, int32_t *pair_j, int32_t *pint_type,
                              float *w, float *dwdx, float *dwdy, int64_t *n) {
     if (!h || !n) return fail(SPS_EINVAL, "null argument");
+    HIP_TRY(hipSetDevice(h->device));
+    int rc0 = sync_search(h);
+    if (rc0) return rc0;
     *n = (int64_t)h->n_pairs;
     if (h->steps == 0) {
         *n = 0;
@@ -1177,8 +1232,12 @@ extern "C" int sps_get_pairs(sps_handle *h, int64_t cap, int32_t *pair_i, int32_
 
 extern "C" int sps_get_stats(sps_handle *h, sps_stats *out) {
     if (!h || !out) return fail(SPS_EINVAL, "null argument");
+    HIP_TRY(hipSetDevice(h->device));
+    int rc0 = sync_search(h);
+    if (rc0) return rc0;
     std::memset(out, 0, sizeof *out);
     out->n_pairs = (int64_t)h->n_pairs;
+    out->pairs_total = (int64_t)h->pairs_total_host;
     out->m_pairs = (int64_t)h->m_pairs;
     for (int k = 0; k < 10; ++k) out->n_by_type[k] = (int64_t)h->n_by_type[k];
     out->ndivx = h->ndivx;
@@ -1280,18 +1339,23 @@ extern "C" void sps_destroy(sps_handle *h) {
     dfree(h->stepv);
     h->S[0].release();
     h->S[1].release();
-    dfree(h->vx0); dfree(h->x0); dfree(h->vel0); dfree(h->rk_vel); dfree(h->rhs2); dfree(h->f_bound);
+    dfree(h->x0); dfree(h->vel0); dfree(h->rk_vel); dfree(h->rhs2); dfree(h->f_bound);
     dfree(h->art_visc); dfree(h->stress0); dfree(h->rk_stress); dfree(h->rhs1); dfree(h->grad_u);
     dfree(h->rk_dev); dfree(h->ddev); dfree(h->grid); dfree(h->split); dfree(h->bparts);
     dfree(h->cellkey); dfree(h->new_cell); dfree(h->cell); dfree(h->kcount); dfree(h->kstart);
-    dfree(h->kfill); dfree(h->tmp_orig); dfree(h->tmp_src); dfree(h->proc); dfree(h->proc_to_slot);
+    dfree(h->kfill); dfree(h->tmp_orig); dfree(h->tmp_src); dfree(h->proc); dfree(h->proc_to_slot); dfree(h->moved);
     dfree(h->cnt); dfree(h->off); dfree(h->scan_tmp); dfree(h->ctot); dfree(h->cstart_rm); dfree(h->err);
     dfree(h->list_ia); dfree(h->list_wa); dfree(h->list_gxa); dfree(h->list_gya); dfree(h->list_ib);
     dfree(h->list_gxb); dfree(h->list_gyb); dfree(h->deg_a); dfree(h->deg_b); dfree(h->tile_ok); dfree(h->tiles); dfree(h->plans);
     dfree(h->list_wb); dfree(h->xsph_v);
     dfree(h->x_build); dfree(h->dmax); dfree(h->surr_count); dfree(h->surr_slot); dfree(h->surr_sn);
     dfree(h->surr_str); dfree(h->surr_feat); dfree(h->surr_stats); dfree(h->plog); dfree(h->surr_mat);
-    dfree(h->mtab); dfree(h->mat_orig);
+    dfree(h->mtab); dfree(h->mat_orig); dfree(h->pairs_total);
+    dfree(h->inv); dfree(h->nodeinfo); dfree(h->x10); dfree(h->disp10);
+    dfree(h->bc_info); dfree(h->bc_or_not); dfree(h->bc_vals); dfree(h->normal); dfree(h->surf_nrm0);
+    dfree(h->surf_pi); dfree(h->surf_pj); dfree(h->surf_pt); dfree(h->surf_sub); dfree(h->surf_gx);
+    dfree(h->surf_gy); dfree(h->surf_cnt); dfree(h->surf_start); dfree(h->surf_cur); dfree(h->surf_ent);
+    dfree(h->surf_flag); dfree(h->surf_tt); dfree(h->surf_tau);
     if (h->pinned) (void)hipHostFree(h->pinned);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
@@ -1299,9 +1363,16 @@ extern "C" void sps_destroy(sps_handle *h) {
 
 extern "C" int sps_get_surface(sps_handle *h, int32_t *bc_or_not, double *normal) {
     if (!h) return fail(SPS_EINVAL, "null argument");
-    if (h->p.no_bcs <= 0 || !h->bcs_set) return fail(SPS_ESTATE, "no BCs set: the free surface is not tracked");
     if (h->mixed) return fail(SPS_EINVAL, "SPS_MODE_FAST does not track the free surface");
+    if (h->p.no_bcs > 0 && !h->bcs_set) return fail(SPS_ESTATE, "no_bcs > 0: sps_set_bcs before sps_get_surface");
+    if (!h->bc_or_not.p)
+        return fail(SPS_ESTATE, "the free surface is tracked with BCs or the position update only");
     HIP_TRY(hipSetDevice(h->device));
+    if (h->surface_pending) {   // the detection of the last step, on its pairs and positions
+        int rc = free_surface(h, h->S[h->cur].view());
+        if (rc) return rc;
+        h->surface_pending = false;
+    }
     if (bc_or_not)
         HIP_TRY(hipMemcpyAsync(bc_or_not, h->bc_or_not.p, (size_t)h->nt * sizeof(int), hipMemcpyDeviceToHost,
                                h->stream));

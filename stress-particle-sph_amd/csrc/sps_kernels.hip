@@ -35,6 +35,11 @@ constexpr int BLOCK = 256;
 constexpr int ERR_WALL_TERMS = 1;   // more wall-force terms on one node than MAX_WALL_TERMS
 constexpr int ERR_BUCKET = 2;       // more particles of one kind in one cell than MAX_BUCKET_SORT
 constexpr int ERR_NONFINITE = 4;    // a non-finite position, velocity or stress in the domain box
+constexpr int ERR_GRID = 8;         // the cell grid exceeded its capacity (sized from the domain box)
+
+// Launch size of a kernel that loops over a device-sized range (cells of
+// the current grid): enough blocks to cover the host's upper bound, capped.
+constexpr unsigned MAX_STRIDE_BLOCKS = 4096;
 
 __device__ __forceinline__ bool finite2(double2 v) { return isfinite(v.x) && isfinite(v.y); }
 __device__ __forceinline__ bool finite4(double4 v) {
@@ -114,7 +119,8 @@ __global__ void k_bounds_partial(const double2 *__restrict__ x, const double *__
 // widens the cells beyond 2 max h for pair lists that are kept over several
 // steps (0: the reference's grid).  Launch with exactly BLOCK threads.
 __global__ void __launch_bounds__(BLOCK) k_bounds_final(const double *__restrict__ part, int nparts,
-                                                        Grid *__restrict__ g, int cell_cap, double skin) {
+                                                        Grid *__restrict__ g, int cell_cap, double skin,
+                                                        int *__restrict__ err) {
     __shared__ double s[5][BLOCK];
     int t = threadIdx.x;
     double mn0 = (double)1.e+10f, mn1 = (double)1.e+10f, mx0 = -(double)1.e+10f,
@@ -155,6 +161,17 @@ __global__ void __launch_bounds__(BLOCK) k_bounds_final(const double *__restrict
     long long ndivt = (long long)nd[0] * nd[1];
     g->overflow = (ndivt > cell_cap || nd[0] < 1 || nd[1] < 1) ? 1 : 0;
     g->ndivt = g->overflow ? 0 : (int)ndivt;
+    if (g->overflow) atomicOr(err, ERR_GRID);   // every particle then sits in the sentinel cell
+}
+
+// Clear the per-(kind, cell) counters of the current grid, 3 (ndivt + 1) + 1
+// entries each (sized on the device: no host round trip).
+__global__ void k_zero_cells(int *__restrict__ a, int *__restrict__ b, const Grid *__restrict__ g) {
+    const long long n = 3LL * (g->ndivt + 1) + 1;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        a[i] = 0;
+        b[i] = 0;
+    }
 }
 
 // ------------------------------------------------------------- cell keys
@@ -228,42 +245,42 @@ constexpr int MAX_BUCKET_SORT = 4096;
 __global__ void k_cell_sort(const int *__restrict__ kstart, int *__restrict__ tmp_orig,
                             int *__restrict__ tmp_src, const Grid *__restrict__ g,
                             int *__restrict__ err) {
-    int ndivt = g->ndivt;
-    int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= 3 * (ndivt + 1)) return;
-    int lo = kstart[b], hi = kstart[b + 1];
-    if (hi - lo < 2) return;
-    if (hi - lo > MAX_BUCKET_SORT) {
-        if (b % (ndivt + 1) != ndivt) atomicOr(err, ERR_BUCKET);   // an in-domain cell
-        return;
-    }
-    for (int a = lo + 1; a < hi; ++a) {
-        int ko = tmp_orig[a], ks = tmp_src[a];
-        int p = a - 1;
-        while (p >= lo && tmp_orig[p] > ko) {
-            tmp_orig[p + 1] = tmp_orig[p];
-            tmp_src[p + 1] = tmp_src[p];
-            --p;
+    const int ndivt = g->ndivt;
+    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < 3 * (ndivt + 1); b += gridDim.x * blockDim.x) {
+        const int lo = kstart[b], hi = kstart[b + 1];
+        if (hi - lo < 2) continue;
+        if (hi - lo > MAX_BUCKET_SORT) {
+            if (b % (ndivt + 1) != ndivt) atomicOr(err, ERR_BUCKET);   // an in-domain cell
+            continue;
         }
-        tmp_orig[p + 1] = ko;
-        tmp_src[p + 1] = ks;
+        for (int a = lo + 1; a < hi; ++a) {
+            int ko = tmp_orig[a], ks = tmp_src[a];
+            int p = a - 1;
+            while (p >= lo && tmp_orig[p] > ko) {
+                tmp_orig[p + 1] = tmp_orig[p];
+                tmp_src[p + 1] = tmp_src[p];
+                --p;
+            }
+            tmp_orig[p + 1] = ko;
+            tmp_src[p + 1] = ks;
+        }
     }
 }
 
 // Particles per cell in the reference's row-major visiting order.
 __global__ void k_cell_totals(const int *__restrict__ kstart, const Grid *__restrict__ g,
                               unsigned *__restrict__ tot) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    int ndivt = g->ndivt;
-    if (c > ndivt) return;
-    if (c == ndivt) {
-        tot[c] = 0;
-        return;
+    const Grid gg = *g;
+    const int ndivt = gg.ndivt, L = ndivt + 1;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c <= ndivt; c += gridDim.x * blockDim.x) {
+        if (c == ndivt) {
+            tot[c] = 0;
+            continue;
+        }
+        const int key = cell_key(c % gg.ndivx, c / gg.ndivx, gg);
+        tot[c] = (unsigned)((kstart[key + 1] - kstart[key]) + (kstart[L + key + 1] - kstart[L + key]) +
+                            (kstart[2 * L + key + 1] - kstart[2 * L + key]));
     }
-    const int L = ndivt + 1;
-    int key = cell_key(c % g->ndivx, c / g->ndivx, *g);
-    tot[c] = (unsigned)((kstart[key + 1] - kstart[key]) + (kstart[L + key + 1] - kstart[L + key]) +
-                        (kstart[2 * L + key + 1] - kstart[2 * L + key]));
 }
 
 // Processing slot (the reference's visiting order: row-major cells, inside
@@ -321,10 +338,18 @@ __device__ __forceinline__ DevParams material_params(const DevParams &P, const S
     return q;
 }
 
+// The state into the new order: slot t takes slot perm[t]'s particle.  The
+// state stays in src: a block of BLOCK slots whose particles all stay in
+// place (most blocks after the first step) does nothing; the others gather
+// into the scratch set dst here and k_permute_commit copies them back once
+// every gather is done (a gather reads any slot of src).
 __global__ void k_permute(StateBufs dst, StateBufs src, const int *__restrict__ perm, int nn,
-                          int nt, int nt2) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nt2) return;
+                          int nt, int nt2, unsigned char *__restrict__ moved) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool mine = t < nt2 && perm[t] != t;
+    const bool any = __syncthreads_or(mine);
+    if (threadIdx.x == 0) moved[blockIdx.x] = any ? 1 : 0;
+    if (!any || t >= nt2) return;
     int s = perm[t];
     dst.x[t] = src.x[s];
     dst.vel[t] = src.vel[s];
@@ -343,6 +368,29 @@ __global__ void k_permute(StateBufs dst, StateBufs src, const int *__restrict__ 
         dst.fdr[t] = src.fdr[s];
     }
     if (src.mat) dst.mat[t] = src.mat[s];
+}
+
+__global__ void k_permute_commit(StateBufs dst, StateBufs src, const unsigned char *__restrict__ moved, int nn,
+                                 int nt, int nt2) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!moved[blockIdx.x] || t >= nt2) return;
+    dst.x[t] = src.x[t];
+    dst.vel[t] = src.vel[t];
+    dst.x00[t] = src.x00[t];
+    dst.stress[t] = src.stress[t];
+    dst.mass[t] = src.mass[t];
+    dst.rho[t] = src.rho[t];
+    dst.hsml[t] = src.hsml[t];
+    dst.orig[t] = src.orig[t];
+    dst.ood[t] = src.ood[t];
+    dst.wallpos[t] = src.wallpos[t];
+    dst.horiz[t] = src.horiz[t];
+    if (t < nn) dst.displ[t] = src.displ[t];
+    if (t < nt) {
+        dst.ivar1[t] = src.ivar1[t];
+        dst.fdr[t] = src.fdr[t];
+    }
+    if (src.mat) dst.mat[t] = src.mat[t];
 }
 
 // Materials of the slots from the materials of the original particles.
@@ -446,9 +494,13 @@ __global__ void k_count(const double2 *__restrict__ x, const double *__restrict_
     }
 }
 
-// Fresh split record: M is the list high-water mark before this search.
-__global__ void k_split_reset(Split *__restrict__ split, unsigned long long M) {
+// Fresh split record: M is the list high-water mark before this search,
+// the largest pair count of the earlier searches since the list started
+// (2_...:1290-1294, 1361-1369): 0 on a fresh list, else max(M, n) of the
+// previous record.
+__global__ void k_split_reset(Split *__restrict__ split, int fresh) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const unsigned long long M = fresh ? 0ull : (split->M > split->n ? split->M : split->n);
     split->n = 0;
     split->M = M;
     split->sstar = -1;
@@ -477,8 +529,10 @@ __global__ void k_split_partner(const double2 *__restrict__ x, const double *__r
                                 const int *__restrict__ cell, const int *__restrict__ kstart,
                                 const int *__restrict__ proc, const unsigned *__restrict__ off,
                                 const int *__restrict__ proc_to_slot, const Grid *__restrict__ g,
-                                Split *__restrict__ split, int nn, int nt) {
+                                Split *__restrict__ split, int nn, int nt,
+                                unsigned long long *__restrict__ pairs_total) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    *pairs_total += split->n;   // pairs found by every search of the handle
     int ps = split->sstar;
     split->pstar = -1;
     if (ps < 0) return;
@@ -1005,29 +1059,49 @@ struct Work {
     double *rk_dev, *ddev;
 };
 
-// Start of a step: vx0, x0 (2_...:114) and the RK4 prologue (2_...:681-690).
-__global__ void k_rk_init(StateBufs S, Work W, int nn, int nt, int nt2) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+// Start of a step and its first RK stage in one pass: vx0, x0 (2_...:114),
+// the RK4 prologue (2_...:681-690: stage-0 state, accumulators and the
+// velocities and stresses zeroed, dummies included) and the stage-0 state
+// with its first adapt_stress2 (2_...:700-701, 715).  The stage-0 update
+// v0 + 0 * rhs of the reference, whose right-hand sides are zero then, is
+// v0 + 0.0 (a zero of either sign becomes +0, as there); node stresses start
+// from the zeroed ones.  The right-hand sides themselves need no clearing:
+// the stage tails write them before any later stage reads them.  vx0 and
+// vel0 are the same velocities: Work.vx0 may alias vel0.
+__global__ void k_stage_first(StateBufs S, Work W, int adapt, double tanfi, double coh, int nn, int nt,
+                              int nt2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nt2) return;
-    if (i < nt) {
-        W.vx0[i] = S.vel[i];
-        W.x0[i] = S.x[i];
-        W.stress0[i] = S.stress[i];
-        W.vel0[i] = S.vel[i];
-        W.rk_stress[i] = make_double4(0, 0, 0, 0);
-        W.rhs1[i] = make_double4(0, 0, 0, 0);
-        W.rk_dev[i] = 0.0;
+    if (i >= nt) {
+        S.vel[i] = make_double2(0, 0);
+        S.stress[i] = make_double4(0, 0, 0, 0);
+        return;
     }
+    const double2 v = S.vel[i];
+    W.vel0[i] = v;
+    if (W.vx0 != W.vel0) W.vx0[i] = v;
+    W.x0[i] = S.x[i];
+    W.rk_dev[i] = 0.0;
+    material_tc(S, i, tanfi, coh);
     if (i < nn) {
         W.rk_vel[i] = make_double2(0, 0);
-        W.rhs2[i] = make_double2(0, 0);
-        W.art_visc[i] = make_double2(0, 0);
+        S.vel[i] = make_double2(v.x + 0.0, v.y + 0.0);
+        double4 s = make_double4(0, 0, 0, 0);
+        if (adapt) adapt_stress(s, tanfi, coh);
+        S.stress[i] = s;
+    } else {
+        const double4 s0 = S.stress[i];
+        W.stress0[i] = s0;
+        W.rk_stress[i] = make_double4(0, 0, 0, 0);
+        S.vel[i] = make_double2(0, 0);
+        double4 s = make_double4(s0.x + 0.0, s0.y + 0.0, s0.z + 0.0, s0.w + 0.0);
+        if (adapt) adapt_stress(s, tanfi, coh);
+        S.stress[i] = s;
     }
-    S.vel[i] = make_double2(0, 0);
-    S.stress[i] = make_double4(0, 0, 0, 0);
 }
 
-// Stage state (2_...:700-701) and the first adapt_stress2 of the stage.
+// Stage state of stages 2-4 (2_...:700-701) and the first adapt_stress2 of
+// the stage.
 __global__ void k_stage_begin(StateBufs S, Work W, double c1, int adapt, double tanfi, double coh,
                               int nn, int nt) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2611,8 +2685,21 @@ __device__ __forceinline__ unsigned block_excl_scan(unsigned v, unsigned *total)
     return base + incl - v;
 }
 
+// Element count of a scan: the host's n, or (with g) a * ndivt + b of the
+// current grid, at most the host's n (the launch bound).
+struct ScanCount {
+    const Grid *g;
+    int a, b;
+    __device__ __forceinline__ long long get(long long n) const {
+        if (!g) return n;
+        const long long m = (long long)a * g->ndivt + b;
+        return m < n ? m : n;
+    }
+};
+
 __global__ void k_scan_local(const unsigned *__restrict__ in, unsigned *__restrict__ out,
-                             unsigned *__restrict__ bsum, long long n) {
+                             unsigned *__restrict__ bsum, long long n_max, ScanCount sc) {
+    const long long n = sc.get(n_max);
     long long base = (long long)blockIdx.x * (BLOCK * 4) + (long long)threadIdx.x * 4;
     unsigned v[4], s = 0;
     for (int k = 0; k < 4; ++k) {
@@ -2628,8 +2715,12 @@ __global__ void k_scan_local(const unsigned *__restrict__ in, unsigned *__restri
     if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
-__global__ void k_scan_sums(unsigned *__restrict__ bsum, int nb, unsigned long long *__restrict__ total) {
-    // one block walks the block sums in chunks of BLOCK
+__global__ void k_scan_sums(unsigned *__restrict__ bsum, int nb_max, unsigned long long *__restrict__ total,
+                            long long n_max, ScanCount sc) {
+    // one block walks the block sums of the blocks that hold elements, in
+    // chunks of BLOCK
+    const long long nbl = (sc.get(n_max) + BLOCK * 4 - 1) / (BLOCK * 4);
+    const int nb = nbl < nb_max ? (int)nbl : nb_max;
     __shared__ unsigned carry;
     if (threadIdx.x == 0) carry = 0;
     __syncthreads();
@@ -2646,7 +2737,9 @@ __global__ void k_scan_sums(unsigned *__restrict__ bsum, int nb, unsigned long l
     if (threadIdx.x == 0 && total) *total = carry;
 }
 
-__global__ void k_scan_add(unsigned *__restrict__ out, const unsigned *__restrict__ bsum, long long n) {
+__global__ void k_scan_add(unsigned *__restrict__ out, const unsigned *__restrict__ bsum, long long n_max,
+                           ScanCount sc) {
+    const long long n = sc.get(n_max);
     long long base = (long long)blockIdx.x * (BLOCK * 4) + (long long)threadIdx.x * 4;
     unsigned add = bsum[blockIdx.x];
     for (int k = 0; k < 4; ++k)

@@ -10,7 +10,7 @@ module sps_binding
   use, intrinsic :: iso_c_binding
   implicit none
 
-  integer(c_int32_t), parameter :: SPS_ABI_VERSION = 3
+  integer(c_int32_t), parameter :: SPS_ABI_VERSION = 4
   integer, parameter :: SPS_MAX_TCURVE = 64
   integer, parameter :: SPS_MAX_BCS = 32, SPS_MAX_CURVES = 16
   integer(c_int), parameter :: SPS_OK = 0
@@ -52,7 +52,7 @@ module sps_binding
   type, bind(C) :: sps_stats_t
      integer(c_int64_t) :: n_pairs, m_pairs, n_by_type(10)
      integer(c_int32_t) :: ndivx, ndivy
-     integer(c_int64_t) :: steps
+     integer(c_int64_t) :: steps, pairs_total
   end type sps_stats_t
 
   interface

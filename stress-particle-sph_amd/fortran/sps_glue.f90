@@ -50,6 +50,7 @@ contains
     t = time_sph
     it = int(itimestep_sph - 1, c_int32_t)
     call sps_check(sps_step(handle, 1_c_int32_t, real(dt_sph, c_double), t, it), 'sps_step')
+    call neighbour_statistics()
     if (sps_glue_sync_every_step) then
        call sps_glue_sync()
     else if (sps_glue_sync_at_output) then
@@ -61,8 +62,62 @@ contains
     if (.not. c_associated(handle)) return
     call get_arrays(x, vel, stress, internal_vars, f_drucker, displ, If_Out_Domain, grad_u, &
                     Ddev_strn, f_bound, art_visc, x_10, disp_10, .not. inside_approach)
-    if (no_bcs > 0) call get_surface(BC_or_not, normal)
+    ! BC_or_not and the normals of the last free-surface detection (run
+    ! every step with the position update, 2_...:150-154; OutputRes writes
+    ! surface_points from them, 3_...:2910-2919)
+    if (no_bcs > 0 .or. update_x) call get_surface(BC_or_not, normal)
   end subroutine sps_glue_sync
+
+  ! The neighbour statistics grid_find_NEW keeps and writes to output_file
+  ! every print_step steps (code/2_SPH_main_2018.f90:1240, 1355-1356,
+  ! 1405-1431; nothing else reads them): each particle's pair count of the
+  ! step's search (countiac),
+  ! the largest and smallest with the first particle that has them, the
+  ! average, niac (set to 0 by the search) and the particles with none.  The
+  ! counts come from the library's pair list; the lines are written with the
+  ! reference's list-directed statements, so output.txt is the reference's.
+  subroutine neighbour_statistics
+    integer(c_int64_t) :: n, k
+    integer(c_int32_t), allocatable, target :: pi_(:), pj_(:)
+    integer, allocatable :: cnt(:)
+    integer :: i, sumiac, maxiac, miniac, noiac, maxp, minp
+    niac = 0
+    if (mod(itimestep_sph, print_step) /= 0) return
+    if (sps_get_pairs(handle, 0_c_int64_t, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, &
+                      c_null_ptr, n) /= 0 .and. n == 0) call sps_check(-1_c_int, 'sps_get_pairs')
+    allocate(pi_(max(n, 1_c_int64_t)), pj_(max(n, 1_c_int64_t)), cnt(ntotal2))
+    call sps_check(sps_get_pairs(handle, n, c_loc(pi_), c_loc(pj_), c_null_ptr, c_null_ptr, c_null_ptr, &
+                                 c_null_ptr, n), 'sps_get_pairs')
+    cnt = 0
+    do k = 1, n
+       cnt(pi_(k)) = cnt(pi_(k)) + 1
+       cnt(pj_(k)) = cnt(pj_(k)) + 1
+    end do
+    sumiac = 0
+    maxiac = 0
+    miniac = 1000
+    noiac  = 0
+    maxp = 0
+    minp = 0
+    do i = 1, ntotal2
+       sumiac = sumiac + cnt(i)
+       if (cnt(i) > maxiac) then
+          maxiac = cnt(i)
+          maxp = i
+       end if
+       if (cnt(i) < miniac) then
+          miniac = cnt(i)
+          minp = i
+       end if
+       if (cnt(i) == 0) noiac = noiac + 1
+    end do
+    write(output_file,*) ' >> Statistics: interactions per particle:', time, dt_sph
+    write(output_file,*)'**** Particle:',maxp, ' maximal interactions:',maxiac
+    write(output_file,*)'**** Particle:',minp, ' minimal interactions:',miniac
+    write(output_file,*)'**** Average :',real(sumiac)/real(ntotal2)
+    write(output_file,*)'**** Total pairs : ',niac
+    write(output_file,*)'**** Particles with no interactions:',noiac
+  end subroutine neighbour_statistics
 
   subroutine sps_glue_finish
     if (c_associated(handle)) call sps_destroy(handle)
@@ -71,8 +126,19 @@ contains
 
   subroutine open_handle
     type(sps_params_t) :: p
-    integer :: k, n
+    integer :: k, n, st
     real, save :: no_walls(1) = 0.0
+    character(len=16) :: mode
+    ! SPS_GLUE_SYNC=output keeps the state on the device between the steps
+    ! after which the driver writes (a driver left as the reference has it
+    ! cannot set the module variables itself); =step copies it back every step
+    call get_environment_variable('SPS_GLUE_SYNC', mode, status=st)
+    if (st == 0 .and. trim(mode) == 'output') then
+       sps_glue_sync_every_step = .false.
+       sps_glue_sync_at_output = .true.
+    else if (st == 0 .and. trim(mode) == 'step') then
+       sps_glue_sync_every_step = .true.
+    end if
     p%abi_version = SPS_ABI_VERSION
     p%nnode = nnode; p%nstress = nstress
     p%ndummy = merge(ndummy, 0, dummy_nodes)

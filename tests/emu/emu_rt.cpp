@@ -21,7 +21,7 @@ bool needs_sync(const char *name) {
     static const char *const sync[] = {"k_bounds_partial", "k_bounds_final", "k_count", "k_spu_nodes",
                                        "k_sp_update",      "k_deriv_nodes",  "k_scan_local", "k_scan_sums",
                                        "k_max_disp",       "k_gemm_bf16",    "k_mlp_loss",   "k_mlp_colsum",
-                                       "k_surrogate_apply", "k_xsph"};
+                                       "k_surrogate_apply", "k_xsph",   "k_permute"};
     for (const char *s : sync)
         if (strncmp(name, s, strlen(s)) == 0) return true;   // k_count also covers k_count_lists
     return false;
@@ -73,7 +73,7 @@ struct Fiber {
     char *stack = nullptr;
 };
 constexpr size_t STACK = 512 * 1024;
-constexpr int MAX_BLOCK = 256;
+constexpr int MAX_BLOCK = 1024;
 Fiber fibers[MAX_BLOCK];
 ucontext_t sched_ctx;
 jmp_buf sched_jb;
