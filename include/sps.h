@@ -254,7 +254,10 @@ typedef struct sps_bcs {
                                   (already times twopi); Tf */
     const int32_t *bc_info;    /* (8,nnode): Get_BCs_on_node's rows of the nodes */
     const int32_t *bc_or_not;  /* (ntotal) Get_BCs_on_node's BC_or_not of every
-                                  particle, nodes then stress points: 0 or 1 */
+                                  particle, nodes then stress points: 0 or 1
+                                  (anything else is refused); the call also
+                                  clears the free-surface normals, and a
+                                  refused call changes nothing */
     const int32_t *curve_npts; /* (ncurves) nptstcurves, 2..SPS_MAX_TCURVE */
     const double *curve_t;     /* (SPS_MAX_TCURVE,ncurves) ttcurves */
     const float *curve_f;      /* (SPS_MAX_TCURVE,ncurves) ftcurves */
@@ -270,6 +273,24 @@ int sps_set_bcs(sps_handle *h, const sps_bcs *bcs);
  * get_nodes_on_free_surface (3_...:1116-1411; the reference's
  * surface_points output, 3_...:2910-2919, lists the nodes with 2). */
 int sps_get_surface(sps_handle *h, int32_t *bc_or_not, double *normal);
+
+/* The time integrator.  SPS_INTEGRATOR_RK4 (the default) is the
+ * reference's classical RK4 (2_...:653-802).  SPS_INTEGRATOR_LEAPFROG has no
+ * counterpart in the reference: kick-drift leapfrog with the node velocities
+ * at half steps and the stresses and positions at whole steps -- the node
+ * accelerations of the step's state kick the velocities by dt (by dt / 2 on
+ * the first step after sps_set_integrator or sps_set_state, whose
+ * velocities are at a whole step), the stress points take the kicked
+ * velocities and their stress rate (elastic, plastic source, Jaumann spin)
+ * updates the stress by dt, and the positions drift by dt with the kicked
+ * velocities.  Two pair-loop passes per step instead of RK4's four stages;
+ * second order.  SPS_MODE_ORDERED, inside approach, without XSPH or the
+ * plastic log; sps_get_state then returns the half-step velocities. */
+enum {
+    SPS_INTEGRATOR_RK4 = 0,
+    SPS_INTEGRATOR_LEAPFROG = 1
+};
+int sps_set_integrator(sps_handle *h, int32_t integrator);
 
 /* Pair lists kept over steps.  No counterpart in the reference, which
  * searches every step; SPS_MODE_FAST only.  With skin > 0 the search also

@@ -15,6 +15,13 @@
  * bit-for-bit against dumps of the reference Fortran compiled from source
  * (oracle/_ref), and against closed forms on small inputs.
  *
+ * Threads: with OMP_NUM_THREADS above 1 the pair loops are split by the
+ * particle that receives each term (each thread walks, in list order, the
+ * pairs touching its own particles), so every particle sums the same terms
+ * in the same order and the results are bitwise those of one thread
+ * (tests/test_oracle_threads.py).  One thread is the reference's schedule,
+ * and the one used below 20,000 particles (SPS_ORACLE_MIN_PARALLEL).
+ *
  * Two stated departures, neither observable:
  *  - the dead store `adapt_stress = 0` (code/3_...:1961) is not executed;
  *  - boundary_forces (code/2_...:1117-1161) visits, for each node, only the
@@ -24,6 +31,7 @@
 #include "sps_oracle.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -96,6 +104,15 @@ struct sps_handle {
     int32_t *mat;
     double *mtab;
     float *av_tmp;
+    /* the integrator (sps_set_integrator): RK4, or leapfrog with the
+     * velocities at half steps (lf_first: the next step starts from a state
+     * whose velocities are at a whole step and kicks by dt/2) */
+    int integrator, lf_first;
+    /* several threads (OMP_NUM_THREADS > 1): each owns a contiguous range of
+     * particle indices and walks, in list order, the pairs that touch it
+     * (sub[t]), updating only its own particles */
+    int nthreads;
+    int64_t **sub, *sub_n, *sub_cap;
 };
 
 /* ------------------------------------------------------------------ utils */
@@ -148,7 +165,90 @@ static void free_all(struct sps_handle *h)
                     h->stress_tmp, h->ivar_tmp, h->av_tmp, h->x10, h->disp10, h->n_int, h->bc_int,
                     h->bc_info, h->bc_or_not, h->normal, h->subset};
     for (size_t k = 0; k < sizeof ptrs / sizeof ptrs[0]; ++k) free(ptrs[k]);
+    if (h->sub)
+        for (int t = 0; t < h->nthreads; ++t) free(h->sub[t]);
+    free(h->sub);
+    free(h->sub_n);
+    free(h->sub_cap);
 }
+
+/* ------------------------------------------------------- threads */
+/* The restatement runs on one thread as the reference does.  With several
+ * (OpenMP), every pair loop keeps the reference's order for each particle:
+ * thread t owns the particles [lo_t, hi_t) and walks the pairs that touch
+ * them in list order, applying only the updates of its own particles, so
+ * each particle receives the same terms in the same order as on one thread
+ * and the results are the same bit for bit; loops over particles are split
+ * between the threads.  The pair lists are the search's, gathered per
+ * thread once per step (pair_sublists). */
+static inline int own_lo(const struct sps_handle *h, int t) { return (int)((int64_t)h->nt2 * t / h->nthreads); }
+static inline int own_hi(const struct sps_handle *h, int t) { return (int)((int64_t)h->nt2 * (t + 1) / h->nthreads); }
+
+static int pair_sublists(struct sps_handle *h)
+{
+    if (h->nthreads <= 1) return 0;
+    if (!h->sub) {
+        h->sub = calloc((size_t)h->nthreads, sizeof *h->sub);
+        h->sub_n = calloc((size_t)h->nthreads, sizeof *h->sub_n);
+        h->sub_cap = calloc((size_t)h->nthreads, sizeof *h->sub_cap);
+        if (!h->sub || !h->sub_n || !h->sub_cap) return fail(SPS_ENOMEM, "thread pair lists");
+    }
+    int bad = 0;
+#pragma omp parallel num_threads(h->nthreads) reduction(| : bad)
+    {
+        const int t = omp_get_thread_num(), lo = own_lo(h, t), hi = own_hi(h, t);
+        int64_t n = 0;
+        for (int64_t q = 0; q < h->n_pairs; ++q) {
+            const int64_t d = h->order[q];
+            const int i = h->pi[d], j = h->pj[d];
+            n += (i >= lo && i < hi) || (j >= lo && j < hi);
+        }
+        if (n > h->sub_cap[t]) {
+            int64_t *a = realloc(h->sub[t], (size_t)(n ? n : 1) * sizeof *a);
+            if (a) {
+                h->sub[t] = a;
+                h->sub_cap[t] = n;
+            } else {
+                bad = 1;
+            }
+        }
+        if (n <= h->sub_cap[t]) {
+            int64_t k = 0;
+            for (int64_t q = 0; q < h->n_pairs; ++q) {
+                const int64_t d = h->order[q];
+                const int i = h->pi[d], j = h->pj[d];
+                if ((i >= lo && i < hi) || (j >= lo && j < hi)) h->sub[t][k++] = d;
+            }
+            h->sub_n[t] = k;
+        }
+    }
+    return bad ? fail(SPS_ENOMEM, "thread pair lists") : 0;
+}
+
+/* Run body(h, d, lo, hi, state) over the pairs in list order: on one thread
+ * every pair with the whole index range, on several each thread its own
+ * list and range. */
+#define PAIR_LOOP(h, body, STATE_T, state_init)                                              \
+    do {                                                                                     \
+        if ((h)->nthreads <= 1) {                                                            \
+            STATE_T st_ = state_init;                                                        \
+            for (int64_t q_ = 0; q_ < (h)->n_pairs; ++q_) body((h), (h)->order[q_], 0, (h)->nt2, &st_); \
+        } else {                                                                             \
+            _Pragma("omp parallel num_threads((h)->nthreads)")                              \
+            {                                                                                \
+                const int t_ = omp_get_thread_num();                                         \
+                const int lo_ = own_lo((h), t_), hi_ = own_hi((h), t_);                      \
+                STATE_T st_ = state_init;                                                    \
+                for (int64_t k_ = 0; k_ < (h)->sub_n[t_]; ++k_) body((h), (h)->sub[t_][k_], lo_, hi_, &st_); \
+            }                                                                                \
+        }                                                                                    \
+    } while (0)
+
+static inline int owns(int p, int lo, int hi) { return p >= lo && p < hi; }
+
+/* a loop over particles split between the threads (each iteration writes
+ * only its own particle's values) */
+#define OMP_FOR _Pragma("omp parallel for num_threads(h->nthreads > 0 ? h->nthreads : 1)")
 
 /* ------------------------------------------------------- Check_Out_Domain */
 /* code/2_SPH_main_2018.f90:1170-1194 (sticky flag).  A non-finite
@@ -157,6 +257,7 @@ static void free_all(struct sps_handle *h)
  * undefined): the search never sees it. */
 static void check_out_domain(struct sps_handle *h)
 {
+#pragma omp parallel for num_threads(h->nthreads > 0 ? h->nthreads : 1)
     for (int i = 0; i < h->nt2; ++i)
         for (int d = 0; d < 2; ++d) {
             double dxx = (h->x[2 * i + d] - h->p.xmin_domain[d]) *
@@ -190,29 +291,73 @@ static void kernel_cubic(double r, const double dxv[2], double hs, double pi,
 
 /* ---------------------------------------------------------- grid_find_NEW */
 /* code/2_SPH_main_2018.f90:1199-1435 */
-static int push_pair(struct sps_handle *h, int it, int jt, double w, const double dw[2])
+static int reserve_pairs(struct sps_handle *h, int64_t n)
 {
-    if (h->n_pairs == h->pcap) {
-        int64_t nc = h->pcap ? h->pcap * 2 : 4096;
-        int32_t *a = realloc(h->pi, nc * sizeof *a), *b, *c;
-        if (!a) return -1;
-        h->pi = a;
-        b = realloc(h->pj, nc * sizeof *b); if (!b) return -1; h->pj = b;
-        c = realloc(h->pt, nc * sizeof *c); if (!c) return -1; h->pt = c;
-        float *f1 = realloc(h->pw, nc * sizeof *f1); if (!f1) return -1; h->pw = f1;
-        float *f2 = realloc(h->pgx, nc * sizeof *f2); if (!f2) return -1; h->pgx = f2;
-        float *f3 = realloc(h->pgy, nc * sizeof *f3); if (!f3) return -1; h->pgy = f3;
-        int64_t *o = realloc(h->order, nc * sizeof *o); if (!o) return -1; h->order = o;
-        h->pcap = nc;
-    }
-    int64_t k = h->n_pairs++;
-    h->pi[k] = it;
-    h->pj[k] = jt;
-    h->pt[k] = 0;               /* 2_...:1374 */
-    h->pw[k] = (float)w;        /* FP32 pair fields, 5_...:19-21 */
-    h->pgx[k] = (float)dw[0];
-    h->pgy[k] = (float)dw[1];
+    if (n <= h->pcap) return 0;
+    int64_t nc = h->pcap ? h->pcap : 4096;
+    while (nc < n) nc *= 2;
+    int32_t *a = realloc(h->pi, nc * sizeof *a), *b, *c;
+    if (!a) return -1;
+    h->pi = a;
+    b = realloc(h->pj, nc * sizeof *b); if (!b) return -1; h->pj = b;
+    c = realloc(h->pt, nc * sizeof *c); if (!c) return -1; h->pt = c;
+    float *f1 = realloc(h->pw, nc * sizeof *f1); if (!f1) return -1; h->pw = f1;
+    float *f2 = realloc(h->pgx, nc * sizeof *f2); if (!f2) return -1; h->pgx = f2;
+    float *f3 = realloc(h->pgy, nc * sizeof *f3); if (!f3) return -1; h->pgy = f3;
+    int64_t *o = realloc(h->order, nc * sizeof *o); if (!o) return -1; h->order = o;
+    h->pcap = nc;
     return 0;
+}
+
+/* Task 3 of the search (2_...:1309-1401) for the cells of one row idy
+ * (1-based): the half-stencil pairs of each particle of each cell of the
+ * row, in the reference's order.  Counts them, and with `write` stores them
+ * from index `base` on. */
+static int64_t row_pairs(struct sps_handle *h, int idy, const int ndiv[2], const int *cnt, const int *start,
+                         int write, int64_t base)
+{
+    int64_t k = base;
+    for (int idx = 1; idx <= ndiv[0]; ++idx) {
+        int idivt = ndiv[0] * (idy - 1) + idx;
+        int npicsi = cnt[idivt - 1];
+        for (int ii = 1; ii <= npicsi; ++ii) {
+            int it = h->cell_list[start[idivt - 1] + ii - 1];
+            int idx0 = idx - 1 > 1 ? idx - 1 : 1;
+            int idx1 = idx + 1 < ndiv[0] ? idx + 1 : ndiv[0];
+            int idy1 = idy + 1 < ndiv[1] ? idy + 1 : ndiv[1];
+            for (int jdy = idy; jdy <= idy1; ++jdy)
+                for (int jdx = idx0; jdx <= idx1; ++jdx) {
+                    int jdivt = ndiv[0] * (jdy - 1) + jdx;
+                    if (jdivt < idivt) continue;
+                    int jj = (idivt == jdivt) ? ii + 1 : 1;
+                    int npicsj = cnt[jdivt - 1];
+                    for (int jc = jj; jc <= npicsj; ++jc) {
+                        int jt = h->cell_list[start[jdivt - 1] + jc - 1];
+                        double dxv[2];
+                        dxv[0] = h->x[2 * it] - h->x[2 * jt];
+                        double driac = dxv[0] * dxv[0];
+                        dxv[1] = h->x[2 * it + 1] - h->x[2 * jt + 1];
+                        driac = driac + dxv[1] * dxv[1];
+                        double mhsml = (h->hsml[it] + h->hsml[jt]) / 2.0;
+                        double r = sqrt(driac);
+                        if (r < 2 * mhsml) {
+                            if (write) {
+                                double w, dw[2];
+                                kernel_cubic(r, dxv, mhsml, h->p.pi, &w, dw);
+                                h->pi[k] = it;
+                                h->pj[k] = jt;
+                                h->pt[k] = 0;               /* 2_...:1374 */
+                                h->pw[k] = (float)w;        /* FP32 pair fields, 5_...:19-21 */
+                                h->pgx[k] = (float)dw[0];
+                                h->pgy[k] = (float)dw[1];
+                            }
+                            ++k;
+                        }
+                    }
+                }
+        }
+    }
+    return k - base;
 }
 
 static int grid_find(struct sps_handle *h, int itimestep)
@@ -277,43 +422,25 @@ static int grid_find(struct sps_handle *h, int itimestep)
         int c = h->which_cell[i] - 1;
         h->cell_list[start[c] + fill[c]++] = i;
     }
-    /* Task 3 (2_...:1309-1401): half-stencil enumeration */
+    /* Task 3 (2_...:1309-1401): half-stencil enumeration, row by row of
+     * cells; a row's pairs follow the rows before it, so the rows are counted
+     * first and then written from their offsets, in any thread order */
     if (itimestep == 1) h->m_pairs = 0;   /* fresh list, 2_...:1290-1294 */
-    h->n_pairs = 0;
-    for (int idy = 1; idy <= ndiv[1]; ++idy)
-        for (int idx = 1; idx <= ndiv[0]; ++idx) {
-            int idivt = ndiv[0] * (idy - 1) + idx;
-            int npicsi = cnt[idivt - 1];
-            for (int ii = 1; ii <= npicsi; ++ii) {
-                int it = h->cell_list[start[idivt - 1] + ii - 1];
-                int idx0 = idx - 1 > 1 ? idx - 1 : 1;
-                int idx1 = idx + 1 < ndiv[0] ? idx + 1 : ndiv[0];
-                int idy1 = idy + 1 < ndiv[1] ? idy + 1 : ndiv[1];
-                for (int jdy = idy; jdy <= idy1; ++jdy)
-                    for (int jdx = idx0; jdx <= idx1; ++jdx) {
-                        int jdivt = ndiv[0] * (jdy - 1) + jdx;
-                        if (jdivt < idivt) continue;
-                        int jj = (idivt == jdivt) ? ii + 1 : 1;
-                        int npicsj = cnt[jdivt - 1];
-                        for (int jc = jj; jc <= npicsj; ++jc) {
-                            int jt = h->cell_list[start[jdivt - 1] + jc - 1];
-                            double dxv[2];
-                            dxv[0] = h->x[2 * it] - h->x[2 * jt];
-                            double driac = dxv[0] * dxv[0];
-                            dxv[1] = h->x[2 * it + 1] - h->x[2 * jt + 1];
-                            driac = driac + dxv[1] * dxv[1];
-                            double mhsml = (h->hsml[it] + h->hsml[jt]) / 2.0;
-                            double r = sqrt(driac);
-                            if (r < 2 * mhsml) {
-                                double w, dw[2];
-                                kernel_cubic(r, dxv, mhsml, h->p.pi, &w, dw);
-                                if (push_pair(h, it, jt, w, dw))
-                                    return fail(SPS_ENOMEM, "pair list allocation");
-                            }
-                        }
-                    }
-            }
-        }
+    const int nth = h->nthreads > 0 ? h->nthreads : 1;
+    int64_t *row_off = malloc(((size_t)ndiv[1] + 1) * sizeof *row_off);
+    if (!row_off) return fail(SPS_ENOMEM, "pair list allocation");
+    row_off[0] = 0;
+#pragma omp parallel for num_threads(nth) schedule(dynamic, 4)
+    for (int idy = 1; idy <= ndiv[1]; ++idy) row_off[idy] = row_pairs(h, idy, ndiv, cnt, start, 0, 0);
+    for (int idy = 1; idy <= ndiv[1]; ++idy) row_off[idy] += row_off[idy - 1];
+    if (reserve_pairs(h, row_off[ndiv[1]])) {
+        free(row_off);
+        return fail(SPS_ENOMEM, "pair list allocation");
+    }
+#pragma omp parallel for num_threads(nth) schedule(dynamic, 4)
+    for (int idy = 1; idy <= ndiv[1]; ++idy) row_pairs(h, idy, ndiv, cnt, start, 1, row_off[idy - 1]);
+    h->n_pairs = row_off[ndiv[1]];
+    free(row_off);
     /* traversal order of the reused linked list (2_...:1309-1319, 1361-1369,
      * 1380-1383, 1398-1401): with M the high-water mark before this search,
      * the list from `last` is [n, n-1, ..., M+1, 1, 2, ..., min(n, M)]. */
@@ -323,7 +450,8 @@ static int grid_find(struct sps_handle *h, int itimestep)
         for (int64_t d = 0; d < M; ++d) h->order[k++] = d;
         h->m_pairs = n;
     } else {
-        for (int64_t d = 0; d < n; ++d) h->order[k++] = d;
+#pragma omp parallel for num_threads(nth)
+        for (int64_t d = 0; d < n; ++d) h->order[d] = d;
     }
     return 0;
 }
@@ -332,6 +460,7 @@ static int grid_find(struct sps_handle *h, int itimestep)
 /* code/3_SPH_material_2018.f90:1555-1615 */
 static void pint_update(struct sps_handle *h)
 {
+#pragma omp parallel for num_threads(h->nthreads > 0 ? h->nthreads : 1)
     for (int64_t k = 0; k < h->n_pairs; ++k) {
         int i = h->pi[k], j = h->pj[k];
         int ii = abs(h->itype[i]), jj = abs(h->itype[j]);
@@ -403,11 +532,35 @@ static void mat_D(const struct sps_handle *h, int i, double D[6])
 /* all of 1..ntotal */
 static void adapt_stress2(struct sps_handle *h)
 {
+#pragma omp parallel for num_threads(h->nthreads > 0 ? h->nthreads : 1)
     for (int i = 0; i < h->nt; ++i) adapt_one(h->stress + 4 * i, mat_tanfi(h, i), mat_coh(h, i));
 }
 
 /* ---------------------------------------------------- stress_point_update */
 /* code/2_SPH_main_2018.f90:403-482, SP_SPH branch */
+/* one pair of stress_point_update's loop: the stress point's and the
+ * node's sums, each where this thread owns the particle */
+static inline void spu_pair(struct sps_handle *h, int64_t d, int lo, int hi, int *unused)
+{
+    (void)unused;
+    if (h->pt[d] != 1) return;
+    const int j = h->pi[d]; /* stress point */
+    const int i = h->pj[d]; /* node */
+    const double w = (double)h->pw[d];
+    if (owns(j, lo, hi)) {
+        const double h2 = (h->mass[i] / h->rho[i]) * w;
+        for (int a = 0; a < 2; ++a) h->vel_tmp[2 * j + a] = h->vel_tmp[2 * j + a] + h->vel[2 * i + a] * h2;
+        h->cspm_norm[j] = h->cspm_norm[j] + (w * h->mass[i]) / h->rho[i];
+    }
+    if (owns(i, lo, hi)) {
+        const double h1 = (h->mass[j] / h->rho[j]) * w;
+        for (int s = 0; s < 4; ++s)
+            h->stress_tmp[4 * i + s] = h->stress_tmp[4 * i + s] + h->stress[4 * j + s] * h1;
+        h->ivar_tmp[i] = h->ivar_tmp[i] + h->ivars[(size_t)h->niv * j] * h1;
+        h->cspm_norm[i] = h->cspm_norm[i] + (w * h->mass[j]) / h->rho[j];
+    }
+}
+
 static void stress_point_update(struct sps_handle *h)
 {
     const int nn = h->nn, nt = h->nt;
@@ -415,26 +568,14 @@ static void stress_point_update(struct sps_handle *h)
     memset(h->vel_tmp, 0, 2 * (size_t)nt * sizeof(double));
     memset(h->ivar_tmp, 0, (size_t)nt * sizeof(double));
     memset(h->cspm_norm, 0, (size_t)nt * sizeof(double));
-    for (int64_t k = 0; k < h->n_pairs; ++k) {
-        int64_t d = h->order[k];
-        if (h->pt[d] != 1) continue;
-        int j = h->pi[d]; /* stress point */
-        int i = h->pj[d]; /* node */
-        double w = (double)h->pw[d];
-        double h1 = (h->mass[j] / h->rho[j]) * w;
-        double h2 = (h->mass[i] / h->rho[i]) * w;
-        for (int a = 0; a < 2; ++a) h->vel_tmp[2 * j + a] = h->vel_tmp[2 * j + a] + h->vel[2 * i + a] * h2;
-        for (int s = 0; s < 4; ++s)
-            h->stress_tmp[4 * i + s] = h->stress_tmp[4 * i + s] + h->stress[4 * j + s] * h1;
-        h->ivar_tmp[i] = h->ivar_tmp[i] + h->ivars[(size_t)h->niv * j] * h1;
-        h->cspm_norm[j] = h->cspm_norm[j] + (w * h->mass[i]) / h->rho[i];
-        h->cspm_norm[i] = h->cspm_norm[i] + (w * h->mass[j]) / h->rho[j];
-    }
+    PAIR_LOOP(h, spu_pair, int, 0);
+#pragma omp parallel for num_threads(h->nthreads > 0 ? h->nthreads : 1)
     for (int i = nn; i < nt; ++i) {
         if (h->cspm_norm[i] == 0) continue;
         h->vel[2 * i] = h->vel_tmp[2 * i] / h->cspm_norm[i];
         h->vel[2 * i + 1] = h->vel_tmp[2 * i + 1] / h->cspm_norm[i];
     }
+#pragma omp parallel for num_threads(h->nthreads > 0 ? h->nthreads : 1)
     for (int i = 0; i < nn; ++i) {
         if (h->cspm_norm[i] != 0) {
             for (int s = 0; s < 4; ++s) h->stress[4 * i + s] = h->stress_tmp[4 * i + s] / h->cspm_norm[i];
@@ -453,79 +594,102 @@ static void stress_point_update(struct sps_handle *h)
 #define G1(a, b, p) h->grad1[4 * (p) + 2 * (b) + (a)] /* grad1_tmp(a,b,p) */
 #define G2(s, b, p) h->grad2[6 * (p) + 3 * (b) + (s)] /* grad2_tmp(s,b,p) */
 #define AE(c, p) h->ae[5 * (p) + (c)]
-static void get_derivatives(struct sps_handle *h)
+/* one pair of get_derivatives' loop (2_...:507-590), each end's sums where
+ * this thread owns it.  The type-6 pair sets the dummy's stress to the
+ * node's and then reads it: the node's sum reads the node's own, the same
+ * value; the type-9 pair's mirrored dummy velocity is the pair's own. */
+static inline void deriv_pair(struct sps_handle *h, int64_t d, int lo, int hi, int *unused)
 {
-    const int nn = h->nn, nt = h->nt;
+    (void)unused;
     const double *x = h->x, *m = h->mass, *rho = h->rho;
     double *vel = h->vel, *st = h->stress;
-    memset(h->grad1, 0, 4 * (size_t)nt * sizeof(double));
-    memset(h->grad2, 0, 6 * (size_t)nt * sizeof(double));
-    memset(h->divf1, 0, 4 * (size_t)nt * sizeof(double));
-    memset(h->divf2, 0, 2 * (size_t)nt * sizeof(double));
-    memset(h->ae, 0, 5 * (size_t)nt * sizeof(double));
-    memset(h->dummy_vel, 0, 2 * (size_t)h->nt2 * sizeof(double));
-    for (int64_t k = 0; k < h->n_pairs; ++k) {
-        int64_t d = h->order[k];
-        int j = h->pi[d], i = h->pj[d];
-        double gx = (double)h->pgx[d], gy = (double)h->pgy[d];
-        double h1 = gx * m[i] / rho[i], h2 = 0.0;
-        int t = h->pt[d];
-        if (t == 1) {
+    const int j = h->pi[d], i = h->pj[d];
+    const double gx = (double)h->pgx[d], gy = (double)h->pgy[d];
+    double h1 = gx * m[i] / rho[i], h2 = 0.0;
+    const int t = h->pt[d];
+    if (t == 1) {
+        if (owns(j, lo, hi)) {
             for (int a = 0; a < 2; ++a) {
                 G1(a, 0, j) = G1(a, 0, j) + (vel[2 * i + a] - vel[2 * j + a]) * h1;
                 h2 = gy * m[i] / rho[i];
                 G1(a, 1, j) = G1(a, 1, j) + (vel[2 * i + a] - vel[2 * j + a]) * h2;
             }
+        }
+        if (owns(i, lo, hi))
             for (int s = 0; s < 3; ++s) {
                 double g1 = gx * (st[4 * i + s] / (rho[i] * rho[i]) + st[4 * j + s] / (rho[j] * rho[j]));
                 G2(s, 0, i) = G2(s, 0, i) - m[j] * g1;
                 double g2 = gy * (st[4 * i + s] / (rho[i] * rho[i]) + st[4 * j + s] / (rho[j] * rho[j]));
                 G2(s, 1, i) = G2(s, 1, i) - m[j] * g2;
             }
-            if (h->p.cspm) {
-                double h1b = -(gx * m[j] / rho[j]);
-                double h2b = -(gy * m[j] / rho[j]);
+        if (h->p.cspm) {
+            h2 = gy * m[i] / rho[i];
+            if (owns(j, lo, hi)) {
                 AE(0, j) = AE(0, j) + (x[2 * i] - x[2 * j]) * h1;
                 AE(1, j) = AE(1, j) + (x[2 * i + 1] - x[2 * j + 1]) * h1;
                 AE(2, j) = AE(2, j) + (x[2 * i] - x[2 * j]) * h2;
                 AE(3, j) = AE(3, j) + (x[2 * i + 1] - x[2 * j + 1]) * h2;
+            }
+            if (owns(i, lo, hi)) {
+                double h1b = -(gx * m[j] / rho[j]);
+                double h2b = -(gy * m[j] / rho[j]);
                 AE(0, i) = AE(0, i) + (x[2 * j] - x[2 * i]) * h1b;
                 AE(1, i) = AE(1, i) + (x[2 * j + 1] - x[2 * i + 1]) * h1b;
                 AE(2, i) = AE(2, i) + (x[2 * j] - x[2 * i]) * h2b;
                 AE(3, i) = AE(3, i) + (x[2 * j + 1] - x[2 * i + 1]) * h2b;
             }
-        } else if (t == 9) { /* i: stress point, j: dummy */
-            double beta_max = 1.5, vel_wall = 0.0, da, db;
-            double wall = (double)h->wallpos[j];
-            if (h->horiz[j] == 1.0f) {
-                da = fabs(x[2 * i + 1] - wall);
-                db = fabs(x[2 * j + 1] - wall);
-            } else {
-                da = fabs(x[2 * i] - wall);
-                db = fabs(x[2 * j] - wall);
-            }
-            double bt = 1 + (db / da);
-            double beta = beta_max < bt ? beta_max : bt; /* min(beta_max, 1+db/da) */
-            for (int a = 0; a < 2; ++a)
-                h->dummy_vel[2 * j + a] = vel[2 * i + a] * (1 - beta) + beta * vel_wall;
+        }
+    } else if (t == 9) { /* i: stress point, j: dummy */
+        double beta_max = 1.5, vel_wall = 0.0, da, db;
+        double wall = (double)h->wallpos[j];
+        if (h->horiz[j] == 1.0f) {
+            da = fabs(x[2 * i + 1] - wall);
+            db = fabs(x[2 * j + 1] - wall);
+        } else {
+            da = fabs(x[2 * i] - wall);
+            db = fabs(x[2 * j] - wall);
+        }
+        double bt = 1 + (db / da);
+        double beta = beta_max < bt ? beta_max : bt; /* min(beta_max, 1+db/da) */
+        double dv[2];
+        for (int a = 0; a < 2; ++a) dv[a] = vel[2 * i + a] * (1 - beta) + beta * vel_wall;
+        if (owns(j, lo, hi))
+            for (int a = 0; a < 2; ++a) h->dummy_vel[2 * j + a] = dv[a];
+        if (owns(i, lo, hi)) {
             h1 = gx * m[j] / rho[j];
             for (int a = 0; a < 2; ++a) {
-                G1(a, 0, i) = G1(a, 0, i) + (vel[2 * i + a] - h->dummy_vel[2 * j + a]) * h1;
+                G1(a, 0, i) = G1(a, 0, i) + (vel[2 * i + a] - dv[a]) * h1;
                 h2 = gy * m[j] / rho[j];
-                G1(a, 1, i) = G1(a, 1, i) + (vel[2 * i + a] - h->dummy_vel[2 * j + a]) * h2;
-            }
-        } else if (t == 6) { /* i: node, j: dummy */
-            for (int s = 0; s < 4; ++s) st[4 * j + s] = st[4 * i + s];
-            for (int s = 0; s < 3; ++s) {
-                double a1 = gx * (st[4 * i + s] / (rho[i] * rho[i]) + st[4 * j + s] / (rho[j] * rho[j]));
-                G2(s, 0, i) = G2(s, 0, i) - m[j] * a1;
-                double a2 = gy * (st[4 * i + s] / (rho[i] * rho[i]) + st[4 * j + s] / (rho[j] * rho[j]));
-                G2(s, 1, i) = G2(s, 1, i) - m[j] * a2;
+                G1(a, 1, i) = G1(a, 1, i) + (vel[2 * i + a] - dv[a]) * h2;
             }
         }
+    } else if (t == 6) { /* i: node, j: dummy */
+        if (owns(j, lo, hi))
+            for (int s = 0; s < 4; ++s) st[4 * j + s] = st[4 * i + s];
+        if (owns(i, lo, hi))
+            for (int s = 0; s < 3; ++s) {
+                const double sj = st[4 * i + s];   /* the dummy's stress, just set to the node's */
+                double a1 = gx * (st[4 * i + s] / (rho[i] * rho[i]) + sj / (rho[j] * rho[j]));
+                G2(s, 0, i) = G2(s, 0, i) - m[j] * a1;
+                double a2 = gy * (st[4 * i + s] / (rho[i] * rho[i]) + sj / (rho[j] * rho[j]));
+                G2(s, 1, i) = G2(s, 1, i) - m[j] * a2;
+            }
     }
+}
+
+static void get_derivatives(struct sps_handle *h)
+{
+    const int nn = h->nn, nt = h->nt;
+    memset(h->grad1, 0, 4 * (size_t)nt * sizeof(double));
+    memset(h->grad2, 0, 6 * (size_t)nt * sizeof(double));
+    memset(h->divf1, 0, 4 * (size_t)nt * sizeof(double));
+    memset(h->divf2, 0, 2 * (size_t)nt * sizeof(double));
+    memset(h->ae, 0, 5 * (size_t)nt * sizeof(double));
+    memset(h->dummy_vel, 0, 2 * (size_t)h->nt2 * sizeof(double));
+    PAIR_LOOP(h, deriv_pair, int, 0);
     if (h->p.cspm) { /* 2_...:594-604 */
         double tol = h->p.cspm_det_tol;
+#pragma omp parallel for num_threads(h->nthreads > 0 ? h->nthreads : 1)
         for (int i = 0; i < nt; ++i) {
             AE(4, i) = AE(0, i) * AE(3, i) - AE(1, i) * AE(2, i);
             if (fabs(AE(4, i)) < tol) {
@@ -543,6 +707,7 @@ static void get_derivatives(struct sps_handle *h)
             for (int p = 0; p < nn; ++p) G2(a, 1, p) = AE(4, p) * (AE(2, p) * G2(a, 0, p) + AE(3, p) * G2(a, 1, p));
         }
     }
+#pragma omp parallel for num_threads(h->nthreads > 0 ? h->nthreads : 1)
     for (int p = nn; p < nt; ++p) { /* 2_...:633-636 */
         double D[6]; /* D11 D22 D12 D33 D41 D42 */
         mat_D(h, p, D);
@@ -751,6 +916,7 @@ static double gravity_factor(const sps_params *P, double time_sph)
 
 static void gravity_force(struct sps_handle *h, double factg)
 {
+    OMP_FOR
     for (int i = 0; i < h->nn; ++i)
         for (int a = 0; a < 2; ++a) {
             double g = h->p.ic_grav ? factg * h->p.cgrav[a] : 0.0;
@@ -829,8 +995,9 @@ static void boundary_forces(struct sps_handle *h)
     const float d0 = bf_d0(h);
     const float scale = bf_term_scale();
     const double reach = h->bw_cell;
-    int cand[4096];
+#pragma omp parallel for num_threads(h->nthreads > 0 ? h->nthreads : 1) schedule(dynamic, 256)
     for (int i = 0; i < h->nn; ++i) {
+        int cand[4096];
         double xi = h->x[2 * i], yi = h->x[2 * i + 1];
         int cx = (int)floor((xi - h->bw_x0) / reach), cy = (int)floor((yi - h->bw_y0) / reach);
         int nc = 0;
@@ -867,37 +1034,45 @@ static void boundary_forces(struct sps_handle *h)
 
 /* --------------------------------------------------- artificial_viscosity */
 /* code/2_SPH_main_2018.f90:826-904, FP32 locals and accumulators */
+static inline void av_pair(struct sps_handle *h, int64_t d, int lo, int hi, float *visc)
+{
+    if (h->pt[d] != 3) return;
+    const double *x = h->x, *v = h->vel, *m = h->mass;
+    const float c = 600;
+    const int i = h->pi[d], j = h->pj[d];
+    if (!owns(i, lo, hi) && !owns(j, lo, hi)) return;
+    float xij = (float)(x[2 * i] - x[2 * j]);
+    float yij = (float)(x[2 * i + 1] - x[2 * j + 1]);
+    float hh = (float)(0.5 * (h->hsml[i] + h->hsml[j]));
+    float rho2 = (float)(0.5 * (h->rho[i] + h->rho[j]));
+    float div_u = (float)((double)xij * (v[2 * i] - v[2 * j]));
+    div_u = (float)((double)div_u + (double)yij * (v[2 * i + 1] - v[2 * j + 1]));
+    float sr = sqrtf(xij * xij + yij * yij);
+    float theta = (hh * div_u) / (sr * sr + 0.01f * (hh * hh));
+    if (div_u < 0) {
+        *visc = (float)((-(h->p.alpha * (double)c * (double)theta) + h->p.beta * (double)(theta * theta)) /
+                        (double)rho2);
+    } else if (div_u >= 0) {
+        *visc = 0;
+    }
+    float gx = h->pgx[d], gy = h->pgy[d];
+    if (owns(i, lo, hi)) {
+        h->av_tmp[2 * i] = (float)((double)h->av_tmp[2 * i] + (double)(*visc * gx) * m[j]);
+        h->av_tmp[2 * i + 1] = (float)((double)h->av_tmp[2 * i + 1] + (double)(*visc * gy) * m[j]);
+    }
+    if (owns(j, lo, hi)) {
+        h->av_tmp[2 * j] = (float)((double)h->av_tmp[2 * j] - (double)(*visc * gx) * m[i]);
+        h->av_tmp[2 * j + 1] = (float)((double)h->av_tmp[2 * j + 1] - (double)(*visc * gy) * m[i]);
+    }
+}
+
 static void artificial_viscosity(struct sps_handle *h)
 {
     const int nn = h->nn;
-    const double *x = h->x, *v = h->vel, *m = h->mass;
     memset(h->av_tmp, 0, 2 * (size_t)nn * sizeof(float));
-    float visc = 0.0f;
-    const float c = 600;
-    for (int64_t k = 0; k < h->n_pairs; ++k) {
-        int64_t d = h->order[k];
-        if (h->pt[d] != 3) continue;
-        int i = h->pi[d], j = h->pj[d];
-        float xij = (float)(x[2 * i] - x[2 * j]);
-        float yij = (float)(x[2 * i + 1] - x[2 * j + 1]);
-        float hh = (float)(0.5 * (h->hsml[i] + h->hsml[j]));
-        float rho2 = (float)(0.5 * (h->rho[i] + h->rho[j]));
-        float div_u = (float)((double)xij * (v[2 * i] - v[2 * j]));
-        div_u = (float)((double)div_u + (double)yij * (v[2 * i + 1] - v[2 * j + 1]));
-        float sr = sqrtf(xij * xij + yij * yij);
-        float theta = (hh * div_u) / (sr * sr + 0.01f * (hh * hh));
-        if (div_u < 0) {
-            visc = (float)((-(h->p.alpha * (double)c * (double)theta) + h->p.beta * (double)(theta * theta)) /
-                           (double)rho2);
-        } else if (div_u >= 0) {
-            visc = 0;
-        }
-        float gx = h->pgx[d], gy = h->pgy[d];
-        h->av_tmp[2 * i] = (float)((double)h->av_tmp[2 * i] + (double)(visc * gx) * m[j]);
-        h->av_tmp[2 * j] = (float)((double)h->av_tmp[2 * j] - (double)(visc * gx) * m[i]);
-        h->av_tmp[2 * i + 1] = (float)((double)h->av_tmp[2 * i + 1] + (double)(visc * gy) * m[j]);
-        h->av_tmp[2 * j + 1] = (float)((double)h->av_tmp[2 * j + 1] - (double)(visc * gy) * m[i]);
-    }
+    /* visc carries over from the last pair with a finite divergence where a
+     * NaN divergence leaves both branches untaken (per thread) */
+    PAIR_LOOP(h, av_pair, float, 0.0f);
     for (int i = 0; i < 2 * nn; ++i) h->art_visc[i] = -(double)h->av_tmp[i];
 }
 
@@ -926,9 +1101,11 @@ static void rk4(struct sps_handle *h, double dt_sph, double time_sph)
     const int do_bforce = h->p.inside_approach && (h->p.bforce_gate ? 1 : h->p.dummy_nodes);
     for (int s = 0; s < 4; ++s) {
         const double c1 = f1rk[s] * (dt_sph);
+        OMP_FOR
         for (int i = 0; i < nn; ++i)
             for (int a = 0; a < 2; ++a)
                 h->vel[2 * i + a] = h->vel0[2 * i + a] + c1 * h->rhs2[2 * i + a];
+        OMP_FOR
         for (int i = nn; i < nt; ++i)
             for (int q = 0; q < 4; ++q)
                 h->stress[4 * i + q] = h->stress0[4 * i + q] + c1 * h->rhs1[4 * i + q];
@@ -938,6 +1115,7 @@ static void rk4(struct sps_handle *h, double dt_sph, double time_sph)
         if (ncrit == 12) adapt_stress2(h);
         normal_bcs(h, time_sph);
         get_derivatives(h);
+        OMP_FOR
         for (int j = nn; j < nt; ++j) {
             double Gs[4], der1;
             plastic_terms(h, j, time_sph, Gs, &der1);
@@ -948,6 +1126,7 @@ static void rk4(struct sps_handle *h, double dt_sph, double time_sph)
         if (do_bforce) boundary_forces(h);
         if (h->p.alpha > 0 || h->p.beta > 0) artificial_viscosity(h);
         if (h->p.update_x) { /* 2_...:751-757 with get_spin_rate_tensor 2_...:1021-1034 */
+            OMP_FOR
             for (int i = nn; i < nt; ++i) {
                 const double *gu = h->grad_u + 4 * i;
                 double om1 = 0.5 * (gu[2] - gu[1]);
@@ -958,23 +1137,29 @@ static void rk4(struct sps_handle *h, double dt_sph, double time_sph)
                 h->spin[4 * i + 2] = om2 * st[0] + om1 * st[1];
             }
         }
+        OMP_FOR
         for (int i = nn; i < nt; ++i)
             for (int q = 0; q < 4; ++q)
                 h->rhs1[4 * i + q] = -h->divf1[4 * i + q] + h->spin[4 * i + q] + h->source_stress[4 * i + q];
+        OMP_FOR
         for (int i = 0; i < nn; ++i)
             for (int a = 0; a < 2; ++a)
                 h->rhs2[2 * i + a] = -h->divf2[2 * i + a] + h->g_grav[2 * i + a] + h->art_visc[2 * i + a] +
                                      h->f_bound[2 * i + a] + 0.0;
+        OMP_FOR
         for (int i = 0; i < nn; ++i)
             for (int a = 0; a < 2; ++a)
                 h->rk_vel[2 * i + a] = h->rk_vel[2 * i + a] + f2rk[s] * h->rhs2[2 * i + a];
+        OMP_FOR
         for (int i = nn; i < nt; ++i)
             for (int q = 0; q < 4; ++q)
                 h->rk_stress[4 * i + q] = h->rk_stress[4 * i + q] + f2rk[s] * h->rhs1[4 * i + q];
     }
+    OMP_FOR
     for (int i = 0; i < nn; ++i)
         for (int a = 0; a < 2; ++a)
             h->vel[2 * i + a] = h->vel0[2 * i + a] + (dt_sph / 6) * h->rk_vel[2 * i + a];
+    OMP_FOR
     for (int i = nn; i < nt; ++i)
         for (int q = 0; q < 4; ++q)
             h->stress[4 * i + q] = h->stress0[4 * i + q] + (dt_sph / 6) * h->rk_stress[4 * i + q];
@@ -997,7 +1182,88 @@ static void rk4(struct sps_handle *h, double dt_sph, double time_sph)
     }
     if (ncrit == 12) adapt_stress2(h);
     normal_bcs(h, time_sph);
+    OMP_FOR
     for (int i = 0; i < nt; ++i) h->ddev[i] = h->rk_dev[i] / 6;
+}
+
+/* --------------------------------------------------------------- leapfrog */
+/* No counterpart in the reference (RK4 only, 2_...:653-802).  Kick-drift
+ * leapfrog with the node velocities at half steps, stresses and positions
+ * at whole steps: the node accelerations of the step's state (the RK4
+ * stage with c1 = 0, as its first stage computes them) kick the velocities
+ * by dt (dt / 2 on the first step from a state whose velocities are at a
+ * whole step); the stress points take the new velocities and their stress
+ * rate from them updates the stress by dt; the positions drift by dt with
+ * the new velocities (time_integration's position update, 2_...:140-147,
+ * with v instead of the trapezoid's half sum).  One pair-loop pass per half
+ * instead of RK4's four stages. */
+static void leapfrog(struct sps_handle *h, double dt_sph, double time_sph)
+{
+    const int nn = h->nn, nt = h->nt;
+    const int ncrit = (int)h->p.props[1];
+    const double kdt = h->lf_first ? 0.5 * dt_sph : dt_sph;
+    memcpy(h->vel0, h->vel, 2 * (size_t)nt * sizeof(double));
+    memcpy(h->stress0, h->stress, 4 * (size_t)nt * sizeof(double));
+    memset(h->rk_dev, 0, (size_t)nt * sizeof(double));
+    memset(h->source_stress, 0, 4 * (size_t)nt * sizeof(double));
+    memset(h->spin, 0, 4 * (size_t)nt * sizeof(double));
+    memset(h->art_visc, 0, 2 * (size_t)nn * sizeof(double));
+    memset(h->vel, 0, 2 * (size_t)h->nt2 * sizeof(double));
+    memset(h->stress, 0, 4 * (size_t)h->nt2 * sizeof(double));
+    const double factg = gravity_factor(&h->p, time_sph);
+    const int do_bforce = h->p.inside_approach && (h->p.bforce_gate ? 1 : h->p.dummy_nodes);
+    /* the state of the step (RK4's first stage) and its node accelerations */
+    for (int i = 0; i < nn; ++i)
+        for (int a = 0; a < 2; ++a) h->vel[2 * i + a] = h->vel0[2 * i + a] + 0.0;
+    for (int i = nn; i < nt; ++i)
+        for (int q = 0; q < 4; ++q) h->stress[4 * i + q] = h->stress0[4 * i + q] + 0.0;
+    if (ncrit == 12) adapt_stress2(h);
+    normal_bcs(h, time_sph);
+    stress_point_update(h);
+    if (ncrit == 12) adapt_stress2(h);
+    normal_bcs(h, time_sph);
+    get_derivatives(h);
+    gravity_force(h, factg);
+    if (do_bforce) boundary_forces(h);
+    if (h->p.alpha > 0 || h->p.beta > 0) artificial_viscosity(h);
+    for (int i = 0; i < nn; ++i)
+        for (int a = 0; a < 2; ++a) {
+            h->rhs2[2 * i + a] = -h->divf2[2 * i + a] + h->g_grav[2 * i + a] + h->art_visc[2 * i + a] +
+                                 h->f_bound[2 * i + a] + 0.0;
+            h->vel[2 * i + a] = h->vel0[2 * i + a] + kdt * h->rhs2[2 * i + a];   /* kick */
+        }
+    normal_bcs(h, time_sph);
+    /* the stress points' velocities and stress rates from the kicked ones */
+    stress_point_update(h);
+    if (ncrit == 12) adapt_stress2(h);
+    normal_bcs(h, time_sph);
+    get_derivatives(h);
+    for (int j = nn; j < nt; ++j) {
+        double Gs[4], der1;
+        plastic_terms(h, j, time_sph, Gs, &der1);
+        for (int q = 0; q < 4; ++q) h->source_stress[4 * j + q] = Gs[q];
+        h->rk_dev[j] = h->rk_dev[j] + der1 * 1.0;
+    }
+    if (h->p.update_x) {
+        for (int i = nn; i < nt; ++i) {
+            const double *gu = h->grad_u + 4 * i;
+            double om1 = 0.5 * (gu[2] - gu[1]);
+            double om2 = -(0.5 * (gu[2] - gu[1]));
+            const double *st = h->stress + 4 * i;
+            h->spin[4 * i + 0] = 2 * om1 * st[2];
+            h->spin[4 * i + 1] = 2 * om2 * st[2];
+            h->spin[4 * i + 2] = om2 * st[0] + om1 * st[1];
+        }
+    }
+    for (int i = nn; i < nt; ++i)
+        for (int q = 0; q < 4; ++q) {
+            h->rhs1[4 * i + q] = -h->divf1[4 * i + q] + h->spin[4 * i + q] + h->source_stress[4 * i + q];
+            h->stress[4 * i + q] = h->stress0[4 * i + q] + dt_sph * h->rhs1[4 * i + q];
+        }
+    if (ncrit == 12) adapt_stress2(h);
+    normal_bcs(h, time_sph);
+    for (int i = 0; i < nt; ++i) h->ddev[i] = h->rk_dev[i];
+    h->lf_first = 0;
 }
 
 /* ------------------------------------------------------------ Normal_BCs */
@@ -1039,6 +1305,7 @@ static void normal_bcs(struct sps_handle *h, double time_sph)
     if (h->p.no_bcs <= 0) return;
     double val[SPS_MAX_BCS];
     for (int b = 0; b < h->nbc; ++b) val[b] = bc_value(h, b, time_sph);
+    OMP_FOR
     for (int i = 0; i < h->nn; ++i) {
         const int32_t *info = h->bc_info + 8 * i;
         if (h->bc_or_not[i] != 1 || info[1] == 0) continue;
@@ -1057,6 +1324,7 @@ static void normal_bcs(struct sps_handle *h, double time_sph)
      * surface nodes without a wall-dummy partner keep only the tangential
      * stress of their normal frame; nodes whose normal is NaN are skipped */
     if (h->ifsigman != 1) return;
+    OMP_FOR
     for (int i = 0; i < h->nn; ++i) {
         if (h->bc_or_not[i] != 2 || h->bc_int[i] == 1) continue;
         const double costh = h->normal[2 * i], sinth = h->normal[2 * i + 1];
@@ -1104,6 +1372,89 @@ static void fs_hf(const struct sps_handle *h, int64_t k, float *h1i, float *h1j,
     *h2j = (float)(-(h->mass[i] * (double)h->pgy[k] / h->rho[i]));
 }
 
+/* the work arrays of the detection, and its three pair loops (each end's
+ * sums where this thread owns it) */
+typedef struct FsWork {
+    float *A, *ff, *tt, *tau, *fint, *gf, *nb;
+} FsWork;
+
+static inline void fs_pair1(struct sps_handle *h, int64_t k, int lo, int hi, FsWork *w)
+{
+    const int t = h->pt[k];
+    if (t != 1 && t != 2 && t != 3) return;
+    const int i = h->pi[k], j = h->pj[k];
+    const double *xi = h->x + 2 * (size_t)i, *xj = h->x + 2 * (size_t)j;
+    float h1i, h1j, h2i, h2j;
+    fs_hf(h, k, &h1i, &h1j, &h2i, &h2j);
+    if (owns(i, lo, hi)) {
+        float *ai = w->A + 4 * (size_t)i;
+        ai[0] = (float)((double)ai[0] + (xj[0] - xi[0]) * (double)h1i);
+        w->ff[2 * i] = w->ff[2 * i] + h1i;
+        ai[1] = (float)((double)ai[1] + (xj[1] - xi[1]) * (double)h1i);
+        ai[2] = (float)((double)ai[2] + (xj[0] - xi[0]) * (double)h2i);
+        ai[3] = (float)((double)ai[3] + (xj[1] - xi[1]) * (double)h2i);
+        w->ff[2 * i + 1] = w->ff[2 * i + 1] + h2i;
+    }
+    if (owns(j, lo, hi)) {
+        float *aj = w->A + 4 * (size_t)j;
+        aj[0] = (float)((double)aj[0] + (xi[0] - xj[0]) * (double)h1j);
+        w->ff[2 * j] = w->ff[2 * j] + h1j;
+        aj[1] = (float)((double)aj[1] + (xi[1] - xj[1]) * (double)h1j);
+        aj[2] = (float)((double)aj[2] + (xi[0] - xj[0]) * (double)h2j);
+        aj[3] = (float)((double)aj[3] + (xi[1] - xj[1]) * (double)h2j);
+        w->ff[2 * j + 1] = w->ff[2 * j + 1] + h2j;
+    }
+}
+
+static inline void fs_pair3(struct sps_handle *h, int64_t k, int lo, int hi, FsWork *w)
+{
+    const int t = h->pt[k], i = h->pi[k], j = h->pj[k];
+    int32_t *sub = h->subset;
+    if (t == 2 || t == 3) {
+        if (owns(i, lo, hi) && sub[i] == 0 && fs_hit(h, w->tt, w->tau, i, j)) { sub[i] = 2; w->fint[i] = -1; }
+        if (owns(j, lo, hi) && sub[j] == 0 && fs_hit(h, w->tt, w->tau, j, i)) { sub[j] = 2; w->fint[j] = -1; }
+    }
+    if ((t == 6 || t == 9) && owns(j, lo, hi) && sub[j] == 0 && fs_hit(h, w->tt, w->tau, j, i)) {
+        sub[j] = 2;
+        w->fint[j] = -1;
+    }
+}
+
+static inline void fs_pair4(struct sps_handle *h, int64_t k, int lo, int hi, FsWork *w)
+{
+    const int t = h->pt[k];
+    if (t != 1 && t != 2 && t != 3) return;
+    const int i = h->pi[k], j = h->pj[k];
+    const int32_t *sub = h->subset;
+    double *nrm = h->normal;
+    if (sub[i] == 1 && sub[j] == 1) {
+        const float x1 = (float)h->x[2 * i], y1 = (float)h->x[2 * i + 1];
+        const float x2 = (float)h->x[2 * j], y2 = (float)h->x[2 * j + 1];
+        const float xv = x2 - x1, yv = y2 - y1;
+        if (owns(i, lo, hi)) {
+            nrm[2 * i] = nrm[2 * i] - (double)yv;
+            nrm[2 * i + 1] = nrm[2 * i + 1] + (double)xv;
+            w->nb[i] = w->nb[i] + 1.0f;
+        }
+        if (owns(j, lo, hi)) {
+            nrm[2 * j] = nrm[2 * j] - (double)yv;
+            nrm[2 * j + 1] = nrm[2 * j + 1] + (double)xv;
+            w->nb[j] = w->nb[j] + 1.0f;
+        }
+    }
+    float h1i, h1j, h2i, h2j;
+    fs_hf(h, k, &h1i, &h1j, &h2i, &h2j);
+    const float *fint = w->fint;
+    if (owns(i, lo, hi)) {
+        w->gf[2 * i] = w->gf[2 * i] + (fint[j] - fint[i]) * h1i;
+        w->gf[2 * i + 1] = w->gf[2 * i + 1] + (fint[j] - fint[i]) * h2i;
+    }
+    if (owns(j, lo, hi)) {
+        w->gf[2 * j] = w->gf[2 * j] + (fint[i] - fint[j]) * h1j;
+        w->gf[2 * j + 1] = w->gf[2 * j + 1] + (fint[i] - fint[j]) * h2j;
+    }
+}
+
 static int free_surface(struct sps_handle *h)
 {
     const int nt = h->nt;
@@ -1121,29 +1472,10 @@ static int free_surface(struct sps_handle *h)
     memset(sub, 0, (size_t)nt * sizeof(int32_t));
     /* step 1 (3_...:1160-1225): renormalisation matrix and sum of m/rho grad W
      * over the node-stress point, stress point and node pairs, in list order */
-    for (int64_t q = 0; q < h->n_pairs; ++q) {
-        const int64_t k = h->order[q];
-        const int t = h->pt[k];
-        if (t != 1 && t != 2 && t != 3) continue;
-        const int i = h->pi[k], j = h->pj[k];
-        const double *xi = x + 2 * (size_t)i, *xj = x + 2 * (size_t)j;
-        float h1i, h1j, h2i, h2j;
-        fs_hf(h, k, &h1i, &h1j, &h2i, &h2j);
-        float *ai = A + 4 * (size_t)i, *aj = A + 4 * (size_t)j;
-        ai[0] = (float)((double)ai[0] + (xj[0] - xi[0]) * (double)h1i);
-        aj[0] = (float)((double)aj[0] + (xi[0] - xj[0]) * (double)h1j);
-        ff[2 * i] = ff[2 * i] + h1i;
-        ff[2 * j] = ff[2 * j] + h1j;
-        ai[1] = (float)((double)ai[1] + (xj[1] - xi[1]) * (double)h1i);
-        ai[2] = (float)((double)ai[2] + (xj[0] - xi[0]) * (double)h2i);
-        ai[3] = (float)((double)ai[3] + (xj[1] - xi[1]) * (double)h2i);
-        aj[1] = (float)((double)aj[1] + (xi[1] - xj[1]) * (double)h1j);
-        aj[2] = (float)((double)aj[2] + (xi[0] - xj[0]) * (double)h2j);
-        aj[3] = (float)((double)aj[3] + (xi[1] - xj[1]) * (double)h2j);
-        ff[2 * i + 1] = ff[2 * i + 1] + h2i;
-        ff[2 * j + 1] = ff[2 * j + 1] + h2j;
-    }
+    FsWork fw = {A, ff, tt, tau, fint, gf, nb};
+    PAIR_LOOP(h, fs_pair1, FsWork, fw);
     /* step 2 (3_...:1229-1245): first normal estimate; step 3's scan frame */
+#pragma omp parallel for num_threads(h->nthreads > 0 ? h->nthreads : 1)
     for (int i = 0; i < nt; ++i) {
         float *a = A + 4 * (size_t)i;
         for (int q = 0; q < 4; ++q)
@@ -1160,15 +1492,7 @@ static int free_surface(struct sps_handle *h)
     }
     /* step 3 (3_...:1247-1319): a particle with a partner in its scan region
      * is inside (subset 2, f_int -1); the rest are on the surface */
-    for (int64_t q = 0; q < h->n_pairs; ++q) {
-        const int64_t k = h->order[q];
-        const int t = h->pt[k], i = h->pi[k], j = h->pj[k];
-        if (t == 2 || t == 3) {
-            if (sub[i] == 0 && fs_hit(h, tt, tau, i, j)) { sub[i] = 2; fint[i] = -1; }
-            if (sub[j] == 0 && fs_hit(h, tt, tau, j, i)) { sub[j] = 2; fint[j] = -1; }
-        }
-        if ((t == 6 || t == 9) && sub[j] == 0 && fs_hit(h, tt, tau, j, i)) { sub[j] = 2; fint[j] = -1; }
-    }
+    PAIR_LOOP(h, fs_pair3, FsWork, fw);
     for (int i = 0; i < nt; ++i)
         if (sub[i] == 0) sub[i] = 1;
     /* 3_...:1329-1349: BC_or_not 2 on the surface, 0 inside; prescribed
@@ -1183,29 +1507,8 @@ static int free_surface(struct sps_handle *h)
     /* step 4 (3_...:1353-1389): the normal from the surface neighbours'
      * edges, oriented against the gradient of f_int */
     memset(nrm, 0, 2 * (size_t)nt * sizeof(double));
-    for (int64_t q = 0; q < h->n_pairs; ++q) {
-        const int64_t k = h->order[q];
-        const int t = h->pt[k];
-        if (t != 1 && t != 2 && t != 3) continue;
-        const int i = h->pi[k], j = h->pj[k];
-        if (sub[i] == 1 && sub[j] == 1) {
-            const float x1 = (float)x[2 * i], y1 = (float)x[2 * i + 1];
-            const float x2 = (float)x[2 * j], y2 = (float)x[2 * j + 1];
-            const float xv = x2 - x1, yv = y2 - y1;
-            nrm[2 * i] = nrm[2 * i] - (double)yv;
-            nrm[2 * i + 1] = nrm[2 * i + 1] + (double)xv;
-            nrm[2 * j] = nrm[2 * j] - (double)yv;
-            nrm[2 * j + 1] = nrm[2 * j + 1] + (double)xv;
-            nb[i] = nb[i] + 1.0f;
-            nb[j] = nb[j] + 1.0f;
-        }
-        float h1i, h1j, h2i, h2j;
-        fs_hf(h, k, &h1i, &h1j, &h2i, &h2j);
-        gf[2 * i] = gf[2 * i] + (fint[j] - fint[i]) * h1i;
-        gf[2 * j] = gf[2 * j] + (fint[i] - fint[j]) * h1j;
-        gf[2 * i + 1] = gf[2 * i + 1] + (fint[j] - fint[i]) * h2i;
-        gf[2 * j + 1] = gf[2 * j + 1] + (fint[i] - fint[j]) * h2j;
-    }
+    PAIR_LOOP(h, fs_pair4, FsWork, fw);
+#pragma omp parallel for num_threads(h->nthreads > 0 ? h->nthreads : 1)
     for (int i = 0; i < nt; ++i) {   /* 3_...:1391-1405 */
         if (sub[i] != 1) continue;
         nrm[2 * i] = nrm[2 * i] / (double)nb[i];
@@ -1345,18 +1648,26 @@ static int time_integration(struct sps_handle *h, int itimestep, double time_sph
 {
     const int nn = h->nn, nt = h->nt;
     const int ncrit = (int)h->p.props[1];
+    /* threads pay off from some tens of thousands of particles on; below
+     * (SPS_ORACLE_MIN_PARALLEL particles, default 20000) one thread runs */
+    const char *minp = getenv("SPS_ORACLE_MIN_PARALLEL");
+    h->nthreads = h->nt2 >= (minp ? atol(minp) : 20000) ? omp_get_max_threads() : 1;
     check_out_domain(h);
     int rc = grid_find(h, itimestep);
     h->pairs_total += h->n_pairs;
     if (rc) return rc;
     pint_update(h);
+    if ((rc = pair_sublists(h))) return rc;
     if (!h->p.inside_approach && h->p.sph_shift && itimestep > 1 && (itimestep - 1) % h->p.shift_update == 0) {
         stress_point_update(h); /* 2_...:99-109 */
         if (ncrit == 12) adapt_stress2(h);
     }
     memcpy(h->vx0, h->vel, 2 * (size_t)nt * sizeof(double));
     memcpy(h->x0, h->x, 2 * (size_t)nt * sizeof(double));
-    rk4(h, dt, time_sph);
+    if (h->integrator == SPS_INTEGRATOR_LEAPFROG)
+        leapfrog(h, dt, time_sph);
+    else
+        rk4(h, dt, time_sph);
     for (int i = nn; i < nt; ++i) /* update_strain, 3_...:1847-1863 */
         h->ivars[(size_t)h->niv * i] = h->ivars[(size_t)h->niv * i] + dt * h->ddev[i];
     stress_point_update(h);
@@ -1369,8 +1680,13 @@ static int time_integration(struct sps_handle *h, int itimestep, double time_sph
         for (int i = 0; i < nn; ++i)
             for (int a = 0; a < 2; ++a) h->displ[2 * i + a] = h->x[2 * i + a] - h->x00[2 * i + a];
     } else if (h->p.update_x) {
+        OMP_FOR
         for (int i = 0; i < nt; ++i)
             for (int a = 0; a < 2; ++a) {
+                if (h->integrator == SPS_INTEGRATOR_LEAPFROG) {   /* drift with the half-step velocity */
+                    h->x[2 * i + a] = h->x0[2 * i + a] + dt * h->vel[2 * i + a];
+                    continue;
+                }
                 float vel_half = (float)(0.5 * (h->vx0[2 * i + a] + h->vel[2 * i + a]));
                 h->x[2 * i + a] = h->x0[2 * i + a] + (double)vel_half * dt;
             }
@@ -1378,11 +1694,27 @@ static int time_integration(struct sps_handle *h, int itimestep, double time_sph
         if (!h->p.inside_approach) shift_stress_points(h, itimestep);
         for (int i = 0; i < nn; ++i)
             for (int a = 0; a < 2; ++a) h->displ[2 * i + a] = h->x[2 * i + a] - h->x00[2 * i + a];
+    } else if (h->integrator == SPS_INTEGRATOR_LEAPFROG) {
+        for (int i = 0; i < nn; ++i)
+            for (int a = 0; a < 2; ++a) h->displ[2 * i + a] = h->displ[2 * i + a] + dt * h->vel[2 * i + a];
     } else {
         for (int i = 0; i < nn; ++i)
             for (int a = 0; a < 2; ++a)
                 h->displ[2 * i + a] = h->displ[2 * i + a] + 0.5 * (h->vx0[2 * i + a] + h->vel[2 * i + a]) * dt;
     }
+    return 0;
+}
+
+int sps_set_integrator(sps_handle *h, int32_t integrator)
+{
+    if (!h) return fail(SPS_EINVAL, "null argument");
+    if (integrator != SPS_INTEGRATOR_RK4 && integrator != SPS_INTEGRATOR_LEAPFROG)
+        return fail(SPS_EINVAL, "unknown integrator %d", integrator);
+    if (integrator == SPS_INTEGRATOR_LEAPFROG &&
+        (!h->p.inside_approach || h->p.xsph || h->p.mode != SPS_MODE_ORDERED || h->log_cap > 0))
+        return fail(SPS_EINVAL, "leapfrog: inside approach, no XSPH, SPS_MODE_ORDERED, no plastic log");
+    h->integrator = integrator;
+    h->lf_first = 1;
     return 0;
 }
 
@@ -1478,6 +1810,7 @@ int sps_set_state(sps_handle *h, const sps_state *st)
     else memset(h->disp10, 0, (size_t)h->nn * sizeof(double));
     if (bw_setup(h)) return fail(SPS_ENOMEM, "wall-dummy grid");
     h->have_state = 1;
+    h->lf_first = 1;   /* velocities at a whole step */
     return 0;
 }
 
@@ -1500,13 +1833,16 @@ int sps_set_bcs(sps_handle *h, const sps_bcs *b)
         for (int k = 2; k < 7; ++k)
             if (b->bc_info[8 * i + k] < 0 || b->bc_info[8 * i + k] > b->no_bcs)
                 return fail(SPS_EINVAL, "node %d: BC number %d of %d", i + 1, b->bc_info[8 * i + k], b->no_bcs);
+    for (int i = 0; i < h->nt; ++i)
+        if (b->bc_or_not[i] != 0 && b->bc_or_not[i] != 1)
+            return fail(SPS_EINVAL, "bc_or_not[%d] = %d (0 or 1)", i, b->bc_or_not[i]);
+    /* everything checked: nothing is changed by a refused call */
     h->nbc = b->no_bcs;
     h->ncurves = b->ncurves;
     memcpy(h->bc_list, b->bc_list, 8 * (size_t)b->no_bcs * sizeof(double));
     memcpy(h->bc_info, b->bc_info, 8 * (size_t)h->nn * sizeof(int32_t));
     memcpy(h->bc_or_not, b->bc_or_not, (size_t)h->nt * sizeof(int32_t));
-    for (int i = 0; i < h->nt; ++i)
-        if (b->bc_or_not[i] < 0 || b->bc_or_not[i] > 3) return fail(SPS_EINVAL, "bc_or_not[%d] = %d", i, b->bc_or_not[i]);
+    memset(h->normal, 0, 2 * (size_t)h->nt * sizeof(double));   /* no surface detected yet */
     h->ifsigman = b->ifsigman;
     for (int c = 0; c < b->ncurves; ++c) {
         h->curve_npts[c] = b->curve_npts[c];
@@ -1632,6 +1968,7 @@ int sps_set_graph(sps_handle *h, int32_t enable)
 int sps_set_plastic_log(sps_handle *h, int64_t cap)
 {
     if (!h || cap < 0 || cap > ((int64_t)1 << 30)) return fail(SPS_EINVAL, "bad argument");
+    if (cap > 0 && h->integrator == SPS_INTEGRATOR_LEAPFROG) return fail(SPS_EINVAL, "the log records RK4 steps");
     free(h->plog);
     h->plog = cap ? (double *)malloc((size_t)cap * SPS_PLOG_ROW * sizeof(double)) : NULL;
     if (cap && !h->plog) return fail(SPS_ENOMEM, "log allocation");

@@ -15,6 +15,8 @@ Usage: python oracle/time_reference.py [--steps-c1 N] [--quick]
 import argparse
 import importlib
 import os
+
+os.environ["OMP_NUM_THREADS"] = "1"   # the oracle on one thread, as the reference runs
 import re
 import shutil
 import subprocess

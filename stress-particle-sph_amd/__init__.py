@@ -28,6 +28,7 @@ PRODUCT_LIB = os.path.join(PKG_DIR, "libsps_hip.so")
 DECK_DIR = os.path.join(PKG_DIR, "decks")
 
 SPS_ABI_VERSION = 4
+INTEGRATOR_RK4, INTEGRATOR_LEAPFROG = 0, 1
 SPS_MAX_TCURVE = 64
 SPS_MAX_BCS = 32
 SPS_MAX_CURVES = 16
@@ -218,8 +219,11 @@ class Simulation:
         self.itimestep_sph = ctypes.c_int32(0)
 
     @classmethod
-    def from_deck(cls, deck: Deck, lib_path=PRODUCT_LIB, mode=MODE_ORDERED, **kw):
-        parts = build_particles(deck)
+    def from_deck(cls, deck: Deck, lib_path=PRODUCT_LIB, mode=MODE_ORDERED, parts=None, **kw):
+        """A simulation of the deck from rest (its particles built here, or
+        `parts`, the same deck's, built before)."""
+        if parts is None:
+            parts = build_particles(deck)
         sim = cls(make_params(deck, parts, mode=mode, **kw), lib_path)
         sim.set_state(initial_state(parts, sim.niv))
         sim.deck, sim.parts = deck, parts
@@ -323,6 +327,10 @@ class Simulation:
         mat = np.ascontiguousarray(mat, dtype=np.int32)
         assert table.ndim == 2 and table.shape[1] == 10 and mat.shape == (self.n2,)
         self._check(self.lib.sps_set_materials(self.handle, table.shape[0], table.ctypes.data, mat.ctypes.data))
+
+    def set_integrator(self, which):
+        """RK4 (the reference's, default) or leapfrog: INTEGRATOR_RK4 / INTEGRATOR_LEAPFROG."""
+        self._check(self.lib.sps_set_integrator(self.handle, int(which)))
 
     def set_graph(self, on=True):
         """Replay the mixed form's list-reusing steps as a captured graph."""

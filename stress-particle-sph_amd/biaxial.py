@@ -53,10 +53,11 @@ def material_table(tan_phi, coh, young, poiss, rel=0.02, levels=LEVELS):
     return table
 
 
-def open_biaxial(sps, deck, seed=2024, rel=0.02, mode=0, lib_path=None):
+def open_biaxial(sps, deck, seed=2024, rel=0.02, mode=0, lib_path=None, parts=None):
     """A Simulation of the deck with each particle's friction angle perturbed
-    by up to rel (seeded); the deck's BCs are set."""
-    sim = sps.Simulation.from_deck(deck, lib_path=lib_path or sps.PRODUCT_LIB, mode=mode)
+    by up to rel (seeded); the deck's BCs are set.  `parts`: the deck's
+    particles if already built."""
+    sim = sps.Simulation.from_deck(deck, lib_path=lib_path or sps.PRODUCT_LIB, mode=mode, parts=parts)
     p = sim.params
     table = material_table(p.props[12], p.props[13], p.props[2], p.props[3], rel)
     mat = phi_levels(sim.n2, seed)

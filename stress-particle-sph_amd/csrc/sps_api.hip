@@ -51,6 +51,11 @@ static inline unsigned nblk8(long long n) {
     unsigned b = nblk(n);
     return (b + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
 }
+// tiles of TILE receivers, a multiple of the XCD count
+static inline unsigned ntile8(long long n) {
+    unsigned b = (unsigned)((n + TILE - 1) / TILE);
+    return (b + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
+}
 
 
 template <typename T>
@@ -154,6 +159,8 @@ struct sps_handle {
     // prescribed BCs (sps_set_bcs): tables on the host, node rows on the device
     bool bcs_set = false;
     bool surface_pending = false;  // the last step's free-surface detection is still to run
+    int integrator = SPS_INTEGRATOR_RK4;
+    bool lf_first = true;          // leapfrog: the velocities are at a whole step (kick by dt / 2)
     int nbc = 0, ncurves = 0;
     double bc_list[8 * SPS_MAX_BCS];
     int curve_npts[SPS_MAX_CURVES];
@@ -493,8 +500,8 @@ extern "C" int sps_create(const sps_params *params, sps_handle **out) {
     rc |= dalloc(h->tmp_orig, nt2); rc |= dalloc(h->tmp_src, nt2); rc |= dalloc(h->proc, nt2);
     rc |= dalloc(h->moved, nblk(nt2));
     rc |= dalloc(h->proc_to_slot, nt2); rc |= dalloc(h->cnt, nt2); rc |= dalloc(h->off, nt2);
-    h->ntile_n = (int)nblk8(nn);
-    h->ntile_s = (int)nblk8(nt - nn);
+    h->ntile_n = (int)ntile8(nn);
+    h->ntile_s = (int)ntile8(nt - nn);
     {
         const size_t wa = (size_t)(h->ntile_n + h->ntile_s) * LGROUPS * TILE, wb = (size_t)h->ntile_n * LGROUPS * TILE;
         rc |= dalloc(h->list_ia, wa); rc |= dalloc(h->list_wa, wa); rc |= dalloc(h->list_gxa, wa);
@@ -579,6 +586,7 @@ extern "C" int sps_set_state(sps_handle *h, const sps_state *st) {
     h->h_uniform = st->hsml[0] > 0 ? st->hsml[0] : 0.0;
     for (int i = 1; i < h->nt2 && h->h_uniform > 0; ++i)
         if (!(st->hsml[i] == h->h_uniform)) h->h_uniform = 0.0;
+    h->dp.h_uni = h->h_uniform > 0;
     if (rc) return rc;
     if (h->dp.do_bforce) {
         double hmin = 1e300;
@@ -623,6 +631,7 @@ extern "C" int sps_set_state(sps_handle *h, const sps_state *st) {
     HIP_TRY(hipMemsetAsync(h->err.p, 0, sizeof(int), s));   // a new state starts with a clear flag
     HIP_TRY(hipStreamSynchronize(s));
     h->have_state = true;
+    h->lf_first = true;
     h->surface_pending = false;
     h->lists_valid = false;
     drop_graphs(h);
@@ -682,16 +691,16 @@ static int search(sps_handle *h, bool fresh_list) {
         const PairLists PL = h->lists();
         // the mixed-precision stage kernels compute the kernel values themselves
         if (nn > 0 && !h->mixed)
-            LAUNCH_LDS(K_COUNT_PAIRS, nblk8(nn), TILE, LAY_COUNT.bytes, (k_count_lists<KIND_NODE, true>), V, C.x.p,
+            LAUNCH_LDS(K_COUNT_PAIRS, ntile8(nn), TILE, LAY_COUNT.bytes, (k_count_lists<KIND_NODE, true>), V, C.x.p,
                        C.hsml.p, h->cnt.p, h->split.p, PL, h->h_uniform, 0, nn, h->skin);
         if (nn > 0 && h->mixed)
-            LAUNCH_LDS(K_COUNT_PAIRS, nblk8(nn), TILE, LAY_COUNT.bytes, (k_count_lists<KIND_NODE, false>), V, C.x.p,
+            LAUNCH_LDS(K_COUNT_PAIRS, ntile8(nn), TILE, LAY_COUNT.bytes, (k_count_lists<KIND_NODE, false>), V, C.x.p,
                        C.hsml.p, h->cnt.p, h->split.p, PL, h->h_uniform, 0, nn, h->skin);
         if (nt > nn && !h->mixed)
-            LAUNCH_LDS(K_COUNT_PAIRS, nblk8(nt - nn), TILE, LAY_COUNT.bytes, (k_count_lists<KIND_SP, true>), V, C.x.p,
+            LAUNCH_LDS(K_COUNT_PAIRS, ntile8(nt - nn), TILE, LAY_COUNT.bytes, (k_count_lists<KIND_SP, true>), V, C.x.p,
                        C.hsml.p, h->cnt.p, h->split.p, PL, h->h_uniform, nn, nt - nn, h->skin);
         if (nt > nn && h->mixed)
-            LAUNCH_LDS(K_COUNT_PAIRS, nblk8(nt - nn), TILE, LAY_COUNT.bytes, (k_count_lists<KIND_SP, false>), V, C.x.p,
+            LAUNCH_LDS(K_COUNT_PAIRS, ntile8(nt - nn), TILE, LAY_COUNT.bytes, (k_count_lists<KIND_SP, false>), V, C.x.p,
                        C.hsml.p, h->cnt.p, h->split.p, PL, h->h_uniform, nn, nt - nn, h->skin);
         if (nt2 > nt)
             LAUNCH(K_COUNT_PAIRS, nblk8(nt2 - nt), BLOCK, k_count, C.x.p, C.hsml.p, h->cell.p, h->kstart.p, h->proc.p,
@@ -769,10 +778,10 @@ static int mixed_step_kernels(sps_handle *h, double dt, double time_sph, double 
         // the outside approach's stress_point_update + adapt_stress2 on the
         // step's lists (2_...:99-109)
         if (nn > 0)
-            LAUNCH_LDS(K_SPU_NODES, nblk8(nn), TILE, LAY_SPU_NODES_MIX.bytes, k_spu_nodes_mix, SB, V, PL, adapt,
+            LAUNCH_LDS(K_SPU_NODES, ntile8(nn), TILE, LAY_SPU_NODES_MIX.bytes, k_spu_nodes_mix, SB, V, PL, adapt,
                        tanfi, coh, hu, nn);
         if (nt > nn)
-            LAUNCH_LDS(K_SP_UPDATE, nblk8(nt - nn), TILE, LAY_SP_UPDATE_MIX.bytes, k_sp_update_mix, SB, W, V, PL,
+            LAUNCH_LDS(K_SP_UPDATE, ntile8(nt - nn), TILE, LAY_SP_UPDATE_MIX.bytes, k_sp_update_mix, SB, W, V, PL,
                        h->dp, time_sph, 0.0, 0, adapt, hu, nn, nt, stepv);
     }
     LAUNCH(K_RK_INIT, nblk(nt2), BLOCK, k_stage_first, SB, W, adapt, tanfi, coh, nn, nt, nt2);
@@ -781,14 +790,14 @@ static int mixed_step_kernels(sps_handle *h, double dt, double time_sph, double 
         if (st > 0) LAUNCH(K_STAGE_BEGIN, nblk(nt), BLOCK, k_stage_begin, SB, W, c1, adapt, tanfi, coh, nn, nt);
         if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, h->bc_vars, h->bc_vals.p, h->ifsigman, h->normal.p, nn);
         if (nn > 0)
-            LAUNCH_LDS(K_SPU_NODES, nblk8(nn), TILE, LAY_SPU_NODES_MIX.bytes, k_spu_nodes_mix, SB, V, PL, adapt,
+            LAUNCH_LDS(K_SPU_NODES, ntile8(nn), TILE, LAY_SPU_NODES_MIX.bytes, k_spu_nodes_mix, SB, V, PL, adapt,
                        tanfi, coh, hu, nn);
         if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, h->bc_vars, h->bc_vals.p, h->ifsigman, h->normal.p, nn);
         if (nt > nn)
-            LAUNCH_LDS(K_SP_UPDATE, nblk8(nt - nn), TILE, LAY_SP_UPDATE_MIX.bytes, k_sp_update_mix, SB, W, V, PL,
+            LAUNCH_LDS(K_SP_UPDATE, ntile8(nt - nn), TILE, LAY_SP_UPDATE_MIX.bytes, k_sp_update_mix, SB, W, V, PL,
                        h->dp, time_sph, f2rk[st], 1, adapt, hu, nn, nt, stepv);
         if (nn > 0)
-            LAUNCH_LDS(K_DERIV_NODES, nblk8(nn), TILE, LAY_DERIV_NODES_MIX.bytes, k_deriv_nodes_mix, SB, W, V, PL,
+            LAUNCH_LDS(K_DERIV_NODES, ntile8(nn), TILE, LAY_DERIV_NODES_MIX.bytes, k_deriv_nodes_mix, SB, W, V, PL,
                        h->dp, gxg, gyg, f2rk[st], h->err.p, hu, nn, stepv);
         if (st == 3 && nt2 > nt)
             LAUNCH(K_DUMMY_STRESS, nblk(nt2 - nt), BLOCK, k_dummy_stress, SB, V, nt, nt2);
@@ -803,19 +812,19 @@ static int mixed_step_kernels(sps_handle *h, double dt, double time_sph, double 
     }
     if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, h->bc_vars, h->bc_vals.p, h->ifsigman, h->normal.p, nn);
     if (nn > 0)
-        LAUNCH_LDS(K_SPU_NODES, nblk8(nn), TILE, LAY_SPU_NODES_MIX.bytes, k_spu_nodes_mix, SB, V, PL, adapt, tanfi,
+        LAUNCH_LDS(K_SPU_NODES, ntile8(nn), TILE, LAY_SPU_NODES_MIX.bytes, k_spu_nodes_mix, SB, V, PL, adapt, tanfi,
                    coh, hu, nn);
     if (nt > nn)
-        LAUNCH_LDS(K_SP_UPDATE, nblk8(nt - nn), TILE, LAY_SP_UPDATE_MIX.bytes, k_sp_update_mix, SB, W, V, PL, h->dp,
+        LAUNCH_LDS(K_SP_UPDATE, ntile8(nt - nn), TILE, LAY_SP_UPDATE_MIX.bytes, k_sp_update_mix, SB, W, V, PL, h->dp,
                    time_sph, 0.0, 0, adapt, hu, nn, nt, stepv);
     if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, h->bc_vars, h->bc_vals.p, h->ifsigman, h->normal.p, nn);
     const bool xsph = h->xsph && h->p.update_x;
     if (xsph && nn > 0)
-        LAUNCH_LDS(K_XSPH, nblk8(nn), TILE, LAY_XSPH_MIX.bytes, (k_xsph_mix<KIND_NODE>), SB, V, PL, hu, h->xsph_v.p);
+        LAUNCH_LDS(K_XSPH, ntile8(nn), TILE, LAY_XSPH_MIX.bytes, (k_xsph_mix<KIND_NODE>), SB, V, PL, hu, h->xsph_v.p);
     if (xsph && nt > nn)
-        LAUNCH_LDS(K_XSPH, nblk8(nt - nn), TILE, LAY_XSPH_MIX.bytes, (k_xsph_mix<KIND_SP>), SB, V, PL, hu, h->xsph_v.p);
+        LAUNCH_LDS(K_XSPH, ntile8(nt - nn), TILE, LAY_XSPH_MIX.bytes, (k_xsph_mix<KIND_SP>), SB, V, PL, hu, h->xsph_v.p);
     LAUNCH(K_POSITION, nblk(nt), BLOCK, k_position, SB, W, dt, h->p.update_x, nn, nt,
-           xsph ? (const double2 *)h->xsph_v.p : (const double2 *)nullptr, h->err.p);
+           xsph ? (const double2 *)h->xsph_v.p : (const double2 *)nullptr, h->err.p, 0);
     if (shift >= 0 && h->p.update_x && nn > 0) {
         const ShiftParams sp{h->p.npoints, h->p.vel_vector, shift, h->p.r_x, h->p.r_y, h->p.disp_tol, h->p.dx};
         LAUNCH(K_SHIFT, nblk(nn), BLOCK, k_shift_sp, SB, h->inv.p, h->nodeinfo.p, h->x10.p, h->disp10.p, sp, nn);
@@ -916,6 +925,20 @@ static int free_surface(sps_handle *h, const StateBufs &SB) {
     return 0;
 }
 
+// Dynamic LDS of the FP64 stage kernels: their layouts put the fields a
+// launch may leave out last (the nodes' x without CSPM, the dummies for the
+// update alone, h when every particle has the same smoothing length), and
+// a launch allocates the bytes before them.
+static size_t lds_sp_update(const sps_handle *h, int deriv) {
+    if (!deriv) return (size_t)LAY_SP_UPDATE.x[KIND_DUMMY];
+    return (size_t)LAY_SP_UPDATE.x[KIND_NODE] + (h->dp.cspm ? 16 * (size_t)CAP : 0);
+}
+static size_t lds_deriv_nodes(const sps_handle *h) {
+    const size_t p1 = (size_t)LAY_DERIV_NODES.x[KIND_SP] + (h->dp.cspm ? 16 * (size_t)CAP : 0);
+    const size_t p2 = h->dp.do_av ? (size_t)LAY_DERIV_NODES.h[KIND_NODE] + (h->dp.h_uni ? 0 : 8 * (size_t)CAP) : 0;
+    return p1 > p2 ? p1 : p2;
+}
+
 static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
     bool reuse;
     int rc = lists_still_valid(h, reuse);
@@ -965,45 +988,74 @@ static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
     if (h->outside && h->p.sph_shift && itimestep > 1 && (itimestep - 1) % h->p.shift_update == 0) {
         // stress_point_update + adapt_stress2 on the new lists (2_...:99-109)
         if (nn > 0)
-            LAUNCH_LDS(K_SPU_NODES, nblk8(nn), TILE, LAY_SPU_NODES.bytes, k_spu_nodes, SB, V, PL, adapt, tanfi, coh, nn);
+            LAUNCH_LDS(K_SPU_NODES, ntile8(nn), TILE, LAY_SPU_NODES.bytes, k_spu_nodes, SB, V, PL, adapt, tanfi, coh, nn);
         if (nt > nn)
-            LAUNCH_LDS(K_SP_UPDATE, nblk8(nt - nn), TILE, LAY_SP_UPDATE.bytes, k_sp_update, SB, W, V, PL, h->dp,
+            LAUNCH_LDS(K_SP_UPDATE, ntile8(nt - nn), TILE, lds_sp_update(h, 0), k_sp_update, SB, W, V, PL, h->dp,
                        time_sph, 0.0, 0, adapt, nn, nt);
     }
+    const bool lf = h->integrator == SPS_INTEGRATOR_LEAPFROG;
     LAUNCH(K_RK_INIT, nblk(nt2), BLOCK, k_stage_first, SB, W, adapt, tanfi, coh, nn, nt, nt2);
-    for (int st = 0; st < 4; ++st) {
+    if (lf) {
+        // leapfrog: the node accelerations of the step's state kick the node
+        // velocities; the stress points' rates from the kicked velocities
+        // update their stresses (sps_set_integrator)
+        const double kdt = h->lf_first ? 0.5 * dt : dt;
+        if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, h->ifsigman, h->normal.p, nn);
+        if (nn > 0)
+            LAUNCH_LDS(K_SPU_NODES, ntile8(nn), TILE, LAY_SPU_NODES.bytes, k_spu_nodes, SB, V, PL, adapt, tanfi, coh, nn);
+        if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, h->ifsigman, h->normal.p, nn);
+        if (nt > nn)
+            LAUNCH_LDS(K_SP_UPDATE, ntile8(nt - nn), TILE, lds_sp_update(h, 0), k_sp_update, SB, W, V, PL, h->dp, time_sph,
+                       0.0, 0, adapt, nn, nt);
+        if (nn > 0)
+            LAUNCH_LDS(K_DERIV_NODES, ntile8(nn), TILE, lds_deriv_nodes(h), k_deriv_nodes, SB, W, V, PL, h->dp, gxg, gyg,
+                       1.0, h->err.p, nn);
+        LAUNCH(K_STAGE_BEGIN, nblk(nn), BLOCK, k_lf_kick, SB, W, kdt, nn);
+        if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, h->ifsigman, h->normal.p, nn);
+        if (nn > 0)
+            LAUNCH_LDS(K_SPU_NODES, ntile8(nn), TILE, LAY_SPU_NODES.bytes, k_spu_nodes, SB, V, PL, adapt, tanfi, coh, nn);
+        if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, h->ifsigman, h->normal.p, nn);
+        if (nt > nn)
+            LAUNCH_LDS(K_SP_UPDATE, ntile8(nt - nn), TILE, lds_sp_update(h, 1), k_sp_update, SB, W, V, PL, h->dp, time_sph,
+                       1.0, 1, adapt, nn, nt);
+        if (nt2 > nt) LAUNCH(K_DUMMY_STRESS, nblk(nt2 - nt), BLOCK, k_dummy_stress, SB, V, nt, nt2);
+        LAUNCH(K_RK_FINAL, nblk(nt), BLOCK, k_lf_final, SB, W, dt, adapt, tanfi, coh, nn, nt, h->err.p);
+        h->lf_first = false;
+    }
+    for (int st = 0; st < (lf ? 0 : 4); ++st) {
         const double c1 = f1rk[st] * (dt);
         if (st > 0) LAUNCH(K_STAGE_BEGIN, nblk(nt), BLOCK, k_stage_begin, SB, W, c1, adapt, tanfi, coh, nn, nt);
         if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, h->ifsigman, h->normal.p, nn);
         if (nn > 0)
-            LAUNCH_LDS(K_SPU_NODES, nblk8(nn), TILE, LAY_SPU_NODES.bytes, k_spu_nodes, SB, V, PL, adapt, tanfi, coh, nn);
+            LAUNCH_LDS(K_SPU_NODES, ntile8(nn), TILE, LAY_SPU_NODES.bytes, k_spu_nodes, SB, V, PL, adapt, tanfi, coh, nn);
         if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, h->ifsigman, h->normal.p, nn);
         if (nt > nn)
-            LAUNCH_LDS(K_SP_UPDATE, nblk8(nt - nn), TILE, LAY_SP_UPDATE.bytes, k_sp_update, SB, W, V, PL, h->dp, time_sph,
+            LAUNCH_LDS(K_SP_UPDATE, ntile8(nt - nn), TILE, lds_sp_update(h, 1), k_sp_update, SB, W, V, PL, h->dp, time_sph,
                        f2rk[st], 1, adapt, nn, nt);
         if (nn > 0)
-            LAUNCH_LDS(K_DERIV_NODES, nblk8(nn), TILE, LAY_DERIV_NODES.bytes, k_deriv_nodes, SB, W, V, PL, h->dp, gxg, gyg,
+            LAUNCH_LDS(K_DERIV_NODES, ntile8(nn), TILE, lds_deriv_nodes(h), k_deriv_nodes, SB, W, V, PL, h->dp, gxg, gyg,
                        f2rk[st], h->err.p, nn);
         if (st == 3 && nt2 > nt)
             LAUNCH(K_DUMMY_STRESS, nblk(nt2 - nt), BLOCK, k_dummy_stress, SB, V, nt, nt2);
     }
     const bool hook = h->log_cap > 0;
-    LAUNCH(K_RK_FINAL, nblk(nt), BLOCK, k_rk_final, SB, W, dt, adapt, tanfi, coh, nn, nt, surrogate_dev(h),
-           hook ? 1 : 0, h->err.p);
+    if (!lf)
+        LAUNCH(K_RK_FINAL, nblk(nt), BLOCK, k_rk_final, SB, W, dt, adapt, tanfi, coh, nn, nt, surrogate_dev(h),
+               hook ? 1 : 0, h->err.p);
     if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, h->ifsigman, h->normal.p, nn);
     if (nn > 0)
-        LAUNCH_LDS(K_SPU_NODES, nblk8(nn), TILE, LAY_SPU_NODES.bytes, k_spu_nodes, SB, V, PL, adapt, tanfi, coh, nn);
+        LAUNCH_LDS(K_SPU_NODES, ntile8(nn), TILE, LAY_SPU_NODES.bytes, k_spu_nodes, SB, V, PL, adapt, tanfi, coh, nn);
     if (nt > nn)
-        LAUNCH_LDS(K_SP_UPDATE, nblk8(nt - nn), TILE, LAY_SP_UPDATE.bytes, k_sp_update, SB, W, V, PL, h->dp, time_sph, 0.0,
+        LAUNCH_LDS(K_SP_UPDATE, ntile8(nt - nn), TILE, lds_sp_update(h, 0), k_sp_update, SB, W, V, PL, h->dp, time_sph, 0.0,
                    0, adapt, nn, nt);
     if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, h->ifsigman, h->normal.p, nn);
     const bool xsph = h->xsph && h->p.update_x;
     if (xsph && nn > 0)
-        LAUNCH_LDS(K_XSPH, nblk8(nn), TILE, LAY_XSPH.bytes, (k_xsph<KIND_NODE>), SB, V, PL, h->xsph_v.p);
+        LAUNCH_LDS(K_XSPH, ntile8(nn), TILE, LAY_XSPH.bytes, (k_xsph<KIND_NODE>), SB, V, PL, h->xsph_v.p);
     if (xsph && nt > nn)
-        LAUNCH_LDS(K_XSPH, nblk8(nt - nn), TILE, LAY_XSPH.bytes, (k_xsph<KIND_SP>), SB, V, PL, h->xsph_v.p);
+        LAUNCH_LDS(K_XSPH, ntile8(nt - nn), TILE, LAY_XSPH.bytes, (k_xsph<KIND_SP>), SB, V, PL, h->xsph_v.p);
     LAUNCH(K_POSITION, nblk(nt), BLOCK, k_position, SB, W, dt, h->p.update_x, nn, nt, xsph ? h->xsph_v.p : nullptr,
-           h->err.p);
+           h->err.p, lf ? 1 : 0);
     if (h->p.update_x) {   // 2_...:150-154
         // The detection's results (BC_or_not, the normals) change the next
         // step only through apply_stress_free (ifsigman = 1) and through
@@ -1040,7 +1092,8 @@ extern "C" int sps_set_bcs(sps_handle *h, const sps_bcs *b) {
     if (b->ifsigman == 1 && h->mixed)
         return fail(SPS_EINVAL, "ifsigman = 1: SPS_MODE_ORDERED only (the mixed form has no free-surface detection)");
     for (int i = 0; i < h->nt; ++i)
-        if (b->bc_or_not[i] < 0 || b->bc_or_not[i] > 3) return fail(SPS_EINVAL, "bc_or_not[%d] = %d", i, b->bc_or_not[i]);
+        if (b->bc_or_not[i] != 0 && b->bc_or_not[i] != 1)
+            return fail(SPS_EINVAL, "bc_or_not[%d] = %d (0 or 1)", i, b->bc_or_not[i]);
     if (b->ncurves < 0 || b->ncurves > SPS_MAX_CURVES || (b->ncurves > 0 && (!b->curve_npts || !b->curve_t || !b->curve_f)))
         return fail(SPS_EINVAL, "bad time curves");
     for (int c = 0; c < b->ncurves; ++c)
@@ -1271,6 +1324,17 @@ extern "C" int sps_set_materials(sps_handle *h, int32_t nmat, const double *tabl
     StateSet &C = h->S[h->cur];
     LAUNCH(K_PERMUTE, nblk(h->nt2), BLOCK, k_material_slots, C.orig.p, h->mat_orig.p, C.mat.p, h->nt2);
     HIP_TRY(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+extern "C" int sps_set_integrator(sps_handle *h, int32_t integrator) {
+    if (!h) return fail(SPS_EINVAL, "null argument");
+    if (integrator != SPS_INTEGRATOR_RK4 && integrator != SPS_INTEGRATOR_LEAPFROG)
+        return fail(SPS_EINVAL, "unknown integrator %d", integrator);
+    if (integrator == SPS_INTEGRATOR_LEAPFROG && (h->outside || h->xsph || h->mixed || h->log_cap > 0))
+        return fail(SPS_EINVAL, "leapfrog: inside approach, no XSPH, SPS_MODE_ORDERED, no plastic log");
+    h->integrator = integrator;
+    h->lf_first = true;
     return 0;
 }
 

@@ -18,6 +18,7 @@ namespace sps {
 struct DevParams {
     int nn, ns, nd, nd2, nt, nt2;      // counts (reference ranges)
     int cspm, update_x, ncrit, ntype_eco, do_bforce, do_av, ic_grav;
+    int h_uni;                         // every particle has the same smoothing length
     double pi, dx, alpha, beta, damping, cspm_tol;
     double D11, D22, D12, D33, D41, D42;
     double tanfi, coh;                 // props(1,13), props(1,14)
@@ -74,33 +75,39 @@ __device__ __forceinline__ void cubic_kernel(double r, double dx0, double dx1, d
     gy = (float)g1;
 }
 
-// The factor 15 / (7 pi h^2) of the kernel for the last smoothing length
-// seen: the same operations on the same h, evaluated once per distinct h.
+// The parts of the kernel that depend on h alone, for the last smoothing
+// length seen: the factor 15 / (7 pi h^2), h^2, factor * 1 / 6 and
+// factor * 1 / 6 * 3 -- the same operations on the same h as cubic_kernel's
+// left-to-right expressions, evaluated once per distinct h.
 struct KernelFactor {
-    double h = -1.0, f = 0.0;
-    __device__ __forceinline__ double get(double hs, double pi) {
+    double h = -1.0, f = 0.0, h2 = 0.0, f6 = 0.0, f63 = 0.0;
+    __device__ __forceinline__ const KernelFactor &get(double hs, double pi) {
         if (!(hs == h)) {
             h = hs;
             f = 15.0 / (7.0 * pi * hs * hs);
+            h2 = hs * hs;
+            f6 = f * 1.0 / 6.0;
+            f63 = f6 * 3.0;
         }
-        return f;
+        return *this;
     }
 };
 
-// cubic_kernel with the factor supplied.
-__device__ __forceinline__ void cubic_kernel_f(double r, double dx0, double dx1, double hs, double factor,
+// cubic_kernel with the h-only parts supplied (K.get(hs, pi)).
+__device__ __forceinline__ void cubic_kernel_f(double r, double dx0, double dx1, double hs, const KernelFactor &K,
                                                float &w, float &gx, float &gy) {
     double q = r / hs;
+    const double factor = K.f;
     double wd, g0, g1;
     if (q >= 0.0 && q <= 1.0) {
         wd = factor * ((double)(2.0f / 3.0f) - q * q + (q * (q * q)) / 2.0);
-        double c = factor * (-2.0 + (double)(3.0f / 2.0f) * q) / (hs * hs);
+        double c = factor * (-2.0 + (double)(3.0f / 2.0f) * q) / K.h2;
         g0 = c * dx0;
         g1 = c * dx1;
     } else if (q > 1.0 && q <= 2.0) {
         double t = 2.0 - q;
-        wd = factor * 1.0 / 6.0 * (t * (t * t));
-        double c = factor * 1.0 / 6.0 * 3.0 * (t * t) / hs;
+        wd = K.f6 * (t * (t * t));
+        double c = K.f63 * (t * t) / hs;
         g0 = -(c * (dx0 / r));
         g1 = -(c * (dx1 / r));
     } else {

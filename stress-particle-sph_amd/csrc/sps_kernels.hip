@@ -17,11 +17,13 @@
 // before the search and n the pairs found, pairs d >= M come first in
 // descending d, then pairs d < M ascending.  For a receiver, ascending d is
 // the order of its partners in (cell, kind, original index) -- the forward
-// stencil walk -- so each stage kernel visits a receiver's partners with a
-// reverse walk (pairs d >= M) followed by a forward walk (pairs d < M) and
-// adds them one by one in that order, recomputing the single-precision w
-// and grad w of each pair (positions are fixed between the search and the
-// end of the step): one lane per receiver, no atomics, no stored pair list,
+// stencil walk -- so a receiver's partners are visited with a reverse walk
+// (pairs d >= M) followed by a forward walk (pairs d < M) and added one by
+// one in that order.  The search (k_count_lists) does that walk once per
+// step and stores each receiver's accepted partners in forward walk order
+// with the pair's single-precision w and grad w (PairLists); the stage
+// kernels read the lists forwards, and backwards for the pairs d >= M, from
+// windows of the partners staged in LDS: one lane per receiver, no atomics,
 // results bitwise equal to the reference.
 #include <type_traits>
 
@@ -665,12 +667,34 @@ struct GlobalFields {
 // the LDS capacity, or that holds a receiver with pairs on both sides of M
 // walks the global storage instead; both paths visit the same pairs in the
 // same order.
-constexpr int TILE = 256;
-constexpr int TROWS = 24;              // most window rows staged
+//
+// The tile width and the window capacities are build parameters
+// (sps_tiles.h, written by tune_tiles.py from timings of the candidates;
+// -DSPS_TILE=... etc. override it).  Results do not depend on them: every
+// receiver sums the same pairs in the same order whichever path it takes.
+#include "sps_tiles.h"
+constexpr int TILE = SPS_TILE;
+constexpr int TROWS = SPS_TROWS;       // most window rows staged
 constexpr int TCOLS = STRIP + 2;
 constexpr int TCELLS = TROWS * TCOLS;
-constexpr int CAP = 640;               // staged partners of a main kind per window
-constexpr int CAP_D = 192;             // staged dummies per window
+constexpr int CAP = SPS_CAP;           // staged partners of a main kind per window
+constexpr int CAP_D = SPS_CAP_D;       // staged dummies per window
+static_assert(TILE % 64 == 0 && TILE >= 128 && TILE <= 1024,
+              "a tile is whole waves, at least two (the stage kernels plan two kinds at once), at most 1024 lanes");
+constexpr int TWAVES = TILE / 64;      // waves of a tile
+static_assert(CAP > 0 && CAP < (1 << 14) && CAP_D > 0 && CAP_D < (1 << 14),
+              "list entries carry a 14-bit window index");
+static_assert(CAP % 8 == 0 && CAP_D % 8 == 0, "every staged field starts 16-byte aligned");
+// waves per SIMD the FP64 stage kernels k_sp_update and k_deriv_nodes are
+// compiled for (their registers otherwise allow 3): 0 leaves it to the
+// compiler; more waves may cost register spills
+#if SPS_STAGE_WAVES > 0
+#define STAGE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(SPS_STAGE_WAVES)))
+#else
+#define STAGE_OCCUPANCY
+#endif
+static_assert(TROWS >= 3 && 2 * TROWS <= 64,
+              "a window holds at least one receiver row and its neighbours; one wave plans its halo rows");
 
 // Dynamic LDS of the tiled stage kernels.  A kernel's layout gives the byte
 // offset of every field it stages per partner kind (-1: not staged) and of
@@ -733,12 +757,15 @@ constexpr LdsLayout layout_spu_nodes() {
     return L;
 }
 
-// stress-point receivers <- nodes (x, v, m, rho, m / rho) and dummies (x, m,
-// rho, wall position and orientation)
+// stress-point receivers <- nodes (v, m, rho, m / rho; x only for the CSPM
+// matrix) and, for the derivative, dummies (x, m, rho, wall position and
+// orientation).  The fields a launch does not need come last, so a launch
+// allocates only the bytes before them (lds_sp_update on the host): the
+// update alone stops before the dummies, and without CSPM the nodes' x is
+// left out.
 constexpr LdsLayout layout_sp_update() {
     LdsLayout L = layout_none();
     int o = 0;
-    L.x[KIND_NODE] = o; o += 16 * CAP;
     L.v[KIND_NODE] = o; o += 16 * CAP;
     L.m[KIND_NODE] = o; o += 8 * CAP;
     L.rho[KIND_NODE] = o; o += 8 * CAP;
@@ -748,17 +775,19 @@ constexpr LdsLayout layout_sp_update() {
     L.rho[KIND_DUMMY] = o; o += 8 * CAP_D;
     L.wall = o; o += 4 * CAP_D;
     L.horiz = o; o += 4 * CAP_D;
+    L.x[KIND_NODE] = o; o += 16 * CAP;
     L.bytes = o;
     return L;
 }
 
-// node receivers, phase 1 <- stress points (x, m, rho, sigma / rho^2 of xx
-// yy xy) and dummies (m, rho); phase 2 <- nodes (x, v, h, m, rho) over the
-// same bytes
+// node receivers, phase 1 <- stress points (m, rho, sigma / rho^2 of xx yy
+// xy; x only for the CSPM matrix) and dummies (m, rho); phase 2 <- nodes
+// (x, v, m, rho; h only when the smoothing lengths differ) over the same
+// bytes.  As above, the fields a launch may leave out come last in their
+// phase (lds_deriv_nodes on the host).
 constexpr LdsLayout layout_deriv_nodes() {
     LdsLayout L = layout_none();
     int o = 0;
-    L.x[KIND_SP] = o; o += 16 * CAP;
     L.m[KIND_SP] = o; o += 8 * CAP;
     L.rho[KIND_SP] = o; o += 8 * CAP;
     L.p0[KIND_SP] = o; o += 8 * CAP;
@@ -766,13 +795,14 @@ constexpr LdsLayout layout_deriv_nodes() {
     L.p2[KIND_SP] = o; o += 8 * CAP;
     L.m[KIND_DUMMY] = o; o += 8 * CAP_D;
     L.rho[KIND_DUMMY] = o; o += 8 * CAP_D;
+    L.x[KIND_SP] = o; o += 16 * CAP;
     const int phase1 = o;
     o = 0;
     L.x[KIND_NODE] = o; o += 16 * CAP;
     L.v[KIND_NODE] = o; o += 16 * CAP;
-    L.h[KIND_NODE] = o; o += 8 * CAP;
     L.m[KIND_NODE] = o; o += 8 * CAP;
     L.rho[KIND_NODE] = o; o += 8 * CAP;
+    L.h[KIND_NODE] = o; o += 8 * CAP;
     L.bytes = o > phase1 ? o : phase1;
     return L;
 }
@@ -1267,18 +1297,18 @@ __global__ __launch_bounds__(TILE) void k_count_lists(SearchView V, const double
     int self_base = 0;
     bool lds = T.fits;
     if (lds) {
-        plan_kind<KIND_NODE>(T, V, C, 0, lds_at<int>(LY.tab[KIND_NODE]), pl[KIND_NODE]);
-        plan_kind<KIND_SP>(T, V, C, 1, lds_at<int>(LY.tab[KIND_SP]), pl[KIND_SP]);
-        plan_kind<KIND_DUMMY>(T, V, C, 2, lds_at<int>(LY.tab[KIND_DUMMY]), pl[KIND_DUMMY]);
+        // kind k is planned by wave k mod the tile's waves (two waves: the
+        // first plans nodes and dummies)
+        plan_kind<KIND_NODE>(T, V, C, 0 % TWAVES, lds_at<int>(LY.tab[KIND_NODE]), pl[KIND_NODE]);
+        plan_kind<KIND_SP>(T, V, C, 1 % TWAVES, lds_at<int>(LY.tab[KIND_SP]), pl[KIND_SP]);
+        plan_kind<KIND_DUMMY>(T, V, C, 2 % TWAVES, lds_at<int>(LY.tab[KIND_DUMMY]), pl[KIND_DUMMY]);
         __syncthreads();
-        {   // keep the plans for the stage kernels: wave k stores kind k's
-            const int wave = (int)(threadIdx.x >> 6);
-            if (wave < 3) {
-                const int *src = reinterpret_cast<const int *>(&pl[wave]);
-                int *dst = reinterpret_cast<int *>(PL.plans + (size_t)ltile * 3 + wave);
-                for (int i = (int)(threadIdx.x & 63); i < (int)(sizeof(StagePlan) / sizeof(int)); i += 64)
-                    dst[i] = src[i];
-            }
+        // keep the plans for the stage kernels: the same wave stores each
+        for (int k = (int)(threadIdx.x >> 6); k < 3; k += TWAVES) {
+            const int *src = reinterpret_cast<const int *>(&pl[k]);
+            int *dst = reinterpret_cast<int *>(PL.plans + (size_t)ltile * 3 + k);
+            for (int i = (int)(threadIdx.x & 63); i < (int)(sizeof(StagePlan) / sizeof(int)); i += 64)
+                dst[i] = src[i];
         }
         lds = pl[KIND_NODE].total <= CAP && pl[KIND_SP].total <= CAP && pl[KIND_DUMMY].total <= CAP_D;
         self_base = pl[KR].mainlo;
@@ -1641,7 +1671,7 @@ __device__ __forceinline__ void sp_stage_tail(const DevParams &P, const StateBuf
 // velocity and stress and the nodes' (which the update leaves alone), so one
 // pass over the node partners after the update gives what the reference's
 // get_derivatives sees after its stress_point_update.
-__global__ __launch_bounds__(TILE) void k_sp_update(StateBufs S, Work W, SearchView V, PairLists PL,
+__global__ __launch_bounds__(TILE) STAGE_OCCUPANCY void k_sp_update(StateBufs S, Work W, SearchView V, PairLists PL,
                                                     DevParams P, double time_sph, double f2, int deriv,
                                                     int adapt, int nn, int nt) {
     __shared__ StagePlan pl[2];
@@ -1670,9 +1700,11 @@ __global__ __launch_bounds__(TILE) void k_sp_update(StateBufs S, Work W, SearchV
             double2 x, v;
             double m, rho;
         };
-        copy_kind<KIND_NODE>(T, V, C, nullptr, pl[0], [&](int g) { return NItem{S.x[g], S.vel[g], S.mass[g], S.rho[g]}; },
+        const bool stage_x = deriv && P.cspm;   // lds_sp_update allocates x only then
+        copy_kind<KIND_NODE>(T, V, C, nullptr, pl[0],
+                             [&](int g) { return NItem{stage_x ? S.x[g] : double2{}, S.vel[g], S.mass[g], S.rho[g]}; },
                              [&](int i, const NItem &n) {
-                                 lds_at<double2>(LY.x[KIND_NODE])[i] = n.x;
+                                 if (stage_x) lds_at<double2>(LY.x[KIND_NODE])[i] = n.x;
                                  lds_at<double2>(LY.v[KIND_NODE])[i] = n.v;
                                  lds_at<double>(LY.m[KIND_NODE])[i] = n.m;
                                  lds_at<double>(LY.rho[KIND_NODE])[i] = n.rho;
@@ -1895,7 +1927,7 @@ __device__ __forceinline__ void node_stage_tail(const StateBufs &S, const Work &
     W.rk_vel[r] = make_double2(rk.x + f2 * rh.x, rk.y + f2 * rh.y);
 }
 
-__global__ __launch_bounds__(TILE) void k_deriv_nodes(StateBufs S, Work W, SearchView V, PairLists PL,
+__global__ __launch_bounds__(TILE) STAGE_OCCUPANCY void k_deriv_nodes(StateBufs S, Work W, SearchView V, PairLists PL,
                                                       DevParams P, double gx_grav, double gy_grav, double f2,
                                                       int *__restrict__ err, int nn) {
     __shared__ StagePlan pl[2];
@@ -1939,9 +1971,11 @@ __global__ __launch_bounds__(TILE) void k_deriv_nodes(StateBufs S, Work W, Searc
             double m, rho;
             double4 st;
         };
-        copy_kind<KIND_SP>(T, V, C, nullptr, pl[0], [&](int g) { return SItem{S.x[g], S.mass[g], S.rho[g], S.stress[g]}; },
+        const bool stage_x = P.cspm;   // lds_deriv_nodes allocates x only then
+        copy_kind<KIND_SP>(T, V, C, nullptr, pl[0],
+                           [&](int g) { return SItem{stage_x ? S.x[g] : double2{}, S.mass[g], S.rho[g], S.stress[g]}; },
                            [&](int i, const SItem &v) {
-                               lds_at<double2>(LY.x[KIND_SP])[i] = v.x;
+                               if (stage_x) lds_at<double2>(LY.x[KIND_SP])[i] = v.x;
                                lds_at<double>(LY.m[KIND_SP])[i] = v.m;
                                lds_at<double>(LY.rho[KIND_SP])[i] = v.rho;
                                const double rs2 = v.rho * v.rho;
@@ -2013,12 +2047,14 @@ __global__ __launch_bounds__(TILE) void k_deriv_nodes(StateBufs S, Work W, Searc
                 double2 x, v;
                 double h, m, rho;
             };
+            const bool stage_h = !P.h_uni;   // lds_deriv_nodes allocates h only then
             copy_kind<KIND_NODE>(T, V, C, nullptr, pl[0],
-                                 [&](int g) { return NItem{S.x[g], S.vel[g], S.hsml[g], S.mass[g], S.rho[g]}; },
+                                 [&](int g) { return NItem{S.x[g], S.vel[g], stage_h ? S.hsml[g] : 0.0, S.mass[g],
+                                                           S.rho[g]}; },
                                  [&](int i, const NItem &n) {
                                      lds_at<double2>(LY.x[KIND_NODE])[i] = n.x;
                                      lds_at<double2>(LY.v[KIND_NODE])[i] = n.v;
-                                     lds_at<double>(LY.h[KIND_NODE])[i] = n.h;
+                                     if (stage_h) lds_at<double>(LY.h[KIND_NODE])[i] = n.h;
                                      lds_at<double>(LY.m[KIND_NODE])[i] = n.m;
                                      lds_at<double>(LY.rho[KIND_NODE])[i] = n.rho;
                                  });
@@ -2028,7 +2064,8 @@ __global__ __launch_bounds__(TILE) void k_deriv_nodes(StateBufs S, Work W, Searc
             double2 xs = A.x(kt, j), vs = A.vel(kt, j);
             float xij = (float)(xr.x - xs.x);
             float yij = (float)(xr.y - xs.y);
-            float hh = (float)(0.5 * (hr + A.h(kt, j)));
+            // one smoothing length for all: the partner's is the receiver's
+            float hh = (float)(0.5 * (hr + (P.h_uni ? hr : A.h(kt, j))));
             float rho2 = (float)(0.5 * (rhor + A.rho(kt, j)));
             float div_u = (float)((double)xij * (vr.x - vs.x));
             div_u = (float)((double)div_u + (double)yij * (vr.y - vs.y));
@@ -2189,17 +2226,68 @@ __global__ void k_rk_final(StateBufs S, Work W, double dt, int adapt, double tan
     }
 }
 
+// Leapfrog (no counterpart in the reference): the kick of the node
+// velocities by kdt with the accelerations of the step's state (W.rhs2).
+__global__ void k_lf_kick(StateBufs S, Work W, double kdt, int nn) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nn) return;
+    const double2 v0 = W.vel0[i], a = W.rhs2[i];
+    S.vel[i] = make_double2(v0.x + kdt * a.x, v0.y + kdt * a.y);
+}
+
+// Leapfrog's stress update: the stress points' stress by dt with the rate
+// of the kicked velocities (W.rhs1), adapt_stress2 of every particle, the
+// plastic strain rate of the step and update_strain (3_...:1847-1863).
+__global__ void k_lf_final(StateBufs S, Work W, double dt, int adapt, double tanfi, double coh, int nn, int nt,
+                           int *__restrict__ err) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nt) return;
+    material_tc(S, i, tanfi, coh);
+    const bool watched = S.ood[i] == 0;
+    if (i < nn) {
+        double4 s = S.stress[i];
+        if (adapt) {
+            adapt_stress(s, tanfi, coh);
+            S.stress[i] = s;
+        }
+        if (watched && !(finite4(s) && finite2(S.vel[i]))) atomicOr(err, ERR_NONFINITE);
+        W.ddev[i] = W.rk_dev[i];
+        return;
+    }
+    const double4 s0 = W.stress0[i], r = W.rhs1[i];
+    double4 s = make_double4(s0.x + dt * r.x, s0.y + dt * r.y, s0.z + dt * r.z, s0.w + dt * r.w);
+    if (adapt) adapt_stress(s, tanfi, coh);
+    S.stress[i] = s;
+    if (watched && !finite4(s)) atomicOr(err, ERR_NONFINITE);
+    const double dd = W.rk_dev[i];
+    W.ddev[i] = dd;
+    S.ivar1[i] = S.ivar1[i] + dt * dd;
+}
+
 // Position update (2_...:140-147) through the single-precision vel_half, and
 // the displacement of the nodes (2_...:171, 179).
 // xv (XSPH): the XSPH sums of k_xsph; the positions then follow
 // XSPH_update (2_...:234-235), x = x0 + dt (v + eps sum), eps = 0.5.
 __global__ void k_position(StateBufs S, Work W, double dt, int update_x, int nn, int nt,
-                           const double2 *__restrict__ xv, int *__restrict__ err) {
+                           const double2 *__restrict__ xv, int *__restrict__ err, int drift) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nt) return;
     double2 v0 = W.vx0[i], v = S.vel[i];
     double2 xn = make_double2(0.0, 0.0);
-    if (update_x && xv) {
+    if (drift) {   // leapfrog: the drift with the half-step velocities
+        if (update_x) {
+            const double2 x0 = W.x0[i];
+            xn = make_double2(x0.x + dt * v.x, x0.y + dt * v.y);
+            S.x[i] = xn;
+            if (i < nn) {
+                const double2 x00 = S.x00[i];
+                S.displ[i] = make_double2(xn.x - x00.x, xn.y - x00.y);
+            }
+        } else if (i < nn) {
+            const double2 d = S.displ[i];
+            S.displ[i] = make_double2(d.x + dt * v.x, d.y + dt * v.y);
+        }
+    } else if (update_x && xv) {
         const double eps = 0.5;
         const double2 x0 = W.x0[i], s = xv[i];
         xn = make_double2(x0.x + dt * (v.x + eps * s.x), x0.y + dt * (v.y + eps * s.y));

@@ -402,6 +402,7 @@ extern "C" int sps_get_surrogate_stats(sps_handle *h, int64_t *evaluated, int64_
 
 extern "C" int sps_set_plastic_log(sps_handle *h, int64_t cap) {
     if (!h || cap < 0 || cap > (1ll << 30)) return fail(SPS_EINVAL, "bad argument");
+    if (cap > 0 && h->integrator == SPS_INTEGRATOR_LEAPFROG) return fail(SPS_EINVAL, "the log records RK4 steps");
     HIP_TRY(hipSetDevice(h->device));
     int rc = dalloc(h->surr_count, 2) | dalloc(h->plog, (size_t)(cap ? cap : 1) * SPS_PLOG_ROW);
     if (rc) return rc;

@@ -94,11 +94,13 @@ def _slope_lattice(inst, dx):
     return (xn[keep], yn[keep], no_int[keep]), (xs[cell_in], ys[cell_in]), W + H * cot
 
 
-def build_ensemble(instances, dx=0.05, sml=1.2):
-    """One particle set of the instances; returns (Particles, material table
-    (n_instances x 10), particle -> instance, instance x offsets, domain box)."""
+def build_ensemble(instances, dx=0.05, sml=1.2, x_start=0.0):
+    """One particle set of the instances, side by side in x from x_start;
+    returns (Particles, material table (n_instances x 10), particle ->
+    instance, instance x offsets, domain box).  An instance built alone at
+    the offset it has in a batch has the same coordinates, bit for bit."""
     nodes, sps, walls_all, offs = [], [], [], []
-    x0 = 0.0
+    x0 = x_start
     for k, inst in enumerate(instances):
         (xn, yn, no_int), (xs, ys), xtoe = _slope_lattice(inst, dx)
         H = inst.height
@@ -166,11 +168,11 @@ def build_ensemble(instances, dx=0.05, sml=1.2):
     return parts, table, inst_of, np.array(offs), box
 
 
-def open_ensemble(sps, base_deck, instances, dx=0.05, mode=1, lib_path=None, skin=0.0):
+def open_ensemble(sps, base_deck, instances, dx=0.05, mode=1, lib_path=None, skin=0.0, x_start=0.0):
     """A Simulation of the ensemble with the base deck's controls (time step,
     gravity curve, damping, viscosity, CSPM), the instances' geometry and
-    materials."""
-    parts, table, inst_of, offs, box = build_ensemble(instances, dx, base_deck.sml)
+    materials; the first instance at x offset x_start."""
+    parts, table, inst_of, offs, box = build_ensemble(instances, dx, base_deck.sml, x_start)
     p = sps.make_params(base_deck, parts, mode=mode)
     p.dx = dx
     for a in range(2):

@@ -2,11 +2,13 @@
 
 Mirrors Read_2DMesh (code/3_SPH_material_2018.f90:346-732),
 set_up_dummy_nodes (:736-878) and Setup_Global_Arrays (:883-1026) for the
-inside approach with one stress point per lattice cell, including their
+inside approach with one to four stress points per lattice cell and the
+outside approach with one to three per node, including their
 single-precision arithmetic (the wall geometry is default real) and the
 lattice-count truncations.  Vectorised with numpy; every expression keeps the
 reference's operation order so the result is bit-identical (checked against
-dumps of the reference in tests/test_setup_golden.py).
+dumps of the reference in tests/test_oracle_golden.py,
+test_setup_matches_reference and test_setup_matches_reference_other_forks).
 """
 from __future__ import annotations
 

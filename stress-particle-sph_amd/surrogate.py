@@ -210,8 +210,10 @@ def augment(lib, s_n, s_tr, tanfi, coh, rng, copies=4, scale_max=100.0, material
     """More plastic trial states around the logged ones: the elastic
     increment sigma_trial - sigma_n scaled by log-uniform factors in
     [1, scale_max] (larger loading steps) and, when `material` = ((c_lo, c_hi),
-    (phi_lo_deg, phi_hi_deg)) is given, drawn materials; labelled with the
-    exact return map and kept where plastic.  Returns (s_n, s_tr, R, c, tanfi)."""
+    (phi_lo_deg, phi_hi_deg)) is given, drawn materials (otherwise each
+    row's own: `tanfi` and `coh` are one value or one per row); labelled
+    with the exact return map and kept where plastic.  Returns (s_n, s_tr,
+    R, c, tanfi)."""
     rows = []
     n = s_n.shape[0]
     for _ in range(copies):
@@ -222,8 +224,8 @@ def augment(lib, s_n, s_tr, tanfi, coh, rng, copies=4, scale_max=100.0, material
             c = rng.uniform(c0, c1, size=n)
             tf = np.tan(np.radians(rng.uniform(p0, p1, size=n)))
         else:
-            c = np.full(n, float(coh))
-            tf = np.full(n, float(tanfi))
+            c = np.broadcast_to(np.asarray(coh, dtype=np.float64), (n,)).copy()
+            tf = np.broadcast_to(np.asarray(tanfi, dtype=np.float64), (n,)).copy()
         ret = return_map(lib, tr, tf, c)
         plastic = np.any(ret != tr, axis=1) & correctable(tr, tf, c)
         rows.append((s_n[plastic], tr[plastic], ret[plastic], c[plastic], tf[plastic]))

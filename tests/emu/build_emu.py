@@ -103,18 +103,23 @@ def generate(gen):
         text = re.sub(r"typedef (short|float) (\w+) __attribute__\(\(ext_vector_type\(4\)\)\);",
                       r"typedef \1 \2 __attribute__((vector_size(4 * sizeof(\1))));", text)
         open(os.path.join(gen, name), "w").write(text)
-    open(os.path.join(gen, "sps_device.h"), "w").write(open(os.path.join(SRC, "sps_device.h")).read())
+    for name in ("sps_device.h", "sps_tiles.h"):
+        open(os.path.join(gen, name), "w").write(open(os.path.join(SRC, name)).read())
 
 
-def build(out=None, asan=True):
-    gen = os.path.join(BUILD, "gen")
+def build(out=None, asan=True, defines=()):
+    """The emulated library; `defines` are extra -D definitions (e.g. another
+    tile width), each build with its own generated sources."""
+    tag = "_".join(d.replace("=", "") for d in defines)
+    gen = os.path.join(BUILD, "gen" + ("_" + tag if tag else ""))
     generate(gen)
-    out = out or os.path.join(BUILD, "libsps_emu.so")
+    out = out or os.path.join(BUILD, f"libsps_emu{'_' + tag if tag else ''}.so")
     flags = ["-std=c++17", "-O1", "-g", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math",
              "-U_FORTIFY_SOURCE", "-D_FORTIFY_SOURCE=0", "-Wno-unknown-pragmas", "-Wno-unused-result",
              "-I" + HERE, "-I" + os.path.join(ROOT, "include"), "-I" + gen]
     if asan:
         flags += ["-fsanitize=address", "-fno-omit-frame-pointer"]
+    flags += ["-D" + d for d in defines]
     subprocess.run(["g++"] + flags + ["-o", out, os.path.join(gen, "sps_api_emu.cpp"),
                                       os.path.join(HERE, "emu_rt.cpp")], check=True)
     return out

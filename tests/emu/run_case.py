@@ -43,6 +43,11 @@ def main(emu_lib, oracle_lib, case, steps, tol="1e-4"):
         print(f"{case} {steps} steps ({sims[1].search_count()} searches): " +
               ("graph replay bitwise equal to direct launches" if not bad else "differ in " + ", ".join(bad)))
         return 1 if bad else 0
+    elif variant == "lf":       # the leapfrog integrator, against the oracle's
+        emu = sps.Simulation.from_deck(deck, lib_path=emu_lib)
+        ora = sps.Simulation.from_deck(deck, lib_path=oracle_lib)
+        emu.set_integrator(sps.INTEGRATOR_LEAPFROG)
+        ora.set_integrator(sps.INTEGRATOR_LEAPFROG)
     elif variant == "nan":      # the watchdog: a stress point's stress set to NaN
         codes = []
         for lib in (emu_lib, oracle_lib):

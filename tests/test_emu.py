@@ -6,10 +6,15 @@ This checks on the CPU what the GPU parity tests check on MI355X -- the
 kernel sources reproduce the reference bit for bit -- and, through the
 sanitizer, that no kernel reads or writes outside its buffers, its LDS or
 its stack on these decks.
+
+Every group of cases runs in a process of its own (each pays the start-up
+of the sanitizer build once), and the processes are started together when
+the module's first test asks for them, so the groups run side by side.
 """
 import os
 import subprocess
 import sys
+import threading
 
 import pytest
 
@@ -28,40 +33,16 @@ def _asan_runtime():
     return path if os.path.isabs(path) and os.path.exists(path) else None
 
 
-@pytest.fixture(scope="module")
-def emu_lib():
-    if _asan_runtime() is None:
-        pytest.skip("g++ with AddressSanitizer runtime not available")
-    if not os.path.exists(ORACLE_LIB):
-        pytest.skip("oracle library not built")
-    return build_emu.build()
-
-
-# several cases per process: each process pays the start-up of the
-# AddressSanitizer build once
+# exact cases: bitwise against the oracle
 EXACT_GROUPS = {
-    "tiny": [("t1_tiny", 20), ("t2_cspm", 20), ("t3_elastic", 20), ("t2_cspm:hvar", 20), ("t2_cspm:graph", 20),
-             ("t4_xsph", 20), ("p1_vm", 20), ("p4_mc", 20), ("t2_cspm:nan", 2)],
+    "tiny": [("t1_tiny", 20), ("t2_cspm", 20), ("t3_elastic", 20)],
+    "tiny2": [("t2_cspm:hvar", 20), ("t2_cspm:graph", 20), ("t4_xsph", 20)],
+    "models": [("p1_vm", 20), ("p4_mc", 20), ("t2_cspm:nan", 2), ("t2_cspm:lf", 20), ("v1_vslope_sp1:lf", 10)],
     "bui": [("b3_bui_sp3", 8), ("b4_bui_outside", 8), ("b6_bui_velvec_shift:kick", 8)],
-    "examples": [("c1_collapse", 2), ("v3_vslope_sp3", 10), ("l1_localisation", 6), ("l1_localisation:graph", 6)],
+    "collapse": [("c1_collapse", 2), ("v3_vslope_sp3", 10)],
+    "localisation": [("l1_localisation", 6), ("l1_localisation:graph", 6)],
     "surface": [("l2_localisation_free", 4), ("l3_localisation_xsph", 4)],
 }
-
-
-def run_cases(emu_lib, cases):
-    env = dict(os.environ, LD_PRELOAD=_asan_runtime(),
-               ASAN_OPTIONS="detect_leaks=0:detect_stack_use_after_return=0:halt_on_error=1")
-    argv = [a for c in cases for a in map(str, c)]
-    return subprocess.run([sys.executable, os.path.join(ROOT, "tests", "emu", "run_case.py"), emu_lib,
-                           ORACLE_LIB] + argv, env=env, capture_output=True, text=True, timeout=1500)
-
-
-@pytest.mark.parametrize("group", sorted(EXACT_GROUPS))
-def test_cpu_build_of_kernels_matches_oracle(emu_lib, group):
-    p = run_cases(emu_lib, EXACT_GROUPS[group])
-    print(p.stdout[-3000:], p.stderr[-4000:])
-    assert p.returncode == 0, p.stderr[-4000:] or p.stdout
-
 
 # the mixed-precision kernels: in bounds (AddressSanitizer) and within single-
 # precision rounding of the oracle where the problem does not amplify it (an
@@ -70,19 +51,105 @@ MIXED_CASES = [("t3_elastic:mixed", 20, "1e-5"), ("t2_cspm:mixed", 1, "1e-5"), (
                ("v3_vslope_sp3:mixed", 10, "1e-5"), ("l1_localisation:mixed", 6, "1e-5"),
                ("t4_xsph:mixed", 1, "1e-5"), ("b1_bui_sp1:skin", 8, "1e-5"), ("b8_bui_out3:skin", 8, "1e-5")]
 
+# the tile width and window capacities are build parameters (csrc/sps_tiles.h,
+# tune_tiles.py): the kernels built with another candidate -- narrower tiles,
+# smaller windows, so more tiles take the global walk -- give the same bits
+TILE_VARIANTS = {"t128s": ("SPS_TILE=128", "SPS_TROWS=16", "SPS_CAP=320", "SPS_CAP_D=96")}
+TILE_CASES = [("c1_collapse", 2), ("b4_bui_outside", 4), ("l2_localisation_free", 3), ("t2_cspm:graph", 10),
+              ("t4_xsph", 10), ("c1_collapse:mixed", 1, "5e-5")]
 
-def test_cpu_build_of_mixed_kernels_near_oracle(emu_lib):
-    p = run_cases(emu_lib, MIXED_CASES)
-    print(p.stdout[-3000:], p.stderr[-4000:])
-    assert p.returncode == 0, p.stderr[-4000:] or p.stdout
+
+class Runs:
+    """The case processes, started together; result(name) waits for one."""
+
+    def __init__(self):
+        self.env = dict(os.environ, LD_PRELOAD=_asan_runtime(),
+                        ASAN_OPTIONS="detect_leaks=0:detect_stack_use_after_return=0:halt_on_error=1")
+        self.procs, self.out = {}, {}
+
+    def start(self, name, argv):
+        self.procs[name] = subprocess.Popen([sys.executable] + argv, env=self.env, stdout=subprocess.PIPE,
+                                            stderr=subprocess.PIPE, text=True)
+
+    def cases(self, name, lib, cases):
+        self.start(name, [os.path.join(ROOT, "tests", "emu", "run_case.py"), lib, ORACLE_LIB] +
+                   [a for c in cases for a in map(str, c)])
+
+    def result(self, name, timeout=2400):
+        if name not in self.out:
+            p = self.procs[name]
+            try:
+                so, se = p.communicate(timeout=timeout)
+            except subprocess.TimeoutExpired:
+                p.kill()
+                so, se = p.communicate()
+            self.out[name] = (p.returncode, so, se)
+        rc, so, se = self.out[name]
+        print(so[-3000:], se[-4000:])
+        return rc, so, se
+
+    def stop(self):
+        for p in self.procs.values():
+            if p.poll() is None:
+                p.kill()
+                p.wait()
+
+
+@pytest.fixture(scope="module")
+def runs():
+    if _asan_runtime() is None:
+        pytest.skip("g++ with AddressSanitizer runtime not available")
+    if not os.path.exists(ORACLE_LIB):
+        pytest.skip("oracle library not built")
+    # the default build and the tile variants' builds side by side
+    libs, errs = {}, []
+
+    def build(key, defines):
+        try:
+            libs[key] = build_emu.build(defines=defines)
+        except Exception as e:          # reported by the tests that need the build
+            errs.append(f"{key}: {e}")
+
+    threads = [threading.Thread(target=build, args=(None, ()))]
+    threads += [threading.Thread(target=build, args=(v, d)) for v, d in TILE_VARIANTS.items()]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    if None not in libs:
+        pytest.fail("emulated build failed: " + "; ".join(errs))
+    r = Runs()
+    for g, cases in EXACT_GROUPS.items():
+        r.cases(g, libs[None], cases)
+    r.cases("mixed", libs[None], MIXED_CASES)
+    r.start("mlp", [os.path.join(ROOT, "tests", "emu", "run_mlp.py"), libs[None]])
+    for v in TILE_VARIANTS:
+        if v in libs:
+            r.cases("tile_" + v, libs[v], TILE_CASES)
+    yield r
+    r.stop()
+
+
+@pytest.mark.parametrize("group", sorted(EXACT_GROUPS))
+def test_cpu_build_of_kernels_matches_oracle(runs, group):
+    rc, so, se = runs.result(group)
+    assert rc == 0, se[-4000:] or so
+
+
+@pytest.mark.parametrize("variant", sorted(TILE_VARIANTS))
+def test_cpu_build_of_other_tile_width_matches_oracle(runs, variant):
+    assert "tile_" + variant in runs.procs, f"the {variant} build failed"
+    rc, so, se = runs.result("tile_" + variant)
+    assert rc == 0, se[-4000:] or so
+
+
+def test_cpu_build_of_mixed_kernels_near_oracle(runs):
+    rc, so, se = runs.result("mixed")
+    assert rc == 0, se[-4000:] or so
 
 
 # the network kernels (matrix-core GEMM emulated from its lane layout, loss,
 # backward, Adam) against the numpy restatement
-def test_cpu_build_of_network_kernels_matches_numpy_oracle(emu_lib):
-    env = dict(os.environ, LD_PRELOAD=_asan_runtime(),
-               ASAN_OPTIONS="detect_leaks=0:detect_stack_use_after_return=0:halt_on_error=1")
-    p = subprocess.run([sys.executable, os.path.join(ROOT, "tests", "emu", "run_mlp.py"), emu_lib], env=env,
-                       capture_output=True, text=True, timeout=900)
-    print(p.stdout[-2000:], p.stderr[-4000:])
-    assert p.returncode == 0, p.stderr[-4000:] or p.stdout
+def test_cpu_build_of_network_kernels_matches_numpy_oracle(runs):
+    rc, so, se = runs.result("mlp")
+    assert rc == 0, se[-4000:] or so

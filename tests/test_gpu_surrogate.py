@@ -64,14 +64,20 @@ def test_forward_and_gradient_match_numpy_oracle():
     lo, go = mo.loss_and_grad(theta, x, t)
     rel = np.linalg.norm(g - go) / np.linalg.norm(go)
     say(f"  loss {loss:.6e} vs {lo:.6e}, gradient rel {rel:.3e}")
+    # both sides round the same operands to bf16 (round to nearest even) and
+    # accumulate in fp32, in different orders: 1.5e-5 measured on MI355X
     assert abs(loss - lo) <= 1e-3 * lo
-    assert rel <= 2e-2
+    assert rel <= 1e-4
     # every parameter block, not just the norm
     offs = np.cumsum([0] + [int(np.prod(s)) for _, s in mo.SHAPES])
+    worst = {}
     for b, (name, _) in enumerate(mo.SHAPES):
         sl = slice(offs[b], offs[b + 1])
         if np.linalg.norm(go[sl]) > 0:
-            assert np.linalg.norm(g[sl] - go[sl]) <= 5e-2 * np.linalg.norm(go[sl]), name
+            worst[name] = float(np.linalg.norm(g[sl] - go[sl]) / np.linalg.norm(go[sl]))
+    say("  per block: " + ", ".join(f"{k} {v:.2e}" for k, v in worst.items()))
+    bad = {k: v for k, v in worst.items() if v > 1e-3}
+    assert not bad, bad
 
 
 def test_training_lowers_the_loss():
