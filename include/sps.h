@@ -358,6 +358,17 @@ int sps_mlp_gradient(sps_mlp *ml, int64_t n, const float *x, const float *t, flo
 int sps_mlp_train(sps_mlp *ml, int64_t n, const float *x, const float *t, int32_t epochs, int32_t batch,
                   float lr, uint64_t seed, float *epoch_loss);
 int sps_return_map(int64_t n, const double *in, const double *tanfi, const double *coh, double *out);
+/* sps_return_map without any allocation.  No counterpart in the reference;
+ * for callers that cannot allocate device or host memory per call (serving
+ * loops and harnesses that forbid hipMalloc).  Same arguments and results
+ * as sps_return_map, bit for bit, for n <= SPS_RETURN_MAP_NOALLOC_MAX
+ * (larger n: SPS_EINVAL).  The device staging of the inputs and outputs is
+ * one fixed array per GPU (80 bytes per stress, 20 MiB), which each GPU
+ * holds until the process ends; every call on a GPU shares that GPU's
+ * array, so calls that overlap on one GPU race.  Synchronous on return, on
+ * the current HIP device. */
+#define SPS_RETURN_MAP_NOALLOC_MAX 262144
+int sps_return_map_noalloc(int64_t n, const double *in, const double *tanfi, const double *coh, double *out);
 int sps_set_surrogate(sps_handle *h, sps_mlp *ml, double stress_scale, double coh_scale, double tol);
 int sps_get_surrogate_stats(sps_handle *h, int64_t *evaluated, int64_t *kept);
 int sps_set_plastic_log(sps_handle *h, int64_t cap);

@@ -1996,6 +1996,14 @@ int sps_return_map(int64_t n, const double *in, const double *tanfi, const doubl
     return 0;
 }
 
+int sps_return_map_noalloc(int64_t n, const double *in, const double *tanfi, const double *coh, double *out)
+{
+    if (n > SPS_RETURN_MAP_NOALLOC_MAX)
+        return fail(SPS_EINVAL, "%lld stresses exceed SPS_RETURN_MAP_NOALLOC_MAX (%d)", (long long)n,
+                    SPS_RETURN_MAP_NOALLOC_MAX);
+    return sps_return_map(n, in, tanfi, coh, out);
+}
+
 /* The learned stress update runs on the GPU only; the oracle has no network
  * (its CPU check is oracle/mlp_oracle.py). */
 int64_t sps_mlp_param_count(void) { return 37264; }

@@ -19,6 +19,7 @@
 #include <cstring>
 #include <vector>
 #include <algorithm>
+#include <mutex>
 
 #include "../../include/sps.h"
 #include "sps_kernels.hip"

@@ -328,6 +328,70 @@ extern "C" int sps_return_map(int64_t n, const double *in, const double *tanfi, 
     return rc;
 }
 
+// ------------------------------------------- the allocation-free return map
+// sps_return_map_noalloc stages its inputs and outputs in one fixed device
+// array per GPU instead of allocating: in (4 n doubles), out (4 n), tan(phi)
+// (n), c (n), each region SPS_RETURN_MAP_NOALLOC_MAX rows long.  Host code
+// never uses the array's name as an address: the address on each device
+// comes from hipGetSymbolAddress, taken once per device with that device
+// current and kept in rm_scratch_dev.
+constexpr long long RM_ROWS = SPS_RETURN_MAP_NOALLOC_MAX;
+static __device__ __align__(256) double scratch[10 * RM_ROWS];
+
+constexpr int RM_MAX_DEVICES = 64;
+static std::mutex rm_scratch_mu;
+static double *rm_scratch_dev[RM_MAX_DEVICES] = {};
+
+static int rm_scratch(int dev, double **out) {
+    if (dev < 0 || dev >= RM_MAX_DEVICES) return fail(SPS_EDEVICE, "device %d beyond %d", dev, RM_MAX_DEVICES);
+    std::lock_guard<std::mutex> lock(rm_scratch_mu);
+    if (!rm_scratch_dev[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(SPS_EDEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+        rm_scratch_dev[dev] = static_cast<double *>(p);
+    }
+    *out = rm_scratch_dev[dev];
+    return 0;
+}
+
+// The return map of row i with its own material read from two arrays.
+__global__ void k_return_map_rows(const double4 *__restrict__ in, const double *__restrict__ tanfi,
+                                  const double *__restrict__ coh, double4 *__restrict__ out, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double4 s = in[i];
+    adapt_stress(s, tanfi[i], coh[i]);
+    out[i] = s;
+}
+
+extern "C" int sps_return_map_noalloc(int64_t n, const double *in, const double *tanfi, const double *coh,
+                                      double *out) {
+    if (n < 0 || (n && (!in || !out || !tanfi || !coh))) return fail(SPS_EINVAL, "bad argument");
+    if (n > RM_ROWS)
+        return fail(SPS_EINVAL, "%lld stresses exceed SPS_RETURN_MAP_NOALLOC_MAX (%lld)", (long long)n, RM_ROWS);
+    if (n == 0) return 0;
+    int ndev = 0, dev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(SPS_EDEVICE, "no HIP device visible: the product library needs an MI355X (gfx950)");
+    HIP_TRY(hipGetDevice(&dev));
+    double *base = nullptr;
+    const int rc = rm_scratch(dev, &base);
+    if (rc) return rc;
+    double4 *in_d = reinterpret_cast<double4 *>(base), *out_d = reinterpret_cast<double4 *>(base + 4 * RM_ROWS);
+    double *tf_d = base + 8 * RM_ROWS, *c_d = base + 9 * RM_ROWS;
+    // every row the kernel reads is written first; it writes every out row
+    // the caller gets back, so nothing needs clearing between calls
+    HIP_TRY(hipMemcpy(in_d, in, (size_t)n * sizeof(double4), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(tf_d, tanfi, (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c_d, coh, (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+    k_return_map_rows<<<(unsigned)((n + 255) / 256), 256, 0, nullptr>>>(in_d, tf_d, c_d, out_d, (int)n);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpy(out, out_d, (size_t)n * sizeof(double4), hipMemcpyDeviceToHost));
+    return 0;
+}
+
 // ------------------------------------------------------- use in the step
 static SurrogateDev surrogate_dev(sps_handle *h) {
     SurrogateDev G{};

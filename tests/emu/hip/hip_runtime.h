@@ -215,6 +215,9 @@ inline hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int *nb, const vo
     return hipSuccess;
 }
 inline hipError_t hipGetDevice(int *d) { *d = 0; return hipSuccess; }
+// a __device__ variable is an ordinary host object here: its address is itself
+#define HIP_SYMBOL(x) (&(x))
+inline hipError_t hipGetSymbolAddress(void **p, const void *sym) { *p = const_cast<void *>(sym); return hipSuccess; }
 inline hipError_t hipGetDeviceCount(int *n) { *n = 1; return hipSuccess; }
 inline hipError_t hipStreamCreate(hipStream_t *s) { *s = nullptr; return hipSuccess; }
 inline hipError_t hipStreamCreateWithFlags(hipStream_t *s, unsigned) { *s = nullptr; return hipSuccess; }
