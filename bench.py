@@ -169,6 +169,11 @@ def run_gpu(args, sps, deck, mode=0, skin=0.0, net=None, graph=False, profile=Tr
     t1 = time.perf_counter()
     st = sim.stats()
     pairs = st["pairs_total"] - p0
+    if skin:
+        # lists kept over steps: each step is counted with the pairs within
+        # 2h that the last search found (list entries in the skin, beyond
+        # 2h, contribute zero and are not counted)
+        pairs = st["n_pairs"] * args.steps
     elapsed = t1 - t0
     prof = {}
     sim.events_s = None
