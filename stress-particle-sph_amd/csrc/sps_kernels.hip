@@ -1239,9 +1239,13 @@ __device__ __forceinline__ void fill_kernel_values(const unsigned long long *ip,
                                                    float4 *gyp, int tile, int n, const Fields &A, double2 xr,
                                                    double hr, double pi, KernelFactor &kf) {
     const int ng = (n + 3) >> 2;
+    // the lane's own words, written by its walk: the next one is loaded
+    // while the current one's kernel values are computed
+    unsigned long long next = ng > 0 ? ip[word_at(tile, 0)] : 0ull;
     for (int g = 0; g < ng; ++g) {
         const size_t o = word_at(tile, g);
-        const unsigned long long word = ip[o];
+        const unsigned long long word = next;
+        if (g + 1 < ng) next = ip[word_at(tile, g + 1)];
         float wv[4] = {0.f, 0.f, 0.f, 0.f}, gxv[4] = {0.f, 0.f, 0.f, 0.f}, gyv[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
