@@ -72,6 +72,10 @@ PLAN = {
     "l2_localisation_free": ([1, 2, 3, 10, 100, 500], [1, 2, 3, 10, 100, 500], False),
     # and with XSPH: its marks next to the surface drop prescribed nodes
     "l3_localisation_xsph": ([1, 2, 3, 10, 100, 300], [1, 2, 3, 10, 100, 300], False),
+    # the localisation test with the outside approach (two stress points per
+    # node, shifted every step): BCs, the position update and the free-surface
+    # detection together with the shifts
+    "l4_localisation_outside": ([1, 2, 3, 10, 100, 500], [1, 2, 3, 10, 100, 500], False),
     # BASELINE config 5 (biaxial compression, Drucker-Prager, rigid platens,
     # confined faces) at dx 1 mm, one material: 20,301 + 20,000 particles
     "c5_biaxial_dx1": ([1, 2, 10, 100, 300], [1], False),

@@ -159,13 +159,14 @@ def test_small_decks_vs_reference(deck_name):
 # the reference's BC time adds an uninitialised local (a subnormal there)
 SUBNORMAL = {("l1_localisation", 1), ("l1_localisation", 2), ("l2_localisation_free", 1),
              ("l2_localisation_free", 2), ("l2_localisation_free", 3), ("l3_localisation_xsph", 1),
-             ("l3_localisation_xsph", 2), ("l3_localisation_xsph", 3)}
+             ("l3_localisation_xsph", 2), ("l3_localisation_xsph", 3), ("l4_localisation_outside", 1),
+             ("l4_localisation_outside", 2)}
 
 
 @pytest.mark.parametrize("deck_name", ["c1_collapse", "b1_bui_sp1", "b2_bui_sp2", "b3_bui_sp3", "b4_bui_outside",
                                        "b5_bui_velvec", "b6_bui_velvec_shift", "v1_vslope_sp1", "v3_vslope_sp3",
                                        "l1_localisation", "b7_bui_out1", "b8_bui_out3", "l2_localisation_free",
-                                       "l3_localisation_xsph", "c5_biaxial_dx1"])
+                                       "l3_localisation_xsph", "c5_biaxial_dx1", "l4_localisation_outside"])
 def test_digest_decks_vs_reference(deck_name):
     """GPU state digests against the reference Fortran's dumps: the granular
     column collapse (1000 steps of dt = 1e-5) and the Bui
@@ -227,7 +228,8 @@ def test_scale_bitwise_vs_oracle(deck_name, nsteps):
 
 
 @pytest.mark.parametrize("deck_name", ["l2_localisation_free", "l3_localisation_xsph", "t4_xsph",
-                                       "b4_bui_outside", "c1_collapse", "c5_biaxial_dx1"])
+                                       "b4_bui_outside", "c1_collapse", "c5_biaxial_dx1",
+                                       "l4_localisation_outside"])
 def test_free_surface_vs_reference(deck_name):
     """Free-surface detection (3_...:1116-1411) on the GPU: BC_or_not of
     every particle and the double-precision normals bitwise against the

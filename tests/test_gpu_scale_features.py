@@ -7,7 +7,12 @@ few steps (the oracle finishes them in seconds).
   shifted every step at fixed offsets, the pre-step interpolation on.
 * The vertical cut with prescribed velocities and stresses on a lattice
   ten times finer (dx = 0.05 m, 40,401 + 40,000 particles).
-* The strain-localisation test on a lattice four times finer (51,681 + 51,200).
+* The strain-localisation test on a lattice four times finer (51,681 + 51,200),
+  and its variants with the traction-free surface (ifsigman 1), with XSPH,
+  and with the outside approach: their free surface (BC_or_not of every
+  particle and the normals) bitwise against the oracle's after every step,
+  with the pair counts past the previous high-water mark that the detection's
+  slot-order remap handles.
 """
 import dataclasses
 import sys
@@ -36,7 +41,7 @@ def variant(name, **change):
     return sps, deck
 
 
-def run_both(sps, deck, steps, tag):
+def run_both(sps, deck, steps, tag, surface=False):
     gpu = sps.Simulation.from_deck(deck)
     ora = sps.Simulation.from_deck(deck, lib_path=ORACLE_LIB)
     assert gpu.backend == "hip-gfx950"
@@ -48,6 +53,11 @@ def run_both(sps, deck, steps, tag):
         g, o = gpu.get_state(), ora.get_state()
         bad = [f"{k}: {describe_diff(o[k], g[k])}" for k in FIELDS if not bits_equal(o[k], g[k])]
         assert not bad, f"{tag} step {st}: " + "; ".join(bad)
+        if surface:
+            (gb, gn), (ob, on) = gpu.get_surface(), ora.get_surface()
+            assert np.array_equal(gb, ob), f"{tag} step {st}: BC_or_not"
+            assert bits_equal(on, gn), f"{tag} step {st}: normal {describe_diff(on, gn)}"
+            assert np.count_nonzero(gb == 2) > 100
     gpu.close()
     ora.close()
 
@@ -71,3 +81,9 @@ def test_bcs_at_scale():
 def test_localisation_at_scale():
     sps, deck = variant("l1_localisation", dx=0.003125)
     run_both(sps, deck, 5, "strain localisation, dx 3.125 mm")
+
+
+@pytest.mark.parametrize("name", ["l2_localisation_free", "l3_localisation_xsph", "l4_localisation_outside"])
+def test_free_surface_at_scale(name):
+    sps, deck = variant(name, dx=0.003125)
+    run_both(sps, deck, 5, f"{name}, dx 3.125 mm", surface=True)

@@ -80,6 +80,7 @@ DIGEST = {"c1_collapse": "BASELINE config 1 (5,151 + 5,000 particles, 1000 steps
           "v1_vslope_sp1": "elastic vertical cut with prescribed velocities and stresses, 2000 steps",
           "v3_vslope_sp3": "the same cut with three stress points per cell, 1000 steps",
           "l1_localisation": "biaxial strain localisation: Perzyna von Mises, BCs with the position update",
+          "l4_localisation_outside": "the same with the outside approach (two points per node, shifted every step)",
           "b7_bui_out1": "outside approach, one point per node, 500 steps",
           "b8_bui_out3": "outside approach, three points per node, 500 steps",
           "l2_localisation_free": "the localisation test with the traction-free surface (ifsigman 1), 500 steps",
@@ -94,7 +95,8 @@ DIGEST = {"c1_collapse": "BASELINE config 1 (5,151 + 5,000 particles, 1000 steps
 # the reference's own pair order at those steps depends on the garbage value
 SUBNORMAL = {("l1_localisation", 1), ("l1_localisation", 2), ("l2_localisation_free", 1),
              ("l2_localisation_free", 2), ("l2_localisation_free", 3), ("l3_localisation_xsph", 1),
-             ("l3_localisation_xsph", 2), ("l3_localisation_xsph", 3)}
+             ("l3_localisation_xsph", 2), ("l3_localisation_xsph", 3), ("l4_localisation_outside", 1),
+             ("l4_localisation_outside", 2)}
 
 
 def fixture_steps(g):
@@ -158,7 +160,7 @@ def test_oracle_digest_decks(deck_name, oracle_lib):
 # prescribed BCs (l2, l3) and without (the position update alone runs the
 # detection every step, 2_SPH_main_2018.f90:150-154)
 SURFACE_DECKS = ["l2_localisation_free", "l3_localisation_xsph", "t4_xsph", "b4_bui_outside", "c1_collapse",
-                 "c5_biaxial_dx1"]
+                 "c5_biaxial_dx1", "l4_localisation_outside"]
 
 
 def surface_matches(g, st, bc, nrm):
