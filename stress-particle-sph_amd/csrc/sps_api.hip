@@ -137,6 +137,12 @@ struct sps_handle {
     // step work arrays
     DBuf<double2> x0, vel0, rk_vel, rhs2, f_bound, art_visc;   // vel0 is also the step's vx0
     DBuf<double4> stress0, rk_stress, rhs1, grad_u;
+    // the stress points' stresses after a stage's second return map, which
+    // k_sp_update writes here so that k_spu_nodes, reading the first, can
+    // run beside it (on stream2; ev_fork / ev_join order the two)
+    DBuf<double4> sig2;
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     DBuf<double> rk_dev, ddev;
     // search
     int cell_cap = 0;
@@ -483,8 +489,12 @@ extern "C" int sps_create(const sps_params *params, sps_handle **out) {
         sps_destroy(h);
         return code;
     };
-    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(SPS_EDEVICE, "hipStreamCreate failed"));
+    if (hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming) != hipSuccess)
+        return bail(fail(SPS_EDEVICE, "hipEventCreate failed"));
     int nt2 = h->nt2, nt = h->nt, nn = h->nn;
     rc = 0;
     rc |= h->S[0].alloc(nn, nt, nt2);
@@ -493,6 +503,7 @@ extern "C" int sps_create(const sps_params *params, sps_handle **out) {
     rc |= dalloc(h->rk_vel, nn); rc |= dalloc(h->rhs2, nn); rc |= dalloc(h->f_bound, nn);
     rc |= dalloc(h->art_visc, nn); rc |= dalloc(h->stress0, nt); rc |= dalloc(h->rk_stress, nt);
     rc |= dalloc(h->rhs1, nt); rc |= dalloc(h->grad_u, nt); rc |= dalloc(h->rk_dev, nt);
+    rc |= dalloc(h->sig2, nt);
     rc |= dalloc(h->ddev, nt);
     rc |= dalloc(h->grid, 1); rc |= dalloc(h->split, 1); rc |= dalloc(h->err, 1); rc |= dalloc(h->pairs_total, 1);
     h->nbparts = std::min(1024, (int)nblk(nt2));
@@ -992,7 +1003,7 @@ static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
             LAUNCH_LDS(K_SPU_NODES, ntile8(nn), TILE, LAY_SPU_NODES.bytes, k_spu_nodes, SB, V, PL, adapt, tanfi, coh, nn);
         if (nt > nn)
             LAUNCH_LDS(K_SP_UPDATE, ntile8(nt - nn), TILE, lds_sp_update(h, 0), k_sp_update, SB, W, V, PL, h->dp,
-                       time_sph, 0.0, 0, adapt, nn, nt);
+                       time_sph, 0.0, 0, adapt, nn, nt, nullptr);
     }
     const bool lf = h->integrator == SPS_INTEGRATOR_LEAPFROG;
     LAUNCH(K_RK_INIT, nblk(nt2), BLOCK, k_stage_first, SB, W, adapt, tanfi, coh, nn, nt, nt2);
@@ -1007,10 +1018,10 @@ static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
         if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, h->ifsigman, h->normal.p, nn);
         if (nt > nn)
             LAUNCH_LDS(K_SP_UPDATE, ntile8(nt - nn), TILE, lds_sp_update(h, 0), k_sp_update, SB, W, V, PL, h->dp, time_sph,
-                       0.0, 0, adapt, nn, nt);
+                       0.0, 0, adapt, nn, nt, nullptr);
         if (nn > 0)
             LAUNCH_LDS(K_DERIV_NODES, ntile8(nn), TILE, lds_deriv_nodes(h), k_deriv_nodes, SB, W, V, PL, h->dp, gxg, gyg,
-                       1.0, h->err.p, nn);
+                       1.0, h->err.p, nn, SB.stress);
         LAUNCH(K_STAGE_BEGIN, nblk(nn), BLOCK, k_lf_kick, SB, W, kdt, nn);
         if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, h->ifsigman, h->normal.p, nn);
         if (nn > 0)
@@ -1018,7 +1029,7 @@ static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
         if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, h->ifsigman, h->normal.p, nn);
         if (nt > nn)
             LAUNCH_LDS(K_SP_UPDATE, ntile8(nt - nn), TILE, lds_sp_update(h, 1), k_sp_update, SB, W, V, PL, h->dp, time_sph,
-                       1.0, 1, adapt, nn, nt);
+                       1.0, 1, adapt, nn, nt, nullptr);
         if (nt2 > nt) LAUNCH(K_DUMMY_STRESS, nblk(nt2 - nt), BLOCK, k_dummy_stress, SB, V, nt, nt2);
         LAUNCH(K_RK_FINAL, nblk(nt), BLOCK, k_lf_final, SB, W, dt, adapt, tanfi, coh, nn, nt, h->err.p);
         h->lf_first = false;
@@ -1027,15 +1038,30 @@ static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
         const double c1 = f1rk[st] * (dt);
         if (st > 0) LAUNCH(K_STAGE_BEGIN, nblk(nt), BLOCK, k_stage_begin, SB, W, c1, adapt, tanfi, coh, nn, nt);
         if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, h->ifsigman, h->normal.p, nn);
-        if (nn > 0)
+        // stress_point_update's two halves are independent: the node side
+        // reads the stress points' stresses before their second return map
+        // (k_sp_update leaves that in sig2) and changes node velocities only
+        // where no stress point reads them (nodes without stress-point
+        // partners); the second BC call sets the node velocities the first
+        // set.  So they run side by side, unless kernel timing (one stream)
+        // is on.
+        const bool fork = !h->prof && nn > 0 && nt > nn;
+        if (fork) {
+            HIP_TRY(hipEventRecord(h->ev_fork, s));
+            HIP_TRY(hipStreamWaitEvent(h->stream2, h->ev_fork, 0));
+            k_spu_nodes<<<ntile8(nn), TILE, LAY_SPU_NODES.bytes, h->stream2>>>(SB, V, PL, adapt, tanfi, coh, nn);
+            HIP_TRY(hipEventRecord(h->ev_join, h->stream2));
+        } else if (nn > 0) {
             LAUNCH_LDS(K_SPU_NODES, ntile8(nn), TILE, LAY_SPU_NODES.bytes, k_spu_nodes, SB, V, PL, adapt, tanfi, coh, nn);
-        if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, h->ifsigman, h->normal.p, nn);
+        }
         if (nt > nn)
             LAUNCH_LDS(K_SP_UPDATE, ntile8(nt - nn), TILE, lds_sp_update(h, 1), k_sp_update, SB, W, V, PL, h->dp, time_sph,
-                       f2rk[st], 1, adapt, nn, nt);
+                       f2rk[st], 1, adapt, nn, nt, h->sig2.p);
+        if (fork) HIP_TRY(hipStreamWaitEvent(s, h->ev_join, 0));
+        if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, h->ifsigman, h->normal.p, nn);
         if (nn > 0)
             LAUNCH_LDS(K_DERIV_NODES, ntile8(nn), TILE, lds_deriv_nodes(h), k_deriv_nodes, SB, W, V, PL, h->dp, gxg, gyg,
-                       f2rk[st], h->err.p, nn);
+                       f2rk[st], h->err.p, nn, nt > nn ? h->sig2.p : SB.stress);
         if (st == 3 && nt2 > nt)
             LAUNCH(K_DUMMY_STRESS, nblk(nt2 - nt), BLOCK, k_dummy_stress, SB, V, nt, nt2);
     }
@@ -1048,7 +1074,7 @@ static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
         LAUNCH_LDS(K_SPU_NODES, ntile8(nn), TILE, LAY_SPU_NODES.bytes, k_spu_nodes, SB, V, PL, adapt, tanfi, coh, nn);
     if (nt > nn)
         LAUNCH_LDS(K_SP_UPDATE, ntile8(nt - nn), TILE, lds_sp_update(h, 0), k_sp_update, SB, W, V, PL, h->dp, time_sph, 0.0,
-                   0, adapt, nn, nt);
+                   0, adapt, nn, nt, nullptr);
     if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, h->ifsigman, h->normal.p, nn);
     const bool xsph = h->xsph && h->p.update_x;
     if (xsph && nn > 0)
@@ -1422,6 +1448,9 @@ extern "C" void sps_destroy(sps_handle *h) {
     dfree(h->surf_gy); dfree(h->surf_cnt); dfree(h->surf_start); dfree(h->surf_cur); dfree(h->surf_ent);
     dfree(h->surf_flag); dfree(h->surf_tt); dfree(h->surf_tau);
     if (h->pinned) (void)hipHostFree(h->pinned);
+    if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
+    if (h->ev_join) (void)hipEventDestroy(h->ev_join);
+    if (h->stream2) (void)hipStreamDestroy(h->stream2);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }

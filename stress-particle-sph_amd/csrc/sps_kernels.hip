@@ -1675,9 +1675,12 @@ __device__ __forceinline__ void sp_stage_tail(const DevParams &P, const StateBuf
 // velocity and stress and the nodes' (which the update leaves alone), so one
 // pass over the node partners after the update gives what the reference's
 // get_derivatives sees after its stress_point_update.
+// sig_out: where the stress after the return map goes (null: the state's
+// stress; the RK stages give a separate buffer, so that the node side of
+// stress_point_update can read the stresses before it at the same time).
 __global__ __launch_bounds__(TILE) STAGE_OCCUPANCY void k_sp_update(StateBufs S, Work W, SearchView V, PairLists PL,
                                                     DevParams P, double time_sph, double f2, int deriv,
-                                                    int adapt, int nn, int nt) {
+                                                    int adapt, int nn, int nt, double4 *__restrict__ sig_out) {
     __shared__ StagePlan pl[2];
     constexpr const LdsLayout &LY = LAY_SP_UPDATE;
     const WalkCtx C = walk_ctx(V);
@@ -1766,10 +1769,11 @@ __global__ __launch_bounds__(TILE) STAGE_OCCUPANCY void k_sp_update(StateBufs S,
     }
     double4 st = S.stress[r];
     const DevParams Pm = material_params(P, S, r);
-    if (adapt) {
-        adapt_stress(st, Pm.tanfi, Pm.coh);
+    if (adapt) adapt_stress(st, Pm.tanfi, Pm.coh);
+    if (sig_out)
+        sig_out[r] = st;
+    else if (adapt)
         S.stress[r] = st;
-    }
     if (!deriv) return;
 
     // get_derivatives
@@ -1931,9 +1935,12 @@ __device__ __forceinline__ void node_stage_tail(const StateBufs &S, const Work &
     W.rk_vel[r] = make_double2(rk.x + f2 * rh.x, rk.y + f2 * rh.y);
 }
 
+// sp_stress: the stress points' stresses after the stage's second return
+// map (k_sp_update's sig_out, or the state's stress).
 __global__ __launch_bounds__(TILE) STAGE_OCCUPANCY void k_deriv_nodes(StateBufs S, Work W, SearchView V, PairLists PL,
                                                       DevParams P, double gx_grav, double gy_grav, double f2,
-                                                      int *__restrict__ err, int nn) {
+                                                      int *__restrict__ err, int nn,
+                                                      const double4 *__restrict__ sp_stress) {
     __shared__ StagePlan pl[2];
     constexpr const LdsLayout &LY = LAY_DERIV_NODES;
     const WalkCtx C = walk_ctx(V);
@@ -1977,7 +1984,7 @@ __global__ __launch_bounds__(TILE) STAGE_OCCUPANCY void k_deriv_nodes(StateBufs 
         };
         const bool stage_x = P.cspm;   // lds_deriv_nodes allocates x only then
         copy_kind<KIND_SP>(T, V, C, nullptr, pl[0],
-                           [&](int g) { return SItem{stage_x ? S.x[g] : double2{}, S.mass[g], S.rho[g], S.stress[g]}; },
+                           [&](int g) { return SItem{stage_x ? S.x[g] : double2{}, S.mass[g], S.rho[g], sp_stress[g]}; },
                            [&](int i, const SItem &v) {
                                if (stage_x) lds_at<double2>(LY.x[KIND_SP])[i] = v.x;
                                lds_at<double>(LY.m[KIND_SP])[i] = v.m;
@@ -2031,8 +2038,12 @@ __global__ __launch_bounds__(TILE) STAGE_OCCUPANCY void k_deriv_nodes(StateBufs 
         if (lds)
             la.run(LF, grad);
         else
-            walk_stencil<MASK_SP | MASK_DUMMY>(GlobalCells{V.kstart, C.g, C.L}, GlobalFields{S}, C.g, cx, cy,
+        {
+            StateBufs S2 = S;   // stress points' stresses from sp_stress (the walk reads no node's)
+            S2.stress = const_cast<double4 *>(sp_stress);
+            walk_stencil<MASK_SP | MASK_DUMMY>(GlobalCells{V.kstart, C.g, C.L}, GlobalFields{S2}, C.g, cx, cy,
                                                xr, hr, KIND_NODE, r, rev_g, C.any_fwd, V.pi, cls, grad);
+        }
     }
 
     // phase 2: artificial viscosity over node partners (type 3), single precision

@@ -13,7 +13,8 @@ decks for the GPU tests that compare whole runs with it:
       s<s>_stress_absmax the largest |stress| (a scale for error reports)
 
 Runs on the CPU of the build container (about 15 minutes for c2_slope_dx02
-over 1000 steps on one thread).
+over 1000 steps on one thread; the oracle uses every OpenMP thread, bitwise
+the same).
 Usage: python tests/golden/make_oracle_digests.py [deck ...]
 """
 import hashlib
@@ -37,6 +38,8 @@ FIELDS = ("x", "vel", "stress", "internal_vars", "f_drucker", "ddev_strn", "grad
 PLAN = {
     # BASELINE config 2 at dx 0.02 (251,001 + 250,000 particles), FP64
     "c2_slope_dx02": [1, 2, 10, 100, 500, 1000],
+    # and at dx 0.01 (1,002,001 + 1,000,000 particles: the bench's deck)
+    "c2_slope_dx01": [1, 2, 10, 100, 500, 1000],
 }
 
 

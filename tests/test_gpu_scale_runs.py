@@ -1,11 +1,12 @@
 """The BASELINE configurations at their stated sizes, on the GPU.
 
-* C2 (the cohesive slope, dx 0.02 m: 251,001 + 250,000 particles): 1000
-  FP64 steps, every state field and the pair list bitwise against the
-  oracle's digests at steps 1, 2, 10, 100, 500 and 1000
-  (tests/golden/c2_slope_dx02_oracle.npz, written in the build container by
-  tests/golden/make_oracle_digests.py; the oracle itself is pinned to the
-  compiled reference on the smaller decks).
+* C2 (the cohesive slope at dx 0.02 m, 251,001 + 250,000 particles, and at
+  dx 0.01 m, 1,002,001 + 1,000,000, the bench's deck): 1000 FP64 steps,
+  every state field and the pair list bitwise against the oracle's digests
+  at steps 1, 2, 10, 100, 500 and 1000 (tests/golden/<deck>_oracle.npz,
+  written in the build container by tests/golden/make_oracle_digests.py;
+  the oracle itself is pinned to the compiled reference on the smaller
+  decks).
 * C3 (64 slope instances from seed 2024 at dx 0.05 m, 7.7 M particles): the
   batch in the FP64 form bitwise against the oracle running the same batch
   over a step (seconds per step for the oracle on the host's threads); then
@@ -67,12 +68,13 @@ def pair_digest(p):
     return hashlib.sha256(rec.tobytes()).hexdigest()
 
 
-def test_c2_dx02_1000_steps_vs_oracle_digests():
+@pytest.mark.parametrize("deck_name", ["c2_slope_dx02", "c2_slope_dx01"])
+def test_c2_1000_steps_vs_oracle_digests(deck_name):
     sps = load_pkg()
-    g = golden("c2_slope_dx02_oracle")
+    g = golden(f"{deck_name}_oracle")
     steps = sorted(int(k[1:].split("_")[0]) for k in g.files if k.endswith("_sha_x"))
-    say(f"[c2 dx 0.02] checkpoints {steps}")
-    gpu = sps.Simulation.from_deck(sps.read_deck(deck_dir("c2_slope_dx02")))
+    say(f"[{deck_name}] checkpoints {steps}")
+    gpu = sps.Simulation.from_deck(sps.read_deck(deck_dir(deck_name)))
     done, t0 = 0, time.time()
     for st in steps:
         gpu.step(st - done)
