@@ -701,6 +701,22 @@ static int search(sps_handle *h, bool fresh_list) {
     {
         const SearchView V{h->cell.p, h->kstart.p, h->proc.p, h->cnt.p, h->off.p, h->grid.p, h->split.p, h->p.pi};
         const PairLists PL = h->lists();
+        // the stress-point receivers' lists (their tiles, counts and list
+        // words are their own; the type histogram is summed with atomics)
+        // are built beside the nodes' on the second stream, unless kernel
+        // timing (one stream) is on
+        const bool fork = !h->prof && nn > 0 && nt > nn;
+        if (fork) {
+            HIP_TRY(hipEventRecord(h->ev_fork, h->stream));
+            HIP_TRY(hipStreamWaitEvent(h->stream2, h->ev_fork, 0));
+            if (!h->mixed)
+                k_count_lists<KIND_SP, true><<<ntile8(nt - nn), TILE, LAY_COUNT.bytes, h->stream2>>>(
+                    V, C.x.p, C.hsml.p, h->cnt.p, h->split.p, PL, h->h_uniform, nn, nt - nn, h->skin);
+            else
+                k_count_lists<KIND_SP, false><<<ntile8(nt - nn), TILE, LAY_COUNT.bytes, h->stream2>>>(
+                    V, C.x.p, C.hsml.p, h->cnt.p, h->split.p, PL, h->h_uniform, nn, nt - nn, h->skin);
+            HIP_TRY(hipEventRecord(h->ev_join, h->stream2));
+        }
         // the mixed-precision stage kernels compute the kernel values themselves
         if (nn > 0 && !h->mixed)
             LAUNCH_LDS(K_COUNT_PAIRS, ntile8(nn), TILE, LAY_COUNT.bytes, (k_count_lists<KIND_NODE, true>), V, C.x.p,
@@ -708,15 +724,16 @@ static int search(sps_handle *h, bool fresh_list) {
         if (nn > 0 && h->mixed)
             LAUNCH_LDS(K_COUNT_PAIRS, ntile8(nn), TILE, LAY_COUNT.bytes, (k_count_lists<KIND_NODE, false>), V, C.x.p,
                        C.hsml.p, h->cnt.p, h->split.p, PL, h->h_uniform, 0, nn, h->skin);
-        if (nt > nn && !h->mixed)
+        if (!fork && nt > nn && !h->mixed)
             LAUNCH_LDS(K_COUNT_PAIRS, ntile8(nt - nn), TILE, LAY_COUNT.bytes, (k_count_lists<KIND_SP, true>), V, C.x.p,
                        C.hsml.p, h->cnt.p, h->split.p, PL, h->h_uniform, nn, nt - nn, h->skin);
-        if (nt > nn && h->mixed)
+        if (!fork && nt > nn && h->mixed)
             LAUNCH_LDS(K_COUNT_PAIRS, ntile8(nt - nn), TILE, LAY_COUNT.bytes, (k_count_lists<KIND_SP, false>), V, C.x.p,
                        C.hsml.p, h->cnt.p, h->split.p, PL, h->h_uniform, nn, nt - nn, h->skin);
         if (nt2 > nt)
             LAUNCH(K_COUNT_PAIRS, nblk8(nt2 - nt), BLOCK, k_count, C.x.p, C.hsml.p, h->cell.p, h->kstart.p, h->proc.p,
                    h->grid.p, h->cnt.p, h->split.p, nn, nt, nt, nt2);
+        if (fork) HIP_TRY(hipStreamWaitEvent(h->stream, h->ev_join, 0));
     }
     HIP_TRY(hipGetLastError());
     // discovery offsets over all slots (slots past the in-domain count hold 0)
