@@ -114,11 +114,13 @@ def implementation_bytes(kernel, st, counts, params):
     cspm = params.cspm
     av = params.alpha > 0 or params.beta > 0
     if kernel == "k_deriv_nodes":
-        lists = 10 * (P1 + P6) + (10 * 2 * P3 if av else 0)
-        node = 16 + 16 + 32 + 8 + 8 + 8 + 16 + 16 + 16 + 16 + 16   # x v sigma m rho h rk_vel(rw) rhs2 av fb
-        sp = 24 + 8 + 8 + (16 if cspm else 0)                        # sigma(xx,yy,xy) m rho (x)
-        dummy = 16 + 8 + 8 + 8 + 4                                   # x m rho h orig
+        lists = 10 * (P1 + P6)
+        node = 16 + 16 + 32 + 8 + 8 + 8 + 16 + 16 + 16 + 8 + 16 + 16   # x v sigma m rho h rk_vel(rw) rhs2 av(in, out) fb
+        sp = 24 + 8 + 8 + (16 if cspm else 0)                            # sigma(xx,yy,xy) m rho (x)
+        dummy = 8 + 8 + 4                                                # m rho orig
         return lists + nn * node + ns * sp + nd * dummy
+    if kernel == "k_av_nodes":
+        return (10 * P3 + nn * (16 + 16 + 8 + 8 + 8 + 8)) if av else 0   # list B; x v m rho h, av out
     if kernel == "k_sp_update":
         # four launches per step with the derivative, one (after the last stage) without
         deriv = 14 * (P1 + P9) + ns * (16 + 16 + 32 + 32 + 8 + 32 + 32 + 64 + 16) \

@@ -141,6 +141,7 @@ struct sps_handle {
     // k_sp_update writes here so that k_spu_nodes, reading the first, can
     // run beside it (on stream2; ev_fork / ev_join order the two)
     DBuf<double4> sig2;
+    DBuf<float2> av_nodes;   // the nodes' artificial viscosity of a stage (k_av_nodes)
     hipStream_t stream2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     DBuf<double> rk_dev, ddev;
@@ -251,13 +252,14 @@ enum KernelId {
     K_CHECK_OOD, K_BOUNDS, K_CELL_COUNT, K_CELL_SCATTER, K_CELL_SORT, K_PERMUTE, K_PROC,
     K_COUNT_PAIRS, K_SCAN, K_SPLIT, K_RK_INIT, K_STAGE_BEGIN, K_SPU_NODES,
     K_SP_UPDATE, K_DERIV_NODES, K_DUMMY_STRESS, K_RK_FINAL, K_POSITION, K_SURROGATE, K_XSPH, K_SHIFT, K_BCS, K_SURF,
-    K_NKERNELS
+    K_AV_NODES, K_NKERNELS
 };
 static const char *const kKernelNames[K_NKERNELS] = {
     "k_check_ood", "k_bounds", "k_cell_count", "k_cell_scatter", "k_cell_sort", "k_permute",
     "k_proc_slot", "k_count", "k_scan", "k_split", "k_stage_first",
     "k_stage_begin", "k_spu_nodes", "k_sp_update", "k_deriv_nodes",
-    "k_dummy_stress", "k_rk_final", "k_position", "k_surrogate", "k_xsph", "k_shift_sp", "k_bcs", "k_surf"};
+    "k_dummy_stress", "k_rk_final", "k_position", "k_surrogate", "k_xsph", "k_shift_sp", "k_bcs", "k_surf",
+    "k_av_nodes"};
 
 static hipEvent_t prof_begin(sps_handle *h) {
     if (!h->prof) return nullptr;
@@ -503,7 +505,7 @@ extern "C" int sps_create(const sps_params *params, sps_handle **out) {
     rc |= dalloc(h->rk_vel, nn); rc |= dalloc(h->rhs2, nn); rc |= dalloc(h->f_bound, nn);
     rc |= dalloc(h->art_visc, nn); rc |= dalloc(h->stress0, nt); rc |= dalloc(h->rk_stress, nt);
     rc |= dalloc(h->rhs1, nt); rc |= dalloc(h->grad_u, nt); rc |= dalloc(h->rk_dev, nt);
-    rc |= dalloc(h->sig2, nt);
+    rc |= dalloc(h->sig2, nt); rc |= dalloc(h->av_nodes, nn > 0 ? nn : 1);
     rc |= dalloc(h->ddev, nt);
     rc |= dalloc(h->grid, 1); rc |= dalloc(h->split, 1); rc |= dalloc(h->err, 1); rc |= dalloc(h->pairs_total, 1);
     h->nbparts = std::min(1024, (int)nblk(nt2));
@@ -955,17 +957,18 @@ static int free_surface(sps_handle *h, const StateBufs &SB) {
 }
 
 // Dynamic LDS of the FP64 stage kernels: their layouts put the fields a
-// launch may leave out last (the nodes' x without CSPM, the dummies for the
+// launch may leave out last (the partners' x without CSPM, the dummies for the
 // update alone, h when every particle has the same smoothing length), and
 // a launch allocates the bytes before them.
 static size_t lds_sp_update(const sps_handle *h, int deriv) {
     if (!deriv) return (size_t)LAY_SP_UPDATE.x[KIND_DUMMY];
     return (size_t)LAY_SP_UPDATE.x[KIND_NODE] + (h->dp.cspm ? 16 * (size_t)CAP : 0);
 }
+static size_t lds_av_nodes(const sps_handle *h) {
+    return (size_t)LAY_AV_NODES.h[KIND_NODE] + (h->dp.h_uni ? 0 : 8 * (size_t)CAP);
+}
 static size_t lds_deriv_nodes(const sps_handle *h) {
-    const size_t p1 = (size_t)LAY_DERIV_NODES.x[KIND_SP] + (h->dp.cspm ? 16 * (size_t)CAP : 0);
-    const size_t p2 = h->dp.do_av ? (size_t)LAY_DERIV_NODES.h[KIND_NODE] + (h->dp.h_uni ? 0 : 8 * (size_t)CAP) : 0;
-    return p1 > p2 ? p1 : p2;
+    return (size_t)LAY_DERIV_NODES.x[KIND_SP] + (h->dp.cspm ? 16 * (size_t)CAP : 0);
 }
 
 static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
@@ -1036,9 +1039,11 @@ static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
         if (nt > nn)
             LAUNCH_LDS(K_SP_UPDATE, ntile8(nt - nn), TILE, lds_sp_update(h, 0), k_sp_update, SB, W, V, PL, h->dp, time_sph,
                        0.0, 0, adapt, nn, nt, nullptr);
+        const bool av = h->dp.do_av && nn > 0;
+        if (av) LAUNCH_LDS(K_AV_NODES, ntile8(nn), TILE, lds_av_nodes(h), k_av_nodes, SB, V, PL, h->dp, h->av_nodes.p, nn);
         if (nn > 0)
-            LAUNCH_LDS(K_DERIV_NODES, ntile8(nn), TILE, lds_deriv_nodes(h), k_deriv_nodes, SB, W, V, PL, h->dp, gxg, gyg,
-                       1.0, h->err.p, nn, SB.stress);
+            LAUNCH_LDS(K_DERIV_NODES, ntile8(nn), TILE, lds_deriv_nodes(h), k_deriv_nodes, SB, W, V, PL, h->dp,
+                       gxg, gyg, 1.0, h->err.p, nn, SB.stress, av ? h->av_nodes.p : nullptr);
         LAUNCH(K_STAGE_BEGIN, nblk(nn), BLOCK, k_lf_kick, SB, W, kdt, nn);
         if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, h->ifsigman, h->normal.p, nn);
         if (nn > 0)
@@ -1062,23 +1067,34 @@ static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
         // partners); the second BC call sets the node velocities the first
         // set.  So they run side by side, unless kernel timing (one stream)
         // is on.
+        // The artificial viscosity (k_av_nodes) reads only node data, after
+        // the node side and the second BC call, and goes with them.
         const bool fork = !h->prof && nn > 0 && nt > nn;
+        const bool av = h->dp.do_av && nn > 0;
         if (fork) {
             HIP_TRY(hipEventRecord(h->ev_fork, s));
             HIP_TRY(hipStreamWaitEvent(h->stream2, h->ev_fork, 0));
-            k_spu_nodes<<<ntile8(nn), TILE, LAY_SPU_NODES.bytes, h->stream2>>>(SB, V, PL, adapt, tanfi, coh, nn);
-            HIP_TRY(hipEventRecord(h->ev_join, h->stream2));
-        } else if (nn > 0) {
-            LAUNCH_LDS(K_SPU_NODES, ntile8(nn), TILE, LAY_SPU_NODES.bytes, k_spu_nodes, SB, V, PL, adapt, tanfi, coh, nn);
+            hipStream_t s2 = h->stream2;
+            k_spu_nodes<<<ntile8(nn), TILE, LAY_SPU_NODES.bytes, s2>>>(SB, V, PL, adapt, tanfi, coh, nn);
+            if (bcs)
+                k_bcs<<<nblk(nn), BLOCK, 0, s2>>>(SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, h->ifsigman,
+                                                 h->normal.p, nn);
+            if (av) k_av_nodes<<<ntile8(nn), TILE, lds_av_nodes(h), s2>>>(SB, V, PL, h->dp, h->av_nodes.p, nn);
+            HIP_TRY(hipEventRecord(h->ev_join, s2));
+        } else {
+            if (nn > 0)
+                LAUNCH_LDS(K_SPU_NODES, ntile8(nn), TILE, LAY_SPU_NODES.bytes, k_spu_nodes, SB, V, PL, adapt, tanfi, coh,
+                           nn);
+            if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, h->ifsigman, h->normal.p, nn);
+            if (av) LAUNCH_LDS(K_AV_NODES, ntile8(nn), TILE, lds_av_nodes(h), k_av_nodes, SB, V, PL, h->dp, h->av_nodes.p, nn);
         }
         if (nt > nn)
             LAUNCH_LDS(K_SP_UPDATE, ntile8(nt - nn), TILE, lds_sp_update(h, 1), k_sp_update, SB, W, V, PL, h->dp, time_sph,
                        f2rk[st], 1, adapt, nn, nt, h->sig2.p);
         if (fork) HIP_TRY(hipStreamWaitEvent(s, h->ev_join, 0));
-        if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, h->ifsigman, h->normal.p, nn);
         if (nn > 0)
-            LAUNCH_LDS(K_DERIV_NODES, ntile8(nn), TILE, lds_deriv_nodes(h), k_deriv_nodes, SB, W, V, PL, h->dp, gxg, gyg,
-                       f2rk[st], h->err.p, nn, nt > nn ? h->sig2.p : SB.stress);
+            LAUNCH_LDS(K_DERIV_NODES, ntile8(nn), TILE, lds_deriv_nodes(h), k_deriv_nodes, SB, W, V, PL, h->dp, gxg,
+                       gyg, f2rk[st], h->err.p, nn, nt > nn ? h->sig2.p : SB.stress, av ? h->av_nodes.p : nullptr);
         if (st == 3 && nt2 > nt)
             LAUNCH(K_DUMMY_STRESS, nblk(nt2 - nt), BLOCK, k_dummy_stress, SB, V, nt, nt2);
     }

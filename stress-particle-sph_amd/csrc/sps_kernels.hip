@@ -199,41 +199,76 @@ __device__ __forceinline__ void cell_xy(int key, const Grid &g, int &cx, int &cy
 }
 
 // ------------------------------------------------------------- cell binning
+// Runs of equal keys among a wave's lanes.  The particles are stored sorted
+// by kind and cell (the previous search's order), so consecutive lanes mostly
+// share a (kind, cell) bucket; one atomic per run instead of one per lane
+// keeps the bucket counters from serialising in L2.  Every lane of the wave
+// calls it (lanes without a particle pass a key of their own, -1).
+struct WaveRun {
+    int head;   // the run's first lane
+    int rank;   // this lane's place in its run
+    int len;    // lanes in the run (meaningful at the head)
+};
+
+__device__ __forceinline__ WaveRun wave_run(int key) {
+    const int lane = (int)(threadIdx.x & 63);
+    const int prev = __shfl_up(key, 1, 64);
+    const unsigned long long heads = __ballot(lane == 0 || prev != key);
+    const unsigned long long upto = (2ull << lane) - 1;   // lanes 0..lane (all for lane 63)
+    const int head = 63 - __clzll(heads & upto);
+    const unsigned long long after = heads & ~upto;
+    const int next = after ? __ffsll(after) - 1 : 64;
+    return WaveRun{head, lane - head, next - lane};
+}
+
 // Task 2 of grid_find_NEW (2_...:1277-1305): cell of every particle and
 // per-(kind, cell) counts.  Out-of-domain particles go to the sentinel cell
 // ndivt at the end of their kind's range.
 __global__ void k_cell_count(const double2 *__restrict__ x, const int *__restrict__ ood,
                              const Grid *__restrict__ g, int *__restrict__ cellkey,
                              int *__restrict__ kcount, int nn, int nt, int nt2) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nt2) return;
-    int ndivt = g->ndivt;
-    int c = ndivt;
-    if (ood[i] != 1 && !g->overflow) {
-        double2 p = x[i];
-        int ix = (int)((p.x - g->xmin[0]) / g->deltx + 1);
-        int iy = (int)((p.y - g->xmin[1]) / g->deltx + 1);
-        if (ix > g->ndivx) ix = g->ndivx;
-        if (iy > g->ndivy) iy = g->ndivy;
-        c = cell_key(ix - 1, iy - 1, *g);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = i < nt2;
+    const int ndivt = g->ndivt;
+    int b = -1;
+    if (live) {
+        int c = ndivt;
+        if (ood[i] != 1 && !g->overflow) {
+            double2 p = x[i];
+            int ix = (int)((p.x - g->xmin[0]) / g->deltx + 1);
+            int iy = (int)((p.y - g->xmin[1]) / g->deltx + 1);
+            if (ix > g->ndivx) ix = g->ndivx;
+            if (iy > g->ndivy) iy = g->ndivy;
+            c = cell_key(ix - 1, iy - 1, *g);
+        }
+        cellkey[i] = c;
+        b = kind_of(i, nn, nt) * (ndivt + 1) + c;
     }
-    cellkey[i] = c;
-    int k = kind_of(i, nn, nt);
-    atomicAdd(&kcount[k * (ndivt + 1) + c], 1);
+    const WaveRun w = wave_run(b);
+    if (live && w.rank == 0) atomicAdd(&kcount[b], w.len);
 }
 
-// Scatter into per-(kind, cell) buckets (arbitrary order inside a bucket).
+// Scatter into per-(kind, cell) buckets (arbitrary order inside a bucket):
+// the run's head claims the run's slots, the lanes take them in lane order.
 __global__ void k_cell_scatter(const int *__restrict__ cellkey, const int *__restrict__ orig,
                                const int *__restrict__ kstart, int *__restrict__ kfill,
                                const Grid *__restrict__ g, int *__restrict__ tmp_orig,
                                int *__restrict__ tmp_src, int *__restrict__ new_cell, int nn,
                                int nt, int nt2) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nt2) return;
-    int ndivt = g->ndivt;
-    int c = cellkey[i];
-    int b = kind_of(i, nn, nt) * (ndivt + 1) + c;
-    int slot = kstart[b] + atomicAdd(&kfill[b], 1);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = i < nt2;
+    const int ndivt = g->ndivt;
+    int b = -1, c = 0;
+    if (live) {
+        c = cellkey[i];
+        b = kind_of(i, nn, nt) * (ndivt + 1) + c;
+    }
+    const WaveRun w = wave_run(b);
+    int first = 0;
+    if (live && w.rank == 0) first = atomicAdd(&kfill[b], w.len);
+    first = __shfl(first, w.head, 64);
+    if (!live) return;
+    const int slot = kstart[b] + first + w.rank;
     tmp_orig[slot] = orig[i];
     tmp_src[slot] = i;
     new_cell[slot] = c;
@@ -780,11 +815,9 @@ constexpr LdsLayout layout_sp_update() {
     return L;
 }
 
-// node receivers, phase 1 <- stress points (m, rho, sigma / rho^2 of xx yy
-// xy; x only for the CSPM matrix) and dummies (m, rho); phase 2 <- nodes
-// (x, v, m, rho; h only when the smoothing lengths differ) over the same
-// bytes.  As above, the fields a launch may leave out come last in their
-// phase (lds_deriv_nodes on the host).
+// node receivers' stress gradient <- stress points (m, rho, sigma / rho^2 of
+// xx yy xy; x only for the CSPM matrix) and dummies (m, rho).  As above, the
+// field a launch may leave out comes last (lds_deriv_nodes on the host).
 constexpr LdsLayout layout_deriv_nodes() {
     LdsLayout L = layout_none();
     int o = 0;
@@ -796,14 +829,21 @@ constexpr LdsLayout layout_deriv_nodes() {
     L.m[KIND_DUMMY] = o; o += 8 * CAP_D;
     L.rho[KIND_DUMMY] = o; o += 8 * CAP_D;
     L.x[KIND_SP] = o; o += 16 * CAP;
-    const int phase1 = o;
-    o = 0;
+    L.bytes = o;
+    return L;
+}
+
+// node receivers' artificial viscosity <- nodes (x, v, m, rho; h last, only
+// when the smoothing lengths differ: lds_av_nodes on the host)
+constexpr LdsLayout layout_av_nodes() {
+    LdsLayout L = layout_none();
+    int o = 0;
     L.x[KIND_NODE] = o; o += 16 * CAP;
     L.v[KIND_NODE] = o; o += 16 * CAP;
     L.m[KIND_NODE] = o; o += 8 * CAP;
     L.rho[KIND_NODE] = o; o += 8 * CAP;
     L.h[KIND_NODE] = o; o += 8 * CAP;
-    L.bytes = o > phase1 ? o : phase1;
+    L.bytes = o;
     return L;
 }
 
@@ -811,6 +851,7 @@ constexpr LdsLayout LAY_COUNT = layout_count();
 constexpr LdsLayout LAY_SPU_NODES = layout_spu_nodes();
 constexpr LdsLayout LAY_SP_UPDATE = layout_sp_update();
 constexpr LdsLayout LAY_DERIV_NODES = layout_deriv_nodes();
+constexpr LdsLayout LAY_AV_NODES = layout_av_nodes();
 
 __device__ __forceinline__ constexpr int sel3(int kind, int a, int b, int c) {
     return kind == 0 ? a : (kind == 1 ? b : c);
@@ -1881,12 +1922,11 @@ __device__ __forceinline__ double2 wall_force(const StateBufs &S, const SearchVi
 }
 
 // get_derivatives for node receivers, fused with gravity_force
-// (3_...:2640-2702), boundary_forces (2_...:1117-1161), artificial_viscosity
-// (2_...:826-904) and the momentum right-hand side and RK accumulation
+// (3_...:2640-2702), boundary_forces (2_...:1117-1161), the artificial
+// viscosity's share (2_...:826-904, summed over node partners, list B, by
+// k_av_nodes) and the momentum right-hand side and RK accumulation
 // (2_...:763-764, 769).  The stress gradient sums over stress-point and dummy
-// partners (list A), the viscosity over node partners (list B); each sum
-// keeps list order, so the window is staged in two phases (stress points
-// and dummies, then nodes) through the same LDS.
+// partners (list A) in list order, from a window of both kinds in LDS.
 // The end of get_derivatives for node r (CSPM correction, div2; 2_...:617-642)
 // from its stress-divergence sums and CSPM moments, then gravity_force,
 // boundary_forces (when a dummy is among its pairs), the artificial
@@ -1940,7 +1980,8 @@ __device__ __forceinline__ void node_stage_tail(const StateBufs &S, const Work &
 __global__ __launch_bounds__(TILE) STAGE_OCCUPANCY void k_deriv_nodes(StateBufs S, Work W, SearchView V, PairLists PL,
                                                       DevParams P, double gx_grav, double gy_grav, double f2,
                                                       int *__restrict__ err, int nn,
-                                                      const double4 *__restrict__ sp_stress) {
+                                                      const double4 *__restrict__ sp_stress,
+                                                      const float2 *__restrict__ av_in) {
     __shared__ StagePlan pl[2];
     constexpr const LdsLayout &LY = LAY_DERIV_NODES;
     const WalkCtx C = walk_ctx(V);
@@ -1959,7 +2000,7 @@ __global__ __launch_bounds__(TILE) STAGE_OCCUPANCY void k_deriv_nodes(StateBufs 
     int cx = 0, cy = 0;
     if (inside) cell_xy(key_r, C.g, cx, cy);
     const int ra = active ? r : T.first;   // a valid slot for the idle lanes' loads
-    const double2 xr = S.x[ra], vr = S.vel[ra];
+    const double2 xr = S.x[ra];
     const double4 sr = S.stress[ra];
     const double rhor = S.rho[ra], hr = S.hsml[ra];
     const double rr2 = rhor * rhor;
@@ -2046,15 +2087,55 @@ __global__ __launch_bounds__(TILE) STAGE_OCCUPANCY void k_deriv_nodes(StateBufs 
         }
     }
 
-    // phase 2: artificial viscosity over node partners (type 3), single precision
+    // the artificial viscosity over node partners (type 3), summed by k_av_nodes
+    float ax = 0.0f, ay = 0.0f;
+    if (!active) return;
+    if (av_in) {
+        const float2 a = av_in[r];
+        ax = a.x;
+        ay = a.y;
+    }
+    const double2 vr = S.vel[r];
+    node_stage_tail(S, W, V, C, P, r, cx, cy, xr, vr, hr, q00, q01, q10, q11, q20, q21, A0, A1, A2, A3, ndum,
+                    ax, ay, gx_grav, gy_grav, f2, err);
+}
+
+// artificial_viscosity (2_...:826-904) for the node receivers: the list-order
+// sums over node partners (list B) in single precision, as the reference
+// accumulates them, written to av_out for k_deriv_nodes.  It reads only node
+// data that the stress points' update leaves alone, so the RK stages run it
+// beside k_sp_update.
+__global__ __launch_bounds__(TILE) void k_av_nodes(StateBufs S, SearchView V, PairLists PL, DevParams P,
+                                                   float2 *__restrict__ av_out, int nn) {
+    __shared__ StagePlan pl[2];
+    constexpr const LdsLayout &LY = LAY_AV_NODES;
+    const WalkCtx C = walk_ctx(V);
+    const int tile = tile_index();
+    const Tile T = PL.tiles[tile];
+    if (T.first >= T.end) return;
+    const int r = T.first + (int)threadIdx.x;
+    const bool active = r < T.end;
+    const int key_r = active ? V.cell[r] : C.g.ndivt;
+    const bool inside = key_r < C.g.ndivt;
+    const bool rev_l = !C.any_fwd && C.any_rev;
+    bool check;
+    const bool tiled = tile_uses_lds(T, V, C, r, check) && PL.ok[tile];
+    int cx = 0, cy = 0;
+    if (inside) cell_xy(key_r, C.g, cx, cy);
+    const int ra = active ? r : T.first;   // a valid slot for the idle lanes' loads
+    const double2 xr = S.x[ra], vr = S.vel[ra];
+    const double rhor = S.rho[ra], hr = S.hsml[ra];
+    const SplitClass cls{check, check ? V.proc[r] : 0, V.proc, V.cnt, V.off, C.sp};
+    const bool rev_g = C.any_fwd ? check : C.any_rev;
+    const LdsFields<LY> LF{};
     float ax = 0.0f, ay = 0.0f, visc = 0.0f;
-    if (P.do_av) {
+    {
         ListWalk<MASK_NODE, false, true> lb;
         lb.start(PL.ib, nullptr, PL.gxb, PL.gyb, tile, inside ? PL.deg_b[r] : 0, rev_l);
         bool lds2 = tiled;
         if (lds2) {
             load_plan<KIND_NODE>(PL, tile, 0, pl[0]);
-            __syncthreads();   // also: the phase-1 walks are done with the shared bytes
+            __syncthreads();
             lds2 = pl[0].total <= CAP;
         }
         if (lds2) {
@@ -2062,7 +2143,7 @@ __global__ __launch_bounds__(TILE) STAGE_OCCUPANCY void k_deriv_nodes(StateBufs 
                 double2 x, v;
                 double h, m, rho;
             };
-            const bool stage_h = !P.h_uni;   // lds_deriv_nodes allocates h only then
+            const bool stage_h = !P.h_uni;   // lds_av_nodes allocates h only then
             copy_kind<KIND_NODE>(T, V, C, nullptr, pl[0],
                                  [&](int g) { return NItem{S.x[g], S.vel[g], stage_h ? S.hsml[g] : 0.0, S.mass[g],
                                                            S.rho[g]}; },
@@ -2104,9 +2185,7 @@ __global__ __launch_bounds__(TILE) STAGE_OCCUPANCY void k_deriv_nodes(StateBufs 
                                         KIND_NODE, r, rev_g, C.any_fwd, V.pi, cls, av);
         }
     }
-    if (!active) return;
-    node_stage_tail(S, W, V, C, P, r, cx, cy, xr, vr, hr, q00, q01, q10, q11, q20, q21, A0, A1, A2, A3, ndum,
-                    ax, ay, gx_grav, gy_grav, f2, err);
+    if (active) av_out[r] = make_float2(ax, ay);
 }
 
 // XSPH_update (2_...:189-239) for the receivers of kind KR: the list-order

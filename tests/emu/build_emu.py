@@ -31,7 +31,7 @@ BUILD = os.path.join(HERE, "_build")
 # kernels that use barriers or wave exchanges (their threads run as fibers)
 SYNC = ("k_bounds_partial", "k_bounds_final", "k_count", "k_spu_nodes", "k_sp_update", "k_deriv_nodes", "k_permute",
         "k_scan_local", "k_scan_sums", "k_max_disp", "k_gemm_bf16", "k_mlp_loss", "k_mlp_colsum",
-        "k_surrogate_apply", "k_xsph")
+        "k_surrogate_apply", "k_xsph", "k_av_nodes", "k_cell_count", "k_cell_scatter")
 
 
 def split_top(s):

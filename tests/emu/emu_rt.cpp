@@ -21,7 +21,8 @@ bool needs_sync(const char *name) {
     static const char *const sync[] = {"k_bounds_partial", "k_bounds_final", "k_count", "k_spu_nodes",
                                        "k_sp_update",      "k_deriv_nodes",  "k_scan_local", "k_scan_sums",
                                        "k_max_disp",       "k_gemm_bf16",    "k_mlp_loss",   "k_mlp_colsum",
-                                       "k_surrogate_apply", "k_xsph",   "k_permute"};
+                                       "k_surrogate_apply", "k_xsph",   "k_permute", "k_av_nodes",
+                                       "k_cell_count",     "k_cell_scatter"};
     for (const char *s : sync)
         if (strncmp(name, s, strlen(s)) == 0) return true;   // k_count also covers k_count_lists
     return false;

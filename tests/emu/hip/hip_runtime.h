@@ -96,6 +96,17 @@ template <typename T> T __shfl_down(T v, unsigned d, int w = 64) {
     emu::wave_put(nullptr, 0);
     return r;
 }
+// lane src of the width-w segment holding this lane
+template <typename T> T __shfl(T v, int src, int w = 64) {
+    emu::wave_put(&v, sizeof(T));
+    int lane = threadIdx.x & 63, seg = lane & ~(w - 1);
+    T r = v;
+    emu::wave_get(seg + (src & (w - 1)), &r, sizeof(T));
+    emu::wave_put(nullptr, 0);
+    return r;
+}
+inline int __clzll(long long v) { return v ? __builtin_clzll((unsigned long long)v) : 64; }
+inline int __ffsll(unsigned long long v) { return __builtin_ffsll((long long)v); }
 template <typename T> T atomicAdd(T *p, T v) { T o = *p; *p = o + v; return o; }
 // hardware reciprocal and square root (within 1 ulp on the device)
 inline float __builtin_amdgcn_rcpf(float x) { return 1.0f / x; }
