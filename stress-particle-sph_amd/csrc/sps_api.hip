@@ -317,6 +317,12 @@ static void drop_graphs(sps_handle *h);
 #include "sps_surrogate_host.hip"
 extern "C" const char *sps_backend(void) { return "hip-gfx950"; }
 
+// Dynamic LDS of the list build: h of the window only when the smoothing
+// lengths differ (its layout puts h last).
+static size_t lds_count(const sps_handle *h) {
+    return h->h_uniform > 0 ? (size_t)LAY_COUNT.h[KIND_NODE] : (size_t)LAY_COUNT.bytes;
+}
+
 // ------------------------------------------------------------------ scans
 // Exclusive scan of n unsigned values; optional device total.  With a
 // device count (sc.g set) n is only the launch bound: the kernels scan the
@@ -712,25 +718,25 @@ static int search(sps_handle *h, bool fresh_list) {
             HIP_TRY(hipEventRecord(h->ev_fork, h->stream));
             HIP_TRY(hipStreamWaitEvent(h->stream2, h->ev_fork, 0));
             if (!h->mixed)
-                k_count_lists<KIND_SP, true><<<ntile8(nt - nn), TILE, LAY_COUNT.bytes, h->stream2>>>(
+                k_count_lists<KIND_SP, true><<<ntile8(nt - nn), TILE, lds_count(h), h->stream2>>>(
                     V, C.x.p, C.hsml.p, h->cnt.p, h->split.p, PL, h->h_uniform, nn, nt - nn, h->skin);
             else
-                k_count_lists<KIND_SP, false><<<ntile8(nt - nn), TILE, LAY_COUNT.bytes, h->stream2>>>(
+                k_count_lists<KIND_SP, false><<<ntile8(nt - nn), TILE, lds_count(h), h->stream2>>>(
                     V, C.x.p, C.hsml.p, h->cnt.p, h->split.p, PL, h->h_uniform, nn, nt - nn, h->skin);
             HIP_TRY(hipEventRecord(h->ev_join, h->stream2));
         }
         // the mixed-precision stage kernels compute the kernel values themselves
         if (nn > 0 && !h->mixed)
-            LAUNCH_LDS(K_COUNT_PAIRS, ntile8(nn), TILE, LAY_COUNT.bytes, (k_count_lists<KIND_NODE, true>), V, C.x.p,
+            LAUNCH_LDS(K_COUNT_PAIRS, ntile8(nn), TILE, lds_count(h), (k_count_lists<KIND_NODE, true>), V, C.x.p,
                        C.hsml.p, h->cnt.p, h->split.p, PL, h->h_uniform, 0, nn, h->skin);
         if (nn > 0 && h->mixed)
-            LAUNCH_LDS(K_COUNT_PAIRS, ntile8(nn), TILE, LAY_COUNT.bytes, (k_count_lists<KIND_NODE, false>), V, C.x.p,
+            LAUNCH_LDS(K_COUNT_PAIRS, ntile8(nn), TILE, lds_count(h), (k_count_lists<KIND_NODE, false>), V, C.x.p,
                        C.hsml.p, h->cnt.p, h->split.p, PL, h->h_uniform, 0, nn, h->skin);
         if (!fork && nt > nn && !h->mixed)
-            LAUNCH_LDS(K_COUNT_PAIRS, ntile8(nt - nn), TILE, LAY_COUNT.bytes, (k_count_lists<KIND_SP, true>), V, C.x.p,
+            LAUNCH_LDS(K_COUNT_PAIRS, ntile8(nt - nn), TILE, lds_count(h), (k_count_lists<KIND_SP, true>), V, C.x.p,
                        C.hsml.p, h->cnt.p, h->split.p, PL, h->h_uniform, nn, nt - nn, h->skin);
         if (!fork && nt > nn && h->mixed)
-            LAUNCH_LDS(K_COUNT_PAIRS, ntile8(nt - nn), TILE, LAY_COUNT.bytes, (k_count_lists<KIND_SP, false>), V, C.x.p,
+            LAUNCH_LDS(K_COUNT_PAIRS, ntile8(nt - nn), TILE, lds_count(h), (k_count_lists<KIND_SP, false>), V, C.x.p,
                        C.hsml.p, h->cnt.p, h->split.p, PL, h->h_uniform, nn, nt - nn, h->skin);
         if (nt2 > nt)
             LAUNCH(K_COUNT_PAIRS, nblk8(nt2 - nt), BLOCK, k_count, C.x.p, C.hsml.p, h->cell.p, h->kstart.p, h->proc.p,

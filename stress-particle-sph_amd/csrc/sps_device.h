@@ -93,27 +93,29 @@ struct KernelFactor {
     }
 };
 
-// cubic_kernel with the h-only parts supplied (K.get(hs, pi)).
+// cubic_kernel with the h-only parts supplied (K.get(hs, pi)).  The two
+// regions' divisions of the gradient factor (by h^2 inside q <= 1, by h
+// beyond) are one division of per-lane operands, so that a wave whose lanes
+// fall in both regions divides once; every value is the one the reference's
+// branch computes.
 __device__ __forceinline__ void cubic_kernel_f(double r, double dx0, double dx1, double hs, const KernelFactor &K,
                                                float &w, float &gx, float &gy) {
-    double q = r / hs;
+    const double q = r / hs;
     const double factor = K.f;
-    double wd, g0, g1;
-    if (q >= 0.0 && q <= 1.0) {
+    const bool in1 = q >= 0.0 && q <= 1.0;
+    const bool in2 = q > 1.0 && q <= 2.0;
+    const double t = 2.0 - q;
+    const double num = in1 ? factor * (-2.0 + (double)(3.0f / 2.0f) * q) : K.f63 * (t * t);
+    const double c = num / (in1 ? K.h2 : hs);
+    double wd = 0.0, g0 = 0.0, g1 = 0.0;
+    if (in1) {
         wd = factor * ((double)(2.0f / 3.0f) - q * q + (q * (q * q)) / 2.0);
-        double c = factor * (-2.0 + (double)(3.0f / 2.0f) * q) / K.h2;
         g0 = c * dx0;
         g1 = c * dx1;
-    } else if (q > 1.0 && q <= 2.0) {
-        double t = 2.0 - q;
+    } else if (in2) {
         wd = K.f6 * (t * (t * t));
-        double c = K.f63 * (t * t) / hs;
         g0 = -(c * (dx0 / r));
         g1 = -(c * (dx1 / r));
-    } else {
-        wd = 0.0;
-        g0 = 0.0;
-        g1 = 0.0;
     }
     w = (float)wd;
     gx = (float)g0;

@@ -759,17 +759,25 @@ constexpr LdsLayout layout_none() {
 
 // count / list build: positions and smoothing lengths of all three kinds and
 // the window-cell tables
+// The list build: x of every kind, the cell tables, then h of every kind,
+// which a launch allocates only when the smoothing lengths differ (with one
+// smoothing length the window is 26 KiB at 256 receivers, six workgroups per
+// CU instead of four).
 constexpr LdsLayout layout_count() {
     LdsLayout L = layout_none();
     int o = 0;
     for (int k = 0; k < 3; ++k) {
-        const int cap = k == KIND_DUMMY ? CAP_D : CAP;
-        L.x[k] = o; o += 16 * cap;
-        L.h[k] = o; o += 8 * cap;
+        L.x[k] = o;
+        o += 16 * (k == KIND_DUMMY ? CAP_D : CAP);
     }
     for (int k = 0; k < 3; ++k) {
         L.tab[k] = o;
         o += 4 * TCELLS;
+    }
+    o = (o + 15) & ~15;
+    for (int k = 0; k < 3; ++k) {
+        L.h[k] = o;
+        o += 8 * (k == KIND_DUMMY ? CAP_D : CAP);
     }
     L.bytes = o;
     return L;
@@ -1278,7 +1286,7 @@ struct IdxWriter {
 template <typename Fields>
 __device__ __forceinline__ void fill_kernel_values(const unsigned long long *ip, float4 *wp, float4 *gxp,
                                                    float4 *gyp, int tile, int n, const Fields &A, double2 xr,
-                                                   double hr, double pi, KernelFactor &kf) {
+                                                   double hr, double h_uniform, double pi, KernelFactor &kf) {
     const int ng = (n + 3) >> 2;
     // the lane's own words, written by its walk: the next one is loaded
     // while the current one's kernel values are computed
@@ -1296,7 +1304,7 @@ __device__ __forceinline__ void fill_kernel_values(const unsigned long long *ip,
                 const double2 xs = A.xk(kind, j);
                 const double dx0 = xr.x - xs.x, dx1 = xr.y - xs.y;
                 double mh;
-                const double rr = pair_distance(dx0, dx1, hr, A.hk(kind, j), mh);
+                const double rr = pair_distance(dx0, dx1, hr, h_uniform > 0 ? h_uniform : A.hk(kind, j), mh);
                 cubic_kernel_f(rr, dx0, dx1, mh, kf.get(mh, pi), wv[k], gxv[k], gyv[k]);
             }
         }
@@ -1359,12 +1367,13 @@ __global__ __launch_bounds__(TILE) void k_count_lists(SearchView V, const double
         self_base = pl[KR].mainlo;
     }
     if (lds) {
-        auto fetch = [&](int g) { return make_double3(x[g].x, x[g].y, hsml[g]); };
+        const bool stage_h = !(h_uniform > 0);   // the launch allocates h only then
+        auto fetch = [&](int g) { return make_double3(x[g].x, x[g].y, stage_h ? hsml[g] : 0.0); };
         auto put = [&](auto kt) {
             constexpr int K = decltype(kt)::value;
             return [&](int i, const double3 &v) {
                 lds_at<double2>(LAY_COUNT.x[K])[i] = make_double2(v.x, v.y);
-                lds_at<double>(LAY_COUNT.h[K])[i] = v.z;
+                if (stage_h) lds_at<double>(LAY_COUNT.h[K])[i] = v.z;
             };
         };
         copy_kind<KIND_NODE>(T, V, C, lds_at<int>(LY.tab[KIND_NODE]), pl[KIND_NODE], fetch, put(KindTag<KIND_NODE>{}));
@@ -1402,9 +1411,18 @@ __global__ __launch_bounds__(TILE) void k_count_lists(SearchView V, const double
                         // index inside a cell and kind)
                         const bool kind_after = cell_after || (same_cell && ks > KR);
                         const bool slot_after = same_cell && ks == KR;
-                        for (int j = rg.x; j < rg.x + rg.y; ++j) {
+                        // a stress point's stress-point partners enter no list (no XSPH): only
+                        // those it discovers count, so the cells before its own, and in its
+                        // own cell the slots up to its own, are not tested
+                        const bool count_only = KR == KIND_SP && ks == KIND_SP && !list_b;
+                        if (count_only && !cell_after && !same_cell) continue;
+                        const int jbeg = count_only && same_cell && self + 1 > rg.x ? self + 1 : rg.x;
+                        const int jend = rg.x + rg.y;
+                        double2 xn = jbeg < jend ? A.xk(ks, jbeg) : make_double2(0.0, 0.0);
+                        for (int j = jbeg; j < jend; ++j) {
+                            const double2 xs = xn;
+                            if (j + 1 < jend) xn = A.xk(ks, j + 1);   // the next position in flight
                             if (ks == KR && j == self) continue;
-                            const double2 xs = A.xk(ks, j);
                             if (!FILL && skin > 0) {
                                 // lists kept over several steps: also the partners within the
                                 // skin; only the pairs themselves are counted
@@ -1473,10 +1491,10 @@ __global__ __launch_bounds__(TILE) void k_count_lists(SearchView V, const double
         const double2 xr = x[r];
         const double hr = hsml[r];
         KernelFactor kf;
-        fill_kernel_values(PL.ia, PL.wa, PL.gxa, PL.gya, ltile, la.n, LdsFields<LY>{}, xr, hr, V.pi, kf);
+        fill_kernel_values(PL.ia, PL.wa, PL.gxa, PL.gya, ltile, la.n, LdsFields<LY>{}, xr, hr, h_uniform, V.pi, kf);
         if (list_b)
             fill_kernel_values(PL.ib, PL.wb, KR == KIND_NODE ? PL.gxb : nullptr, KR == KIND_NODE ? PL.gyb : nullptr,
-                               ltile, lb.n, LdsFields<LY>{}, xr, hr, V.pi, kf);
+                               ltile, lb.n, LdsFields<LY>{}, xr, hr, h_uniform, V.pi, kf);
     }
     if (r < T.end) {
         PL.deg_a[r] = (unsigned short)(la.n < LMAX ? la.n : LMAX);

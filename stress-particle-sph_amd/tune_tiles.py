@@ -43,11 +43,14 @@ BUILD = os.path.join(ROOT, "_build", "tiles")
 # partners of a kind at TILE 256 on the 1 M slope, about 260 at 128.  LDS per
 # workgroup on that deck (no CSPM, one smoothing length): the list build
 # 37.3 KiB at t256, 33.6 at t256s, 22.9 at t128, 19.1 at t128s; the stage
-# kernels at most 35 KiB at t256.  The stage kernels k_sp_update and
+# kernels at most 35 KiB at t256.  With one smoothing length the list build
+# leaves h out of its window: 27.9 KiB with its plans at t256, five
+# workgroups per CU; 26.9 KiB at t256m (608), six.  The stage kernels k_sp_update and
 # k_deriv_nodes are held to 3 waves per SIMD by their registers unless
 # compiled for 4 (STAGE_WAVES), which spills a few of them.
 CANDIDATES = {
     "t256": (256, 24, 640, 192, 0),
+    "t256m": (256, 24, 608, 192, 0),
     "t256w4": (256, 24, 640, 192, 4),
     "t256s": (256, 24, 576, 160, 0),
     "t256sw4": (256, 24, 576, 160, 4),
