@@ -95,8 +95,8 @@ def kernel_algorithmic_bytes(kernel, counts):
         return STAGE_NODE_B * nn, f"{STAGE_NODE_B} B per velocity particle per stage x {nn}"
     if kernel == "k_sp_update":        # the stress-particle half of a stage
         return STAGE_SP_B * ns, f"{STAGE_SP_B} B per stress particle per stage x {ns}"
-    if kernel == "k_count":            # the search (three launches: nodes, stress points, dummies)
-        return SEARCH_B * (nn + ns) / 3, f"{SEARCH_B} B per particle per step / 3 launches"
+    if kernel == "k_count_lists":      # the search (two launches: node and stress-point receivers)
+        return SEARCH_B * (nn + ns) / 2, f"{SEARCH_B} B per particle per step / 2 launches"
     return None, None
 
 
@@ -120,7 +120,7 @@ def implementation_bytes(kernel, st, counts, params):
         dummy = 8 + 8 + 4                                                # m rho orig
         return lists + nn * node + ns * sp + nd * dummy
     if kernel == "k_av_nodes":
-        return (10 * P3 + nn * (16 + 16 + 8 + 8 + 8 + 8)) if av else 0   # list B; x v m rho h, av out
+        return (10 * 2 * P3 + nn * (16 + 16 + 8 + 8 + 8 + 8)) if av else 0   # list B; x v m rho h, av out
     if kernel == "k_sp_update":
         # four launches per step with the derivative, one (after the last stage) without
         deriv = 14 * (P1 + P9) + ns * (16 + 16 + 32 + 32 + 8 + 32 + 32 + 64 + 16) \
@@ -129,11 +129,12 @@ def implementation_bytes(kernel, st, counts, params):
         return (4 * deriv + update) / 5
     if kernel == "k_spu_nodes":
         return 6 * (P1 + P6) + nn * (32 + 32 + 8 + 16) + ns * (32 + 8 + 8 + 8)
-    if kernel == "k_count":
-        # three launches per step (node, stress-point and dummy receivers)
+    if kernel == "k_count_lists":
+        # two launches per step (node and stress-point receivers); each
+        # stages x of the window (h too when the smoothing lengths differ)
         entries_a = (P1 + P6) + (P1 + P9)
         lists = 14 * entries_a + 10 * 2 * P3 + 2 * (nn + ns) + 2 * nn
-        return ((nn + ns + nd) * (16 + 8 + 4 + 4 + 4) + lists) / 3
+        return ((nn + ns + nd) * (16 + 4 + 4 + 4) + lists) / 2
     return None
 
 

@@ -252,14 +252,14 @@ enum KernelId {
     K_CHECK_OOD, K_BOUNDS, K_CELL_COUNT, K_CELL_SCATTER, K_CELL_SORT, K_PERMUTE, K_PROC,
     K_COUNT_PAIRS, K_SCAN, K_SPLIT, K_RK_INIT, K_STAGE_BEGIN, K_SPU_NODES,
     K_SP_UPDATE, K_DERIV_NODES, K_DUMMY_STRESS, K_RK_FINAL, K_POSITION, K_SURROGATE, K_XSPH, K_SHIFT, K_BCS, K_SURF,
-    K_AV_NODES, K_NKERNELS
+    K_AV_NODES, K_COUNT_DUMMY, K_NKERNELS
 };
 static const char *const kKernelNames[K_NKERNELS] = {
     "k_check_ood", "k_bounds", "k_cell_count", "k_cell_scatter", "k_cell_sort", "k_permute",
-    "k_proc_slot", "k_count", "k_scan", "k_split", "k_stage_first",
+    "k_proc_slot", "k_count_lists", "k_scan", "k_split", "k_stage_first",
     "k_stage_begin", "k_spu_nodes", "k_sp_update", "k_deriv_nodes",
     "k_dummy_stress", "k_rk_final", "k_position", "k_surrogate", "k_xsph", "k_shift_sp", "k_bcs", "k_surf",
-    "k_av_nodes"};
+    "k_av_nodes", "k_count"};
 
 static hipEvent_t prof_begin(sps_handle *h) {
     if (!h->prof) return nullptr;
@@ -739,7 +739,7 @@ static int search(sps_handle *h, bool fresh_list) {
             LAUNCH_LDS(K_COUNT_PAIRS, ntile8(nt - nn), TILE, lds_count(h), (k_count_lists<KIND_SP, false>), V, C.x.p,
                        C.hsml.p, h->cnt.p, h->split.p, PL, h->h_uniform, nn, nt - nn, h->skin);
         if (nt2 > nt)
-            LAUNCH(K_COUNT_PAIRS, nblk8(nt2 - nt), BLOCK, k_count, C.x.p, C.hsml.p, h->cell.p, h->kstart.p, h->proc.p,
+            LAUNCH(K_COUNT_DUMMY, nblk8(nt2 - nt), BLOCK, k_count, C.x.p, C.hsml.p, h->cell.p, h->kstart.p, h->proc.p,
                    h->grid.p, h->cnt.p, h->split.p, nn, nt, nt, nt2);
         if (fork) HIP_TRY(hipStreamWaitEvent(h->stream, h->ev_join, 0));
     }
