@@ -819,23 +819,38 @@ static int mixed_step_kernels(sps_handle *h, double dt, double time_sph, double 
                        tanfi, coh, hu, nn);
         if (nt > nn)
             LAUNCH_LDS(K_SP_UPDATE, ntile8(nt - nn), TILE, LAY_SP_UPDATE_MIX.bytes, k_sp_update_mix, SB, W, V, PL,
-                       h->dp, time_sph, 0.0, 0, adapt, hu, nn, nt, stepv);
+                       h->dp, time_sph, 0.0, 0, adapt, hu, nn, nt, stepv, nullptr);
     }
     LAUNCH(K_RK_INIT, nblk(nt2), BLOCK, k_stage_first, SB, W, adapt, tanfi, coh, nn, nt, nt2);
     for (int st = 0; st < 4; ++st) {
         const double c1 = f1rk[st] * (dt);
         if (st > 0) LAUNCH(K_STAGE_BEGIN, nblk(nt), BLOCK, k_stage_begin, SB, W, c1, adapt, tanfi, coh, nn, nt);
         if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, h->bc_vars, h->bc_vals.p, h->ifsigman, h->normal.p, nn);
-        if (nn > 0)
-            LAUNCH_LDS(K_SPU_NODES, ntile8(nn), TILE, LAY_SPU_NODES_MIX.bytes, k_spu_nodes_mix, SB, V, PL, adapt,
-                       tanfi, coh, hu, nn);
-        if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, h->bc_vars, h->bc_vals.p, h->ifsigman, h->normal.p, nn);
+        // the two halves of stress_point_update side by side, as in the FP64
+        // step; one stream when kernels are timed or the step is captured
+        const bool fork = !h->prof && !stepv && nn > 0 && nt > nn;
+        if (fork) {
+            HIP_TRY(hipEventRecord(h->ev_fork, h->stream));
+            HIP_TRY(hipStreamWaitEvent(h->stream2, h->ev_fork, 0));
+            hipStream_t s2 = h->stream2;
+            k_spu_nodes_mix<<<ntile8(nn), TILE, LAY_SPU_NODES_MIX.bytes, s2>>>(SB, V, PL, adapt, tanfi, coh, hu, nn);
+            if (bcs)
+                k_bcs<<<nblk(nn), BLOCK, 0, s2>>>(SB, h->bc_info.p, h->bc_or_not.p, h->bc_vars, h->bc_vals.p,
+                                                 h->ifsigman, h->normal.p, nn);
+            HIP_TRY(hipEventRecord(h->ev_join, s2));
+        } else {
+            if (nn > 0)
+                LAUNCH_LDS(K_SPU_NODES, ntile8(nn), TILE, LAY_SPU_NODES_MIX.bytes, k_spu_nodes_mix, SB, V, PL, adapt,
+                           tanfi, coh, hu, nn);
+            if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, h->bc_vars, h->bc_vals.p, h->ifsigman, h->normal.p, nn);
+        }
         if (nt > nn)
             LAUNCH_LDS(K_SP_UPDATE, ntile8(nt - nn), TILE, LAY_SP_UPDATE_MIX.bytes, k_sp_update_mix, SB, W, V, PL,
-                       h->dp, time_sph, f2rk[st], 1, adapt, hu, nn, nt, stepv);
+                       h->dp, time_sph, f2rk[st], 1, adapt, hu, nn, nt, stepv, h->sig2.p);
+        if (fork) HIP_TRY(hipStreamWaitEvent(h->stream, h->ev_join, 0));
         if (nn > 0)
             LAUNCH_LDS(K_DERIV_NODES, ntile8(nn), TILE, LAY_DERIV_NODES_MIX.bytes, k_deriv_nodes_mix, SB, W, V, PL,
-                       h->dp, gxg, gyg, f2rk[st], h->err.p, hu, nn, stepv);
+                       h->dp, gxg, gyg, f2rk[st], h->err.p, hu, nn, stepv, nt > nn ? h->sig2.p : SB.stress);
         if (st == 3 && nt2 > nt)
             LAUNCH(K_DUMMY_STRESS, nblk(nt2 - nt), BLOCK, k_dummy_stress, SB, V, nt, nt2);
     }
@@ -853,7 +868,7 @@ static int mixed_step_kernels(sps_handle *h, double dt, double time_sph, double 
                    coh, hu, nn);
     if (nt > nn)
         LAUNCH_LDS(K_SP_UPDATE, ntile8(nt - nn), TILE, LAY_SP_UPDATE_MIX.bytes, k_sp_update_mix, SB, W, V, PL, h->dp,
-                   time_sph, 0.0, 0, adapt, hu, nn, nt, stepv);
+                   time_sph, 0.0, 0, adapt, hu, nn, nt, stepv, nullptr);
     if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, h->bc_vars, h->bc_vals.p, h->ifsigman, h->normal.p, nn);
     const bool xsph = h->xsph && h->p.update_x;
     if (xsph && nn > 0)

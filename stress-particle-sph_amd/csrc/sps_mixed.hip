@@ -246,10 +246,13 @@ __global__ __launch_bounds__(TILE) void k_spu_nodes_mix(StateBufs S, SearchView 
 
 // Stress-point receivers (as k_sp_update).
 // stepv (a captured step): time_sph from stepv[0] instead of the argument.
+// sig_out: where the stress after the return map goes (null: the state's
+// stress), as in k_sp_update.
 __global__ __launch_bounds__(TILE) void k_sp_update_mix(StateBufs S, Work W, SearchView V, PairLists PL,
                                                         DevParams P, double time_sph, double f2, int deriv,
                                                         int adapt, float hu, int nn, int nt,
-                                                        const double *__restrict__ stepv) {
+                                                        const double *__restrict__ stepv,
+                                                        double4 *__restrict__ sig_out) {
     if (stepv) time_sph = stepv[0];
     __shared__ StagePlan pl[2];
     constexpr const LdsLayout &LY = LAY_SP_UPDATE_MIX;
@@ -336,10 +339,11 @@ __global__ __launch_bounds__(TILE) void k_sp_update_mix(StateBufs S, Work W, Sea
     }
     double4 st = S.stress[r];
     const DevParams Pm = material_params(P, S, r);
-    if (adapt) {
-        adapt_stress(st, Pm.tanfi, Pm.coh);
+    if (adapt) adapt_stress(st, Pm.tanfi, Pm.coh);
+    if (sig_out)
+        sig_out[r] = st;
+    else if (adapt)
         S.stress[r] = st;
-    }
     if (!deriv) return;
 
     // get_derivatives
@@ -402,10 +406,13 @@ __global__ __launch_bounds__(TILE) void k_sp_update_mix(StateBufs S, Work W, Sea
 
 // Node receivers (as k_deriv_nodes).
 // stepv (a captured step): the gravity terms from stepv[1:3].
+// sp_stress: the stress points' stresses after the stage's return map
+// (k_sp_update_mix's sig_out, or the state's stress).
 __global__ __launch_bounds__(TILE) void k_deriv_nodes_mix(StateBufs S, Work W, SearchView V, PairLists PL,
                                                           DevParams P, double gx_grav, double gy_grav, double f2,
                                                           int *__restrict__ err, float hu, int nn,
-                                                          const double *__restrict__ stepv) {
+                                                          const double *__restrict__ stepv,
+                                                          const double4 *__restrict__ sp_stress) {
     if (stepv) {
         gx_grav = stepv[1];
         gy_grav = stepv[2];
@@ -452,7 +459,7 @@ __global__ __launch_bounds__(TILE) void k_deriv_nodes_mix(StateBufs S, Work W, S
             double4 st;
         };
         copy_kind<KIND_SP>(T, V, C, nullptr, pl[0],
-                           [&](int g) { return SItem{S.x[g], S.hsml[g], S.mass[g], S.rho[g], S.stress[g]}; },
+                           [&](int g) { return SItem{S.x[g], S.hsml[g], S.mass[g], S.rho[g], sp_stress[g]}; },
                            [&](int i, const SItem &v) {
                                lds_at<float2>(LY.x[KIND_SP])[i] = to_rel(v.x, o);
                                lds_at<float>(LY.h[KIND_SP])[i] = (float)v.h;
@@ -515,8 +522,12 @@ __global__ __launch_bounds__(TILE) void k_deriv_nodes_mix(StateBufs S, Work W, S
         if (lds)
             la.run(LF, on_the_fly(grad, xrr, (float)hr, hu));
         else
-            walk_stencil<MASK_SP | MASK_DUMMY>(GlobalCells{V.kstart, C.g, C.L}, GF, C.g, cx, cy, xr, hr, KIND_NODE,
-                                               r, false, true, V.pi, NoClass{}, grad);
+        {
+            StateBufs S2 = S;   // stress points' stresses from sp_stress (the walk reads no node's)
+            S2.stress = const_cast<double4 *>(sp_stress);
+            walk_stencil<MASK_SP | MASK_DUMMY>(GlobalCells{V.kstart, C.g, C.L}, GlobalFieldsMix{S2, o}, C.g, cx,
+                                               cy, xr, hr, KIND_NODE, r, false, true, V.pi, NoClass{}, grad);
+        }
     }
 
     // phase 2: artificial viscosity over node partners (type 3)
