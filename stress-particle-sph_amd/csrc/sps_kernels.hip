@@ -709,6 +709,13 @@ struct GlobalFields {
 // receiver sums the same pairs in the same order whichever path it takes.
 #include "sps_tiles.h"
 constexpr int TILE = SPS_TILE;
+// list words in flight in the walks of the two register-heaviest stage
+// kernels (k_sp_update's gradient walk, k_deriv_nodes' walk): one keeps them
+// at 124 and 126 registers, 4 waves per SIMD (two: 131 and 134, 3 waves;
+// measured 1 % slower per FP64 step on the 1 M slope)
+#ifndef SPS_WALK_DEPTH
+#define SPS_WALK_DEPTH 1
+#endif
 constexpr int TROWS = SPS_TROWS;       // most window rows staged
 constexpr int TCOLS = STRIP + 2;
 constexpr int TCELLS = TROWS * TCOLS;
@@ -1537,13 +1544,13 @@ struct ListWord {
     float4 w, gx, gy;
 };
 
-// A receiver's list walk with two words in flight: start() issues the loads
-// of the first two words (callers start before the window staging so that
-// these loads overlap it), run() calls f(kind_tag, fields, j, w, dW/dx_r,
-// dW/dy_r) over the entries of the kinds in KMASK, forwards or backwards
-// (rev is the same for every lane).  NW / NG: the kernel reads w / the
-// gradient.
-template <int KMASK, bool NW, bool NG> struct ListWalk {
+// A receiver's list walk with DEPTH words in flight (2, or 1 where registers
+// are short): start() issues the loads of the first words (callers start
+// before the window staging so that these loads overlap it), run() calls
+// f(kind_tag, fields, j, w, dW/dx_r, dW/dy_r) over the entries of the kinds
+// in KMASK, forwards or backwards (rev is the same for every lane).  NW / NG:
+// the kernel reads w / the gradient.
+template <int KMASK, bool NW, bool NG, int DEPTH = 2> struct ListWalk {
     const unsigned long long *ip;
     const float4 *wp, *gxp, *gyp;
     int tile, deg, ng;
@@ -1572,13 +1579,17 @@ template <int KMASK, bool NW, bool NG> struct ListWalk {
         const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
         a = b = ListWord{0ull, z, z, z};
         if (ng > 0) load(a, 0);
-        if (ng > 1) load(b, 1);
+        if (DEPTH > 1 && ng > 1) load(b, 1);
     }
     template <typename Fields, typename F> __device__ __forceinline__ void run(const Fields &A, F &&f) {
         for (int gi = 0; gi < ng; ++gi) {
             const ListWord c = a;
-            a = b;
-            if (gi + 2 < ng) load(b, gi + 2);
+            if constexpr (DEPTH > 1) {
+                a = b;
+                if (gi + 2 < ng) load(b, gi + 2);
+            } else if (gi + 1 < ng) {
+                load(a, gi + 1);
+            }
             const int g = rev ? ng - 1 - gi : gi;
             const int ne = deg - 4 * g < 4 ? deg - 4 * g : 4;
             if (rev)
@@ -1883,7 +1894,7 @@ __global__ __launch_bounds__(TILE) STAGE_OCCUPANCY void k_sp_update(StateBufs S,
     };
     if (inside) {
         if (lds) {
-            ListWalk<MASK_NODE | MASK_DUMMY, false, true> lg;
+            ListWalk<MASK_NODE | MASK_DUMMY, false, true, SPS_WALK_DEPTH> lg;
             lg.start(PL.ia, PL.wa, PL.gxa, PL.gya, ltile, deg, rev_l);
             lg.run(LF, grad);
         } else
@@ -2011,7 +2022,7 @@ __global__ __launch_bounds__(TILE) STAGE_OCCUPANCY void k_deriv_nodes(StateBufs 
     const int key_r = active ? V.cell[r] : C.g.ndivt;
     const bool inside = key_r < C.g.ndivt;
     const bool rev_l = !C.any_fwd && C.any_rev;
-    ListWalk<MASK_SP | MASK_DUMMY, false, true> la;
+    ListWalk<MASK_SP | MASK_DUMMY, false, true, SPS_WALK_DEPTH> la;
     la.start(PL.ia, PL.wa, PL.gxa, PL.gya, tile, inside ? PL.deg_a[r] : 0, rev_l);
     bool check;
     const bool tiled = tile_uses_lds(T, V, C, r, check) && PL.ok[tile];
