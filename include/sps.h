@@ -232,8 +232,10 @@ int sps_set_materials(sps_handle *h, int32_t nmat, const double *table, const in
  * BC_or_not becomes 2 on the surface and 0 inside (1 stays 1), with the
  * surface normals; ifsigman = 1 then applies apply_stress_free
  * (3_...:1758-1841) after Normal_BCs at every BC call: each surface node's
- * stress keeps only its tangential part, sigma_tt in the normal frame
- * (not with wall dummies, whose node pairs set the bc_int it reads).  The
+ * stress keeps only its tangential part, sigma_tt in the normal frame,
+ * except at nodes with a wall-dummy pair (bc_int = 1) in the list of the
+ * last derivative pass: the previous step's list for the first two BC
+ * calls of a step, the step's own for the rest.  The
  * mixed form (SPS_MODE_FAST) does not run the detection and rejects
  * ifsigman = 1; in it BC_or_not stays as set.  The BC time is
  * time_sph: the reference adds ic_time*dt_sph, ic_time being a local it never

@@ -13,7 +13,10 @@
  *
  * Pinning (see tests/test_oracle_golden.py): every routine is checked
  * bit-for-bit against dumps of the reference Fortran compiled from source
- * (oracle/_ref), and against closed forms on small inputs.
+ * (oracle/_ref), and against closed forms on small inputs.  Parity unpinned:
+ * with ifsigman = 1 and wall dummies, that the first two BC calls of a step
+ * read bc_int from the previous step's list (the reference's order; no
+ * fixture has a surface node gain or lose a wall-dummy pair between steps).
  *
  * Threads: with OMP_NUM_THREADS above 1 the pair loops are split by the
  * particle that receives each term (each thread walks, in list order, the
@@ -680,6 +683,16 @@ static inline void deriv_pair(struct sps_handle *h, int64_t d, int lo, int hi, i
 static void get_derivatives(struct sps_handle *h)
 {
     const int nn = h->nn, nt = h->nt;
+    if (h->ifsigman == 1 && h->nd > 0) {
+        /* bc_int (2_...:506, 579): 1 for a node with a node-dummy pair in the
+         * list; apply_stress_free reads it at the BC calls that follow, so the
+         * first BC calls of a step see the previous step's list.  The outside
+         * approach sets it again from the same list after the position update
+         * (shift_stress_points), its only other reader. */
+        memset(h->bc_int, 0, (size_t)nn * sizeof(int32_t));
+        for (int64_t k = 0; k < h->n_pairs; ++k)
+            if (h->pt[k] == 6) h->bc_int[h->pj[k]] = 1; /* pair_j: the node */
+    }
     memset(h->grad1, 0, 4 * (size_t)nt * sizeof(double));
     memset(h->grad2, 0, 6 * (size_t)nt * sizeof(double));
     memset(h->divf1, 0, 4 * (size_t)nt * sizeof(double));
@@ -1819,8 +1832,6 @@ int sps_set_bcs(sps_handle *h, const sps_bcs *b)
     if (!h || !b || !b->bc_list || !b->bc_info || !b->bc_or_not) return fail(SPS_EINVAL, "null argument");
     if (b->no_bcs != h->p.no_bcs || b->no_bcs < 1) return fail(SPS_EINVAL, "no_bcs %d, sps_params says %d", b->no_bcs, h->p.no_bcs);
     if ((b->ifsigman != 0 && b->ifsigman != 1) || b->reserved0 != 0) return fail(SPS_EINVAL, "ifsigman must be 0 or 1, reserved0 0");
-    if (b->ifsigman == 1 && h->nd > 0)
-        return fail(SPS_EINVAL, "ifsigman = 1 with wall dummies is not supported (apply_stress_free reads bc_int)");
     if (b->ncurves < 0 || b->ncurves > SPS_MAX_CURVES || (b->ncurves > 0 && (!b->curve_npts || !b->curve_t || !b->curve_f)))
         return fail(SPS_EINVAL, "bad time curves");
     for (int c = 0; c < b->ncurves; ++c)

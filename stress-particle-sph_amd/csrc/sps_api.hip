@@ -162,6 +162,11 @@ struct sps_handle {
     // the outside approach: each node's stress points follow it
     bool outside = false;
     DBuf<int> inv, nodeinfo;      // slot of each original particle; per node slot n_int, bc_int
+    // ifsigman = 1 with wall dummies: apply_stress_free skips nodes with bc_int
+    // = 1 (a node-dummy pair), which the first BC calls of a step take from the
+    // previous step's list: kept by original index at the end of each step
+    bool bcint = false;
+    DBuf<int> bcint_prev;
     DBuf<double2> x10;            // x_10 and disp_10 by original node index
     DBuf<double> disp10;
     // prescribed BCs (sps_set_bcs): tables on the host, node rows on the device
@@ -243,7 +248,7 @@ struct sps_handle {
     PairLists lists() {
         return PairLists{list_ia.p,  list_ib.p,  list_wa.p, list_gxa.p,          list_gya.p,
                          list_gxb.p, list_gyb.p, xsph ? list_wb.p : nullptr, deg_a.p, deg_b.p,
-                         tile_ok.p,  ntile_n,    tiles.p,   plans.p, outside ? nodeinfo.p : nullptr};
+                         tile_ok.p,  ntile_n,    tiles.p,   plans.p, (outside || bcint) ? nodeinfo.p : nullptr};
     }
 };
 
@@ -544,6 +549,9 @@ extern "C" int sps_create(const sps_params *params, sps_handle **out) {
         rc |= dalloc(h->inv, nt2); rc |= dalloc(h->nodeinfo, nn);
         rc |= dalloc(h->x10, nn); rc |= dalloc(h->disp10, nn);
     }
+    // bc_int for apply_stress_free (sps_set_bcs with ifsigman = 1 and wall
+    // dummies): zero before the first derivative pass (cleared below)
+    if (params->no_bcs > 0 && nt2 > nt) rc |= dalloc(h->nodeinfo, nn) | dalloc(h->bcint_prev, nn);
     if (rc) return bail(rc);
     if (hipHostMalloc((void **)&h->pinned, 64 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
         return bail(fail(SPS_ENOMEM, "pinned staging allocation"));
@@ -551,6 +559,7 @@ extern "C" int sps_create(const sps_params *params, sps_handle **out) {
         return bail(fail(SPS_EDEVICE, "event creation failed"));
     if ((h->bc_or_not.p && (hipMemsetAsync(h->bc_or_not.p, 0, nt * sizeof(int), h->stream) != hipSuccess ||
                             hipMemsetAsync(h->normal.p, 0, nt * sizeof(double2), h->stream) != hipSuccess)) ||
+        (h->bcint_prev.p && hipMemsetAsync(h->bcint_prev.p, 0, nn * sizeof(int), h->stream) != hipSuccess) ||
         hipMemsetAsync(h->f_bound.p, 0, nn * sizeof(double2), h->stream) != hipSuccess ||
         hipMemsetAsync(h->art_visc.p, 0, nn * sizeof(double2), h->stream) != hipSuccess ||
         hipMemsetAsync(h->grad_u.p, 0, nt * sizeof(double4), h->stream) != hipSuccess ||
@@ -825,7 +834,7 @@ static int mixed_step_kernels(sps_handle *h, double dt, double time_sph, double 
     for (int st = 0; st < 4; ++st) {
         const double c1 = f1rk[st] * (dt);
         if (st > 0) LAUNCH(K_STAGE_BEGIN, nblk(nt), BLOCK, k_stage_begin, SB, W, c1, adapt, tanfi, coh, nn, nt);
-        if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, h->bc_vars, h->bc_vals.p, h->ifsigman, h->normal.p, nn);
+        if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, h->bc_vars, h->bc_vals.p, h->ifsigman, h->normal.p, nn, nullptr, nullptr);
         // the two halves of stress_point_update side by side, as in the FP64
         // step; one stream when kernels are timed or the step is captured
         const bool fork = !h->prof && !stepv && nn > 0 && nt > nn;
@@ -836,13 +845,13 @@ static int mixed_step_kernels(sps_handle *h, double dt, double time_sph, double 
             k_spu_nodes_mix<<<ntile8(nn), TILE, LAY_SPU_NODES_MIX.bytes, s2>>>(SB, V, PL, adapt, tanfi, coh, hu, nn);
             if (bcs)
                 k_bcs<<<nblk(nn), BLOCK, 0, s2>>>(SB, h->bc_info.p, h->bc_or_not.p, h->bc_vars, h->bc_vals.p,
-                                                 h->ifsigman, h->normal.p, nn);
+                                                 h->ifsigman, h->normal.p, nn, nullptr, nullptr);
             HIP_TRY(hipEventRecord(h->ev_join, s2));
         } else {
             if (nn > 0)
                 LAUNCH_LDS(K_SPU_NODES, ntile8(nn), TILE, LAY_SPU_NODES_MIX.bytes, k_spu_nodes_mix, SB, V, PL, adapt,
                            tanfi, coh, hu, nn);
-            if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, h->bc_vars, h->bc_vals.p, h->ifsigman, h->normal.p, nn);
+            if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, h->bc_vars, h->bc_vals.p, h->ifsigman, h->normal.p, nn, nullptr, nullptr);
         }
         if (nt > nn)
             LAUNCH_LDS(K_SP_UPDATE, ntile8(nt - nn), TILE, LAY_SP_UPDATE_MIX.bytes, k_sp_update_mix, SB, W, V, PL,
@@ -862,14 +871,14 @@ static int mixed_step_kernels(sps_handle *h, double dt, double time_sph, double 
         int rc = surrogate_correct(h, SB);
         if (rc) return rc;
     }
-    if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, h->bc_vars, h->bc_vals.p, h->ifsigman, h->normal.p, nn);
+    if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, h->bc_vars, h->bc_vals.p, h->ifsigman, h->normal.p, nn, nullptr, nullptr);
     if (nn > 0)
         LAUNCH_LDS(K_SPU_NODES, ntile8(nn), TILE, LAY_SPU_NODES_MIX.bytes, k_spu_nodes_mix, SB, V, PL, adapt, tanfi,
                    coh, hu, nn);
     if (nt > nn)
         LAUNCH_LDS(K_SP_UPDATE, ntile8(nt - nn), TILE, LAY_SP_UPDATE_MIX.bytes, k_sp_update_mix, SB, W, V, PL, h->dp,
                    time_sph, 0.0, 0, adapt, hu, nn, nt, stepv, nullptr);
-    if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, h->bc_vars, h->bc_vals.p, h->ifsigman, h->normal.p, nn);
+    if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, h->bc_vars, h->bc_vals.p, h->ifsigman, h->normal.p, nn, nullptr, nullptr);
     const bool xsph = h->xsph && h->p.update_x;
     if (xsph && nn > 0)
         LAUNCH_LDS(K_XSPH, ntile8(nn), TILE, LAY_XSPH_MIX.bytes, (k_xsph_mix<KIND_NODE>), SB, V, PL, hu, h->xsph_v.p);
@@ -1047,16 +1056,20 @@ static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
                        time_sph, 0.0, 0, adapt, nn, nt, nullptr);
     }
     const bool lf = h->integrator == SPS_INTEGRATOR_LEAPFROG;
+    // bc_int for apply_stress_free: the step's list after its first derivative
+    // pass, the previous step's before it
+    const int *bci_cur = h->bcint ? h->nodeinfo.p : nullptr;
+    const int *bci_prev = h->bcint ? h->bcint_prev.p : nullptr;
     LAUNCH(K_RK_INIT, nblk(nt2), BLOCK, k_stage_first, SB, W, adapt, tanfi, coh, nn, nt, nt2);
     if (lf) {
         // leapfrog: the node accelerations of the step's state kick the node
         // velocities; the stress points' rates from the kicked velocities
         // update their stresses (sps_set_integrator)
         const double kdt = h->lf_first ? 0.5 * dt : dt;
-        if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, h->ifsigman, h->normal.p, nn);
+        if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, h->ifsigman, h->normal.p, nn, nullptr, bci_prev);
         if (nn > 0)
             LAUNCH_LDS(K_SPU_NODES, ntile8(nn), TILE, LAY_SPU_NODES.bytes, k_spu_nodes, SB, V, PL, adapt, tanfi, coh, nn);
-        if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, h->ifsigman, h->normal.p, nn);
+        if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, h->ifsigman, h->normal.p, nn, nullptr, bci_prev);
         if (nt > nn)
             LAUNCH_LDS(K_SP_UPDATE, ntile8(nt - nn), TILE, lds_sp_update(h, 0), k_sp_update, SB, W, V, PL, h->dp, time_sph,
                        0.0, 0, adapt, nn, nt, nullptr);
@@ -1066,10 +1079,10 @@ static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
             LAUNCH_LDS(K_DERIV_NODES, ntile8(nn), TILE, lds_deriv_nodes(h), k_deriv_nodes, SB, W, V, PL, h->dp,
                        gxg, gyg, 1.0, h->err.p, nn, SB.stress, av ? h->av_nodes.p : nullptr);
         LAUNCH(K_STAGE_BEGIN, nblk(nn), BLOCK, k_lf_kick, SB, W, kdt, nn);
-        if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, h->ifsigman, h->normal.p, nn);
+        if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, h->ifsigman, h->normal.p, nn, bci_cur, nullptr);
         if (nn > 0)
             LAUNCH_LDS(K_SPU_NODES, ntile8(nn), TILE, LAY_SPU_NODES.bytes, k_spu_nodes, SB, V, PL, adapt, tanfi, coh, nn);
-        if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, h->ifsigman, h->normal.p, nn);
+        if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, h->ifsigman, h->normal.p, nn, bci_cur, nullptr);
         if (nt > nn)
             LAUNCH_LDS(K_SP_UPDATE, ntile8(nt - nn), TILE, lds_sp_update(h, 1), k_sp_update, SB, W, V, PL, h->dp, time_sph,
                        1.0, 1, adapt, nn, nt, nullptr);
@@ -1080,7 +1093,7 @@ static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
     for (int st = 0; st < (lf ? 0 : 4); ++st) {
         const double c1 = f1rk[st] * (dt);
         if (st > 0) LAUNCH(K_STAGE_BEGIN, nblk(nt), BLOCK, k_stage_begin, SB, W, c1, adapt, tanfi, coh, nn, nt);
-        if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, h->ifsigman, h->normal.p, nn);
+        if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, h->ifsigman, h->normal.p, nn, st ? bci_cur : nullptr, st ? nullptr : bci_prev);
         // stress_point_update's two halves are independent: the node side
         // reads the stress points' stresses before their second return map
         // (k_sp_update leaves that in sig2) and changes node velocities only
@@ -1099,14 +1112,15 @@ static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
             k_spu_nodes<<<ntile8(nn), TILE, LAY_SPU_NODES.bytes, s2>>>(SB, V, PL, adapt, tanfi, coh, nn);
             if (bcs)
                 k_bcs<<<nblk(nn), BLOCK, 0, s2>>>(SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, h->ifsigman,
-                                                 h->normal.p, nn);
+                                                 h->normal.p, nn, st ? bci_cur : nullptr,
+                                                 st ? nullptr : bci_prev);
             if (av) k_av_nodes<<<ntile8(nn), TILE, lds_av_nodes(h), s2>>>(SB, V, PL, h->dp, h->av_nodes.p, nn);
             HIP_TRY(hipEventRecord(h->ev_join, s2));
         } else {
             if (nn > 0)
                 LAUNCH_LDS(K_SPU_NODES, ntile8(nn), TILE, LAY_SPU_NODES.bytes, k_spu_nodes, SB, V, PL, adapt, tanfi, coh,
                            nn);
-            if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, h->ifsigman, h->normal.p, nn);
+            if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, h->ifsigman, h->normal.p, nn, st ? bci_cur : nullptr, st ? nullptr : bci_prev);
             if (av) LAUNCH_LDS(K_AV_NODES, ntile8(nn), TILE, lds_av_nodes(h), k_av_nodes, SB, V, PL, h->dp, h->av_nodes.p, nn);
         }
         if (nt > nn)
@@ -1123,13 +1137,15 @@ static int one_step(sps_handle *h, int itimestep, double time_sph, double dt) {
     if (!lf)
         LAUNCH(K_RK_FINAL, nblk(nt), BLOCK, k_rk_final, SB, W, dt, adapt, tanfi, coh, nn, nt, surrogate_dev(h),
                hook ? 1 : 0, h->err.p);
-    if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, h->ifsigman, h->normal.p, nn);
+    if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, h->ifsigman, h->normal.p, nn, bci_cur, nullptr);
     if (nn > 0)
         LAUNCH_LDS(K_SPU_NODES, ntile8(nn), TILE, LAY_SPU_NODES.bytes, k_spu_nodes, SB, V, PL, adapt, tanfi, coh, nn);
     if (nt > nn)
         LAUNCH_LDS(K_SP_UPDATE, ntile8(nt - nn), TILE, lds_sp_update(h, 0), k_sp_update, SB, W, V, PL, h->dp, time_sph, 0.0,
                    0, adapt, nn, nt, nullptr);
-    if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, h->ifsigman, h->normal.p, nn);
+    if (bcs) LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcs, SB, h->bc_info.p, h->bc_or_not.p, BV, h->bc_vals.p, h->ifsigman, h->normal.p, nn, bci_cur, nullptr);
+    if (h->bcint && nn > 0)   // the next step's first BC calls see this step's list
+        LAUNCH(K_BCS, nblk(nn), BLOCK, k_bcint_keep, h->nodeinfo.p, SB.orig, h->bcint_prev.p, nn);
     const bool xsph = h->xsph && h->p.update_x;
     if (xsph && nn > 0)
         LAUNCH_LDS(K_XSPH, ntile8(nn), TILE, LAY_XSPH.bytes, (k_xsph<KIND_NODE>), SB, V, PL, h->xsph_v.p);
@@ -1168,8 +1184,6 @@ extern "C" int sps_set_bcs(sps_handle *h, const sps_bcs *b) {
         return fail(SPS_EINVAL, "no_bcs %d, sps_params says %d", b->no_bcs, h->p.no_bcs);
     if ((b->ifsigman != 0 && b->ifsigman != 1) || b->reserved0 != 0)
         return fail(SPS_EINVAL, "ifsigman must be 0 or 1, reserved0 0");
-    if (b->ifsigman == 1 && h->nt2 > h->nt)
-        return fail(SPS_EINVAL, "ifsigman = 1 with wall dummies is not supported (apply_stress_free reads bc_int)");
     if (b->ifsigman == 1 && h->mixed)
         return fail(SPS_EINVAL, "ifsigman = 1: SPS_MODE_ORDERED only (the mixed form has no free-surface detection)");
     for (int i = 0; i < h->nt; ++i)
@@ -1193,6 +1207,7 @@ extern "C" int sps_set_bcs(sps_handle *h, const sps_bcs *b) {
     HIP_TRY(hipSetDevice(h->device));
     h->nbc = b->no_bcs;
     h->ncurves = b->ncurves;
+    h->bcint = b->ifsigman == 1 && h->nt2 > h->nt;
     std::memcpy(h->bc_list, b->bc_list, 8 * (size_t)b->no_bcs * sizeof(double));
     for (int c = 0; c < b->ncurves; ++c) {
         h->curve_npts[c] = b->curve_npts[c];
@@ -1496,7 +1511,7 @@ extern "C" void sps_destroy(sps_handle *h) {
     dfree(h->x_build); dfree(h->dmax); dfree(h->surr_count); dfree(h->surr_slot); dfree(h->surr_sn);
     dfree(h->surr_str); dfree(h->surr_feat); dfree(h->surr_stats); dfree(h->plog); dfree(h->surr_mat);
     dfree(h->mtab); dfree(h->mat_orig); dfree(h->pairs_total);
-    dfree(h->inv); dfree(h->nodeinfo); dfree(h->x10); dfree(h->disp10);
+    dfree(h->inv); dfree(h->nodeinfo); dfree(h->x10); dfree(h->disp10); dfree(h->bcint_prev);
     dfree(h->bc_info); dfree(h->bc_or_not); dfree(h->bc_vals); dfree(h->normal); dfree(h->surf_nrm0);
     dfree(h->surf_pi); dfree(h->surf_pj); dfree(h->surf_pt); dfree(h->surf_sub); dfree(h->surf_gx);
     dfree(h->surf_gy); dfree(h->surf_cnt); dfree(h->surf_start); dfree(h->surf_cur); dfree(h->surf_ent);

@@ -2458,15 +2458,20 @@ __global__ void k_bc_values(BcValues V, double *__restrict__ vals) {
 // that order; vals holds the step's values (k_bc_values), V the variables.
 // ifsigman: apply_stress_free (3_...:1758-1841) on the nodes with BC_or_not
 // 2, which Normal_BCs leaves alone, so one thread does either: the stress
-// keeps sigma_tt of the normal frame.  Its bc_int test always passes: the
-// option is refused with wall dummies, the only source of bc_int = 1.
+// keeps sigma_tt of the normal frame, except at nodes with bc_int = 1 (a
+// node-dummy pair in the list of the last derivative pass): bit 0 of
+// bci_slot (the step's list, by slot) or bci_orig (the previous step's, by
+// original index); both null without wall dummies, where bc_int is 0.
 __global__ void k_bcs(StateBufs S, const int *__restrict__ bc_info, const int *__restrict__ bc_or_not, BcValues V,
-                      const double *__restrict__ vals, int ifsigman, const double2 *__restrict__ normal, int nn) {
+                      const double *__restrict__ vals, int ifsigman, const double2 *__restrict__ normal, int nn,
+                      const int *__restrict__ bci_slot, const int *__restrict__ bci_orig) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nn) return;
     const int i = S.orig[r];
     const int *info = bc_info + 8 * (size_t)i;
     if (ifsigman && bc_or_not[i] == 2) {
+        const int bc_int = bci_slot ? (bci_slot[r] & 1) : (bci_orig ? bci_orig[i] : 0);
+        if (bc_int == 1) return;
         const double2 n = normal[i];
         if (__builtin_isnan(n.x) || __builtin_isnan(n.y)) return;
         const double s2 = n.y * n.y, c2 = n.x * n.x, sc = n.y * n.x;
@@ -2494,6 +2499,15 @@ __global__ void k_bcs(StateBufs S, const int *__restrict__ bc_info, const int *_
     }
     if (dv) S.vel[r] = v;
     if (ds) S.stress[r] = s;
+}
+
+// bc_int of each node as the step's derivative passes leave it (bit 0 of the
+// search's per-node count), by original index, for the next step's first BC
+// calls, which run before that step's first derivative pass
+__global__ void k_bcint_keep(const int *__restrict__ nodeinfo, const int *__restrict__ orig,
+                             int *__restrict__ bci_orig, int nn) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nn) bci_orig[orig[r]] = nodeinfo[r] & 1;
 }
 
 // ------------------------------------------------------------ free surface

@@ -76,6 +76,11 @@ PLAN = {
     # node, shifted every step): BCs, the position update and the free-surface
     # detection together with the shifts
     "l4_localisation_outside": ([1, 2, 3, 10, 100, 500], [1, 2, 3, 10, 100, 500], False),
+    # the column collapse with the traction-free surface (ifsigman 1, one BC
+    # that no node takes): apply_stress_free skips the surface nodes with a
+    # wall-dummy pair (bc_int), which the front sliding on the floor gains
+    # and loses
+    "c3_collapse_free": ([1, 2, 3, 10, 100, 300], [1, 2, 3, 10], False),
     # BASELINE config 5 (biaxial compression, Drucker-Prager, rigid platens,
     # confined faces) at dx 1 mm, one material: 20,301 + 20,000 particles
     "c5_biaxial_dx1": ([1, 2, 10, 100, 300], [1], False),

@@ -42,7 +42,7 @@ EXACT_GROUPS = {
     "collapse": [("c1_collapse", 2), ("v3_vslope_sp3", 10)],
     "localisation": [("l1_localisation", 6), ("l1_localisation:graph", 6)],
     "surface": [("l2_localisation_free", 4), ("l3_localisation_xsph", 4)],
-    "outside_bcs": [("l4_localisation_outside", 3)],
+    "bcs": [("l4_localisation_outside", 3), ("c3_collapse_free", 3)],
 }
 
 # the mixed-precision kernels: in bounds (AddressSanitizer) and within single-

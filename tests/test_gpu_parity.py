@@ -166,7 +166,8 @@ SUBNORMAL = {("l1_localisation", 1), ("l1_localisation", 2), ("l2_localisation_f
 @pytest.mark.parametrize("deck_name", ["c1_collapse", "b1_bui_sp1", "b2_bui_sp2", "b3_bui_sp3", "b4_bui_outside",
                                        "b5_bui_velvec", "b6_bui_velvec_shift", "v1_vslope_sp1", "v3_vslope_sp3",
                                        "l1_localisation", "b7_bui_out1", "b8_bui_out3", "l2_localisation_free",
-                                       "l3_localisation_xsph", "c5_biaxial_dx1", "l4_localisation_outside"])
+                                       "l3_localisation_xsph", "c5_biaxial_dx1", "l4_localisation_outside",
+                                       "c3_collapse_free"])
 def test_digest_decks_vs_reference(deck_name):
     """GPU state digests against the reference Fortran's dumps: the granular
     column collapse (1000 steps of dt = 1e-5) and the Bui
@@ -229,7 +230,7 @@ def test_scale_bitwise_vs_oracle(deck_name, nsteps):
 
 @pytest.mark.parametrize("deck_name", ["l2_localisation_free", "l3_localisation_xsph", "t4_xsph",
                                        "b4_bui_outside", "c1_collapse", "c5_biaxial_dx1",
-                                       "l4_localisation_outside"])
+                                       "l4_localisation_outside", "c3_collapse_free"])
 def test_free_surface_vs_reference(deck_name):
     """Free-surface detection (3_...:1116-1411) on the GPU: BC_or_not of
     every particle and the double-precision normals bitwise against the

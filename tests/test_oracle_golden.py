@@ -85,7 +85,8 @@ DIGEST = {"c1_collapse": "BASELINE config 1 (5,151 + 5,000 particles, 1000 steps
           "b8_bui_out3": "outside approach, three points per node, 500 steps",
           "l2_localisation_free": "the localisation test with the traction-free surface (ifsigman 1), 500 steps",
           "l3_localisation_xsph": "the same with XSPH (its marks next to the surface), 300 steps",
-          "c5_biaxial_dx1": "BASELINE config 5 at dx 1 mm: Drucker-Prager biaxial test, platens, 300 steps"}
+          "c5_biaxial_dx1": "BASELINE config 5 at dx 1 mm: Drucker-Prager biaxial test, platens, 300 steps",
+          "c3_collapse_free": "the column collapse with the traction-free surface next to wall dummies (bc_int), 300 steps"}
 # (deck, step) where the reference reads an uninitialised local: its BCs
 # take t = time_sph + ic_time*dt_sph with ic_time never set (3_...:1622-1627;
 # the compiled reference finds about 6.95e-310 there, a stack address), so
@@ -160,7 +161,7 @@ def test_oracle_digest_decks(deck_name, oracle_lib):
 # prescribed BCs (l2, l3) and without (the position update alone runs the
 # detection every step, 2_SPH_main_2018.f90:150-154)
 SURFACE_DECKS = ["l2_localisation_free", "l3_localisation_xsph", "t4_xsph", "b4_bui_outside", "c1_collapse",
-                 "c5_biaxial_dx1", "l4_localisation_outside"]
+                 "c5_biaxial_dx1", "l4_localisation_outside", "c3_collapse_free"]
 
 
 def surface_matches(g, st, bc, nrm):
