@@ -40,7 +40,7 @@ EXACT_GROUPS = {
     "models": [("p1_vm", 20), ("p4_mc", 20), ("t2_cspm:nan", 2), ("t2_cspm:lf", 20), ("v1_vslope_sp1:lf", 10)],
     "bui": [("b3_bui_sp3", 8), ("b4_bui_outside", 8), ("b6_bui_velvec_shift:kick", 8)],
     "collapse": [("c1_collapse", 2), ("v3_vslope_sp3", 10)],
-    "localisation": [("l1_localisation", 6), ("l1_localisation:graph", 6)],
+    "localisation": [("l1_localisation", 3), ("l1_localisation:graph", 3)],
     "surface": [("l2_localisation_free", 4), ("l3_localisation_xsph", 4)],
     "bcs": [("l4_localisation_outside", 3), ("c3_collapse_free", 3)],
 }
