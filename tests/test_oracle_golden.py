@@ -6,11 +6,22 @@ every state field and for the pair list (indices, types, single-precision
 w and gradients, in linked-list order).
 """
 import hashlib
+import os
 
 import numpy as np
 import pytest
 
 from conftest import bits_equal, deck_dir, describe_diff, golden, load_pkg
+
+
+@pytest.fixture(autouse=True)
+def _oracle_threads(monkeypatch):
+    """The oracle's threaded pair loops on these decks too (their results
+    are bitwise those of one thread: test_oracle_threads.py), unless
+    pytest-xdist runs the tests in parallel workers (one thread each then)."""
+    if not os.environ.get("PYTEST_XDIST_WORKER") and "SPS_ORACLE_MIN_PARALLEL" not in os.environ:
+        monkeypatch.setenv("SPS_ORACLE_MIN_PARALLEL", "4000")
+
 
 SMALL = {"t1_tiny": [1, 2, 3, 10, 50, 200], "t2_cspm": [1, 2, 50, 200], "t3_elastic": [1, 50, 200],
          "t4_xsph": [1, 2, 3, 10, 50, 200], "p1_vm": [1, 2, 10, 50, 100, 200], "p2_dp": [1, 2, 10, 50, 100, 200],
