@@ -37,7 +37,7 @@ def _asan_runtime():
 EXACT_GROUPS = {
     "tiny": [("t1_tiny", 20), ("t2_cspm", 20), ("t3_elastic", 20)],
     "tiny2": [("t2_cspm:hvar", 20), ("t2_cspm:graph", 20), ("t4_xsph", 20)],
-    "models": [("p1_vm", 20), ("p4_mc", 20), ("t2_cspm:nan", 2), ("t2_cspm:lf", 20), ("v1_vslope_sp1:lf", 10)],
+    "models": [("p1_vm", 10), ("p4_mc", 10), ("t2_cspm:nan", 2), ("t2_cspm:lf", 10), ("v1_vslope_sp1:lf", 10)],
     "bui": [("b3_bui_sp3", 8), ("b4_bui_outside", 8), ("b6_bui_velvec_shift:kick", 8)],
     "collapse": [("c1_collapse", 2), ("v3_vslope_sp3", 10)],
     "localisation": [("l1_localisation", 3), ("l1_localisation:graph", 3)],
