@@ -846,6 +846,11 @@ static int mixed_step_kernels(sps_handle *h, double dt, double time_sph, double 
             if (bcs)
                 k_bcs<<<nblk(nn), BLOCK, 0, s2>>>(SB, h->bc_info.p, h->bc_or_not.p, h->bc_vars, h->bc_vals.p,
                                                  h->ifsigman, h->normal.p, nn, nullptr, nullptr);
+            // the nodes' artificial viscosity reads only node data, after the
+            // node side and the second BC call (k_deriv_nodes_mix takes it)
+            if (h->dp.do_av)
+                k_av_nodes_mix<<<ntile8(nn), TILE, LAY_DERIV_NODES_MIX.bytes, s2>>>(SB, V, PL, h->dp, hu,
+                                                                                     h->av_nodes.p, nn);
             HIP_TRY(hipEventRecord(h->ev_join, s2));
         } else {
             if (nn > 0)
@@ -859,7 +864,8 @@ static int mixed_step_kernels(sps_handle *h, double dt, double time_sph, double 
         if (fork) HIP_TRY(hipStreamWaitEvent(h->stream, h->ev_join, 0));
         if (nn > 0)
             LAUNCH_LDS(K_DERIV_NODES, ntile8(nn), TILE, LAY_DERIV_NODES_MIX.bytes, k_deriv_nodes_mix, SB, W, V, PL,
-                       h->dp, gxg, gyg, f2rk[st], h->err.p, hu, nn, stepv, nt > nn ? h->sig2.p : SB.stress);
+                       h->dp, gxg, gyg, f2rk[st], h->err.p, hu, nn, stepv, nt > nn ? h->sig2.p : SB.stress,
+                       fork && h->dp.do_av ? h->av_nodes.p : nullptr);
         if (st == 3 && nt2 > nt)
             LAUNCH(K_DUMMY_STRESS, nblk(nt2 - nt), BLOCK, k_dummy_stress, SB, V, nt, nt2);
     }

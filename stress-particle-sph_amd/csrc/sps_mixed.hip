@@ -404,6 +404,68 @@ __global__ __launch_bounds__(TILE) void k_sp_update_mix(StateBufs S, Work W, Sea
                   A3.get(), time_sph, f2);
 }
 
+// artificial_viscosity (2_...:826-904) of node receiver r in the mixed
+// form: its node partners (list B) with kernel values computed in single
+// precision from window-relative positions, and compensated sums; rounded
+// to single precision as node_stage_tail takes them.  Every thread of the
+// block calls it (it stages the node window through LDS).
+__device__ __forceinline__ float2 av_mix(const StateBufs &S, const SearchView &V, const PairLists &PL,
+                                         const DevParams &P, const WalkCtx &C, const Tile &T, int tile, int r,
+                                         bool inside, bool tiled, int cx, int cy, double2 xr, double2 vr,
+                                         double hr, double rhor, float2 xrr, double2 o, float hu, StagePlan &pl) {
+    constexpr const LdsLayout &LY = LAY_DERIV_NODES_MIX;
+    const LdsFieldsMix<LY> LF{};
+    const GlobalFieldsMix GF{S, o};
+    KSum ax, ay;
+    if (!P.do_av) return make_float2(0.0f, 0.0f);
+    ListWalk<MASK_NODE, false, false> lb;
+    lb.start(PL.ib, nullptr, nullptr, nullptr, tile, inside ? PL.deg_b[r] : 0, false);
+    bool lds2 = tiled;
+    if (lds2) {
+        load_plan<KIND_NODE>(PL, tile, 0, pl);
+        __syncthreads();   // also: the phase-1 walks are done with the shared bytes
+        lds2 = pl.total <= CAP;
+    }
+    if (lds2) {
+        struct NItem {
+            double2 x, v;
+            double h, m, rho;
+        };
+        copy_kind<KIND_NODE>(T, V, C, nullptr, pl,
+                             [&](int g) { return NItem{S.x[g], S.vel[g], S.hsml[g], S.mass[g], S.rho[g]}; },
+                             [&](int i, const NItem &n) {
+                                 lds_at<float2>(LY.x[KIND_NODE])[i] = to_rel(n.x, o);
+                                 lds_at<float2>(LY.v[KIND_NODE])[i] = make_float2((float)n.v.x, (float)n.v.y);
+                                 lds_at<float>(LY.h[KIND_NODE])[i] = (float)n.h;
+                                 lds_at<float>(LY.m[KIND_NODE])[i] = (float)n.m;
+                                 lds_at<float>(LY.rho[KIND_NODE])[i] = (float)n.rho;
+                             });
+        __syncthreads();
+    }
+    const float vrx = (float)vr.x, vry = (float)vr.y, hrf = (float)hr, rhof = (float)rhor;
+    const float alpha = (float)P.alpha, beta = (float)P.beta;
+    auto av = [&](auto kt, const auto &A, int j, float, float gxf, float gyf) __attribute__((always_inline)) {
+        const float2 xs = A.x(kt, j), vs = A.vel(kt, j);
+        const float xij = xrr.x - xs.x, yij = xrr.y - xs.y;
+        const float hh = 0.5f * (hrf + A.h(kt, j));
+        const float rho2 = 0.5f * (rhof + A.rho(kt, j));
+        const float div_u = xij * (vrx - vs.x) + yij * (vry - vs.y);
+        const float theta = (hh * div_u) / (xij * xij + yij * yij + 0.01f * (hh * hh));
+        const float visc = div_u < 0.0f ? (-(alpha * 600.0f * theta) + beta * (theta * theta)) / rho2 : 0.0f;
+        const float ms = A.m(kt, j);
+        ax.add(visc * gxf * ms);
+        ay.add(visc * gyf * ms);
+    };
+    if (inside) {
+        if (lds2)
+            lb.run(LF, on_the_fly(av, xrr, hrf, hu));
+        else
+            walk_stencil<MASK_NODE>(GlobalCells{V.kstart, C.g, C.L}, GF, C.g, cx, cy, xr, hr, KIND_NODE, r,
+                                    false, true, V.pi, NoClass{}, av);
+    }
+    return make_float2((float)ax.get(), (float)ay.get());
+}
+
 // Node receivers (as k_deriv_nodes).
 // stepv (a captured step): the gravity terms from stepv[1:3].
 // sp_stress: the stress points' stresses after the stage's return map
@@ -412,7 +474,8 @@ __global__ __launch_bounds__(TILE) void k_deriv_nodes_mix(StateBufs S, Work W, S
                                                           DevParams P, double gx_grav, double gy_grav, double f2,
                                                           int *__restrict__ err, float hu, int nn,
                                                           const double *__restrict__ stepv,
-                                                          const double4 *__restrict__ sp_stress) {
+                                                          const double4 *__restrict__ sp_stress,
+                                                          const float2 *__restrict__ av_in) {
     if (stepv) {
         gx_grav = stepv[1];
         gy_grav = stepv[2];
@@ -530,59 +593,44 @@ __global__ __launch_bounds__(TILE) void k_deriv_nodes_mix(StateBufs S, Work W, S
         }
     }
 
-    // phase 2: artificial viscosity over node partners (type 3)
-    KSum ax, ay;
-    if (P.do_av) {
-        ListWalk<MASK_NODE, false, false> lb;
-        lb.start(PL.ib, nullptr, nullptr, nullptr, tile, inside ? PL.deg_b[r] : 0, false);
-        bool lds2 = tiled;
-        if (lds2) {
-            load_plan<KIND_NODE>(PL, tile, 0, pl[0]);
-            __syncthreads();   // also: the phase-1 walks are done with the shared bytes
-            lds2 = pl[0].total <= CAP;
-        }
-        if (lds2) {
-            struct NItem {
-                double2 x, v;
-                double h, m, rho;
-            };
-            copy_kind<KIND_NODE>(T, V, C, nullptr, pl[0],
-                                 [&](int g) { return NItem{S.x[g], S.vel[g], S.hsml[g], S.mass[g], S.rho[g]}; },
-                                 [&](int i, const NItem &n) {
-                                     lds_at<float2>(LY.x[KIND_NODE])[i] = to_rel(n.x, o);
-                                     lds_at<float2>(LY.v[KIND_NODE])[i] = make_float2((float)n.v.x, (float)n.v.y);
-                                     lds_at<float>(LY.h[KIND_NODE])[i] = (float)n.h;
-                                     lds_at<float>(LY.m[KIND_NODE])[i] = (float)n.m;
-                                     lds_at<float>(LY.rho[KIND_NODE])[i] = (float)n.rho;
-                                 });
-            __syncthreads();
-        }
-        const float vrx = (float)vr.x, vry = (float)vr.y, hrf = (float)hr, rhof = (float)rhor;
-        const float alpha = (float)P.alpha, beta = (float)P.beta;
-        auto av = [&](auto kt, const auto &A, int j, float, float gxf, float gyf) __attribute__((always_inline)) {
-            const float2 xs = A.x(kt, j), vs = A.vel(kt, j);
-            const float xij = xrr.x - xs.x, yij = xrr.y - xs.y;
-            const float hh = 0.5f * (hrf + A.h(kt, j));
-            const float rho2 = 0.5f * (rhof + A.rho(kt, j));
-            const float div_u = xij * (vrx - vs.x) + yij * (vry - vs.y);
-            const float theta = (hh * div_u) / (xij * xij + yij * yij + 0.01f * (hh * hh));
-            const float visc = div_u < 0.0f ? (-(alpha * 600.0f * theta) + beta * (theta * theta)) / rho2 : 0.0f;
-            const float ms = A.m(kt, j);
-            ax.add(visc * gxf * ms);
-            ay.add(visc * gyf * ms);
-        };
-        if (inside) {
-            if (lds2)
-                lb.run(LF, on_the_fly(av, xrr, hrf, hu));
-            else
-                walk_stencil<MASK_NODE>(GlobalCells{V.kstart, C.g, C.L}, GF, C.g, cx, cy, xr, hr, KIND_NODE, r,
-                                        false, true, V.pi, NoClass{}, av);
-        }
+    // phase 2: artificial viscosity over node partners (type 3), here or,
+    // beside k_sp_update_mix, by k_av_nodes_mix (av_in)
+    float2 avv = make_float2(0.0f, 0.0f);
+    if (av_in) {
+        if (active) avv = av_in[r];
+    } else {
+        avv = av_mix(S, V, PL, P, C, T, tile, r, inside, tiled, cx, cy, xr, vr, hr, rhor, xrr, o, hu, pl[0]);
     }
     if (!active) return;
     node_stage_tail(S, W, V, C, P, r, cx, cy, xr, vr, hr, q00.get(), q01.get(), q10.get(), q11.get(), q20.get(),
-                    q21.get(), A0.get(), A1.get(), A2.get(), A3.get(), ndum, (float)ax.get(), (float)ay.get(),
-                    gx_grav, gy_grav, f2, err);
+                    q21.get(), A0.get(), A1.get(), A2.get(), A3.get(), ndum, avv.x, avv.y, gx_grav, gy_grav, f2, err);
+}
+
+// artificial_viscosity of the node receivers alone (av_mix), written to
+// av_out for k_deriv_nodes_mix: it reads only node data that the stress
+// points' update leaves alone, so the RK stages run it beside
+// k_sp_update_mix.
+__global__ __launch_bounds__(TILE) void k_av_nodes_mix(StateBufs S, SearchView V, PairLists PL, DevParams P,
+                                                       float hu, float2 *__restrict__ av_out, int nn) {
+    __shared__ StagePlan pl[1];
+    const WalkCtx C = walk_ctx(V);
+    const int tile = tile_index();
+    const Tile T = PL.tiles[tile];
+    if (T.first >= T.end) return;
+    const int r = T.first + (int)threadIdx.x;
+    const bool active = r < T.end;
+    const int key_r = active ? V.cell[r] : C.g.ndivt;
+    const bool inside = key_r < C.g.ndivt;
+    const double2 o = S.x[T.first];
+    const bool tiled = T.fits && PL.ok[tile];
+    int cx = 0, cy = 0;
+    if (inside) cell_xy(key_r, C.g, cx, cy);
+    const int ra = active ? r : T.first;   // a valid slot for the idle lanes' loads
+    const double2 xr = S.x[ra], vr = S.vel[ra];
+    const double rhor = S.rho[ra], hr = S.hsml[ra];
+    const float2 a = av_mix(S, V, PL, P, C, T, tile, r, inside, tiled, cx, cy, xr, vr, hr, rhor, to_rel(xr, o), o,
+                            hu, pl[0]);
+    if (active) av_out[r] = a;
 }
 
 // ------------------------------------------------------ captured steps
