@@ -11,7 +11,7 @@
   results that fail the yield-function check fall back to the exact return
   map.  What the trained network gains over the method without it -- the
   exact return map (adapt_stress2, 3_SPH_material_2018.f90:2066-2140),
-  which it learns and which stays its fallback and judge -- is measured in
+  which it learns and which stays its fallback and check -- is measured in
   time per step and in error, and reported: one network evaluation costs
   about 7.5e4 flops against about 30 for the exact map, so the network is
   slower and less exact than the map it replaces; the test holds its error
